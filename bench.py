@@ -1,0 +1,198 @@
+"""TreePM force-evaluation benchmark on one MI355X.
+
+One step = one full periodic TreePM force evaluation (refresh on, all N active):
+unwrap, path keys + sort, octree build, multipoles, CIC deposit, r2c FFT,
+Green's function, three gradient c2r FFTs, fused PM interpolation and the
+short-range residual walk. Inputs are resident in HBM before the timed region.
+
+Workload: 256^3 particles in a 200 Mpc/h box, 20% in 1000 NFW halos, 512^3
+mesh, r_s = 1.25 cells, r_cut = 4.5 r_s, theta = 0.5, leaf 16, eps = spacing/30.
+value = mixed-precision particles/s; the FP64 GPU form is timed on the same
+data and reported beside it, with the force error of both against each other
+and (on sample targets) against an FP64 Ewald sum.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for p in (str(ROOT / "chui-dev_amd"), str(ROOT)):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK = 8.0e12          # B/s, MI355X
+FP32_VALU_PEAK = 157.3e12  # FLOP/s, MI355X vector FP32
+FLOP_PER_PAIR = 40         # accounting constant per residual pair
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def error_metrics(value, ref, floor_fraction=1e-3):
+    err = np.sqrt(np.sum((value - ref) ** 2, axis=0))
+    rn = np.sqrt(np.sum(ref**2, axis=0))
+    rms_ref = np.sqrt(np.mean(rn**2))
+    norm = np.sort(err / np.maximum(rn, floor_fraction * rms_ref))
+    p99 = float(norm[min(int(np.ceil(0.99 * norm.size)), norm.size) - 1])
+    return {"relative_l2": float(np.sqrt(np.sum((value - ref) ** 2) / np.sum(ref**2))), "p99": p99}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-side", type=int, default=256)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--ewald-samples", type=int, default=2048)
+    ap.add_argument("--cpu-n-side", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import cosmosim._lib as L
+    from cosmosim import ForceAccumulator, TreePmCoordinator, ics
+
+    t0 = time.time()
+    w = ics.config(2, n_side=args.n_side)
+    n = w.x.size
+    M = w.mesh**3
+    log(f"workload {w.name}: N={n} mesh={w.mesh}^3 box={w.box} eps={w.softening:.4g} ({time.time()-t0:.1f}s)")
+    dev = [torch.from_numpy(a).to("cuda:0") for a in (w.x, w.y, w.z, w.mass)]
+    opts = L.default_options(box_size=w.box, gravitational_constant=ics.G_CODE, opening_theta=0.5,
+                             rcut_cells=4.5 * 1.25, asmth_cells=1.25, softening_epsilon=w.softening,
+                             max_leaf_size=16)
+    coord = TreePmCoordinator(w.mesh)
+
+    def run(precision, steps, warmup):
+        opts.precision = precision
+        dt = torch.float32 if precision == L.PRECISION_MIXED else torch.float64
+        acc = tuple(torch.zeros(n, dtype=dt, device="cuda:0") for _ in range(3))
+        fa = ForceAccumulator(acc)
+        for i in range(warmup):
+            coord.solve_active_set(*dev, fa, opts)
+            torch.cuda.synchronize()
+            log(f"warmup {i + 1}/{warmup} precision={precision}")
+        diag = coord.solve_active_set(*dev, fa, opts, want_diagnostics=True).diagnostics
+        coord.profiling(True)
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        for _ in range(steps):
+            coord.solve_active_set(*dev, fa, opts)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - start
+        prof = coord.profile_read()
+        coord.profiling(False)
+        log(f"precision={precision}: {steps} steps in {elapsed:.3f}s, stages {prof}")
+        return elapsed, prof, diag, np.stack([a.double().cpu().numpy() for a in acc])
+
+    t_mixed, prof_mixed, diag, acc_mixed = run(L.PRECISION_MIXED, args.steps, args.warmup)
+    t_fp64, prof_fp64, diag64, acc_fp64 = run(L.PRECISION_FP64, args.fp64_steps, 1)
+    value = n * args.steps / t_mixed
+    value_fp64 = n * args.fp64_steps / t_fp64
+    err_mix_vs_fp64 = error_metrics(acc_mixed, acc_fp64)
+    log(f"mixed {value:.4g} p/s, fp64 {value_fp64:.4g} p/s, ratio {value / value_fp64:.3f}, "
+        f"mixed-vs-fp64 {err_mix_vs_fp64}")
+
+    # FP64 Ewald on sample targets (periodic_ewald_reference.cpp semantics).
+    ewald = None
+    if args.ewald_samples > 0:
+        rng = np.random.default_rng(ics.BASE_SEED + 99)
+        tgt = np.sort(rng.choice(n, min(args.ewald_samples, n), replace=False)).astype(np.uint32)
+        dt_ = torch.from_numpy(tgt.view(np.int32)).to("cuda:0")
+        out = [torch.zeros(tgt.size, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+        te = time.time()
+        alpha = 11.8 / w.box
+        coord.ewald(*dev, dt_, (w.box,) * 3, ics.G_CODE, alpha, 0, 24, w.softening, *out)
+        ref = np.stack([o.cpu().numpy() for o in out])
+        ewald = {"samples": int(tgt.size), "alpha_box": 11.8, "modes": 24, "seconds": round(time.time() - te, 2),
+                 "fp64_vs_ewald": error_metrics(acc_fp64[:, tgt], ref),
+                 "mixed_vs_ewald": error_metrics(acc_mixed[:, tgt], ref)}
+        log(f"ewald: {ewald}")
+
+    # Roofline of the dominant kernel (the short-range walk with fused PM
+    # interpolation): algorithmic bytes per launch / measured launch time.
+    s_p, s_a, s_m = 4, 4, 4
+    nodes = diag["node_count"]
+    walk_bytes = n * (3 * s_p + 3 * s_a) + n * (3 * s_p + 0) + 3 * s_m * M + nodes * 80
+    walk_ms = prof_mixed["walk_ms"] / max(prof_mixed["solves"], 1)
+    achieved = walk_bytes / (walk_ms * 1e-3)
+    pairs = diag["direct_pair_evaluations"]
+    path_bytes = 185 * n + 64 * M
+    step_ms = 1e3 * t_mixed / args.steps
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        threads = O.set_threads(min(os.cpu_count() or 1, 16))
+        cw = ics.config(2, n_side=args.cpu_n_side)
+        tc = time.time()
+        O.solve(cw.x, cw.y, cw.z, cw.mass, opts_fp64_for(L, cw, ics), cw.mesh)
+        tcpu = time.time() - tc
+        cpu = {"value": cw.x.size / tcpu, "unit": "particles/s", "cores": threads, "kind": "port",
+               "sample": f"oracle FP64 CPU restatement, one full force evaluation of the same workload law at "
+                         f"{args.cpu_n_side}^3 particles / {cw.mesh}^3 mesh (box {cw.box:g} Mpc/h), "
+                         f"{tcpu:.1f}s on {threads} host threads"}
+        log(f"cpu baseline {cpu}")
+
+    line = {
+        "metric": "TreePM accel particles/s at 256^3 (mixed vs FP64) at RMS force err <=1e-3; grad/fwd ratio",
+        "value": value,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fixed32 (mixed); fp64 drift/kick",
+        "data": "synthetic (seeded Zel'dovich lattice + NFW halos)",
+        "config": {"workload": w.name, "particles": n, "mesh": w.mesh, "box_mpc_h": w.box,
+                   "halo_fraction": w.halo_fraction, "halos": w.halo_count, "theta": 0.5, "leaf": 16,
+                   "asmth_cells": 1.25, "rcut_over_rs": 4.5, "softening": w.softening,
+                   "parallelism": "single GPU"},
+        "fp64_gpu": {"value": value_fp64, "ms_per_step": 1e3 * t_fp64 / args.fp64_steps,
+                     "mixed_over_fp64": value / value_fp64},
+        "force_error": {"mixed_vs_fp64": err_mix_vs_fp64, "ewald": ewald},
+        "grad_fwd_ratio": None,
+        "stages_ms_mixed": {k: v / max(prof_mixed["solves"], 1) for k, v in prof_mixed.items() if k.endswith("_ms")},
+        "stages_ms_fp64": {k: v / max(prof_fp64["solves"], 1) for k, v in prof_fp64.items() if k.endswith("_ms")},
+        "roofline": {"bound": "hbm", "kernel": "walkmix_kernel", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "algorithmic_bytes_per_launch": walk_bytes, "launch_ms": walk_ms,
+                     "valu": {"pairs": pairs, "flop_per_pair": FLOP_PER_PAIR,
+                              "achieved_tflops": pairs * FLOP_PER_PAIR / (walk_ms * 1e-3) / 1e12,
+                              "peak_tflops": FP32_VALU_PEAK / 1e12},
+                     "whole_path": {"algorithmic_bytes": path_bytes,
+                                    "frac": path_bytes / (step_ms * 1e-3) / HBM_PEAK}},
+        "cpu_baseline": cpu,
+        "counters": {k: diag[k] for k in ("node_count", "max_depth", "direct_pair_evaluations",
+                                          "accepted_internal_multipoles", "accepted_leaves", "visited_nodes")},
+    }
+    coord.close()
+    print(json.dumps(line), flush=True)
+
+
+def opts_fp64_for(L, w, ics):
+    return L.default_options(box_size=w.box, gravitational_constant=ics.G_CODE, opening_theta=0.5,
+                             rcut_cells=4.5 * 1.25, asmth_cells=1.25, softening_epsilon=w.softening,
+                             max_leaf_size=16)
+
+
+if __name__ == "__main__":
+    main()

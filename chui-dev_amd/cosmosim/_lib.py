@@ -1,0 +1,220 @@
+"""ctypes binding of libchui_treepm.so (the C ABI in include/chui_treepm.h).
+
+The library is loaded from the package directory; there is no fallback: a
+missing or unloadable library raises ImportError at first use so that no code
+path can silently run without the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PACKAGE_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PACKAGE_DIR / "libchui_treepm.so"
+
+CHUI_OK = 0
+CHUI_ERR_INVALID_ARGUMENT = 1
+CHUI_ERR_RUNTIME = 2
+CHUI_ERR_OVERFLOW = 3
+CHUI_ERR_LOGIC = 4
+CHUI_ERR_DEVICE = 5
+CHUI_ERR_OUT_OF_MEMORY = 6
+CHUI_ERR_UNSUPPORTED = 7
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+
+class ChuiError(RuntimeError):
+    """Base class; .status carries the chui_status code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+class ChuiInvalidArgument(ChuiError, ValueError):
+    pass
+
+
+class ChuiOverflow(ChuiError, OverflowError):
+    pass
+
+
+class ChuiUnsupported(ChuiError, NotImplementedError):
+    pass
+
+
+class ChuiDeviceError(ChuiError):
+    pass
+
+
+class ChuiOutOfMemory(ChuiError, MemoryError):
+    pass
+
+
+_ERRORS = {
+    CHUI_ERR_INVALID_ARGUMENT: ChuiInvalidArgument,
+    CHUI_ERR_OVERFLOW: ChuiOverflow,
+    CHUI_ERR_UNSUPPORTED: ChuiUnsupported,
+    CHUI_ERR_DEVICE: ChuiDeviceError,
+    CHUI_ERR_OUT_OF_MEMORY: ChuiOutOfMemory,
+}
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("box_size", C.c_double * 3),
+        ("scale_factor", C.c_double),
+        ("gravitational_constant", C.c_double),
+        ("assignment_scheme", C.c_int32),
+        ("window_deconvolution", C.c_int32),
+        ("asmth_cells", C.c_double),
+        ("rcut_cells", C.c_double),
+        ("opening_theta", C.c_double),
+        ("opening_criterion", C.c_int32),
+        ("multipole_order", C.c_int32),
+        ("max_leaf_size", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("softening_epsilon", C.c_double),
+        ("relative_force_tolerance", C.c_double),
+        ("relative_force_acceleration_floor", C.c_double),
+        ("acceptance_policy", C.c_int32),
+        ("residual_block_size", C.c_int32),
+        ("precision", C.c_int32),
+        ("reserved1", C.c_int32),
+    ]
+
+
+class Diagnostics(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("reserved0", C.c_uint32),
+        ("node_count", C.c_uint64),
+        ("leaf_count", C.c_uint64),
+        ("max_depth", C.c_uint64),
+        ("direct_pair_evaluations", C.c_uint64),
+        ("accepted_internal_multipoles", C.c_uint64),
+        ("accepted_leaves", C.c_uint64),
+        ("cutoff_pruned_nodes", C.c_uint64),
+        ("visited_nodes", C.c_uint64),
+        ("split_scale_comoving", C.c_double),
+        ("cutoff_radius_comoving", C.c_double),
+        ("mesh_spacing_comoving", C.c_double),
+        ("tree_root_half_size_comoving", C.c_double),
+        ("unwrap_anchor", C.c_double * 3),
+        ("ms_total", C.c_double),
+        ("ms_unwrap_keys_sort", C.c_double),
+        ("ms_tree_build", C.c_double),
+        ("ms_multipoles", C.c_double),
+        ("ms_pm_deposit", C.c_double),
+        ("ms_pm_fft_poisson", C.c_double),
+        ("ms_pm_interpolate", C.c_double),
+        ("ms_walk", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            if name.startswith("reserved") or name == "struct_size":
+                continue
+            v = getattr(self, name)
+            out[name] = list(v) if name == "unwrap_anchor" else v
+        return out
+
+
+class KdkFactors(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "scale_factor_begin", "scale_factor_midpoint", "scale_factor_end", "drift_factor",
+        "first_kick_factor", "second_kick_factor", "first_hubble_drag", "second_hubble_drag")]
+
+
+class SplitDiagnostics(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "split_scale_comoving", "cutoff_radius_comoving", "short_range_factor_at_split",
+        "long_range_factor_at_split", "short_range_factor_at_cutoff", "long_range_factor_at_cutoff",
+        "composition_error_at_split", "max_relative_composition_error")]
+
+
+_P = C.c_void_p
+_D = C.c_double
+_U64 = C.c_uint64
+_I32 = C.c_int32
+
+# name -> argtypes (restype is chui_status = int unless listed in _RESTYPES)
+SIGNATURES = {
+    "chui_treepm_default_options": [C.POINTER(Options)],
+    "chui_treepm_options_set": [C.POINTER(Options), C.c_char_p, C.c_char_p],
+    "chui_treepm_options_validate": [C.POINTER(Options), C.POINTER(_I32)],
+    "chui_treepm_create": [C.POINTER(_I32), _I32, C.POINTER(_P)],
+    "chui_treepm_destroy": [_P],
+    "chui_treepm_set_stream": [_P, _P],
+    "chui_treepm_synchronize": [_P],
+    "chui_treepm_solve": [_P, _P, _P, _P, _P, _U64, _P, _U64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                          C.POINTER(Options), _I32, C.POINTER(Diagnostics)],
+    "chui_treepm_solve_device": [_P, _P, _P, _P, _P, _U64, _P, _U64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                 C.POINTER(Options), _I32, C.POINTER(Diagnostics)],
+    "chui_treepm_profiling": [_P, _I32],
+    "chui_treepm_profile_read": [_P, C.POINTER(_D), C.POINTER(_U64)],
+    "chui_treepm_kdk_factors": [_D, _D, _D, _D, _D, C.POINTER(KdkFactors)],
+    "chui_treepm_kick_device": [_P, _P, _P, _P, _P, _P, _P, _I32, _P, _U64, _D, _D],
+    "chui_treepm_drift_device": [_P, _P, _P, _P, _P, _P, _P, _U64, _D, C.POINTER(_D)],
+    "chui_treepm_mass_momentum_device": [_P, _P, _P, _P, _P, _U64, C.POINTER(_D)],
+    "chui_treepm_unwrap_device": [_P, _P, _U64, _D, _P, C.POINTER(_D)],
+    "chui_treepm_morton_device": [_P, _P, _P, _P, _U64, _P, _P],
+    "chui_treepm_tree_size": [_P, C.POINTER(_U64), C.POINTER(_U64)],
+    "chui_treepm_tree_copy": [_P, _P, _P, _P, _P, _P, _P],
+    "chui_treepm_ewald_device": [_P, _P, _P, _P, _P, _U64, _P, _U64, C.POINTER(_D), _D, _D, _I32, _I32, _D,
+                                 _P, _P, _P],
+    "chui_treepm_split_diagnostics": [_D, _D, _D, C.POINTER(SplitDiagnostics)],
+    "chui_treepm_last_error": [],
+    "chui_treepm_abi_version": [],
+}
+_RESTYPES = {"chui_treepm_last_error": C.c_char_p, "chui_treepm_abi_version": C.c_int32}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libchui_treepm.so once; raises ImportError when it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("CHUI_TREEPM_LIB", str(LIB_PATH)))
+    if not path.exists():
+        raise ImportError(f"libchui_treepm.so not built at {path}; run __graft_entry__.build()")
+    lib = C.CDLL(str(path))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status == CHUI_OK:
+        return
+    msg = load().chui_treepm_last_error().decode(errors="replace")
+    raise _ERRORS.get(status, ChuiError)(status, msg)
+
+
+def default_options(**overrides) -> Options:
+    opts = Options()
+    check(load().chui_treepm_default_options(C.byref(opts)))
+    for key, value in overrides.items():
+        if key == "box_size":
+            if isinstance(value, (int, float)):
+                value = (value, value, value)
+            opts.box_size = (C.c_double * 3)(*value)
+        else:
+            setattr(opts, key, value)
+    return opts
+
+
+def set_param(opts: Options, key: str, value) -> None:
+    if isinstance(value, bool):
+        value = "true" if value else "false"
+    check(load().chui_treepm_options_set(C.byref(opts), key.encode(), str(value).encode()))

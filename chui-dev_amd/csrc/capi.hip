@@ -1,0 +1,933 @@
+// capi.hip — TreePM coordinator on the device and the C ABI around it.
+//
+// One chui_treepm_context plays the role of gravity::TreePmCoordinator
+// (tree_pm_coupling.hpp:527-814): it owns the mesh workspaces, the tree and all
+// scratch across calls (grow-only device buffers, so steady-state solves do not
+// allocate), and evaluates PM + short-range residual accelerations for an
+// active set. Every exported function catches all exceptions and returns a
+// chui_status; the message is kept per thread for chui_treepm_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/chui_treepm.h"
+#include "common.hpp"
+#include "kdk.hpp"
+#include "pm.hpp"
+#include "primitives.hpp"
+#include "tree.hpp"
+#include "walk.hpp"
+
+using chui::DeviceBuffer;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+enum Stage { kEvStart, kEvTree, kEvMultipoles, kEvPm, kEvTargets, kEvWalk, kEvCount };
+
+}  // namespace
+
+struct chui_treepm_context {
+  int device = 0;
+  int mesh[3] = {0, 0, 0};
+  int mesh_lg[3] = {0, 0, 0};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  chui::Tree tree;
+  chui::PmWorkspace pm64, pm32;
+  DeviceBuffer sx, sy, sz, sm64, qx, qy, qz, sm32;
+  DeviceBuffer hx, hy, hz, hm, hactive, hout[9];
+  DeviceBuffer tpos, tslot, tkeys_alt, tvals_alt, tsort_scratch, flag;
+  DeviceBuffer counters, red_scratch, ewald_scratch;
+  hipEvent_t ev[kEvCount] = {};
+  bool events = false;
+  // profiling mode: events recorded every solve, harvested at the next solve's
+  // first synchronisation point or by chui_treepm_profile_read
+  bool profiling = false;
+  bool pending = false;
+  double prof_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t prof_solves = 0;
+  std::vector<float> host_float;
+};
+
+namespace {
+
+using Ctx = chui_treepm_context;
+
+// ---------------------------------------------------------------------------
+chui_status fail(chui_status code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <class F>
+chui_status guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return CHUI_OK;
+  } catch (const chui::UnsupportedError& e) {
+    return fail(CHUI_ERR_UNSUPPORTED, e.what());
+  } catch (const chui::OutOfMemoryError& e) {
+    return fail(CHUI_ERR_OUT_OF_MEMORY, e.what());
+  } catch (const chui::DeviceError& e) {
+    return fail(CHUI_ERR_DEVICE, e.what());
+  } catch (const std::invalid_argument& e) {
+    return fail(CHUI_ERR_INVALID_ARGUMENT, e.what());
+  } catch (const std::overflow_error& e) {
+    return fail(CHUI_ERR_OVERFLOW, e.what());
+  } catch (const std::logic_error& e) {
+    return fail(CHUI_ERR_LOGIC, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(CHUI_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(CHUI_ERR_RUNTIME, e.what());
+  } catch (...) {
+    return fail(CHUI_ERR_RUNTIME, "unknown failure");
+  }
+}
+
+int ilog2_exact(int n) {
+  if (n < 2 || (n & (n - 1)) != 0) return -1;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+struct Split {
+  double h, a, rcut;
+};
+
+Split split_of(const chui_treepm_options& o, const int mesh[3]) {
+  // gravity_runtime.cpp:574-578 (cube root of the three mesh spacings) and
+  // makeTreePmSplitPolicyFromMeshSpacing (tree_pm_split_kernel.hpp:50-63).
+  const double hx = o.box_size[0] / mesh[0], hy = o.box_size[1] / mesh[1], hz = o.box_size[2] / mesh[2];
+  Split s;
+  s.h = std::cbrt(hx * hy * hz);
+  s.a = o.asmth_cells * s.h;
+  s.rcut = o.rcut_cells * s.h;
+  return s;
+}
+
+void validate_options(const chui_treepm_options* o, const int mesh[3]) {
+  if (!o) throw std::invalid_argument("TreePM options pointer is null");
+  if (o->struct_size != sizeof(chui_treepm_options))
+    throw std::invalid_argument("chui_treepm_options.struct_size does not match this library");
+  for (int a = 0; a < 3; ++a) {
+    if (!(std::isfinite(o->box_size[a]) && o->box_size[a] > 0.0))
+      throw std::invalid_argument("Periodic TreePM tree geometry requires finite positive axis lengths");
+    if (mesh[a] <= 0) throw std::invalid_argument("numerics.treepm_pm_grid_n{xyz} must all be > 0");
+  }
+  if (!(std::isfinite(o->gravitational_constant) && o->gravitational_constant > 0.0))
+    throw std::overflow_error("TreePM code-unit gravitational prefactor is non-finite or non-positive");
+  if (!(std::isfinite(o->opening_theta) && o->opening_theta > 0.0))
+    throw std::invalid_argument("numerics.treepm_tree_opening_theta must be finite and > 0");
+  if (o->max_leaf_size < 1 || o->max_leaf_size > 1024)
+    throw std::invalid_argument("numerics.treepm_tree_max_leaf_size must be in [1,1024]");
+  const int b = o->residual_block_size;
+  if (b != 16 && b != 32 && b != 64 && b != 128 && b != 256)
+    throw std::invalid_argument("numerics.treepm_residual_block_size must be 16, 32, 64, 128 or 256");
+  if (!(std::isfinite(o->softening_epsilon) && o->softening_epsilon >= 0.0))
+    throw std::invalid_argument("Invalid tree gravity options");
+  if (!(std::isfinite(o->asmth_cells) && o->asmth_cells > 0.0))
+    throw std::invalid_argument("TreePM asmth_cells must be > 0");
+  if (!(std::isfinite(o->rcut_cells) && o->rcut_cells > 0.0))
+    throw std::invalid_argument("TreePM rcut_cells must be > 0");
+  if (o->assignment_scheme != CHUI_ASSIGN_CIC && o->assignment_scheme != CHUI_ASSIGN_TSC)
+    throw std::invalid_argument("Unknown PM assignment scheme");
+  if (o->opening_criterion < 0 || o->opening_criterion > 2)
+    throw std::invalid_argument("Invalid tree gravity opening criterion");
+  if (o->multipole_order != 0 && o->multipole_order != 2)
+    throw std::invalid_argument("Invalid tree gravity multipole order");
+  if (o->precision != CHUI_PRECISION_FP64 && o->precision != CHUI_PRECISION_MIXED)
+    throw std::invalid_argument("Unknown precision");
+  const Split s = split_of(*o, mesh);
+  // config.cpp:3669 and :3682-3692
+  if (o->softening_epsilon > 0.20 * s.a)
+    throw std::invalid_argument("periodic TreePM Plummer softening exceeds the certified epsilon/r_s <= 0.20 envelope");
+  const double lmin = std::min({o->box_size[0], o->box_size[1], o->box_size[2]});
+  if (s.rcut >= std::nextafter(0.5 * lmin, 0.0))
+    throw std::invalid_argument("periodic TreePM requires r_cut < half the shortest box axis");
+  if (o->acceptance_policy != CHUI_ACCEPT_STRICT)
+    throw chui::UnsupportedError("adaptive acceptance policy is not built on the GPU path yet");
+  if (o->opening_criterion == CHUI_OPEN_RELATIVE && o->multipole_order == 2) {
+    // Without an acceleration history the relative criterion falls back to the
+    // COM-distance MAC (tree_interaction_common.hpp:60-62); handled in the walk.
+  }
+}
+
+// ---- gather kernels ---------------------------------------------------------
+__global__ void gather64_kernel(const uint32_t* __restrict__ order, std::size_t n, const double* __restrict__ x,
+                                const double* __restrict__ y, const double* __restrict__ z,
+                                const double* __restrict__ m, double* __restrict__ ox, double* __restrict__ oy,
+                                double* __restrict__ oz, double* __restrict__ om) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = order[i];
+  ox[i] = x[r];
+  oy[i] = y[r];
+  oz[i] = z[r];
+  if (om) om[i] = m[r];
+}
+
+__device__ __forceinline__ uint32_t fixed_of(double x, double L) {
+  // canonical wrap (periodic_domain.hpp:8-16), then 32-bit fixed point
+  double w = fmod(x, L);
+  if (w < 0.0) w += L;
+  if (!(w < L)) w = 0.0;
+  const double f = floor(w * (4294967296.0 / L));
+  return f >= 4294967295.0 ? 0xFFFFFFFFu : static_cast<uint32_t>(f);
+}
+
+__global__ void gather_fixed_kernel(const uint32_t* __restrict__ order, std::size_t n, const double* __restrict__ x,
+                                    const double* __restrict__ y, const double* __restrict__ z,
+                                    const double* __restrict__ m, double Lx, double Ly, double Lz,
+                                    uint32_t* __restrict__ qx, uint32_t* __restrict__ qy, uint32_t* __restrict__ qz,
+                                    float* __restrict__ om) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = order[i];
+  qx[i] = fixed_of(x[r], Lx);
+  qy[i] = fixed_of(y[r], Ly);
+  qz[i] = fixed_of(z[r], Lz);
+  if (om) om[i] = static_cast<float>(m[r]);
+}
+
+__global__ void target_keys_kernel(const uint32_t* __restrict__ active, std::size_t na,
+                                   const uint32_t* __restrict__ pos_of_row, std::size_t n, uint32_t* __restrict__ keys,
+                                   uint32_t* __restrict__ vals, uint32_t* __restrict__ bad) {
+  const std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= na) return;
+  const uint32_t r = active[s];
+  if (r >= n) {
+    atomicOr(bad, 1u);
+    keys[s] = 0;
+  } else {
+    keys[s] = pos_of_row[r];
+  }
+  vals[s] = static_cast<uint32_t>(s);
+}
+
+template <class T>
+__global__ void to_double_kernel(const T* __restrict__ in, double* __restrict__ out, std::size_t n) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<double>(in[i]);
+}
+
+double read_device_double(const double* p, hipStream_t s) {
+  double v = 0.0;
+  CHUI_HIP(hipMemcpyAsync(&v, p, sizeof(double), hipMemcpyDeviceToHost, s));
+  CHUI_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+void record(Ctx* c, Stage st) {
+  if (c->events) CHUI_HIP(hipEventRecord(c->ev[st], c->stream));
+}
+
+float elapsed(Ctx* c, Stage a, Stage b) {
+  float ms = 0.0f;
+  CHUI_HIP(hipEventElapsedTime(&ms, c->ev[a], c->ev[b]));
+  return ms;
+}
+
+void harvest(Ctx* c) {
+  if (!c->pending) return;
+  CHUI_HIP(hipEventSynchronize(c->ev[kEvWalk]));
+  c->prof_ms[0] += elapsed(c, kEvStart, kEvWalk);
+  c->prof_ms[1] += elapsed(c, kEvStart, kEvTree);
+  c->prof_ms[2] += elapsed(c, kEvTree, kEvMultipoles);
+  c->prof_ms[3] += elapsed(c, kEvMultipoles, kEvPm);
+  c->prof_ms[4] += elapsed(c, kEvPm, kEvTargets);
+  c->prof_ms[5] += elapsed(c, kEvTargets, kEvWalk);
+  c->prof_solves += 1;
+  c->pending = false;
+}
+
+// Core force evaluation on device arrays (tree_pm_coupling.cpp:2282-3519 for
+// one rank: PM refresh/reuse, PM interpolation, unwrap + build, residual walk).
+void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z, const double* m, uint64_t n,
+                       const uint32_t* active, uint64_t n_active, void* ax, void* ay, void* az, void* pmx,
+                       void* pmy, void* pmz, void* srx, void* sry, void* srz, const chui_treepm_options* o,
+                       int refresh, chui_treepm_diagnostics* diag) {
+  validate_options(o, c->mesh);
+  if (n > 0xFFFFFFFEull) throw std::overflow_error("Tree gravity source count exceeds the 32-bit tree-index contract");
+  if (active == nullptr) n_active = n;
+  if (n_active > 0 && (!ax || !ay || !az)) throw std::invalid_argument("TreePM accumulator output lanes are null");
+  if (n > 0 && (!x || !y || !z || !m)) throw std::invalid_argument("TreePM source lanes are null");
+  if ((pmx || pmy || pmz) && !(pmx && pmy && pmz))
+    throw std::invalid_argument("TreePM long-range capture requires all three lanes");
+  if ((srx || sry || srz) && !(srx && sry && srz))
+    throw std::invalid_argument("TreePM short-range lanes require all three lanes");
+  const bool mixed = o->precision == CHUI_PRECISION_MIXED;
+  hipStream_t s = c->stream;
+  harvest(c);
+  c->events = diag != nullptr || c->profiling;
+  if (c->events && !c->ev[0])
+    for (auto& e : c->ev) CHUI_HIP(hipEventCreate(&e));
+  record(c, kEvStart);
+  const Split sp = split_of(*o, c->mesh);
+
+  // --- tree ---
+  chui::TreeParams tp;
+  tp.n = n;
+  for (int a = 0; a < 3; ++a) tp.L[a] = o->box_size[a];
+  tp.leaf_size = static_cast<uint32_t>(o->max_leaf_size);
+  tp.quadrupole = o->multipole_order == 2;
+  tp.mixed = mixed;
+  chui::build_tree(c->tree, x, y, z, m, tp, o->softening_epsilon, s);
+  record(c, kEvTree);
+  const bool uniform = n > 0 && c->tree.mass_uniform;
+  const double um = uniform ? c->tree.uniform_mass : 0.0;
+
+  // --- tree-ordered source lanes ---
+  if (n > 0) {
+    if (!mixed) {
+      c->sx.reserve(n * 8);
+      c->sy.reserve(n * 8);
+      c->sz.reserve(n * 8);
+      if (!uniform) c->sm64.reserve(n * 8);
+      gather64_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(
+          c->tree.order.as<uint32_t>(), n, x, y, z, m, c->sx.as<double>(), c->sy.as<double>(), c->sz.as<double>(),
+          uniform ? nullptr : c->sm64.as<double>());
+      CHUI_LAUNCH_CHECK("gather64_kernel");
+    } else {
+      c->qx.reserve(n * 4);
+      c->qy.reserve(n * 4);
+      c->qz.reserve(n * 4);
+      if (!uniform) c->sm32.reserve(n * 4);
+      gather_fixed_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(
+          c->tree.order.as<uint32_t>(), n, x, y, z, m, o->box_size[0], o->box_size[1], o->box_size[2],
+          c->qx.as<uint32_t>(), c->qy.as<uint32_t>(), c->qz.as<uint32_t>(), uniform ? nullptr : c->sm32.as<float>());
+      CHUI_LAUNCH_CHECK("gather_fixed_kernel");
+    }
+  }
+  record(c, kEvMultipoles);
+
+  // --- PM (refresh or cadence reuse) ---
+  chui::PmGeometry g{};
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = c->mesh[a];
+    g.lg[a] = c->mesh_lg[a];
+    g.L[a] = o->box_size[a];
+  }
+  g.nzc = static_cast<std::size_t>(c->mesh[2] / 2 + 1);
+  g.row = 2 * g.nzc;
+  chui::PmOptions po;
+  po.G = o->gravitational_constant;
+  po.split_scale = sp.a;
+  po.assignment = o->assignment_scheme;
+  po.deconvolution = o->window_deconvolution;
+  chui::PmWorkspace& pw = mixed ? c->pm32 : c->pm64;
+  const double key[8] = {o->box_size[0], o->box_size[1], o->box_size[2], o->gravitational_constant, sp.a,
+                         static_cast<double>(o->assignment_scheme), static_cast<double>(o->window_deconvolution),
+                         o->scale_factor};
+  const bool key_match = pw.valid && std::equal(key, key + 8, pw.key);
+  if (refresh || !key_match) {
+    if (!refresh)
+      throw std::runtime_error("TreePM long-range field reuse requested without a matching field (refresh required)");
+    if (!mixed) {
+      chui::pm_solve_fp64(pw, g, po, c->sx.as<double>(), c->sy.as<double>(), c->sz.as<double>(),
+                          uniform ? nullptr : c->sm64.as<double>(), um, n, s);
+    } else {
+      chui::pm_solve_mixed(pw, g, po, c->qx.as<uint32_t>(), c->qy.as<uint32_t>(), c->qz.as<uint32_t>(),
+                           uniform ? nullptr : c->sm32.as<float>(), static_cast<float>(um), n, s);
+    }
+    std::copy(key, key + 8, pw.key);
+  }
+  record(c, kEvPm);
+
+  // --- targets ---
+  const uint32_t* target_pos = nullptr;
+  const uint32_t* target_slot = nullptr;
+  int identity = 1;
+  if (active != nullptr && n_active > 0) {
+    chui::build_pos_of_row(c->tree, s);
+    c->tpos.reserve(n_active * 4);
+    c->tslot.reserve(n_active * 4);
+    c->tkeys_alt.reserve(n_active * 4);
+    c->tvals_alt.reserve(n_active * 4);
+    c->tsort_scratch.reserve(chui::radix_sort_scratch_bytes(n_active));
+    c->flag.reserve(16);
+    chui::fill_u32(c->flag.as<uint32_t>(), 0u, 1, s);
+    target_keys_kernel<<<chui::grid_for(n_active, 256), 256, 0, s>>>(active, n_active,
+                                                                     c->tree.pos_of_row.as<uint32_t>(), n,
+                                                                     c->tpos.as<uint32_t>(), c->tslot.as<uint32_t>(),
+                                                                     c->flag.as<uint32_t>());
+    CHUI_LAUNCH_CHECK("target_keys_kernel");
+    uint32_t bad = 0;
+    CHUI_HIP(hipMemcpyAsync(&bad, c->flag.ptr, 4, hipMemcpyDeviceToHost, s));
+    CHUI_HIP(hipStreamSynchronize(s));
+    if (bad) throw std::invalid_argument("TreePM active particle index out of range");
+    // Visit targets in tree order (tree_pm_coupling.cpp:3801-3838).
+    chui::radix_sort_pairs32(c->tpos.as<uint32_t>(), c->tslot.as<uint32_t>(), c->tkeys_alt.as<uint32_t>(),
+                             c->tvals_alt.as<uint32_t>(), n_active, 0, 32, c->tsort_scratch.ptr, s);
+    target_pos = c->tpos.as<uint32_t>();
+    target_slot = c->tslot.as<uint32_t>();
+    identity = 0;
+  }
+  record(c, kEvTargets);
+
+  // --- walk (+ fused PM interpolation) ---
+  c->counters.reserve(8 * sizeof(unsigned long long));
+  chui::fill_u64(c->counters.as<uint64_t>(), 0ull, 8, s);
+  chui::WalkParams wp{};
+  for (int a = 0; a < 3; ++a) {
+    wp.L[a] = o->box_size[a];
+    wp.mesh[a] = c->mesh[a];
+    wp.mesh_log2[a] = c->mesh_lg[a];
+    wp.fx_to_len[a] = static_cast<float>(o->box_size[a] / 4294967296.0);
+  }
+  wp.G = o->gravitational_constant;
+  wp.a = sp.a;
+  wp.rcut = sp.rcut;
+  wp.rcut2 = sp.rcut * sp.rcut;
+  wp.theta = o->opening_theta;
+  wp.eps = o->softening_epsilon;
+  wp.criterion = o->opening_criterion == CHUI_OPEN_GEOMETRIC ? 0 : 1;
+  wp.quad = o->multipole_order == 2;
+  wp.node_count = static_cast<uint32_t>(c->tree.node_count);
+  wp.n_targets = static_cast<uint32_t>(n_active);
+  wp.identity_targets = identity;
+  wp.uniform_mass = um;
+  wp.uniform_mass_f = static_cast<float>(um);
+  wp.pm_enabled = 1;
+  wp.assignment = o->assignment_scheme;
+  wp.mesh_row = g.row;
+  wp.c4 = static_cast<float>(0.25 / (sp.a * sp.a));
+  wp.c8 = static_cast<float>(0.125 / (sp.a * sp.a * sp.a));
+  chui::WalkOutputs out{ax, ay, az, pmx, pmy, pmz, srx, sry, srz};
+  if (n_active > 0 && n > 0) {
+    if (!mixed) {
+      chui::walk_fp64(c->tree, wp, c->sx.as<double>(), c->sy.as<double>(), c->sz.as<double>(),
+                      uniform ? nullptr : c->sm64.as<double>(), target_pos, target_slot,
+                      static_cast<const double*>(pw.force[0]), static_cast<const double*>(pw.force[1]),
+                      static_cast<const double*>(pw.force[2]), out,
+                      reinterpret_cast<unsigned long long*>(c->counters.ptr), s);
+    } else {
+      chui::fit_gaussian_coefficient(wp, wp.rcut2 * (0.25 / (sp.a * sp.a)) * (1.0 + 1.0e-3));
+      chui::walk_mixed(c->tree, wp, c->qx.as<uint32_t>(), c->qy.as<uint32_t>(), c->qz.as<uint32_t>(),
+                       uniform ? nullptr : c->sm32.as<float>(), target_pos, target_slot,
+                       static_cast<const float*>(pw.force[0]), static_cast<const float*>(pw.force[1]),
+                       static_cast<const float*>(pw.force[2]), out,
+                       reinterpret_cast<unsigned long long*>(c->counters.ptr), s);
+    }
+  }
+  record(c, kEvWalk);
+
+  if (diag) {
+    if (diag->struct_size != sizeof(chui_treepm_diagnostics))
+      throw std::invalid_argument("chui_treepm_diagnostics.struct_size does not match this library");
+    unsigned long long cnt[8] = {};
+    CHUI_HIP(hipMemcpyAsync(cnt, c->counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    CHUI_HIP(hipStreamSynchronize(s));
+    diag->node_count = c->tree.node_count;
+    diag->leaf_count = c->tree.leaf_count;
+    diag->max_depth = c->tree.max_depth;
+    diag->visited_nodes = cnt[0];
+    diag->cutoff_pruned_nodes = cnt[1];
+    diag->direct_pair_evaluations = cnt[2];
+    diag->accepted_internal_multipoles = cnt[3];
+    diag->accepted_leaves = cnt[4];
+    diag->split_scale_comoving = sp.a;
+    diag->cutoff_radius_comoving = sp.rcut;
+    diag->mesh_spacing_comoving = sp.h;
+    diag->tree_root_half_size_comoving = c->tree.root_half;
+    for (int a = 0; a < 3; ++a) diag->unwrap_anchor[a] = c->tree.unwrap.anchor[a];
+    diag->ms_total = elapsed(c, kEvStart, kEvWalk);
+    diag->ms_unwrap_keys_sort = 0.0;
+    diag->ms_tree_build = elapsed(c, kEvStart, kEvTree);
+    diag->ms_multipoles = elapsed(c, kEvTree, kEvMultipoles);
+    diag->ms_pm_deposit = 0.0;
+    diag->ms_pm_fft_poisson = elapsed(c, kEvMultipoles, kEvPm);
+    diag->ms_pm_interpolate = 0.0;
+    diag->ms_walk = elapsed(c, kEvTargets, kEvWalk);
+  }
+  if (c->profiling) c->pending = true;
+  c->events = false;
+}
+
+void require_ctx(const Ctx* c) {
+  if (!c) throw std::invalid_argument("TreePM context is null");
+}
+
+void set_device(const Ctx* c) { CHUI_HIP(hipSetDevice(c->device)); }
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+chui_status chui_treepm_default_options(chui_treepm_options* o) {
+  return guarded([&] {
+    if (!o) throw std::invalid_argument("options pointer is null");
+    std::memset(o, 0, sizeof(*o));
+    o->struct_size = sizeof(chui_treepm_options);
+    o->box_size[0] = o->box_size[1] = o->box_size[2] = 0.0;
+    o->scale_factor = 1.0;
+    o->gravitational_constant = 1.0;
+    o->assignment_scheme = CHUI_ASSIGN_CIC;       // PmSolveOptions default (pm_solver.hpp:231)
+    o->window_deconvolution = 0;                  // PmSolveOptions default (pm_solver.hpp:232)
+    o->asmth_cells = 1.25;                        // numerics.treepm_asmth_cells
+    o->rcut_cells = 6.25;                         // numerics.treepm_rcut_cells
+    o->opening_theta = 0.7;                       // TreeGravityOptions default
+    o->opening_criterion = CHUI_OPEN_COM_DISTANCE;
+    o->multipole_order = 2;
+    o->max_leaf_size = 16;
+    o->softening_epsilon = 0.0;
+    o->relative_force_tolerance = 0.005;
+    o->relative_force_acceleration_floor = 1.0e-30;
+    o->acceptance_policy = CHUI_ACCEPT_STRICT;
+    o->residual_block_size = 64;
+    o->precision = CHUI_PRECISION_FP64;
+  });
+}
+
+chui_status chui_treepm_options_set(chui_treepm_options* o, const char* key_c, const char* value_c) {
+  return guarded([&] {
+    if (!o || !key_c || !value_c) throw std::invalid_argument("null argument");
+    std::string key(key_c), value(value_c);
+    for (const char* prefix : {"numerics.", "cosmology.", "units."})
+      if (key.rfind(prefix, 0) == 0) key = key.substr(std::strlen(prefix));
+    auto num = [&]() {
+      char* end = nullptr;
+      const double v = std::strtod(value.c_str(), &end);
+      if (end == value.c_str()) throw std::invalid_argument("value of " + key + " is not a number: " + value);
+      return v;
+    };
+    auto boolean = [&]() {
+      if (value == "true" || value == "1" || value == "yes" || value == "on") return 1;
+      if (value == "false" || value == "0" || value == "no" || value == "off") return 0;
+      throw std::invalid_argument("value of " + key + " is not a boolean: " + value);
+    };
+    if (key == "box_size" || key == "box_size_mpc_comoving") {
+      o->box_size[0] = o->box_size[1] = o->box_size[2] = num();
+    } else if (key == "box_size_x") {
+      o->box_size[0] = num();
+    } else if (key == "box_size_y") {
+      o->box_size[1] = num();
+    } else if (key == "box_size_z") {
+      o->box_size[2] = num();
+    } else if (key == "treepm_asmth_cells") {
+      o->asmth_cells = num();
+    } else if (key == "treepm_rcut_cells") {
+      o->rcut_cells = num();
+    } else if (key == "treepm_tree_opening_theta") {
+      o->opening_theta = num();
+    } else if (key == "treepm_tree_opening_criterion") {
+      if (value == "geometric" || value == "barnes_hut_geometric") o->opening_criterion = CHUI_OPEN_GEOMETRIC;
+      else if (value == "com_distance") o->opening_criterion = CHUI_OPEN_COM_DISTANCE;
+      else if (value == "relative" || value == "relative_force_error") o->opening_criterion = CHUI_OPEN_RELATIVE;
+      else throw std::invalid_argument("unknown treepm_tree_opening_criterion: " + value);
+    } else if (key == "treepm_tree_max_leaf_size") {
+      o->max_leaf_size = static_cast<int32_t>(num());
+    } else if (key == "treepm_tree_relative_force_tolerance") {
+      o->relative_force_tolerance = num();
+    } else if (key == "treepm_tree_relative_force_acceleration_floor") {
+      o->relative_force_acceleration_floor = num();
+    } else if (key == "treepm_assignment_scheme") {
+      if (value == "cic") o->assignment_scheme = CHUI_ASSIGN_CIC;
+      else if (value == "tsc") o->assignment_scheme = CHUI_ASSIGN_TSC;
+      else throw std::invalid_argument("unknown treepm_assignment_scheme: " + value);
+    } else if (key == "treepm_enable_window_deconvolution") {
+      o->window_deconvolution = boolean();
+    } else if (key == "treepm_adaptive_acceptance_enabled") {
+      o->acceptance_policy = boolean() ? CHUI_ACCEPT_ADAPTIVE : CHUI_ACCEPT_STRICT;
+    } else if (key == "treepm_residual_block_size") {
+      o->residual_block_size = static_cast<int32_t>(num());
+    } else if (key == "gravity_softening") {
+      // "<value> kpc" -> comoving Mpc (gravity_runtime.cpp:566-567)
+      o->softening_epsilon = num() * 1.0e-3;
+    } else if (key == "gravity_softening_mpc") {
+      o->softening_epsilon = num();
+    } else if (key == "gravitational_constant") {
+      o->gravitational_constant = num();
+    } else if (key == "treepm_precision") {
+      if (value == "fp64") o->precision = CHUI_PRECISION_FP64;
+      else if (value == "mixed") o->precision = CHUI_PRECISION_MIXED;
+      else throw std::invalid_argument("unknown treepm_precision: " + value);
+    } else if (key == "treepm_pm_force_gradient") {
+      if (value != "spectral") throw chui::UnsupportedError("only the spectral PM gradient is built");
+    } else if (key == "treepm_pm_grid" || key == "treepm_pm_grid_nx" || key == "treepm_pm_grid_ny" ||
+               key == "treepm_pm_grid_nz") {
+      throw std::invalid_argument(key + " is a context (mesh) property: pass it to chui_treepm_create");
+    } else {
+      throw std::invalid_argument("unknown TreePM option key: " + key);
+    }
+  });
+}
+
+chui_status chui_treepm_options_validate(const chui_treepm_options* o, const int32_t mesh[3]) {
+  return guarded([&] {
+    if (!mesh) throw std::invalid_argument("mesh pointer is null");
+    const int m[3] = {mesh[0], mesh[1], mesh[2]};
+    validate_options(o, m);
+  });
+}
+
+chui_status chui_treepm_create(const int32_t mesh[3], int32_t device, chui_treepm_context** out) {
+  return guarded([&] {
+    if (!mesh || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    for (int a = 0; a < 3; ++a) {
+      if (mesh[a] <= 0) throw std::invalid_argument("numerics.treepm_pm_grid_n{xyz} must all be > 0");
+      if (ilog2_exact(mesh[a]) < 1)
+        throw chui::UnsupportedError("PM mesh axes must be powers of two >= 2 on the GPU path");
+    }
+    int count = 0;
+    CHUI_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) throw std::invalid_argument("HIP device ordinal out of range");
+    auto* c = new chui_treepm_context();
+    c->device = device;
+    for (int a = 0; a < 3; ++a) {
+      c->mesh[a] = mesh[a];
+      c->mesh_lg[a] = ilog2_exact(mesh[a]);
+    }
+    try {
+      CHUI_HIP(hipSetDevice(device));
+      CHUI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      c->own_stream = true;
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+chui_status chui_treepm_destroy(chui_treepm_context* c) {
+  return guarded([&] {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev)
+      if (e) (void)hipEventDestroy(e);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+  });
+}
+
+chui_status chui_treepm_set_stream(chui_treepm_context* c, void* stream) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    CHUI_HIP(hipStreamSynchronize(c->stream));
+    if (c->own_stream && c->stream) CHUI_HIP(hipStreamDestroy(c->stream));
+    if (stream) {
+      c->stream = static_cast<hipStream_t>(stream);
+      c->own_stream = false;
+    } else {
+      CHUI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      c->own_stream = true;
+    }
+  });
+}
+
+chui_status chui_treepm_synchronize(chui_treepm_context* c) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    CHUI_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+chui_status chui_treepm_solve_device(chui_treepm_context* c, const double* x, const double* y, const double* z,
+                                     const double* m, uint64_t n, const uint32_t* active, uint64_t n_active,
+                                     void* ax, void* ay, void* az, void* pmx, void* pmy, void* pmz, void* srx,
+                                     void* sry, void* srz, const chui_treepm_options* o, int32_t refresh,
+                                     chui_treepm_diagnostics* diag) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    solve_device_impl(c, x, y, z, m, n, active, n_active, ax, ay, az, pmx, pmy, pmz, srx, sry, srz, o, refresh,
+                      diag);
+  });
+}
+
+chui_status chui_treepm_solve(chui_treepm_context* c, const double* x, const double* y, const double* z,
+                              const double* m, uint64_t n, const uint32_t* active, uint64_t n_active, double* ax,
+                              double* ay, double* az, double* pmx, double* pmy, double* pmz, double* srx,
+                              double* sry, double* srz, const chui_treepm_options* o, int32_t refresh,
+                              chui_treepm_diagnostics* diag) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    validate_options(o, c->mesh);
+    const uint64_t na = active ? n_active : n;
+    hipStream_t s = c->stream;
+    const std::size_t nb = std::max<uint64_t>(n, 1) * sizeof(double);
+    c->hx.reserve(nb);
+    c->hy.reserve(nb);
+    c->hz.reserve(nb);
+    c->hm.reserve(nb);
+    if (n > 0) {
+      if (!x || !y || !z || !m) throw std::invalid_argument("TreePM source lanes are null");
+      CHUI_HIP(hipMemcpyAsync(c->hx.ptr, x, n * 8, hipMemcpyHostToDevice, s));
+      CHUI_HIP(hipMemcpyAsync(c->hy.ptr, y, n * 8, hipMemcpyHostToDevice, s));
+      CHUI_HIP(hipMemcpyAsync(c->hz.ptr, z, n * 8, hipMemcpyHostToDevice, s));
+      CHUI_HIP(hipMemcpyAsync(c->hm.ptr, m, n * 8, hipMemcpyHostToDevice, s));
+    }
+    if (active && na > 0) {
+      c->hactive.reserve(na * 4);
+      CHUI_HIP(hipMemcpyAsync(c->hactive.ptr, active, na * 4, hipMemcpyHostToDevice, s));
+    }
+    const bool mixed = o->precision == CHUI_PRECISION_MIXED;
+    const std::size_t eb = mixed ? 4 : 8;
+    for (auto& b : c->hout) b.reserve(std::max<uint64_t>(na, 1) * eb);
+    double* hosts[9] = {ax, ay, az, pmx, pmy, pmz, srx, sry, srz};
+    void* dev[9];
+    for (int i = 0; i < 9; ++i) dev[i] = hosts[i] ? c->hout[i].ptr : nullptr;
+    solve_device_impl(c, c->hx.as<double>(), c->hy.as<double>(), c->hz.as<double>(), c->hm.as<double>(), n,
+                      active ? c->hactive.as<uint32_t>() : nullptr, na, dev[0], dev[1], dev[2], dev[3], dev[4],
+                      dev[5], dev[6], dev[7], dev[8], o, refresh, diag);
+    if (!mixed) {
+      for (int i = 0; i < 9; ++i)
+        if (hosts[i] && na) CHUI_HIP(hipMemcpyAsync(hosts[i], dev[i], na * 8, hipMemcpyDeviceToHost, s));
+      CHUI_HIP(hipStreamSynchronize(s));
+    } else {
+      c->host_float.resize(na);
+      for (int i = 0; i < 9; ++i) {
+        if (!hosts[i] || !na) continue;
+        CHUI_HIP(hipMemcpyAsync(c->host_float.data(), dev[i], na * 4, hipMemcpyDeviceToHost, s));
+        CHUI_HIP(hipStreamSynchronize(s));
+        for (uint64_t k = 0; k < na; ++k) hosts[i][k] = c->host_float[k];
+      }
+    }
+  });
+}
+
+chui_status chui_treepm_profiling(chui_treepm_context* c, int32_t enable) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    CHUI_HIP(hipStreamSynchronize(c->stream));
+    c->pending = false;
+    c->profiling = enable != 0;
+    std::fill(c->prof_ms, c->prof_ms + 8, 0.0);
+    c->prof_solves = 0;
+  });
+}
+
+chui_status chui_treepm_profile_read(chui_treepm_context* c, double stage_ms[8], uint64_t* solves) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    CHUI_HIP(hipStreamSynchronize(c->stream));
+    harvest(c);
+    if (stage_ms) std::copy(c->prof_ms, c->prof_ms + 8, stage_ms);
+    if (solves) *solves = c->prof_solves;
+  });
+}
+
+chui_status chui_treepm_kdk_factors(double a0, double a1, double H0, double om, double ol, chui_kdk_factors* out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("null output");
+    if (!(a0 > 0.0) || !(a1 > 0.0)) throw std::invalid_argument("scale factors must be positive");
+    if (a1 < a0) throw std::invalid_argument("scale_factor_end must be >= scale_factor_begin");
+    if (!(H0 > 0.0)) throw std::invalid_argument("H0 must be positive");
+    const double ok = 1.0 - om - ol;
+    auto H = [&](double a) { return H0 * std::sqrt(om / (a * a * a) + ok / (a * a) + ol); };
+    // cosmic time t(a) from a0 (composite Simpson, 4096 panels)
+    auto cosmic = [&](double lo, double hi) {
+      const int np = 4096;
+      const double hstep = (hi - lo) / np;
+      if (hstep == 0.0) return 0.0;
+      double acc = 0.0;
+      for (int i = 0; i <= np; ++i) {
+        const double a = lo + i * hstep;
+        const double f = 1.0 / (a * H(a));
+        acc += (i == 0 || i == np) ? f : (i % 2 ? 4.0 * f : 2.0 * f);
+      }
+      return acc * hstep / 3.0;
+    };
+    const double t_total = cosmic(a0, a1);
+    double lo = a0, hi = a1;
+    for (int it = 0; it < 200; ++it) {  // scale factor at half the cosmic time
+      const double mid = 0.5 * (lo + hi);
+      if (cosmic(a0, mid) < 0.5 * t_total) lo = mid;
+      else hi = mid;
+    }
+    const double am = 0.5 * (lo + hi);
+    auto driftlike = [&](double b, double e) {  // time_integration.cpp:242-266, 64 midpoint samples
+      const int samples = 64;
+      const double da = (e - b) / samples;
+      if (da == 0.0) return 0.0;
+      double acc = 0.0;
+      for (int i = 0; i < samples; ++i) {
+        const double a = b + (i + 0.5) * da;
+        acc += 1.0 / (a * a * H(a));
+      }
+      return acc * da;
+    };
+    out->scale_factor_begin = a0;
+    out->scale_factor_midpoint = am;
+    out->scale_factor_end = a1;
+    out->drift_factor = driftlike(a0, a1);
+    out->first_kick_factor = driftlike(a0, am) / am;
+    out->second_kick_factor = driftlike(am, a1) / a1;
+    out->first_hubble_drag = a0 / am;
+    out->second_hubble_drag = am / a1;
+  });
+}
+
+chui_status chui_treepm_kick_device(chui_treepm_context* c, double* vx, double* vy, double* vz, const void* ax,
+                                    const void* ay, const void* az, int32_t is_float, const uint32_t* active,
+                                    uint64_t n, double kick, double drag) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    chui::kick_device(vx, vy, vz, ax, ay, az, is_float, active, n, kick, drag, c->stream);
+  });
+}
+
+chui_status chui_treepm_drift_device(chui_treepm_context* c, double* x, double* y, double* z, const double* vx,
+                                     const double* vy, const double* vz, uint64_t n, double d, const double L[3]) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    if (!L) throw std::invalid_argument("box size pointer is null");
+    chui::drift_device(x, y, z, vx, vy, vz, n, d, L, c->stream);
+  });
+}
+
+chui_status chui_treepm_mass_momentum_device(chui_treepm_context* c, const double* m, const double* vx,
+                                             const double* vy, const double* vz, uint64_t n, double out[4]) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    c->red_scratch.reserve((4 * 1024 + 8) * sizeof(double));
+    chui::mass_momentum_device(m, vx, vy, vz, n, c->red_scratch.as<double>(), out, c->stream);
+  });
+}
+
+chui_status chui_treepm_unwrap_device(chui_treepm_context* c, const double* in, uint64_t n, double L, double* out,
+                                      double* anchor) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    const double a = chui::unwrap_axis_device(c->tree, in, n, L, out, c->stream);
+    CHUI_HIP(hipStreamSynchronize(c->stream));
+    if (anchor) *anchor = a;
+  });
+}
+
+chui_status chui_treepm_morton_device(chui_treepm_context* c, const double* x, const double* y, const double* z,
+                                      uint64_t n, uint64_t* keys, uint32_t* order) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    if (n > 0xFFFFFFFEull) throw std::overflow_error("Morton ordering exceeds the configured local tree-index contract");
+    chui::morton_order_device(c->tree, x, y, z, n, keys, order, c->stream);
+    CHUI_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+chui_status chui_treepm_tree_size(chui_treepm_context* c, uint64_t* nodes, uint64_t* particles) {
+  return guarded([&] {
+    require_ctx(c);
+    if (nodes) *nodes = c->tree.node_count;
+    if (particles) *particles = c->tree.n;
+  });
+}
+
+chui_status chui_treepm_tree_copy(chui_treepm_context* c, double* geometry, double* moments, uint32_t* pbegin,
+                                  uint32_t* pcount, uint32_t* child_index, uint32_t* order) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    const std::size_t nn = c->tree.node_count;
+    hipStream_t s = c->stream;
+    if (nn > 0) {
+      std::vector<chui::NodeLink> link(nn);
+      std::vector<chui::NodeGeo64> geo(nn);
+      std::vector<chui::NodeMom64> mom(nn);
+      std::vector<chui::NodeQuad64> quad(nn);
+      CHUI_HIP(hipMemcpyAsync(link.data(), c->tree.link.ptr, nn * sizeof(chui::NodeLink), hipMemcpyDeviceToHost, s));
+      CHUI_HIP(hipMemcpyAsync(geo.data(), c->tree.geo64.ptr, nn * sizeof(chui::NodeGeo64), hipMemcpyDeviceToHost, s));
+      CHUI_HIP(hipMemcpyAsync(mom.data(), c->tree.mom64.ptr, nn * sizeof(chui::NodeMom64), hipMemcpyDeviceToHost, s));
+      CHUI_HIP(hipMemcpyAsync(quad.data(), c->tree.quad64.ptr, nn * sizeof(chui::NodeQuad64), hipMemcpyDeviceToHost, s));
+      if (child_index)
+        CHUI_HIP(hipMemcpyAsync(child_index, c->tree.child_index.ptr, nn * 8 * 4, hipMemcpyDeviceToHost, s));
+      CHUI_HIP(hipStreamSynchronize(s));
+      for (std::size_t i = 0; i < nn; ++i) {
+        if (geometry) {
+          geometry[4 * i] = geo[i].cx;
+          geometry[4 * i + 1] = geo[i].cy;
+          geometry[4 * i + 2] = geo[i].cz;
+          geometry[4 * i + 3] = geo[i].half;
+        }
+        if (moments) {
+          double* q = moments + 11 * i;
+          q[0] = mom[i].mass; q[1] = mom[i].comx; q[2] = mom[i].comy; q[3] = mom[i].comz;
+          q[4] = quad[i].qxx; q[5] = quad[i].qxy; q[6] = quad[i].qxz; q[7] = quad[i].qyy; q[8] = quad[i].qyz;
+          q[9] = quad[i].qzz; q[10] = quad[i].trace;
+        }
+        if (pbegin) pbegin[i] = link[i].begin;
+        if (pcount) pcount[i] = link[i].count;
+      }
+    }
+    if (order && c->tree.n > 0) {
+      CHUI_HIP(hipMemcpyAsync(order, c->tree.order.ptr, c->tree.n * 4, hipMemcpyDeviceToHost, s));
+      CHUI_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
+chui_status chui_treepm_ewald_device(chui_treepm_context* c, const double* x, const double* y, const double* z,
+                                     const double* m, uint64_t n, const uint32_t* targets, uint64_t nt,
+                                     const double L[3], double G, double alpha, int32_t real_images, int32_t modes,
+                                     double eps, double* ax, double* ay, double* az) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    if (!L) throw std::invalid_argument("box size pointer is null");
+    if (real_images < 0 || modes < 0 || !(alpha > 0.0)) throw std::invalid_argument("invalid Ewald parameters");
+    c->ewald_scratch.reserve(chui::ewald_scratch_bytes(modes));
+    chui::ewald_device(x, y, z, m, n, targets, nt, L, G, alpha, real_images, modes, eps, ax, ay, az,
+                       c->ewald_scratch.ptr, c->stream);
+    CHUI_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+chui_status chui_treepm_split_diagnostics(double asmth, double rcut_cells, double h, chui_split_diagnostics* out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("null output");
+    if (!(asmth > 0.0 && rcut_cells > 0.0 && h > 0.0)) throw std::invalid_argument("TreePM split policy must be > 0");
+    const double a = asmth * h, rc = rcut_cells * h;
+    auto S = [&](double r) {
+      const double q = (0.5 * r) / a;
+      return std::erfc(q) + 2.0 * 0.564189583547756286948079451560772586 * q * std::exp(-q * q);
+    };
+    auto Lf = [&](double r) {
+      const double q = (0.5 * r) / a;
+      return std::erf(q) - 2.0 * 0.564189583547756286948079451560772586 * q * std::exp(-q * q);
+    };
+    out->split_scale_comoving = a;
+    out->cutoff_radius_comoving = rc;
+    out->short_range_factor_at_split = S(a);
+    out->long_range_factor_at_split = Lf(a);
+    out->short_range_factor_at_cutoff = S(rc);
+    out->long_range_factor_at_cutoff = Lf(rc);
+    out->composition_error_at_split = std::abs(S(a) + Lf(a) - 1.0);
+    double worst = 0.0;
+    for (int i = 1; i <= 256; ++i) {
+      const double r = rc * i / 256.0;
+      worst = std::max(worst, std::abs(S(r) + Lf(r) - 1.0));
+    }
+    out->max_relative_composition_error = worst;
+  });
+}
+
+const char* chui_treepm_last_error(void) { return g_last_error.c_str(); }
+
+int32_t chui_treepm_abi_version(void) { return CHUI_TREEPM_ABI_VERSION; }
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// common.hpp — shared host/device utilities of the MI355X TreePM library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace chui {
+
+// Exceptions used inside the library; the C ABI maps each class to a
+// chui_status code (include/chui_treepm.h) so nothing crosses the boundary.
+struct DeviceError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct UnsupportedError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct OutOfMemoryError : std::runtime_error { using std::runtime_error::runtime_error; };
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory)
+    throw OutOfMemoryError(std::string(what) + ": " + hipGetErrorString(e));
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define CHUI_HIP(call) ::chui::hip_check((call), #call)
+#define CHUI_LAUNCH_CHECK(name) ::chui::hip_check(hipGetLastError(), name)
+
+constexpr int kWave = 64;
+
+inline unsigned grid_for(std::size_t n, unsigned block) {
+  const std::size_t g = (n + block - 1) / block;
+  return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+// Grow-only device buffer. Allocation happens only when a call needs more
+// capacity than any earlier call; steady-state solves allocate nothing.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  std::size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+  void release() noexcept {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  void reserve(std::size_t need) {
+    if (need <= bytes) return;
+    release();
+    CHUI_HIP(hipMalloc(&ptr, need));
+    bytes = need;
+  }
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// ---- device helpers -------------------------------------------------------
+__device__ __forceinline__ uint64_t dbits(double v) { return static_cast<uint64_t>(__double_as_longlong(v)); }
+__device__ __forceinline__ double bitsd(uint64_t b) { return __longlong_as_double(static_cast<long long>(b)); }
+
+// Order-preserving map of any double onto uint64 (total order, -0 < +0).
+__device__ __forceinline__ uint64_t order_key(double v) {
+  const uint64_t b = dbits(v);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double order_key_inv(uint64_t k) {
+  return bitsd((k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k);
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const T o = __shfl_xor(v, off, kWave);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const T o = __shfl_xor(v, off, kWave);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+}  // namespace chui

@@ -1,0 +1,567 @@
+// pm.hip — PM long-range solve: CIC/TSC deposit, hand-written batched radix-2
+// FFT passes through LDS, Green's function with the Gaussian split filter and
+// spectral gradient (pm_solver.cpp:925-951, :2398-2705, :4016-4061, :4581-4940).
+//
+// Layout: complex spectrum C[ix][iy][kz], kz in [0, nz/2] (row stride nzc);
+// real meshes use the same storage with padded rows of 2*nzc reals, so the
+// r2c and the three c2r transforms run in place. Forward passes: z (r2c, two
+// real rows packed into one complex FFT), y, x (+ Green's function). Inverse:
+// x twice (with/without -i kx), y three times, z (c2r) three times; the
+// constant 1/(cell volume * M) of the reference's density normalisation and
+// inverse FFT is folded into the Green's function.
+#include "pm.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace chui {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr int kFftThreads = 256;
+
+template <class T> struct C2;
+template <> struct C2<float> { using type = float2; };
+template <> struct C2<double> { using type = double2; };
+
+template <class V> __device__ __forceinline__ V cmul(V a, V b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <class V> __device__ __forceinline__ V cadd(V a, V b) { V r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <class V> __device__ __forceinline__ V csub(V a, V b) { V r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <class V, class T> __device__ __forceinline__ V mk(T x, T y) { V r; r.x = x; r.y = y; return r; }
+
+__device__ __forceinline__ int bitrev(int i, int lg) {
+  return lg == 0 ? 0 : static_cast<int>(__brev(static_cast<unsigned>(i)) >> (32 - lg));
+}
+
+// In-place radix-2 DIT on nlines LDS lines (input already bit-reversed).
+template <class T>
+__device__ void lds_fft(typename C2<T>::type* X, int nlines, int n, int lg, int stride,
+                        const typename C2<T>::type* tw, bool inverse) {
+  using V = typename C2<T>::type;
+  if (lg == 0) return;
+  const int halfn = n >> 1;
+  const int nb = nlines * halfn;
+  for (int s = 1; s <= lg; ++s) {
+    const int half = 1 << (s - 1);
+    const int tstep = n >> s;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      const int line = b / halfn;
+      const int q = b - line * halfn;
+      const int pos = q & (half - 1);
+      const int grp = q >> (s - 1);
+      const int i0 = grp * 2 * half + pos;
+      const int i1 = i0 + half;
+      V w = tw[pos * tstep];
+      if (inverse) w.y = -w.y;
+      V* Lp = X + line * stride;
+      const V u = Lp[i0];
+      const V vw = cmul(Lp[i1], w);
+      Lp[i0] = cadd(u, vw);
+      Lp[i1] = csub(u, vw);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- deposit ------------------------------------------------------------------
+__device__ __forceinline__ double wrap_position(double x, double L) {
+  const double w = fmod(x, L);
+  return w < 0.0 ? w + L : w;
+}
+__device__ __forceinline__ int wrap_index(long i, int n) {
+  long w = i % n;
+  if (w < 0) w += n;
+  return static_cast<int>(w);
+}
+
+struct DepositArgs {
+  int n[3];
+  int lg[3];
+  double inv_h[3];  // n / L
+  double L[3];
+  std::size_t row;
+  int scheme;
+};
+
+__global__ __launch_bounds__(256) void deposit64_kernel(DepositArgs a, const double* __restrict__ x,
+                                                        const double* __restrict__ y, const double* __restrict__ z,
+                                                        const double* __restrict__ m, double um, std::size_t np,
+                                                        double* __restrict__ rho) {
+  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double g[3] = {wrap_position(x[p], a.L[0]) * a.inv_h[0], wrap_position(y[p], a.L[1]) * a.inv_h[1],
+                       wrap_position(z[p], a.L[2]) * a.inv_h[2]};
+  int idx[3][3];
+  double w[3][3];
+  int cnt;
+  for (int ax = 0; ax < 3; ++ax) {
+    if (a.scheme == 0) {
+      const double base = floor(g[ax]);
+      const long i0 = static_cast<long>(base);
+      const double t = g[ax] - base;
+      idx[ax][0] = wrap_index(i0, a.n[ax]);
+      idx[ax][1] = wrap_index(i0 + 1, a.n[ax]);
+      w[ax][0] = 1.0 - t;
+      w[ax][1] = t;
+    } else {
+      const double c = floor(g[ax] + 0.5);
+      const long ic = static_cast<long>(c);
+      const double d = g[ax] - c;
+      idx[ax][0] = wrap_index(ic - 1, a.n[ax]);
+      idx[ax][1] = wrap_index(ic, a.n[ax]);
+      idx[ax][2] = wrap_index(ic + 1, a.n[ax]);
+      w[ax][0] = 0.5 * (0.5 - d) * (0.5 - d);
+      w[ax][1] = 0.75 - d * d;
+      w[ax][2] = 0.5 * (0.5 + d) * (0.5 + d);
+    }
+  }
+  cnt = a.scheme == 0 ? 2 : 3;
+  const double mm = m ? m[p] : um;
+  for (int i = 0; i < cnt; ++i)
+    for (int j = 0; j < cnt; ++j) {
+      const std::size_t rb = (static_cast<std::size_t>(idx[0][i]) * a.n[1] + idx[1][j]) * a.row;
+      for (int k = 0; k < cnt; ++k) {
+        const double wt = w[0][i] * w[1][j] * w[2][k];
+        atomicAdd(&rho[rb + idx[2][k]], mm * wt);
+      }
+    }
+}
+
+__global__ __launch_bounds__(256) void deposit32_kernel(DepositArgs a, const uint32_t* __restrict__ qx,
+                                                        const uint32_t* __restrict__ qy,
+                                                        const uint32_t* __restrict__ qz, const float* __restrict__ m,
+                                                        float um, std::size_t np, float* __restrict__ rho) {
+  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const uint32_t q[3] = {qx[p], qy[p], qz[p]};
+  int idx[3][3];
+  float w[3][3];
+  for (int ax = 0; ax < 3; ++ax) {
+    const int sh = 32 - a.lg[ax];
+    const int nn = a.n[ax];
+    const int cell = static_cast<int>(q[ax] >> sh);
+    const float t = static_cast<float>(q[ax] & ((1u << sh) - 1u)) * __builtin_ldexpf(1.0f, -sh);
+    if (a.scheme == 0) {
+      idx[ax][0] = cell;
+      idx[ax][1] = (cell + 1) & (nn - 1);
+      w[ax][0] = 1.0f - t;
+      w[ax][1] = t;
+    } else {
+      const int c = t >= 0.5f ? cell + 1 : cell;
+      const float d = t >= 0.5f ? t - 1.0f : t;
+      idx[ax][0] = (c - 1) & (nn - 1);
+      idx[ax][1] = c & (nn - 1);
+      idx[ax][2] = (c + 1) & (nn - 1);
+      w[ax][0] = 0.5f * (0.5f - d) * (0.5f - d);
+      w[ax][1] = 0.75f - d * d;
+      w[ax][2] = 0.5f * (0.5f + d) * (0.5f + d);
+    }
+  }
+  const int cnt = a.scheme == 0 ? 2 : 3;
+  const float mm = m ? m[p] : um;
+  for (int i = 0; i < cnt; ++i)
+    for (int j = 0; j < cnt; ++j) {
+      const std::size_t rb = (static_cast<std::size_t>(idx[0][i]) * a.n[1] + idx[1][j]) * a.row;
+      const float mij = mm * w[0][i] * w[1][j];
+      for (int k = 0; k < cnt; ++k) atomicAdd(&rho[rb + idx[2][k]], mij * w[2][k]);
+    }
+}
+
+// ---- z passes (real <-> half complex, two rows per complex FFT) -----------------
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::size_t rows, int nz, int lg,
+                                                                std::size_t row, const typename C2<T>::type* twz,
+                                                                int pairs_per_block) {
+  using V = typename C2<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  V* tw = reinterpret_cast<V*>(smem_raw);
+  V* X = tw + (nz >> 1);
+  const int stride = nz + 1;
+  for (int i = threadIdx.x; i < (nz >> 1); i += blockDim.x) tw[i] = twz[i];
+  const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
+  for (int idx = threadIdx.x; idx < pairs_per_block * nz; idx += blockDim.x) {
+    const int pp = idx / nz, i = idx - pp * nz;
+    const std::size_t pr = pair0 + pp;
+    V v = mk<V, T>(T(0), T(0));
+    if (2 * pr < rows) {
+      const T* ra = data + (2 * pr) * row;
+      v.x = ra[i];
+      if (2 * pr + 1 < rows) v.y = ra[row + i];
+    }
+    X[pp * stride + bitrev(i, lg)] = v;
+  }
+  __syncthreads();
+  lds_fft<T>(X, pairs_per_block, nz, lg, stride, tw, false);
+  const int nzc = (nz >> 1) + 1;
+  for (int idx = threadIdx.x; idx < pairs_per_block * nzc; idx += blockDim.x) {
+    const int pp = idx / nzc, k = idx - pp * nzc;
+    const std::size_t pr = pair0 + pp;
+    if (2 * pr >= rows) continue;
+    const V zk = X[pp * stride + (k == 0 ? 0 : k)];
+    const V zm = X[pp * stride + ((nz - k) & (nz - 1))];
+    const V zmc = mk<V, T>(zm.x, -zm.y);
+    const V s = cadd(zk, zmc), d = csub(zk, zmc);
+    V* ca = reinterpret_cast<V*>(data + (2 * pr) * row);
+    ca[k] = mk<V, T>(T(0.5) * s.x, T(0.5) * s.y);
+    if (2 * pr + 1 < rows) {
+      V* cb = reinterpret_cast<V*>(data + (2 * pr + 1) * row);
+      cb[k] = mk<V, T>(T(0.5) * d.y, T(-0.5) * d.x);  // (d) * (-i/2)
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::size_t rows, int nz, int lg,
+                                                                std::size_t row, const typename C2<T>::type* twz,
+                                                                const T* __restrict__ dkz, int premul,
+                                                                int pairs_per_block) {
+  using V = typename C2<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  V* tw = reinterpret_cast<V*>(smem_raw);
+  V* X = tw + (nz >> 1);
+  const int stride = nz + 1;
+  const int nzc = (nz >> 1) + 1;
+  for (int i = threadIdx.x; i < (nz >> 1); i += blockDim.x) tw[i] = twz[i];
+  const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
+  for (int idx = threadIdx.x; idx < pairs_per_block * nzc; idx += blockDim.x) {
+    const int pp = idx / nzc, k = idx - pp * nzc;
+    const std::size_t pr = pair0 + pp;
+    V xa = mk<V, T>(T(0), T(0)), xb = mk<V, T>(T(0), T(0));
+    if (2 * pr < rows) {
+      xa = reinterpret_cast<const V*>(data + (2 * pr) * row)[k];
+      if (2 * pr + 1 < rows) xb = reinterpret_cast<const V*>(data + (2 * pr + 1) * row)[k];
+    }
+    if (premul) {  // multiply by -i k_z
+      const T kk = dkz[k];
+      xa = mk<V, T>(xa.y * kk, -xa.x * kk);
+      xb = mk<V, T>(xb.y * kk, -xb.x * kk);
+    }
+    if (k == 0 || k == nz / 2) {
+      X[pp * stride + bitrev(k & (nz - 1), lg)] = mk<V, T>(xa.x, xb.x);
+    } else {
+      X[pp * stride + bitrev(k, lg)] = mk<V, T>(xa.x - xb.y, xa.y + xb.x);          // xa + i xb
+      X[pp * stride + bitrev(nz - k, lg)] = mk<V, T>(xa.x + xb.y, -xa.y + xb.x);    // conj(xa) + i conj(xb)
+    }
+  }
+  __syncthreads();
+  lds_fft<T>(X, pairs_per_block, nz, lg, stride, tw, true);
+  for (int idx = threadIdx.x; idx < pairs_per_block * nz; idx += blockDim.x) {
+    const int pp = idx / nz, i = idx - pp * nz;
+    const std::size_t pr = pair0 + pp;
+    if (2 * pr >= rows) continue;
+    const V v = X[pp * stride + i];
+    data[(2 * pr) * row + i] = v.x;
+    if (2 * pr + 1 < rows) data[(2 * pr + 1) * row + i] = v.y;
+  }
+}
+
+// ---- complex line passes (y and x) ---------------------------------------------
+struct LineArgs {
+  int n, lg;
+  std::size_t line_stride;   // complex elements between points of a line
+  std::size_t inner_count;   // contiguous line starts per outer index
+  std::size_t outer_stride;
+  int tj;                    // lines per block (power of two)
+  int inverse;
+  int premul;                // multiply by -i dk[i] before the transform
+  int green;                 // multiply by the Green's function after (x pass only)
+  // Green's function support (x pass): inner index j = iy * nzc + kz
+  std::size_t nzc;
+  double pref, split, fold;
+  int deconv;
+};
+
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void fft_lines_kernel(const typename C2<T>::type* in,
+                                                                typename C2<T>::type* out, LineArgs a,
+                                                                const typename C2<T>::type* twl,
+                                                                const T* __restrict__ dk,
+                                                                const T* __restrict__ kx, const T* __restrict__ ky,
+                                                                const T* __restrict__ kz, const T* __restrict__ wx,
+                                                                const T* __restrict__ wy, const T* __restrict__ wz) {
+  using V = typename C2<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  V* tw = reinterpret_cast<V*>(smem_raw);
+  V* X = tw + (a.n >> 1);
+  const int stride = a.n + 1;
+  for (int i = threadIdx.x; i < (a.n >> 1); i += blockDim.x) tw[i] = twl[i];
+  const std::size_t j0 = static_cast<std::size_t>(blockIdx.x) * a.tj;
+  const std::size_t obase = static_cast<std::size_t>(blockIdx.y) * a.outer_stride;
+  const int tjm = a.tj - 1;
+  int tjl = 0;
+  while ((1 << tjl) < a.tj) ++tjl;
+  for (int idx = threadIdx.x; idx < a.tj * a.n; idx += blockDim.x) {
+    const int jj = idx & tjm, i = idx >> tjl;
+    const std::size_t j = j0 + jj;
+    V v = mk<V, T>(T(0), T(0));
+    if (j < a.inner_count) v = in[obase + j + static_cast<std::size_t>(i) * a.line_stride];
+    if (a.premul) {
+      const T kk = dk[i];
+      v = mk<V, T>(v.y * kk, -v.x * kk);
+    }
+    X[jj * stride + bitrev(i, a.lg)] = v;
+  }
+  __syncthreads();
+  lds_fft<T>(X, a.tj, a.n, a.lg, stride, tw, a.inverse != 0);
+  for (int idx = threadIdx.x; idx < a.tj * a.n; idx += blockDim.x) {
+    const int jj = idx & tjm, i = idx >> tjl;
+    const std::size_t j = j0 + jj;
+    if (j >= a.inner_count) continue;
+    V v = X[jj * stride + i];
+    if (a.green) {
+      const std::size_t iy = j / a.nzc, iz = j - iy * a.nzc;
+      const T k2 = kx[i] * kx[i] + ky[iy] * ky[iy] + kz[iz] * kz[iz];
+      T gval = T(0);
+      if (k2 != T(0)) {
+        T corr = T(1);
+        if (a.deconv) {
+          const T tw_ = wx[i] * wy[iy] * wz[iz];
+          corr = T(1) / fmax(tw_ * tw_, T(1.0e-12));
+        }
+        T filt = T(1);
+        if (a.split > 0.0) {
+          const T ka = sqrt(k2) * T(a.split);
+          filt = exp(-(ka * ka));
+        }
+        gval = T(a.pref) * corr * filt / k2 * T(a.fold);
+      }
+      v.x *= gval;
+      v.y *= gval;
+    }
+    out[obase + j + static_cast<std::size_t>(i) * a.line_stride] = v;
+  }
+}
+
+// ---- host -------------------------------------------------------------------
+template <class T>
+struct Tables {
+  T *kx, *ky, *kz, *dkx, *dky, *dkz, *wx, *wy, *wz;
+  typename C2<T>::type *twx, *twy, *twz;
+};
+
+template <class T>
+Tables<T> upload_tables(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, hipStream_t s) {
+  using V = typename C2<T>::type;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const std::size_t nzc = g.nzc;
+  std::vector<T> host;
+  auto push_k = [&](int n, std::size_t count, double L, bool deriv, bool window, int wexp) {
+    const double dk = 2.0 * kPi / L;
+    for (std::size_t i = 0; i < count; ++i) {
+      const long mode = i <= static_cast<std::size_t>(n) / 2 ? static_cast<long>(i) : static_cast<long>(i) - n;
+      const double k = dk * static_cast<double>(mode);
+      if (window) {
+        double wv = 1.0;
+        if (o.deconvolution) {
+          const double xx = 0.5 * k * L / static_cast<double>(n);
+          const double sn = std::abs(xx) < 1.0e-12 ? 1.0 : std::sin(xx) / xx;
+          wv = std::pow(sn, wexp);
+        }
+        host.push_back(static_cast<T>(wv));
+      } else if (deriv) {
+        host.push_back(static_cast<T>((n % 2 == 0 && i == static_cast<std::size_t>(n) / 2) ? 0.0 : k));
+      } else {
+        host.push_back(static_cast<T>(k));
+      }
+    }
+  };
+  const int wexp = o.assignment == 0 ? 2 : 3;
+  std::size_t off[9];
+  off[0] = host.size(); push_k(nx, nx, g.L[0], false, false, wexp);
+  off[1] = host.size(); push_k(ny, ny, g.L[1], false, false, wexp);
+  off[2] = host.size(); push_k(nz, nzc, g.L[2], false, false, wexp);
+  off[3] = host.size(); push_k(nx, nx, g.L[0], true, false, wexp);
+  off[4] = host.size(); push_k(ny, ny, g.L[1], true, false, wexp);
+  off[5] = host.size(); push_k(nz, nzc, g.L[2], true, false, wexp);
+  off[6] = host.size(); push_k(nx, nx, g.L[0], false, true, wexp);
+  off[7] = host.size(); push_k(ny, ny, g.L[1], false, true, wexp);
+  off[8] = host.size(); push_k(nz, nzc, g.L[2], false, true, wexp);
+  while (host.size() % 4) host.push_back(T(0));
+  const std::size_t scal = host.size();
+  std::vector<V> tws;
+  std::size_t toff[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    toff[ax] = tws.size();
+    const int n = g.n[ax];
+    for (int k = 0; k < std::max(n / 2, 1); ++k) {
+      const double ang = -2.0 * kPi * static_cast<double>(k) / static_cast<double>(n);
+      V v;
+      v.x = static_cast<T>(std::cos(ang));
+      v.y = static_cast<T>(std::sin(ang));
+      tws.push_back(v);
+    }
+  }
+  const std::size_t bytes = scal * sizeof(T) + tws.size() * sizeof(V);
+  w.tables.reserve(bytes);
+  CHUI_HIP(hipMemcpyAsync(w.tables.ptr, host.data(), scal * sizeof(T), hipMemcpyHostToDevice, s));
+  CHUI_HIP(hipMemcpyAsync(static_cast<char*>(w.tables.ptr) + scal * sizeof(T), tws.data(), tws.size() * sizeof(V),
+                          hipMemcpyHostToDevice, s));
+  CHUI_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+  T* base = w.tables.as<T>();
+  V* tb = reinterpret_cast<V*>(base + scal);
+  return Tables<T>{base + off[0], base + off[1], base + off[2], base + off[3], base + off[4], base + off[5],
+                   base + off[6], base + off[7], base + off[8], tb + toff[0], tb + toff[1], tb + toff[2]};
+}
+
+template <class T>
+int lines_per_block(int n) {
+  using V = typename C2<T>::type;
+  int tj = 16;
+  while (tj > 1 && static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + (n / 2) * sizeof(V) > 64 * 1024) tj >>= 1;
+  return tj;
+}
+
+template <class T>
+void launch_lines(const typename C2<T>::type* in, typename C2<T>::type* out, LineArgs a, std::size_t outer_count,
+                  const typename C2<T>::type* tw, const T* dk, const Tables<T>& tb, hipStream_t s) {
+  using V = typename C2<T>::type;
+  a.tj = lines_per_block<T>(a.n);
+  const std::size_t smem = static_cast<std::size_t>(a.tj) * (a.n + 1) * sizeof(V) + (a.n / 2) * sizeof(V);
+  dim3 grid(static_cast<unsigned>((a.inner_count + a.tj - 1) / a.tj), static_cast<unsigned>(outer_count));
+  fft_lines_kernel<T><<<grid, kFftThreads, smem, s>>>(in, out, a, tw, dk, tb.kx, tb.ky, tb.kz, tb.wx, tb.wy, tb.wz);
+  CHUI_LAUNCH_CHECK("fft_lines_kernel");
+}
+
+template <class T>
+int pairs_per_block(int nz) {
+  using V = typename C2<T>::type;
+  int pp = 8;
+  while (pp > 1 && static_cast<std::size_t>(pp) * (nz + 1) * sizeof(V) + (nz / 2) * sizeof(V) > 48 * 1024) pp >>= 1;
+  return pp;
+}
+
+template <class T>
+void z_r2c(T* data, const PmGeometry& g, const Tables<T>& tb, hipStream_t s) {
+  using V = typename C2<T>::type;
+  const std::size_t rows = static_cast<std::size_t>(g.n[0]) * g.n[1];
+  const int pp = pairs_per_block<T>(g.n[2]);
+  const std::size_t pairs = (rows + 1) / 2;
+  const std::size_t smem = static_cast<std::size_t>(pp) * (g.n[2] + 1) * sizeof(V) + (g.n[2] / 2) * sizeof(V);
+  fft_z_r2c_kernel<T><<<static_cast<unsigned>((pairs + pp - 1) / pp), kFftThreads, smem, s>>>(
+      data, rows, g.n[2], g.lg[2], g.row, tb.twz, pp);
+  CHUI_LAUNCH_CHECK("fft_z_r2c_kernel");
+}
+
+template <class T>
+void z_c2r(T* data, const PmGeometry& g, const Tables<T>& tb, bool premul, hipStream_t s) {
+  using V = typename C2<T>::type;
+  const std::size_t rows = static_cast<std::size_t>(g.n[0]) * g.n[1];
+  const int pp = pairs_per_block<T>(g.n[2]);
+  const std::size_t pairs = (rows + 1) / 2;
+  const std::size_t smem = static_cast<std::size_t>(pp) * (g.n[2] + 1) * sizeof(V) + (g.n[2] / 2) * sizeof(V);
+  fft_z_c2r_kernel<T><<<static_cast<unsigned>((pairs + pp - 1) / pp), kFftThreads, smem, s>>>(
+      data, rows, g.n[2], g.lg[2], g.row, tb.twz, tb.dkz, premul ? 1 : 0, pp);
+  CHUI_LAUNCH_CHECK("fft_z_c2r_kernel");
+}
+
+template <class T>
+void solve_spectral(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const Tables<T>& tb, hipStream_t s) {
+  using V = typename C2<T>::type;
+  T* A = w.buf[0].as<T>();
+  T* B = w.buf[1].as<T>();
+  T* Cb = w.buf[2].as<T>();
+  V* Ac = reinterpret_cast<V*>(A);
+  V* Bc = reinterpret_cast<V*>(B);
+  V* Cc = reinterpret_cast<V*>(Cb);
+  const int nx = g.n[0], ny = g.n[1];
+  const std::size_t nzc = g.nzc;
+  const double M = static_cast<double>(g.n[0]) * g.n[1] * g.n[2];
+  const double cell_volume = (g.L[0] * g.L[1] * g.L[2]) / M;
+  // forward z (r2c)
+  z_r2c<T>(A, g, tb, s);
+  // forward y: lines along y, inner = kz (contiguous), outer = ix
+  LineArgs ya{};
+  ya.n = ny; ya.lg = g.lg[1]; ya.line_stride = nzc; ya.inner_count = nzc; ya.outer_stride = static_cast<std::size_t>(ny) * nzc;
+  ya.inverse = 0; ya.premul = 0; ya.green = 0; ya.nzc = nzc;
+  launch_lines<T>(Ac, Ac, ya, nx, tb.twy, nullptr, tb, s);
+  // forward x + Green: lines along x, inner = (iy, kz)
+  LineArgs xa{};
+  xa.n = nx; xa.lg = g.lg[0]; xa.line_stride = static_cast<std::size_t>(ny) * nzc; xa.inner_count = static_cast<std::size_t>(ny) * nzc;
+  xa.outer_stride = 0; xa.inverse = 0; xa.premul = 0; xa.green = 1; xa.nzc = nzc;
+  xa.pref = -4.0 * kPi * o.G;
+  xa.split = o.split_scale;
+  xa.fold = 1.0 / (cell_volume * M);
+  xa.deconv = o.deconvolution;
+  launch_lines<T>(Ac, Ac, xa, 1, tb.twx, nullptr, tb, s);
+  // inverse x: Qx = IFFT_x(-i kx Phi) -> B ; P = IFFT_x(Phi) -> C
+  LineArgs xi = xa;
+  xi.inverse = 1; xi.green = 0; xi.premul = 1;
+  launch_lines<T>(Ac, Bc, xi, 1, tb.twx, tb.dkx, tb, s);
+  xi.premul = 0;
+  launch_lines<T>(Ac, Cc, xi, 1, tb.twx, nullptr, tb, s);
+  // inverse y: B -> B ; C (-i ky) -> A ; C -> C
+  LineArgs yi = ya;
+  yi.inverse = 1;
+  launch_lines<T>(Bc, Bc, yi, nx, tb.twy, nullptr, tb, s);
+  yi.premul = 1;
+  launch_lines<T>(Cc, Ac, yi, nx, tb.twy, tb.dky, tb, s);
+  yi.premul = 0;
+  launch_lines<T>(Cc, Cc, yi, nx, tb.twy, nullptr, tb, s);
+  // inverse z (c2r) in place: B -> Ax, A -> Ay, C (-i kz) -> Az
+  z_c2r<T>(B, g, tb, false, s);
+  z_c2r<T>(A, g, tb, false, s);
+  z_c2r<T>(Cb, g, tb, true, s);
+  w.force[0] = B;
+  w.force[1] = A;
+  w.force[2] = Cb;
+}
+
+template <class T>
+void reserve(PmWorkspace& w, const PmGeometry& g) {
+  const std::size_t bytes = g.complex_cells() * 2 * sizeof(T);
+  for (auto& b : w.buf) b.reserve(bytes);
+  w.cells = g.complex_cells();
+  w.elem_bytes = sizeof(T);
+}
+
+DepositArgs deposit_args(const PmGeometry& g, const PmOptions& o) {
+  DepositArgs a{};
+  for (int ax = 0; ax < 3; ++ax) {
+    a.n[ax] = g.n[ax];
+    a.lg[ax] = g.lg[ax];
+    a.L[ax] = g.L[ax];
+    a.inv_h[ax] = static_cast<double>(g.n[ax]) / g.L[ax];
+  }
+  a.row = g.row;
+  a.scheme = o.assignment;
+  return a;
+}
+
+}  // namespace
+
+void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double* x, const double* y,
+                   const double* z, const double* m, double uniform_mass, std::size_t n, hipStream_t s) {
+  reserve<double>(w, g);
+  const Tables<double> tb = upload_tables<double>(w, g, o, s);
+  double* rho = w.buf[0].as<double>();
+  CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(double), s));
+  if (n > 0) {
+    deposit64_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), x, y, z, m, uniform_mass, n, rho);
+    CHUI_LAUNCH_CHECK("deposit64_kernel");
+  }
+  solve_spectral<double>(w, g, o, tb, s);
+  w.valid = true;
+}
+
+void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint32_t* qx,
+                    const uint32_t* qy, const uint32_t* qz, const float* m, float uniform_mass, std::size_t n,
+                    hipStream_t s) {
+  reserve<float>(w, g);
+  const Tables<float> tb = upload_tables<float>(w, g, o, s);
+  float* rho = w.buf[0].as<float>();
+  CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(float), s));
+  if (n > 0) {
+    deposit32_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), qx, qy, qz, m, uniform_mass, n, rho);
+    CHUI_LAUNCH_CHECK("deposit32_kernel");
+  }
+  solve_spectral<float>(w, g, o, tb, s);
+  w.valid = true;
+}
+
+}  // namespace chui

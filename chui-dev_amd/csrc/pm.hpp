@@ -1,0 +1,55 @@
+// pm.hpp — periodic particle-mesh long-range solve on the device.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace chui {
+
+struct PmGeometry {
+  int n[3];          // mesh (powers of two)
+  int lg[3];         // log2(n)
+  double L[3];       // box lengths
+  std::size_t nzc;   // nz/2 + 1
+  std::size_t row;   // padded real row stride = 2 * nzc
+  std::size_t complex_cells() const { return static_cast<std::size_t>(n[0]) * n[1] * nzc; }
+};
+
+struct PmOptions {
+  double G = 1.0;
+  double split_scale = 0.0;
+  int assignment = 0;      // 0 CIC, 1 TSC
+  int deconvolution = 0;
+};
+
+// Device mesh workspace for one precision. Three complex buffers; the real
+// force meshes are written in place (padded rows) at the end of the solve.
+struct PmWorkspace {
+  DeviceBuffer buf[3];
+  DeviceBuffer tables;   // k, derivative-k, window tables + twiddles
+  std::size_t cells = 0;
+  int elem_bytes = 0;
+  // After a solve: real force meshes (padded stride row) per axis.
+  void* force[3] = {nullptr, nullptr, nullptr};
+  bool valid = false;
+  // identity of the field (cadence reuse)
+  double key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Deposit (tree-ordered particles), forward r2c FFT, Green's function with the
+// Gaussian long-range filter, spectral gradient and three c2r inverses.
+// FP64: positions are raw caller doubles; masses double (null = uniform).
+void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double* x, const double* y,
+                   const double* z, const double* m, double uniform_mass, std::size_t n, hipStream_t s);
+// Mixed: positions are 32-bit fixed-point periodic coordinates, FP32 mesh/FFT.
+void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint32_t* qx,
+                    const uint32_t* qy, const uint32_t* qz, const float* m, float uniform_mass, std::size_t n,
+                    hipStream_t s);
+
+// Stage timers (ms) of the last solve, filled when events are enabled.
+struct PmTimings {
+  double deposit_ms = 0, fft_ms = 0;
+};
+
+}  // namespace chui

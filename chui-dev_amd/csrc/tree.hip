@@ -1,0 +1,1061 @@
+// tree.hip — periodic unwrap, octant-path keys, level-synchronous octree build,
+// preorder numbering, in-leaf Morton order and FP64 multipoles on gfx950.
+//
+// The reference builds its octree top-down by recursive stable octant
+// partitions of the Morton-presorted particle list against FP64 cube centres
+// (tree_gravity.cpp:1081-1206). The same tree is produced here without
+// recursion:
+//   1. every particle computes its 21-level octant path against the exact FP64
+//      centre sequence c +- h/2 (the centres the recursion would visit);
+//   2. one stable radix sort by that path gives the preorder leaf sequence;
+//   3. nodes are discovered level by level with binary searches on the sorted
+//      paths (a node is split while count > leaf_size and half > 1e-12);
+//   4. inside each leaf the particles are put back into Morton order, because
+//      stable partitions preserve the Morton order of a leaf's members.
+// All FP64 arithmetic that decides topology or feeds multipoles is written in
+// the reference's operation order and compiled with -ffp-contract=off, so keys,
+// order, node membership and moments are bit-identical to the CPU path.
+#include "tree.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "primitives.hpp"
+
+namespace chui {
+namespace {
+
+constexpr uint64_t kEmpty = ~0ull;
+constexpr uint32_t kInvalidIndex = 0xFFFFFFFFu;
+constexpr int kPathLevels = 21;
+
+struct Axes3 {
+  const double* p[3];
+};
+struct UnwrapArgs {
+  double L[3];
+  int canonical[3];
+  double anchor[3];
+};
+
+__device__ __forceinline__ double wrap_coord(double x, double L, int canonical) {
+  // tree_pm_coupling.cpp:164-175
+  double v = canonical ? x : x - L * floor(x / L);
+  if (v >= L) v = 0.0;
+  else if (v < 0.0) v += L;
+  if (v == 0.0) v = 0.0;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Source statistics: per-axis min/max (order keys), finiteness and mass checks.
+struct SourceStats {
+  uint64_t lo[3];
+  uint64_t hi[3];
+  uint32_t bad_coord;
+  uint32_t bad_mass;
+  uint32_t mass_nonuniform;
+  uint32_t pad;
+};
+
+__global__ void stats_init_kernel(SourceStats* st) {
+  for (int a = 0; a < 3; ++a) { st->lo[a] = kEmpty; st->hi[a] = 0; }
+  st->bad_coord = 0;
+  st->bad_mass = 0;
+  st->mass_nonuniform = 0;
+  st->pad = 0;
+}
+
+__global__ __launch_bounds__(256) void source_stats_kernel(Axes3 p, const double* __restrict__ m, std::size_t n,
+                                                           SourceStats* st) {
+  uint64_t lo[3] = {kEmpty, kEmpty, kEmpty}, hi[3] = {0, 0, 0};
+  uint32_t bc = 0, bm = 0, nu = 0;
+  const double m0 = (m != nullptr && n > 0) ? m[0] : 0.0;
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double v = p.p[a][i];
+      if (!isfinite(v)) {
+        bc = 1;
+      } else {
+        const uint64_t k = order_key(v);
+        lo[a] = k < lo[a] ? k : lo[a];
+        hi[a] = k > hi[a] ? k : hi[a];
+      }
+    }
+    if (m != nullptr) {
+      const double mm = m[i];
+      if (!(isfinite(mm) && mm >= 0.0)) bm = 1;
+      if (mm != m0) nu = 1;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = wave_min(lo[a]);
+    hi[a] = wave_max(hi[a]);
+  }
+  bc = wave_max(bc);
+  bm = wave_max(bm);
+  nu = wave_max(nu);
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (lo[a] != kEmpty) atomicMin(reinterpret_cast<unsigned long long*>(&st->lo[a]), lo[a]);
+      if (hi[a] != 0) atomicMax(reinterpret_cast<unsigned long long*>(&st->hi[a]), hi[a]);
+    }
+    if (bc) atomicOr(&st->bad_coord, 1u);
+    if (bm) atomicOr(&st->bad_mass, 1u);
+    if (nu) atomicOr(&st->mass_nonuniform, 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Periodic unwrap (tree_pm_coupling.cpp:141-218) without a sort.
+// With B >= 2n buckets of width L/B over [0, L), every gap between two values
+// in one bucket is < L/(2n)(1+4e-9) while the largest circular gap is >= L/n,
+// so the largest gap (and every gap tying with it) lies between the maximum of
+// a non-empty bucket and the minimum of the next non-empty bucket. Per-bucket
+// min/max of the IEEE bits (non-negative doubles order like their bits), an
+// exclusive max-scan and one argmax (ties -> smaller anchor) give the exact
+// anchor the reference obtains after its 8-pass radix sort.
+struct BucketArgs {
+  double L[3];
+  double scale[3];  // B / L
+  int canonical[3];
+  int naxes;
+  uint64_t B;
+};
+
+__global__ __launch_bounds__(256) void bucket_kernel(Axes3 p, std::size_t n, BucketArgs a,
+                                                     uint64_t* __restrict__ bmin, uint64_t* __restrict__ bmax) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    for (int ax = 0; ax < a.naxes; ++ax) {
+      const double v = wrap_coord(p.p[ax][i], a.L[ax], a.canonical[ax]);
+      uint64_t b = static_cast<uint64_t>(v * a.scale[ax]);
+      if (b >= a.B) b = a.B - 1;
+      const uint64_t bits = dbits(v);
+      atomicMin(reinterpret_cast<unsigned long long*>(&bmin[ax * a.B + b]), bits);
+      atomicMax(reinterpret_cast<unsigned long long*>(&bmax[ax * a.B + b]), bits + 1);  // 0 = empty
+    }
+  }
+}
+
+constexpr int kGapThreads = 256;
+constexpr int kGapItems = 8;
+constexpr int kGapTile = kGapThreads * kGapItems;
+
+__device__ __forceinline__ uint64_t block_max_u64(uint64_t v) {
+  __shared__ uint64_t wmax[kGapThreads / kWave];
+  v = wave_max(v);
+  if (lane_id() == 0) wmax[threadIdx.x / kWave] = v;
+  __syncthreads();
+  uint64_t r = 0;
+#pragma unroll
+  for (int i = 0; i < kGapThreads / kWave; ++i) r = wmax[i] > r ? wmax[i] : r;
+  __syncthreads();
+  return r;
+}
+
+// Exclusive max-scan over the block (blocked thread order); returns the
+// exclusive prefix for this thread and the block total.
+__device__ __forceinline__ uint64_t block_excl_max_u64(uint64_t v, uint64_t* total) {
+  __shared__ uint64_t wtot[kGapThreads / kWave];
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  uint64_t inc = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint64_t o = __shfl_up(inc, off, kWave);
+    if (lane >= off) inc = o > inc ? o : inc;
+  }
+  uint64_t excl = __shfl_up(inc, 1, kWave);
+  if (lane == 0) excl = 0;
+  if (lane == kWave - 1) wtot[w] = inc;
+  __syncthreads();
+  uint64_t woff = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kGapThreads / kWave; ++i) {
+    if (i < w) woff = wtot[i] > woff ? wtot[i] : woff;
+    tot = wtot[i] > tot ? wtot[i] : tot;
+  }
+  __syncthreads();
+  *total = tot;
+  return excl > woff ? excl : woff;
+}
+
+__global__ __launch_bounds__(kGapThreads) void bucket_blockmax_kernel(const uint64_t* __restrict__ bmax, uint64_t B,
+                                                                      uint64_t* __restrict__ blockmax, uint64_t nblk) {
+  const int ax = blockIdx.y;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kGapTile;
+  uint64_t mx = 0;
+#pragma unroll
+  for (int k = 0; k < kGapItems; ++k) {
+    const uint64_t idx = base + static_cast<uint64_t>(k) * kGapThreads + threadIdx.x;
+    if (idx < B) {
+      const uint64_t v = bmax[ax * B + idx];
+      mx = v > mx ? v : mx;
+    }
+  }
+  mx = block_max_u64(mx);
+  if (threadIdx.x == 0) blockmax[ax * nblk + blockIdx.x] = mx;
+}
+
+// One block per axis: exclusive max-scan of the block maxima in place.
+__global__ __launch_bounds__(kGapThreads) void blockmax_scan_kernel(uint64_t* blockmax, uint64_t nblk,
+                                                                    uint64_t* total) {
+  const int ax = blockIdx.x;
+  uint64_t carry = 0;
+  for (uint64_t c = 0; c < nblk; c += kGapThreads) {
+    const uint64_t idx = c + threadIdx.x;
+    const uint64_t v = idx < nblk ? blockmax[ax * nblk + idx] : 0;
+    uint64_t tot;
+    uint64_t ex = block_excl_max_u64(v, &tot);
+    ex = ex > carry ? ex : carry;
+    if (idx < nblk) blockmax[ax * nblk + idx] = ex;
+    carry = tot > carry ? tot : carry;
+  }
+  if (threadIdx.x == 0) total[ax] = carry;
+}
+
+struct GapBest {
+  double gap;
+  double anchor;
+  double umax;  // largest unwrapped coordinate if this gap is chosen
+  int valid;
+  int pad;
+};
+
+__device__ __forceinline__ bool better(const GapBest& a, const GapBest& b) {
+  if (!a.valid) return false;
+  if (!b.valid) return true;
+  if (a.gap > b.gap) return true;
+  if (a.gap == b.gap && a.anchor < b.anchor) return true;
+  return false;
+}
+
+__device__ __forceinline__ GapBest shfl_best(const GapBest& v, int src) {
+  GapBest r;
+  r.gap = __shfl(v.gap, src, kWave);
+  r.anchor = __shfl(v.anchor, src, kWave);
+  r.umax = __shfl(v.umax, src, kWave);
+  r.valid = __shfl(v.valid, src, kWave);
+  r.pad = 0;
+  return r;
+}
+
+__device__ GapBest block_best(GapBest b) {
+  __shared__ GapBest wb[kGapThreads / kWave];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const GapBest o = shfl_best(b, lane_id() ^ off);
+    if (better(o, b)) b = o;
+  }
+  if (lane_id() == 0) wb[threadIdx.x / kWave] = b;
+  __syncthreads();
+  GapBest r = wb[0];
+  for (int i = 1; i < kGapThreads / kWave; ++i)
+    if (better(wb[i], r)) r = wb[i];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kGapThreads) void gap_kernel(const uint64_t* __restrict__ bmin,
+                                                          const uint64_t* __restrict__ bmax, uint64_t B,
+                                                          const uint64_t* __restrict__ block_prefix,
+                                                          const uint64_t* __restrict__ total, uint64_t nblk,
+                                                          BucketArgs a, GapBest* __restrict__ block_out) {
+  const int ax = blockIdx.y;
+  const double L = a.L[ax];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kGapTile + static_cast<uint64_t>(threadIdx.x) * kGapItems;
+  uint64_t tmax = 0;
+#pragma unroll
+  for (int k = 0; k < kGapItems; ++k) {
+    const uint64_t idx = base + k;
+    if (idx < B) {
+      const uint64_t v = bmax[ax * B + idx];
+      tmax = v > tmax ? v : tmax;
+    }
+  }
+  uint64_t tot;
+  uint64_t run = block_excl_max_u64(tmax, &tot);
+  const uint64_t bp = block_prefix[ax * nblk + blockIdx.x];
+  run = run > bp ? run : bp;
+  const double gmax = bitsd(total[ax] - 1);
+  GapBest best{0.0, 0.0, 0.0, 0, 0};
+  for (int k = 0; k < kGapItems; ++k) {
+    const uint64_t idx = base + k;
+    if (idx >= B) break;
+    const uint64_t mn = bmin[ax * B + idx];
+    if (mn != kEmpty) {
+      const double v = bitsd(mn);
+      GapBest c;
+      c.valid = 1;
+      c.pad = 0;
+      c.anchor = v;
+      if (run == 0) {  // first non-empty bucket: circular wrap gap
+        c.gap = (v + L) - gmax;
+        c.umax = gmax;
+      } else {
+        const double pr = bitsd(run - 1);
+        c.gap = v - pr;
+        c.umax = pr + L;
+      }
+      if (better(c, best)) best = c;
+    }
+    const uint64_t mx = bmax[ax * B + idx];
+    run = mx > run ? mx : run;
+  }
+  best = block_best(best);
+  if (threadIdx.x == 0) block_out[ax * nblk + blockIdx.x] = best;
+}
+
+__global__ __launch_bounds__(kGapThreads) void gap_final_kernel(const GapBest* __restrict__ block_out, uint64_t nblk,
+                                                                UnwrapResult* res) {
+  const int ax = blockIdx.x;
+  GapBest best{0.0, 0.0, 0.0, 0, 0};
+  for (uint64_t i = threadIdx.x; i < nblk; i += kGapThreads) {
+    const GapBest c = block_out[ax * nblk + i];
+    if (better(c, best)) best = c;
+  }
+  best = block_best(best);
+  if (threadIdx.x == 0) {
+    res->anchor[ax] = best.anchor;
+    res->umin[ax] = best.anchor;
+    res->umax[ax] = best.umax;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Keys.
+__device__ __forceinline__ uint64_t expand21(uint32_t v) {
+  uint64_t x = v & 0x1FFFFFu;
+  x = (x | (x << 32)) & 0x1F00000000FFFFull;
+  x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+  x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+  x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+struct MortonArgs {
+  double lo[3];
+  double extent;  // max(max extent, 1e-12)  (tree_ordering.cpp:151)
+};
+
+// tree_ordering.cpp:159-167: clamp((x-min)/extent,0,1) * (2^21-1), llround.
+__device__ __forceinline__ uint64_t morton_key(double x, double y, double z, const MortonArgs& a) {
+  const double grid = 2097151.0;
+  const double nx = fmin(fmax((x - a.lo[0]) / a.extent, 0.0), 1.0);
+  const double ny = fmin(fmax((y - a.lo[1]) / a.extent, 0.0), 1.0);
+  const double nz = fmin(fmax((z - a.lo[2]) / a.extent, 0.0), 1.0);
+  const uint32_t qx = static_cast<uint32_t>(llround(nx * grid));
+  const uint32_t qy = static_cast<uint32_t>(llround(ny * grid));
+  const uint32_t qz = static_cast<uint32_t>(llround(nz * grid));
+  return (expand21(qx) << 2) | (expand21(qy) << 1) | expand21(qz);
+}
+
+__device__ __forceinline__ double unwrap_coord(double x, double L, int canonical, double anchor) {
+  const double v = wrap_coord(x, L, canonical);
+  return v < anchor ? v + L : v;
+}
+
+struct PathArgs {
+  UnwrapArgs u;
+  double c0[3];
+  double h0;
+};
+
+// 21-level octant path of every particle against the reference's FP64 centres.
+__global__ __launch_bounds__(256) void path_key_kernel(Axes3 p, std::size_t n, PathArgs a, uint64_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double ux = unwrap_coord(p.p[0][i], a.u.L[0], a.u.canonical[0], a.u.anchor[0]);
+  const double uy = unwrap_coord(p.p[1][i], a.u.L[1], a.u.canonical[1], a.u.anchor[1]);
+  const double uz = unwrap_coord(p.p[2][i], a.u.L[2], a.u.canonical[2], a.u.anchor[2]);
+  double cx = a.c0[0], cy = a.c0[1], cz = a.c0[2], h = a.h0;
+  uint64_t key = 0;
+  for (int lvl = 0; lvl < kPathLevels; ++lvl) {
+    const uint32_t ox = ux >= cx ? 1u : 0u, oy = uy >= cy ? 1u : 0u, oz = uz >= cz ? 1u : 0u;
+    key = (key << 3) | (ox << 2) | (oy << 1) | oz;
+    const double ch = 0.5 * h;
+    cx = cx + (ox ? ch : -ch);
+    cy = cy + (oy ? ch : -ch);
+    cz = cz + (oz ? ch : -ch);
+    h = ch;
+  }
+  keys[i] = key;
+  vals[i] = static_cast<uint32_t>(i);
+}
+
+__global__ __launch_bounds__(256) void morton_keys_kernel(Axes3 p, std::size_t n, MortonArgs a,
+                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = morton_key(p.p[0][i], p.p[1][i], p.p[2][i], a);
+  vals[i] = static_cast<uint32_t>(i);
+}
+
+// Morton key of the particle at each tree position (unwrapped coordinates).
+__global__ __launch_bounds__(256) void leaf_morton_kernel(Axes3 p, std::size_t n, UnwrapArgs u, MortonArgs a,
+                                                          const uint32_t* __restrict__ order,
+                                                          uint64_t* __restrict__ mk) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = order[i];
+  mk[i] = morton_key(unwrap_coord(p.p[0][r], u.L[0], u.canonical[0], u.anchor[0]),
+                     unwrap_coord(p.p[1][r], u.L[1], u.canonical[1], u.anchor[1]),
+                     unwrap_coord(p.p[2][r], u.L[2], u.canonical[2], u.anchor[2]), a);
+}
+
+// ---------------------------------------------------------------------------
+// Level-synchronous node discovery on the sorted path keys (BFS numbering).
+// meta = depth | octant << 8 | leaf << 16
+__device__ __forceinline__ uint32_t lower_bound_digit(const uint64_t* __restrict__ k, uint32_t b, uint32_t e,
+                                                      int shift, uint32_t digit) {
+  while (b < e) {
+    const uint32_t mid = b + ((e - b) >> 1);
+    if (static_cast<uint32_t>((k[mid] >> shift) & 7u) < digit) b = mid + 1;
+    else e = mid;
+  }
+  return b;
+}
+
+struct LevelArgs {
+  uint32_t lvl_begin, lvl_count, next_begin, leaf_size;
+};
+
+__global__ __launch_bounds__(256) void level_count_kernel(LevelArgs a, const uint32_t* __restrict__ nb,
+                                                          const uint32_t* __restrict__ nc, uint32_t* __restrict__ meta,
+                                                          const double4* __restrict__ geo,
+                                                          const uint64_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ nchild, uint32_t* __restrict__ flags) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.lvl_count) return;
+  const uint32_t node = a.lvl_begin + k;
+  const uint32_t cnt = nc[node];
+  const double half = geo[node].w;
+  const uint32_t depth = meta[node] & 0xFFu;
+  if (cnt <= a.leaf_size || half <= 1.0e-12) {  // tree_gravity.cpp:1133
+    nchild[node] = 0;
+    meta[node] |= 1u << 16;
+    atomicAdd(&flags[2], 1u);
+    return;
+  }
+  if (depth >= static_cast<uint32_t>(kPathLevels)) {
+    atomicOr(&flags[0], 1u);
+    nchild[node] = 0;
+    meta[node] |= 1u << 16;
+    return;
+  }
+  const int shift = 3 * (kPathLevels - 1 - static_cast<int>(depth));
+  const uint32_t b = nb[node], e = b + cnt;
+  uint32_t prev = b, c = 0;
+  for (uint32_t o = 1; o <= 8; ++o) {
+    const uint32_t hi = o == 8 ? e : lower_bound_digit(keys, prev, e, shift, o);
+    c += hi > prev ? 1u : 0u;
+    prev = hi;
+  }
+  nchild[node] = c;
+}
+
+__global__ __launch_bounds__(256) void level_emit_kernel(LevelArgs a, uint32_t* __restrict__ nb,
+                                                         uint32_t* __restrict__ nc, uint32_t* __restrict__ meta,
+                                                         double4* __restrict__ geo, const uint64_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ nchild,
+                                                         const uint32_t* __restrict__ cbase,
+                                                         uint32_t* __restrict__ first_child) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.lvl_count) return;
+  const uint32_t node = a.lvl_begin + k;
+  if (nchild[node] == 0) {
+    first_child[node] = kInvalidIndex;
+    return;
+  }
+  const uint32_t depth = meta[node] & 0xFFu;
+  const int shift = 3 * (kPathLevels - 1 - static_cast<int>(depth));
+  const uint32_t b = nb[node], e = b + nc[node];
+  const double4 g = geo[node];
+  const uint32_t fc = a.next_begin + cbase[k];
+  first_child[node] = fc;
+  uint32_t j = 0, prev = b;
+  for (uint32_t o = 0; o < 8; ++o) {
+    const uint32_t hi = o == 7 ? e : lower_bound_digit(keys, prev, e, shift, o + 1);
+    if (hi > prev) {
+      const uint32_t c = fc + j++;
+      nb[c] = prev;
+      nc[c] = hi - prev;
+      meta[c] = (depth + 1u) | (o << 8);
+      const double ch = 0.5 * g.w;  // tree_gravity.cpp:1180-1184
+      double4 cg;
+      cg.x = g.x + ((o & 4u) ? ch : -ch);
+      cg.y = g.y + ((o & 2u) ? ch : -ch);
+      cg.z = g.z + ((o & 1u) ? ch : -ch);
+      cg.w = ch;
+      geo[c] = cg;
+    }
+    prev = hi;
+  }
+}
+
+__global__ void root_init_kernel(uint32_t* nb, uint32_t* nc, uint32_t* meta, double4* geo, uint32_t n, double cx,
+                                 double cy, double cz, double h) {
+  nb[0] = 0;
+  nc[0] = n;
+  meta[0] = 0;
+  geo[0] = make_double4(cx, cy, cz, h);
+}
+
+// Bottom-up subtree sizes (in nodes), one level per launch.
+__global__ __launch_bounds__(256) void subtree_kernel(uint32_t lb, uint32_t cnt, const uint32_t* __restrict__ nchild,
+                                                      const uint32_t* __restrict__ first_child,
+                                                      uint32_t* __restrict__ subtree) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  const uint32_t node = lb + k;
+  uint32_t s = 1;
+  const uint32_t c = nchild[node];
+  for (uint32_t j = 0; j < c; ++j) s += subtree[first_child[node] + j];
+  subtree[node] = s;
+}
+
+// Top-down preorder indices, one level per launch.
+__global__ __launch_bounds__(256) void preorder_kernel(uint32_t lb, uint32_t cnt, const uint32_t* __restrict__ nchild,
+                                                       const uint32_t* __restrict__ first_child,
+                                                       const uint32_t* __restrict__ subtree,
+                                                       uint32_t* __restrict__ pre) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  const uint32_t node = lb + k;
+  uint32_t run = pre[node] + 1;
+  const uint32_t c = nchild[node];
+  for (uint32_t j = 0; j < c; ++j) {
+    const uint32_t ch = first_child[node] + j;
+    pre[ch] = run;
+    run += subtree[ch];
+  }
+}
+
+// In-leaf Morton order: insertion sort of (morton key, source row) pairs.
+__global__ __launch_bounds__(256) void leaf_sort_kernel(uint32_t total_nodes, const uint32_t* __restrict__ nchild,
+                                                        const uint32_t* __restrict__ nb, const uint32_t* __restrict__ nc,
+                                                        uint64_t* __restrict__ mk, uint32_t* __restrict__ order) {
+  const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= total_nodes || nchild[node] != 0) return;
+  const uint32_t b = nb[node], e = b + nc[node];
+  for (uint32_t i = b + 1; i < e; ++i) {
+    const uint64_t kk = mk[i];
+    const uint32_t kv = order[i];
+    uint32_t j = i;
+    while (j > b && (mk[j - 1] > kk || (mk[j - 1] == kk && order[j - 1] > kv))) {
+      mk[j] = mk[j - 1];
+      order[j] = order[j - 1];
+      --j;
+    }
+    mk[j] = kk;
+    order[j] = kv;
+  }
+}
+
+// Multipoles, one BFS level per launch from the deepest level up
+// (tree_gravity.cpp:1208-1380, same operation order).
+struct MomArgs {
+  UnwrapArgs u;
+  uint32_t lb, cnt;
+  int quad;
+};
+
+__global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, Axes3 p, const double* __restrict__ m,
+                                                        const uint32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ nb,
+                                                        const uint32_t* __restrict__ nc,
+                                                        const uint32_t* __restrict__ nchild,
+                                                        const uint32_t* __restrict__ first_child,
+                                                        const double4* __restrict__ geo, double4* __restrict__ mom,
+                                                        double* __restrict__ quad /* 8 per node */) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.cnt) return;
+  const uint32_t node = a.lb + k;
+  const double4 g = geo[node];
+  double* q = quad + 8ull * node;
+  const uint32_t c = nchild[node];
+  if (c == 0) {
+    const uint32_t b = nb[node], e = b + nc[node];
+    double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+    for (uint32_t i = b; i < e; ++i) {
+      const uint32_t r = order[i];
+      const double mm = m[r];
+      tm += mm;
+      wx += mm * unwrap_coord(p.p[0][r], a.u.L[0], a.u.canonical[0], a.u.anchor[0]);
+      wy += mm * unwrap_coord(p.p[1][r], a.u.L[1], a.u.canonical[1], a.u.anchor[1]);
+      wz += mm * unwrap_coord(p.p[2][r], a.u.L[2], a.u.canonical[2], a.u.anchor[2]);
+    }
+    double4 M = make_double4(g.x, g.y, g.z, tm);
+    if (tm > 0.0) {
+      M.x = wx / tm;
+      M.y = wy / tm;
+      M.z = wz / tm;
+    }
+    mom[node] = M;
+    double ixx = 0, ixy = 0, ixz = 0, iyy = 0, iyz = 0, izz = 0;
+    if (a.quad && tm > 0.0) {
+      for (uint32_t i = b; i < e; ++i) {
+        const uint32_t r = order[i];
+        const double mm = m[r];
+        const double rx = unwrap_coord(p.p[0][r], a.u.L[0], a.u.canonical[0], a.u.anchor[0]) - M.x;
+        const double ry = unwrap_coord(p.p[1][r], a.u.L[1], a.u.canonical[1], a.u.anchor[1]) - M.y;
+        const double rz = unwrap_coord(p.p[2][r], a.u.L[2], a.u.canonical[2], a.u.anchor[2]) - M.z;
+        ixx += mm * rx * rx;
+        ixy += mm * rx * ry;
+        ixz += mm * rx * rz;
+        iyy += mm * ry * ry;
+        iyz += mm * ry * rz;
+        izz += mm * rz * rz;
+      }
+      const double tr = ixx + iyy + izz;
+      q[0] = 3.0 * ixx - tr;
+      q[1] = 3.0 * ixy;
+      q[2] = 3.0 * ixz;
+      q[3] = 3.0 * iyy - tr;
+      q[4] = 3.0 * iyz;
+      q[5] = 3.0 * izz - tr;
+      q[6] = tr;
+    } else {
+      for (int j = 0; j < 7; ++j) q[j] = 0.0;
+    }
+    const double ox = g.x - M.x, oy = g.y - M.y, oz = g.z - M.z;
+    q[7] = sqrt(ox * ox + oy * oy + oz * oz);
+    return;
+  }
+  const uint32_t fc = first_child[node];
+  double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+  for (uint32_t j = 0; j < c; ++j) {
+    const double4 cm = mom[fc + j];
+    tm += cm.w;
+    wx += cm.w * cm.x;
+    wy += cm.w * cm.y;
+    wz += cm.w * cm.z;
+  }
+  double4 M = make_double4(g.x, g.y, g.z, tm);
+  if (tm > 0.0) {
+    M.x = wx / tm;
+    M.y = wy / tm;
+    M.z = wz / tm;
+  }
+  mom[node] = M;
+  if (a.quad && tm > 0.0) {
+    double qxx = 0, qxy = 0, qxz = 0, qyy = 0, qyz = 0, qzz = 0, tr = 0;
+    for (uint32_t j = 0; j < c; ++j) {
+      const double4 cm = mom[fc + j];
+      const double* cq = quad + 8ull * (fc + j);
+      const double dx = cm.x - M.x, dy = cm.y - M.y, dz = cm.z - M.z;
+      const double mm = cm.w;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      qxx += cq[0] + mm * (3.0 * dx * dx - d2);
+      qxy += cq[1] + mm * (3.0 * dx * dy);
+      qxz += cq[2] + mm * (3.0 * dx * dz);
+      qyy += cq[3] + mm * (3.0 * dy * dy - d2);
+      qyz += cq[4] + mm * (3.0 * dy * dz);
+      qzz += cq[5] + mm * (3.0 * dz * dz - d2);
+      tr += cq[6] + mm * d2;
+    }
+    q[0] = qxx; q[1] = qxy; q[2] = qxz; q[3] = qyy; q[4] = qyz; q[5] = qzz; q[6] = tr;
+  } else {
+    for (int j = 0; j < 7; ++j) q[j] = 0.0;
+  }
+  const double ox = g.x - M.x, oy = g.y - M.y, oz = g.z - M.z;
+  q[7] = sqrt(ox * ox + oy * oy + oz * oz);
+}
+
+__device__ __forceinline__ uint32_t to_fixed(double c, double scale) {
+  // scale = 2^32 / L; wraps modulo 2^32 (periodic)
+  const double f = floor(c * scale);
+  return static_cast<uint32_t>(static_cast<uint64_t>(static_cast<int64_t>(f)));
+}
+
+struct RecordArgs {
+  uint32_t total;
+  int mixed;
+  double fscale[3];
+};
+
+// BFS -> preorder records (FP64 always, fixed-point/FP32 in mixed mode).
+__global__ __launch_bounds__(256) void records_kernel(RecordArgs a, const uint32_t* __restrict__ pre,
+                                                      const uint32_t* __restrict__ nb, const uint32_t* __restrict__ nc,
+                                                      const uint32_t* __restrict__ nchild,
+                                                      const uint32_t* __restrict__ first_child,
+                                                      const uint32_t* __restrict__ meta,
+                                                      const uint32_t* __restrict__ subtree,
+                                                      const double4* __restrict__ geo,
+                                                      const double4* __restrict__ mom,
+                                                      const double* __restrict__ quad, NodeLink* __restrict__ link,
+                                                      NodeGeo64* __restrict__ geo64, NodeMom64* __restrict__ mom64,
+                                                      NodeQuad64* __restrict__ quad64,
+                                                      uint32_t* __restrict__ child_index, NodeGeo32* __restrict__ geo32,
+                                                      NodeMom32* __restrict__ mom32,
+                                                      NodeQuad32* __restrict__ quad32) {
+  const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= a.total) return;
+  const uint32_t p = pre[node];
+  const uint32_t c = nchild[node];
+  link[p] = NodeLink{nb[node], nc[node], subtree[node], c};
+  const double4 g = geo[node];
+  const double4 M = mom[node];
+  const double* q = quad + 8ull * node;
+  geo64[p] = NodeGeo64{g.x, g.y, g.z, g.w};
+  mom64[p] = NodeMom64{M.x, M.y, M.z, M.w};
+  quad64[p] = NodeQuad64{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]};
+  uint32_t ci[8];
+#pragma unroll
+  for (int o = 0; o < 8; ++o) ci[o] = kInvalidIndex;
+  for (uint32_t j = 0; j < c; ++j) {
+    const uint32_t ch = first_child[node] + j;
+    ci[(meta[ch] >> 8) & 7u] = pre[ch];
+  }
+#pragma unroll
+  for (int o = 0; o < 8; ++o) child_index[8ull * p + o] = ci[o];
+  if (a.mixed) {
+    geo32[p] = NodeGeo32{to_fixed(g.x, a.fscale[0]), to_fixed(g.y, a.fscale[1]), to_fixed(g.z, a.fscale[2]),
+                         static_cast<float>(g.w)};
+    mom32[p] = NodeMom32{to_fixed(M.x, a.fscale[0]), to_fixed(M.y, a.fscale[1]), to_fixed(M.z, a.fscale[2]),
+                         static_cast<float>(M.w)};
+    quad32[p] = NodeQuad32{static_cast<float>(q[0]), static_cast<float>(q[1]), static_cast<float>(q[2]),
+                           static_cast<float>(q[3]), static_cast<float>(q[4]), static_cast<float>(q[5]),
+                           static_cast<float>(q[6]), static_cast<float>(q[7])};
+  }
+}
+
+__global__ void unwrap_apply_kernel(const double* in, std::size_t n, double L, int canonical, double anchor,
+                                    double* out) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = unwrap_coord(in[i], L, canonical, anchor);
+}
+
+__global__ void pos_of_row_kernel(const uint32_t* __restrict__ order, std::size_t n, uint32_t* __restrict__ pos) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) pos[order[i]] = static_cast<uint32_t>(i);
+}
+
+template <class T>
+void grow_copy(DeviceBuffer& b, std::size_t old_count, std::size_t new_count, hipStream_t s) {
+  if (new_count * sizeof(T) <= b.bytes) return;
+  DeviceBuffer nb;
+  nb.reserve(new_count * sizeof(T));
+  if (old_count) CHUI_HIP(hipMemcpyAsync(nb.ptr, b.ptr, old_count * sizeof(T), hipMemcpyDeviceToDevice, s));
+  std::swap(b.ptr, nb.ptr);
+  std::swap(b.bytes, nb.bytes);
+  CHUI_HIP(hipStreamSynchronize(s));  // old buffer freed by nb's destructor after the copy
+}
+
+// Source statistics -> host.
+SourceStats compute_stats(Tree& t, Axes3 p, const double* m, std::size_t n, hipStream_t s) {
+  t.bounds.reserve(sizeof(SourceStats));
+  SourceStats* st = t.bounds.as<SourceStats>();
+  stats_init_kernel<<<1, 1, 0, s>>>(st);
+  CHUI_LAUNCH_CHECK("stats_init_kernel");
+  if (n > 0) {
+    source_stats_kernel<<<std::min<unsigned>(grid_for(n, 256), 2048u), 256, 0, s>>>(p, m, n, st);
+    CHUI_LAUNCH_CHECK("source_stats_kernel");
+  }
+  SourceStats h{};
+  CHUI_HIP(hipMemcpyAsync(&h, st, sizeof(SourceStats), hipMemcpyDeviceToHost, s));
+  CHUI_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+double host_order_key_inv(uint64_t k) {
+  const uint64_t b = (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  double d;
+  std::memcpy(&d, &b, 8);
+  return d;
+}
+
+// Unwraps naxes axes at once; fills res (host).
+void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, const int* canonical,
+                 UnwrapResult& res, hipStream_t s) {
+  BucketArgs a{};
+  const uint64_t B = std::max<uint64_t>(2ull * n, 2ull);
+  a.B = B;
+  a.naxes = naxes;
+  for (int ax = 0; ax < naxes; ++ax) {
+    a.L[ax] = L[ax];
+    a.scale[ax] = static_cast<double>(B) / L[ax];
+    a.canonical[ax] = canonical[ax];
+  }
+  t.bucket_min.reserve(naxes * B * sizeof(uint64_t));
+  t.bucket_max.reserve(naxes * B * sizeof(uint64_t));
+  const uint64_t nblk = (B + kGapTile - 1) / kGapTile;
+  t.block_max.reserve((naxes * nblk + 8) * sizeof(uint64_t));
+  t.block_best.reserve((naxes * nblk + 1) * sizeof(GapBest) + sizeof(UnwrapResult));
+  uint64_t* bmin = t.bucket_min.as<uint64_t>();
+  uint64_t* bmax = t.bucket_max.as<uint64_t>();
+  uint64_t* blockmax = t.block_max.as<uint64_t>();
+  uint64_t* total = blockmax + naxes * nblk;
+  GapBest* best = t.block_best.as<GapBest>();
+  UnwrapResult* dres = reinterpret_cast<UnwrapResult*>(best + naxes * nblk + 1);
+  fill_u64(bmin, kEmpty, naxes * B, s);
+  fill_u64(bmax, 0, naxes * B, s);
+  bucket_kernel<<<std::min<unsigned>(grid_for(n, 256), 8192u), 256, 0, s>>>(p, n, a, bmin, bmax);
+  CHUI_LAUNCH_CHECK("bucket_kernel");
+  bucket_blockmax_kernel<<<dim3(static_cast<unsigned>(nblk), naxes), kGapThreads, 0, s>>>(bmax, B, blockmax, nblk);
+  CHUI_LAUNCH_CHECK("bucket_blockmax_kernel");
+  blockmax_scan_kernel<<<naxes, kGapThreads, 0, s>>>(blockmax, nblk, total);
+  CHUI_LAUNCH_CHECK("blockmax_scan_kernel");
+  gap_kernel<<<dim3(static_cast<unsigned>(nblk), naxes), kGapThreads, 0, s>>>(bmin, bmax, B, blockmax, total, nblk,
+                                                                              a, best);
+  CHUI_LAUNCH_CHECK("gap_kernel");
+  gap_final_kernel<<<naxes, kGapThreads, 0, s>>>(best, nblk, dres);
+  CHUI_LAUNCH_CHECK("gap_final_kernel");
+  UnwrapResult h{};
+  CHUI_HIP(hipMemcpyAsync(&h, dres, sizeof(UnwrapResult), hipMemcpyDeviceToHost, s));
+  CHUI_HIP(hipStreamSynchronize(s));
+  for (int ax = 0; ax < naxes; ++ax) {
+    res.anchor[ax] = h.anchor[ax];
+    res.umin[ax] = h.umin[ax];
+    res.umax[ax] = h.umax[ax];
+    res.canonical[ax] = canonical[ax];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void build_tree(Tree& t, const double* x, const double* y, const double* z, const double* m, const TreeParams& P,
+                double softening, hipStream_t s) {
+  (void)softening;
+  const std::size_t n = P.n;
+  if (n > 0xFFFFFFFEull) throw std::overflow_error("Tree gravity source count exceeds the 32-bit tree-index contract");
+  t.n = n;
+  t.node_count = 0;
+  t.leaf_count = 0;
+  t.max_depth = 0;
+  t.level_start.clear();
+  Axes3 p{{x, y, z}};
+  const SourceStats st = compute_stats(t, p, m, n, s);
+  if (st.bad_coord) throw std::invalid_argument("Periodic TreePM tree geometry requires finite source coordinates");
+  if (st.bad_mass) throw std::invalid_argument("Tree gravity requires finite coordinates and finite non-negative masses");
+  if (n == 0) return;
+  t.mass_uniform = st.mass_nonuniform == 0;
+  t.uniform_mass = 0.0;
+  CHUI_HIP(hipMemcpyAsync(&t.uniform_mass, m, sizeof(double), hipMemcpyDeviceToHost, s));
+  CHUI_HIP(hipStreamSynchronize(s));
+  int canonical[3];
+  for (int a = 0; a < 3; ++a) {
+    const double lo = host_order_key_inv(st.lo[a]);
+    const double hi = host_order_key_inv(st.hi[a]);
+    canonical[a] = (lo >= 0.0 && hi < P.L[a]) ? 1 : 0;  // tree_pm_coupling.cpp:3185-3195
+  }
+  unwrap_axes(t, p, 3, n, P.L, canonical, t.unwrap, s);
+  const UnwrapResult& U = t.unwrap;
+
+  // Root cube (tree_gravity.cpp:443-452) from the unwrapped bounds.
+  double ext = 0.0;
+  for (int a = 0; a < 3; ++a) ext = std::max(ext, U.umax[a] - U.umin[a]);
+  const double max_extent = std::max(ext, 1.0e-10);
+  for (int a = 0; a < 3; ++a) t.root_centre[a] = 0.5 * (U.umin[a] + U.umax[a]);
+  t.root_half = 0.5 * max_extent * (1.0 + 1.0e-8);
+
+  UnwrapArgs ua{};
+  for (int a = 0; a < 3; ++a) {
+    ua.L[a] = P.L[a];
+    ua.canonical[a] = U.canonical[a];
+    ua.anchor[a] = U.anchor[a];
+  }
+  // Path keys + stable sort.
+  t.keys.reserve(n * sizeof(uint64_t));
+  t.keys_alt.reserve(n * sizeof(uint64_t));
+  t.order.reserve(n * sizeof(uint32_t));
+  t.vals_alt.reserve(n * sizeof(uint32_t));
+  t.sort_scratch.reserve(radix_sort_scratch_bytes(n));
+  PathArgs pa{};
+  pa.u = ua;
+  for (int a = 0; a < 3; ++a) pa.c0[a] = t.root_centre[a];
+  pa.h0 = t.root_half;
+  uint64_t* keys = t.keys.as<uint64_t>();
+  uint32_t* order = t.order.as<uint32_t>();
+  path_key_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, pa, keys, order);
+  CHUI_LAUNCH_CHECK("path_key_kernel");
+  radix_sort_pairs(keys, order, t.keys_alt.as<uint64_t>(), t.vals_alt.as<uint32_t>(), n, 0, 3 * kPathLevels,
+                   t.sort_scratch.ptr, s);
+
+  // Level-synchronous discovery.
+  const std::size_t initial_cap = std::max<std::size_t>(1024, n / 4 + 64);
+  auto ensure = [&](std::size_t need, std::size_t used) {
+    if (need <= t.bfs_capacity && t.bfs_begin.ptr) return;
+    std::size_t nc = std::max({need, 2 * t.bfs_capacity, initial_cap});
+    grow_copy<uint32_t>(t.bfs_begin, used, nc, s);
+    grow_copy<uint32_t>(t.bfs_count, used, nc, s);
+    grow_copy<uint32_t>(t.bfs_first_child, used, nc, s);
+    grow_copy<uint32_t>(t.bfs_nchild, used, nc, s);
+    grow_copy<uint32_t>(t.bfs_meta, used, nc, s);
+    grow_copy<double4>(t.bfs_geo, used, nc, s);
+    t.bfs_capacity = nc;
+  };
+  ensure(initial_cap, 0);
+  t.flags.reserve(16);
+  fill_u32(t.flags.as<uint32_t>(), 0u, 4, s);
+  root_init_kernel<<<1, 1, 0, s>>>(t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
+                                   t.bfs_geo.as<double4>(), static_cast<uint32_t>(n), t.root_centre[0],
+                                   t.root_centre[1], t.root_centre[2], t.root_half);
+  CHUI_LAUNCH_CHECK("root_init_kernel");
+  t.level_start = {0, 1};
+  for (int d = 0;; ++d) {
+    const std::size_t lb = t.level_start[d], le = t.level_start[d + 1];
+    const uint32_t cnt = static_cast<uint32_t>(le - lb);
+    if (cnt == 0) break;
+    t.level_scratch.reserve((cnt + 4) * sizeof(uint32_t));
+    t.bfs_scan.reserve(scan_scratch_bytes(cnt) + 64);
+    LevelArgs la{static_cast<uint32_t>(lb), cnt, static_cast<uint32_t>(le), P.leaf_size};
+    level_count_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(la, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(),
+                                                          t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(), keys,
+                                                          t.bfs_nchild.as<uint32_t>(), t.flags.as<uint32_t>());
+    CHUI_LAUNCH_CHECK("level_count_kernel");
+    uint32_t* cbase = t.level_scratch.as<uint32_t>();
+    exclusive_scan_u32(t.bfs_nchild.as<uint32_t>() + lb, cbase, cnt, t.flags.as<uint32_t>() + 1, t.bfs_scan.ptr, s);
+    uint32_t hflags[2] = {0, 0};
+    CHUI_HIP(hipMemcpyAsync(hflags, t.flags.ptr, sizeof(hflags), hipMemcpyDeviceToHost, s));
+    CHUI_HIP(hipStreamSynchronize(s));
+    if (hflags[0]) {
+      throw UnsupportedError(
+          "octree deeper than 21 levels (more than max_leaf_size coincident or near-coincident particles) "
+          "is not built by the GPU path yet");
+    }
+    const uint32_t total_children = hflags[1];
+    if (total_children == 0) break;
+    ensure(le + total_children, le);
+    level_emit_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(la, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(),
+                                                         t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(), keys,
+                                                         t.bfs_nchild.as<uint32_t>(), cbase,
+                                                         t.bfs_first_child.as<uint32_t>());
+    CHUI_LAUNCH_CHECK("level_emit_kernel");
+    t.level_start.push_back(le + total_children);
+    t.max_depth = static_cast<uint32_t>(d + 1);
+  }
+  const std::size_t total = t.level_start.back();
+  const int nlevels = static_cast<int>(t.level_start.size()) - 1;
+  t.node_count = total;
+  t.bfs_subtree.reserve(total * sizeof(uint32_t));
+  t.bfs_pre.reserve(total * sizeof(uint32_t));
+  t.bfs_mom.reserve(total * sizeof(double4));
+  t.bfs_quad.reserve(total * 8 * sizeof(double));
+  uint32_t* nchild = t.bfs_nchild.as<uint32_t>();
+  uint32_t* fchild = t.bfs_first_child.as<uint32_t>();
+  for (int d = nlevels - 1; d >= 0; --d) {
+    const uint32_t lb = static_cast<uint32_t>(t.level_start[d]);
+    const uint32_t cnt = static_cast<uint32_t>(t.level_start[d + 1] - lb);
+    subtree_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(lb, cnt, nchild, fchild, t.bfs_subtree.as<uint32_t>());
+    CHUI_LAUNCH_CHECK("subtree_kernel");
+  }
+  fill_u32(t.bfs_pre.as<uint32_t>(), 0u, 1, s);
+  for (int d = 0; d < nlevels; ++d) {
+    const uint32_t lb = static_cast<uint32_t>(t.level_start[d]);
+    const uint32_t cnt = static_cast<uint32_t>(t.level_start[d + 1] - lb);
+    preorder_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(lb, cnt, nchild, fchild, t.bfs_subtree.as<uint32_t>(),
+                                                       t.bfs_pre.as<uint32_t>());
+    CHUI_LAUNCH_CHECK("preorder_kernel");
+  }
+  // In-leaf Morton order (tree_ordering.cpp:151 extent with the 1e-12 floor).
+  MortonArgs ma{};
+  double mext = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    ma.lo[a] = U.umin[a];
+    mext = std::max(mext, U.umax[a] - U.umin[a]);
+  }
+  ma.extent = std::max(mext, 1.0e-12);
+  uint64_t* mk = t.keys_alt.as<uint64_t>();
+  leaf_morton_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, ua, ma, order, mk);
+  CHUI_LAUNCH_CHECK("leaf_morton_kernel");
+  leaf_sort_kernel<<<grid_for(total, 256), 256, 0, s>>>(static_cast<uint32_t>(total), nchild,
+                                                        t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), mk,
+                                                        order);
+  CHUI_LAUNCH_CHECK("leaf_sort_kernel");
+  // Multipoles bottom-up.
+  for (int d = nlevels - 1; d >= 0; --d) {
+    MomArgs ma2{};
+    ma2.u = ua;
+    ma2.lb = static_cast<uint32_t>(t.level_start[d]);
+    ma2.cnt = static_cast<uint32_t>(t.level_start[d + 1] - t.level_start[d]);
+    ma2.quad = P.quadrupole ? 1 : 0;
+    multipole_kernel<<<grid_for(ma2.cnt, 128), 128, 0, s>>>(
+        ma2, p, m, order, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), nchild, fchild,
+        t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>());
+    CHUI_LAUNCH_CHECK("multipole_kernel");
+  }
+  // Preorder records.
+  t.link.reserve(total * sizeof(NodeLink));
+  t.geo64.reserve(total * sizeof(NodeGeo64));
+  t.mom64.reserve(total * sizeof(NodeMom64));
+  t.quad64.reserve(total * sizeof(NodeQuad64));
+  t.child_index.reserve(total * 8 * sizeof(uint32_t));
+  if (P.mixed) {
+    t.geo32.reserve(total * sizeof(NodeGeo32));
+    t.mom32.reserve(total * sizeof(NodeMom32));
+    t.quad32.reserve(total * sizeof(NodeQuad32));
+  }
+  RecordArgs ra{};
+  ra.total = static_cast<uint32_t>(total);
+  ra.mixed = P.mixed ? 1 : 0;
+  for (int a = 0; a < 3; ++a) ra.fscale[a] = 4294967296.0 / P.L[a];
+  records_kernel<<<grid_for(total, 256), 256, 0, s>>>(
+      ra, t.bfs_pre.as<uint32_t>(), t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), nchild, fchild,
+      t.bfs_meta.as<uint32_t>(), t.bfs_subtree.as<uint32_t>(), t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(),
+      t.bfs_quad.as<double>(), t.link.as<NodeLink>(), t.geo64.as<NodeGeo64>(), t.mom64.as<NodeMom64>(),
+      t.quad64.as<NodeQuad64>(), t.child_index.as<uint32_t>(), P.mixed ? t.geo32.as<NodeGeo32>() : nullptr,
+      P.mixed ? t.mom32.as<NodeMom32>() : nullptr, P.mixed ? t.quad32.as<NodeQuad32>() : nullptr);
+  CHUI_LAUNCH_CHECK("records_kernel");
+  uint32_t hleaf = 0;
+  CHUI_HIP(hipMemcpyAsync(&hleaf, t.flags.as<uint32_t>() + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CHUI_HIP(hipStreamSynchronize(s));
+  t.leaf_count = hleaf;
+}
+
+void build_pos_of_row(Tree& t, hipStream_t s) {
+  t.pos_of_row.reserve(std::max<std::size_t>(t.n, 1) * sizeof(uint32_t));
+  if (t.n == 0) return;
+  pos_of_row_kernel<<<grid_for(t.n, 256), 256, 0, s>>>(t.order.as<uint32_t>(), t.n, t.pos_of_row.as<uint32_t>());
+  CHUI_LAUNCH_CHECK("pos_of_row_kernel");
+}
+
+void morton_order_device(Tree& t, const double* x, const double* y, const double* z, std::size_t n,
+                         uint64_t* keys_sorted, uint32_t* order, hipStream_t s) {
+  Axes3 p{{x, y, z}};
+  const SourceStats st = compute_stats(t, p, nullptr, n, s);
+  if (st.bad_coord) throw std::invalid_argument("Tree bounds requires finite particle coordinates");
+  if (n == 0) return;
+  MortonArgs ma{};
+  double ext = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    ma.lo[a] = host_order_key_inv(st.lo[a]);
+    ext = std::max(ext, host_order_key_inv(st.hi[a]) - ma.lo[a]);
+  }
+  ma.extent = std::max(ext, 1.0e-12);
+  t.keys_alt.reserve(n * sizeof(uint64_t));
+  t.vals_alt.reserve(n * sizeof(uint32_t));
+  t.sort_scratch.reserve(radix_sort_scratch_bytes(n));
+  morton_keys_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, ma, keys_sorted, order);
+  CHUI_LAUNCH_CHECK("morton_keys_kernel");
+  radix_sort_pairs(keys_sorted, order, t.keys_alt.as<uint64_t>(), t.vals_alt.as<uint32_t>(), n, 0, 63,
+                   t.sort_scratch.ptr, s);
+}
+
+double unwrap_axis_device(Tree& t, const double* in, std::size_t n, double L, double* out, hipStream_t s) {
+  if (!(std::isfinite(L) && L > 0.0))
+    throw std::invalid_argument("Periodic TreePM tree geometry requires finite positive axis lengths");
+  Axes3 p{{in, in, in}};
+  const SourceStats st = compute_stats(t, p, nullptr, n, s);
+  if (st.bad_coord) throw std::invalid_argument("Periodic TreePM tree geometry requires finite source coordinates");
+  if (n == 0) return 0.0;
+  const double lo = host_order_key_inv(st.lo[0]), hi = host_order_key_inv(st.hi[0]);
+  int canonical[3] = {(lo >= 0.0 && hi < L) ? 1 : 0, 0, 0};
+  double Ls[3] = {L, L, L};
+  UnwrapResult r{};
+  unwrap_axes(t, p, 1, n, Ls, canonical, r, s);
+  unwrap_apply_kernel<<<grid_for(n, 256), 256, 0, s>>>(in, n, L, canonical[0], r.anchor[0], out);
+  CHUI_LAUNCH_CHECK("unwrap_apply_kernel");
+  return r.anchor[0];
+}
+
+}  // namespace chui

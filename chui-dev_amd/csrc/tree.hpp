@@ -1,0 +1,87 @@
+// tree.hpp — periodic unwrap, octree construction and node records.
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace chui {
+
+// Preorder node records. Node i's subtree is the index range [i, i + link.skip),
+// so a stackless traversal moves to i + 1 to open a node and to i + skip to
+// leave it; visiting nodes in increasing index reproduces the reference's DFS
+// order (children pushed in reverse octant order, tree_pm_coupling.cpp:916).
+struct alignas(16) NodeLink {
+  uint32_t begin;        // first tree-order particle
+  uint32_t count;        // particle count
+  uint32_t skip;         // subtree size in nodes (1 for leaves)
+  uint32_t child_count;  // 0 = leaf
+};
+struct alignas(32) NodeGeo64 { double cx, cy, cz, half; };
+struct alignas(32) NodeMom64 { double comx, comy, comz, mass; };
+// Traceless quadrupole, raw second-moment trace and |centre - COM|.
+struct alignas(64) NodeQuad64 { double qxx, qxy, qxz, qyy, qyz, qzz, trace, offset; };
+
+// Mixed-precision records: 32-bit fixed-point periodic coordinates
+// (x -> x * 2^32 / L, wrapping modulo 2^32) so that minimum-image deltas are an
+// exact int32 subtraction.
+struct alignas(16) NodeGeo32 { uint32_t cx, cy, cz; float half; };
+struct alignas(16) NodeMom32 { uint32_t comx, comy, comz; float mass; };
+struct alignas(32) NodeQuad32 { float qxx, qxy, qxz, qyy, qyz, qzz, trace, offset; };
+
+struct UnwrapResult {
+  double anchor[3];
+  double umin[3], umax[3];
+  int canonical[3];
+};
+
+struct TreeParams {
+  std::size_t n = 0;
+  double L[3] = {0, 0, 0};
+  uint32_t leaf_size = 16;
+  bool quadrupole = true;
+  bool mixed = false;  // also emit the fixed-point records
+};
+
+// Device state of one built tree. Buffers are grow-only and owned by Tree.
+struct Tree {
+  std::size_t n = 0;
+  std::size_t node_count = 0;
+  std::size_t leaf_count = 0;
+  uint32_t max_depth = 0;
+  UnwrapResult unwrap{};
+  bool mass_uniform = false;
+  double uniform_mass = 0.0;
+  double root_centre[3] = {0, 0, 0};
+  double root_half = 0.0;
+  // particle level
+  DeviceBuffer order;      // uint32[n]: tree position -> source row
+  DeviceBuffer pos_of_row; // uint32[n]: source row -> tree position (built on demand)
+  // preorder node level
+  DeviceBuffer link, geo64, mom64, quad64, child_index;  // child_index: uint32[8*nodes]
+  DeviceBuffer geo32, mom32, quad32;
+  // construction scratch
+  DeviceBuffer keys, keys_alt, vals_alt, sort_scratch;
+  DeviceBuffer bucket_min, bucket_max, block_best, block_max;
+  DeviceBuffer bfs_begin, bfs_count, bfs_first_child, bfs_nchild, bfs_meta, bfs_geo, bfs_subtree, bfs_pre,
+      bfs_mom, bfs_quad, bfs_scan, level_scratch, flags, bounds;
+  std::size_t bfs_capacity = 0;
+  std::vector<std::size_t> level_start;  // BFS level ranges (host copy)
+};
+
+// Bounds/canonical check, periodic unwrap on all three axes, Morton/path keys,
+// stable sort, level-synchronous octree, preorder numbering, in-leaf Morton
+// order and multipoles. Inputs are device pointers (caller positions/masses).
+void build_tree(Tree& t, const double* x, const double* y, const double* z, const double* m,
+                const TreeParams& p, double softening, hipStream_t s);
+
+// Morton keys + stable order of given (unwrapped) device coordinates.
+void morton_order_device(Tree& t, const double* x, const double* y, const double* z, std::size_t n,
+                         uint64_t* keys_sorted, uint32_t* order, hipStream_t s);
+// Single-axis periodic unwrap (parity entry point).
+double unwrap_axis_device(Tree& t, const double* in, std::size_t n, double L, double* out, hipStream_t s);
+
+// row -> tree position map for active-subset targets.
+void build_pos_of_row(Tree& t, hipStream_t s);
+
+}  // namespace chui

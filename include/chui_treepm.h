@@ -1,0 +1,278 @@
+/*
+ * chui_treepm.h — C ABI of the MI355X TreePM gravity library (libchui_treepm.so).
+ *
+ * Drop-in boundary for CHUI's periodic TreePM force evaluation. Every entry
+ * point takes plain pointers and sizes, returns a chui_status and never lets a
+ * C++ exception cross the ABI. The message of the last failure on the calling
+ * thread is available from chui_treepm_last_error().
+ *
+ * Interfaces replaced (reference file:line):
+ *   chui_treepm_create / chui_treepm_destroy
+ *       gravity::TreePmCoordinator::TreePmCoordinator(PmGridShape)
+ *       include/cosmosim/gravity/tree_pm_coupling.hpp:545-551
+ *   chui_treepm_solve / chui_treepm_solve_device
+ *       TreePmCoordinator::solveActiveSetWithPmCadence(...)
+ *       include/cosmosim/gravity/tree_pm_coupling.hpp:563-573
+ *       (solveActiveSet, :575-584, is the same call with refresh = 1)
+ *   chui_treepm_options_*  TreePmOptions / PmSolveOptions / TreeGravityOptions /
+ *       TreePmSplitPolicy (tree_pm_coupling.hpp:243-332, pm_solver.hpp:223-254,
+ *       tree_gravity.hpp:30-43, tree_pm_split_kernel.hpp:20-27) and the
+ *       [numerics] .param.txt keys mapped in src/workflows/gravity_runtime.cpp:486-590
+ *   chui_treepm_kick_device / chui_treepm_drift_device
+ *       velocity kick src/workflows/gravity_runtime.cpp:3045-3058 and drift +
+ *       canonical wrap src/workflows/reference_runtime_composition.cpp:59-108
+ *   chui_treepm_kdk_factors
+ *       CosmologicalTimeline::prepareStep src/core/time_integration.cpp:407-464
+ *   chui_treepm_split_diagnostics
+ *       computeTreePmDiagnostics src/gravity/tree_pm_coupling.cpp:6276-6312
+ *   chui_treepm_tree_* (read-back of the last built tree)
+ *       TreeGravitySolver::nodes()/ordering() include/cosmosim/gravity/tree_gravity.hpp:93-129,186-190
+ *   chui_treepm_morton_device
+ *       buildMortonOrderingInPlace src/gravity/tree_ordering.cpp:128-173
+ *   chui_treepm_unwrap_device
+ *       unwrapPeriodicAxis src/gravity/tree_pm_coupling.cpp:141-218
+ *   chui_treepm_ewald_device (FP64 periodic Ewald sum on sample targets)
+ *       tests/support/periodic_ewald_reference.cpp:202-400 (test-only in the reference)
+ */
+#ifndef CHUI_TREEPM_H
+#define CHUI_TREEPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHUI_TREEPM_ABI_VERSION 1
+
+/* Status codes. The C++ adapter maps them back onto the reference's exception
+ * classes (tree_pm_coupling.cpp / tree_index.hpp:21-28). */
+typedef enum chui_status {
+  CHUI_OK = 0,
+  CHUI_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+  CHUI_ERR_RUNTIME = 2,          /* std::runtime_error */
+  CHUI_ERR_OVERFLOW = 3,         /* std::overflow_error (32-bit index / depth limits) */
+  CHUI_ERR_LOGIC = 4,            /* std::logic_error (internal invariant) */
+  CHUI_ERR_DEVICE = 5,           /* HIP runtime failure */
+  CHUI_ERR_OUT_OF_MEMORY = 6,    /* device allocation failed */
+  CHUI_ERR_UNSUPPORTED = 7       /* option accepted by the reference, not built here yet */
+} chui_status;
+
+typedef enum chui_precision {
+  CHUI_PRECISION_FP64 = 0,  /* FP64 everywhere (walk, FFT, deposit, interpolation) */
+  CHUI_PRECISION_MIXED = 1  /* 32-bit fixed-point positions, FP32 walk/FFT/interp with
+                               compensated accumulation; FP64 drift/kick/reductions */
+} chui_precision;
+
+typedef enum chui_assignment {
+  CHUI_ASSIGN_CIC = 0,
+  CHUI_ASSIGN_TSC = 1
+} chui_assignment;
+
+typedef enum chui_opening_criterion {
+  CHUI_OPEN_GEOMETRIC = 0,     /* kBarnesHutGeometric */
+  CHUI_OPEN_COM_DISTANCE = 1,  /* kBarnesHutComDistance (reference default) */
+  CHUI_OPEN_RELATIVE = 2       /* kRelativeForceError */
+} chui_opening_criterion;
+
+typedef enum chui_acceptance_policy {
+  CHUI_ACCEPT_STRICT = 0,   /* kStrictReference (reference default) */
+  CHUI_ACCEPT_ADAPTIVE = 1  /* kAdaptiveRelative */
+} chui_acceptance_policy;
+
+/* Flat options: the fields of TreePmOptions that change a force evaluation.
+ * Defaults (chui_treepm_default_options) equal the reference's struct defaults. */
+typedef struct chui_treepm_options {
+  uint32_t struct_size;              /* = sizeof(chui_treepm_options) */
+  /* PmSolveOptions */
+  double box_size[3];                /* comoving axis lengths (Lx, Ly, Lz) */
+  double scale_factor;
+  double gravitational_constant;     /* G in code units */
+  int32_t assignment_scheme;         /* chui_assignment */
+  int32_t window_deconvolution;      /* 0/1 */
+  /* TreePmSplitPolicy (from mesh spacing cbrt(hx*hy*hz)) */
+  double asmth_cells;
+  double rcut_cells;
+  /* TreeGravityOptions */
+  double opening_theta;
+  int32_t opening_criterion;         /* chui_opening_criterion */
+  int32_t multipole_order;           /* 0 monopole, 2 quadrupole */
+  int32_t max_leaf_size;
+  int32_t reserved0;
+  double softening_epsilon;          /* Plummer epsilon, comoving */
+  double relative_force_tolerance;
+  double relative_force_acceleration_floor;
+  /* TreePmOptions */
+  int32_t acceptance_policy;         /* chui_acceptance_policy */
+  int32_t residual_block_size;       /* accepted for compatibility (16..256); GPU walks 64-wide */
+  /* this library */
+  int32_t precision;                 /* chui_precision */
+  int32_t reserved1;
+} chui_treepm_options;
+
+/* Per-solve counters and timings. Counter names follow TreePmTraversalCounters
+ * (tree_pm_coupling.hpp:39-87) and the profile events (:505-524, pm_solver.hpp:256-302). */
+typedef struct chui_treepm_diagnostics {
+  uint32_t struct_size;
+  uint32_t reserved0;
+  uint64_t node_count;
+  uint64_t leaf_count;
+  uint64_t max_depth;
+  uint64_t direct_pair_evaluations;
+  uint64_t accepted_internal_multipoles;
+  uint64_t accepted_leaves;
+  uint64_t cutoff_pruned_nodes;
+  uint64_t visited_nodes;
+  double split_scale_comoving;
+  double cutoff_radius_comoving;
+  double mesh_spacing_comoving;
+  double tree_root_half_size_comoving;
+  double unwrap_anchor[3];
+  /* device timings in milliseconds (HIP events on the library stream) */
+  double ms_total;
+  double ms_unwrap_keys_sort;
+  double ms_tree_build;
+  double ms_multipoles;
+  double ms_pm_deposit;
+  double ms_pm_fft_poisson;
+  double ms_pm_interpolate;
+  double ms_walk;
+} chui_treepm_diagnostics;
+
+typedef struct chui_treepm_context chui_treepm_context;
+
+/* ---- options ----------------------------------------------------------- */
+chui_status chui_treepm_default_options(chui_treepm_options* options);
+/* Applies one [numerics]/[cosmology] .param.txt key (config.cpp:1723-1766 names,
+ * e.g. "treepm_asmth_cells", "treepm_tree_opening_theta", "box_size_mpc_comoving"). */
+chui_status chui_treepm_options_set(chui_treepm_options* options, const char* key, const char* value);
+/* Same validation rules as the reference: residual_block_size in [16,256], leaf in
+ * [1,1024] (config.cpp:3370-3372), eps/r_s <= 0.2 (:3669), r_cut < L_min/2 (:3682-3692). */
+chui_status chui_treepm_options_validate(const chui_treepm_options* options, const int32_t mesh[3]);
+
+/* ---- context ----------------------------------------------------------- */
+/* mesh = {nx, ny, nz}; device = HIP ordinal. Mesh axes must be powers of two. */
+chui_status chui_treepm_create(const int32_t mesh[3], int32_t device, chui_treepm_context** out);
+chui_status chui_treepm_destroy(chui_treepm_context* ctx);
+/* Stream the context enqueues on (hipStream_t; NULL = library-owned stream). */
+chui_status chui_treepm_set_stream(chui_treepm_context* ctx, void* stream);
+chui_status chui_treepm_synchronize(chui_treepm_context* ctx);
+
+/* ---- force evaluation ---------------------------------------------------- */
+/* Host-pointer drop-in for solveActiveSetWithPmCadence. Positions/masses are
+ * double[n]; active = uint32[n_active] source rows or NULL for all rows.
+ * accel_* = double[n_active] total (PM + short-range residual) accelerations.
+ * pm_accel_* (nullable) capture the PM-only part; short_accel_* (nullable) the
+ * tree-only residual, both per active slot. refresh_long_range = 0 reuses the
+ * previous PM mesh field (cadence) when the mesh/split/box are unchanged. */
+chui_status chui_treepm_solve(chui_treepm_context* ctx,
+                              const double* pos_x, const double* pos_y, const double* pos_z,
+                              const double* mass, uint64_t n,
+                              const uint32_t* active, uint64_t n_active,
+                              double* accel_x, double* accel_y, double* accel_z,
+                              double* pm_accel_x, double* pm_accel_y, double* pm_accel_z,
+                              double* short_accel_x, double* short_accel_y, double* short_accel_z,
+                              const chui_treepm_options* options, int32_t refresh_long_range,
+                              chui_treepm_diagnostics* diagnostics);
+
+/* Device-pointer form: every array is device memory already resident in HBM.
+ * Accelerations are double in FP64 precision and float in MIXED precision
+ * (accel_* then point to float[n_active]). Work is enqueued on the context
+ * stream; the call returns after enqueue unless diagnostics != NULL. */
+chui_status chui_treepm_solve_device(chui_treepm_context* ctx,
+                                     const double* pos_x, const double* pos_y, const double* pos_z,
+                                     const double* mass, uint64_t n,
+                                     const uint32_t* active, uint64_t n_active,
+                                     void* accel_x, void* accel_y, void* accel_z,
+                                     void* pm_accel_x, void* pm_accel_y, void* pm_accel_z,
+                                     void* short_accel_x, void* short_accel_y, void* short_accel_z,
+                                     const chui_treepm_options* options, int32_t refresh_long_range,
+                                     chui_treepm_diagnostics* diagnostics);
+
+/* Per-stage device timing accumulated over solves with HIP events on the
+ * context stream (no extra synchronisation inside a solve).
+ * chui_treepm_profiling(ctx, 1) resets and enables, (ctx, 0) disables.
+ * chui_treepm_profile_read synchronises and returns stage_ms[8] =
+ * {total, tree build (unwrap, keys, sort, nodes, multipoles), source gather,
+ *  PM (deposit + FFTs + Green + gradients), target set-up, walk (+ fused PM
+ *  interpolation), reserved, reserved} summed over *solves solves. */
+chui_status chui_treepm_profiling(chui_treepm_context* ctx, int32_t enable);
+chui_status chui_treepm_profile_read(chui_treepm_context* ctx, double stage_ms[8], uint64_t* solves);
+
+/* ---- KDK integrator pieces (FP64 on the device) -------------------------- */
+typedef struct chui_kdk_factors {
+  double scale_factor_begin, scale_factor_midpoint, scale_factor_end;
+  double drift_factor;            /* int_{a0}^{a1} da / (a^2 H(a)), code units */
+  double first_kick_factor;       /* int_{a0}^{am} da / (a^2 H(a)) / am */
+  double second_kick_factor;      /* int_{am}^{a1} da / (a^2 H(a)) / a1 */
+  double first_hubble_drag;       /* a0 / am */
+  double second_hubble_drag;      /* am / a1 */
+} chui_kdk_factors;
+
+/* LambdaCDM factors for one global KDK step from a0 to a1. hubble0_code = H0 in
+ * inverse code time, H(a) = H0 sqrt(omega_m a^-3 + omega_lambda) (flat,
+ * cosmology.hpp:77-87). am is the scale factor at the cosmic-time midpoint; the
+ * integrals use 64-sample midpoint quadrature (time_integration.cpp:242-266,3538-3556). */
+chui_status chui_treepm_kdk_factors(double scale_factor_begin, double scale_factor_end, double hubble0_code,
+                                    double omega_m, double omega_lambda, chui_kdk_factors* out);
+/* v = drag * v + kick * a  (active rows or all rows when active == NULL). accel is
+ * double (is_float_accel = 0) or float (= 1), indexed by active slot. */
+chui_status chui_treepm_kick_device(chui_treepm_context* ctx, double* vel_x, double* vel_y, double* vel_z,
+                                    const void* accel_x, const void* accel_y, const void* accel_z,
+                                    int32_t is_float_accel, const uint32_t* active, uint64_t n_active,
+                                    double kick_factor, double hubble_drag);
+/* x += drift * v, then canonical wrap into [0, L) per axis. */
+chui_status chui_treepm_drift_device(chui_treepm_context* ctx, double* pos_x, double* pos_y, double* pos_z,
+                                     const double* vel_x, const double* vel_y, const double* vel_z,
+                                     uint64_t n, double drift_factor, const double box_size[3]);
+/* FP64 global reductions on the device: out[0] = sum m, out[1..3] = sum m v. */
+chui_status chui_treepm_mass_momentum_device(chui_treepm_context* ctx, const double* mass,
+                                             const double* vel_x, const double* vel_y, const double* vel_z,
+                                             uint64_t n, double out_host[4]);
+
+/* ---- parity / inspection entry points ----------------------------------- */
+/* Periodic unwrap of one axis (device arrays): out = unwrapped coordinate, returns anchor. */
+chui_status chui_treepm_unwrap_device(chui_treepm_context* ctx, const double* in, uint64_t n,
+                                      double box_length, double* out, double* anchor_host);
+/* Morton keys + stable order of (already unwrapped) device coordinates:
+ * keys_sorted = uint64[n] keys in sorted order, order = uint32[n] source rows. */
+chui_status chui_treepm_morton_device(chui_treepm_context* ctx, const double* x, const double* y,
+                                      const double* z, uint64_t n, uint64_t* keys_sorted, uint32_t* order);
+/* Size / copy-out of the tree built by the last solve (preorder node numbering). */
+chui_status chui_treepm_tree_size(chui_treepm_context* ctx, uint64_t* node_count, uint64_t* particle_count);
+/* Host arrays of node_count entries (child_index: 8*node_count; order: particle_count).
+ * Any pointer may be NULL. geometry = double[node_count*4] (cx,cy,cz,half);
+ * moments = double[node_count*11] (mass, comx, comy, comz, qxx, qxy, qxz, qyy, qyz, qzz, trace). */
+chui_status chui_treepm_tree_copy(chui_treepm_context* ctx, double* geometry, double* moments,
+                                  uint32_t* particle_begin, uint32_t* particle_count,
+                                  uint32_t* child_index, uint32_t* order);
+
+/* FP64 periodic Ewald accelerations at n_targets target rows (device arrays),
+ * alpha in inverse length, image/mode limits inclusive. Used as the accuracy
+ * oracle of the force error figures. */
+chui_status chui_treepm_ewald_device(chui_treepm_context* ctx, const double* pos_x, const double* pos_y,
+                                     const double* pos_z, const double* mass, uint64_t n,
+                                     const uint32_t* targets, uint64_t n_targets, const double box_size[3],
+                                     double gravitational_constant, double alpha, int32_t real_images,
+                                     int32_t reciprocal_modes, double plummer_epsilon,
+                                     double* accel_x, double* accel_y, double* accel_z);
+
+/* Split-kernel diagnostics (S, L at r_s and r_cut; composition error). */
+typedef struct chui_split_diagnostics {
+  double split_scale_comoving, cutoff_radius_comoving;
+  double short_range_factor_at_split, long_range_factor_at_split;
+  double short_range_factor_at_cutoff, long_range_factor_at_cutoff;
+  double composition_error_at_split, max_relative_composition_error;
+} chui_split_diagnostics;
+chui_status chui_treepm_split_diagnostics(double asmth_cells, double rcut_cells, double mesh_spacing,
+                                          chui_split_diagnostics* out);
+
+/* ---- misc ----------------------------------------------------------------- */
+const char* chui_treepm_last_error(void);
+int32_t chui_treepm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHUI_TREEPM_H */

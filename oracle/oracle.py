@@ -1,0 +1,163 @@
+"""ctypes binding of oracle/liboracle.so — TEST INFRASTRUCTURE ONLY.
+
+The CPU FP64 restatement of the reference path (treepm_oracle.cpp). Imported by
+tests/, the smoke check and the CPU-baseline leg of bench.py; never by the
+product (chui-dev_amd/).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        _lib.oracle_last_error.restype = C.c_char_p
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().oracle_last_error().decode()
+        raise {1: ValueError, 3: OverflowError}.get(rc, RuntimeError)(msg)
+
+
+def set_threads(n: int) -> int:
+    return load().oracle_set_threads(C.c_int(n))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def unwrap_axis(coord, box_length, canonical=None):
+    coord = _f64(coord)
+    if canonical is None:
+        canonical = bool(coord.size == 0 or (coord.min() >= 0.0 and coord.max() < box_length))
+    out = np.empty_like(coord)
+    anchor = C.c_double()
+    _check(load().oracle_unwrap_axis(_p(coord), C.c_uint64(coord.size), C.c_double(box_length),
+                                     C.c_int(1 if canonical else 0), _p(out), C.byref(anchor)))
+    return out, anchor.value
+
+
+def morton_keys(x, y, z):
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    keys = np.empty(x.size, np.uint64)
+    _check(load().oracle_morton_keys(_p(x), _p(y), _p(z), C.c_uint64(x.size), _p(keys)))
+    return keys
+
+
+def morton_order(x, y, z):
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    keys = np.empty(x.size, np.uint64)
+    order = np.empty(x.size, np.uint32)
+    _check(load().oracle_morton_order(_p(x), _p(y), _p(z), C.c_uint64(x.size), _p(keys), _p(order)))
+    return keys, order
+
+
+def build_tree(x, y, z, m, leaf=16, quad=True, eps=0.0):
+    x, y, z, m = _f64(x), _f64(y), _f64(z), _f64(m)
+    nodes = C.c_uint64()
+    _check(load().oracle_build_tree(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), C.c_int(leaf),
+                                    C.c_int(1 if quad else 0), C.c_double(eps), C.byref(nodes)))
+    return last_tree(nodes.value, x.size)
+
+
+def last_tree(node_count, n):
+    geometry = np.zeros((node_count, 4))
+    moments = np.zeros((node_count, 11))
+    begin = np.zeros(node_count, np.uint32)
+    count = np.zeros(node_count, np.uint32)
+    child = np.zeros((node_count, 8), np.uint32)
+    order = np.zeros(n, np.uint32)
+    subtree = np.zeros(node_count, np.uint32)
+    _check(load().oracle_tree_copy(_p(geometry), _p(moments), _p(begin), _p(count), _p(child), _p(order),
+                                   _p(subtree)))
+    return dict(geometry=geometry, moments=moments, begin=begin, count=count, child_index=child, order=order,
+                subtree=subtree)
+
+
+def split_factors(r, a, eps):
+    r = _f64(r)
+    S, L, R = np.empty_like(r), np.empty_like(r), np.empty_like(r)
+    _check(load().oracle_split_factors(_p(r), C.c_uint64(r.size), C.c_double(a), _p(S), _p(L), _p(R),
+                                       C.c_double(eps)))
+    return S, L, R
+
+
+def _mesh(mesh):
+    if isinstance(mesh, int):
+        mesh = (mesh, mesh, mesh)
+    return (C.c_int32 * 3)(*mesh)
+
+
+def pm_accel(x, y, z, m, options, mesh, want_density=False):
+    x, y, z, m = _f64(x), _f64(y), _f64(z), _f64(m)
+    a = [np.empty(x.size) for _ in range(3)]
+    msh = _mesh(mesh)
+    dens = np.empty(msh[0] * msh[1] * msh[2]) if want_density else None
+    _check(load().oracle_pm_accel(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), C.byref(options), msh,
+                                  _p(a[0]), _p(a[1]), _p(a[2]), _p(dens)))
+    return (a, dens) if want_density else a
+
+
+def short_range(x, y, z, m, options, mesh):
+    x, y, z, m = _f64(x), _f64(y), _f64(z), _f64(m)
+    a = [np.empty(x.size) for _ in range(3)]
+    counters = np.zeros(6, np.uint64)
+    anchors = np.zeros(3)
+    _check(load().oracle_short_range(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), C.byref(options),
+                                     _mesh(mesh), _p(a[0]), _p(a[1]), _p(a[2]), _p(counters), _p(anchors)))
+    names = ("visited", "pruned", "pairs", "multipoles", "leaves", "skipped_pairs")
+    return a, dict(zip(names, counters.tolist())), anchors
+
+
+def solve(x, y, z, m, options, mesh):
+    x, y, z, m = _f64(x), _f64(y), _f64(z), _f64(m)
+    a = [np.empty(x.size) for _ in range(3)]
+    counters = np.zeros(6, np.uint64)
+    _check(load().oracle_solve(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), C.byref(options), _mesh(mesh),
+                               _p(a[0]), _p(a[1]), _p(a[2]), _p(counters)))
+    return a
+
+
+def ewald(x, y, z, m, targets, box, G, alpha, real_images, modes, eps=0.0):
+    x, y, z, m = _f64(x), _f64(y), _f64(z), _f64(m)
+    targets = np.ascontiguousarray(targets, dtype=np.uint32)
+    a = [np.empty(targets.size) for _ in range(3)]
+    L = (C.c_double * 3)(*box)
+    _check(load().oracle_ewald(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), _p(targets),
+                               C.c_uint64(targets.size), L, C.c_double(G), C.c_double(alpha), C.c_int(real_images),
+                               C.c_int(modes), C.c_double(eps), _p(a[0]), _p(a[1]), _p(a[2])))
+    return a
+
+
+def kick(vx, vy, vz, ax, ay, az, kick_factor, drag):
+    load().oracle_kick(_p(vx), _p(vy), _p(vz), _p(_f64(ax)), _p(_f64(ay)), _p(_f64(az)), C.c_uint64(vx.size),
+                       C.c_double(kick_factor), C.c_double(drag))
+
+
+def drift(x, y, z, vx, vy, vz, drift_factor, box):
+    L = (C.c_double * 3)(*box)
+    load().oracle_drift(_p(x), _p(y), _p(z), _p(vx), _p(vy), _p(vz), C.c_uint64(x.size),
+                        C.c_double(drift_factor), L)
