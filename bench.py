@@ -182,7 +182,8 @@ def main():
                                     "frac": path_bytes / (step_ms * 1e-3) / HBM_PEAK}},
         "cpu_baseline": cpu,
         "counters": {k: diag[k] for k in ("node_count", "max_depth", "direct_pair_evaluations",
-                                          "accepted_internal_multipoles", "accepted_leaves", "visited_nodes")},
+                                          "accepted_internal_multipoles", "accepted_leaves", "visited_nodes",
+                                          "wave_node_visits", "wave_leaf_source_slots")},
     }
     coord.close()
     print(json.dumps(line), flush=True)

@@ -113,6 +113,8 @@ class Diagnostics(C.Structure):
         ("ms_pm_fft_poisson", C.c_double),
         ("ms_pm_interpolate", C.c_double),
         ("ms_walk", C.c_double),
+        ("wave_node_visits", C.c_uint64),
+        ("wave_leaf_source_slots", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

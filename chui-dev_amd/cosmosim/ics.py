@@ -72,7 +72,7 @@ def zeldovich_displacement(n_side: int, box: float, spectral_index: float, rms_s
     for axis, ka in enumerate((kx, ky, kzz)):
         psi_k = np.zeros_like(dk)
         psi_k[sel] = 1j * ka[sel] / k2[sel] * dk[sel]
-        disp[axis] = np.fft.irfftn(psi_k, s=(n_side, n_side, n_side)).reshape(-1)
+        disp[axis] = np.fft.irfftn(psi_k, s=(n_side, n_side, n_side), axes=(0, 1, 2)).reshape(-1)
     rms = np.sqrt(np.mean(disp**2))
     spacing = box / n_side
     if rms > 0:

@@ -43,7 +43,8 @@ struct chui_treepm_context {
   bool own_stream = false;
   chui::Tree tree;
   chui::PmWorkspace pm64, pm32;
-  DeviceBuffer sx, sy, sz, sm64, qx, qy, qz, sm32;
+  DeviceBuffer src64, src32, htab;  // tree-ordered packed sources; H(t) table
+  double htab_key[2] = {0.0, 0.0};
   DeviceBuffer hx, hy, hz, hm, hactive, hout[9];
   DeviceBuffer tpos, tslot, tkeys_alt, tvals_alt, tsort_scratch, flag;
   DeviceBuffer counters, red_scratch, ewald_scratch;
@@ -165,17 +166,16 @@ void validate_options(const chui_treepm_options* o, const int mesh[3]) {
 }
 
 // ---- gather kernels ---------------------------------------------------------
+// Tree-order copies of the caller's sources (tree_pm_coupling.cpp:3732-3800):
+// FP64 keeps the caller's values verbatim; mixed stores 32-bit fixed-point
+// periodic coordinates and the FP32 mass.
 __global__ void gather64_kernel(const uint32_t* __restrict__ order, std::size_t n, const double* __restrict__ x,
                                 const double* __restrict__ y, const double* __restrict__ z,
-                                const double* __restrict__ m, double* __restrict__ ox, double* __restrict__ oy,
-                                double* __restrict__ oz, double* __restrict__ om) {
+                                const double* __restrict__ m, double4* __restrict__ out) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t r = order[i];
-  ox[i] = x[r];
-  oy[i] = y[r];
-  oz[i] = z[r];
-  if (om) om[i] = m[r];
+  out[i] = make_double4(x[r], y[r], z[r], m[r]);
 }
 
 __device__ __forceinline__ uint32_t fixed_of(double x, double L) {
@@ -190,15 +190,12 @@ __device__ __forceinline__ uint32_t fixed_of(double x, double L) {
 __global__ void gather_fixed_kernel(const uint32_t* __restrict__ order, std::size_t n, const double* __restrict__ x,
                                     const double* __restrict__ y, const double* __restrict__ z,
                                     const double* __restrict__ m, double Lx, double Ly, double Lz,
-                                    uint32_t* __restrict__ qx, uint32_t* __restrict__ qy, uint32_t* __restrict__ qz,
-                                    float* __restrict__ om) {
+                                    uint4* __restrict__ out) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t r = order[i];
-  qx[i] = fixed_of(x[r], Lx);
-  qy[i] = fixed_of(y[r], Ly);
-  qz[i] = fixed_of(z[r], Lz);
-  if (om) om[i] = static_cast<float>(m[r]);
+  out[i] = make_uint4(fixed_of(x[r], Lx), fixed_of(y[r], Ly), fixed_of(z[r], Lz),
+                      __float_as_uint(static_cast<float>(m[r])));
 }
 
 __global__ void target_keys_kernel(const uint32_t* __restrict__ active, std::size_t na,
@@ -291,22 +288,15 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   // --- tree-ordered source lanes ---
   if (n > 0) {
     if (!mixed) {
-      c->sx.reserve(n * 8);
-      c->sy.reserve(n * 8);
-      c->sz.reserve(n * 8);
-      if (!uniform) c->sm64.reserve(n * 8);
-      gather64_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(
-          c->tree.order.as<uint32_t>(), n, x, y, z, m, c->sx.as<double>(), c->sy.as<double>(), c->sz.as<double>(),
-          uniform ? nullptr : c->sm64.as<double>());
+      c->src64.reserve(n * sizeof(double4));
+      gather64_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(c->tree.order.as<uint32_t>(), n, x, y, z, m,
+                                                             c->src64.as<double4>());
       CHUI_LAUNCH_CHECK("gather64_kernel");
     } else {
-      c->qx.reserve(n * 4);
-      c->qy.reserve(n * 4);
-      c->qz.reserve(n * 4);
-      if (!uniform) c->sm32.reserve(n * 4);
-      gather_fixed_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(
-          c->tree.order.as<uint32_t>(), n, x, y, z, m, o->box_size[0], o->box_size[1], o->box_size[2],
-          c->qx.as<uint32_t>(), c->qy.as<uint32_t>(), c->qz.as<uint32_t>(), uniform ? nullptr : c->sm32.as<float>());
+      c->src32.reserve(n * sizeof(uint4));
+      gather_fixed_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(c->tree.order.as<uint32_t>(), n, x, y, z, m,
+                                                                 o->box_size[0], o->box_size[1], o->box_size[2],
+                                                                 c->src32.as<uint4>());
       CHUI_LAUNCH_CHECK("gather_fixed_kernel");
     }
   }
@@ -335,11 +325,9 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     if (!refresh)
       throw std::runtime_error("TreePM long-range field reuse requested without a matching field (refresh required)");
     if (!mixed) {
-      chui::pm_solve_fp64(pw, g, po, c->sx.as<double>(), c->sy.as<double>(), c->sz.as<double>(),
-                          uniform ? nullptr : c->sm64.as<double>(), um, n, s);
+      chui::pm_solve_fp64(pw, g, po, c->src64.as<double4>(), uniform ? 1 : 0, um, n, s);
     } else {
-      chui::pm_solve_mixed(pw, g, po, c->qx.as<uint32_t>(), c->qy.as<uint32_t>(), c->qz.as<uint32_t>(),
-                           uniform ? nullptr : c->sm32.as<float>(), static_cast<float>(um), n, s);
+      chui::pm_solve_mixed(pw, g, po, c->src32.as<uint4>(), uniform ? 1 : 0, static_cast<float>(um), n, s);
     }
     std::copy(key, key + 8, pw.key);
   }
@@ -379,6 +367,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   // --- walk (+ fused PM interpolation) ---
   c->counters.reserve(8 * sizeof(unsigned long long));
   chui::fill_u64(c->counters.as<uint64_t>(), 0ull, 8, s);
+  const bool check_walk_progress_ = diag != nullptr;
   chui::WalkParams wp{};
   for (int a = 0; a < 3; ++a) {
     wp.L[a] = o->box_size[a];
@@ -397,6 +386,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   wp.node_count = static_cast<uint32_t>(c->tree.node_count);
   wp.n_targets = static_cast<uint32_t>(n_active);
   wp.identity_targets = identity;
+  wp.count = diag != nullptr ? 1 : 0;
   wp.uniform_mass = um;
   wp.uniform_mass_f = static_cast<float>(um);
   wp.pm_enabled = 1;
@@ -404,24 +394,46 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   wp.mesh_row = g.row;
   wp.c4 = static_cast<float>(0.25 / (sp.a * sp.a));
   wp.c8 = static_cast<float>(0.125 / (sp.a * sp.a * sp.a));
+  wp.cubic = (o->box_size[0] == o->box_size[1] && o->box_size[0] == o->box_size[2]) ? 1 : 0;
+  {
+    // cutoff radii in separation units: fixed-point units for a cubic box
+    const double sep = wp.cubic ? o->box_size[0] / 4294967296.0 : 1.0;
+    const double r2 = wp.rcut2 / (sep * sep);
+    wp.rcut2_sep = static_cast<float>(r2);
+    wp.prune2_sep = static_cast<float>(r2 * (1.0 + 1.0e-5));
+    wp.contain2_sep = static_cast<float>(r2 * (1.0 - 1.0e-5));
+  }
   chui::WalkOutputs out{ax, ay, az, pmx, pmy, pmz, srx, sry, srz};
   if (n_active > 0 && n > 0) {
     if (!mixed) {
-      chui::walk_fp64(c->tree, wp, c->sx.as<double>(), c->sy.as<double>(), c->sz.as<double>(),
-                      uniform ? nullptr : c->sm64.as<double>(), target_pos, target_slot,
+      chui::walk_fp64(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, target_pos, target_slot,
                       static_cast<const double*>(pw.force[0]), static_cast<const double*>(pw.force[1]),
                       static_cast<const double*>(pw.force[2]), out,
                       reinterpret_cast<unsigned long long*>(c->counters.ptr), s);
     } else {
-      chui::fit_gaussian_coefficient(wp, wp.rcut2 * (0.25 / (sp.a * sp.a)) * (1.0 + 1.0e-3));
-      chui::walk_mixed(c->tree, wp, c->qx.as<uint32_t>(), c->qy.as<uint32_t>(), c->qz.as<uint32_t>(),
-                       uniform ? nullptr : c->sm32.as<float>(), target_pos, target_slot,
-                       static_cast<const float*>(pw.force[0]), static_cast<const float*>(pw.force[1]),
+      const double t_max = wp.rcut2 * (0.25 / (sp.a * sp.a)) * (1.0 + 1.0e-3);
+      std::vector<float> tab = chui::gaussian_coefficient_table(wp, t_max);
+      if (c->htab_key[0] != t_max || c->htab_key[1] != sp.a || !c->htab.ptr) {
+        c->htab.reserve(tab.size() * sizeof(float));
+        CHUI_HIP(hipMemcpyAsync(c->htab.ptr, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        CHUI_HIP(hipStreamSynchronize(s));
+        c->htab_key[0] = t_max;
+        c->htab_key[1] = sp.a;
+      }
+      chui::walk_mixed(c->tree, wp, c->src32.as<uint4>(), uniform ? 1 : 0, c->htab.as<float4>(), target_pos,
+                       target_slot, static_cast<const float*>(pw.force[0]), static_cast<const float*>(pw.force[1]),
                        static_cast<const float*>(pw.force[2]), out,
                        reinterpret_cast<unsigned long long*>(c->counters.ptr), s);
     }
   }
   record(c, kEvWalk);
+  if (check_walk_progress_) {
+    unsigned long long bad = 0;
+    CHUI_HIP(hipMemcpyAsync(&bad, reinterpret_cast<unsigned long long*>(c->counters.ptr) + chui::kWalkCounters,
+                            sizeof(bad), hipMemcpyDeviceToHost, s));
+    CHUI_HIP(hipStreamSynchronize(s));
+    if (bad) throw std::logic_error("TreePM walk lost progress (wave reduction selected no pending node)");
+  }
 
   if (diag) {
     if (diag->struct_size != sizeof(chui_treepm_diagnostics))
@@ -437,6 +449,8 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     diag->direct_pair_evaluations = cnt[2];
     diag->accepted_internal_multipoles = cnt[3];
     diag->accepted_leaves = cnt[4];
+    diag->wave_node_visits = cnt[5];
+    diag->wave_leaf_source_slots = cnt[6];
     diag->split_scale_comoving = sp.a;
     diag->cutoff_radius_comoving = sp.rcut;
     diag->mesh_spacing_comoving = sp.h;

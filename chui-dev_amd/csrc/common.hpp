@@ -87,6 +87,49 @@ __device__ __forceinline__ T wave_max(T v) {
   }
   return v;
 }
+// Wave64 min/max of a 32-bit value through DPP lane moves (no LDS traffic):
+// quad swaps, row rotations by 4 and 8, then the gfx9 row broadcasts of lanes
+// 15 and 31; lane 63 ends with the result, returned as a wave-uniform value.
+// All 64 lanes must be active.
+__device__ __forceinline__ uint32_t dpp_min_step(uint32_t v, uint32_t o) { return o < v ? o : v; }
+__device__ __forceinline__ uint32_t dpp_max_step(uint32_t v, uint32_t o) { return o > v ? o : v; }
+
+#define CHUI_DPP(v, ctrl, rmask, ident) \
+  static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(ident), static_cast<int>(v), ctrl, rmask, 0xf, false))
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+  v = dpp_min_step(v, CHUI_DPP(v, 0xb1, 0xf, 0xFFFFFFFFu));   // quad_perm [1,0,3,2]
+  v = dpp_min_step(v, CHUI_DPP(v, 0x4e, 0xf, 0xFFFFFFFFu));   // quad_perm [2,3,0,1]
+  v = dpp_min_step(v, CHUI_DPP(v, 0x124, 0xf, 0xFFFFFFFFu));  // row_ror:4
+  v = dpp_min_step(v, CHUI_DPP(v, 0x128, 0xf, 0xFFFFFFFFu));  // row_ror:8
+  v = dpp_min_step(v, CHUI_DPP(v, 0x142, 0xa, 0xFFFFFFFFu));  // row_bcast:15
+  v = dpp_min_step(v, CHUI_DPP(v, 0x143, 0xc, 0xFFFFFFFFu));  // row_bcast:31
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = dpp_max_step(v, CHUI_DPP(v, 0xb1, 0xf, 0u));
+  v = dpp_max_step(v, CHUI_DPP(v, 0x4e, 0xf, 0u));
+  v = dpp_max_step(v, CHUI_DPP(v, 0x124, 0xf, 0u));
+  v = dpp_max_step(v, CHUI_DPP(v, 0x128, 0xf, 0u));
+  v = dpp_max_step(v, CHUI_DPP(v, 0x142, 0xa, 0u));
+  v = dpp_max_step(v, CHUI_DPP(v, 0x143, 0xc, 0u));
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t lane) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
+}
+__device__ __forceinline__ float readlane_f32(float v, uint32_t lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), static_cast<int>(lane)));
+}
+__device__ __forceinline__ double readlane_f64(double v, uint32_t lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xFFFFFFFFll), static_cast<int>(lane));
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), static_cast<int>(lane));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -89,14 +89,14 @@ struct DepositArgs {
   int scheme;
 };
 
-__global__ __launch_bounds__(256) void deposit64_kernel(DepositArgs a, const double* __restrict__ x,
-                                                        const double* __restrict__ y, const double* __restrict__ z,
-                                                        const double* __restrict__ m, double um, std::size_t np,
+__global__ __launch_bounds__(256) void deposit64_kernel(DepositArgs a, const double4* __restrict__ src,
+                                                        int uniform, double um, std::size_t np,
                                                         double* __restrict__ rho) {
   const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const double g[3] = {wrap_position(x[p], a.L[0]) * a.inv_h[0], wrap_position(y[p], a.L[1]) * a.inv_h[1],
-                       wrap_position(z[p], a.L[2]) * a.inv_h[2]};
+  const double4 sp = src[p];
+  const double g[3] = {wrap_position(sp.x, a.L[0]) * a.inv_h[0], wrap_position(sp.y, a.L[1]) * a.inv_h[1],
+                       wrap_position(sp.z, a.L[2]) * a.inv_h[2]};
   int idx[3][3];
   double w[3][3];
   int cnt;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void deposit64_kernel(DepositArgs a, const dou
     }
   }
   cnt = a.scheme == 0 ? 2 : 3;
-  const double mm = m ? m[p] : um;
+  const double mm = uniform ? um : sp.w;
   for (int i = 0; i < cnt; ++i)
     for (int j = 0; j < cnt; ++j) {
       const std::size_t rb = (static_cast<std::size_t>(idx[0][i]) * a.n[1] + idx[1][j]) * a.row;
@@ -133,13 +133,12 @@ __global__ __launch_bounds__(256) void deposit64_kernel(DepositArgs a, const dou
     }
 }
 
-__global__ __launch_bounds__(256) void deposit32_kernel(DepositArgs a, const uint32_t* __restrict__ qx,
-                                                        const uint32_t* __restrict__ qy,
-                                                        const uint32_t* __restrict__ qz, const float* __restrict__ m,
+__global__ __launch_bounds__(256) void deposit32_kernel(DepositArgs a, const uint4* __restrict__ src, int uniform,
                                                         float um, std::size_t np, float* __restrict__ rho) {
   const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const uint32_t q[3] = {qx[p], qy[p], qz[p]};
+  const uint4 sp = src[p];
+  const uint32_t q[3] = {sp.x, sp.y, sp.z};
   int idx[3][3];
   float w[3][3];
   for (int ax = 0; ax < 3; ++ax) {
@@ -164,7 +163,7 @@ __global__ __launch_bounds__(256) void deposit32_kernel(DepositArgs a, const uin
     }
   }
   const int cnt = a.scheme == 0 ? 2 : 3;
-  const float mm = m ? m[p] : um;
+  const float mm = uniform ? um : __uint_as_float(sp.w);
   for (int i = 0; i < cnt; ++i)
     for (int j = 0; j < cnt; ++j) {
       const std::size_t rb = (static_cast<std::size_t>(idx[0][i]) * a.n[1] + idx[1][j]) * a.row;
@@ -535,29 +534,28 @@ DepositArgs deposit_args(const PmGeometry& g, const PmOptions& o) {
 
 }  // namespace
 
-void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double* x, const double* y,
-                   const double* z, const double* m, double uniform_mass, std::size_t n, hipStream_t s) {
+void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double4* src, int uniform_mass,
+                   double um, std::size_t n, hipStream_t s) {
   reserve<double>(w, g);
   const Tables<double> tb = upload_tables<double>(w, g, o, s);
   double* rho = w.buf[0].as<double>();
   CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(double), s));
   if (n > 0) {
-    deposit64_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), x, y, z, m, uniform_mass, n, rho);
+    deposit64_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, n, rho);
     CHUI_LAUNCH_CHECK("deposit64_kernel");
   }
   solve_spectral<double>(w, g, o, tb, s);
   w.valid = true;
 }
 
-void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint32_t* qx,
-                    const uint32_t* qy, const uint32_t* qz, const float* m, float uniform_mass, std::size_t n,
-                    hipStream_t s) {
+void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint4* src, int uniform_mass,
+                    float um, std::size_t n, hipStream_t s) {
   reserve<float>(w, g);
   const Tables<float> tb = upload_tables<float>(w, g, o, s);
   float* rho = w.buf[0].as<float>();
   CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(float), s));
   if (n > 0) {
-    deposit32_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), qx, qy, qz, m, uniform_mass, n, rho);
+    deposit32_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, n, rho);
     CHUI_LAUNCH_CHECK("deposit32_kernel");
   }
   solve_spectral<float>(w, g, o, tb, s);

@@ -39,13 +39,14 @@ struct PmWorkspace {
 
 // Deposit (tree-ordered particles), forward r2c FFT, Green's function with the
 // Gaussian long-range filter, spectral gradient and three c2r inverses.
-// FP64: positions are raw caller doubles; masses double (null = uniform).
-void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double* x, const double* y,
-                   const double* z, const double* m, double uniform_mass, std::size_t n, hipStream_t s);
-// Mixed: positions are 32-bit fixed-point periodic coordinates, FP32 mesh/FFT.
-void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint32_t* qx,
-                    const uint32_t* qy, const uint32_t* qz, const float* m, float uniform_mass, std::size_t n,
-                    hipStream_t s);
+// FP64: packed (x, y, z, m) caller doubles in tree order; uniform_mass != 0
+// means every source carries mass um.
+void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double4* src, int uniform_mass,
+                   double um, std::size_t n, hipStream_t s);
+// Mixed: packed 32-bit fixed-point periodic coordinates + float mass bits,
+// FP32 mesh and FFTs.
+void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint4* src, int uniform_mass,
+                    float um, std::size_t n, hipStream_t s);
 
 // Stage timers (ms) of the last solve, filled when events are enabled.
 struct PmTimings {

@@ -11,12 +11,12 @@
 // (tree_pm_coupling.cpp:4026-4272), and leaf pairs are evaluated with the
 // source shared by the whole wave.
 //
-// walk64_kernel  : FP64, the reference's arithmetic (erf/exp split kernel,
-//                  nearbyint minimum image, monopole + quadrupole residual).
+// walk64_kernel  : FP64, the reference's arithmetic (erf/exp split
+//                  kernel, nearbyint minimum image, monopole + quadrupole residual).
 // walkmix_kernel : 32-bit fixed-point positions (minimum image = int32
-//                  subtraction), FP32 pair kernel with a polynomial for the
-//                  Gaussian coefficient H(t), per-leaf partial sums folded into
-//                  a Kahan-compensated FP32 accumulator.
+//                  subtraction), FP32 pair kernel with a cubic-cell LDS table
+//                  for the Gaussian coefficient H(t), per-drain partial sums
+//                  folded into a Kahan-compensated FP32 accumulator.
 #include "walk.hpp"
 
 #include <algorithm>
@@ -94,7 +94,7 @@ __device__ __forceinline__ bool within_cutoff(double d2, double c, double c2) {
 }
 
 // monopolePlusQuadrupoleAccelPeriodicReference (tree_pm_coupling.cpp:743-827)
-__device__ void node_accel64(const NodeMom64& M, const NodeQuad64& Q, double dx, double dy, double dz,
+__device__ __forceinline__ void node_accel64(const NodeMom64& M, const NodeQuad64& Q, double dx, double dy, double dz,
                              const WalkParams& P, double& ax, double& ay, double& az) {
   const double r2 = dx * dx + dy * dy + dz * dz;
   const double safe_r2 = fmax(r2, 1.0e-30);
@@ -163,52 +163,52 @@ __device__ __forceinline__ int wrap_index(long i, int n) {
   if (w < 0) w += n;
   return static_cast<int>(w);
 }
-struct Stencil64 {
-  int idx[3];
-  double w[3];
-  int count;
-};
-__device__ __forceinline__ Stencil64 stencil64(double g, int scheme, int n) {
-  Stencil64 s;
-  if (scheme == 0) {
+
+// Stencil of one axis with a compile-time point count (CIC 2, TSC 3) so the
+// gather below is fully unrolled and stays in registers.
+template <int SCHEME>
+__device__ __forceinline__ void stencil64(double g, int n, int (&idx)[SCHEME + 2], double (&w)[SCHEME + 2]) {
+  if constexpr (SCHEME == 0) {
     const double base = floor(g);
     const long i0 = static_cast<long>(base);
     const double t = g - base;
-    s.idx[0] = wrap_index(i0, n);
-    s.idx[1] = wrap_index(i0 + 1, n);
-    s.idx[2] = 0;
-    s.w[0] = 1.0 - t;
-    s.w[1] = t;
-    s.w[2] = 0.0;
-    s.count = 2;
+    idx[0] = wrap_index(i0, n);
+    idx[1] = wrap_index(i0 + 1, n);
+    w[0] = 1.0 - t;
+    w[1] = t;
   } else {
     const double c = floor(g + 0.5);
     const long ic = static_cast<long>(c);
     const double d = g - c;
-    s.idx[0] = wrap_index(ic - 1, n);
-    s.idx[1] = wrap_index(ic, n);
-    s.idx[2] = wrap_index(ic + 1, n);
-    s.w[0] = 0.5 * (0.5 - d) * (0.5 - d);
-    s.w[1] = 0.75 - d * d;
-    s.w[2] = 0.5 * (0.5 + d) * (0.5 + d);
-    s.count = 3;
+    idx[0] = wrap_index(ic - 1, n);
+    idx[1] = wrap_index(ic, n);
+    idx[2] = wrap_index(ic + 1, n);
+    w[0] = 0.5 * (0.5 - d) * (0.5 - d);
+    w[1] = 0.75 - d * d;
+    w[2] = 0.5 * (0.5 + d) * (0.5 + d);
   }
-  return s;
 }
 
+template <int SCHEME>
 __device__ void interp64(const WalkParams& P, const double* __restrict__ fx, const double* __restrict__ fy,
                          const double* __restrict__ fz, double x, double y, double z, double& gx, double& gy,
                          double& gz) {
-  const Stencil64 sx = stencil64(wrap_position(x, P.L[0]) * (static_cast<double>(P.mesh[0]) / P.L[0]), P.assignment, P.mesh[0]);
-  const Stencil64 sy = stencil64(wrap_position(y, P.L[1]) * (static_cast<double>(P.mesh[1]) / P.L[1]), P.assignment, P.mesh[1]);
-  const Stencil64 sz = stencil64(wrap_position(z, P.L[2]) * (static_cast<double>(P.mesh[2]) / P.L[2]), P.assignment, P.mesh[2]);
+  constexpr int K = SCHEME + 2;
+  int ix[K], iy[K], iz[K];
+  double wx[K], wy[K], wz[K];
+  stencil64<SCHEME>(wrap_position(x, P.L[0]) * (static_cast<double>(P.mesh[0]) / P.L[0]), P.mesh[0], ix, wx);
+  stencil64<SCHEME>(wrap_position(y, P.L[1]) * (static_cast<double>(P.mesh[1]) / P.L[1]), P.mesh[1], iy, wy);
+  stencil64<SCHEME>(wrap_position(z, P.L[2]) * (static_cast<double>(P.mesh[2]) / P.L[2]), P.mesh[2], iz, wz);
   gx = gy = gz = 0.0;
-  for (int a = 0; a < sx.count; ++a)
-    for (int b = 0; b < sy.count; ++b) {
-      const std::size_t row = (static_cast<std::size_t>(sx.idx[a]) * P.mesh[1] + sy.idx[b]) * P.mesh_row;
-      for (int c = 0; c < sz.count; ++c) {
-        const double w = sx.w[a] * sy.w[b] * sz.w[c];
-        const std::size_t i = row + sz.idx[c];
+#pragma unroll
+  for (int a = 0; a < K; ++a)
+#pragma unroll
+    for (int b = 0; b < K; ++b) {
+      const std::size_t row = (static_cast<std::size_t>(ix[a]) * P.mesh[1] + iy[b]) * P.mesh_row;
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const double w = wx[a] * wy[b] * wz[c];
+        const std::size_t i = row + iz[c];
         gx += w * fx[i];
         gy += w * fy[i];
         gz += w * fz[i];
@@ -218,52 +218,49 @@ __device__ void interp64(const WalkParams& P, const double* __restrict__ fx, con
 
 // Stencil from a 32-bit fixed-point coordinate on a power-of-two mesh: the
 // cell index is the top log2(n) bits, the fraction the remaining bits.
-struct Stencil32 {
-  int idx[3];
-  float w[3];
-  int count;
-};
-__device__ __forceinline__ Stencil32 stencil32(uint32_t q, int scheme, int n, int lg) {
-  Stencil32 s;
+template <int SCHEME>
+__device__ __forceinline__ void stencil32(uint32_t q, int n, int lg, int (&idx)[SCHEME + 2], float (&w)[SCHEME + 2]) {
   const int sh = 32 - lg;
   const int cell = static_cast<int>(q >> sh);
-  const float t = static_cast<float>(q & ((sh == 32) ? 0xFFFFFFFFu : ((1u << sh) - 1u))) * __builtin_ldexpf(1.0f, -sh);
-  if (scheme == 0) {
-    s.idx[0] = cell;
-    s.idx[1] = (cell + 1) & (n - 1);
-    s.idx[2] = 0;
-    s.w[0] = 1.0f - t;
-    s.w[1] = t;
-    s.w[2] = 0.0f;
-    s.count = 2;
+  const float t = static_cast<float>(q & ((1u << sh) - 1u)) * __builtin_ldexpf(1.0f, -sh);
+  if constexpr (SCHEME == 0) {
+    idx[0] = cell;
+    idx[1] = (cell + 1) & (n - 1);
+    w[0] = 1.0f - t;
+    w[1] = t;
   } else {
     const int c = t >= 0.5f ? cell + 1 : cell;
     const float d = t >= 0.5f ? t - 1.0f : t;
-    s.idx[0] = (c - 1) & (n - 1);
-    s.idx[1] = c & (n - 1);
-    s.idx[2] = (c + 1) & (n - 1);
-    s.w[0] = 0.5f * (0.5f - d) * (0.5f - d);
-    s.w[1] = 0.75f - d * d;
-    s.w[2] = 0.5f * (0.5f + d) * (0.5f + d);
-    s.count = 3;
+    idx[0] = (c - 1) & (n - 1);
+    idx[1] = c & (n - 1);
+    idx[2] = (c + 1) & (n - 1);
+    w[0] = 0.5f * (0.5f - d) * (0.5f - d);
+    w[1] = 0.75f - d * d;
+    w[2] = 0.5f * (0.5f + d) * (0.5f + d);
   }
-  return s;
 }
 
+template <int SCHEME>
 __device__ void interp32(const WalkParams& P, const float* __restrict__ fx, const float* __restrict__ fy,
                          const float* __restrict__ fz, uint32_t qx, uint32_t qy, uint32_t qz, float& gx, float& gy,
                          float& gz) {
-  const Stencil32 sx = stencil32(qx, P.assignment, P.mesh[0], P.mesh_log2[0]);
-  const Stencil32 sy = stencil32(qy, P.assignment, P.mesh[1], P.mesh_log2[1]);
-  const Stencil32 sz = stencil32(qz, P.assignment, P.mesh[2], P.mesh_log2[2]);
+  constexpr int K = SCHEME + 2;
+  int ix[K], iy[K], iz[K];
+  float wx[K], wy[K], wz[K];
+  stencil32<SCHEME>(qx, P.mesh[0], P.mesh_log2[0], ix, wx);
+  stencil32<SCHEME>(qy, P.mesh[1], P.mesh_log2[1], iy, wy);
+  stencil32<SCHEME>(qz, P.mesh[2], P.mesh_log2[2], iz, wz);
   gx = gy = gz = 0.0f;
-  for (int a = 0; a < sx.count; ++a)
-    for (int b = 0; b < sy.count; ++b) {
-      const std::size_t row = (static_cast<std::size_t>(sx.idx[a]) * P.mesh[1] + sy.idx[b]) * P.mesh_row;
-      const float wab = sx.w[a] * sy.w[b];
-      for (int c = 0; c < sz.count; ++c) {
-        const float w = wab * sz.w[c];
-        const std::size_t i = row + sz.idx[c];
+#pragma unroll
+  for (int a = 0; a < K; ++a)
+#pragma unroll
+    for (int b = 0; b < K; ++b) {
+      const std::size_t row = (static_cast<std::size_t>(ix[a]) * P.mesh[1] + iy[b]) * P.mesh_row;
+      const float wab = wx[a] * wy[b];
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const float w = wab * wz[c];
+        const std::size_t i = row + iz[c];
         gx = fmaf(w, fx[i], gx);
         gy = fmaf(w, fy[i], gy);
         gz = fmaf(w, fz[i], gz);
@@ -286,116 +283,13 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, con
   }
 }
 
-__global__ __launch_bounds__(kWalkThreads) void walk64_kernel(
-    WalkParams P, const NodeLink* __restrict__ link, const NodeGeo64* __restrict__ geo,
-    const NodeMom64* __restrict__ mom, const NodeQuad64* __restrict__ quad, const uint32_t* __restrict__ order,
-    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    const double* __restrict__ sm, const uint32_t* __restrict__ tpos, const uint32_t* __restrict__ tslot,
-    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz, WalkOutputs out,
-    unsigned long long* __restrict__ counters) {
-  const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
-  const bool valid = k < P.n_targets;
-  const uint32_t tp = valid ? (P.identity_targets ? k : tpos[k]) : 0u;
-  const uint32_t slot = valid ? (P.identity_targets ? order[k] : tslot[k]) : 0u;
-  const double px = valid ? sx[tp] : 0.0, py = valid ? sy[tp] : 0.0, pz = valid ? sz[tp] : 0.0;
-  double gx = 0.0, gy = 0.0, gz = 0.0;
-  if (P.pm_enabled && valid) interp64(P, fx, fy, fz, px, py, pz, gx, gy, gz);
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0};
-  uint32_t nxt = valid ? 0u : P.node_count;
-  const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
-  while (true) {
-    const uint32_t n = __builtin_amdgcn_readfirstlane(wave_min(nxt));
-    if (n >= P.node_count) break;
-    const NodeLink lk = link[n];
-    const NodeGeo64 g = geo[n];
-    const bool act = nxt == n;
-    bool do_leaf = false;
-    if (act) {
-      ++cnt[0];
-      const double h = g.half;
-      const double ndx = mi64(g.cx - px, L0), ndy = mi64(g.cy - py, L1), ndz = mi64(g.cz - pz, L2);
-      const double ax_ = fabs(ndx), ay_ = fabs(ndy), az_ = fabs(ndz);
-      const double ex = fmax(0.0, ax_ - h), ey = fmax(0.0, ay_ - h), ez = fmax(0.0, az_ - h);
-      if (!within_cutoff(ex * ex + ey * ey + ez * ez, P.rcut, P.rcut2)) {
-        ++cnt[1];
-        nxt = n + lk.skip;
-      } else if (lk.child_count == 0) {
-        ++cnt[4];
-        do_leaf = true;
-        nxt = n + 1;
-      } else {
-        const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
-        const double mx = ax_ + h, my = ay_ + h, mz = az_ + h;
-        const bool contained = within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
-        const NodeMom64 M = mom[n];
-        const double dx = mi64(M.comx - px, L0), dy = mi64(M.comy - py, L1), dz = mi64(M.comz - pz, L2);
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const NodeQuad64 Q = quad[n];
-        if (accept64(h, Q.offset, inside, contained, r2, P)) {
-          ++cnt[3];
-          node_accel64(M, Q, dx, dy, dz, P, ax, ay, az);
-          nxt = n + lk.skip;
-        } else {
-          nxt = n + 1;
-        }
-      }
-    }
-    if (__ballot(do_leaf)) {
-      const uint32_t b0 = lk.begin, e0 = lk.begin + lk.count;
-      for (uint32_t j = b0; j < e0; ++j) {
-        const double sxj = sx[j], syj = sy[j], szj = sz[j];
-        const double mj = sm ? sm[j] : P.uniform_mass;
-        if (do_leaf && j != tp) {
-          const double qx = mi64(sxj - px, L0), qy = mi64(syj - py, L1), qz = mi64(szj - pz, L2);
-          const double r2 = qx * qx + qy * qy + qz * qz;
-          if (r2 <= P.rcut2) {
-            ++cnt[2];
-            if (!(qx == 0.0 && qy == 0.0 && qz == 0.0)) {
-              const double f = residual_inv_r3(r2, P.eps, P.a) * P.G;
-              ax += f * mj * qx;
-              ay += f * mj * qy;
-              az += f * mj * qz;
-            }
-          }
-        }
-      }
-    }
-  }
-  flush_counters(counters, cnt);
-  if (!valid) return;
-  double* oax = static_cast<double*>(out.ax);
-  double* oay = static_cast<double*>(out.ay);
-  double* oaz = static_cast<double*>(out.az);
-  oax[slot] = gx + ax;
-  oay[slot] = gy + ay;
-  oaz[slot] = gz + az;
-  if (out.pm_x) {
-    static_cast<double*>(out.pm_x)[slot] = gx;
-    static_cast<double*>(out.pm_y)[slot] = gy;
-    static_cast<double*>(out.pm_z)[slot] = gz;
-  }
-  if (out.sr_x) {
-    static_cast<double*>(out.sr_x)[slot] = ax;
-    static_cast<double*>(out.sr_y)[slot] = ay;
-    static_cast<double*>(out.sr_z)[slot] = az;
-  }
-}
-
-// ---- mixed precision ---------------------------------------------------------
+// ---- mixed-precision pair pieces -----------------------------------------------
 __device__ __forceinline__ float fx_delta(uint32_t a, uint32_t b, float scale) {
   return static_cast<float>(static_cast<int32_t>(a - b)) * scale;
 }
 
-__device__ __forceinline__ float gaussian_h(float t, const WalkParams& P) {
-  const float s = (t - P.t_mid) * P.t_inv_half;
-  float h = P.poly[P.poly_degree];
-  for (int i = P.poly_degree - 1; i >= 0; --i) h = fmaf(h, s, P.poly[i]);
-  return h;
-}
-
-__device__ void node_accel32(const NodeMom32& M, const NodeQuad32& Q, float dx, float dy, float dz,
-                             const WalkParams& P, float& ax, float& ay, float& az) {
+__device__ __forceinline__ void node_accel32(const NodeMom32& M, const NodeQuad32& Q, float dx, float dy, float dz,
+                                          const WalkParams& P, float& ax, float& ay, float& az) {
   const float r2 = dx * dx + dy * dy + dz * dz;
   const float safe_r2 = fmaxf(r2, 1.0e-30f);
   const float r = sqrtf(safe_r2);
@@ -426,7 +320,7 @@ __device__ void node_accel32(const NodeMom32& M, const NodeQuad32& Q, float dx, 
     const float isp = static_cast<float>(kInvSqrtPi);
     const float s1 = -0.5f * isp * r2 * ex / (a * a2);
     const float s2 = isp * ex * (-r / (a * a2) + 0.25f * r * r2 / (a * a2 * a2));
-    const float d52 = soft / (denom);  // denom^-5/2
+    const float d52 = soft / denom;  // denom^-5/2
     const float f1 = -3.0f * r * d52;
     const float f2 = -3.0f * d52 + 15.0f * r2 * d52 / denom;
     const float n1 = -3.0f / (safe_r2 * safe_r2);
@@ -439,9 +333,9 @@ __device__ void node_accel32(const NodeMom32& M, const NodeQuad32& Q, float dx, 
     cy += pref * (mdy * fo + 0.5f * tr * dy * fo + 0.5f * dy * dmd * cr);
     cz += pref * (mdz * fo + 0.5f * tr * dz * fo + 0.5f * dz * dmd * cr);
   }
-  ax += cx;
-  ay += cy;
-  az += cz;
+  ax = cx;
+  ay = cy;
+  az = cz;
 }
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
@@ -451,109 +345,344 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   s = t;
 }
 
-__global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(
-    WalkParams P, const NodeLink* __restrict__ link, const NodeGeo32* __restrict__ geo,
-    const NodeMom32* __restrict__ mom, const NodeQuad32* __restrict__ quad, const uint32_t* __restrict__ order,
-    const uint32_t* __restrict__ qxs, const uint32_t* __restrict__ qys, const uint32_t* __restrict__ qzs,
-    const float* __restrict__ sm, const uint32_t* __restrict__ tpos, const uint32_t* __restrict__ tslot,
-    const float* __restrict__ fx, const float* __restrict__ fy, const float* __restrict__ fz, WalkOutputs out,
-    unsigned long long* __restrict__ counters) {
+// ---- the walk -------------------------------------------------------------------
+// Per node, every lane whose next node it is applies the reference's cutoff
+// prune; whether the node is a leaf is wave-uniform, so leaf and internal-node
+// work are scalar branches. While the wave streams a leaf's sources (wave-
+// uniform scalar loads of packed records) each lane only forms the separation,
+// applies the cutoff and self test and appends the source index to its own LDS
+// queue; the residual kernel then runs over the queues, so lanes spend it only
+// on pairs inside r_cut. A queue is drained before it can overflow and before
+// any lane adds an accepted multipole, so each lane's contributions are summed
+// in the order of the reference's sequential walk. The next node is i + 1
+// whenever any lane is waiting on it (one ballot), otherwise the DPP wave
+// minimum of the lanes' next nodes.
+constexpr int kWalkWaves = kWalkThreads / kWave;
+constexpr int kQueue = 32;
+constexpr int kChunk = 16;
+
+struct WalkData {
+  const NodeLink* link;
+  const NodeGeo64* geo64;
+  const NodeMom64* mom64;
+  const NodeQuad64* quad64;
+  const NodeGeo32* geo32;
+  const NodeMom32* mom32;
+  const NodeQuad32* quad32;
+  const uint32_t* order;
+  const double4* src64;
+  const uint4* src32;
+  const float4* htab;
+  int uniform_mass;
+  const uint32_t *tpos, *tslot;
+  const double *f64x, *f64y, *f64z;
+  const float *f32x, *f32y, *f32z;
+};
+
+__device__ __forceinline__ uint32_t next_node(uint32_t n, uint32_t nxt) {
+  return __ballot(nxt == n + 1) ? n + 1 : wave_min_u32(nxt);
+}
+
+// ---- FP64 walk (reference arithmetic) ---------------------------------------------
+template <int SCHEME>
+__global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, WalkData D, WalkOutputs out,
+                                                               unsigned long long* __restrict__ counters) {
+  __shared__ uint32_t queue[kWalkWaves][kQueue][kWave];
+  const int w = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
   const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
   const bool valid = k < P.n_targets;
-  const uint32_t tp = valid ? (P.identity_targets ? k : tpos[k]) : 0u;
-  const uint32_t slot = valid ? (P.identity_targets ? order[k] : tslot[k]) : 0u;
-  const uint32_t px = valid ? qxs[tp] : 0u, py = valid ? qys[tp] : 0u, pz = valid ? qzs[tp] : 0u;
-  float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-  if (P.pm_enabled && valid) interp32(P, fx, fy, fz, px, py, pz, gx, gy, gz);
-  const float s0 = P.fx_to_len[0], s1 = P.fx_to_len[1], s2 = P.fx_to_len[2];
-  const float rcut = static_cast<float>(P.rcut);
-  const float rcut2 = static_cast<float>(P.rcut2);
-  const float prune2 = static_cast<float>(P.rcut2 * (1.0 + 1.0e-5));
-  const float contain2 = static_cast<float>(P.rcut2 * (1.0 - 1.0e-5));
-  const float eps2 = static_cast<float>(P.eps * P.eps);
-  const float theta = static_cast<float>(P.theta);
-  const float gm = static_cast<float>(P.G);
-  float sx_ = 0.0f, cx_ = 0.0f, sy_ = 0.0f, cy_ = 0.0f, sz_ = 0.0f, cz_ = 0.0f;
-  uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0};
+  const uint32_t tp = valid ? (P.identity_targets ? k : D.tpos[k]) : 0u;
+  const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
+  uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t qlen = 0, queued = 0;
   uint32_t nxt = valid ? 0u : P.node_count;
-  (void)rcut;
-  while (true) {
-    const uint32_t n = __builtin_amdgcn_readfirstlane(wave_min(nxt));
-    if (n >= P.node_count) break;
-    const NodeLink lk = link[n];
-    const NodeGeo32 g = geo[n];
+  double px = 0.0, py = 0.0, pz = 0.0;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  if (valid) {
+    const double4 t = D.src64[tp];
+    px = t.x;
+    py = t.y;
+    pz = t.z;
+    if (P.pm_enabled) interp64<SCHEME>(P, D.f64x, D.f64y, D.f64z, px, py, pz, gx, gy, gz);
+  }
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
+
+  auto drain = [&]() {
+    const uint32_t m = wave_max_u32(qlen);
+    for (uint32_t q = 0; q < m; ++q) {
+      if (q < qlen) {
+        const uint32_t j = queue[w][q][lane];
+        const double4 sj = D.src64[j];
+        const double qx = mi64(sj.x - px, L0), qy = mi64(sj.y - py, L1), qz = mi64(sj.z - pz, L2);
+        const double r2 = qx * qx + qy * qy + qz * qz;
+        const double mj = D.uniform_mass ? P.uniform_mass : sj.w;
+        const double f = residual_inv_r3(r2, P.eps, P.a) * P.G;
+        ax += f * mj * qx;
+        ay += f * mj * qy;
+        az += f * mj * qz;
+      }
+    }
+    queued += qlen;
+    qlen = 0;
+  };
+
+  uint32_t n = wave_min_u32(nxt);
+  while (n < P.node_count) {
+    if (lane == 0) ++cnt[5];
+    const NodeLink lk = D.link[n];
+    const NodeGeo64 g = D.geo64[n];
     const bool act = nxt == n;
-    bool do_leaf = false;
-    if (act) {
-      ++cnt[0];
-      const float h = g.half;
-      const float ax_ = fabsf(fx_delta(g.cx, px, s0)), ay_ = fabsf(fx_delta(g.cy, py, s1)),
-                  az_ = fabsf(fx_delta(g.cz, pz, s2));
-      const float ex = fmaxf(0.0f, ax_ - h), ey = fmaxf(0.0f, ay_ - h), ez = fmaxf(0.0f, az_ - h);
-      if (ex * ex + ey * ey + ez * ez > prune2) {
-        ++cnt[1];
-        nxt = n + lk.skip;
-      } else if (lk.child_count == 0) {
+    if (!__ballot(act)) {  // the chosen node must be some lane's next node
+      if (lane == 0) atomicOr(&counters[kWalkCounters], 1ull);
+      break;
+    }
+    cnt[0] += act;
+    const double h = g.half;
+    const double ndx = mi64(g.cx - px, L0), ndy = mi64(g.cy - py, L1), ndz = mi64(g.cz - pz, L2);
+    const double ax_ = fabs(ndx), ay_ = fabs(ndy), az_ = fabs(ndz);
+    const double ex = fmax(0.0, ax_ - h), ey = fmax(0.0, ay_ - h), ez = fmax(0.0, az_ - h);
+    const bool keep = act && within_cutoff(ex * ex + ey * ey + ez * ez, P.rcut, P.rcut2);
+    if (act && !keep) {
+      ++cnt[1];
+      nxt = n + lk.skip;
+    }
+    if (lk.child_count == 0) {
+      if (keep) {
         ++cnt[4];
-        do_leaf = true;
         nxt = n + 1;
-      } else {
+      }
+      if (__ballot(keep)) {
+        const uint32_t e0 = lk.begin + lk.count;
+        for (uint32_t c0 = lk.begin; c0 < e0; c0 += kChunk) {
+          const uint32_t c1 = c0 + kChunk < e0 ? c0 + kChunk : e0;
+          if (lane == 0) cnt[6] += c1 - c0;
+          for (uint32_t j = c0; j < c1; ++j) {
+            const double4 sj = D.src64[j];
+            const double qx = mi64(sj.x - px, L0), qy = mi64(sj.y - py, L1), qz = mi64(sj.z - pz, L2);
+            const double r2 = qx * qx + qy * qy + qz * qz;
+            if (keep && j != tp && r2 <= P.rcut2) {
+              if (qx == 0.0 && qy == 0.0 && qz == 0.0) {
+                ++cnt[2];  // counted coincident pair, zero vector
+              } else {
+                queue[w][qlen][lane] = j;
+                ++qlen;
+              }
+            }
+          }
+          if (wave_max_u32(qlen) > static_cast<uint32_t>(kQueue - kChunk)) drain();
+        }
+      }
+    } else if (__ballot(keep)) {
+      bool take_mp = false;
+      double mdx = 0.0, mdy = 0.0, mdz = 0.0;
+      if (keep) {
+        const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+        const double mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+        const bool contained = within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
+        const NodeMom64 M = D.mom64[n];
+        mdx = mi64(M.comx - px, L0);
+        mdy = mi64(M.comy - py, L1);
+        mdz = mi64(M.comz - pz, L2);
+        const double r2 = mdx * mdx + mdy * mdy + mdz * mdz;
+        take_mp = accept64(h, D.quad64[n].offset, inside, contained, r2, P);
+        nxt = take_mp ? n + lk.skip : n + 1;
+      }
+      if (__ballot(take_mp)) {
+        drain();
+        if (take_mp) {
+          ++cnt[3];
+          node_accel64(D.mom64[n], D.quad64[n], mdx, mdy, mdz, P, ax, ay, az);
+        }
+      }
+    }
+    n = next_node(n, nxt);
+  }
+  drain();
+  cnt[2] += queued;
+  flush_counters(counters, cnt);
+  if (!valid) return;
+  static_cast<double*>(out.ax)[slot] = gx + ax;
+  static_cast<double*>(out.ay)[slot] = gy + ay;
+  static_cast<double*>(out.az)[slot] = gz + az;
+  if (out.pm_x) {
+    static_cast<double*>(out.pm_x)[slot] = gx;
+    static_cast<double*>(out.pm_y)[slot] = gy;
+    static_cast<double*>(out.pm_z)[slot] = gz;
+  }
+  if (out.sr_x) {
+    static_cast<double*>(out.sr_x)[slot] = ax;
+    static_cast<double*>(out.sr_y)[slot] = ay;
+    static_cast<double*>(out.sr_z)[slot] = az;
+  }
+}
+
+// ---- mixed walk -----------------------------------------------------------------
+// CUBIC: equal box lengths, so separations stay in fixed-point units (one
+// scale factor for all axes) and only the pair kernel converts to lengths.
+// COUNT: keep the per-target counters (diagnostic launches only).
+// A leaf chunk of up to 64 sources is staged in LDS with one coalesced load
+// and read back with wave-uniform (broadcast) LDS reads; the per-source test
+// and the queue drain are branch-free so loads can be issued ahead.
+template <int SCHEME, bool CUBIC, bool COUNT>
+__global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out,
+                                                                unsigned long long* __restrict__ counters) {
+  __shared__ uint32_t queue[kWalkWaves][kQueue][kWave];
+  __shared__ uint4 stage[kWalkWaves][kWave];
+  __shared__ float4 htab[kHTableCells];
+  for (int i = threadIdx.x; i < kHTableCells; i += kWalkThreads) htab[i] = D.htab[i];
+  __syncthreads();
+  const int w = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
+  const bool valid = k < P.n_targets;
+  const uint32_t tp = valid ? (P.identity_targets ? k : D.tpos[k]) : 0u;
+  const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
+  uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t qlen = 0, queued = 0;
+  uint32_t nxt = valid ? 0u : P.node_count;
+  uint32_t px = 0, py = 0, pz = 0;
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  if (valid) {
+    const uint4 t = D.src32[tp];
+    px = t.x;
+    py = t.y;
+    pz = t.z;
+    if (P.pm_enabled) interp32<SCHEME>(P, D.f32x, D.f32y, D.f32z, px, py, pz, gx, gy, gz);
+  }
+  float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f, cx_ = 0.0f, cy_ = 0.0f, cz_ = 0.0f;
+  // separations: fixed-point units (CUBIC) or lengths
+  const float s0 = CUBIC ? 1.0f : P.fx_to_len[0];
+  const float s1 = CUBIC ? 1.0f : P.fx_to_len[1];
+  const float s2 = CUBIC ? 1.0f : P.fx_to_len[2];
+  const float to_len = CUBIC ? P.fx_to_len[0] : 1.0f;          // separation unit -> length
+  const float to_len2 = to_len * to_len;
+  const float inv_len = CUBIC ? 1.0f / P.fx_to_len[0] : 1.0f;  // length -> separation unit
+  const float rc2 = P.rcut2_sep;
+  const float prune2 = P.prune2_sep;
+  const float contain2 = P.contain2_sep;
+  const float eps2 = static_cast<float>(P.eps * P.eps);
+  const float theta2 = static_cast<float>(P.theta * P.theta);
+
+  auto drain = [&]() {
+    const uint32_t m = wave_max_u32(qlen);
+    float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+#pragma unroll 2
+    for (uint32_t q = 0; q < m; ++q) {
+      const bool live = q < qlen;
+      const uint32_t qj = queue[w][q][lane];
+      const uint32_t j = live ? qj : tp;
+      const uint4 sj = D.src32[j];
+      const float dx = fx_delta(sj.x, px, s0), dy = fx_delta(sj.y, py, s1), dz = fx_delta(sj.z, pz, s2);
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) * to_len2;
+      const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
+      const float soft = rinv * rinv * rinv;
+      // H(t), t = r^2/(4 a^2): cubic Hermite cell from the LDS table
+      const float u = r2 * P.t_to_cell;
+      const int ci = min(static_cast<int>(u), kHTableCells - 1);
+      const float fu = u - static_cast<float>(ci);
+      const float4 c = htab[ci];
+      const float hcoef = fmaf(fmaf(fmaf(c.w, fu, c.z), fu, c.y), fu, c.x);
+      const float mj = D.uniform_mass ? P.uniform_mass_f : __uint_as_float(sj.w);
+      const float fr = fmaf(-hcoef, P.c8, soft) * mj;
+      const float f = live ? fr : 0.0f;
+      lx = fmaf(f, dx, lx);
+      ly = fmaf(f, dy, ly);
+      lz = fmaf(f, dz, lz);
+    }
+    const float gm = static_cast<float>(P.G) * to_len;
+    kahan_add(sx_, cx_, lx * gm);
+    kahan_add(sy_, cy_, ly * gm);
+    kahan_add(sz_, cz_, lz * gm);
+    queued += qlen;
+    qlen = 0;
+  };
+
+  uint32_t n = wave_min_u32(nxt);
+  while (n < P.node_count) {
+    if (COUNT && lane == 0) ++cnt[5];
+    const NodeLink lk = D.link[n];
+    const NodeGeo32 g = D.geo32[n];
+    const bool act = nxt == n;
+    if (!__ballot(act)) {
+      if (lane == 0) atomicOr(&counters[kWalkCounters], 1ull);
+      break;
+    }
+    if (COUNT) cnt[0] += act;
+    const float h = g.half * inv_len;
+    const float ax_ = fabsf(fx_delta(g.cx, px, s0)), ay_ = fabsf(fx_delta(g.cy, py, s1)),
+                az_ = fabsf(fx_delta(g.cz, pz, s2));
+    const float ex = fmaxf(0.0f, ax_ - h), ey = fmaxf(0.0f, ay_ - h), ez = fmaxf(0.0f, az_ - h);
+    const bool keep = act && !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
+    if (COUNT) cnt[1] += act && !keep;
+    nxt = (act && !keep) ? n + lk.skip : nxt;
+    if (lk.child_count == 0) {
+      if (COUNT) cnt[4] += keep;
+      nxt = keep ? n + 1 : nxt;
+      if (__ballot(keep)) {
+        const uint32_t e0 = lk.begin + lk.count;
+        for (uint32_t c0 = lk.begin; c0 < e0; c0 += kWave) {
+          const uint32_t cn = e0 - c0 < static_cast<uint32_t>(kWave) ? e0 - c0 : static_cast<uint32_t>(kWave);
+          if (COUNT && lane == 0) cnt[6] += cn;
+          stage[w][lane] = D.src32[c0 + (static_cast<uint32_t>(lane) < cn ? lane : 0)];
+          __builtin_amdgcn_wave_barrier();
+          for (uint32_t sb = 0; sb < cn; sb += kChunk) {
+            const uint32_t se = sb + kChunk < cn ? sb + kChunk : cn;
+#pragma unroll 4
+            for (uint32_t si = sb; si < se; ++si) {
+              const uint4 sj = stage[w][si];
+              const float dx = fx_delta(sj.x, px, s0), dy = fx_delta(sj.y, py, s1), dz = fx_delta(sj.z, pz, s2);
+              const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+              const bool pair = keep && (c0 + si) != tp && r2 <= rc2;
+              if (COUNT) cnt[2] += pair && r2 == 0.0f;  // coincident pair: counted, zero vector
+              queue[w][qlen][lane] = c0 + si;           // slot qlen is free; kept only if appended
+              qlen += (pair && r2 != 0.0f) ? 1u : 0u;
+            }
+            if (wave_max_u32(qlen) > static_cast<uint32_t>(kQueue - kChunk)) drain();
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+    } else if (__ballot(keep)) {
+      bool take_mp = false;
+      float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;
+      if (keep) {
         const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
         const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
-        const bool contained = mx * mx + my * my + mz * mz <= contain2;
-        const NodeMom32 M = mom[n];
-        const float dx = fx_delta(M.comx, px, s0), dy = fx_delta(M.comy, py, s1), dz = fx_delta(M.comz, pz, s2);
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const NodeQuad32 Q = quad[n];
-        const float r = sqrtf(fmaxf(r2, 1.0e-30f));
-        const bool env_ok = P.quad && (2.0f * h / r) < 0.08f;
-        bool mac = false;
-        if (!inside) {
-          const float rr = sqrtf(r2 + 1.0e-30f);
-          mac = P.criterion == 0 ? (2.0f * h / rr) < theta : ((2.0f * h + Q.offset) / rr) < theta;
-        }
-        if (mac && env_ok && contained && !inside) {
-          ++cnt[3];
-          float mxa = 0.0f, mya = 0.0f, mza = 0.0f;
-          node_accel32(M, Q, dx, dy, dz, P, mxa, mya, mza);
-          kahan_add(sx_, cx_, mxa);
-          kahan_add(sy_, cy_, mya);
-          kahan_add(sz_, cz_, mza);
-          nxt = n + lk.skip;
-        } else {
-          nxt = n + 1;
+        const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= contain2;
+        const NodeMom32 M = D.mom32[n];
+        mfx = fx_delta(M.comx, px, s0) * to_len;
+        mfy = fx_delta(M.comy, py, s1) * to_len;
+        mfz = fx_delta(M.comz, pz, s2) * to_len;
+        const float r2 = mfx * mfx + mfy * mfy + mfz * mfz;
+        const float w2 = 2.0f * g.half;
+        // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
+        const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
+        const float lead = P.criterion == 0 ? w2 : w2 + D.quad32[n].offset;
+        const bool mac = !inside && lead * lead < theta2 * (r2 + 1.0e-30f);
+        take_mp = mac && env_ok && contained && !inside;
+        nxt = take_mp ? n + lk.skip : n + 1;
+      }
+      if (__ballot(take_mp)) {
+        drain();
+        if (take_mp) {
+          if (COUNT) ++cnt[3];
+          float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+          node_accel32(D.mom32[n], D.quad32[n], mfx, mfy, mfz, P, cx, cy, cz);
+          kahan_add(sx_, cx_, cx);
+          kahan_add(sy_, cy_, cy);
+          kahan_add(sz_, cz_, cz);
         }
       }
     }
-    if (__ballot(do_leaf)) {
-      const uint32_t b0 = lk.begin, e0 = lk.begin + lk.count;
-      float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-      for (uint32_t j = b0; j < e0; ++j) {
-        const uint32_t qx = qxs[j], qy = qys[j], qz = qzs[j];
-        const float mj = sm ? sm[j] : P.uniform_mass_f;
-        const float dx = fx_delta(qx, px, s0), dy = fx_delta(qy, py, s1), dz = fx_delta(qz, pz, s2);
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        if (do_leaf && j != tp && r2 <= rcut2) {
-          ++cnt[2];
-          if (r2 > 0.0f) {
-            const float rinv = rsqrtf(r2 + eps2);
-            const float soft = rinv * rinv * rinv;
-            const float hcoef = gaussian_h(r2 * P.c4, P);
-            const float f = fmaf(-hcoef, P.c8, soft) * mj;
-            lx = fmaf(f, dx, lx);
-            ly = fmaf(f, dy, ly);
-            lz = fmaf(f, dz, lz);
-          }
-        }
-      }
-      if (do_leaf) {
-        kahan_add(sx_, cx_, lx * gm);
-        kahan_add(sy_, cy_, ly * gm);
-        kahan_add(sz_, cz_, lz * gm);
-      }
-    }
+    n = next_node(n, nxt);
   }
-  flush_counters(counters, cnt);
+  drain();
+  if (COUNT) {
+    cnt[2] += queued;
+    flush_counters(counters, cnt);
+  }
   if (!valid) return;
   const float ax = sx_ - cx_, ay = sy_ - cy_, az = sz_ - cz_;
   static_cast<float*>(out.ax)[slot] = gx + ax;
@@ -582,75 +711,107 @@ double gaussian_h_exact(double t) {
   return (std::erf(q) - 2.0 * kInvSqrtPi * q * std::exp(-t)) / (t * q);
 }
 
-}  // namespace
-
-void fit_gaussian_coefficient(WalkParams& p, double t_max) {
-  const double mid = 0.5 * t_max, half = 0.5 * t_max;
-  for (int deg = 6; deg <= kMaxPolyDegree; ++deg) {
-    const int m = deg + 1;
-    // Chebyshev interpolation at Chebyshev nodes, then conversion to monomials in s.
-    std::vector<double> f(m), c(m, 0.0);
-    for (int j = 0; j < m; ++j) {
-      const double s = std::cos(M_PI * (j + 0.5) / m);
-      f[j] = gaussian_h_exact(mid + half * s);
-    }
-    for (int k = 0; k < m; ++k) {
-      double acc = 0.0;
-      for (int j = 0; j < m; ++j) acc += f[j] * std::cos(M_PI * k * (j + 0.5) / m);
-      c[k] = (k == 0 ? 1.0 : 2.0) * acc / m;
-    }
-    // T_k(s) in monomials: T0=1, T1=s, T_{k+1}=2 s T_k - T_{k-1}
-    std::vector<std::vector<double>> T(m, std::vector<double>(m, 0.0));
-    T[0][0] = 1.0;
-    if (m > 1) T[1][1] = 1.0;
-    for (int k = 1; k + 1 < m; ++k)
-      for (int i = 0; i < m; ++i) {
-        double v = -T[k - 1][i];
-        if (i > 0) v += 2.0 * T[k][i - 1];
-        T[k + 1][i] = v;
-      }
-    std::vector<double> mono(m, 0.0);
-    for (int k = 0; k < m; ++k)
-      for (int i = 0; i < m; ++i) mono[i] += c[k] * T[k][i];
-    double worst = 0.0;
-    for (int j = 0; j <= 4000; ++j) {
-      const double t = t_max * j / 4000.0;
-      const double s = (t - mid) / half;
-      double h = mono[deg];
-      for (int i = deg - 1; i >= 0; --i) h = h * s + mono[i];
-      const double e = gaussian_h_exact(t);
-      worst = std::max(worst, std::abs(h - e) / std::abs(e));
-    }
-    if (worst < 2.0e-9 || deg == kMaxPolyDegree) {
-      p.poly_degree = deg;
-      for (int i = 0; i <= deg; ++i) p.poly[i] = static_cast<float>(mono[i]);
-      p.t_mid = static_cast<float>(mid);
-      p.t_inv_half = static_cast<float>(1.0 / half);
-      return;
-    }
+// dH/dt: derivative of the series below t = 1/64, of the closed form above
+// (the same split as tree_pm_split_kernel.hpp:248-285).
+double gaussian_h_slope(double t) {
+  constexpr double c = 2.25675833419102514779231780624309034;
+  if (t < 0.015625) {
+    const double coef[8] = {1.0 / 3.0, -1.0 / 5.0, 1.0 / 14.0, -1.0 / 54.0,
+                            1.0 / 264.0, -1.0 / 1560.0, 1.0 / 10800.0, -1.0 / 85680.0};
+    double d = 0.0;
+    for (int i = 7; i >= 1; --i) d = d * t + i * coef[i];
+    return c * d;
   }
+  return (0.5 * c * std::exp(-t) - 1.5 * gaussian_h_exact(t)) / t;
 }
 
-void walk_fp64(const Tree& t, const WalkParams& p, const double* sx, const double* sy, const double* sz,
-               const double* sm, const uint32_t* target_pos, const uint32_t* target_slot, const double* fx,
-               const double* fy, const double* fz, const WalkOutputs& out, unsigned long long* counters,
-               hipStream_t s) {
-  if (p.n_targets == 0) return;
-  walk64_kernel<<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
-      p, t.link.as<NodeLink>(), t.geo64.as<NodeGeo64>(), t.mom64.as<NodeMom64>(), t.quad64.as<NodeQuad64>(),
-      t.order.as<uint32_t>(), sx, sy, sz, sm, target_pos, target_slot, fx, fy, fz, out, counters);
+template <int SCHEME>
+void launch64(const WalkParams& p, const WalkData& d, const WalkOutputs& out, unsigned long long* counters,
+              hipStream_t s) {
+  walk64_kernel<SCHEME><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(p, d, out, counters);
   CHUI_LAUNCH_CHECK("walk64_kernel");
 }
-
-void walk_mixed(const Tree& t, const WalkParams& p, const uint32_t* qx, const uint32_t* qy, const uint32_t* qz,
-                const float* sm, const uint32_t* target_pos, const uint32_t* target_slot, const float* fx,
-                const float* fy, const float* fz, const WalkOutputs& out, unsigned long long* counters,
-                hipStream_t s) {
-  if (p.n_targets == 0) return;
-  walkmix_kernel<<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
-      p, t.link.as<NodeLink>(), t.geo32.as<NodeGeo32>(), t.mom32.as<NodeMom32>(), t.quad32.as<NodeQuad32>(),
-      t.order.as<uint32_t>(), qx, qy, qz, sm, target_pos, target_slot, fx, fy, fz, out, counters);
+template <int SCHEME, bool CUBIC>
+void launchmix(const WalkParams& p, const WalkData& d, const WalkOutputs& out, unsigned long long* counters,
+               hipStream_t s) {
+  if (p.count)
+    walkmix_kernel<SCHEME, CUBIC, true><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(p, d, out,
+                                                                                                     counters);
+  else
+    walkmix_kernel<SCHEME, CUBIC, false><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(p, d, out,
+                                                                                                      counters);
   CHUI_LAUNCH_CHECK("walkmix_kernel");
+}
+
+}  // namespace
+
+std::vector<float> gaussian_coefficient_table(WalkParams& p, double t_max) {
+  // Cubic Hermite cells on [0, t_max]: value c0 + c1 u + c2 u^2 + c3 u^3 with u
+  // the position inside the cell; interpolation error <= h^4/384 max|d4H/dt4|,
+  // about 1e-10 for 256 cells on t <= 6.
+  const int K = kHTableCells;
+  const double h = t_max / K;
+  std::vector<float> tab(4 * K);
+  for (int i = 0; i < K; ++i) {
+    const double t0 = i * h, t1 = (i + 1) * h;
+    const double f0 = gaussian_h_exact(t0), f1 = gaussian_h_exact(t1);
+    const double d0 = h * gaussian_h_slope(t0), d1 = h * gaussian_h_slope(t1);
+    tab[4 * i + 0] = static_cast<float>(f0);
+    tab[4 * i + 1] = static_cast<float>(d0);
+    tab[4 * i + 2] = static_cast<float>(3.0 * (f1 - f0) - 2.0 * d0 - d1);
+    tab[4 * i + 3] = static_cast<float>(2.0 * (f0 - f1) + d0 + d1);
+  }
+  p.t_to_cell = static_cast<float>(static_cast<double>(p.c4) / h);
+  return tab;
+}
+
+void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
+               const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
+               const double* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s) {
+  if (p.n_targets == 0) return;
+  WalkData d{};
+  d.link = t.link.as<NodeLink>();
+  d.geo64 = t.geo64.as<NodeGeo64>();
+  d.mom64 = t.mom64.as<NodeMom64>();
+  d.quad64 = t.quad64.as<NodeQuad64>();
+  d.order = t.order.as<uint32_t>();
+  d.src64 = src;
+  d.uniform_mass = uniform_mass;
+  d.tpos = target_pos;
+  d.tslot = target_slot;
+  d.f64x = fx;
+  d.f64y = fy;
+  d.f64z = fz;
+  if (p.assignment == 0) launch64<0>(p, d, out, counters, s);
+  else launch64<1>(p, d, out, counters, s);
+}
+
+void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int uniform_mass, const float4* htab,
+                const uint32_t* target_pos, const uint32_t* target_slot, const float* fx, const float* fy,
+                const float* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s) {
+  if (p.n_targets == 0) return;
+  WalkData d{};
+  d.link = t.link.as<NodeLink>();
+  d.geo32 = t.geo32.as<NodeGeo32>();
+  d.mom32 = t.mom32.as<NodeMom32>();
+  d.quad32 = t.quad32.as<NodeQuad32>();
+  d.order = t.order.as<uint32_t>();
+  d.src32 = src;
+  d.htab = htab;
+  d.uniform_mass = uniform_mass;
+  d.tpos = target_pos;
+  d.tslot = target_slot;
+  d.f32x = fx;
+  d.f32y = fy;
+  d.f32z = fz;
+  const bool cic = p.assignment == 0;
+  if (p.cubic) {
+    if (cic) launchmix<0, true>(p, d, out, counters, s);
+    else launchmix<1, true>(p, d, out, counters, s);
+  } else {
+    if (cic) launchmix<0, false>(p, d, out, counters, s);
+    else launchmix<1, false>(p, d, out, counters, s);
+  }
 }
 
 }  // namespace chui

@@ -2,12 +2,14 @@
 // tree_pm_coupling.cpp:4026-4272) with the PM interpolation fused in front.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 #include "tree.hpp"
 
 namespace chui {
 
-constexpr int kMaxPolyDegree = 24;
+constexpr int kHTableCells = 256;  // cubic cells of the Gaussian coefficient H(t)
 
 struct WalkParams {
   double L[3];
@@ -20,6 +22,7 @@ struct WalkParams {
   int quad;         // multipole order 2
   uint32_t node_count;
   uint32_t n_targets;
+  int count;             // accumulate walk counters (diagnostic launches)
   int identity_targets;  // targets are tree positions 0..n-1, output slot = order[k]
   float uniform_mass_f;  // mixed: mass when uniform
   double uniform_mass;   // used when the mass lane is null
@@ -29,13 +32,14 @@ struct WalkParams {
   int mesh[3];
   int mesh_log2[3];
   std::size_t mesh_row;   // padded real row stride (elements)
-  // Mixed precision: fixed-point scale and the H(t) polynomial (Horner in
-  // s = (t - t_mid) * t_inv_half, ascending coefficients).
+  // Mixed precision: fixed-point scale, cutoff radii in separation units
+  // (fixed-point units when the box is cubic, lengths otherwise) and the
+  // cubic-cell table of H(t) = L(sqrt t)/t^{3/2}, t = r^2/(4 a^2).
+  int cubic;
   float fx_to_len[3];  // L / 2^32
+  float rcut2_sep, prune2_sep, contain2_sep;
   float c4, c8;        // 1/(4 a^2), 1/(8 a^3)
-  float t_mid, t_inv_half;
-  int poly_degree;
-  float poly[kMaxPolyDegree + 1];
+  float t_to_cell;     // r^2 -> table cell coordinate
 };
 
 struct WalkOutputs {
@@ -50,20 +54,21 @@ struct WalkOutputs {
   void* sr_z;
 };
 
-// Counters accumulated on the device: visited, pruned, pairs, multipoles, leaves.
-constexpr int kWalkCounters = 5;
+// Counters accumulated on the device: visited, pruned, pairs, multipoles,
+// leaves (per target), wave node visits and wave leaf-source slots (per wave).
+constexpr int kWalkCounters = 7;
 
-void walk_fp64(const Tree& t, const WalkParams& p, const double* sx, const double* sy, const double* sz,
-               const double* sm, const uint32_t* target_pos, const uint32_t* target_slot, const double* fx,
-               const double* fy, const double* fz, const WalkOutputs& out, unsigned long long* counters,
-               hipStream_t s);
+// Tree-ordered sources: FP64 (x, y, z, m); mixed (qx, qy, qz, float bits of m).
+void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
+               const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
+               const double* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s);
 
-void walk_mixed(const Tree& t, const WalkParams& p, const uint32_t* qx, const uint32_t* qy, const uint32_t* qz,
-                const float* sm, const uint32_t* target_pos, const uint32_t* target_slot, const float* fx,
-                const float* fy, const float* fz, const WalkOutputs& out, unsigned long long* counters,
-                hipStream_t s);
+void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int uniform_mass, const float4* htab,
+                const uint32_t* target_pos, const uint32_t* target_slot, const float* fx, const float* fy,
+                const float* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s);
 
-// Host FP64 fit of H(t) = L(sqrt t)/t^{3/2} on [0, t_max]; fills p.poly/degree.
-void fit_gaussian_coefficient(WalkParams& p, double t_max);
+// Host FP64 table of H(t) on [0, t_max] (kHTableCells cubic cells, 4 floats
+// each); sets p.t_to_cell.
+std::vector<float> gaussian_coefficient_table(WalkParams& p, double t_max);
 
 }  // namespace chui

@@ -138,6 +138,10 @@ typedef struct chui_treepm_diagnostics {
   double ms_pm_fft_poisson;
   double ms_pm_interpolate;
   double ms_walk;
+  /* wave-level work of the walk: node records processed per wave (union of
+   * the 64 lanes' walks) and leaf source slots streamed per wave */
+  uint64_t wave_node_visits;
+  uint64_t wave_leaf_source_slots;
 } chui_treepm_diagnostics;
 
 typedef struct chui_treepm_context chui_treepm_context;

@@ -34,7 +34,7 @@ def clustered(n, box, seed, n_halos=5, frac=0.5, scale=0.02):
     return pos[0].copy(), pos[1].copy(), pos[2].copy()
 
 
-def gpu_solve(w_x, w_y, w_z, m, opts, mesh, precision=0, active=None, capture=True):
+def gpu_solve(w_x, w_y, w_z, m, opts, mesh, precision=0, active=None, capture=True, diagnostics=True):
     dt = torch.float64 if precision == 0 else torch.float32
     x, y, z, mm = to_dev(w_x, w_y, w_z, m)
     na = len(w_x) if active is None else len(active)
@@ -46,7 +46,7 @@ def gpu_solve(w_x, w_y, w_z, m, opts, mesh, precision=0, active=None, capture=Tr
     with TreePmCoordinator(mesh) as coord:
         res = coord.solve_active_set(x, y, z, mm, ForceAccumulator(tuple(acc), act, tuple(pm) if pm else None,
                                                                  tuple(sr) if sr else None), opts,
-                                     want_diagnostics=True)
+                                     want_diagnostics=diagnostics)
         torch.cuda.synchronize()
         tree = coord.tree_arrays()
     host = lambda ts: [t.double().cpu().numpy() for t in ts] if ts else None  # noqa: E731
@@ -180,6 +180,19 @@ def test_active_subset_equals_full_rows():
     for a in range(3):
         assert np.array_equal(sub_sr[a], full_sr[a][active])
     assert rel_l2(sub, [f[active] for f in full]) < 1e-13
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_walk_without_diagnostics_matches_counted_walk(precision):
+    # Diagnostic solves run a walk variant that also counts; the plain variant
+    # must produce the same short-range accelerations bit for bit.
+    w = ics.config(1, n_side=32)
+    opts = options_struct(w.box, eps=w.softening)
+    _, _, sr_d, diag, _ = gpu_solve(w.x, w.y, w.z, w.mass, opts, w.mesh, precision=precision)
+    _, _, sr_p, none, _ = gpu_solve(w.x, w.y, w.z, w.mass, opts, w.mesh, precision=precision, diagnostics=False)
+    assert diag and not none
+    for a in range(3):
+        assert np.array_equal(sr_d[a], sr_p[a])
 
 
 @pytest.mark.parametrize("case", ["one", "two", "coincident16", "noncanonical", "rectangular", "nonuniform_mass"])
