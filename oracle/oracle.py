@@ -161,3 +161,46 @@ def drift(x, y, z, vx, vy, vz, drift_factor, box):
     L = (C.c_double * 3)(*box)
     load().oracle_drift(_p(x), _p(y), _p(z), _p(vx), _p(vy), _p(vz), C.c_uint64(x.size),
                         C.c_double(drift_factor), L)
+
+
+def pm_mesh(density, options, mesh, split_scale):
+    """Mesh-level solve of a given density (shape mesh, row-major) -> (fx, fy, fz) meshes."""
+    density = np.ascontiguousarray(density, dtype=np.float64)
+    msh = _mesh(mesh)
+    out = [np.empty(density.shape) for _ in range(3)]
+    _check(load().oracle_pm_mesh(_p(density), C.byref(options), msh, C.c_double(split_scale),
+                                 _p(out[0]), _p(out[1]), _p(out[2])))
+    return out
+
+
+def pm_interpolate(fields, x, y, z, options, mesh):
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    f = [np.ascontiguousarray(a, dtype=np.float64) for a in fields]
+    a = [np.empty(x.size) for _ in range(3)]
+    _check(load().oracle_pm_interpolate(_p(f[0]), _p(f[1]), _p(f[2]), _mesh(mesh), _p(x), _p(y), _p(z),
+                                        C.c_uint64(x.size), C.byref(options), _p(a[0]), _p(a[1]), _p(a[2])))
+    return a
+
+
+def long_range_inv_r3(r, a):
+    r = _f64(r)
+    out = np.empty_like(r)
+    _check(load().oracle_long_range_inv_r3(_p(r), C.c_uint64(r.size), C.c_double(a), _p(out)))
+    return out
+
+
+def residual_inv_r3(r2, eps, a):
+    r2 = _f64(r2)
+    out = np.empty_like(r2)
+    _check(load().oracle_residual_inv_r3(_p(r2), C.c_uint64(r2.size), C.c_double(eps), C.c_double(a), _p(out)))
+    return out
+
+
+def cluster_node_accel(x, y, z, m, target, G, eps, a, quad=True):
+    """Residual acceleration of a source cluster evaluated as one accepted node."""
+    x, y, z, m = _f64(x), _f64(y), _f64(z), _f64(m)
+    t = (C.c_double * 3)(*target)
+    out = (C.c_double * 3)()
+    _check(load().oracle_cluster_node_accel(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), t, C.c_double(G),
+                                            C.c_double(eps), C.c_double(a), C.c_int(1 if quad else 0), out))
+    return np.array(out[:])

@@ -590,6 +590,9 @@ struct PmField {
   std::vector<double> fx, fy, fz;  // real-space force meshes, row-major (ix*ny+iy)*nz+iz
 };
 
+void pm_mesh_solve(const std::vector<double>& rho, const chui_treepm_options& o, const int mesh[3],
+                   double split_scale, PmField& F);
+
 void pm_solve(const double* x, const double* y, const double* z, const double* m, size_t n,
               const chui_treepm_options& o, const int mesh[3], double split_scale, PmField& F,
               std::vector<double>* density_out = nullptr) {
@@ -616,9 +619,20 @@ void pm_solve(const double* x, const double* y, const double* z, const double* m
     }
   }
   const double cell_volume = (Lx * Ly * Lz) / static_cast<double>(M);
-  double sum = 0.0;
-  for (size_t i = 0; i < M; ++i) { rho[i] /= cell_volume; sum += rho[i]; }
+  for (size_t i = 0; i < M; ++i) rho[i] /= cell_volume;
   if (density_out) *density_out = rho;
+  pm_mesh_solve(rho, o, mesh, split_scale, F);
+}
+
+// Mesh part of the solve: density (row-major (ix*ny+iy)*nz+iz) -> three real
+// force meshes.
+void pm_mesh_solve(const std::vector<double>& rho, const chui_treepm_options& o, const int mesh[3],
+                   double split_scale, PmField& F) {
+  const size_t nx = mesh[0], ny = mesh[1], nz = mesh[2];
+  const size_t M = nx * ny * nz;
+  const double Lx = o.box_size[0], Ly = o.box_size[1], Lz = o.box_size[2];
+  double sum = 0.0;
+  for (size_t i = 0; i < M; ++i) sum += rho[i];
   const double mean = sum / static_cast<double>(M);
   std::vector<std::complex<double>> spec(M);
   for (size_t i = 0; i < M; ++i) spec[i] = rho[i] - mean;
@@ -941,6 +955,60 @@ ORACLE_API int oracle_pm_accel(const double* x, const double* y, const double* z
     pm_solve(x, y, z, m, n, *o, msh, a, F, density_out ? &rho : nullptr);
     if (density_out) std::copy(rho.begin(), rho.end(), density_out);
     pm_interpolate(F, x, y, z, n, *o, ax, ay, az);
+  })
+}
+
+// Mesh-level long-range solve of a given density (M doubles, row-major) with
+// split scale a (0 = unfiltered Newtonian PM); writes three M-double meshes.
+ORACLE_API int oracle_pm_mesh(const double* density, const chui_treepm_options* o, const int32_t mesh[3],
+                              double split_scale, double* fx, double* fy, double* fz) {
+  ORACLE_TRY({
+    const int msh[3] = {mesh[0], mesh[1], mesh[2]};
+    const size_t M = static_cast<size_t>(msh[0]) * msh[1] * msh[2];
+    std::vector<double> rho(density, density + M);
+    PmField F;
+    pm_mesh_solve(rho, *o, msh, split_scale, F);
+    std::copy(F.fx.begin(), F.fx.end(), fx);
+    std::copy(F.fy.begin(), F.fy.end(), fy);
+    std::copy(F.fz.begin(), F.fz.end(), fz);
+  })
+}
+
+// Interpolation of three given meshes at particle positions (assignment from o).
+ORACLE_API int oracle_pm_interpolate(const double* fx, const double* fy, const double* fz, const int32_t mesh[3],
+                                     const double* x, const double* y, const double* z, uint64_t n,
+                                     const chui_treepm_options* o, double* ax, double* ay, double* az) {
+  ORACLE_TRY({
+    const size_t M = static_cast<size_t>(mesh[0]) * mesh[1] * mesh[2];
+    PmField F;
+    F.nx = mesh[0]; F.ny = mesh[1]; F.nz = mesh[2];
+    F.fx.assign(fx, fx + M); F.fy.assign(fy, fy + M); F.fz.assign(fz, fz + M);
+    pm_interpolate(F, x, y, z, n, *o, ax, ay, az);
+  })
+}
+
+// Split-kernel coefficients: L(r)/r^3 and the softened residual at r^2.
+ORACLE_API int oracle_long_range_inv_r3(const double* r, uint64_t n, double a, double* out) {
+  ORACLE_TRY(for (size_t i = 0; i < n; ++i) out[i] = long_range_inv_r3(r[i], a))
+}
+ORACLE_API int oracle_residual_inv_r3(const double* r2, uint64_t n, double eps, double a, double* out) {
+  ORACLE_TRY(for (size_t i = 0; i < n; ++i) out[i] = residual_inv_r3(r2[i], eps, a))
+}
+
+// Residual acceleration at target t of n sources taken as ONE accepted node
+// (monopole, plus quadrupole when quad != 0), non-periodic: the accepted-node
+// kernel on its own.
+ORACLE_API int oracle_cluster_node_accel(const double* x, const double* y, const double* z, const double* m,
+                                         uint64_t n, const double t[3], double G, double eps, double a, int quad,
+                                         double out[3]) {
+  ORACLE_TRY({
+    if (n == 0) throw std::invalid_argument("empty cluster");
+    Tree tr;
+    build_tree(x, y, z, m, n, n, quad != 0, eps, tr);
+    WalkParams P{};
+    P.G = G; P.a = a; P.eps = eps; P.quad = quad != 0;
+    const auto c = node_accel(tr, 0, tr.comx[0] - t[0], tr.comy[0] - t[1], tr.comz[0] - t[2], P);
+    out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
   })
 }
 
