@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--ewald-samples", type=int, default=2048)
     ap.add_argument("--cpu-n-side", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="one plain device-to-device copy of this size before the timed region; its HBM counters "
+                         "scale the walk's FETCH_SIZE/WRITE_SIZE readings under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
@@ -99,13 +102,30 @@ def main():
         log(f"precision={precision}: {steps} steps in {elapsed:.3f}s, stages {prof}")
         return elapsed, prof, diag, np.stack([a.double().cpu().numpy() for a in acc])
 
+    if args.copy_calibration_mib > 0:
+        nbytes = args.copy_calibration_mib << 20
+        src_buf = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0").fill_(1)
+        dst_buf = torch.empty_like(src_buf)
+        torch.cuda.synchronize()
+        dst_buf.copy_(src_buf)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nbytes} bytes read and written once")
+        del src_buf, dst_buf
+
     t_mixed, prof_mixed, diag, acc_mixed = run(L.PRECISION_MIXED, args.steps, args.warmup)
-    t_fp64, prof_fp64, diag64, acc_fp64 = run(L.PRECISION_FP64, args.fp64_steps, 1)
     value = n * args.steps / t_mixed
-    value_fp64 = n * args.fp64_steps / t_fp64
-    err_mix_vs_fp64 = error_metrics(acc_mixed, acc_fp64)
-    log(f"mixed {value:.4g} p/s, fp64 {value_fp64:.4g} p/s, ratio {value / value_fp64:.3f}, "
-        f"mixed-vs-fp64 {err_mix_vs_fp64}")
+    fp64 = None
+    err_mix_vs_fp64 = None
+    acc_fp64 = None
+    if args.fp64_steps > 0:
+        t_fp64, prof_fp64, diag64, acc_fp64 = run(L.PRECISION_FP64, args.fp64_steps, 1)
+        value_fp64 = n * args.fp64_steps / t_fp64
+        err_mix_vs_fp64 = error_metrics(acc_mixed, acc_fp64)
+        fp64 = {"value": value_fp64, "ms_per_step": 1e3 * t_fp64 / args.fp64_steps,
+                "mixed_over_fp64": value / value_fp64,
+                "stages_ms": {k: v / max(prof_fp64["solves"], 1) for k, v in prof_fp64.items() if k.endswith("_ms")}}
+        log(f"mixed {value:.4g} p/s, fp64 {value_fp64:.4g} p/s, ratio {value / value_fp64:.3f}, "
+            f"mixed-vs-fp64 {err_mix_vs_fp64}")
 
     # FP64 Ewald on sample targets (periodic_ewald_reference.cpp semantics).
     ewald = None
@@ -119,7 +139,7 @@ def main():
         coord.ewald(*dev, dt_, (w.box,) * 3, ics.G_CODE, alpha, 0, 24, w.softening, *out)
         ref = np.stack([o.cpu().numpy() for o in out])
         ewald = {"samples": int(tgt.size), "alpha_box": 11.8, "modes": 24, "seconds": round(time.time() - te, 2),
-                 "fp64_vs_ewald": error_metrics(acc_fp64[:, tgt], ref),
+                 "fp64_vs_ewald": error_metrics(acc_fp64[:, tgt], ref) if acc_fp64 is not None else None,
                  "mixed_vs_ewald": error_metrics(acc_mixed[:, tgt], ref)}
         log(f"ewald: {ewald}")
 
@@ -166,14 +186,12 @@ def main():
                    "halo_fraction": w.halo_fraction, "halos": w.halo_count, "theta": 0.5, "leaf": 16,
                    "asmth_cells": 1.25, "rcut_over_rs": 4.5, "softening": w.softening,
                    "parallelism": "single GPU"},
-        "fp64_gpu": {"value": value_fp64, "ms_per_step": 1e3 * t_fp64 / args.fp64_steps,
-                     "mixed_over_fp64": value / value_fp64},
+        "fp64_gpu": fp64,
         "force_error": {"mixed_vs_fp64": err_mix_vs_fp64, "ewald": ewald},
         "grad_fwd_ratio": None,
         "stages_ms_mixed": {k: v / max(prof_mixed["solves"], 1) for k, v in prof_mixed.items() if k.endswith("_ms")},
-        "stages_ms_fp64": {k: v / max(prof_fp64["solves"], 1) for k, v in prof_fp64.items() if k.endswith("_ms")},
         "roofline": {"bound": "hbm", "kernel": "walkmix_kernel", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(w.name),
                      "algorithmic_bytes_per_launch": walk_bytes, "launch_ms": walk_ms,
                      "valu": {"pairs": pairs, "flop_per_pair": FLOP_PER_PAIR,
                               "achieved_tflops": pairs * FLOP_PER_PAIR / (walk_ms * 1e-3) / 1e12,
@@ -187,6 +205,25 @@ def main():
     }
     coord.close()
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per walk launch from the committed rocprofv3 --pmc summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = ROOT / "profiles" / "walkmix_kernel_treepm256_traffic.txt"
+    if not path.exists():
+        return None
+    fields = {}
+    for line in path.read_text().splitlines():
+        if ":" in line and not line.startswith(" "):
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    if fields.get("workload") != workload:
+        return None
+    try:
+        return float(fields["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def opts_fp64_for(L, w, ics):
