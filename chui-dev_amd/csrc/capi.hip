@@ -403,6 +403,14 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     wp.prune2_sep = static_cast<float>(r2 * (1.0 + 1.0e-5));
     wp.contain2_sep = static_cast<float>(r2 * (1.0 - 1.0e-5));
   }
+  for (int a = 0; a < 3; ++a) {
+    // float offsets from a wave origin: targets within L/16 (precision) and
+    // within L/2 - r_cut (no wrap of the origin's frame inside a pair), with a
+    // margin of 2^24 units for the r_cut conversion
+    const double rcut_units = sp.rcut / o->box_size[a] * 4294967296.0;
+    const double lim = std::min(268435456.0, 2147483648.0 - rcut_units - 16777216.0);
+    wp.fast_half_extent[a] = lim > 0.0 ? static_cast<uint32_t>(lim) : 0u;
+  }
   chui::WalkOutputs out{ax, ay, az, pmx, pmy, pmz, srx, sry, srz};
   if (n_active > 0 && n > 0) {
     if (!mixed) {

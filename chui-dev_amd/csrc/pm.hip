@@ -1,5 +1,5 @@
-// pm.hip — PM long-range solve: CIC/TSC deposit, hand-written batched radix-2
-// FFT passes through LDS, Green's function with the Gaussian split filter and
+// pm.hip — PM long-range solve: CIC/TSC deposit, hand-written batched
+// Stockham FFT passes (radix 8/4/2) through LDS, Green's function with the Gaussian split filter and
 // spectral gradient (pm_solver.cpp:925-951, :2398-2705, :4016-4061, :4581-4940).
 //
 // Layout: complex spectrum C[ix][iy][kz], kz in [0, nz/2] (row stride nzc);
@@ -20,6 +20,14 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846264338327950288;
 constexpr int kFftThreads = 256;
+constexpr int kFftMaxButterflies = 4;  // per thread and pass
+
+// Smallest radix of the pass plan of lds_fft for a line of 2^lg points.
+inline int fft_min_radix(int lg) {
+  const int n8 = (lg % 3 == 1 && lg >= 4) ? (lg - 4) / 3 : lg / 3;
+  const int tail = lg - 3 * n8;
+  return tail == 0 ? 8 : (tail == 1 ? 2 : 4);
+}
 
 template <class T> struct C2;
 template <> struct C2<float> { using type = float2; };
@@ -35,38 +43,130 @@ template <class V> __device__ __forceinline__ V cadd(V a, V b) { V r; r.x = a.x 
 template <class V> __device__ __forceinline__ V csub(V a, V b) { V r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
 template <class V, class T> __device__ __forceinline__ V mk(T x, T y) { V r; r.x = x; r.y = y; return r; }
 
-__device__ __forceinline__ int bitrev(int i, int lg) {
-  return lg == 0 ? 0 : static_cast<int>(__brev(static_cast<unsigned>(i)) >> (32 - lg));
+// ---- LDS line FFTs: Stockham autosort passes of radix 8, 4 and 2 ---------------
+// Pass with radix R after sub-transforms of length Ns: butterfly j reads
+// x[j + r n/R], multiplies by exp(-+2 pi i (j mod Ns) r / (Ns R)), applies the
+// length-R DFT and writes x[(j / Ns) Ns R + (j mod Ns) + r Ns]. Natural order
+// in and out (no bit reversal); log2(n) = 9 takes three passes of radix 8.
+// Each thread keeps its butterflies in registers across the barrier between
+// the read and the write phase, so one LDS buffer serves every pass.
+template <class V, bool INV> __device__ __forceinline__ V rot_mi(V a) {  // * (-i) forward, * (+i) inverse
+  V r;
+  if (INV) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; }
+  return r;
 }
 
-// In-place radix-2 DIT on nlines LDS lines (input already bit-reversed).
+template <class V, bool INV> __device__ __forceinline__ void dft4(V& a0, V& a1, V& a2, V& a3) {
+  const V t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = rot_mi<V, INV>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <class V, int R, bool INV> __device__ __forceinline__ void dft(V* v) {
+  if constexpr (R == 2) {
+    const V t = v[0];
+    v[0] = cadd(t, v[1]);
+    v[1] = csub(t, v[1]);
+  } else if constexpr (R == 4) {
+    dft4<V, INV>(v[0], v[1], v[2], v[3]);
+  } else {
+    V e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    V o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<V, INV>(e0, e1, e2, e3);
+    dft4<V, INV>(o0, o1, o2, o3);
+    using T = decltype(o1.x);
+    const T h = T(0.70710678118654752440084436210484904);
+    // o_k *= W8^k, W8 = exp(-+2 pi i / 8)
+    V w1, w3;
+    if (INV) {
+      w1.x = (o1.x - o1.y) * h; w1.y = (o1.x + o1.y) * h;
+      w3.x = -(o3.x + o3.y) * h; w3.y = (o3.x - o3.y) * h;
+    } else {
+      w1.x = (o1.x + o1.y) * h; w1.y = (o1.y - o1.x) * h;
+      w3.x = (o3.y - o3.x) * h; w3.y = -(o3.x + o3.y) * h;
+    }
+    const V w2 = rot_mi<V, INV>(o2);
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, w1); v[5] = csub(e1, w1);
+    v[2] = cadd(e2, w2); v[6] = csub(e2, w2);
+    v[3] = cadd(e3, w3); v[7] = csub(e3, w3);
+  }
+}
+
+// exp(-+2 pi i m / n) for 0 <= m < n from the half table tw[k] = exp(-2 pi i k / n).
+template <class V, bool INV> __device__ __forceinline__ V twiddle(const V* tw, int m, int halfn) {
+  V w = tw[m < halfn ? m : m - halfn];
+  if (m >= halfn) { w.x = -w.x; w.y = -w.y; }
+  if (INV) w.y = -w.y;
+  return w;
+}
+
+template <class T, int R, bool INV>
+__device__ __attribute__((noinline)) void stockham_pass(typename C2<T>::type* X, int nlines, int lgn, int stride,
+                                              const typename C2<T>::type* tw, int lgNs) {
+  using V = typename C2<T>::type;
+  constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+  constexpr int MAXB = kFftMaxButterflies;  // the host sizes blocks so lines * n / R <= MAXB * kFftThreads
+  const int lgq = lgn - LR;
+  const int q = 1 << lgq;
+  const int total = nlines << lgq;
+  const int Ns = 1 << lgNs;
+  const int halfn = 1 << (lgn - 1);
+  const int sh = lgn - lgNs - LR;
+  V v[MAXB][R];
+  int dst[MAXB];
+#pragma unroll
+  for (int u = 0; u < MAXB; ++u) {
+    const int b = threadIdx.x + u * kFftThreads;
+    dst[u] = -1;
+    if (b < total) {
+      const int line = b >> lgq, j = b & (q - 1);
+      const V* L = X + line * stride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[u][r] = L[j + r * q];
+      const int jm = j & (Ns - 1);
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[u][r] = cmul(v[u][r], twiddle<V, INV>(tw, (jm * r) << sh, halfn));
+      }
+      dft<V, R, INV>(v[u]);
+      dst[u] = line * stride + ((j >> lgNs) << (lgNs + LR)) + jm;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < MAXB; ++u) {
+    if (dst[u] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) X[dst[u] + r * Ns] = v[u][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, bool INV>
+__device__ void lds_fft_dir(typename C2<T>::type* X, int nlines, int lg, int stride, const typename C2<T>::type* tw) {
+  // plan: radix-8 passes, then a tail of 4+4, 4 or 2 (log2 tail 4, 2 or 1)
+  const int n8 = (lg % 3 == 1 && lg >= 4) ? (lg - 4) / 3 : lg / 3;
+  const int tail = lg - 3 * n8;
+#pragma nounroll
+  for (int i = 0; i < n8; ++i) stockham_pass<T, 8, INV>(X, nlines, lg, stride, tw, 3 * i);
+  const int done = 3 * n8;
+  if (tail >= 2) stockham_pass<T, 4, INV>(X, nlines, lg, stride, tw, done);
+  if (tail == 4) stockham_pass<T, 4, INV>(X, nlines, lg, stride, tw, done + 2);
+  if (tail == 1) stockham_pass<T, 2, INV>(X, nlines, lg, stride, tw, done);
+}
+
+// In-place FFT of nlines LDS lines (natural order in and out); blockDim must
+// be kFftThreads and nlines * n <= 8192.
 template <class T>
 __device__ void lds_fft(typename C2<T>::type* X, int nlines, int n, int lg, int stride,
                         const typename C2<T>::type* tw, bool inverse) {
-  using V = typename C2<T>::type;
-  if (lg == 0) return;
-  const int halfn = n >> 1;
-  const int nb = nlines * halfn;
-  for (int s = 1; s <= lg; ++s) {
-    const int half = 1 << (s - 1);
-    const int tstep = n >> s;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-      const int line = b / halfn;
-      const int q = b - line * halfn;
-      const int pos = q & (half - 1);
-      const int grp = q >> (s - 1);
-      const int i0 = grp * 2 * half + pos;
-      const int i1 = i0 + half;
-      V w = tw[pos * tstep];
-      if (inverse) w.y = -w.y;
-      V* Lp = X + line * stride;
-      const V u = Lp[i0];
-      const V vw = cmul(Lp[i1], w);
-      Lp[i0] = cadd(u, vw);
-      Lp[i1] = csub(u, vw);
-    }
-    __syncthreads();
-  }
+  (void)n;
+  if (inverse) lds_fft_dir<T, true>(X, nlines, lg, stride, tw);
+  else lds_fft_dir<T, false>(X, nlines, lg, stride, tw);
 }
 
 // ---- deposit ------------------------------------------------------------------
@@ -193,7 +293,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
       v.x = ra[i];
       if (2 * pr + 1 < rows) v.y = ra[row + i];
     }
-    X[pp * stride + bitrev(i, lg)] = v;
+    X[pp * stride + i] = v;
   }
   __syncthreads();
   lds_fft<T>(X, pairs_per_block, nz, lg, stride, tw, false);
@@ -242,10 +342,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::si
       xb = mk<V, T>(xb.y * kk, -xb.x * kk);
     }
     if (k == 0 || k == nz / 2) {
-      X[pp * stride + bitrev(k & (nz - 1), lg)] = mk<V, T>(xa.x, xb.x);
+      X[pp * stride + (k & (nz - 1))] = mk<V, T>(xa.x, xb.x);
     } else {
-      X[pp * stride + bitrev(k, lg)] = mk<V, T>(xa.x - xb.y, xa.y + xb.x);          // xa + i xb
-      X[pp * stride + bitrev(nz - k, lg)] = mk<V, T>(xa.x + xb.y, -xa.y + xb.x);    // conj(xa) + i conj(xb)
+      X[pp * stride + k] = mk<V, T>(xa.x - xb.y, xa.y + xb.x);          // xa + i xb
+      X[pp * stride + (nz - k)] = mk<V, T>(xa.x + xb.y, -xa.y + xb.x);  // conj(xa) + i conj(xb)
     }
   }
   __syncthreads();
@@ -304,7 +404,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_lines_kernel(const typename C
       const T kk = dk[i];
       v = mk<V, T>(v.y * kk, -v.x * kk);
     }
-    X[jj * stride + bitrev(i, a.lg)] = v;
+    X[jj * stride + i] = v;
   }
   __syncthreads();
   lds_fft<T>(X, a.tj, a.n, a.lg, stride, tw, a.inverse != 0);
@@ -411,8 +511,13 @@ Tables<T> upload_tables(PmWorkspace& w, const PmGeometry& g, const PmOptions& o,
 template <class T>
 int lines_per_block(int n) {
   using V = typename C2<T>::type;
+  int lg = 0;
+  while ((1 << lg) < n) ++lg;
+  const int cap = kFftMaxButterflies * kFftThreads * fft_min_radix(lg);  // points per block
   int tj = 16;
-  while (tj > 1 && static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + (n / 2) * sizeof(V) > 64 * 1024) tj >>= 1;
+  while (tj > 1 && (static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + (n / 2) * sizeof(V) > 64 * 1024 ||
+                    tj * n > cap))
+    tj >>= 1;
   return tj;
 }
 
@@ -430,8 +535,13 @@ void launch_lines(const typename C2<T>::type* in, typename C2<T>::type* out, Lin
 template <class T>
 int pairs_per_block(int nz) {
   using V = typename C2<T>::type;
+  int lg = 0;
+  while ((1 << lg) < nz) ++lg;
+  const int cap = kFftMaxButterflies * kFftThreads * fft_min_radix(lg);  // points per block
   int pp = 8;
-  while (pp > 1 && static_cast<std::size_t>(pp) * (nz + 1) * sizeof(V) + (nz / 2) * sizeof(V) > 48 * 1024) pp >>= 1;
+  while (pp > 1 && (static_cast<std::size_t>(pp) * (nz + 1) * sizeof(V) + (nz / 2) * sizeof(V) > 48 * 1024 ||
+                    pp * nz > cap))
+    pp >>= 1;
   return pp;
 }
 

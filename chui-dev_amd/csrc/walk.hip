@@ -522,25 +522,43 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
 // CUBIC: equal box lengths, so separations stay in fixed-point units (one
 // scale factor for all axes) and only the pair kernel converts to lengths.
 // COUNT: keep the per-target counters (diagnostic launches only).
-// A leaf chunk of up to 64 sources is staged in LDS with one coalesced load
-// and read back with wave-uniform (broadcast) LDS reads; the per-source test
-// and the queue drain are branch-free so loads can be issued ahead.
+//
+// Each wave keeps a window of up to kWindow leaf sources in LDS. A kept leaf is
+// staged with one coalesced load per lane; each lane then tests the window's
+// new sources with broadcast LDS reads and appends the window slot (8 bits) of
+// every accepted pair to its own LDS queue. The queue can hold a whole window,
+// so it is drained only when the window is full, before an accepted multipole
+// (keeping each lane's reference summation order) and at the end; the drain
+// reads sources from the window, not from global memory.
+//
+// Window format. Most waves hold 64 consecutive tree-order targets inside a
+// small region: their window stores float offsets from the wave origin O (the
+// centre of the targets' fixed-point bounding box), so a separation is three
+// float subtractions. The exact minimum image is kept because every offset
+// int32(q - O) is exact and, when all targets lie within min(L/16, L/2 - r_cut)
+// of O (per axis), a pair inside r_cut cannot straddle the wrap of O's frame.
+// Waves that straddle a jump of the space-filling order (targets far apart)
+// store raw fixed-point coordinates and form int32(q_s - q_t) per pair.
+constexpr int kWindow = 64;
+constexpr int kQueueStride = kWindow + 4;  // bytes per lane row (bank spread)
+
 template <int SCHEME, bool CUBIC, bool COUNT>
 __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out,
                                                                 unsigned long long* __restrict__ counters) {
-  __shared__ uint32_t queue[kWalkWaves][kQueue][kWave];
-  __shared__ uint4 stage[kWalkWaves][kWave];
+  __shared__ float4 win[kWalkWaves][kWindow];
+  __shared__ uint8_t queue[kWalkWaves][kWave * kQueueStride];
   __shared__ float4 htab[kHTableCells];
   for (int i = threadIdx.x; i < kHTableCells; i += kWalkThreads) htab[i] = D.htab[i];
   __syncthreads();
   const int w = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
+  uint8_t* const myq = &queue[w][lane * kQueueStride];
   const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
   const bool valid = k < P.n_targets;
   const uint32_t tp = valid ? (P.identity_targets ? k : D.tpos[k]) : 0u;
   const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
   uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0};
-  uint32_t qlen = 0, queued = 0;
+  uint32_t qlen = 0, queued = 0, wfill = 0;
   uint32_t nxt = valid ? 0u : P.node_count;
   uint32_t px = 0, py = 0, pz = 0;
   float gx = 0.0f, gy = 0.0f, gz = 0.0f;
@@ -551,7 +569,6 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
     pz = t.z;
     if (P.pm_enabled) interp32<SCHEME>(P, D.f32x, D.f32y, D.f32z, px, py, pz, gx, gy, gz);
   }
-  float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f, cx_ = 0.0f, cy_ = 0.0f, cz_ = 0.0f;
   // separations: fixed-point units (CUBIC) or lengths
   const float s0 = CUBIC ? 1.0f : P.fx_to_len[0];
   const float s1 = CUBIC ? 1.0f : P.fx_to_len[1];
@@ -564,17 +581,56 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
   const float contain2 = P.contain2_sep;
   const float eps2 = static_cast<float>(P.eps * P.eps);
   const float theta2 = static_cast<float>(P.theta * P.theta);
+  const float umass = P.uniform_mass_f;
+  // wave origin: centre of the valid targets' bounding box in fixed point
+  uint32_t org[3];
+  bool near = true;
+  {
+    const uint32_t pp[3] = {px, py, pz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t o0 = readlane_u32(pp[a], 0);
+      const int32_t off = valid ? static_cast<int32_t>(pp[a] - o0) : 0;
+      const int32_t lo = wave_min(off), hi = wave_max(off);
+      const int64_t mid = (static_cast<int64_t>(lo) + static_cast<int64_t>(hi)) / 2;
+      org[a] = o0 + static_cast<uint32_t>(static_cast<int32_t>(mid));
+      const int64_t half_extent = (static_cast<int64_t>(hi) - static_cast<int64_t>(lo)) / 2 + 1;
+      near = near && half_extent <= static_cast<int64_t>(P.fast_half_extent[a]);
+    }
+  }
+  const uint32_t ox = org[0], oy = org[1], oz = org[2];
+  const float tx = fx_delta(px, ox, s0), ty = fx_delta(py, oy, s1), tz = fx_delta(pz, oz, s2);
+  float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f, cx_ = 0.0f, cy_ = 0.0f, cz_ = 0.0f;
 
-  auto drain = [&]() {
+  // source -> window record, and window record -> separation from this lane
+  auto stage_record = [&]<bool NEAR>(const uint4& sv, float mj) -> float4 {
+    if constexpr (NEAR)
+      return make_float4(fx_delta(sv.x, ox, s0), fx_delta(sv.y, oy, s1), fx_delta(sv.z, oz, s2), mj);
+    else
+      return make_float4(__uint_as_float(sv.x), __uint_as_float(sv.y), __uint_as_float(sv.z), mj);
+  };
+  auto separation = [&]<bool NEAR>(const float4& sj, float& dx, float& dy, float& dz) {
+    if constexpr (NEAR) {
+      dx = sj.x - tx;
+      dy = sj.y - ty;
+      dz = sj.z - tz;
+    } else {
+      dx = fx_delta(__float_as_uint(sj.x), px, s0);
+      dy = fx_delta(__float_as_uint(sj.y), py, s1);
+      dz = fx_delta(__float_as_uint(sj.z), pz, s2);
+    }
+  };
+
+  auto drain = [&]<bool NEAR>() {
     const uint32_t m = wave_max_u32(qlen);
     float lx = 0.0f, ly = 0.0f, lz = 0.0f;
 #pragma unroll 2
     for (uint32_t q = 0; q < m; ++q) {
       const bool live = q < qlen;
-      const uint32_t qj = queue[w][q][lane];
-      const uint32_t j = live ? qj : tp;
-      const uint4 sj = D.src32[j];
-      const float dx = fx_delta(sj.x, px, s0), dy = fx_delta(sj.y, py, s1), dz = fx_delta(sj.z, pz, s2);
+      const uint32_t sl = live ? static_cast<uint32_t>(myq[q]) : 0u;
+      const float4 sj = win[w][sl];
+      float dx, dy, dz;
+      separation.template operator()<NEAR>(sj, dx, dy, dz);
       const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) * to_len2;
       const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
       const float soft = rinv * rinv * rinv;
@@ -584,8 +640,7 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
       const float fu = u - static_cast<float>(ci);
       const float4 c = htab[ci];
       const float hcoef = fmaf(fmaf(fmaf(c.w, fu, c.z), fu, c.y), fu, c.x);
-      const float mj = D.uniform_mass ? P.uniform_mass_f : __uint_as_float(sj.w);
-      const float fr = fmaf(-hcoef, P.c8, soft) * mj;
+      const float fr = fmaf(-hcoef, P.c8, soft) * sj.w;
       const float f = live ? fr : 0.0f;
       lx = fmaf(f, dx, lx);
       ly = fmaf(f, dy, ly);
@@ -597,88 +652,96 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
     kahan_add(sz_, cz_, lz * gm);
     queued += qlen;
     qlen = 0;
+    wfill = 0;
   };
 
-  uint32_t n = wave_min_u32(nxt);
-  while (n < P.node_count) {
-    if (COUNT && lane == 0) ++cnt[5];
-    const NodeLink lk = D.link[n];
-    const NodeGeo32 g = D.geo32[n];
-    const bool act = nxt == n;
-    if (!__ballot(act)) {
-      if (lane == 0) atomicOr(&counters[kWalkCounters], 1ull);
-      break;
-    }
-    if (COUNT) cnt[0] += act;
-    const float h = g.half * inv_len;
-    const float ax_ = fabsf(fx_delta(g.cx, px, s0)), ay_ = fabsf(fx_delta(g.cy, py, s1)),
-                az_ = fabsf(fx_delta(g.cz, pz, s2));
-    const float ex = fmaxf(0.0f, ax_ - h), ey = fmaxf(0.0f, ay_ - h), ez = fmaxf(0.0f, az_ - h);
-    const bool keep = act && !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
-    if (COUNT) cnt[1] += act && !keep;
-    nxt = (act && !keep) ? n + lk.skip : nxt;
-    if (lk.child_count == 0) {
-      if (COUNT) cnt[4] += keep;
-      nxt = keep ? n + 1 : nxt;
-      if (__ballot(keep)) {
-        const uint32_t e0 = lk.begin + lk.count;
-        for (uint32_t c0 = lk.begin; c0 < e0; c0 += kWave) {
-          const uint32_t cn = e0 - c0 < static_cast<uint32_t>(kWave) ? e0 - c0 : static_cast<uint32_t>(kWave);
-          if (COUNT && lane == 0) cnt[6] += cn;
-          stage[w][lane] = D.src32[c0 + (static_cast<uint32_t>(lane) < cn ? lane : 0)];
-          __builtin_amdgcn_wave_barrier();
-          for (uint32_t sb = 0; sb < cn; sb += kChunk) {
-            const uint32_t se = sb + kChunk < cn ? sb + kChunk : cn;
+  auto walk = [&]<bool NEAR>() {
+    uint32_t n = wave_min_u32(nxt);
+    while (n < P.node_count) {
+      if (COUNT && lane == 0) ++cnt[5];
+      const NodeLink lk = D.link[n];
+      const NodeGeo32 g = D.geo32[n];
+      const bool act = nxt == n;
+      if (!__ballot(act)) {
+        if (lane == 0) atomicOr(&counters[kWalkCounters], 1ull);
+        break;
+      }
+      if (COUNT) cnt[0] += act;
+      const float h = g.half * inv_len;
+      const float ax_ = fabsf(fx_delta(g.cx, px, s0)), ay_ = fabsf(fx_delta(g.cy, py, s1)),
+                  az_ = fabsf(fx_delta(g.cz, pz, s2));
+      const float ex = fmaxf(0.0f, ax_ - h), ey = fmaxf(0.0f, ay_ - h), ez = fmaxf(0.0f, az_ - h);
+      const bool keep = act && !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
+      if (COUNT) cnt[1] += act && !keep;
+      nxt = (act && !keep) ? n + lk.skip : nxt;
+      if (lk.child_count == 0) {
+        if (COUNT) cnt[4] += keep;
+        nxt = keep ? n + 1 : nxt;
+        if (__ballot(keep)) {
+          const uint32_t e0 = lk.begin + lk.count;
+          for (uint32_t c0 = lk.begin; c0 < e0; c0 += kWindow) {
+            const uint32_t cn = e0 - c0 < static_cast<uint32_t>(kWindow) ? e0 - c0 : static_cast<uint32_t>(kWindow);
+            if (COUNT && lane == 0) cnt[6] += cn;
+            if (wfill + cn > static_cast<uint32_t>(kWindow)) drain.template operator()<NEAR>();
+            {
+              const uint4 sv = D.src32[c0 + (static_cast<uint32_t>(lane) < cn ? lane : 0)];
+              const float mj = D.uniform_mass ? umass : __uint_as_float(sv.w);
+              if (static_cast<uint32_t>(lane) < cn) win[w][wfill + lane] = stage_record.template operator()<NEAR>(sv, mj);
+            }
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll 4
-            for (uint32_t si = sb; si < se; ++si) {
-              const uint4 sj = stage[w][si];
-              const float dx = fx_delta(sj.x, px, s0), dy = fx_delta(sj.y, py, s1), dz = fx_delta(sj.z, pz, s2);
+            for (uint32_t si = 0; si < cn; ++si) {
+              const float4 sj = win[w][wfill + si];
+              float dx, dy, dz;
+              separation.template operator()<NEAR>(sj, dx, dy, dz);
               const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
               const bool pair = keep && (c0 + si) != tp && r2 <= rc2;
               if (COUNT) cnt[2] += pair && r2 == 0.0f;  // coincident pair: counted, zero vector
-              queue[w][qlen][lane] = c0 + si;           // slot qlen is free; kept only if appended
+              myq[qlen] = static_cast<uint8_t>(wfill + si);  // slot qlen is free; kept only if appended
               qlen += (pair && r2 != 0.0f) ? 1u : 0u;
             }
-            if (wave_max_u32(qlen) > static_cast<uint32_t>(kQueue - kChunk)) drain();
+            wfill += cn;
           }
-          __builtin_amdgcn_wave_barrier();
+        }
+      } else if (__ballot(keep)) {
+        bool take_mp = false;
+        float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;
+        if (keep) {
+          const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+          const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+          const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= contain2;
+          const NodeMom32 M = D.mom32[n];
+          mfx = fx_delta(M.comx, px, s0) * to_len;
+          mfy = fx_delta(M.comy, py, s1) * to_len;
+          mfz = fx_delta(M.comz, pz, s2) * to_len;
+          const float r2 = mfx * mfx + mfy * mfy + mfz * mfz;
+          const float w2 = 2.0f * g.half;
+          // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
+          const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
+          const float lead = P.criterion == 0 ? w2 : w2 + D.quad32[n].offset;
+          const bool mac = !inside && lead * lead < theta2 * (r2 + 1.0e-30f);
+          take_mp = mac && env_ok && contained && !inside;
+          nxt = take_mp ? n + lk.skip : n + 1;
+        }
+        if (__ballot(take_mp)) {
+          drain.template operator()<NEAR>();
+          if (take_mp) {
+            if (COUNT) ++cnt[3];
+            float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+            node_accel32(D.mom32[n], D.quad32[n], mfx, mfy, mfz, P, cx, cy, cz);
+            kahan_add(sx_, cx_, cx);
+            kahan_add(sy_, cy_, cy);
+            kahan_add(sz_, cz_, cz);
+          }
         }
       }
-    } else if (__ballot(keep)) {
-      bool take_mp = false;
-      float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;
-      if (keep) {
-        const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
-        const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
-        const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= contain2;
-        const NodeMom32 M = D.mom32[n];
-        mfx = fx_delta(M.comx, px, s0) * to_len;
-        mfy = fx_delta(M.comy, py, s1) * to_len;
-        mfz = fx_delta(M.comz, pz, s2) * to_len;
-        const float r2 = mfx * mfx + mfy * mfy + mfz * mfz;
-        const float w2 = 2.0f * g.half;
-        // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
-        const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
-        const float lead = P.criterion == 0 ? w2 : w2 + D.quad32[n].offset;
-        const bool mac = !inside && lead * lead < theta2 * (r2 + 1.0e-30f);
-        take_mp = mac && env_ok && contained && !inside;
-        nxt = take_mp ? n + lk.skip : n + 1;
-      }
-      if (__ballot(take_mp)) {
-        drain();
-        if (take_mp) {
-          if (COUNT) ++cnt[3];
-          float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-          node_accel32(D.mom32[n], D.quad32[n], mfx, mfy, mfz, P, cx, cy, cz);
-          kahan_add(sx_, cx_, cx);
-          kahan_add(sy_, cy_, cy);
-          kahan_add(sz_, cz_, cz);
-        }
-      }
+      n = next_node(n, nxt);
     }
-    n = next_node(n, nxt);
-  }
-  drain();
+    drain.template operator()<NEAR>();
+  };
+  if (near) walk.template operator()<true>();
+  else walk.template operator()<false>();
+
   if (COUNT) {
     cnt[2] += queued;
     flush_counters(counters, cnt);

@@ -40,6 +40,10 @@ struct WalkParams {
   float rcut2_sep, prune2_sep, contain2_sep;
   float c4, c8;        // 1/(4 a^2), 1/(8 a^3)
   float t_to_cell;     // r^2 -> table cell coordinate
+  // Largest half-extent (fixed-point units, per axis) of a wave's targets for
+  // which the walk uses float offsets from the wave origin:
+  // min(2^28, 2^31 - r_cut - 2^24) in units of L_a / 2^32.
+  uint32_t fast_half_extent[3];
 };
 
 struct WalkOutputs {

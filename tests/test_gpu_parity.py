@@ -167,6 +167,25 @@ def test_mixed_precision_error_budget():
     assert m["p99"] < 1e-3
 
 
+def test_mixed_waves_spanning_the_box():
+    # Consecutive tree-order targets can lie far apart (jumps of the octant
+    # order), so one wave may hold targets from opposite ends of the box.
+    # Every target must still get the exact minimum image: compare the mixed
+    # walk with the FP64 walk particle by particle.
+    rng = np.random.default_rng(17)
+    box = 50.0
+    centres = np.array([[1.0, 1.0, 1.0], [49.0, 49.0, 49.0], [25.0, 1.0, 49.0], [1.0, 49.0, 25.0]])
+    pts = np.concatenate([c + rng.normal(0.0, 0.4, (40, 3)) for c in centres])
+    pts = np.mod(pts, box)
+    x, y, z = (pts[:, a].copy() for a in range(3))
+    m = np.ones(x.size)
+    opts = options_struct(box, eps=0.01)
+    _, _, sr64, _, _ = gpu_solve(x, y, z, m, opts, 32, precision=0)
+    _, _, sr32, _, _ = gpu_solve(x, y, z, m, opts, 32, precision=1)
+    e = np.linalg.norm(np.stack(sr32) - np.stack(sr64), axis=0) / np.linalg.norm(np.stack(sr64), axis=0)
+    assert e.max() < 1e-4
+
+
 def test_active_subset_equals_full_rows():
     # The short-range walk is deterministic per target, so the tree-only lanes
     # match bit for bit; the PM deposit uses FP64 atomics, so totals agree to
