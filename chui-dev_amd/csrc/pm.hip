@@ -20,7 +20,10 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846264338327950288;
 constexpr int kFftThreads = 256;
-constexpr int kFftMaxButterflies = 4;  // per thread and pass
+// Butterflies a thread keeps in registers per pass (float2: 4 x 8 points,
+// double2: 2 x 8 points); blocks are sized so that lines * n / R fits.
+template <class T> constexpr int fft_max_butterflies() { return sizeof(T) == 4 ? 4 : 2; }
+constexpr int kFftLoadBatch = 8;  // global loads a thread issues before its LDS stores
 
 // Smallest radix of the pass plan of lds_fft for a line of 2^lg points.
 inline int fft_min_radix(int lg) {
@@ -108,7 +111,7 @@ __device__ __attribute__((noinline)) void stockham_pass(typename C2<T>::type* X,
                                               const typename C2<T>::type* tw, int lgNs) {
   using V = typename C2<T>::type;
   constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
-  constexpr int MAXB = kFftMaxButterflies;  // the host sizes blocks so lines * n / R <= MAXB * kFftThreads
+  constexpr int MAXB = fft_max_butterflies<T>();  // the host sizes blocks so lines * n / R <= MAXB * kFftThreads
   const int lgq = lgn - LR;
   const int q = 1 << lgq;
   const int total = nlines << lgq;
@@ -284,16 +287,26 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
   const int stride = nz + 1;
   for (int i = threadIdx.x; i < (nz >> 1); i += blockDim.x) tw[i] = twz[i];
   const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
-  for (int idx = threadIdx.x; idx < pairs_per_block * nz; idx += blockDim.x) {
-    const int pp = idx / nz, i = idx - pp * nz;
-    const std::size_t pr = pair0 + pp;
-    V v = mk<V, T>(T(0), T(0));
-    if (2 * pr < rows) {
-      const T* ra = data + (2 * pr) * row;
-      v.x = ra[i];
-      if (2 * pr + 1 < rows) v.y = ra[row + i];
+  const int npts = pairs_per_block * nz;
+  for (int base = threadIdx.x; base < npts; base += kFftLoadBatch * kFftThreads) {
+    V v[kFftLoadBatch];
+#pragma unroll
+    for (int u = 0; u < kFftLoadBatch; ++u) {  // all loads in flight before the LDS stores
+      const int idx = base + u * kFftThreads;
+      v[u] = mk<V, T>(T(0), T(0));
+      const int pp = idx >> lg, i = idx & (nz - 1);
+      const std::size_t pr = pair0 + pp;
+      if (idx < npts && 2 * pr < rows) {
+        const T* ra = data + (2 * pr) * row;
+        v[u].x = ra[i];
+        if (2 * pr + 1 < rows) v[u].y = ra[row + i];
+      }
     }
-    X[pp * stride + i] = v;
+#pragma unroll
+    for (int u = 0; u < kFftLoadBatch; ++u) {
+      const int idx = base + u * kFftThreads;
+      if (idx < npts) X[(idx >> lg) * stride + (idx & (nz - 1))] = v[u];
+    }
   }
   __syncthreads();
   lds_fft<T>(X, pairs_per_block, nz, lg, stride, tw, false);
@@ -328,24 +341,38 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::si
   const int nzc = (nz >> 1) + 1;
   for (int i = threadIdx.x; i < (nz >> 1); i += blockDim.x) tw[i] = twz[i];
   const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
-  for (int idx = threadIdx.x; idx < pairs_per_block * nzc; idx += blockDim.x) {
-    const int pp = idx / nzc, k = idx - pp * nzc;
-    const std::size_t pr = pair0 + pp;
-    V xa = mk<V, T>(T(0), T(0)), xb = mk<V, T>(T(0), T(0));
-    if (2 * pr < rows) {
-      xa = reinterpret_cast<const V*>(data + (2 * pr) * row)[k];
-      if (2 * pr + 1 < rows) xb = reinterpret_cast<const V*>(data + (2 * pr + 1) * row)[k];
+  const int npts = pairs_per_block * nzc;
+  for (int base = threadIdx.x; base < npts; base += kFftLoadBatch * kFftThreads) {
+    V va[kFftLoadBatch], vb[kFftLoadBatch];
+#pragma unroll
+    for (int u = 0; u < kFftLoadBatch; ++u) {  // all loads in flight before the LDS stores
+      const int idx = base + u * kFftThreads;
+      va[u] = mk<V, T>(T(0), T(0));
+      vb[u] = mk<V, T>(T(0), T(0));
+      const int pp = idx / nzc, k = idx - pp * nzc;
+      const std::size_t pr = pair0 + pp;
+      if (idx < npts && 2 * pr < rows) {
+        va[u] = reinterpret_cast<const V*>(data + (2 * pr) * row)[k];
+        if (2 * pr + 1 < rows) vb[u] = reinterpret_cast<const V*>(data + (2 * pr + 1) * row)[k];
+      }
     }
-    if (premul) {  // multiply by -i k_z
-      const T kk = dkz[k];
-      xa = mk<V, T>(xa.y * kk, -xa.x * kk);
-      xb = mk<V, T>(xb.y * kk, -xb.x * kk);
-    }
-    if (k == 0 || k == nz / 2) {
-      X[pp * stride + (k & (nz - 1))] = mk<V, T>(xa.x, xb.x);
-    } else {
-      X[pp * stride + k] = mk<V, T>(xa.x - xb.y, xa.y + xb.x);          // xa + i xb
-      X[pp * stride + (nz - k)] = mk<V, T>(xa.x + xb.y, -xa.y + xb.x);  // conj(xa) + i conj(xb)
+#pragma unroll
+    for (int u = 0; u < kFftLoadBatch; ++u) {
+      const int idx = base + u * kFftThreads;
+      if (idx >= npts) continue;
+      const int pp = idx / nzc, k = idx - pp * nzc;
+      V xa = va[u], xb = vb[u];
+      if (premul) {  // multiply by -i k_z
+        const T kk = dkz[k];
+        xa = mk<V, T>(xa.y * kk, -xa.x * kk);
+        xb = mk<V, T>(xb.y * kk, -xb.x * kk);
+      }
+      if (k == 0 || k == nz / 2) {
+        X[pp * stride + (k & (nz - 1))] = mk<V, T>(xa.x, xb.x);
+      } else {
+        X[pp * stride + k] = mk<V, T>(xa.x - xb.y, xa.y + xb.x);          // xa + i xb
+        X[pp * stride + (nz - k)] = mk<V, T>(xa.x + xb.y, -xa.y + xb.x);  // conj(xa) + i conj(xb)
+      }
     }
   }
   __syncthreads();
@@ -395,16 +422,29 @@ __global__ __launch_bounds__(kFftThreads) void fft_lines_kernel(const typename C
   const int tjm = a.tj - 1;
   int tjl = 0;
   while ((1 << tjl) < a.tj) ++tjl;
-  for (int idx = threadIdx.x; idx < a.tj * a.n; idx += blockDim.x) {
-    const int jj = idx & tjm, i = idx >> tjl;
-    const std::size_t j = j0 + jj;
-    V v = mk<V, T>(T(0), T(0));
-    if (j < a.inner_count) v = in[obase + j + static_cast<std::size_t>(i) * a.line_stride];
-    if (a.premul) {
-      const T kk = dk[i];
-      v = mk<V, T>(v.y * kk, -v.x * kk);
+  const int npts = a.tj * a.n;
+  for (int base = threadIdx.x; base < npts; base += kFftLoadBatch * kFftThreads) {
+    V v[kFftLoadBatch];
+#pragma unroll
+    for (int u = 0; u < kFftLoadBatch; ++u) {  // all loads in flight before the LDS stores
+      const int idx = base + u * kFftThreads;
+      const int jj = idx & tjm, i = idx >> tjl;
+      const std::size_t j = j0 + jj;
+      v[u] = mk<V, T>(T(0), T(0));
+      if (idx < npts && j < a.inner_count) v[u] = in[obase + j + static_cast<std::size_t>(i) * a.line_stride];
     }
-    X[jj * stride + i] = v;
+#pragma unroll
+    for (int u = 0; u < kFftLoadBatch; ++u) {
+      const int idx = base + u * kFftThreads;
+      if (idx >= npts) continue;
+      const int jj = idx & tjm, i = idx >> tjl;
+      V vv = v[u];
+      if (a.premul) {
+        const T kk = dk[i];
+        vv = mk<V, T>(vv.y * kk, -vv.x * kk);
+      }
+      X[jj * stride + i] = vv;
+    }
   }
   __syncthreads();
   lds_fft<T>(X, a.tj, a.n, a.lg, stride, tw, a.inverse != 0);
@@ -508,14 +548,28 @@ Tables<T> upload_tables(PmWorkspace& w, const PmGeometry& g, const PmOptions& o,
                    base + off[6], base + off[7], base + off[8], tb + toff[0], tb + toff[1], tb + toff[2]};
 }
 
+// Dynamic LDS a line-pass block may use: 80 KB (two blocks per CU, 16 lines
+// of 512 float2) when the device allows more than 64 KB per block.
+std::size_t fft_lds_limit() {
+  static std::size_t limit = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return std::size_t(64 * 1024);
+    return std::min<std::size_t>(static_cast<std::size_t>(v), 80 * 1024);
+  }();
+  return limit;
+}
+
 template <class T>
 int lines_per_block(int n) {
   using V = typename C2<T>::type;
   int lg = 0;
   while ((1 << lg) < n) ++lg;
-  const int cap = kFftMaxButterflies * kFftThreads * fft_min_radix(lg);  // points per block
+  const int cap = fft_max_butterflies<T>() * kFftThreads * fft_min_radix(lg);  // points per block
+  const std::size_t lim = fft_lds_limit();
   int tj = 16;
-  while (tj > 1 && (static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + (n / 2) * sizeof(V) > 64 * 1024 ||
+  while (tj > 1 && (static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + (n / 2) * sizeof(V) > lim ||
                     tj * n > cap))
     tj >>= 1;
   return tj;
@@ -527,6 +581,14 @@ void launch_lines(const typename C2<T>::type* in, typename C2<T>::type* out, Lin
   using V = typename C2<T>::type;
   a.tj = lines_per_block<T>(a.n);
   const std::size_t smem = static_cast<std::size_t>(a.tj) * (a.n + 1) * sizeof(V) + (a.n / 2) * sizeof(V);
+  if (smem > 64 * 1024) {
+    static std::size_t granted = 0;
+    if (smem > granted) {
+      CHUI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lines_kernel<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem)));
+      granted = smem;
+    }
+  }
   dim3 grid(static_cast<unsigned>((a.inner_count + a.tj - 1) / a.tj), static_cast<unsigned>(outer_count));
   fft_lines_kernel<T><<<grid, kFftThreads, smem, s>>>(in, out, a, tw, dk, tb.kx, tb.ky, tb.kz, tb.wx, tb.wy, tb.wz);
   CHUI_LAUNCH_CHECK("fft_lines_kernel");
@@ -537,7 +599,7 @@ int pairs_per_block(int nz) {
   using V = typename C2<T>::type;
   int lg = 0;
   while ((1 << lg) < nz) ++lg;
-  const int cap = kFftMaxButterflies * kFftThreads * fft_min_radix(lg);  // points per block
+  const int cap = fft_max_butterflies<T>() * kFftThreads * fft_min_radix(lg);  // points per block
   int pp = 8;
   while (pp > 1 && (static_cast<std::size_t>(pp) * (nz + 1) * sizeof(V) + (nz / 2) * sizeof(V) > 48 * 1024 ||
                     pp * nz > cap))
