@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-side", type=int, default=256)
     ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--ewald-samples", type=int, default=2048)
+    ap.add_argument("--ewald-samples", type=int, default=8192)
     ap.add_argument("--cpu-n-side", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -201,7 +201,8 @@ def main():
         "cpu_baseline": cpu,
         "counters": {k: diag[k] for k in ("node_count", "max_depth", "direct_pair_evaluations",
                                           "accepted_internal_multipoles", "accepted_leaves", "visited_nodes",
-                                          "wave_node_visits", "wave_leaf_source_slots")},
+                                          "wave_node_visits", "wave_leaf_source_slots",
+                                          "wave_drain_iterations")},
     }
     coord.close()
     print(json.dumps(line), flush=True)

@@ -365,8 +365,8 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   record(c, kEvTargets);
 
   // --- walk (+ fused PM interpolation) ---
-  c->counters.reserve(8 * sizeof(unsigned long long));
-  chui::fill_u64(c->counters.as<uint64_t>(), 0ull, 8, s);
+  c->counters.reserve((chui::kWalkCounters + 1) * sizeof(unsigned long long));
+  chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounters + 1, s);
   const bool check_walk_progress_ = diag != nullptr;
   chui::WalkParams wp{};
   for (int a = 0; a < 3; ++a) {
@@ -421,6 +421,15 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     } else {
       const double t_max = wp.rcut2 * (0.25 / (sp.a * sp.a)) * (1.0 + 1.0e-3);
       std::vector<float> tab = chui::gaussian_coefficient_table(wp, t_max);
+      // pair arithmetic in table units: lengths * sqrt(t_to_cell)
+      const double t2c = 0.25 / (sp.a * sp.a) / (t_max / chui::kHTableCells);
+      const double sq = std::sqrt(t2c);
+      for (int a = 0; a < 3; ++a) wp.tab_scale[a] = static_cast<float>(o->box_size[a] / 4294967296.0 * sq);
+      wp.rcut2_tab = static_cast<float>(wp.rcut2 * t2c);
+      wp.eps2_tab = static_cast<float>(wp.eps * wp.eps * t2c);
+      wp.c8_tab = static_cast<float>(0.125 / (sp.a * sp.a * sp.a) / (t2c * sq));
+      wp.force_scale = static_cast<float>(wp.G * t2c);
+      wp.accept_half_max = static_cast<float>(0.041 * sp.rcut);
       if (c->htab_key[0] != t_max || c->htab_key[1] != sp.a || !c->htab.ptr) {
         c->htab.reserve(tab.size() * sizeof(float));
         CHUI_HIP(hipMemcpyAsync(c->htab.ptr, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, s));
@@ -446,7 +455,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   if (diag) {
     if (diag->struct_size != sizeof(chui_treepm_diagnostics))
       throw std::invalid_argument("chui_treepm_diagnostics.struct_size does not match this library");
-    unsigned long long cnt[8] = {};
+    unsigned long long cnt[chui::kWalkCounters] = {};
     CHUI_HIP(hipMemcpyAsync(cnt, c->counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, s));
     CHUI_HIP(hipStreamSynchronize(s));
     diag->node_count = c->tree.node_count;
@@ -459,6 +468,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     diag->accepted_leaves = cnt[4];
     diag->wave_node_visits = cnt[5];
     diag->wave_leaf_source_slots = cnt[6];
+    diag->wave_drain_iterations = cnt[7];
     diag->split_scale_comoving = sp.a;
     diag->cutoff_radius_comoving = sp.rcut;
     diag->mesh_spacing_comoving = sp.h;

@@ -73,24 +73,39 @@ __global__ __launch_bounds__(256) void source_stats_kernel(Axes3 p, const double
   uint64_t lo[3] = {kEmpty, kEmpty, kEmpty}, hi[3] = {0, 0, 0};
   uint32_t bc = 0, bm = 0, nu = 0;
   const double m0 = (m != nullptr && n > 0) ? m[0] : 0.0;
-  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+  auto take = [&](const double* v, double mm) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const double v = p.p[a][i];
-      if (!isfinite(v)) {
+      if (!isfinite(v[a])) {
         bc = 1;
       } else {
-        const uint64_t k = order_key(v);
+        const uint64_t k = order_key(v[a]);
         lo[a] = k < lo[a] ? k : lo[a];
         hi[a] = k > hi[a] ? k : hi[a];
       }
     }
     if (m != nullptr) {
-      const double mm = m[i];
       if (!(isfinite(mm) && mm >= 0.0)) bm = 1;
       if (mm != m0) nu = 1;
     }
+  };
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  constexpr int kU = 4;  // loads of kU elements in flight per thread
+  for (; i + (kU - 1) * stride < n; i += kU * stride) {
+    double v[kU][3], mm[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[u][a] = p.p[a][i + u * stride];
+      mm[u] = m != nullptr ? m[i + u * stride] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) take(v[u], mm[u]);
+  }
+  for (; i < n; i += stride) {
+    const double v[3] = {p.p[0][i], p.p[1][i], p.p[2][i]};
+    take(v, m != nullptr ? m[i] : 0.0);
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -114,34 +129,70 @@ __global__ __launch_bounds__(256) void source_stats_kernel(Axes3 p, const double
 
 // ---------------------------------------------------------------------------
 // Periodic unwrap (tree_pm_coupling.cpp:141-218) without a sort.
-// With B >= 2n buckets of width L/B over [0, L), every gap between two values
-// in one bucket is < L/(2n)(1+4e-9) while the largest circular gap is >= L/n,
-// so the largest gap (and every gap tying with it) lies between the maximum of
-// a non-empty bucket and the minimum of the next non-empty bucket. Per-bucket
-// min/max of the IEEE bits (non-negative doubles order like their bits), an
-// exclusive max-scan and one argmax (ties -> smaller anchor) give the exact
-// anchor the reference obtains after its 8-pass radix sort.
+// Values are binned into B >= 2n buckets of width w = L/B. The largest
+// circular gap is >= L/n = 2w, while two values in one bucket differ by less
+// than w (up to rounding), so the largest gap, and every gap tying with it,
+// runs from the maximum of an occupied bucket p to the minimum of the next
+// occupied bucket i. If e = i - p - 1 empty buckets lie between them, the gap
+// is within (e w, (e + 2) w) up to rounding of the bucket computation, so only
+// runs with e >= R - 2 (R the longest empty run) can hold the largest gap.
+// Steps per axis: an occupancy bitmap (one 32-bit OR per value); R from a scan
+// of the bitmap; the end buckets of candidate runs and their predecessors are
+// marked; exact FP64 min/max (IEEE bits; non-negative doubles order like their
+// bits) are gathered only for marked buckets, ranked by a popcount scan; the
+// candidate gaps are evaluated with the reference's tie rule (smaller anchor).
 struct BucketArgs {
   double L[3];
   double scale[3];  // B / L
   int canonical[3];
   int naxes;
-  uint64_t B;
+  uint64_t B;       // multiple of 32
+  uint64_t W;       // B / 32 bitmap words per axis
 };
 
-__global__ __launch_bounds__(256) void bucket_kernel(Axes3 p, std::size_t n, BucketArgs a,
-                                                     uint64_t* __restrict__ bmin, uint64_t* __restrict__ bmax) {
+__device__ __forceinline__ uint64_t bucket_of(double v, const BucketArgs& a, int ax) {
+  uint64_t b = static_cast<uint64_t>(v * a.scale[ax]);
+  return b >= a.B ? a.B - 1 : b;
+}
+
+__global__ __launch_bounds__(256) void occupancy_kernel(Axes3 p, std::size_t n, BucketArgs a,
+                                                        uint32_t* __restrict__ occ) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    for (int ax = 0; ax < a.naxes; ++ax) {
+      const uint64_t b = bucket_of(wrap_coord(p.p[ax][i], a.L[ax], a.canonical[ax]), a, ax);
+      atomicOr(&occ[ax * a.W + (b >> 5)], 1u << (b & 31));
+    }
+  }
+}
+
+// Exact min/max (IEEE bits, max stored +1 so 0 = empty) of the marked buckets.
+__global__ __launch_bounds__(256) void marked_minmax_kernel(Axes3 p, std::size_t n, BucketArgs a,
+                                                            const uint32_t* __restrict__ mark,
+                                                            const uint32_t* __restrict__ rank_base,
+                                                            uint64_t* __restrict__ cmin, uint64_t* __restrict__ cmax) {
   for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     for (int ax = 0; ax < a.naxes; ++ax) {
       const double v = wrap_coord(p.p[ax][i], a.L[ax], a.canonical[ax]);
-      uint64_t b = static_cast<uint64_t>(v * a.scale[ax]);
-      if (b >= a.B) b = a.B - 1;
-      const uint64_t bits = dbits(v);
-      atomicMin(reinterpret_cast<unsigned long long*>(&bmin[ax * a.B + b]), bits);
-      atomicMax(reinterpret_cast<unsigned long long*>(&bmax[ax * a.B + b]), bits + 1);  // 0 = empty
+      const uint64_t b = bucket_of(v, a, ax);
+      const uint64_t wi = ax * a.W + (b >> 5);
+      const uint32_t mw = mark[wi];
+      const uint32_t bit = 1u << (b & 31);
+      if (mw & bit) {
+        const uint32_t r = rank_base[wi] + __popc(mw & (bit - 1u));
+        const uint64_t bits = dbits(v);
+        atomicMin(reinterpret_cast<unsigned long long*>(&cmin[r]), bits);
+        atomicMax(reinterpret_cast<unsigned long long*>(&cmax[r]), bits + 1);
+      }
     }
   }
+}
+
+__global__ __launch_bounds__(256) void popc_kernel(const uint32_t* __restrict__ w, std::size_t nw,
+                                                   uint32_t* __restrict__ out) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < nw) out[i] = __popc(w[i]);
 }
 
 constexpr int kGapThreads = 256;
@@ -186,17 +237,21 @@ __device__ __forceinline__ uint64_t block_excl_max_u64(uint64_t v, uint64_t* tot
   return excl > woff ? excl : woff;
 }
 
-__global__ __launch_bounds__(kGapThreads) void bucket_blockmax_kernel(const uint64_t* __restrict__ bmax, uint64_t B,
-                                                                      uint64_t* __restrict__ blockmax, uint64_t nblk) {
+// Per block of kGapTile bitmap words: 1 + the last occupied bucket (0 if none).
+__global__ __launch_bounds__(kGapThreads) void occ_blockmax_kernel(const uint32_t* __restrict__ occ, uint64_t W,
+                                                                   uint64_t* __restrict__ blockmax, uint64_t nblk) {
   const int ax = blockIdx.y;
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kGapTile;
   uint64_t mx = 0;
 #pragma unroll
   for (int k = 0; k < kGapItems; ++k) {
-    const uint64_t idx = base + static_cast<uint64_t>(k) * kGapThreads + threadIdx.x;
-    if (idx < B) {
-      const uint64_t v = bmax[ax * B + idx];
-      mx = v > mx ? v : mx;
+    const uint64_t wi = base + static_cast<uint64_t>(k) * kGapThreads + threadIdx.x;
+    if (wi < W) {
+      const uint32_t v = occ[ax * W + wi];
+      if (v) {
+        const uint64_t last = wi * 32 + 31 - __clz(v);
+        mx = last + 1 > mx ? last + 1 : mx;
+      }
     }
   }
   mx = block_max_u64(mx);
@@ -262,54 +317,106 @@ __device__ GapBest block_best(GapBest b) {
   return r;
 }
 
-__global__ __launch_bounds__(kGapThreads) void gap_kernel(const uint64_t* __restrict__ bmin,
-                                                          const uint64_t* __restrict__ bmax, uint64_t B,
-                                                          const uint64_t* __restrict__ block_prefix,
-                                                          const uint64_t* __restrict__ total, uint64_t nblk,
-                                                          BucketArgs a, GapBest* __restrict__ block_out) {
+// Walks the occupied buckets of a block's words in order. For each occupied
+// bucket i it knows its predecessor p (the previous occupied bucket, or for the
+// first one the last occupied bucket minus B: the wrap) and the empty run
+// e = i - p - 1. MODE 0: the longest run per axis (atomicMax). MODE 1: marks
+// the end buckets of runs with e >= R - 2 and their predecessors. MODE 2:
+// evaluates the marked gaps from the exact min/max and keeps the best per block.
+struct RunArgs {
+  const uint32_t* occ;
+  const uint64_t* block_prefix;  // exclusive max-scan of the block maxima
+  const uint64_t* total;         // 1 + last occupied bucket, per axis
+  unsigned long long* longest;   // per axis (MODE 0 writes, MODE 1 reads)
+  uint32_t* mark;                // MODE 1 writes
+  uint32_t* ends;                // MODE 1 writes
+  const uint32_t* rank_base;     // MODE 2
+  const uint64_t* cmin;          // MODE 2
+  const uint64_t* cmax;          // MODE 2
+  GapBest* block_out;            // MODE 2
+  uint64_t nblk;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kGapThreads) void runs_kernel(BucketArgs a, RunArgs r) {
   const int ax = blockIdx.y;
-  const double L = a.L[ax];
-  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kGapTile + static_cast<uint64_t>(threadIdx.x) * kGapItems;
+  const uint64_t W = a.W, B = a.B;
+  const uint64_t wbase = static_cast<uint64_t>(blockIdx.x) * kGapTile + static_cast<uint64_t>(threadIdx.x) * kGapItems;
+  // 1 + last occupied bucket among this thread's words
   uint64_t tmax = 0;
+  uint32_t words[kGapItems];
 #pragma unroll
   for (int k = 0; k < kGapItems; ++k) {
-    const uint64_t idx = base + k;
-    if (idx < B) {
-      const uint64_t v = bmax[ax * B + idx];
-      tmax = v > tmax ? v : tmax;
+    const uint64_t wi = wbase + k;
+    words[k] = wi < W ? r.occ[ax * W + wi] : 0u;
+    if (words[k]) {
+      const uint64_t last = wi * 32 + 31 - __clz(words[k]);
+      tmax = last + 1 > tmax ? last + 1 : tmax;
     }
   }
   uint64_t tot;
   uint64_t run = block_excl_max_u64(tmax, &tot);
-  const uint64_t bp = block_prefix[ax * nblk + blockIdx.x];
+  const uint64_t bp = r.block_prefix[ax * r.nblk + blockIdx.x];
   run = run > bp ? run : bp;
-  const double gmax = bitsd(total[ax] - 1);
+  const uint64_t last_all = r.total[ax] - 1;  // last occupied bucket of the axis
+  const unsigned long long R = MODE == 1 ? r.longest[ax] : 0ull;
+  const double L = a.L[ax];
+  uint64_t longest = 0;
   GapBest best{0.0, 0.0, 0.0, 0, 0};
+  const auto rank_of = [&](uint64_t b) -> uint32_t {
+    const uint64_t wi = ax * W + (b >> 5);
+    const uint32_t bit = 1u << (b & 31);
+    return r.rank_base[wi] + __popc(r.mark[wi] & (bit - 1u));
+  };
   for (int k = 0; k < kGapItems; ++k) {
-    const uint64_t idx = base + k;
-    if (idx >= B) break;
-    const uint64_t mn = bmin[ax * B + idx];
-    if (mn != kEmpty) {
-      const double v = bitsd(mn);
-      GapBest c;
-      c.valid = 1;
-      c.pad = 0;
-      c.anchor = v;
-      if (run == 0) {  // first non-empty bucket: circular wrap gap
-        c.gap = (v + L) - gmax;
-        c.umax = gmax;
+    uint32_t v = words[k];
+    const uint64_t wi = wbase + k;
+    uint32_t endbits = 0;
+    while (v) {
+      const int bitpos = __ffs(v) - 1;
+      v &= v - 1;
+      const uint64_t i = wi * 32 + bitpos;
+      // run == 0: first occupied bucket; its predecessor is the last one, across the wrap
+      const uint64_t e = run == 0 ? (B - 1 - last_all) + i : i - run;  // run = 1 + predecessor
+      if (MODE == 0) {
+        longest = e > longest ? e : longest;
+      } else if (MODE == 1) {
+        if (e + 2 >= R) {
+          endbits |= 1u << bitpos;
+          const uint64_t pb = run == 0 ? last_all : run - 1;
+          atomicOr(&r.mark[ax * W + (pb >> 5)], 1u << (pb & 31));
+          atomicOr(&r.mark[ax * W + (i >> 5)], 1u << (i & 31));
+        }
       } else {
-        const double pr = bitsd(run - 1);
-        c.gap = v - pr;
-        c.umax = pr + L;
+        if ((r.ends[ax * W + wi] >> bitpos) & 1u) {
+          const double vmin = bitsd(r.cmin[rank_of(i)]);
+          GapBest c;
+          c.valid = 1;
+          c.pad = 0;
+          c.anchor = vmin;
+          if (run == 0) {
+            const double gmax = bitsd(r.cmax[rank_of(last_all)] - 1);
+            c.gap = (vmin + L) - gmax;
+            c.umax = gmax;
+          } else {
+            const double pr = bitsd(r.cmax[rank_of(run - 1)] - 1);
+            c.gap = vmin - pr;
+            c.umax = pr + L;
+          }
+          if (better(c, best)) best = c;
+        }
       }
-      if (better(c, best)) best = c;
+      run = i + 1;
     }
-    const uint64_t mx = bmax[ax * B + idx];
-    run = mx > run ? mx : run;
+    if (MODE == 1 && endbits) r.ends[ax * W + wi] = endbits;
   }
-  best = block_best(best);
-  if (threadIdx.x == 0) block_out[ax * nblk + blockIdx.x] = best;
+  if (MODE == 0) {
+    longest = wave_max(longest);
+    if (lane_id() == 0 && longest) atomicMax(&r.longest[ax], static_cast<unsigned long long>(longest));
+  } else if (MODE == 2) {
+    best = block_best(best);
+    if (threadIdx.x == 0) r.block_out[ax * r.nblk + blockIdx.x] = best;
+  }
 }
 
 __global__ __launch_bounds__(kGapThreads) void gap_final_kernel(const GapBest* __restrict__ block_out, uint64_t nblk,
@@ -776,37 +883,82 @@ double host_order_key_inv(uint64_t k) {
 // Unwraps naxes axes at once; fills res (host).
 void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, const int* canonical,
                  UnwrapResult& res, hipStream_t s) {
+  for (int ax = 0; ax < naxes; ++ax) {
+    res.anchor[ax] = 0.0;
+    res.umin[ax] = 0.0;
+    res.umax[ax] = 0.0;
+    res.canonical[ax] = canonical[ax];
+  }
+  if (n == 0) return;
   BucketArgs a{};
-  const uint64_t B = std::max<uint64_t>(2ull * n, 2ull);
+  const uint64_t B = (std::max<uint64_t>(2ull * n, 64ull) + 31) / 32 * 32;
   a.B = B;
+  a.W = B / 32;
   a.naxes = naxes;
   for (int ax = 0; ax < naxes; ++ax) {
     a.L[ax] = L[ax];
     a.scale[ax] = static_cast<double>(B) / L[ax];
     a.canonical[ax] = canonical[ax];
   }
-  t.bucket_min.reserve(naxes * B * sizeof(uint64_t));
-  t.bucket_max.reserve(naxes * B * sizeof(uint64_t));
-  const uint64_t nblk = (B + kGapTile - 1) / kGapTile;
+  const uint64_t W = a.W, NW = naxes * W;
+  const uint64_t nblk = (W + kGapTile - 1) / kGapTile;
+  // bitmaps: occupancy, marks, run ends; ranks; block scan data
+  t.bucket_min.reserve(3 * NW * sizeof(uint32_t) + 2 * NW * sizeof(uint32_t));
+  uint32_t* occ = t.bucket_min.as<uint32_t>();
+  uint32_t* mark = occ + NW;
+  uint32_t* ends = mark + NW;
+  uint32_t* popc = ends + NW;
+  uint32_t* rank_base = popc + NW;
   t.block_max.reserve((naxes * nblk + 8) * sizeof(uint64_t));
-  t.block_best.reserve((naxes * nblk + 1) * sizeof(GapBest) + sizeof(UnwrapResult));
-  uint64_t* bmin = t.bucket_min.as<uint64_t>();
-  uint64_t* bmax = t.bucket_max.as<uint64_t>();
   uint64_t* blockmax = t.block_max.as<uint64_t>();
-  uint64_t* total = blockmax + naxes * nblk;
+  uint64_t* total = blockmax + naxes * nblk;               // 3 entries
+  unsigned long long* longest = reinterpret_cast<unsigned long long*>(total + 3);  // 3 entries
+  uint32_t* marked = reinterpret_cast<uint32_t*>(total + 6);  // total marked buckets
+  t.block_best.reserve((naxes * nblk + 1) * sizeof(GapBest) + sizeof(UnwrapResult));
   GapBest* best = t.block_best.as<GapBest>();
   UnwrapResult* dres = reinterpret_cast<UnwrapResult*>(best + naxes * nblk + 1);
-  fill_u64(bmin, kEmpty, naxes * B, s);
-  fill_u64(bmax, 0, naxes * B, s);
-  bucket_kernel<<<std::min<unsigned>(grid_for(n, 256), 8192u), 256, 0, s>>>(p, n, a, bmin, bmax);
-  CHUI_LAUNCH_CHECK("bucket_kernel");
-  bucket_blockmax_kernel<<<dim3(static_cast<unsigned>(nblk), naxes), kGapThreads, 0, s>>>(bmax, B, blockmax, nblk);
-  CHUI_LAUNCH_CHECK("bucket_blockmax_kernel");
+  fill_u32(occ, 0u, 3 * NW, s);
+  fill_u64(total, 0ull, 8, s);
+  const unsigned pgrid = std::min<unsigned>(grid_for(n, 256), 8192u);
+  occupancy_kernel<<<pgrid, 256, 0, s>>>(p, n, a, occ);
+  CHUI_LAUNCH_CHECK("occupancy_kernel");
+  occ_blockmax_kernel<<<dim3(static_cast<unsigned>(nblk), naxes), kGapThreads, 0, s>>>(occ, W, blockmax, nblk);
+  CHUI_LAUNCH_CHECK("occ_blockmax_kernel");
   blockmax_scan_kernel<<<naxes, kGapThreads, 0, s>>>(blockmax, nblk, total);
   CHUI_LAUNCH_CHECK("blockmax_scan_kernel");
-  gap_kernel<<<dim3(static_cast<unsigned>(nblk), naxes), kGapThreads, 0, s>>>(bmin, bmax, B, blockmax, total, nblk,
-                                                                              a, best);
-  CHUI_LAUNCH_CHECK("gap_kernel");
+  RunArgs ra{};
+  ra.occ = occ;
+  ra.block_prefix = blockmax;
+  ra.total = total;
+  ra.longest = longest;
+  ra.mark = mark;
+  ra.ends = ends;
+  ra.rank_base = rank_base;
+  ra.block_out = best;
+  ra.nblk = nblk;
+  const dim3 rgrid(static_cast<unsigned>(nblk), naxes);
+  runs_kernel<0><<<rgrid, kGapThreads, 0, s>>>(a, ra);
+  CHUI_LAUNCH_CHECK("runs_kernel<0>");
+  runs_kernel<1><<<rgrid, kGapThreads, 0, s>>>(a, ra);
+  CHUI_LAUNCH_CHECK("runs_kernel<1>");
+  popc_kernel<<<grid_for(NW, 256), 256, 0, s>>>(mark, NW, popc);
+  CHUI_LAUNCH_CHECK("popc_kernel");
+  t.bfs_scan.reserve(scan_scratch_bytes(NW) + 64);
+  exclusive_scan_u32(popc, rank_base, NW, marked, t.bfs_scan.ptr, s);
+  uint32_t hm = 0;
+  CHUI_HIP(hipMemcpyAsync(&hm, marked, sizeof(hm), hipMemcpyDeviceToHost, s));
+  CHUI_HIP(hipStreamSynchronize(s));
+  t.bucket_max.reserve(2 * std::max<uint64_t>(hm, 1) * sizeof(uint64_t));
+  uint64_t* cmin = t.bucket_max.as<uint64_t>();
+  uint64_t* cmax = cmin + std::max<uint64_t>(hm, 1);
+  fill_u64(cmin, kEmpty, hm, s);
+  fill_u64(cmax, 0ull, hm, s);
+  marked_minmax_kernel<<<pgrid, 256, 0, s>>>(p, n, a, mark, rank_base, cmin, cmax);
+  CHUI_LAUNCH_CHECK("marked_minmax_kernel");
+  ra.cmin = cmin;
+  ra.cmax = cmax;
+  runs_kernel<2><<<rgrid, kGapThreads, 0, s>>>(a, ra);
+  CHUI_LAUNCH_CHECK("runs_kernel<2>");
   gap_final_kernel<<<naxes, kGapThreads, 0, s>>>(best, nblk, dres);
   CHUI_LAUNCH_CHECK("gap_final_kernel");
   UnwrapResult h{};
@@ -816,7 +968,6 @@ void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, co
     res.anchor[ax] = h.anchor[ax];
     res.umin[ax] = h.umin[ax];
     res.umax[ax] = h.umax[ax];
-    res.canonical[ax] = canonical[ax];
   }
 }
 

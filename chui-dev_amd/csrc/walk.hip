@@ -394,7 +394,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
   const bool valid = k < P.n_targets;
   const uint32_t tp = valid ? (P.identity_targets ? k : D.tpos[k]) : 0u;
   const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
-  uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t qlen = 0, queued = 0;
   uint32_t nxt = valid ? 0u : P.node_count;
   double px = 0.0, py = 0.0, pz = 0.0;
@@ -546,18 +546,20 @@ template <int SCHEME, bool CUBIC, bool COUNT>
 __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out,
                                                                 unsigned long long* __restrict__ counters) {
   __shared__ float4 win[kWalkWaves][kWindow];
-  __shared__ uint8_t queue[kWalkWaves][kWave * kQueueStride];
+  __shared__ __align__(16) uint8_t queue[kWalkWaves][kWave * kQueueStride];
   __shared__ float4 htab[kHTableCells];
   for (int i = threadIdx.x; i < kHTableCells; i += kWalkThreads) htab[i] = D.htab[i];
   __syncthreads();
   const int w = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   uint8_t* const myq = &queue[w][lane * kQueueStride];
+  // every queue byte always holds a window slot number (< kWindow)
+  for (int i = 0; i < kQueueStride; i += 4) *reinterpret_cast<uint32_t*>(myq + i) = 0u;
   const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
   const bool valid = k < P.n_targets;
   const uint32_t tp = valid ? (P.identity_targets ? k : D.tpos[k]) : 0u;
   const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
-  uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t qlen = 0, queued = 0, wfill = 0;
   uint32_t nxt = valid ? 0u : P.node_count;
   uint32_t px = 0, py = 0, pz = 0;
@@ -569,18 +571,20 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
     pz = t.z;
     if (P.pm_enabled) interp32<SCHEME>(P, D.f32x, D.f32y, D.f32z, px, py, pz, gx, gy, gz);
   }
-  // separations: fixed-point units (CUBIC) or lengths
+  // node tests: fixed-point units (CUBIC) or lengths
   const float s0 = CUBIC ? 1.0f : P.fx_to_len[0];
   const float s1 = CUBIC ? 1.0f : P.fx_to_len[1];
   const float s2 = CUBIC ? 1.0f : P.fx_to_len[2];
-  const float to_len = CUBIC ? P.fx_to_len[0] : 1.0f;          // separation unit -> length
-  const float to_len2 = to_len * to_len;
-  const float inv_len = CUBIC ? 1.0f / P.fx_to_len[0] : 1.0f;  // length -> separation unit
-  const float rc2 = P.rcut2_sep;
+  const float to_len = CUBIC ? P.fx_to_len[0] : 1.0f;          // node separation unit -> length
+  const float inv_len = CUBIC ? 1.0f / P.fx_to_len[0] : 1.0f;  // length -> node separation unit
   const float prune2 = P.prune2_sep;
   const float contain2 = P.contain2_sep;
-  const float eps2 = static_cast<float>(P.eps * P.eps);
   const float theta2 = static_cast<float>(P.theta * P.theta);
+  // pairs: table units (lengths * sqrt(t_to_cell)), so r^2 is the H-table coordinate
+  const float u0 = P.tab_scale[0], u1 = P.tab_scale[1], u2 = P.tab_scale[2];
+  const float rc2 = P.rcut2_tab;
+  const float eps2 = P.eps2_tab;
+  const float c8 = P.c8_tab;
   const float umass = P.uniform_mass_f;
   // wave origin: centre of the valid targets' bounding box in fixed point
   uint32_t org[3];
@@ -599,13 +603,13 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
     }
   }
   const uint32_t ox = org[0], oy = org[1], oz = org[2];
-  const float tx = fx_delta(px, ox, s0), ty = fx_delta(py, oy, s1), tz = fx_delta(pz, oz, s2);
+  const float tx = fx_delta(px, ox, u0), ty = fx_delta(py, oy, u1), tz = fx_delta(pz, oz, u2);
   float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f, cx_ = 0.0f, cy_ = 0.0f, cz_ = 0.0f;
 
   // source -> window record, and window record -> separation from this lane
   auto stage_record = [&]<bool NEAR>(const uint4& sv, float mj) -> float4 {
     if constexpr (NEAR)
-      return make_float4(fx_delta(sv.x, ox, s0), fx_delta(sv.y, oy, s1), fx_delta(sv.z, oz, s2), mj);
+      return make_float4(fx_delta(sv.x, ox, u0), fx_delta(sv.y, oy, u1), fx_delta(sv.z, oz, u2), mj);
     else
       return make_float4(__uint_as_float(sv.x), __uint_as_float(sv.y), __uint_as_float(sv.z), mj);
   };
@@ -615,38 +619,39 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
       dy = sj.y - ty;
       dz = sj.z - tz;
     } else {
-      dx = fx_delta(__float_as_uint(sj.x), px, s0);
-      dy = fx_delta(__float_as_uint(sj.y), py, s1);
-      dz = fx_delta(__float_as_uint(sj.z), pz, s2);
+      dx = fx_delta(__float_as_uint(sj.x), px, u0);
+      dy = fx_delta(__float_as_uint(sj.y), py, u1);
+      dz = fx_delta(__float_as_uint(sj.z), pz, u2);
     }
   };
 
   auto drain = [&]<bool NEAR>() {
     const uint32_t m = wave_max_u32(qlen);
+    if (COUNT && lane == 0) cnt[7] += m;
     float lx = 0.0f, ly = 0.0f, lz = 0.0f;
 #pragma unroll 2
     for (uint32_t q = 0; q < m; ++q) {
       const bool live = q < qlen;
-      const uint32_t sl = live ? static_cast<uint32_t>(myq[q]) : 0u;
+      // entries past qlen are stale slot numbers; read them anyway and
+      // discard their contribution
+      const uint32_t sl = myq[q];
       const float4 sj = win[w][sl];
       float dx, dy, dz;
       separation.template operator()<NEAR>(sj, dx, dy, dz);
-      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) * to_len2;
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));  // = H-table coordinate
       const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
       const float soft = rinv * rinv * rinv;
-      // H(t), t = r^2/(4 a^2): cubic Hermite cell from the LDS table
-      const float u = r2 * P.t_to_cell;
-      const int ci = min(static_cast<int>(u), kHTableCells - 1);
-      const float fu = u - static_cast<float>(ci);
+      const int ci = min(static_cast<int>(r2), kHTableCells - 1);
+      const float fu = __builtin_amdgcn_fractf(r2);
       const float4 c = htab[ci];
       const float hcoef = fmaf(fmaf(fmaf(c.w, fu, c.z), fu, c.y), fu, c.x);
-      const float fr = fmaf(-hcoef, P.c8, soft) * sj.w;
+      const float fr = fmaf(-hcoef, c8, soft) * sj.w;
       const float f = live ? fr : 0.0f;
       lx = fmaf(f, dx, lx);
       ly = fmaf(f, dy, ly);
       lz = fmaf(f, dz, lz);
     }
-    const float gm = static_cast<float>(P.G) * to_len;
+    const float gm = P.force_scale;
     kahan_add(sx_, cx_, lx * gm);
     kahan_add(sy_, cy_, ly * gm);
     kahan_add(sz_, cz_, lz * gm);
@@ -678,6 +683,8 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
         if (COUNT) cnt[4] += keep;
         nxt = keep ? n + 1 : nxt;
         if (__ballot(keep)) {
+          // a lane that does not keep the leaf tests against a negative cutoff
+          const float lim = keep ? rc2 : -1.0f;
           const uint32_t e0 = lk.begin + lk.count;
           for (uint32_t c0 = lk.begin; c0 < e0; c0 += kWindow) {
             const uint32_t cn = e0 - c0 < static_cast<uint32_t>(kWindow) ? e0 - c0 : static_cast<uint32_t>(kWindow);
@@ -695,8 +702,11 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
               float dx, dy, dz;
               separation.template operator()<NEAR>(sj, dx, dy, dz);
               const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-              const bool pair = keep && (c0 + si) != tp && r2 <= rc2;
-              if (COUNT) cnt[2] += pair && r2 == 0.0f;  // coincident pair: counted, zero vector
+              // The target itself has r2 == 0 exactly (same fixed-point value)
+              // and, like any coincident source, is never queued; the counting
+              // variant also excludes it from the coincident-pair count.
+              const bool pair = r2 <= lim;
+              if (COUNT) cnt[2] += pair && r2 == 0.0f && (c0 + si) != tp;
               myq[qlen] = static_cast<uint8_t>(wfill + si);  // slot qlen is free; kept only if appended
               qlen += (pair && r2 != 0.0f) ? 1u : 0u;
             }
@@ -704,25 +714,29 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
           }
         }
       } else if (__ballot(keep)) {
+        // strict acceptance needs 2h/r < 0.08 with r inside r_cut: nodes with
+        // h >= 0.041 r_cut are opened by every lane without the test
         bool take_mp = false;
         float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;
-        if (keep) {
-          const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
-          const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
-          const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= contain2;
-          const NodeMom32 M = D.mom32[n];
-          mfx = fx_delta(M.comx, px, s0) * to_len;
-          mfy = fx_delta(M.comy, py, s1) * to_len;
-          mfz = fx_delta(M.comz, pz, s2) * to_len;
-          const float r2 = mfx * mfx + mfy * mfy + mfz * mfz;
-          const float w2 = 2.0f * g.half;
-          // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
-          const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
-          const float lead = P.criterion == 0 ? w2 : w2 + D.quad32[n].offset;
-          const bool mac = !inside && lead * lead < theta2 * (r2 + 1.0e-30f);
-          take_mp = mac && env_ok && contained && !inside;
-          nxt = take_mp ? n + lk.skip : n + 1;
+        if (g.half < P.accept_half_max) {
+          if (keep) {
+            const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+            const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+            const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= contain2;
+            const NodeMom32 M = D.mom32[n];
+            mfx = fx_delta(M.comx, px, s0) * to_len;
+            mfy = fx_delta(M.comy, py, s1) * to_len;
+            mfz = fx_delta(M.comz, pz, s2) * to_len;
+            const float r2 = mfx * mfx + mfy * mfy + mfz * mfz;
+            const float w2 = 2.0f * g.half;
+            // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
+            const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
+            const float lead = P.criterion == 0 ? w2 : w2 + D.quad32[n].offset;
+            const bool mac = !inside && lead * lead < theta2 * (r2 + 1.0e-30f);
+            take_mp = mac && env_ok && contained && !inside;
+          }
         }
+        if (keep) nxt = take_mp ? n + lk.skip : n + 1;
         if (__ballot(take_mp)) {
           drain.template operator()<NEAR>();
           if (take_mp) {

@@ -44,6 +44,14 @@ struct WalkParams {
   // which the walk uses float offsets from the wave origin:
   // min(2^28, 2^31 - r_cut - 2^24) in units of L_a / 2^32.
   uint32_t fast_half_extent[3];
+  // Mixed walk in table units (lengths times sqrt(t_to_cell), so r^2 is the
+  // H-table coordinate): per-axis scale of fixed-point differences, cutoff^2,
+  // softening^2, the H coefficient c8 / t_to_cell^1.5, the force scale
+  // G * t_to_cell, and the node half-size from which no strict acceptance is
+  // possible (0.041 r_cut).
+  float tab_scale[3];
+  float rcut2_tab, eps2_tab, c8_tab, force_scale;
+  float accept_half_max;
 };
 
 struct WalkOutputs {
@@ -59,8 +67,9 @@ struct WalkOutputs {
 };
 
 // Counters accumulated on the device: visited, pruned, pairs, multipoles,
-// leaves (per target), wave node visits and wave leaf-source slots (per wave).
-constexpr int kWalkCounters = 7;
+// leaves (per target), wave node visits, wave leaf-source slots and wave
+// pair-kernel iterations (per wave).
+constexpr int kWalkCounters = 8;
 
 // Tree-ordered sources: FP64 (x, y, z, m); mixed (qx, qy, qz, float bits of m).
 void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,

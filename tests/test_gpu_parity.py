@@ -77,7 +77,7 @@ def test_morton_keys_and_order_bit_exact(case):
 
 
 @pytest.mark.parametrize("case", ["uniform", "seam_cluster", "two_points", "single", "duplicates", "noncanonical",
-                                  "equal_gaps"])
+                                  "equal_gaps", "dyadic_ties", "bucket_edges", "large_clustered"])
 def test_unwrap_anchor_bit_exact(case):
     rng = np.random.default_rng(11)
     L_box = 10.0
@@ -93,6 +93,13 @@ def test_unwrap_anchor_bit_exact(case):
         v = np.repeat(rng.uniform(0, L_box, 50), 20)
     elif case == "noncanonical":
         v = rng.uniform(-25, 35, 4000)
+    elif case == "dyadic_ties":  # values on a coarse dyadic grid: many equal gaps
+        v = rng.integers(0, 64, 5000) * (L_box / 64)
+    elif case == "bucket_edges":  # every value on a bucket boundary of the binned search
+        v = np.arange(256) * (L_box / 256)
+        v = np.delete(v, [17, 18, 100, 101, 200])
+    elif case == "large_clustered":
+        v = np.mod(np.concatenate([rng.normal(c, 0.05, 40000) for c in (1.0, 4.5, 9.9)]), L_box)
     else:  # a lattice: every gap ties, the smallest anchor wins
         v = (np.arange(64) + 0.5) * (L_box / 64)
         rng.shuffle(v)
