@@ -192,22 +192,10 @@ struct DepositArgs {
   int scheme;
 };
 
-// Particle handled by thread t: lane l of wave w takes particle l * S + w with
-// S = ceil(n / 64). Sources are in tree order, so the 64 lanes of a wave
-// deposit into 64 distant parts of the mesh (no atomic contention on shared
-// cells inside a wave) while each lane still advances through neighbouring
-// particles from one wave to the next.
-__device__ __forceinline__ std::size_t deposit_particle(std::size_t np) {
-  const std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const std::size_t S = (np + kWave - 1) / kWave;
-  const std::size_t w = t / kWave;
-  return w < S ? (t % kWave) * S + w : np;  // threads past 64 S take no particle
-}
-
 __global__ __launch_bounds__(256) void deposit64_kernel(DepositArgs a, const double4* __restrict__ src,
                                                         int uniform, double um, std::size_t np,
                                                         double* __restrict__ rho) {
-  const std::size_t p = deposit_particle(np);
+  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const double4 sp = src[p];
   const double g[3] = {wrap_position(sp.x, a.L[0]) * a.inv_h[0], wrap_position(sp.y, a.L[1]) * a.inv_h[1],
@@ -250,7 +238,7 @@ __global__ __launch_bounds__(256) void deposit64_kernel(DepositArgs a, const dou
 
 __global__ __launch_bounds__(256) void deposit32_kernel(DepositArgs a, const uint4* __restrict__ src, int uniform,
                                                         float um, std::size_t np, float* __restrict__ rho) {
-  const std::size_t p = deposit_particle(np);
+  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const uint4 sp = src[p];
   const uint32_t q[3] = {sp.x, sp.y, sp.z};
@@ -703,9 +691,6 @@ void reserve(PmWorkspace& w, const PmGeometry& g) {
   w.elem_bytes = sizeof(T);
 }
 
-// Threads covering every particle under deposit_particle's mapping.
-std::size_t deposit_threads(std::size_t n) { return kWave * ((n + kWave - 1) / kWave); }
-
 DepositArgs deposit_args(const PmGeometry& g, const PmOptions& o) {
   DepositArgs a{};
   for (int ax = 0; ax < 3; ++ax) {
@@ -728,8 +713,7 @@ void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, cons
   double* rho = w.buf[0].as<double>();
   CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(double), s));
   if (n > 0) {
-    deposit64_kernel<<<grid_for(deposit_threads(n), 256), 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, n,
-                                                                       rho);
+    deposit64_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, n, rho);
     CHUI_LAUNCH_CHECK("deposit64_kernel");
   }
   solve_spectral<double>(w, g, o, tb, s);
@@ -743,8 +727,7 @@ void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, con
   float* rho = w.buf[0].as<float>();
   CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(float), s));
   if (n > 0) {
-    deposit32_kernel<<<grid_for(deposit_threads(n), 256), 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, n,
-                                                                       rho);
+    deposit32_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, n, rho);
     CHUI_LAUNCH_CHECK("deposit32_kernel");
   }
   solve_spectral<float>(w, g, o, tb, s);
