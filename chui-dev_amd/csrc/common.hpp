@@ -117,6 +117,16 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
 
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+  v |= CHUI_DPP(v, 0xb1, 0xf, 0u);
+  v |= CHUI_DPP(v, 0x4e, 0xf, 0u);
+  v |= CHUI_DPP(v, 0x124, 0xf, 0u);
+  v |= CHUI_DPP(v, 0x128, 0xf, 0u);
+  v |= CHUI_DPP(v, 0x142, 0xa, 0u);
+  v |= CHUI_DPP(v, 0x143, 0xc, 0u);
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t lane) {
   return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
 }

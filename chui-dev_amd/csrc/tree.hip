@@ -115,7 +115,32 @@ __global__ __launch_bounds__(256) void source_stats_kernel(Axes3 p, const double
   bc = wave_max(bc);
   bm = wave_max(bm);
   nu = wave_max(nu);
+  // block reduction, then one set of atomics per block (few blocks)
+  __shared__ uint64_t slo[4][3], shi[4][3];
+  __shared__ uint32_t sflag[4][3];
+  const int w = threadIdx.x / kWave;
   if (lane_id() == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      slo[w][a] = lo[a];
+      shi[w][a] = hi[a];
+    }
+    sflag[w][0] = bc;
+    sflag[w][1] = bm;
+    sflag[w][2] = nu;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int v = 1; v < static_cast<int>(blockDim.x) / kWave; ++v) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = slo[v][a] < lo[a] ? slo[v][a] : lo[a];
+        hi[a] = shi[v][a] > hi[a] ? shi[v][a] : hi[a];
+      }
+      bc |= sflag[v][0];
+      bm |= sflag[v][1];
+      nu |= sflag[v][2];
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (lo[a] != kEmpty) atomicMin(reinterpret_cast<unsigned long long*>(&st->lo[a]), lo[a]);
@@ -155,13 +180,30 @@ __device__ __forceinline__ uint64_t bucket_of(double v, const BucketArgs& a, int
   return b >= a.B ? a.B - 1 : b;
 }
 
+// Neighbouring input values often share a bitmap word (lattice-ordered inputs
+// do on two of three axes): the lanes whose word equals lane 0's combine their
+// bits with one DPP reduction and lane 0 sets them with a single atomic.
 __global__ __launch_bounds__(256) void occupancy_kernel(Axes3 p, std::size_t n, BucketArgs a,
                                                         uint32_t* __restrict__ occ) {
-  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  for (std::size_t base = static_cast<std::size_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~(kWave - 1));
+       base < n; base += stride) {  // wave-uniform loop: all lanes take part in the reductions
+    const std::size_t i = base + lane_id();
+    const bool valid = i < n;  // valid lanes form a prefix, so lane 0 is valid
     for (int ax = 0; ax < a.naxes; ++ax) {
-      const uint64_t b = bucket_of(wrap_coord(p.p[ax][i], a.L[ax], a.canonical[ax]), a, ax);
-      atomicOr(&occ[ax * a.W + (b >> 5)], 1u << (b & 31));
+      uint64_t wi = 0;
+      uint32_t bit = 0;
+      if (valid) {
+        const uint64_t b = bucket_of(wrap_coord(p.p[ax][i], a.L[ax], a.canonical[ax]), a, ax);
+        wi = ax * a.W + (b >> 5);
+        bit = 1u << (b & 31);
+      }
+      const uint64_t w0 = (static_cast<uint64_t>(readlane_u32(static_cast<uint32_t>(wi >> 32), 0)) << 32) |
+                          readlane_u32(static_cast<uint32_t>(wi), 0);
+      const bool same = valid && wi == w0;
+      const uint32_t grp = wave_or_u32(same ? bit : 0u);
+      if (lane_id() == 0) atomicOr(&occ[w0], grp);
+      if (valid && !same) atomicOr(&occ[wi], bit);
     }
   }
 }
@@ -864,7 +906,7 @@ SourceStats compute_stats(Tree& t, Axes3 p, const double* m, std::size_t n, hipS
   stats_init_kernel<<<1, 1, 0, s>>>(st);
   CHUI_LAUNCH_CHECK("stats_init_kernel");
   if (n > 0) {
-    source_stats_kernel<<<std::min<unsigned>(grid_for(n, 256), 2048u), 256, 0, s>>>(p, m, n, st);
+    source_stats_kernel<<<std::min<unsigned>(grid_for(n, 256), 1024u), 256, 0, s>>>(p, m, n, st);
     CHUI_LAUNCH_CHECK("source_stats_kernel");
   }
   SourceStats h{};
