@@ -543,8 +543,12 @@ constexpr int kWindow = 64;
 constexpr int kQueueStride = kWindow + 4;  // bytes per lane row (bank spread)
 
 template <int SCHEME, bool CUBIC, bool COUNT>
-__global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out,
-                                                                unsigned long long* __restrict__ counters) {
+__global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : (SCHEME == 0 ? 6 : 5))))
+void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long long* __restrict__ counters,
+                    const NodeLink* __restrict__ node_link, const NodeGeo32* __restrict__ node_geo,
+                    const NodeMom32* __restrict__ node_mom, const NodeQuad32* __restrict__ node_quad) {
+  // node records are read-only for the kernel's lifetime and indexed by the
+  // wave-uniform node number, so they come through the scalar cache
   __shared__ float4 win[kWalkWaves][kWindow];
   __shared__ __align__(16) uint8_t queue[kWalkWaves][kWave * kQueueStride];
   __shared__ float4 htab[kHTableCells];
@@ -664,8 +668,8 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
     uint32_t n = wave_min_u32(nxt);
     while (n < P.node_count) {
       if (COUNT && lane == 0) ++cnt[5];
-      const NodeLink lk = D.link[n];
-      const NodeGeo32 g = D.geo32[n];
+      const NodeLink lk = node_link[n];
+      const NodeGeo32 g = node_geo[n];
       const bool act = nxt == n;
       if (!__ballot(act)) {
         if (lane == 0) atomicOr(&counters[kWalkCounters], 1ull);
@@ -723,7 +727,7 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
             const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
             const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
             const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= contain2;
-            const NodeMom32 M = D.mom32[n];
+            const NodeMom32 M = node_mom[n];
             mfx = fx_delta(M.comx, px, s0) * to_len;
             mfy = fx_delta(M.comy, py, s1) * to_len;
             mfz = fx_delta(M.comz, pz, s2) * to_len;
@@ -731,7 +735,7 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
             const float w2 = 2.0f * g.half;
             // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
             const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
-            const float lead = P.criterion == 0 ? w2 : w2 + D.quad32[n].offset;
+            const float lead = P.criterion == 0 ? w2 : w2 + node_quad[n].offset;
             const bool mac = !inside && lead * lead < theta2 * (r2 + 1.0e-30f);
             take_mp = mac && env_ok && contained && !inside;
           }
@@ -742,7 +746,7 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_kernel(WalkParams P, Wal
           if (take_mp) {
             if (COUNT) ++cnt[3];
             float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-            node_accel32(D.mom32[n], D.quad32[n], mfx, mfy, mfz, P, cx, cy, cz);
+            node_accel32(node_mom[n], node_quad[n], mfx, mfy, mfz, P, cx, cy, cz);
             kahan_add(sx_, cx_, cx);
             kahan_add(sy_, cy_, cy);
             kahan_add(sz_, cz_, cz);
@@ -812,11 +816,11 @@ template <int SCHEME, bool CUBIC>
 void launchmix(const WalkParams& p, const WalkData& d, const WalkOutputs& out, unsigned long long* counters,
                hipStream_t s) {
   if (p.count)
-    walkmix_kernel<SCHEME, CUBIC, true><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(p, d, out,
-                                                                                                     counters);
+    walkmix_kernel<SCHEME, CUBIC, true><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
+        p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32);
   else
-    walkmix_kernel<SCHEME, CUBIC, false><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(p, d, out,
-                                                                                                      counters);
+    walkmix_kernel<SCHEME, CUBIC, false><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
+        p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32);
   CHUI_LAUNCH_CHECK("walkmix_kernel");
 }
 
