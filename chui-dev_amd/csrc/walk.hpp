@@ -9,7 +9,7 @@
 
 namespace chui {
 
-constexpr int kHTableCells = 256;  // cubic cells of the Gaussian coefficient H(t)
+constexpr int kHTableCells = 128;  // cubic cells of the Gaussian coefficient H(t)
 
 struct WalkParams {
   double L[3];
