@@ -426,7 +426,10 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       const double sq = std::sqrt(t2c);
       for (int a = 0; a < 3; ++a) wp.tab_scale[a] = static_cast<float>(o->box_size[a] / 4294967296.0 * sq);
       wp.rcut2_tab = static_cast<float>(wp.rcut2 * t2c);
-      wp.eps2_tab = static_cast<float>(wp.eps * wp.eps * t2c);
+      // floor 1e-12 (table units, r^2 <= 128 for every queued pair): a queued
+      // coincident pair (r^2 = 0) keeps a finite softened term (< 1e18 times
+      // its mass), so its zero separation gives a zero force
+      wp.eps2_tab = static_cast<float>(std::max(wp.eps * wp.eps * t2c, 1.0e-12));
       wp.c8_tab = static_cast<float>(0.125 / (sp.a * sp.a * sp.a) / (t2c * sq));
       wp.force_scale = static_cast<float>(wp.G * t2c);
       wp.accept_half_max = static_cast<float>(0.041 * sp.rcut);

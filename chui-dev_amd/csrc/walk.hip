@@ -718,13 +718,19 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
               float dx, dy, dz;
               separation.template operator()<NEAR>(sj, dx, dy, dz);
               const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-              // The target itself has r2 == 0 exactly (same fixed-point value)
-              // and, like any coincident source, is never queued; the counting
-              // variant also excludes it from the coincident-pair count.
+              // The target itself has r2 == 0 exactly (same fixed-point value),
+              // as does any coincident source. The counting variant keeps them
+              // out of the queue and counts the coincident ones as the
+              // reference does; the timed variant queues them, where they add
+              // exactly zero (dx = dy = dz = 0 against a finite softened term).
               const bool pair = r2 <= lim;
-              if (COUNT) cnt[2] += pair && r2 == 0.0f && (c0 + si) != tp;
               myq[qlen] = static_cast<uint8_t>(wfill + si);  // slot qlen is free; kept only if appended
-              qlen += (pair && r2 != 0.0f) ? 1u : 0u;
+              if constexpr (COUNT) {
+                cnt[2] += pair && r2 == 0.0f && (c0 + si) != tp;
+                qlen += (pair && r2 != 0.0f) ? 1u : 0u;
+              } else {
+                qlen += pair ? 1u : 0u;
+              }
             }
             wfill += cn;
           }
