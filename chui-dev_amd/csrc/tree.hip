@@ -1071,8 +1071,16 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
   uint32_t* order = t.order.as<uint32_t>();
   path_key_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, pa, keys, order);
   CHUI_LAUNCH_CHECK("path_key_kernel");
-  radix_sort_pairs(keys, order, t.keys_alt.as<uint64_t>(), t.vals_alt.as<uint32_t>(), n, 0, 3 * kPathLevels,
-                   t.sort_scratch.ptr, s);
+  // Sort by the bits of levels 1..16 only (6 digit passes instead of 8). This
+  // is the reference's partition order for every node up to depth 16, and the
+  // order inside a leaf is rebuilt below by (Morton key, row), so it suffices
+  // unless some depth-16 node still splits; the discovery then restarts after
+  // sorting by all 21 levels.
+  constexpr int kPartialSortLevels = 16;
+  constexpr int kPartialSortLow = 3 * (kPathLevels - kPartialSortLevels);
+  radix_sort_pairs(keys, order, t.keys_alt.as<uint64_t>(), t.vals_alt.as<uint32_t>(), n, kPartialSortLow,
+                   3 * kPathLevels, t.sort_scratch.ptr, s);
+  bool partial = true;
 
   // Level-synchronous discovery.
   const std::size_t initial_cap = std::max<std::size_t>(1024, n / 4 + 64);
@@ -1089,12 +1097,14 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
   };
   ensure(initial_cap, 0);
   t.flags.reserve(16);
+restart_discovery:
   fill_u32(t.flags.as<uint32_t>(), 0u, 4, s);
   root_init_kernel<<<1, 1, 0, s>>>(t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
                                    t.bfs_geo.as<double4>(), static_cast<uint32_t>(n), t.root_centre[0],
                                    t.root_centre[1], t.root_centre[2], t.root_half);
   CHUI_LAUNCH_CHECK("root_init_kernel");
   t.level_start = {0, 1};
+  t.max_depth = 0;
   for (int d = 0;; ++d) {
     const std::size_t lb = t.level_start[d], le = t.level_start[d + 1];
     const uint32_t cnt = static_cast<uint32_t>(le - lb);
@@ -1118,6 +1128,15 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
     }
     const uint32_t total_children = hflags[1];
     if (total_children == 0) break;
+    if (partial && d + 1 > kPartialSortLevels) {
+      // A depth-16 node splits (whether it does depends on its count, not on
+      // the unsorted low digits). Sort by every level and start over; the sort
+      // is stable, so equal keys keep their row order.
+      radix_sort_pairs(keys, order, t.keys_alt.as<uint64_t>(), t.vals_alt.as<uint32_t>(), n, 0, 3 * kPathLevels,
+                       t.sort_scratch.ptr, s);
+      partial = false;
+      goto restart_discovery;
+    }
     ensure(le + total_children, le);
     level_emit_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(la, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(),
                                                          t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(), keys,

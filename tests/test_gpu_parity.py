@@ -6,6 +6,8 @@ written in each test.
 """
 from __future__ import annotations
 
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -116,6 +118,16 @@ def _tree_cases():
     yield "config0_16", ics.config(0, n_side=16)
     w = ics.config(1, n_side=32)
     yield "config1_32", w
+    # 40 particles within ~2^-19 of the box around one point: leaves at depth
+    # 18, below the 16 levels the first key sort covers, so the build must
+    # fall back to the full-width sort and still match bit for bit.
+    d = ics.config(0, n_side=16)
+    rng = np.random.default_rng(7)
+    x, y, z = d.x.copy(), d.y.copy(), d.z.copy()
+    centre = np.array([1.234567, 3.456789, 0.987654])
+    tight = centre[:, None] + rng.normal(0.0, d.box * 2.0**-19, (3, 40))
+    x[:40], y[:40], z[:40] = tight
+    yield "deep_cluster", dataclasses.replace(d, x=x, y=y, z=z)
 
 
 @pytest.mark.parametrize("name,w", list(_tree_cases()))
