@@ -202,7 +202,7 @@ def main():
         "counters": {k: diag[k] for k in ("node_count", "max_depth", "direct_pair_evaluations",
                                           "accepted_internal_multipoles", "accepted_leaves", "visited_nodes",
                                           "wave_node_visits", "wave_leaf_source_slots",
-                                          "wave_drain_iterations")},
+                                          "wave_sum_flushes")},
     }
     coord.close()
     print(json.dumps(line), flush=True)

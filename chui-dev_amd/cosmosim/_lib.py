@@ -115,7 +115,7 @@ class Diagnostics(C.Structure):
         ("ms_walk", C.c_double),
         ("wave_node_visits", C.c_uint64),
         ("wave_leaf_source_slots", C.c_uint64),
-        ("wave_drain_iterations", C.c_uint64),
+        ("wave_sum_flushes", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -426,12 +426,13 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       const double sq = std::sqrt(t2c);
       for (int a = 0; a < 3; ++a) wp.tab_scale[a] = static_cast<float>(o->box_size[a] / 4294967296.0 * sq);
       wp.rcut2_tab = static_cast<float>(wp.rcut2 * t2c);
-      // floor 1e-12 (table units, r^2 <= 128 for every queued pair): a queued
+      // floor 1e-12 (table units, r^2 <= 128 for every pair inside r_cut): a
       // coincident pair (r^2 = 0) keeps a finite softened term (< 1e18 times
       // its mass), so its zero separation gives a zero force
       wp.eps2_tab = static_cast<float>(std::max(wp.eps * wp.eps * t2c, 1.0e-12));
       wp.c8_tab = static_cast<float>(0.125 / (sp.a * sp.a * sp.a) / (t2c * sq));
-      wp.force_scale = static_cast<float>(wp.G * t2c);
+      // with uniform masses the pair kernel leaves the mass to this factor
+      wp.force_scale = static_cast<float>(wp.G * t2c * (uniform ? um : 1.0));
       wp.accept_half_max = static_cast<float>(0.041 * sp.rcut);
       if (c->htab_key[0] != t_max || c->htab_key[1] != sp.a || !c->htab.ptr) {
         c->htab.reserve(tab.size() * sizeof(float));
@@ -471,7 +472,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     diag->accepted_leaves = cnt[4];
     diag->wave_node_visits = cnt[5];
     diag->wave_leaf_source_slots = cnt[6];
-    diag->wave_drain_iterations = cnt[7];
+    diag->wave_sum_flushes = cnt[7];
     diag->split_scale_comoving = sp.a;
     diag->cutoff_radius_comoving = sp.rcut;
     diag->mesh_spacing_comoving = sp.h;

@@ -523,13 +523,15 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
 // scale factor for all axes) and only the pair kernel converts to lengths.
 // COUNT: keep the per-target counters (diagnostic launches only).
 //
-// Each wave keeps a window of up to kWindow leaf sources in LDS. A kept leaf is
-// staged with one coalesced load per lane; each lane then tests the window's
-// new sources with broadcast LDS reads and appends the window slot (8 bits) of
-// every accepted pair to its own LDS queue. The queue can hold a whole window,
-// so it is drained only when the window is full, before an accepted multipole
-// (keeping each lane's reference summation order) and at the end; the drain
-// reads sources from the window, not from global memory.
+// A leaf that any lane keeps is staged in the wave's LDS window with one
+// coalesced load (one source per lane). Every lane then evaluates the residual
+// pair kernel against each staged source in turn, reading it with a broadcast
+// LDS load, and keeps the term when the pair lies inside r_cut (a lane that
+// pruned the leaf compares against a negative cutoff). A leaf's pair terms
+// are summed in registers and added to a per-lane partial sum in LDS, which is
+// folded into the lane's Kahan-compensated totals (also in LDS) every kWindow
+// sources. Keeping these sums out of the node loop's registers spares the
+// loop-carried copies the compiler would otherwise make on every node.
 //
 // Window format. Most waves hold 64 consecutive tree-order targets inside a
 // small region: their window stores float offsets from the wave origin O (the
@@ -540,7 +542,6 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
 // Waves that straddle a jump of the space-filling order (targets far apart)
 // store raw fixed-point coordinates and form int32(q_s - q_t) per pair.
 constexpr int kWindow = 64;
-constexpr int kQueueStride = kWindow + 4;  // bytes per lane row (bank spread)
 
 template <int SCHEME, bool CUBIC, bool COUNT>
 __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : (SCHEME == 0 ? 6 : 5))))
@@ -550,24 +551,19 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   // node records are read-only for the kernel's lifetime and indexed by the
   // wave-uniform node number, so they come through the scalar cache
   __shared__ float4 win[kWalkWaves][kWindow];
-  __shared__ __align__(16) uint8_t queue[kWalkWaves][kWave * kQueueStride];
   __shared__ float4 htab[kHTableCells];
-  // Kahan compensations of the per-lane force sums, touched only at drains
-  // and multipoles (keeps the walk loop's register state small)
-  __shared__ float kcomp[kWalkWaves][3][kWave];
+  // per lane: force totals [0..2], their Kahan compensations [3..5] and the
+  // partial pair sums since the last fold [6..8]
+  __shared__ float ksum[kWalkWaves][9][kWave];
   for (int i = threadIdx.x; i < kHTableCells; i += kWalkThreads) htab[i] = D.htab[i];
   __syncthreads();
   const int w = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
-  uint8_t* const myq = &queue[w][lane * kQueueStride];
-  // every queue byte always holds a window slot number (< kWindow)
-  for (int i = 0; i < kQueueStride; i += 4) *reinterpret_cast<uint32_t*>(myq + i) = 0u;
   const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
   const bool valid = k < P.n_targets;
   const uint32_t tp = valid ? (P.identity_targets ? k : D.tpos[k]) : 0u;
   const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
   uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
-  uint32_t qlen = 0, queued = 0, wfill = 0;
   uint32_t nxt = valid ? 0u : P.node_count;
   uint32_t px = 0, py = 0, pz = 0;
   float gx = 0.0f, gy = 0.0f, gz = 0.0f;
@@ -592,7 +588,6 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   const float rc2 = P.rcut2_tab;
   const float eps2 = P.eps2_tab;
   const float c8 = P.c8_tab;
-  const float umass = P.uniform_mass_f;
   // wave origin: centre of the valid targets' bounding box in fixed point
   uint32_t org[3];
   bool near = true;
@@ -611,17 +606,17 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   }
   const uint32_t ox = org[0], oy = org[1], oz = org[2];
   const float tx = fx_delta(px, ox, u0), ty = fx_delta(py, oy, u1), tz = fx_delta(pz, oz, u2);
-  float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) kcomp[w][c][lane] = 0.0f;
+  for (int c = 0; c < 9; ++c) ksum[w][c][lane] = 0.0f;
   auto accumulate = [&](float vx, float vy, float vz) {
-    float cx = kcomp[w][0][lane], cy = kcomp[w][1][lane], cz = kcomp[w][2][lane];
-    kahan_add(sx_, cx, vx);
-    kahan_add(sy_, cy, vy);
-    kahan_add(sz_, cz, vz);
-    kcomp[w][0][lane] = cx;
-    kcomp[w][1][lane] = cy;
-    kcomp[w][2][lane] = cz;
+    const float v[3] = {vx, vy, vz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float sum = ksum[w][c][lane], comp = ksum[w][3 + c][lane];
+      kahan_add(sum, comp, v[c]);
+      ksum[w][c][lane] = sum;
+      ksum[w][3 + c][lane] = comp;
+    }
   };
 
   // source -> window record, and window record -> separation from this lane
@@ -643,37 +638,48 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
     }
   };
 
-  auto drain = [&]<bool NEAR>() {
-    const uint32_t m = wave_max_u32(qlen);
-    if (COUNT && lane == 0) cnt[7] += m;
+  uint32_t pend = 0;  // wave-uniform count of sources in the partial sums
+  auto fold = [&]() {
+    const float gm = P.force_scale;
+    accumulate(ksum[w][6][lane] * gm, ksum[w][7][lane] * gm, ksum[w][8][lane] * gm);
+#pragma unroll
+    for (int c = 6; c < 9; ++c) ksum[w][c][lane] = 0.0f;
+    pend = 0;
+    if (COUNT && lane == 0) ++cnt[7];
+  };
+
+  // the residual pair kernel against the cn staged sources; UMASS: every
+  // source has the uniform mass, which the force scale already carries
+  auto pairs = [&]<bool NEAR, bool UMASS>(uint32_t c0, uint32_t cn, float lim) {
     float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-#pragma unroll 2
-    for (uint32_t q = 0; q < m; ++q) {
-      const bool live = q < qlen;
-      // entries past qlen are stale slot numbers; read them anyway and
-      // discard their contribution
-      const uint32_t sl = myq[q];
-      const float4 sj = win[w][sl];
+#pragma unroll 4
+    for (uint32_t si = 0; si < cn; ++si) {
+      const float4 sj = win[w][si];
       float dx, dy, dz;
       separation.template operator()<NEAR>(sj, dx, dy, dz);
       const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));  // = H-table coordinate
       const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
       const float soft = rinv * rinv * rinv;
-      const int ci = min(static_cast<int>(r2), kHTableCells - 1);
+      const int ci = static_cast<int>(fminf(r2, static_cast<float>(kHTableCells - 1)));
       const float fu = __builtin_amdgcn_fractf(r2);
       const float4 c = htab[ci];
       const float hcoef = fmaf(fmaf(fmaf(c.w, fu, c.z), fu, c.y), fu, c.x);
-      const float fr = fmaf(-hcoef, c8, soft) * sj.w;
-      const float f = live ? fr : 0.0f;
+      float fr = fmaf(-hcoef, c8, soft);
+      if constexpr (!UMASS) fr *= sj.w;
+      // The target itself has r2 == 0 exactly (same fixed-point value), as
+      // does any coincident source: its term is exactly zero (dx = dy = dz = 0
+      // against a finite softened value), so no self test is needed. The
+      // counting variant counts coincident sources as the reference does.
+      const bool pair = r2 <= lim;
+      if constexpr (COUNT) cnt[2] += pair && (r2 != 0.0f || (c0 + si) != tp);
+      const float f = pair ? fr : 0.0f;
       lx = fmaf(f, dx, lx);
       ly = fmaf(f, dy, ly);
       lz = fmaf(f, dz, lz);
     }
-    const float gm = P.force_scale;
-    accumulate(lx * gm, ly * gm, lz * gm);
-    queued += qlen;
-    qlen = 0;
-    wfill = 0;
+    ksum[w][6][lane] += lx;
+    ksum[w][7][lane] += ly;
+    ksum[w][8][lane] += lz;
   };
 
   auto walk = [&]<bool NEAR>() {
@@ -694,48 +700,35 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
       const float ex = fmaxf(0.0f, ax_ - h), ey = fmaxf(0.0f, ay_ - h), ez = fmaxf(0.0f, az_ - h);
       const bool keep = act && !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
       if (COUNT) cnt[1] += act && !keep;
-      nxt = (act && !keep) ? n + lk.skip : nxt;
       if (lk.child_count == 0) {
+        // a leaf's subtree is itself: every waiting lane moves on to n + 1,
+        // so that is the wave's next node too
         if (COUNT) cnt[4] += keep;
-        nxt = keep ? n + 1 : nxt;
+        nxt = act ? n + 1 : nxt;
         if (__ballot(keep)) {
-          // a lane that does not keep the leaf tests against a negative cutoff
           const float lim = keep ? rc2 : -1.0f;
           const uint32_t e0 = lk.begin + lk.count;
           for (uint32_t c0 = lk.begin; c0 < e0; c0 += kWindow) {
             const uint32_t cn = e0 - c0 < static_cast<uint32_t>(kWindow) ? e0 - c0 : static_cast<uint32_t>(kWindow);
             if (COUNT && lane == 0) cnt[6] += cn;
-            if (wfill + cn > static_cast<uint32_t>(kWindow)) drain.template operator()<NEAR>();
+            if (pend + cn > static_cast<uint32_t>(kWindow)) fold();
+            pend += cn;
             {
               const uint4 sv = D.src32[c0 + (static_cast<uint32_t>(lane) < cn ? lane : 0)];
-              const float mj = D.uniform_mass ? umass : __uint_as_float(sv.w);
-              if (static_cast<uint32_t>(lane) < cn) win[w][wfill + lane] = stage_record.template operator()<NEAR>(sv, mj);
+              const float mj = D.uniform_mass ? 1.0f : __uint_as_float(sv.w);
+              if (static_cast<uint32_t>(lane) < cn) win[w][lane] = stage_record.template operator()<NEAR>(sv, mj);
             }
             __builtin_amdgcn_wave_barrier();
-#pragma unroll 4
-            for (uint32_t si = 0; si < cn; ++si) {
-              const float4 sj = win[w][wfill + si];
-              float dx, dy, dz;
-              separation.template operator()<NEAR>(sj, dx, dy, dz);
-              const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-              // The target itself has r2 == 0 exactly (same fixed-point value),
-              // as does any coincident source. The counting variant keeps them
-              // out of the queue and counts the coincident ones as the
-              // reference does; the timed variant queues them, where they add
-              // exactly zero (dx = dy = dz = 0 against a finite softened term).
-              const bool pair = r2 <= lim;
-              myq[qlen] = static_cast<uint8_t>(wfill + si);  // slot qlen is free; kept only if appended
-              if constexpr (COUNT) {
-                cnt[2] += pair && r2 == 0.0f && (c0 + si) != tp;
-                qlen += (pair && r2 != 0.0f) ? 1u : 0u;
-              } else {
-                qlen += pair ? 1u : 0u;
-              }
-            }
-            wfill += cn;
+            if (D.uniform_mass) pairs.template operator()<NEAR, true>(c0, cn, lim);
+            else pairs.template operator()<NEAR, false>(c0, cn, lim);
+            __builtin_amdgcn_wave_barrier();
           }
         }
-      } else if (__ballot(keep)) {
+        ++n;
+        continue;
+      }
+      nxt = (act && !keep) ? n + lk.skip : nxt;
+      if (__ballot(keep)) {
         // strict acceptance needs 2h/r < 0.08 with r inside r_cut: nodes with
         // h >= 0.041 r_cut are opened by every lane without the test
         bool take_mp = false;
@@ -759,29 +752,25 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
           }
         }
         if (keep) nxt = take_mp ? n + lk.skip : n + 1;
-        if (__ballot(take_mp)) {
-          drain.template operator()<NEAR>();
-          if (take_mp) {
-            if (COUNT) ++cnt[3];
-            float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-            node_accel32(node_mom[n], node_quad[n], mfx, mfy, mfz, P, cx, cy, cz);
-            accumulate(cx, cy, cz);
-          }
+        if (__ballot(take_mp) && take_mp) {
+          if (COUNT) ++cnt[3];
+          float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+          node_accel32(node_mom[n], node_quad[n], mfx, mfy, mfz, P, cx, cy, cz);
+          accumulate(cx, cy, cz);
         }
       }
       n = next_node(n, nxt);
     }
-    drain.template operator()<NEAR>();
+    fold();
   };
   if (near) walk.template operator()<true>();
   else walk.template operator()<false>();
 
-  if (COUNT) {
-    cnt[2] += queued;
-    flush_counters(counters, cnt);
-  }
+  if (COUNT) flush_counters(counters, cnt);
   if (!valid) return;
-  const float ax = sx_ - kcomp[w][0][lane], ay = sy_ - kcomp[w][1][lane], az = sz_ - kcomp[w][2][lane];
+  const float ax = ksum[w][0][lane] - ksum[w][3][lane];
+  const float ay = ksum[w][1][lane] - ksum[w][4][lane];
+  const float az = ksum[w][2][lane] - ksum[w][5][lane];
   static_cast<float*>(out.ax)[slot] = gx + ax;
   static_cast<float*>(out.ay)[slot] = gy + ay;
   static_cast<float*>(out.az)[slot] = gz + az;

@@ -142,9 +142,9 @@ typedef struct chui_treepm_diagnostics {
    * the 64 lanes' walks) and leaf source slots streamed per wave */
   uint64_t wave_node_visits;
   uint64_t wave_leaf_source_slots;
-  /* pair-kernel iterations per wave (mixed precision; each iteration serves
-   * the 64 lanes' next queued pairs at once) */
-  uint64_t wave_drain_iterations;
+  /* mixed precision: times a wave folded its register pair sums (at most 64
+   * sources since the previous fold) into the lanes' compensated totals */
+  uint64_t wave_sum_flushes;
 } chui_treepm_diagnostics;
 
 typedef struct chui_treepm_context chui_treepm_context;
