@@ -724,23 +724,47 @@ __global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, Axes3 p, cons
                                                         const uint32_t* __restrict__ nchild,
                                                         const uint32_t* __restrict__ first_child,
                                                         const double4* __restrict__ geo, double4* __restrict__ mom,
-                                                        double* __restrict__ quad /* 8 per node */) {
+                                                        double* __restrict__ quad /* 8 per node */,
+                                                        double* __restrict__ box /* 6 per node or null */) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.cnt) return;
   const uint32_t node = a.lb + k;
   const double4 g = geo[node];
   double* q = quad + 8ull * node;
   const uint32_t c = nchild[node];
+  // tight bounding box of the node's particles (unwrapped): lo[3], hi[3]
+  double lo[3] = {g.x, g.y, g.z}, hi[3] = {g.x, g.y, g.z};
   if (c == 0) {
     const uint32_t b = nb[node], e = b + nc[node];
     double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+    if (e > b) {
+      for (int ax = 0; ax < 3; ++ax) {
+        lo[ax] = INFINITY;
+        hi[ax] = -INFINITY;
+      }
+    }
     for (uint32_t i = b; i < e; ++i) {
       const uint32_t r = order[i];
       const double mm = m[r];
+      const double ux = unwrap_coord(p.p[0][r], a.u.L[0], a.u.canonical[0], a.u.anchor[0]);
+      const double uy = unwrap_coord(p.p[1][r], a.u.L[1], a.u.canonical[1], a.u.anchor[1]);
+      const double uz = unwrap_coord(p.p[2][r], a.u.L[2], a.u.canonical[2], a.u.anchor[2]);
       tm += mm;
-      wx += mm * unwrap_coord(p.p[0][r], a.u.L[0], a.u.canonical[0], a.u.anchor[0]);
-      wy += mm * unwrap_coord(p.p[1][r], a.u.L[1], a.u.canonical[1], a.u.anchor[1]);
-      wz += mm * unwrap_coord(p.p[2][r], a.u.L[2], a.u.canonical[2], a.u.anchor[2]);
+      wx += mm * ux;
+      wy += mm * uy;
+      wz += mm * uz;
+      lo[0] = fmin(lo[0], ux);
+      lo[1] = fmin(lo[1], uy);
+      lo[2] = fmin(lo[2], uz);
+      hi[0] = fmax(hi[0], ux);
+      hi[1] = fmax(hi[1], uy);
+      hi[2] = fmax(hi[2], uz);
+    }
+    if (box) {
+      for (int ax = 0; ax < 3; ++ax) {
+        box[6ull * node + ax] = lo[ax];
+        box[6ull * node + 3 + ax] = hi[ax];
+      }
     }
     double4 M = make_double4(g.x, g.y, g.z, tm);
     if (tm > 0.0) {
@@ -788,6 +812,23 @@ __global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, Axes3 p, cons
     wy += cm.w * cm.y;
     wz += cm.w * cm.z;
   }
+  if (box) {
+    for (int ax = 0; ax < 3; ++ax) {
+      lo[ax] = INFINITY;
+      hi[ax] = -INFINITY;
+    }
+    for (uint32_t j = 0; j < c; ++j) {
+      const double* cb = box + 6ull * (fc + j);
+      for (int ax = 0; ax < 3; ++ax) {
+        lo[ax] = fmin(lo[ax], cb[ax]);
+        hi[ax] = fmax(hi[ax], cb[3 + ax]);
+      }
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+      box[6ull * node + ax] = lo[ax];
+      box[6ull * node + 3 + ax] = hi[ax];
+    }
+  }
   double4 M = make_double4(g.x, g.y, g.z, tm);
   if (tm > 0.0) {
     M.x = wx / tm;
@@ -828,8 +869,15 @@ __device__ __forceinline__ uint32_t to_fixed(double c, double scale) {
 struct RecordArgs {
   uint32_t total;
   int mixed;
+  int cubic;  // separation units of the mixed walk: fixed point (cubic box) or lengths
   double fscale[3];
 };
+
+// x rounded up to a float (x >= 0)
+__device__ __forceinline__ float float_up(double x) {
+  const float f = static_cast<float>(x);
+  return static_cast<double>(f) < x ? nextafterf(f, INFINITY) : f;
+}
 
 // BFS -> preorder records (FP64 always, fixed-point/FP32 in mixed mode).
 __global__ __launch_bounds__(256) void records_kernel(RecordArgs a, const uint32_t* __restrict__ pre,
@@ -845,7 +893,9 @@ __global__ __launch_bounds__(256) void records_kernel(RecordArgs a, const uint32
                                                       NodeQuad64* __restrict__ quad64,
                                                       uint32_t* __restrict__ child_index, NodeGeo32* __restrict__ geo32,
                                                       NodeMom32* __restrict__ mom32,
-                                                      NodeQuad32* __restrict__ quad32) {
+                                                      NodeQuad32* __restrict__ quad32,
+                                                      const double* __restrict__ box,
+                                                      NodeBox32* __restrict__ box32) {
   const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= a.total) return;
   const uint32_t p = pre[node];
@@ -874,6 +924,27 @@ __global__ __launch_bounds__(256) void records_kernel(RecordArgs a, const uint32
     quad32[p] = NodeQuad32{static_cast<float>(q[0]), static_cast<float>(q[1]), static_cast<float>(q[2]),
                            static_cast<float>(q[3]), static_cast<float>(q[4]), static_cast<float>(q[5]),
                            static_cast<float>(q[6]), static_cast<float>(q[7])};
+    // Fixed point floors: the centre and every particle move by less than one
+    // unit each, so two units on top of the half-extent keep every particle's
+    // fixed-point position inside the box.
+    const double* bx = box + 6ull * node;
+    NodeBox32 r;
+    uint32_t cf[3];
+    float hf[3];
+    for (int ax = 0; ax < 3; ++ax) {
+      const double half = 0.5 * (bx[3 + ax] - bx[ax]);
+      cf[ax] = to_fixed(0.5 * (bx[ax] + bx[3 + ax]), a.fscale[ax]);
+      hf[ax] = a.cubic ? float_up(half * a.fscale[ax] + 2.0) : float_up(half + 2.0 / a.fscale[ax]);
+    }
+    r.cx = cf[0];
+    r.cy = cf[1];
+    r.cz = cf[2];
+    r.hx = hf[0];
+    r.hy = hf[1];
+    r.hz = hf[2];
+    r.cell_half = static_cast<float>(g.w);
+    r.pad = 0u;
+    box32[p] = r;
   }
 }
 
@@ -1184,7 +1255,8 @@ restart_discovery:
                                                         t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), mk,
                                                         order);
   CHUI_LAUNCH_CHECK("leaf_sort_kernel");
-  // Multipoles bottom-up.
+  // Multipoles (and, for the mixed records, tight boxes) bottom-up.
+  if (P.mixed) t.bfs_box.reserve(total * 6 * sizeof(double));
   for (int d = nlevels - 1; d >= 0; --d) {
     MomArgs ma2{};
     ma2.u = ua;
@@ -1193,7 +1265,8 @@ restart_discovery:
     ma2.quad = P.quadrupole ? 1 : 0;
     multipole_kernel<<<grid_for(ma2.cnt, 128), 128, 0, s>>>(
         ma2, p, m, order, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), nchild, fchild,
-        t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>());
+        t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>(),
+        P.mixed ? t.bfs_box.as<double>() : nullptr);
     CHUI_LAUNCH_CHECK("multipole_kernel");
   }
   // Preorder records.
@@ -1206,17 +1279,20 @@ restart_discovery:
     t.geo32.reserve(total * sizeof(NodeGeo32));
     t.mom32.reserve(total * sizeof(NodeMom32));
     t.quad32.reserve(total * sizeof(NodeQuad32));
+    t.box32.reserve(total * sizeof(NodeBox32));
   }
   RecordArgs ra{};
   ra.total = static_cast<uint32_t>(total);
   ra.mixed = P.mixed ? 1 : 0;
+  ra.cubic = (P.L[0] == P.L[1] && P.L[0] == P.L[2]) ? 1 : 0;
   for (int a = 0; a < 3; ++a) ra.fscale[a] = 4294967296.0 / P.L[a];
   records_kernel<<<grid_for(total, 256), 256, 0, s>>>(
       ra, t.bfs_pre.as<uint32_t>(), t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), nchild, fchild,
       t.bfs_meta.as<uint32_t>(), t.bfs_subtree.as<uint32_t>(), t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(),
       t.bfs_quad.as<double>(), t.link.as<NodeLink>(), t.geo64.as<NodeGeo64>(), t.mom64.as<NodeMom64>(),
       t.quad64.as<NodeQuad64>(), t.child_index.as<uint32_t>(), P.mixed ? t.geo32.as<NodeGeo32>() : nullptr,
-      P.mixed ? t.mom32.as<NodeMom32>() : nullptr, P.mixed ? t.quad32.as<NodeQuad32>() : nullptr);
+      P.mixed ? t.mom32.as<NodeMom32>() : nullptr, P.mixed ? t.quad32.as<NodeQuad32>() : nullptr,
+      P.mixed ? t.bfs_box.as<double>() : nullptr, P.mixed ? t.box32.as<NodeBox32>() : nullptr);
   CHUI_LAUNCH_CHECK("records_kernel");
   uint32_t hleaf = 0;
   CHUI_HIP(hipMemcpyAsync(&hleaf, t.flags.as<uint32_t>() + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));

@@ -28,6 +28,12 @@ struct alignas(64) NodeQuad64 { double qxx, qxy, qxz, qyy, qyz, qzz, trace, offs
 struct alignas(16) NodeGeo32 { uint32_t cx, cy, cz; float half; };
 struct alignas(16) NodeMom32 { uint32_t comx, comy, comz; float mass; };
 struct alignas(32) NodeQuad32 { float qxx, qxy, qxz, qyy, qyz, qzz, trace, offset; };
+// Tight bounding box of a node's particles for the mixed walk's cutoff prune:
+// centre in fixed point, half-extents in separation units (fixed-point units
+// for a cubic box, lengths otherwise) rounded up to cover the fixed-point
+// rounding of the centre and of every particle; plus the octree cell's half
+// size in length units.
+struct alignas(32) NodeBox32 { uint32_t cx, cy, cz; float hx, hy, hz, cell_half; uint32_t pad; };
 
 struct UnwrapResult {
   double anchor[3];
@@ -59,12 +65,12 @@ struct Tree {
   DeviceBuffer pos_of_row; // uint32[n]: source row -> tree position (built on demand)
   // preorder node level
   DeviceBuffer link, geo64, mom64, quad64, child_index;  // child_index: uint32[8*nodes]
-  DeviceBuffer geo32, mom32, quad32;
+  DeviceBuffer geo32, mom32, quad32, box32;
   // construction scratch
   DeviceBuffer keys, keys_alt, vals_alt, sort_scratch;
   DeviceBuffer bucket_min, bucket_max, block_best, block_max;
   DeviceBuffer bfs_begin, bfs_count, bfs_first_child, bfs_nchild, bfs_meta, bfs_geo, bfs_subtree, bfs_pre,
-      bfs_mom, bfs_quad, bfs_scan, level_scratch, flags, bounds;
+      bfs_mom, bfs_quad, bfs_box, bfs_scan, level_scratch, flags, bounds;
   std::size_t bfs_capacity = 0;
   std::vector<std::size_t> level_start;  // BFS level ranges (host copy)
 };

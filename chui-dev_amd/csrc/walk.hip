@@ -369,6 +369,7 @@ struct WalkData {
   const NodeGeo32* geo32;
   const NodeMom32* mom32;
   const NodeQuad32* quad32;
+  const NodeBox32* box32;
   const uint32_t* order;
   const double4* src64;
   const uint4* src32;
@@ -547,7 +548,8 @@ template <int SCHEME, bool CUBIC, bool COUNT>
 __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : (SCHEME == 0 ? 6 : 5))))
 void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long long* __restrict__ counters,
                     const NodeLink* __restrict__ node_link, const NodeGeo32* __restrict__ node_geo,
-                    const NodeMom32* __restrict__ node_mom, const NodeQuad32* __restrict__ node_quad) {
+                    const NodeMom32* __restrict__ node_mom, const NodeQuad32* __restrict__ node_quad,
+                    const NodeBox32* __restrict__ node_box) {
   // node records are read-only for the kernel's lifetime and indexed by the
   // wave-uniform node number, so they come through the scalar cache
   __shared__ float4 win[kWalkWaves][kWindow];
@@ -682,28 +684,53 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
     ksum[w][8][lane] += lz;
   };
 
+  // Strict acceptance of internal node n (cell centre and half from g) for
+  // this lane, with the COM separation it needs left in mf*. Nodes with
+  // h >= 0.041 r_cut are never accepted (2h/r < 0.08 with r inside r_cut), so
+  // the caller skips the test for them.
+  auto accepts = [&](uint32_t n, const NodeGeo32& g, float& mfx, float& mfy, float& mfz) -> bool {
+    const float h = g.half * inv_len;
+    const float ax_ = fabsf(fx_delta(g.cx, px, s0)), ay_ = fabsf(fx_delta(g.cy, py, s1)),
+                az_ = fabsf(fx_delta(g.cz, pz, s2));
+    const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+    const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+    const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= contain2;
+    const NodeMom32 M = node_mom[n];
+    mfx = fx_delta(M.comx, px, s0) * to_len;
+    mfy = fx_delta(M.comy, py, s1) * to_len;
+    mfz = fx_delta(M.comz, pz, s2) * to_len;
+    const float r2 = mfx * mfx + mfy * mfy + mfz * mfz;
+    const float w2 = 2.0f * g.half;
+    // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
+    const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
+    const float lead = P.criterion == 0 ? w2 : w2 + node_quad[n].offset;
+    const bool mac = !inside && lead * lead < theta2 * (r2 + 1.0e-30f);
+    return mac && env_ok && contained;
+  };
+
+  // The evaluating walk prunes against each node's tight particle box. Every
+  // pair inside r_cut still lies in a kept leaf, and a node the reference
+  // accepts as a multipole (its whole cell inside r_cut) is always kept, so
+  // the pairs and multipoles are the reference's; only nodes whose particles
+  // all lie beyond r_cut are skipped earlier than with the octree cells.
   auto walk = [&]<bool NEAR>() {
     uint32_t n = wave_min_u32(nxt);
     while (n < P.node_count) {
       if (COUNT && lane == 0) ++cnt[5];
       const NodeLink lk = node_link[n];
-      const NodeGeo32 g = node_geo[n];
+      const NodeBox32 b = node_box[n];
       const bool act = nxt == n;
       if (!__ballot(act)) {
         if (lane == 0) atomicOr(&counters[kWalkCounters], 1ull);
         break;
       }
-      if (COUNT) cnt[0] += act;
-      const float h = g.half * inv_len;
-      const float ax_ = fabsf(fx_delta(g.cx, px, s0)), ay_ = fabsf(fx_delta(g.cy, py, s1)),
-                  az_ = fabsf(fx_delta(g.cz, pz, s2));
-      const float ex = fmaxf(0.0f, ax_ - h), ey = fmaxf(0.0f, ay_ - h), ez = fmaxf(0.0f, az_ - h);
+      const float ax_ = fabsf(fx_delta(b.cx, px, s0)), ay_ = fabsf(fx_delta(b.cy, py, s1)),
+                  az_ = fabsf(fx_delta(b.cz, pz, s2));
+      const float ex = fmaxf(0.0f, ax_ - b.hx), ey = fmaxf(0.0f, ay_ - b.hy), ez = fmaxf(0.0f, az_ - b.hz);
       const bool keep = act && !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
-      if (COUNT) cnt[1] += act && !keep;
       if (lk.child_count == 0) {
         // a leaf's subtree is itself: every waiting lane moves on to n + 1,
         // so that is the wave's next node too
-        if (COUNT) cnt[4] += keep;
         nxt = act ? n + 1 : nxt;
         if (__ballot(keep)) {
           const float lim = keep ? rc2 : -1.0f;
@@ -729,31 +756,13 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
       }
       nxt = (act && !keep) ? n + lk.skip : nxt;
       if (__ballot(keep)) {
-        // strict acceptance needs 2h/r < 0.08 with r inside r_cut: nodes with
-        // h >= 0.041 r_cut are opened by every lane without the test
         bool take_mp = false;
         float mfx = 0.0f, mfy = 0.0f, mfz = 0.0f;
-        if (g.half < P.accept_half_max) {
-          if (keep) {
-            const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
-            const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
-            const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= contain2;
-            const NodeMom32 M = node_mom[n];
-            mfx = fx_delta(M.comx, px, s0) * to_len;
-            mfy = fx_delta(M.comy, py, s1) * to_len;
-            mfz = fx_delta(M.comz, pz, s2) * to_len;
-            const float r2 = mfx * mfx + mfy * mfy + mfz * mfz;
-            const float w2 = 2.0f * g.half;
-            // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
-            const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
-            const float lead = P.criterion == 0 ? w2 : w2 + node_quad[n].offset;
-            const bool mac = !inside && lead * lead < theta2 * (r2 + 1.0e-30f);
-            take_mp = mac && env_ok && contained && !inside;
-          }
+        if (b.cell_half < P.accept_half_max) {
+          if (keep) take_mp = accepts(n, node_geo[n], mfx, mfy, mfz);
         }
         if (keep) nxt = take_mp ? n + lk.skip : n + 1;
         if (__ballot(take_mp) && take_mp) {
-          if (COUNT) ++cnt[3];
           float cx = 0.0f, cy = 0.0f, cz = 0.0f;
           node_accel32(node_mom[n], node_quad[n], mfx, mfy, mfz, P, cx, cy, cz);
           accumulate(cx, cy, cz);
@@ -765,6 +774,41 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   };
   if (near) walk.template operator()<true>();
   else walk.template operator()<false>();
+
+  // Diagnostic launches also replay the reference's walk over the octree
+  // cells (no pair work) for its per-target node counters.
+  if constexpr (COUNT) {
+    uint32_t cnxt = valid ? 0u : P.node_count;
+    uint32_t n = wave_min_u32(cnxt);
+    while (n < P.node_count) {
+      const NodeLink lk = node_link[n];
+      const NodeGeo32 g = node_geo[n];
+      const bool act = cnxt == n;
+      if (!__ballot(act)) break;  // reported by the evaluating walk
+      cnt[0] += act;
+      const float h = g.half * inv_len;
+      const float ax_ = fabsf(fx_delta(g.cx, px, s0)), ay_ = fabsf(fx_delta(g.cy, py, s1)),
+                  az_ = fabsf(fx_delta(g.cz, pz, s2));
+      const float ex = fmaxf(0.0f, ax_ - h), ey = fmaxf(0.0f, ay_ - h), ez = fmaxf(0.0f, az_ - h);
+      const bool keep = act && !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
+      cnt[1] += act && !keep;
+      if (lk.child_count == 0) {
+        cnt[4] += keep;
+        cnxt = act ? n + 1 : cnxt;
+        ++n;
+        continue;
+      }
+      cnxt = (act && !keep) ? n + lk.skip : cnxt;
+      bool take_mp = false;
+      if (keep && g.half < P.accept_half_max) {
+        float mfx, mfy, mfz;
+        take_mp = accepts(n, g, mfx, mfy, mfz);
+      }
+      cnt[3] += take_mp;
+      if (keep) cnxt = take_mp ? n + lk.skip : n + 1;
+      n = next_node(n, cnxt);
+    }
+  }
 
   if (COUNT) flush_counters(counters, cnt);
   if (!valid) return;
@@ -822,10 +866,10 @@ void launchmix(const WalkParams& p, const WalkData& d, const WalkOutputs& out, u
                hipStream_t s) {
   if (p.count)
     walkmix_kernel<SCHEME, CUBIC, true><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
-        p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32);
+        p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
   else
     walkmix_kernel<SCHEME, CUBIC, false><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
-        p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32);
+        p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
   CHUI_LAUNCH_CHECK("walkmix_kernel");
 }
 
@@ -881,6 +925,7 @@ void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int unifor
   d.geo32 = t.geo32.as<NodeGeo32>();
   d.mom32 = t.mom32.as<NodeMom32>();
   d.quad32 = t.quad32.as<NodeQuad32>();
+  d.box32 = t.box32.as<NodeBox32>();
   d.order = t.order.as<uint32_t>();
   d.src32 = src;
   d.htab = htab;
