@@ -324,10 +324,19 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   if (refresh || !key_match) {
     if (!refresh)
       throw std::runtime_error("TreePM long-range field reuse requested without a matching field (refresh required)");
+    chui::DepositTiling dt{};
+    for (int a = 0; a < 3; ++a) {
+      dt.n[a] = g.n[a];
+      dt.inv_h[a] = static_cast<double>(g.n[a]) / g.L[a];
+    }
+    dt.scheme = po.assignment;
+    dt.max_cells = chui::kDepositTileBytes / (mixed ? sizeof(float) : sizeof(double));
+    dt.chunk = chui::kDepositChunk;
+    const chui::DepositUnits units = chui::build_deposit_units(c->tree, dt, s);
     if (!mixed) {
-      chui::pm_solve_fp64(pw, g, po, c->src64.as<double4>(), uniform ? 1 : 0, um, n, s);
+      chui::pm_solve_fp64(pw, g, po, c->src64.as<double4>(), uniform ? 1 : 0, um, n, units, s);
     } else {
-      chui::pm_solve_mixed(pw, g, po, c->src32.as<uint4>(), uniform ? 1 : 0, static_cast<float>(um), n, s);
+      chui::pm_solve_mixed(pw, g, po, c->src32.as<uint4>(), uniform ? 1 : 0, static_cast<float>(um), n, units, s);
     }
     std::copy(key, key + 8, pw.key);
   }

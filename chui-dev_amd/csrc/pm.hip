@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
 #include <vector>
 
 namespace chui {
@@ -192,17 +193,13 @@ struct DepositArgs {
   int scheme;
 };
 
-__global__ __launch_bounds__(256) void deposit64_kernel(DepositArgs a, const double4* __restrict__ src,
-                                                        int uniform, double um, std::size_t np,
-                                                        double* __restrict__ rho) {
-  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  const double4 sp = src[p];
+// Stencil of one source: wrapped cell indices and weights per axis (2 cells
+// for CIC, 3 for TSC), first the FP64 form (pm_solver.cpp assignment with
+// periodic wrap), then the fixed-point form of the mixed path, whose top lg
+// bits are the cell and whose remaining bits the position inside it.
+__device__ __forceinline__ void stencil(const DepositArgs& a, const double4& sp, int idx[3][3], double w[3][3]) {
   const double g[3] = {wrap_position(sp.x, a.L[0]) * a.inv_h[0], wrap_position(sp.y, a.L[1]) * a.inv_h[1],
                        wrap_position(sp.z, a.L[2]) * a.inv_h[2]};
-  int idx[3][3];
-  double w[3][3];
-  int cnt;
   for (int ax = 0; ax < 3; ++ax) {
     if (a.scheme == 0) {
       const double base = floor(g[ax]);
@@ -224,26 +221,11 @@ __global__ __launch_bounds__(256) void deposit64_kernel(DepositArgs a, const dou
       w[ax][2] = 0.5 * (0.5 + d) * (0.5 + d);
     }
   }
-  cnt = a.scheme == 0 ? 2 : 3;
-  const double mm = uniform ? um : sp.w;
-  for (int i = 0; i < cnt; ++i)
-    for (int j = 0; j < cnt; ++j) {
-      const std::size_t rb = (static_cast<std::size_t>(idx[0][i]) * a.n[1] + idx[1][j]) * a.row;
-      for (int k = 0; k < cnt; ++k) {
-        const double wt = w[0][i] * w[1][j] * w[2][k];
-        atomicAdd(&rho[rb + idx[2][k]], mm * wt);
-      }
-    }
 }
+__device__ __forceinline__ double source_mass(const double4& sp) { return sp.w; }
 
-__global__ __launch_bounds__(256) void deposit32_kernel(DepositArgs a, const uint4* __restrict__ src, int uniform,
-                                                        float um, std::size_t np, float* __restrict__ rho) {
-  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  const uint4 sp = src[p];
+__device__ __forceinline__ void stencil(const DepositArgs& a, const uint4& sp, int idx[3][3], float w[3][3]) {
   const uint32_t q[3] = {sp.x, sp.y, sp.z};
-  int idx[3][3];
-  float w[3][3];
   for (int ax = 0; ax < 3; ++ax) {
     const int sh = 32 - a.lg[ax];
     const int nn = a.n[ax];
@@ -265,14 +247,77 @@ __global__ __launch_bounds__(256) void deposit32_kernel(DepositArgs a, const uin
       w[ax][2] = 0.5f * (0.5f + d) * (0.5f + d);
     }
   }
-  const int cnt = a.scheme == 0 ? 2 : 3;
-  const float mm = uniform ? um : __uint_as_float(sp.w);
-  for (int i = 0; i < cnt; ++i)
-    for (int j = 0; j < cnt; ++j) {
-      const std::size_t rb = (static_cast<std::size_t>(idx[0][i]) * a.n[1] + idx[1][j]) * a.row;
-      const float mij = mm * w[0][i] * w[1][j];
-      for (int k = 0; k < cnt; ++k) atomicAdd(&rho[rb + idx[2][k]], mij * w[2][k]);
+}
+__device__ __forceinline__ float source_mass(const uint4& sp) { return __uint_as_float(sp.w); }
+
+// One block per chunk of a deposit unit. A tiled unit accumulates its chunk in
+// an LDS block of cells and then adds the block to the mesh one z row at a
+// time, so neighbouring lanes hit neighbouring cells and each atomic
+// instruction touches one or two cache lines instead of up to 64. A source
+// whose stencil leaves the block (only through rounding at the block's edge)
+// and every source of an untiled unit go straight to the mesh.
+template <class T, class S>
+__global__ __launch_bounds__(256) void deposit_kernel(DepositArgs a, const S* __restrict__ src, int uniform, T um,
+                                                      DepositUnits du, T* __restrict__ rho) {
+  __shared__ T tile[kDepositTileBytes / sizeof(T)];
+  __shared__ uint32_t s_unit;
+  if (threadIdx.x == 0) {
+    uint32_t lo = 0, hi = du.count - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (du.first_block[mid] <= blockIdx.x) lo = mid;
+      else hi = mid - 1;
     }
+    s_unit = lo;
+  }
+  __syncthreads();
+  const uint32_t ui = s_unit;
+  const DepositUnit u = du.units[ui];
+  const uint32_t p0 = u.begin + (blockIdx.x - du.first_block[ui]) * kDepositChunk;
+  const uint32_t p1 = min(p0 + kDepositChunk, u.begin + u.count);
+  const int cnt = a.scheme == 0 ? 2 : 3;
+  const bool tiled = u.dims[0] > 0;
+  const int dy = u.dims[1], dz = u.dims[2];
+  const int vol = tiled ? u.dims[0] * dy * dz : 0;
+  for (int i = threadIdx.x; i < vol; i += blockDim.x) tile[i] = T(0);
+  __syncthreads();
+  for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const S sp = src[p];
+    int idx[3][3];
+    T w[3][3];
+    stencil(a, sp, idx, w);
+    const T mm = uniform ? um : source_mass(sp);
+    int o[3];
+    bool in = tiled;
+    for (int ax = 0; ax < 3; ++ax) {
+      o[ax] = (idx[ax][0] - u.lo[ax]) & (a.n[ax] - 1);
+      in = in && o[ax] + cnt <= u.dims[ax];
+    }
+    for (int i = 0; i < cnt; ++i)
+      for (int j = 0; j < cnt; ++j) {
+        const T mij = mm * w[0][i] * w[1][j];
+        if (in) {
+          T* t = &tile[((o[0] + i) * dy + (o[1] + j)) * dz + o[2]];
+          for (int k = 0; k < cnt; ++k) atomicAdd(t + k, mij * w[2][k]);
+        } else {
+          const std::size_t rb = (static_cast<std::size_t>(idx[0][i]) * a.n[1] + idx[1][j]) * a.row;
+          for (int k = 0; k < cnt; ++k) atomicAdd(&rho[rb + idx[2][k]], mij * w[2][k]);
+        }
+      }
+  }
+  if (!tiled) return;  // block-uniform
+  __syncthreads();
+  const int rows = u.dims[0] * dy;
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int r = wv; r < rows; r += blockDim.x / kWave) {
+    const int ix = r / dy, iy = r - ix * dy;
+    const std::size_t base =
+        (static_cast<std::size_t>((u.lo[0] + ix) & (a.n[0] - 1)) * a.n[1] + ((u.lo[1] + iy) & (a.n[1] - 1))) * a.row;
+    for (int iz = lane; iz < dz; iz += kWave) {
+      const T v = tile[r * dz + iz];
+      if (v != T(0)) atomicAdd(&rho[base + ((u.lo[2] + iz) & (a.n[2] - 1))], v);
+    }
+  }
 }
 
 // ---- z passes (real <-> half complex, two rows per complex FFT) -----------------
@@ -707,28 +752,31 @@ DepositArgs deposit_args(const PmGeometry& g, const PmOptions& o) {
 }  // namespace
 
 void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double4* src, int uniform_mass,
-                   double um, std::size_t n, hipStream_t s) {
+                   double um, std::size_t n, const DepositUnits& units, hipStream_t s) {
   reserve<double>(w, g);
   const Tables<double> tb = upload_tables<double>(w, g, o, s);
   double* rho = w.buf[0].as<double>();
   CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(double), s));
   if (n > 0) {
-    deposit64_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, n, rho);
-    CHUI_LAUNCH_CHECK("deposit64_kernel");
+    if (units.count == 0) throw std::logic_error("PM deposit without deposit units");
+    deposit_kernel<double, double4><<<units.blocks, 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, units,
+                                                                  rho);
+    CHUI_LAUNCH_CHECK("deposit_kernel");
   }
   solve_spectral<double>(w, g, o, tb, s);
   w.valid = true;
 }
 
 void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint4* src, int uniform_mass,
-                    float um, std::size_t n, hipStream_t s) {
+                    float um, std::size_t n, const DepositUnits& units, hipStream_t s) {
   reserve<float>(w, g);
   const Tables<float> tb = upload_tables<float>(w, g, o, s);
   float* rho = w.buf[0].as<float>();
   CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(float), s));
   if (n > 0) {
-    deposit32_kernel<<<grid_for(n, 256), 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, n, rho);
-    CHUI_LAUNCH_CHECK("deposit32_kernel");
+    if (units.count == 0) throw std::logic_error("PM deposit without deposit units");
+    deposit_kernel<float, uint4><<<units.blocks, 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, units, rho);
+    CHUI_LAUNCH_CHECK("deposit_kernel");
   }
   solve_spectral<float>(w, g, o, tb, s);
   w.valid = true;

@@ -37,16 +37,42 @@ struct PmWorkspace {
   double key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
+// Mesh-deposit work: subtrees whose particles (contiguous in tree order) all
+// land on a small block of mesh cells. A unit with dims[0] == 0 has no such
+// block and is deposited straight into the mesh. Each unit is split into
+// chunks of kDepositChunk particles, one thread block per chunk;
+// first_block[u] is the first block of unit u.
+struct DepositUnit {
+  uint32_t begin, count;
+  int32_t lo[3];    // first cell of the block (wrapped into the mesh)
+  int32_t dims[3];  // cells per axis
+};
+struct DepositUnits {
+  const DepositUnit* units = nullptr;
+  const uint32_t* first_block = nullptr;
+  uint32_t count = 0, blocks = 0;
+};
+// What the unit builder needs to know about the mesh.
+struct DepositTiling {
+  int n[3];
+  double inv_h[3];    // n / L
+  int scheme;         // 0 CIC, 1 TSC
+  uint32_t max_cells; // LDS tile capacity
+  uint32_t chunk;
+};
+constexpr uint32_t kDepositChunk = 2048;
+constexpr uint32_t kDepositTileBytes = 49152;
+
 // Deposit (tree-ordered particles), forward r2c FFT, Green's function with the
 // Gaussian long-range filter, spectral gradient and three c2r inverses.
 // FP64: packed (x, y, z, m) caller doubles in tree order; uniform_mass != 0
 // means every source carries mass um.
 void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double4* src, int uniform_mass,
-                   double um, std::size_t n, hipStream_t s);
+                   double um, std::size_t n, const DepositUnits& units, hipStream_t s);
 // Mixed: packed 32-bit fixed-point periodic coordinates + float mass bits,
 // FP32 mesh and FFTs.
 void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint4* src, int uniform_mass,
-                    float um, std::size_t n, hipStream_t s);
+                    float um, std::size_t n, const DepositUnits& units, hipStream_t s);
 
 // Stage timers (ms) of the last solve, filled when events are enabled.
 struct PmTimings {

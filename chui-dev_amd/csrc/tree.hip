@@ -1255,8 +1255,8 @@ restart_discovery:
                                                         t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), mk,
                                                         order);
   CHUI_LAUNCH_CHECK("leaf_sort_kernel");
-  // Multipoles (and, for the mixed records, tight boxes) bottom-up.
-  if (P.mixed) t.bfs_box.reserve(total * 6 * sizeof(double));
+  // Multipoles and tight particle boxes bottom-up.
+  t.bfs_box.reserve(total * 6 * sizeof(double));
   for (int d = nlevels - 1; d >= 0; --d) {
     MomArgs ma2{};
     ma2.u = ua;
@@ -1265,8 +1265,7 @@ restart_discovery:
     ma2.quad = P.quadrupole ? 1 : 0;
     multipole_kernel<<<grid_for(ma2.cnt, 128), 128, 0, s>>>(
         ma2, p, m, order, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), nchild, fchild,
-        t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>(),
-        P.mixed ? t.bfs_box.as<double>() : nullptr);
+        t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>(), t.bfs_box.as<double>());
     CHUI_LAUNCH_CHECK("multipole_kernel");
   }
   // Preorder records.
@@ -1298,6 +1297,107 @@ restart_discovery:
   CHUI_HIP(hipMemcpyAsync(&hleaf, t.flags.as<uint32_t>() + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   CHUI_HIP(hipStreamSynchronize(s));
   t.leaf_count = hleaf;
+}
+
+// ---- mesh-deposit units ---------------------------------------------------------
+// The first cell a particle at unwrapped coordinate u writes (CIC: its cell;
+// TSC: the cell below the nearest one); the deposit kernels wrap it into the
+// mesh, so the tile of a box is [first(lo), first(hi) + span - 1] per axis.
+__device__ __forceinline__ long deposit_first_cell(double u, double inv_h, int scheme) {
+  const double g = u * inv_h;
+  return scheme == 0 ? static_cast<long>(floor(g)) : static_cast<long>(floor(g + 0.5)) - 1;
+}
+
+__device__ bool deposit_tile(const double* box, const DepositTiling& d, int32_t lo[3], int32_t dims[3]) {
+  const long span = d.scheme == 0 ? 2 : 3;
+  long vol = 1;
+  bool ok = true;
+  for (int a = 0; a < 3; ++a) {
+    const long l = deposit_first_cell(box[a], d.inv_h[a], d.scheme);
+    const long ext = deposit_first_cell(box[3 + a], d.inv_h[a], d.scheme) + span - l;
+    long w = l % d.n[a];
+    if (w < 0) w += d.n[a];
+    lo[a] = static_cast<int32_t>(w);
+    ok = ok && ext <= d.n[a];
+    dims[a] = ok ? static_cast<int32_t>(ext) : 0;
+    vol *= ok ? ext : 1;
+  }
+  return ok && vol <= static_cast<long>(d.max_cells);
+}
+
+// Units form a cut of the tree: a node is one when it fits (or is a leaf) and
+// its parent does not, so every particle lies in exactly one unit. The parent
+// decides for its children; flags start at zero.
+__global__ __launch_bounds__(256) void deposit_flag_kernel(uint32_t total, const uint32_t* __restrict__ nchild,
+                                                           const uint32_t* __restrict__ first_child,
+                                                           const double* __restrict__ box, DepositTiling d,
+                                                           uint32_t* __restrict__ flag) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  int32_t lo[3], dims[3];
+  const uint32_t c = nchild[k];
+  const bool fits = deposit_tile(box + 6ull * k, d, lo, dims);
+  if (k == 0) flag[0] = (fits || c == 0) ? 1u : 0u;
+  if (fits || c == 0) return;
+  const uint32_t fc = first_child[k];
+  for (uint32_t j = 0; j < c; ++j) {
+    const uint32_t ch = fc + j;
+    flag[ch] = (nchild[ch] == 0 || deposit_tile(box + 6ull * ch, d, lo, dims)) ? 1u : 0u;
+  }
+}
+
+__global__ __launch_bounds__(256) void deposit_emit_kernel(uint32_t total, const uint32_t* __restrict__ flag,
+                                                           const uint32_t* __restrict__ index,
+                                                           const uint32_t* __restrict__ nb,
+                                                           const uint32_t* __restrict__ nc,
+                                                           const double* __restrict__ box, DepositTiling d,
+                                                           DepositUnit* __restrict__ units,
+                                                           uint32_t* __restrict__ blocks) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total || !flag[k]) return;
+  DepositUnit u;
+  u.begin = nb[k];
+  u.count = nc[k];
+  if (!deposit_tile(box + 6ull * k, d, u.lo, u.dims)) u.dims[0] = 0;  // leaf spanning too many cells
+  const uint32_t i = index[k];
+  units[i] = u;
+  blocks[i] = (u.count + d.chunk - 1) / d.chunk;
+}
+
+DepositUnits build_deposit_units(Tree& t, const DepositTiling& d, hipStream_t s) {
+  DepositUnits r;
+  const std::size_t total = t.node_count;
+  if (total == 0 || t.n == 0) return r;
+  t.dep_flag.reserve(total * sizeof(uint32_t));
+  t.dep_index.reserve(total * sizeof(uint32_t));
+  t.dep_units.reserve(total * sizeof(DepositUnit));
+  t.dep_blocks.reserve((total + 1) * sizeof(uint32_t));
+  t.dep_scan.reserve(scan_scratch_bytes(total) + 64);
+  t.dep_total.reserve(2 * sizeof(uint32_t));
+  uint32_t* flag = t.dep_flag.as<uint32_t>();
+  uint32_t* totals = t.dep_total.as<uint32_t>();  // [0] units, [1] blocks
+  fill_u32(flag, 0u, total, s);
+  deposit_flag_kernel<<<grid_for(total, 256), 256, 0, s>>>(static_cast<uint32_t>(total), t.bfs_nchild.as<uint32_t>(),
+                                                           t.bfs_first_child.as<uint32_t>(), t.bfs_box.as<double>(),
+                                                           d, flag);
+  CHUI_LAUNCH_CHECK("deposit_flag_kernel");
+  exclusive_scan_u32(flag, t.dep_index.as<uint32_t>(), total, totals, t.dep_scan.ptr, s);
+  deposit_emit_kernel<<<grid_for(total, 256), 256, 0, s>>>(
+      static_cast<uint32_t>(total), flag, t.dep_index.as<uint32_t>(), t.bfs_begin.as<uint32_t>(),
+      t.bfs_count.as<uint32_t>(), t.bfs_box.as<double>(), d, t.dep_units.as<DepositUnit>(),
+      t.dep_blocks.as<uint32_t>());
+  CHUI_LAUNCH_CHECK("deposit_emit_kernel");
+  uint32_t host[2] = {0, 0};
+  CHUI_HIP(hipMemcpyAsync(host, totals, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CHUI_HIP(hipStreamSynchronize(s));
+  exclusive_scan_u32(t.dep_blocks.as<uint32_t>(), t.dep_blocks.as<uint32_t>(), host[0], totals + 1, t.dep_scan.ptr, s);
+  CHUI_HIP(hipMemcpyAsync(host + 1, totals + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CHUI_HIP(hipStreamSynchronize(s));
+  r.units = t.dep_units.as<DepositUnit>();
+  r.first_block = t.dep_blocks.as<uint32_t>();
+  r.count = host[0];
+  r.blocks = host[1];
+  return r;
 }
 
 void build_pos_of_row(Tree& t, hipStream_t s) {

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "pm.hpp"
 
 #include <vector>
 
@@ -71,6 +72,7 @@ struct Tree {
   DeviceBuffer bucket_min, bucket_max, block_best, block_max;
   DeviceBuffer bfs_begin, bfs_count, bfs_first_child, bfs_nchild, bfs_meta, bfs_geo, bfs_subtree, bfs_pre,
       bfs_mom, bfs_quad, bfs_box, bfs_scan, level_scratch, flags, bounds;
+  DeviceBuffer dep_flag, dep_index, dep_units, dep_blocks, dep_scan, dep_total;
   std::size_t bfs_capacity = 0;
   std::vector<std::size_t> level_start;  // BFS level ranges (host copy)
 };
@@ -80,6 +82,11 @@ struct Tree {
 // order and multipoles. Inputs are device pointers (caller positions/masses).
 void build_tree(Tree& t, const double* x, const double* y, const double* z, const double* m,
                 const TreeParams& p, double softening, hipStream_t s);
+
+// Mesh-deposit units of the last built tree: the shallowest subtrees whose
+// particles span at most d.max_cells cells (or leaves that never do). Reads
+// back the unit and block counts (one synchronisation).
+DepositUnits build_deposit_units(Tree& t, const DepositTiling& d, hipStream_t s);
 
 // Morton keys + stable order of given (unwrapped) device coordinates.
 void morton_order_device(Tree& t, const double* x, const double* y, const double* z, std::size_t n,
