@@ -545,7 +545,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
 constexpr int kWindow = 64;
 
 template <int SCHEME, bool CUBIC, bool COUNT>
-__global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : (SCHEME == 0 ? 6 : 5))))
+__global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : 8)))
 void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long long* __restrict__ counters,
                     const NodeLink* __restrict__ node_link, const NodeGeo32* __restrict__ node_geo,
                     const NodeMom32* __restrict__ node_mom, const NodeQuad32* __restrict__ node_quad,
