@@ -5,7 +5,8 @@
 // each wave ranks 64 consecutive items per round with 8 ballots (one per digit
 // bit), keeps running per-digit counts in LDS across rounds, and the block
 // combines the four waves' counts in wave order. Item order inside a tile is
-// therefore (wave, round, lane) = input order.
+// therefore (wave, round, lane) = input order. The scatter first sorts the
+// tile in LDS and then stores it in order, one digit run after another.
 #include "primitives.hpp"
 
 namespace chui {
@@ -15,6 +16,33 @@ constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / kWave;                 // 4
 constexpr int kSortRounds = 8;                                   // rounds of 64 per wave
 constexpr int kSortTile = kSortWaves * kSortRounds * kWave;      // 2048 items
+
+// ---- exclusive scan ---------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* block_total) {
+  __shared__ uint32_t wave_tot[kScanThreads / kWave];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  uint32_t inc = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t o = __shfl_up(inc, off, kWave);
+    if (lane >= off) inc += o;
+  }
+  if (lane == kWave - 1) wave_tot[w] = inc;
+  __syncthreads();
+  uint32_t wave_off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kScanThreads / kWave; ++i) {
+    if (i < w) wave_off += wave_tot[i];
+    tot += wave_tot[i];
+  }
+  __syncthreads();
+  *block_total = tot;
+  return wave_off + inc - v;
+}
 
 template <class K>
 __global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(const K* __restrict__ keys, std::size_t n,
@@ -40,6 +68,9 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
     const uint32_t* __restrict__ offsets, uint32_t tiles) {
   __shared__ uint32_t wcount[kSortWaves][256];
   __shared__ uint32_t gbase[256];
+  __shared__ uint32_t lbase[256];
+  __shared__ K skeys[kSortTile];
+  __shared__ uint32_t svals[kSortTile];
   const int w = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortThreads) (&wcount[0][0])[i] = 0;
@@ -89,44 +120,31 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
       wcount[ww][d] = acc;
       acc += c;
     }
+    uint32_t tile_total;
+    lbase[d] = block_exclusive_sum(acc, &tile_total);  // digit d's first slot in the sorted tile
   }
   __syncthreads();
+  // Sort the tile in LDS, then write it out in order: consecutive lanes store
+  // consecutive positions of one digit's run, so the stores coalesce.
 #pragma unroll
   for (int r = 0; r < kSortRounds; ++r) {
     if (dig[r] >= 0) {
-      const uint32_t pos = gbase[dig[r]] + wcount[w][dig[r]] + rank[r];
-      if (pos >= n) continue;  // unreachable when the histogram is consistent
-      okeys[pos] = k[r];
-      ovals[pos] = v[r];
+      const uint32_t lp = lbase[dig[r]] + wcount[w][dig[r]] + rank[r];
+      skeys[lp] = k[r];
+      svals[lp] = v[r];
     }
   }
-}
-
-// ---- exclusive scan ---------------------------------------------------------
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;
-
-__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* block_total) {
-  __shared__ uint32_t wave_tot[kScanThreads / kWave];
-  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  uint32_t inc = v;
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const uint32_t o = __shfl_up(inc, off, kWave);
-    if (lane >= off) inc += o;
-  }
-  if (lane == kWave - 1) wave_tot[w] = inc;
   __syncthreads();
-  uint32_t wave_off = 0, tot = 0;
-#pragma unroll
-  for (int i = 0; i < kScanThreads / kWave; ++i) {
-    if (i < w) wave_off += wave_tot[i];
-    tot += wave_tot[i];
+  const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * kSortTile;
+  const uint32_t in_tile = static_cast<uint32_t>(n - t0 < static_cast<std::size_t>(kSortTile) ? n - t0 : kSortTile);
+  for (uint32_t i = threadIdx.x; i < in_tile; i += kSortThreads) {
+    const K key = skeys[i];
+    const uint32_t d = static_cast<uint32_t>(key >> shift) & dmask;
+    const uint32_t pos = gbase[d] + (i - lbase[d]);
+    if (pos >= n) continue;  // unreachable when the histogram is consistent
+    okeys[pos] = key;
+    ovals[pos] = svals[i];
   }
-  __syncthreads();
-  *block_total = tot;
-  return wave_off + inc - v;
 }
 
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in, std::size_t n,
