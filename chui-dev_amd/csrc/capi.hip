@@ -187,15 +187,14 @@ __device__ __forceinline__ uint32_t fixed_of(double x, double L) {
   return f >= 4294967295.0 ? 0xFFFFFFFFu : static_cast<uint32_t>(f);
 }
 
-__global__ void gather_fixed_kernel(const uint32_t* __restrict__ order, std::size_t n, const double* __restrict__ x,
-                                    const double* __restrict__ y, const double* __restrict__ z,
-                                    const double* __restrict__ m, double Lx, double Ly, double Lz,
+// from the tree's own tree-order copy (canonical coordinates and mass)
+__global__ void gather_fixed_kernel(const double4* __restrict__ pk, std::size_t n, double Lx, double Ly, double Lz,
                                     uint4* __restrict__ out) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t r = order[i];
-  out[i] = make_uint4(fixed_of(x[r], Lx), fixed_of(y[r], Ly), fixed_of(z[r], Lz),
-                      __float_as_uint(static_cast<float>(m[r])));
+  const double4 q = pk[i];
+  out[i] = make_uint4(fixed_of(q.x, Lx), fixed_of(q.y, Ly), fixed_of(q.z, Lz),
+                      __float_as_uint(static_cast<float>(q.w)));
 }
 
 __global__ void target_keys_kernel(const uint32_t* __restrict__ active, std::size_t na,
@@ -294,8 +293,8 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       CHUI_LAUNCH_CHECK("gather64_kernel");
     } else {
       c->src32.reserve(n * sizeof(uint4));
-      gather_fixed_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(c->tree.order.as<uint32_t>(), n, x, y, z, m,
-                                                                 o->box_size[0], o->box_size[1], o->box_size[2],
+      gather_fixed_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(c->tree.pk.as<double4>(), n, o->box_size[0],
+                                                                 o->box_size[1], o->box_size[2],
                                                                  c->src32.as<uint4>());
       CHUI_LAUNCH_CHECK("gather_fixed_kernel");
     }

@@ -549,15 +549,30 @@ __global__ __launch_bounds__(256) void morton_keys_kernel(Axes3 p, std::size_t n
 }
 
 // Morton key of the particle at each tree position (unwrapped coordinates).
-__global__ __launch_bounds__(256) void leaf_morton_kernel(Axes3 p, std::size_t n, UnwrapArgs u, MortonArgs a,
-                                                          const uint32_t* __restrict__ order,
-                                                          uint64_t* __restrict__ mk) {
+// Tree-order copy of the sources: canonical coordinates (wrap_coord) and mass.
+// One gather through the sort order; everything after it reads the copy.
+__global__ __launch_bounds__(256) void gather_sources_kernel(Axes3 p, const double* __restrict__ m, std::size_t n,
+                                                             UnwrapArgs u, const uint32_t* __restrict__ order,
+                                                             double4* __restrict__ pk) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t r = order[i];
-  mk[i] = morton_key(unwrap_coord(p.p[0][r], u.L[0], u.canonical[0], u.anchor[0]),
-                     unwrap_coord(p.p[1][r], u.L[1], u.canonical[1], u.anchor[1]),
-                     unwrap_coord(p.p[2][r], u.L[2], u.canonical[2], u.anchor[2]), a);
+  pk[i] = make_double4(wrap_coord(p.p[0][r], u.L[0], u.canonical[0]), wrap_coord(p.p[1][r], u.L[1], u.canonical[1]),
+                       wrap_coord(p.p[2][r], u.L[2], u.canonical[2]), m[r]);
+}
+
+// unwrap_coord of a canonical coordinate
+__device__ __forceinline__ double unwrap_canonical(double v, double anchor, double L) {
+  return v < anchor ? v + L : v;
+}
+
+__global__ __launch_bounds__(256) void leaf_morton_kernel(const double4* __restrict__ pk, std::size_t n, UnwrapArgs u,
+                                                          MortonArgs a, uint64_t* __restrict__ mk) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 q = pk[i];
+  mk[i] = morton_key(unwrap_canonical(q.x, u.anchor[0], u.L[0]), unwrap_canonical(q.y, u.anchor[1], u.L[1]),
+                     unwrap_canonical(q.z, u.anchor[2], u.L[2]), a);
 }
 
 // ---------------------------------------------------------------------------
@@ -691,21 +706,25 @@ __global__ __launch_bounds__(256) void preorder_kernel(uint32_t lb, uint32_t cnt
 // In-leaf Morton order: insertion sort of (morton key, source row) pairs.
 __global__ __launch_bounds__(256) void leaf_sort_kernel(uint32_t total_nodes, const uint32_t* __restrict__ nchild,
                                                         const uint32_t* __restrict__ nb, const uint32_t* __restrict__ nc,
-                                                        uint64_t* __restrict__ mk, uint32_t* __restrict__ order) {
+                                                        uint64_t* __restrict__ mk, uint32_t* __restrict__ order,
+                                                        double4* __restrict__ pk) {
   const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= total_nodes || nchild[node] != 0) return;
   const uint32_t b = nb[node], e = b + nc[node];
   for (uint32_t i = b + 1; i < e; ++i) {
     const uint64_t kk = mk[i];
     const uint32_t kv = order[i];
+    const double4 kq = pk[i];
     uint32_t j = i;
     while (j > b && (mk[j - 1] > kk || (mk[j - 1] == kk && order[j - 1] > kv))) {
       mk[j] = mk[j - 1];
       order[j] = order[j - 1];
+      pk[j] = pk[j - 1];
       --j;
     }
     mk[j] = kk;
     order[j] = kv;
+    pk[j] = kq;
   }
 }
 
@@ -717,8 +736,7 @@ struct MomArgs {
   int quad;
 };
 
-__global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, Axes3 p, const double* __restrict__ m,
-                                                        const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, const double4* __restrict__ pk,
                                                         const uint32_t* __restrict__ nb,
                                                         const uint32_t* __restrict__ nc,
                                                         const uint32_t* __restrict__ nchild,
@@ -744,11 +762,11 @@ __global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, Axes3 p, cons
       }
     }
     for (uint32_t i = b; i < e; ++i) {
-      const uint32_t r = order[i];
-      const double mm = m[r];
-      const double ux = unwrap_coord(p.p[0][r], a.u.L[0], a.u.canonical[0], a.u.anchor[0]);
-      const double uy = unwrap_coord(p.p[1][r], a.u.L[1], a.u.canonical[1], a.u.anchor[1]);
-      const double uz = unwrap_coord(p.p[2][r], a.u.L[2], a.u.canonical[2], a.u.anchor[2]);
+      const double4 q = pk[i];
+      const double mm = q.w;
+      const double ux = unwrap_canonical(q.x, a.u.anchor[0], a.u.L[0]);
+      const double uy = unwrap_canonical(q.y, a.u.anchor[1], a.u.L[1]);
+      const double uz = unwrap_canonical(q.z, a.u.anchor[2], a.u.L[2]);
       tm += mm;
       wx += mm * ux;
       wy += mm * uy;
@@ -776,11 +794,11 @@ __global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, Axes3 p, cons
     double ixx = 0, ixy = 0, ixz = 0, iyy = 0, iyz = 0, izz = 0;
     if (a.quad && tm > 0.0) {
       for (uint32_t i = b; i < e; ++i) {
-        const uint32_t r = order[i];
-        const double mm = m[r];
-        const double rx = unwrap_coord(p.p[0][r], a.u.L[0], a.u.canonical[0], a.u.anchor[0]) - M.x;
-        const double ry = unwrap_coord(p.p[1][r], a.u.L[1], a.u.canonical[1], a.u.anchor[1]) - M.y;
-        const double rz = unwrap_coord(p.p[2][r], a.u.L[2], a.u.canonical[2], a.u.anchor[2]) - M.z;
+        const double4 q = pk[i];
+        const double mm = q.w;
+        const double rx = unwrap_canonical(q.x, a.u.anchor[0], a.u.L[0]) - M.x;
+        const double ry = unwrap_canonical(q.y, a.u.anchor[1], a.u.L[1]) - M.y;
+        const double rz = unwrap_canonical(q.z, a.u.anchor[2], a.u.L[2]) - M.z;
         ixx += mm * rx * rx;
         ixy += mm * rx * ry;
         ixz += mm * rx * rz;
@@ -1249,11 +1267,15 @@ restart_discovery:
   }
   ma.extent = std::max(mext, 1.0e-12);
   uint64_t* mk = t.keys_alt.as<uint64_t>();
-  leaf_morton_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, ua, ma, order, mk);
+  t.pk.reserve(n * sizeof(double4));
+  double4* pk = t.pk.as<double4>();
+  gather_sources_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, m, n, ua, order, pk);
+  CHUI_LAUNCH_CHECK("gather_sources_kernel");
+  leaf_morton_kernel<<<grid_for(n, 256), 256, 0, s>>>(pk, n, ua, ma, mk);
   CHUI_LAUNCH_CHECK("leaf_morton_kernel");
   leaf_sort_kernel<<<grid_for(total, 256), 256, 0, s>>>(static_cast<uint32_t>(total), nchild,
                                                         t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), mk,
-                                                        order);
+                                                        order, pk);
   CHUI_LAUNCH_CHECK("leaf_sort_kernel");
   // Multipoles and tight particle boxes bottom-up.
   t.bfs_box.reserve(total * 6 * sizeof(double));
@@ -1264,7 +1286,7 @@ restart_discovery:
     ma2.cnt = static_cast<uint32_t>(t.level_start[d + 1] - t.level_start[d]);
     ma2.quad = P.quadrupole ? 1 : 0;
     multipole_kernel<<<grid_for(ma2.cnt, 128), 128, 0, s>>>(
-        ma2, p, m, order, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), nchild, fchild,
+        ma2, pk, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), nchild, fchild,
         t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>(), t.bfs_box.as<double>());
     CHUI_LAUNCH_CHECK("multipole_kernel");
   }

@@ -63,6 +63,7 @@ struct Tree {
   double root_half = 0.0;
   // particle level
   DeviceBuffer order;      // uint32[n]: tree position -> source row
+  DeviceBuffer pk;         // double4[n]: tree-order canonical coordinates and mass
   DeviceBuffer pos_of_row; // uint32[n]: source row -> tree position (built on demand)
   // preorder node level
   DeviceBuffer link, geo64, mom64, quad64, child_index;  // child_index: uint32[8*nodes]
