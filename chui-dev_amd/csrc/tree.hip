@@ -707,24 +707,43 @@ __global__ __launch_bounds__(256) void preorder_kernel(uint32_t lb, uint32_t cnt
 __global__ __launch_bounds__(256) void leaf_sort_kernel(uint32_t total_nodes, const uint32_t* __restrict__ nchild,
                                                         const uint32_t* __restrict__ nb, const uint32_t* __restrict__ nc,
                                                         uint64_t* __restrict__ mk, uint32_t* __restrict__ order,
-                                                        double4* __restrict__ pk) {
+                                                        uint32_t* __restrict__ from, double4* __restrict__ pk) {
   const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= total_nodes || nchild[node] != 0) return;
   const uint32_t b = nb[node], e = b + nc[node];
+  for (uint32_t i = b; i < e; ++i) from[i] = i;
   for (uint32_t i = b + 1; i < e; ++i) {
     const uint64_t kk = mk[i];
     const uint32_t kv = order[i];
-    const double4 kq = pk[i];
+    const uint32_t kf = from[i];
     uint32_t j = i;
     while (j > b && (mk[j - 1] > kk || (mk[j - 1] == kk && order[j - 1] > kv))) {
       mk[j] = mk[j - 1];
       order[j] = order[j - 1];
-      pk[j] = pk[j - 1];
+      from[j] = from[j - 1];
       --j;
     }
     mk[j] = kk;
     order[j] = kv;
-    pk[j] = kq;
+    from[j] = kf;
+  }
+  // Apply the leaf's permutation to the source copy in place, one cycle at a
+  // time: position j takes the record that was at from[j]; a written position
+  // is marked by from[j] = j, like a fixed point.
+  for (uint32_t c = b; c < e; ++c) {
+    if (from[c] == c) continue;
+    const double4 first = pk[c];
+    uint32_t j = c;
+    for (;;) {
+      const uint32_t k = from[j];
+      from[j] = j;
+      if (k == c) {
+        pk[j] = first;
+        break;
+      }
+      pk[j] = pk[k];
+      j = k;
+    }
   }
 }
 
@@ -1275,7 +1294,7 @@ restart_discovery:
   CHUI_LAUNCH_CHECK("leaf_morton_kernel");
   leaf_sort_kernel<<<grid_for(total, 256), 256, 0, s>>>(static_cast<uint32_t>(total), nchild,
                                                         t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), mk,
-                                                        order, pk);
+                                                        order, t.vals_alt.as<uint32_t>(), pk);
   CHUI_LAUNCH_CHECK("leaf_sort_kernel");
   // Multipoles and tight particle boxes bottom-up.
   t.bfs_box.reserve(total * 6 * sizeof(double));
