@@ -592,43 +592,63 @@ struct LevelArgs {
   uint32_t lvl_begin, lvl_count, next_begin, leaf_size;
 };
 
+// Eight threads per node: thread o of a splitting node binary-searches the
+// first particle of octant o (the searches are independent, so a level costs
+// one search depth rather than seven), the octant boundaries are kept for the
+// emit pass and the group counts the non-empty octants.
 __global__ __launch_bounds__(256) void level_count_kernel(LevelArgs a, const uint32_t* __restrict__ nb,
                                                           const uint32_t* __restrict__ nc, uint32_t* __restrict__ meta,
                                                           const double4* __restrict__ geo,
                                                           const uint64_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ nchild, uint32_t* __restrict__ flags) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.lvl_count) return;
-  const uint32_t node = a.lvl_begin + k;
-  const uint32_t cnt = nc[node];
-  const double half = geo[node].w;
-  const uint32_t depth = meta[node] & 0xFFu;
-  if (cnt <= a.leaf_size || half <= 1.0e-12) {  // tree_gravity.cpp:1133
-    nchild[node] = 0;
-    meta[node] |= 1u << 16;
-    atomicAdd(&flags[2], 1u);
+                                                          uint32_t* __restrict__ nchild, uint32_t* __restrict__ bounds,
+                                                          uint32_t* __restrict__ flags) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = t >> 3, o = t & 7u;
+  const bool in = k < a.lvl_count;
+  const uint32_t node = a.lvl_begin + (in ? k : 0u);
+  const uint32_t cnt = in ? nc[node] : 0u;
+  const double half = in ? geo[node].w : 0.0;
+  const uint32_t depth = in ? (meta[node] & 0xFFu) : 0u;
+  const bool leaf = cnt <= a.leaf_size || half <= 1.0e-12;  // tree_gravity.cpp:1133
+  const bool too_deep = !leaf && depth >= static_cast<uint32_t>(kPathLevels);
+  const bool split = in && !leaf && !too_deep;
+  const uint32_t b = in ? nb[node] : 0u, e = b + cnt;
+  uint32_t lo = b;
+  if (split && o > 0) lo = lower_bound_digit(keys, b, e, 3 * (kPathLevels - 1 - static_cast<int>(depth)), o);
+  // octant o covers [lo, hi) with hi the next thread's lo (e for octant 7)
+  const uint32_t nxt = __shfl_down(lo, 1, 8);
+  const uint32_t hi = o == 7u ? e : nxt;
+  uint32_t c = split && hi > lo ? 1u : 0u;
+  c += __shfl_xor(c, 1, 8);
+  c += __shfl_xor(c, 2, 8);
+  c += __shfl_xor(c, 4, 8);
+  if (!in) return;
+  if (split) bounds[8ull * k + o] = lo;
+  if (o != 0u) return;
+  if (split) {
+    nchild[node] = c;
     return;
   }
-  if (depth >= static_cast<uint32_t>(kPathLevels)) {
-    atomicOr(&flags[0], 1u);
-    nchild[node] = 0;
-    meta[node] |= 1u << 16;
-    return;
-  }
-  const int shift = 3 * (kPathLevels - 1 - static_cast<int>(depth));
-  const uint32_t b = nb[node], e = b + cnt;
-  uint32_t prev = b, c = 0;
-  for (uint32_t o = 1; o <= 8; ++o) {
-    const uint32_t hi = o == 8 ? e : lower_bound_digit(keys, prev, e, shift, o);
-    c += hi > prev ? 1u : 0u;
-    prev = hi;
-  }
-  nchild[node] = c;
+  nchild[node] = 0;
+  meta[node] |= 1u << 16;
+  if (too_deep) atomicOr(&flags[0], 1u);
+}
+
+// Number of leaves (nodes without children), one atomic per wave.
+__global__ __launch_bounds__(256) void count_leaves_kernel(const uint32_t* __restrict__ nchild, std::size_t total,
+                                                           uint32_t* __restrict__ out) {
+  uint32_t c = 0;
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x)
+    c += nchild[i] == 0 ? 1u : 0u;
+  const unsigned long long w = wave_sum(static_cast<unsigned long long>(c));
+  if (lane_id() == 0 && w) atomicAdd(out, static_cast<uint32_t>(w));
 }
 
 __global__ __launch_bounds__(256) void level_emit_kernel(LevelArgs a, uint32_t* __restrict__ nb,
                                                          uint32_t* __restrict__ nc, uint32_t* __restrict__ meta,
-                                                         double4* __restrict__ geo, const uint64_t* __restrict__ keys,
+                                                         double4* __restrict__ geo,
+                                                         const uint32_t* __restrict__ bounds,
                                                          const uint32_t* __restrict__ nchild,
                                                          const uint32_t* __restrict__ cbase,
                                                          uint32_t* __restrict__ first_child) {
@@ -640,18 +660,18 @@ __global__ __launch_bounds__(256) void level_emit_kernel(LevelArgs a, uint32_t* 
     return;
   }
   const uint32_t depth = meta[node] & 0xFFu;
-  const int shift = 3 * (kPathLevels - 1 - static_cast<int>(depth));
-  const uint32_t b = nb[node], e = b + nc[node];
+  const uint32_t e = nb[node] + nc[node];
   const double4 g = geo[node];
   const uint32_t fc = a.next_begin + cbase[k];
   first_child[node] = fc;
-  uint32_t j = 0, prev = b;
+  uint32_t j = 0;
   for (uint32_t o = 0; o < 8; ++o) {
-    const uint32_t hi = o == 7 ? e : lower_bound_digit(keys, prev, e, shift, o + 1);
-    if (hi > prev) {
+    const uint32_t lo = bounds[8ull * k + o];
+    const uint32_t hi = o == 7 ? e : bounds[8ull * k + o + 1];
+    if (hi > lo) {
       const uint32_t c = fc + j++;
-      nb[c] = prev;
-      nc[c] = hi - prev;
+      nb[c] = lo;
+      nc[c] = hi - lo;
       meta[c] = (depth + 1u) | (o << 8);
       const double ch = 0.5 * g.w;  // tree_gravity.cpp:1180-1184
       double4 cg;
@@ -661,7 +681,6 @@ __global__ __launch_bounds__(256) void level_emit_kernel(LevelArgs a, uint32_t* 
       cg.w = ch;
       geo[c] = cg;
     }
-    prev = hi;
   }
 }
 
@@ -1218,11 +1237,12 @@ restart_discovery:
     const uint32_t cnt = static_cast<uint32_t>(le - lb);
     if (cnt == 0) break;
     t.level_scratch.reserve((cnt + 4) * sizeof(uint32_t));
+    t.level_bounds.reserve(8ull * cnt * sizeof(uint32_t));
     t.bfs_scan.reserve(scan_scratch_bytes(cnt) + 64);
     LevelArgs la{static_cast<uint32_t>(lb), cnt, static_cast<uint32_t>(le), P.leaf_size};
-    level_count_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(la, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(),
-                                                          t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(), keys,
-                                                          t.bfs_nchild.as<uint32_t>(), t.flags.as<uint32_t>());
+    level_count_kernel<<<grid_for(8ull * cnt, 256), 256, 0, s>>>(
+        la, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(),
+        keys, t.bfs_nchild.as<uint32_t>(), t.level_bounds.as<uint32_t>(), t.flags.as<uint32_t>());
     CHUI_LAUNCH_CHECK("level_count_kernel");
     uint32_t* cbase = t.level_scratch.as<uint32_t>();
     exclusive_scan_u32(t.bfs_nchild.as<uint32_t>() + lb, cbase, cnt, t.flags.as<uint32_t>() + 1, t.bfs_scan.ptr, s);
@@ -1247,9 +1267,9 @@ restart_discovery:
     }
     ensure(le + total_children, le);
     level_emit_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(la, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(),
-                                                         t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(), keys,
-                                                         t.bfs_nchild.as<uint32_t>(), cbase,
-                                                         t.bfs_first_child.as<uint32_t>());
+                                                         t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(),
+                                                         t.level_bounds.as<uint32_t>(), t.bfs_nchild.as<uint32_t>(),
+                                                         cbase, t.bfs_first_child.as<uint32_t>());
     CHUI_LAUNCH_CHECK("level_emit_kernel");
     t.level_start.push_back(le + total_children);
     t.max_depth = static_cast<uint32_t>(d + 1);
@@ -1334,6 +1354,9 @@ restart_discovery:
       P.mixed ? t.mom32.as<NodeMom32>() : nullptr, P.mixed ? t.quad32.as<NodeQuad32>() : nullptr,
       P.mixed ? t.bfs_box.as<double>() : nullptr, P.mixed ? t.box32.as<NodeBox32>() : nullptr);
   CHUI_LAUNCH_CHECK("records_kernel");
+  count_leaves_kernel<<<std::min<unsigned>(grid_for(total, 256), 1024u), 256, 0, s>>>(nchild, total,
+                                                                                      t.flags.as<uint32_t>() + 2);
+  CHUI_LAUNCH_CHECK("count_leaves_kernel");
   uint32_t hleaf = 0;
   CHUI_HIP(hipMemcpyAsync(&hleaf, t.flags.as<uint32_t>() + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   CHUI_HIP(hipStreamSynchronize(s));

@@ -72,7 +72,7 @@ struct Tree {
   DeviceBuffer keys, keys_alt, vals_alt, sort_scratch;
   DeviceBuffer bucket_min, bucket_max, block_best, block_max;
   DeviceBuffer bfs_begin, bfs_count, bfs_first_child, bfs_nchild, bfs_meta, bfs_geo, bfs_subtree, bfs_pre,
-      bfs_mom, bfs_quad, bfs_box, bfs_scan, level_scratch, flags, bounds;
+      bfs_mom, bfs_quad, bfs_box, bfs_scan, level_scratch, level_bounds, flags, bounds;
   DeviceBuffer dep_flag, dep_index, dep_units, dep_blocks, dep_scan, dep_total;
   std::size_t bfs_capacity = 0;
   std::vector<std::size_t> level_start;  // BFS level ranges (host copy)
