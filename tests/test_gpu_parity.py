@@ -144,6 +144,7 @@ def test_tree_topology_and_moments_bit_exact(name, w):
     assert np.array_equal(tree["geometry"], ref["geometry"])
     assert np.array_equal(tree["moments"], ref["moments"])
     assert diag["node_count"] == ref["count"].size
+    assert diag["leaf_count"] == int(np.all(ref["child_index"] == 0xFFFFFFFF, axis=1).sum())
 
 
 @pytest.mark.parametrize("assignment,deconv", [(0, 0), (0, 1), (1, 1)])
