@@ -1,0 +1,225 @@
+"""Differentiable TreePM + KDK forward and its adjoint behind torch.autograd.
+
+The reference has no autograd or adjoint code; this module is the
+differentiation layer the MI355X build adds on top of the force evaluation
+(tree_pm_coupling.hpp:563-584) and the FP64 KDK pieces
+(time_integration.cpp:242-266, 448-450, 3530-3556).
+
+Forward model (all FP64 on the device, caller order of particles):
+  x_0 = wrap(q + psi),  v_0 = a_0 H(a_0) f(a_0) psi          (Zel'dovich start)
+  step n (a_n -> a_{n+1}):  v_h     = d1_n v_n + K1_n g(x_n)
+                            x_{n+1} = wrap(x_n + D_n v_h)
+                            v_{n+1} = d2_n v_h + K2_n g(x_{n+1})
+with g the total TreePM acceleration (chui_treepm_solve_device, every source
+active). The adjoint runs the steps backwards:
+  lam_g   = K2_n lam_v + c_{n+1}       (c_{n+1} = K1_{n+1} lam_vh of step n+1)
+  lam_x  += J(x_{n+1})^T lam_g         (chui_treepm_force_vjp_device)
+  lam_vh  = d2_n lam_v + D_n lam_x
+  lam_v   = d1_n lam_vh,  c_n = K1_n lam_vh
+and finally lam_x += J(x_0)^T c_0, d loss / d psi = lam_x + a_0 H f lam_v. The
+wrap has unit derivative. Positions are checkpointed every `checkpoint_every`
+steps; the backward pass recomputes the positions inside each segment.
+
+The force VJP requires multipole_order 0: with the strict acceptance policy no
+node is accepted (tree_pm_acceptance.hpp:62-66), so the short-range force is
+the exact pair sum within r_cut and its Jacobian is the pairwise one.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib, ics
+from .treepm import ForceAccumulator, TreePmCoordinator, kdk_factors
+
+
+@dataclass
+class KdkProblem:
+    """One periodic box, its lattice and its KDK schedule."""
+
+    n_side: int
+    box: float
+    mesh: int
+    scale_factors: list
+    softening: float
+    loss_mesh: int
+    checkpoint_every: int = 2
+    omega_m: float = ics.OMEGA_M
+    device: str = "cuda:0"
+    options: object = None
+    coord: TreePmCoordinator | None = None
+    lattice: torch.Tensor | None = None
+    mass: torch.Tensor | None = None
+    factors: list = field(default_factory=list)
+    vel_factor: float = 0.0
+    force_evaluations: int = 0
+    vjp_evaluations: int = 0
+
+    def __post_init__(self):
+        n = self.n_side**3
+        spacing = self.box / self.n_side
+        idx = torch.arange(self.n_side, dtype=torch.float64, device=self.device)
+        gx, gy, gz = torch.meshgrid(idx, idx, idx, indexing="ij")
+        self.lattice = torch.stack([(g.reshape(-1) + 0.5) * spacing for g in (gx, gy, gz)]).contiguous()
+        rho_crit = 3.0 * ics.H0_CODE**2 / (8.0 * np.pi * ics.G_CODE)
+        self.mass = torch.full((n,), rho_crit * self.omega_m * self.box**3 / n, dtype=torch.float64,
+                               device=self.device)
+        self.options = _lib.default_options(
+            box_size=self.box, gravitational_constant=ics.G_CODE, opening_theta=0.5, asmth_cells=1.25,
+            rcut_cells=4.5 * 1.25, softening_epsilon=self.softening, multipole_order=0,
+            precision=_lib.PRECISION_FP64)
+        self.coord = TreePmCoordinator(self.mesh)
+        self.factors = [kdk_factors(a0, a1, ics.H0_CODE, self.omega_m, 1.0 - self.omega_m)
+                        for a0, a1 in zip(self.scale_factors[:-1], self.scale_factors[1:])]
+        a0 = self.scale_factors[0]
+        self.vel_factor = a0 * ics.hubble(a0, self.omega_m) * ics.growth_rate(a0, self.omega_m)
+
+    @property
+    def steps(self) -> int:
+        return len(self.scale_factors) - 1
+
+    def close(self):
+        if self.coord is not None:
+            self.coord.close()
+            self.coord = None
+
+    # -- device pieces ----------------------------------------------------------
+    # The library runs on its own stream: each call waits for torch's queued
+    # work on its inputs and is waited for before torch reads its outputs.
+    def _call(self, fn, *args):
+        torch.cuda.current_stream(self.device).synchronize()
+        fn(*args)
+        self.coord.synchronize()
+
+    def wrap(self, x: torch.Tensor) -> torch.Tensor:
+        return torch.remainder(x, self.box)
+
+    def forces(self, x: torch.Tensor) -> torch.Tensor:
+        acc = torch.empty_like(x)
+        self._call(self.coord.solve_active_set, x[0], x[1], x[2], self.mass,
+                   ForceAccumulator((acc[0], acc[1], acc[2])), self.options)
+        self.force_evaluations += 1
+        return acc
+
+    def force_vjp(self, x: torch.Tensor, lam: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(x)
+        lam = lam.contiguous()
+        self._call(self.coord.force_vjp, x[0], x[1], x[2], self.mass, (lam[0], lam[1], lam[2]),
+                   (out[0], out[1], out[2]), self.options)
+        self.vjp_evaluations += 1
+        return out
+
+    def cic_density(self, x: torch.Tensor) -> torch.Tensor:
+        m = self.loss_mesh
+        delta = torch.empty(m, m, m, dtype=torch.float64, device=x.device)
+        self._call(self.coord.cic_density, x[0], x[1], x[2], self.box, m, delta)
+        return delta
+
+    def cic_density_vjp(self, x: torch.Tensor, grad_delta: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(x)
+        self._call(self.coord.cic_density_vjp, x[0], x[1], x[2], self.box, self.loss_mesh,
+                   grad_delta.contiguous(), (out[0], out[1], out[2]))
+        return out
+
+    def kdk_step(self, n: int, x, v, g):
+        f = self.factors[n]
+        vh = f["first_hubble_drag"] * v + f["first_kick_factor"] * g
+        x = self.wrap(x + f["drift_factor"] * vh)
+        g = self.forces(x)
+        v = f["second_hubble_drag"] * vh + f["second_kick_factor"] * g
+        return x, v, g
+
+
+class KdkTrajectory(torch.autograd.Function):
+    """psi (3, N) FP64 initial displacements -> final positions (3, N)."""
+
+    @staticmethod
+    def forward(ctx, psi: torch.Tensor, prob: KdkProblem):
+        x = prob.wrap(prob.lattice + psi)
+        v = prob.vel_factor * psi
+        g = prob.forces(x)
+        checkpoints = {0: (x, v, g)}
+        for n in range(prob.steps):
+            x, v, g = prob.kdk_step(n, x, v, g)
+            if (n + 1) % prob.checkpoint_every == 0 and n + 1 < prob.steps:
+                checkpoints[n + 1] = (x, v, g)
+        ctx.prob = prob
+        ctx.checkpoints = checkpoints
+        ctx.final = x
+        return x.clone()
+
+    @staticmethod
+    def backward(ctx, lam_final: torch.Tensor):
+        prob = ctx.prob
+        S = prob.steps
+        lam_x = lam_final.contiguous().clone()
+        lam_v = torch.zeros_like(lam_x)
+        carry = None
+        starts = sorted(ctx.checkpoints)
+        for seg_start in reversed(starts):
+            seg_end = min(seg_start + prob.checkpoint_every, S)
+            # positions x_seg_start .. x_seg_end (recomputed inside the segment)
+            x, v, g = ctx.checkpoints[seg_start]
+            xs = [x]
+            for n in range(seg_start, seg_end - 1):
+                x, v, g = prob.kdk_step(n, x, v, g)
+                xs.append(x)
+            xs.append(ctx.final if seg_end == S else ctx.checkpoints[seg_end][0])
+            for n in range(seg_end - 1, seg_start - 1, -1):
+                f = prob.factors[n]
+                lam_g = f["second_kick_factor"] * lam_v
+                if carry is not None:
+                    lam_g = lam_g + carry
+                lam_x = lam_x + prob.force_vjp(xs[n + 1 - seg_start], lam_g)
+                lam_vh = f["second_hubble_drag"] * lam_v + f["drift_factor"] * lam_x
+                lam_v = f["first_hubble_drag"] * lam_vh
+                carry = f["first_kick_factor"] * lam_vh
+        lam_x = lam_x + prob.force_vjp(ctx.checkpoints[0][0], carry)
+        return lam_x + prob.vel_factor * lam_v, None
+
+
+class CicDensity(torch.autograd.Function):
+    """positions (3, N) -> CIC density contrast on a loss_mesh^3 grid."""
+
+    @staticmethod
+    def forward(ctx, x: torch.Tensor, prob: KdkProblem):
+        ctx.prob = prob
+        ctx.save_for_backward(x)
+        return prob.cic_density(x)
+
+    @staticmethod
+    def backward(ctx, grad_delta: torch.Tensor):
+        (x,) = ctx.saved_tensors
+        return ctx.prob.cic_density_vjp(x, grad_delta), None
+
+
+def density_loss(psi: torch.Tensor, target: torch.Tensor, prob: KdkProblem) -> torch.Tensor:
+    """sum over cells of (delta_final - target)^2; psi may be FP32 (promoted)."""
+    x = KdkTrajectory.apply(psi.to(torch.float64), prob)
+    delta = CicDensity.apply(x, prob)
+    return ((delta - target) ** 2).sum()
+
+
+def adjoint_problem(n_side: int = 64, steps: int = 10, a_begin: float = 0.05, a_end: float = 0.5,
+                    checkpoint_every: int = 2, rms_spacing: float = 0.1, device: str = "cuda:0",
+                    seed: int = ics.BASE_SEED + 3):
+    """The differentiable workload: n_side^3 particles in a box of 50 (n_side/64)
+    Mpc/h, mesh 2 n_side, KDK steps uniform in a, a seeded displacement
+    (P(k) ~ k^-2.5, RMS rms_spacing mean spacings at a_begin: 0.1 grows to about
+    one spacing by a = 0.5) and a target density field from a second seeded
+    displacement of ten times that RMS. Returns (problem, psi (3, N) FP32 on the
+    device, target (loss_mesh^3) FP64)."""
+    box = 50.0 * n_side / 64
+    rng = np.random.default_rng(seed)
+    disp = ics.zeldovich_displacement(n_side, box, -2.5, rms_spacing, rng)
+    tdisp = ics.zeldovich_displacement(n_side, box, -2.5, 10.0 * rms_spacing, np.random.default_rng(seed + 1000))
+    prob = KdkProblem(n_side=n_side, box=box, mesh=2 * n_side,
+                      scale_factors=list(np.linspace(a_begin, a_end, steps + 1)),
+                      softening=box / n_side / 30.0, loss_mesh=n_side, checkpoint_every=checkpoint_every,
+                      device=device)
+    psi = torch.tensor(disp, dtype=torch.float32, device=device)
+    # target: the CIC density contrast of the lattice displaced by the second field
+    tx = prob.wrap(prob.lattice + torch.tensor(tdisp, dtype=torch.float64, device=device))
+    return prob, psi, prob.cic_density(tx)

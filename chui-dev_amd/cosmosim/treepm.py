@@ -161,6 +161,23 @@ class TreePmCoordinator:
         _lib.check(self._lib.chui_treepm_drift_device(self._h, _ptr(x), _ptr(y), _ptr(z), _ptr(vx), _ptr(vy),
                                                       _ptr(vz), int(x.shape[0]), float(drift), L))
 
+    def force_vjp(self, x, y, z, m, lam, out, options):
+        """out_k = sum_i (d a_i / d x_k)^T lam_i for every source (FP64 device
+        arrays in caller order; options must be FP64 with multipole_order 0)."""
+        _lib.check(self._lib.chui_treepm_force_vjp_device(
+            self._h, _ptr(x), _ptr(y), _ptr(z), _ptr(m), int(x.shape[0]), _ptr(lam[0]), _ptr(lam[1]),
+            _ptr(lam[2]), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), C.byref(options)))
+
+    def cic_density(self, x, y, z, box, mesh, delta):
+        """delta = CIC count * mesh^3 / n - 1 on a mesh^3 grid (row-major)."""
+        _lib.check(self._lib.chui_treepm_cic_density_device(self._h, _ptr(x), _ptr(y), _ptr(z), int(x.shape[0]),
+                                                            float(box), int(mesh), _ptr(delta)))
+
+    def cic_density_vjp(self, x, y, z, box, mesh, grad_delta, out):
+        _lib.check(self._lib.chui_treepm_cic_density_vjp_device(
+            self._h, _ptr(x), _ptr(y), _ptr(z), int(x.shape[0]), float(box), int(mesh), _ptr(grad_delta),
+            _ptr(out[0]), _ptr(out[1]), _ptr(out[2])))
+
     def mass_momentum(self, m, vx, vy, vz):
         out = (C.c_double * 4)()
         _lib.check(self._lib.chui_treepm_mass_momentum_device(self._h, _ptr(m), _ptr(vx), _ptr(vy), _ptr(vz),

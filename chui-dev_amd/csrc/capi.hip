@@ -43,6 +43,8 @@ struct chui_treepm_context {
   bool own_stream = false;
   chui::Tree tree;
   chui::PmWorkspace pm64, pm32;
+  chui::PmWorkspace vjp_fwd, vjp_adj;      // force VJP: forward field and adjoint potential
+  DeviceBuffer vjp_lam, vjp_comp[3];       // force VJP: tree-ordered lam and deposit lanes
   DeviceBuffer src64, src32, htab;  // tree-ordered packed sources; H(t) table
   double htab_key[2] = {0.0, 0.0};
   DeviceBuffer hx, hy, hz, hm, hactive, hout[9];
@@ -187,6 +189,23 @@ __device__ __forceinline__ uint32_t fixed_of(double x, double L) {
   return f >= 4294967295.0 ? 0xFFFFFFFFu : static_cast<uint32_t>(f);
 }
 
+// Force VJP: tree-ordered lam (x, y, z, 0) and the three deposit lanes
+// (position, lam_a) of the adjoint PM deposit.
+__global__ void gather_lam_kernel(const uint32_t* __restrict__ order, std::size_t n, const double4* __restrict__ src,
+                                  const double* __restrict__ lx, const double* __restrict__ ly,
+                                  const double* __restrict__ lz, double4* __restrict__ lam, double4* __restrict__ c0,
+                                  double4* __restrict__ c1, double4* __restrict__ c2) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = order[i];
+  const double4 p = src[i];
+  const double a = lx[r], b = ly[r], c = lz[r];
+  lam[i] = make_double4(a, b, c, 0.0);
+  c0[i] = make_double4(p.x, p.y, p.z, a);
+  c1[i] = make_double4(p.x, p.y, p.z, b);
+  c2[i] = make_double4(p.x, p.y, p.z, c);
+}
+
 // from the tree's own tree-order copy (canonical coordinates and mass)
 __global__ void gather_fixed_kernel(const double4* __restrict__ pk, std::size_t n, double Lx, double Ly, double Lz,
                                     uint4* __restrict__ out) {
@@ -246,6 +265,49 @@ void harvest(Ctx* c) {
   c->prof_ms[5] += elapsed(c, kEvTargets, kEvWalk);
   c->prof_solves += 1;
   c->pending = false;
+}
+
+// Walk parameters shared by the force walk and the force VJP walk (FP64 and
+// mixed fields that depend only on the options, the split and the mesh).
+chui::WalkParams base_walk_params(const Ctx* c, const chui_treepm_options* o, const Split& sp, std::size_t mesh_row) {
+  chui::WalkParams wp{};
+  for (int a = 0; a < 3; ++a) {
+    wp.L[a] = o->box_size[a];
+    wp.mesh[a] = c->mesh[a];
+    wp.mesh_log2[a] = c->mesh_lg[a];
+    wp.fx_to_len[a] = static_cast<float>(o->box_size[a] / 4294967296.0);
+  }
+  wp.G = o->gravitational_constant;
+  wp.a = sp.a;
+  wp.rcut = sp.rcut;
+  wp.rcut2 = sp.rcut * sp.rcut;
+  wp.theta = o->opening_theta;
+  wp.eps = o->softening_epsilon;
+  wp.criterion = o->opening_criterion == CHUI_OPEN_GEOMETRIC ? 0 : 1;
+  wp.quad = o->multipole_order == 2;
+  wp.assignment = o->assignment_scheme;
+  wp.c4 = static_cast<float>(0.25 / (sp.a * sp.a));
+  wp.c8 = static_cast<float>(0.125 / (sp.a * sp.a * sp.a));
+  wp.cubic = (o->box_size[0] == o->box_size[1] && o->box_size[0] == o->box_size[2]) ? 1 : 0;
+  {
+    // cutoff radii in separation units: fixed-point units for a cubic box
+    const double sep = wp.cubic ? o->box_size[0] / 4294967296.0 : 1.0;
+    const double r2 = wp.rcut2 / (sep * sep);
+    wp.rcut2_sep = static_cast<float>(r2);
+    wp.prune2_sep = static_cast<float>(r2 * (1.0 + 1.0e-5));
+    wp.contain2_sep = static_cast<float>(r2 * (1.0 - 1.0e-5));
+  }
+  for (int a = 0; a < 3; ++a) {
+    // float offsets from a wave origin: targets within L/16 (precision) and
+    // within L/2 - r_cut (no wrap of the origin's frame inside a pair), with a
+    // margin of 2^24 units for the r_cut conversion
+    const double rcut_units = sp.rcut / o->box_size[a] * 4294967296.0;
+    const double lim = std::min(268435456.0, 2147483648.0 - rcut_units - 16777216.0);
+    wp.fast_half_extent[a] = lim > 0.0 ? static_cast<uint32_t>(lim) : 0u;
+  }
+  wp.node_count = static_cast<uint32_t>(c->tree.node_count);
+  wp.mesh_row = mesh_row;
+  return wp;
 }
 
 // Core force evaluation on device arrays (tree_pm_coupling.cpp:2282-3519 for
@@ -376,49 +438,13 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   c->counters.reserve((chui::kWalkCounters + 1) * sizeof(unsigned long long));
   chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounters + 1, s);
   const bool check_walk_progress_ = diag != nullptr;
-  chui::WalkParams wp{};
-  for (int a = 0; a < 3; ++a) {
-    wp.L[a] = o->box_size[a];
-    wp.mesh[a] = c->mesh[a];
-    wp.mesh_log2[a] = c->mesh_lg[a];
-    wp.fx_to_len[a] = static_cast<float>(o->box_size[a] / 4294967296.0);
-  }
-  wp.G = o->gravitational_constant;
-  wp.a = sp.a;
-  wp.rcut = sp.rcut;
-  wp.rcut2 = sp.rcut * sp.rcut;
-  wp.theta = o->opening_theta;
-  wp.eps = o->softening_epsilon;
-  wp.criterion = o->opening_criterion == CHUI_OPEN_GEOMETRIC ? 0 : 1;
-  wp.quad = o->multipole_order == 2;
-  wp.node_count = static_cast<uint32_t>(c->tree.node_count);
+  chui::WalkParams wp = base_walk_params(c, o, sp, g.row);
   wp.n_targets = static_cast<uint32_t>(n_active);
   wp.identity_targets = identity;
   wp.count = diag != nullptr ? 1 : 0;
   wp.uniform_mass = um;
   wp.uniform_mass_f = static_cast<float>(um);
   wp.pm_enabled = 1;
-  wp.assignment = o->assignment_scheme;
-  wp.mesh_row = g.row;
-  wp.c4 = static_cast<float>(0.25 / (sp.a * sp.a));
-  wp.c8 = static_cast<float>(0.125 / (sp.a * sp.a * sp.a));
-  wp.cubic = (o->box_size[0] == o->box_size[1] && o->box_size[0] == o->box_size[2]) ? 1 : 0;
-  {
-    // cutoff radii in separation units: fixed-point units for a cubic box
-    const double sep = wp.cubic ? o->box_size[0] / 4294967296.0 : 1.0;
-    const double r2 = wp.rcut2 / (sep * sep);
-    wp.rcut2_sep = static_cast<float>(r2);
-    wp.prune2_sep = static_cast<float>(r2 * (1.0 + 1.0e-5));
-    wp.contain2_sep = static_cast<float>(r2 * (1.0 - 1.0e-5));
-  }
-  for (int a = 0; a < 3; ++a) {
-    // float offsets from a wave origin: targets within L/16 (precision) and
-    // within L/2 - r_cut (no wrap of the origin's frame inside a pair), with a
-    // margin of 2^24 units for the r_cut conversion
-    const double rcut_units = sp.rcut / o->box_size[a] * 4294967296.0;
-    const double lim = std::min(268435456.0, 2147483648.0 - rcut_units - 16777216.0);
-    wp.fast_half_extent[a] = lim > 0.0 ? static_cast<uint32_t>(lim) : 0u;
-  }
   chui::WalkOutputs out{ax, ay, az, pmx, pmy, pmz, srx, sry, srz};
   if (n_active > 0 && n > 0) {
     if (!mixed) {
@@ -683,6 +709,103 @@ chui_status chui_treepm_synchronize(chui_treepm_context* c) {
   });
 }
 
+// Transpose of the Jacobian of the total acceleration with respect to the
+// source positions, applied to lam: out_k = sum_i (d a_i / d x_k)^T lam_i over
+// all N sources (all active). FP64 only, and monopole moments only: with the
+// strict policy no node is then accepted (tree_pm_acceptance.hpp:62-66), the
+// short-range force is the exact pair sum within r_cut and its Jacobian is the
+// pairwise one. The tree and both PM fields are rebuilt at x.
+void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, const double* m, uint64_t n,
+                    const double* lx, const double* ly, const double* lz, double* ox, double* oy, double* oz,
+                    const chui_treepm_options* o) {
+  validate_options(o, c->mesh);
+  if (o->precision != CHUI_PRECISION_FP64)
+    throw chui::UnsupportedError("the force VJP is built for FP64 precision only");
+  if (o->multipole_order != 0)
+    throw chui::UnsupportedError("the force VJP needs multipole_order 0 (exact pair sums within r_cut)");
+  if (n > 0xFFFFFFFEull) throw std::overflow_error("Tree gravity source count exceeds the 32-bit tree-index contract");
+  if (n > 0 && (!x || !y || !z || !m || !lx || !ly || !lz || !ox || !oy || !oz))
+    throw std::invalid_argument("TreePM force VJP lanes are null");
+  if (n == 0) return;
+  hipStream_t s = c->stream;
+  harvest(c);
+  const Split sp = split_of(*o, c->mesh);
+  chui::TreeParams tp;
+  tp.n = n;
+  for (int a = 0; a < 3; ++a) tp.L[a] = o->box_size[a];
+  tp.leaf_size = static_cast<uint32_t>(o->max_leaf_size);
+  tp.quadrupole = false;
+  tp.mixed = false;
+  chui::build_tree(c->tree, x, y, z, m, tp, o->softening_epsilon, s);
+  const bool uniform = c->tree.mass_uniform;
+  const double um = uniform ? c->tree.uniform_mass : 0.0;
+  c->src64.reserve(n * sizeof(double4));
+  gather64_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(c->tree.order.as<uint32_t>(), n, x, y, z, m,
+                                                         c->src64.as<double4>());
+  CHUI_LAUNCH_CHECK("gather64_kernel");
+  c->vjp_lam.reserve(n * sizeof(double4));
+  for (auto& b : c->vjp_comp) b.reserve(n * sizeof(double4));
+  gather_lam_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(
+      c->tree.order.as<uint32_t>(), n, c->src64.as<double4>(), lx, ly, lz, c->vjp_lam.as<double4>(),
+      c->vjp_comp[0].as<double4>(), c->vjp_comp[1].as<double4>(), c->vjp_comp[2].as<double4>());
+  CHUI_LAUNCH_CHECK("gather_lam_kernel");
+
+  chui::PmGeometry g{};
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = c->mesh[a];
+    g.lg[a] = c->mesh_lg[a];
+    g.L[a] = o->box_size[a];
+  }
+  g.nzc = static_cast<std::size_t>(c->mesh[2] / 2 + 1);
+  g.row = 2 * g.nzc;
+  chui::PmOptions po;
+  po.G = o->gravitational_constant;
+  po.split_scale = sp.a;
+  po.assignment = o->assignment_scheme;
+  po.deconvolution = o->window_deconvolution;
+  chui::DepositTiling dt{};
+  for (int a = 0; a < 3; ++a) {
+    dt.n[a] = g.n[a];
+    dt.inv_h[a] = static_cast<double>(g.n[a]) / g.L[a];
+  }
+  dt.scheme = po.assignment;
+  dt.max_cells = chui::kDepositTileBytes / sizeof(double);
+  dt.chunk = chui::kDepositChunk;
+  const chui::DepositUnits units = chui::build_deposit_units(c->tree, dt, s);
+  chui::pm_solve_fp64(c->vjp_fwd, g, po, c->src64.as<double4>(), uniform ? 1 : 0, um, n, units, s);
+  c->vjp_fwd.valid = false;  // a field at the VJP's positions, never reused by a solve
+  const double4* comp[3] = {c->vjp_comp[0].as<double4>(), c->vjp_comp[1].as<double4>(),
+                            c->vjp_comp[2].as<double4>()};
+  chui::pm_adjoint_fp64(c->vjp_adj, g, po, comp, n, units, s);
+
+  c->counters.reserve((chui::kWalkCounters + 1) * sizeof(unsigned long long));
+  chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounters + 1, s);
+  chui::WalkParams wp = base_walk_params(c, o, sp, g.row);
+  wp.quad = 0;
+  wp.n_targets = static_cast<uint32_t>(n);
+  wp.identity_targets = 1;
+  wp.count = 0;
+  wp.uniform_mass = um;
+  wp.uniform_mass_f = static_cast<float>(um);
+  wp.pm_enabled = 1;
+  chui::WalkOutputs out{ox, oy, oz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  chui::walk_fp64_vjp(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, c->vjp_lam.as<double4>(),
+                      static_cast<const double*>(c->vjp_fwd.force[0]), static_cast<const double*>(c->vjp_fwd.force[1]),
+                      static_cast<const double*>(c->vjp_fwd.force[2]), c->vjp_adj.buf[0].as<double>(), out,
+                      reinterpret_cast<unsigned long long*>(c->counters.ptr), s);
+}
+
+chui_status chui_treepm_force_vjp_device(chui_treepm_context* c, const double* x, const double* y, const double* z,
+                                         const double* m, uint64_t n, const double* lam_x, const double* lam_y,
+                                         const double* lam_z, double* out_x, double* out_y, double* out_z,
+                                         const chui_treepm_options* o) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    force_vjp_impl(c, x, y, z, m, n, lam_x, lam_y, lam_z, out_x, out_y, out_z, o);
+  });
+}
+
 chui_status chui_treepm_solve_device(chui_treepm_context* c, const double* x, const double* y, const double* z,
                                      const double* m, uint64_t n, const uint32_t* active, uint64_t n_active,
                                      void* ax, void* ay, void* az, void* pmx, void* pmy, void* pmz, void* srx,
@@ -839,6 +962,31 @@ chui_status chui_treepm_drift_device(chui_treepm_context* c, double* x, double* 
     set_device(c);
     if (!L) throw std::invalid_argument("box size pointer is null");
     chui::drift_device(x, y, z, vx, vy, vz, n, d, L, c->stream);
+  });
+}
+
+chui_status chui_treepm_cic_density_device(chui_treepm_context* c, const double* x, const double* y, const double* z,
+                                          uint64_t n, double box, int32_t mesh, double* delta) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    if (!(box > 0.0) || mesh <= 0) throw std::invalid_argument("CIC density needs box > 0 and mesh > 0");
+    if (!delta || (n > 0 && (!x || !y || !z))) throw std::invalid_argument("CIC density lanes are null");
+    chui::cic_density_device(x, y, z, n, box, mesh, delta, c->stream);
+  });
+}
+
+chui_status chui_treepm_cic_density_vjp_device(chui_treepm_context* c, const double* x, const double* y,
+                                              const double* z, uint64_t n, double box, int32_t mesh,
+                                              const double* grad_delta, double* out_x, double* out_y,
+                                              double* out_z) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    if (!(box > 0.0) || mesh <= 0) throw std::invalid_argument("CIC density needs box > 0 and mesh > 0");
+    if (n > 0 && (!x || !y || !z || !grad_delta || !out_x || !out_y || !out_z))
+      throw std::invalid_argument("CIC density VJP lanes are null");
+    chui::cic_density_vjp_device(x, y, z, n, box, mesh, grad_delta, out_x, out_y, out_z, c->stream);
   });
 }
 

@@ -57,6 +57,68 @@ __global__ void drift_kernel(double* __restrict__ x, double* __restrict__ y, dou
   z[i] = canonical_position(z[i] + vz[i] * d, Lz);
 }
 
+// ---- CIC density contrast of unit-weight particles and its transpose ----------
+// (the loss mesh of the differentiable forward; the stencil is the CIC of
+// pm_solver.cpp:925-936 on its own n^3 mesh, cells row-major without padding)
+__device__ __forceinline__ void cic_axis(double x, double L, int n, int (&i)[2], double (&w)[2], double& dw) {
+  const double g = canonical_position(x, L) * (static_cast<double>(n) / L);
+  const double base = floor(g);
+  const long i0 = static_cast<long>(base);
+  const double t = g - base;
+  i[0] = static_cast<int>(((i0 % n) + n) % n);
+  i[1] = (i[0] + 1) % n;
+  w[0] = 1.0 - t;
+  w[1] = t;
+  dw = static_cast<double>(n) / L;  // d w1 / dx; d w0 / dx = -dw
+}
+
+__global__ void cic_density_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                   const double* __restrict__ z, std::size_t np, double L, int n, double scale,
+                                   double* __restrict__ mesh) {
+  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  int ix[2], iy[2], iz[2];
+  double wx[2], wy[2], wz[2], d;
+  cic_axis(x[p], L, n, ix, wx, d);
+  cic_axis(y[p], L, n, iy, wy, d);
+  cic_axis(z[p], L, n, iz, wz, d);
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c)
+        atomicAdd(&mesh[(static_cast<std::size_t>(ix[a]) * n + iy[b]) * n + iz[c]], scale * wx[a] * wy[b] * wz[c]);
+}
+
+__global__ void add_constant_kernel(double* __restrict__ v, std::size_t n, double c) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) v[i] += c;
+}
+
+__global__ void cic_density_vjp_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                       const double* __restrict__ z, std::size_t np, double L, int n, double scale,
+                                       const double* __restrict__ g, double* __restrict__ ox,
+                                       double* __restrict__ oy, double* __restrict__ oz) {
+  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  int ix[2], iy[2], iz[2];
+  double wx[2], wy[2], wz[2], dx, dy, dz;
+  cic_axis(x[p], L, n, ix, wx, dx);
+  cic_axis(y[p], L, n, iy, wy, dy);
+  cic_axis(z[p], L, n, iz, wz, dz);
+  const double sx[2] = {-dx, dx}, sy[2] = {-dy, dy}, sz[2] = {-dz, dz};
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c) {
+        const double v = g[(static_cast<std::size_t>(ix[a]) * n + iy[b]) * n + iz[c]];
+        gx += sx[a] * wy[b] * wz[c] * v;
+        gy += wx[a] * sy[b] * wz[c] * v;
+        gz += wx[a] * wy[b] * sz[c] * v;
+      }
+  ox[p] = scale * gx;
+  oy[p] = scale * gy;
+  oz[p] = scale * gz;
+}
+
 constexpr int kRedThreads = 256;
 constexpr int kRedBlocks = 1024;
 
@@ -273,6 +335,26 @@ std::size_t ewald_scratch_bytes(int modes) {
   const std::size_t side = 2 * static_cast<std::size_t>(modes) + 1;
   const std::size_t half = (side * side * side) / 2;
   return (2 * half + 8) * sizeof(double);
+}
+
+void cic_density_device(const double* x, const double* y, const double* z, std::size_t np, double L, int n,
+                        double* delta, hipStream_t s) {
+  const std::size_t cells = static_cast<std::size_t>(n) * n * n;
+  fill_f64(delta, 0.0, cells, s);
+  if (np == 0) return;
+  const double scale = static_cast<double>(cells) / static_cast<double>(np);  // 1 / mean count
+  cic_density_kernel<<<grid_for(np, 256), 256, 0, s>>>(x, y, z, np, L, n, scale, delta);
+  CHUI_LAUNCH_CHECK("cic_density_kernel");
+  add_constant_kernel<<<grid_for(cells, 256), 256, 0, s>>>(delta, cells, -1.0);
+  CHUI_LAUNCH_CHECK("add_constant_kernel");
+}
+
+void cic_density_vjp_device(const double* x, const double* y, const double* z, std::size_t np, double L, int n,
+                            const double* g, double* ox, double* oy, double* oz, hipStream_t s) {
+  if (np == 0) return;
+  const double scale = static_cast<double>(n) * n * n / static_cast<double>(np);
+  cic_density_vjp_kernel<<<grid_for(np, 256), 256, 0, s>>>(x, y, z, np, L, n, scale, g, ox, oy, oz);
+  CHUI_LAUNCH_CHECK("cic_density_vjp_kernel");
 }
 
 void ewald_device(const double* x, const double* y, const double* z, const double* m, std::size_t n,

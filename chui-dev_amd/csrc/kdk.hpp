@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "primitives.hpp"
 
 namespace chui {
 
@@ -14,6 +15,13 @@ void mass_momentum_device(const double* m, const double* vx, const double* vy, c
                           double* partial_scratch, double out_host[4], hipStream_t s);
 
 std::size_t ewald_scratch_bytes(int modes);
+// delta(c) = count(c) * n^3 / np - 1 on an n^3 mesh (row-major, no padding),
+// CIC weights, unit-weight particles.
+void cic_density_device(const double* x, const double* y, const double* z, std::size_t np, double L, int n,
+                        double* delta, hipStream_t s);
+// out_p = d/dx_p sum_c g(c) delta(c) for the density above.
+void cic_density_vjp_device(const double* x, const double* y, const double* z, std::size_t np, double L, int n,
+                            const double* g, double* ox, double* oy, double* oz, hipStream_t s);
 void ewald_device(const double* x, const double* y, const double* z, const double* m, std::size_t n,
                   const uint32_t* targets, std::size_t nt, const double L[3], double G, double alpha,
                   int real_images, int modes, double eps, double* ax, double* ay, double* az, void* scratch,

@@ -676,6 +676,52 @@ void z_c2r(T* data, const PmGeometry& g, const Tables<T>& tb, bool premul, hipSt
   CHUI_LAUNCH_CHECK("fft_z_c2r_kernel");
 }
 
+// Forward r2c FFT of the real mesh in A followed by the Green's function with
+// its normalisation (the operator up to the gradient), in place.
+template <class T>
+void forward_green(T* A, const PmGeometry& g, const PmOptions& o, const Tables<T>& tb, hipStream_t s) {
+  using V = typename C2<T>::type;
+  V* Ac = reinterpret_cast<V*>(A);
+  const int nx = g.n[0], ny = g.n[1];
+  const std::size_t nzc = g.nzc;
+  const double M = static_cast<double>(g.n[0]) * g.n[1] * g.n[2];
+  const double cell_volume = (g.L[0] * g.L[1] * g.L[2]) / M;
+  z_r2c<T>(A, g, tb, s);
+  LineArgs ya{};
+  ya.n = ny; ya.lg = g.lg[1]; ya.line_stride = nzc; ya.inner_count = nzc; ya.outer_stride = static_cast<std::size_t>(ny) * nzc;
+  ya.inverse = 0; ya.premul = 0; ya.green = 0; ya.nzc = nzc;
+  launch_lines<T>(Ac, Ac, ya, nx, tb.twy, nullptr, tb, s);
+  LineArgs xa{};
+  xa.n = nx; xa.lg = g.lg[0]; xa.line_stride = static_cast<std::size_t>(ny) * nzc; xa.inner_count = static_cast<std::size_t>(ny) * nzc;
+  xa.outer_stride = 0; xa.inverse = 0; xa.premul = 0; xa.green = 1; xa.nzc = nzc;
+  xa.pref = -4.0 * kPi * o.G;
+  xa.split = o.split_scale;
+  xa.fold = 1.0 / (cell_volume * M);
+  xa.deconv = o.deconvolution;
+  launch_lines<T>(Ac, Ac, xa, 1, tb.twx, nullptr, tb, s);
+}
+
+// nu_hat = i (kx mu0_hat + ky mu1_hat + kz mu2_hat), written over mu0_hat: the
+// transpose of the three gradient operators -i k_a (same Nyquist rule), summed.
+template <class T>
+__global__ __launch_bounds__(256) void divergence_kernel(typename C2<T>::type* __restrict__ a0,
+                                                         const typename C2<T>::type* __restrict__ a1,
+                                                         const typename C2<T>::type* __restrict__ a2, int ny,
+                                                         std::size_t nzc, std::size_t cells, const T* __restrict__ dkx,
+                                                         const T* __restrict__ dky, const T* __restrict__ dkz) {
+  using V = typename C2<T>::type;
+  const std::size_t c = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  const std::size_t kz = c % nzc, r = c / nzc;
+  const std::size_t iy = r % static_cast<std::size_t>(ny), ix = r / static_cast<std::size_t>(ny);
+  const V u0 = a0[c], u1 = a1[c], u2 = a2[c];
+  const T kx = dkx[ix], ky = dky[iy], kz_ = dkz[kz];
+  V v;
+  v.x = -(kx * u0.y + ky * u1.y + kz_ * u2.y);
+  v.y = kx * u0.x + ky * u1.x + kz_ * u2.x;
+  a0[c] = v;
+}
+
 template <class T>
 void solve_spectral(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const Tables<T>& tb, hipStream_t s) {
   using V = typename C2<T>::type;
@@ -687,26 +733,16 @@ void solve_spectral(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, con
   V* Cc = reinterpret_cast<V*>(Cb);
   const int nx = g.n[0], ny = g.n[1];
   const std::size_t nzc = g.nzc;
-  const double M = static_cast<double>(g.n[0]) * g.n[1] * g.n[2];
-  const double cell_volume = (g.L[0] * g.L[1] * g.L[2]) / M;
-  // forward z (r2c)
-  z_r2c<T>(A, g, tb, s);
-  // forward y: lines along y, inner = kz (contiguous), outer = ix
+  // forward z (r2c), y and x lines, Green's function (Phi in A)
+  forward_green<T>(A, g, o, tb, s);
+  // lines along y: inner = kz (contiguous), outer = ix; along x: inner = (iy, kz)
   LineArgs ya{};
   ya.n = ny; ya.lg = g.lg[1]; ya.line_stride = nzc; ya.inner_count = nzc; ya.outer_stride = static_cast<std::size_t>(ny) * nzc;
   ya.inverse = 0; ya.premul = 0; ya.green = 0; ya.nzc = nzc;
-  launch_lines<T>(Ac, Ac, ya, nx, tb.twy, nullptr, tb, s);
-  // forward x + Green: lines along x, inner = (iy, kz)
-  LineArgs xa{};
-  xa.n = nx; xa.lg = g.lg[0]; xa.line_stride = static_cast<std::size_t>(ny) * nzc; xa.inner_count = static_cast<std::size_t>(ny) * nzc;
-  xa.outer_stride = 0; xa.inverse = 0; xa.premul = 0; xa.green = 1; xa.nzc = nzc;
-  xa.pref = -4.0 * kPi * o.G;
-  xa.split = o.split_scale;
-  xa.fold = 1.0 / (cell_volume * M);
-  xa.deconv = o.deconvolution;
-  launch_lines<T>(Ac, Ac, xa, 1, tb.twx, nullptr, tb, s);
+  LineArgs xi{};
+  xi.n = nx; xi.lg = g.lg[0]; xi.line_stride = static_cast<std::size_t>(ny) * nzc; xi.inner_count = static_cast<std::size_t>(ny) * nzc;
+  xi.outer_stride = 0; xi.nzc = nzc;
   // inverse x: Qx = IFFT_x(-i kx Phi) -> B ; P = IFFT_x(Phi) -> C
-  LineArgs xi = xa;
   xi.inverse = 1; xi.green = 0; xi.premul = 1;
   launch_lines<T>(Ac, Bc, xi, 1, tb.twx, tb.dkx, tb, s);
   xi.premul = 0;
@@ -765,6 +801,40 @@ void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, cons
   }
   solve_spectral<double>(w, g, o, tb, s);
   w.valid = true;
+}
+
+void pm_adjoint_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double4* const comp[3],
+                     std::size_t n, const DepositUnits& units, hipStream_t s) {
+  using V = double2;
+  reserve<double>(w, g);
+  const Tables<double> tb = upload_tables<double>(w, g, o, s);
+  for (int a = 0; a < 3; ++a) {
+    double* mu = w.buf[a].as<double>();
+    CHUI_HIP(hipMemsetAsync(mu, 0, g.complex_cells() * 2 * sizeof(double), s));
+    if (n > 0) {
+      if (units.count == 0) throw std::logic_error("PM adjoint deposit without deposit units");
+      deposit_kernel<double, double4><<<units.blocks, 256, 0, s>>>(deposit_args(g, o), comp[a], 0, 0.0, units, mu);
+      CHUI_LAUNCH_CHECK("deposit_kernel");
+    }
+    forward_green<double>(mu, g, o, tb, s);
+  }
+  V* A = w.buf[0].as<V>();
+  const std::size_t cells = g.complex_cells();
+  divergence_kernel<double><<<grid_for(cells, 256), 256, 0, s>>>(A, w.buf[1].as<V>(), w.buf[2].as<V>(), g.n[1],
+                                                                g.nzc, cells, tb.dkx, tb.dky, tb.dkz);
+  CHUI_LAUNCH_CHECK("divergence_kernel");
+  LineArgs xi{};
+  xi.n = g.n[0]; xi.lg = g.lg[0]; xi.line_stride = static_cast<std::size_t>(g.n[1]) * g.nzc;
+  xi.inner_count = static_cast<std::size_t>(g.n[1]) * g.nzc; xi.outer_stride = 0; xi.inverse = 1; xi.premul = 0;
+  xi.green = 0; xi.nzc = g.nzc;
+  launch_lines<double>(A, A, xi, 1, tb.twx, nullptr, tb, s);
+  LineArgs yi{};
+  yi.n = g.n[1]; yi.lg = g.lg[1]; yi.line_stride = g.nzc; yi.inner_count = g.nzc;
+  yi.outer_stride = static_cast<std::size_t>(g.n[1]) * g.nzc; yi.inverse = 1; yi.premul = 0; yi.green = 0;
+  yi.nzc = g.nzc;
+  launch_lines<double>(A, A, yi, g.n[0], tb.twy, nullptr, tb, s);
+  z_c2r<double>(w.buf[0].as<double>(), g, tb, false, s);
+  w.valid = false;  // holds nu, not a force field
 }
 
 void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint4* src, int uniform_mass,

@@ -69,6 +69,13 @@ constexpr uint32_t kDepositTileBytes = 49152;
 // means every source carries mass um.
 void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double4* src, int uniform_mass,
                    double um, std::size_t n, const DepositUnits& units, hipStream_t s);
+// Adjoint of the PM force field for the force VJP (FP64): deposits the three
+// components of lam (comp[a] = tree-ordered (x, y, z, lam_a)), applies the
+// transposed operator and leaves the real adjoint potential
+// nu = sum_a S_a^T mu_a in buf[0] (padded rows), with S_a the operator that
+// maps the deposited mass mesh to force component a.
+void pm_adjoint_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double4* const comp[3],
+                     std::size_t n, const DepositUnits& units, hipStream_t s);
 // Mixed: packed 32-bit fixed-point periodic coordinates + float mass bits,
 // FP32 mesh and FFTs.
 void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint4* src, int uniform_mass,

@@ -78,6 +78,29 @@ __device__ __forceinline__ double residual_inv_r3(double r2, double eps, double 
   return softened_inv_r3(r2, eps) - long_inv_r3(sqrt(r2), a);
 }
 
+// d/dt of residual_inv_r3 at t = r^2. With q = r/(2a) and
+// L(r) = erf(q) - 2q exp(-q^2)/sqrt(pi):
+//   d/dt (t + eps^2)^{-3/2} = -1.5 (t + eps^2)^{-5/2},
+//   d/dt [L(r)/r^3] = exp(-q^2) / (4 sqrt(pi) a^3 r^2) - 1.5 L(r)/r^5,
+// the latter from the series of L(r)/r^3 = H(q^2)/(8a^3) below q = 1/8.
+__device__ __forceinline__ double residual_inv_r3_dt(double r2, double eps, double a) {
+  const double s = r2 + eps * eps;
+  const double dsoft = s > 0.0 ? -1.5 / (s * s * sqrt(s)) : 0.0;
+  const double r = sqrt(r2);
+  const double q = (0.5 * r) / a;
+  double dlong;
+  if (q < 0.125) {
+    constexpr double c = 2.25675833419102514779231780624309034;
+    const double u = q * q;
+    const double dh = c * (-1.0 / 5.0 + u * (2.0 / 14.0 + u * (-3.0 / 54.0 + u * (4.0 / 264.0 +
+                      u * (-5.0 / 1560.0 + u * (6.0 / 10800.0 - 7.0 * u / 85680.0))))));
+    dlong = (0.125 / (a * a * a)) * dh * (0.25 / (a * a));  // dH/du * du/dt, u = t / (4 a^2)
+  } else {
+    dlong = exp(-q * q) * kInvSqrtPi / (4.0 * a * a * a * r2) - 1.5 * long_factor(r, a) / (r2 * r2 * r);
+  }
+  return dsoft - dlong;
+}
+
 // Minimum image (tree_pm_geometry.hpp:12-25): nearbyint semantics, with the
 // division skipped whenever |d| <= L lies outside the half-box tie band.
 __device__ __forceinline__ double mi64(double d, double L) {
@@ -214,6 +237,62 @@ __device__ void interp64(const WalkParams& P, const double* __restrict__ fx, con
         gz += w * fz[i];
       }
     }
+}
+
+// Stencil weights and their derivatives with respect to the mesh coordinate g
+// (multiply by n/L for a length derivative).
+template <int SCHEME>
+__device__ __forceinline__ void stencil64_grad(double g, int n, int (&idx)[SCHEME + 2], double (&w)[SCHEME + 2],
+                                               double (&dw)[SCHEME + 2]) {
+  stencil64<SCHEME>(g, n, idx, w);
+  if constexpr (SCHEME == 0) {
+    dw[0] = -1.0;
+    dw[1] = 1.0;
+  } else {
+    const double d = g - floor(g + 0.5);
+    dw[0] = -(0.5 - d);
+    dw[1] = -2.0 * d;
+    dw[2] = 0.5 + d;
+  }
+}
+
+// Transpose of the PM force's Jacobian for one source k (FP64):
+//   s_k = grad_{x_k} sum_c W(x_k, c) [lam_k . F(c) + m_k nu(c)]
+// with F the force meshes (interpolation's own dependence on x_k) and nu the
+// adjoint potential sum_a S_a^T mu_a of the deposited lam (deposit's
+// dependence on x_k).
+template <int SCHEME>
+__device__ void vjp_gather64(const WalkParams& P, const double* __restrict__ fx, const double* __restrict__ fy,
+                             const double* __restrict__ fz, const double* __restrict__ nu, double lx, double ly,
+                             double lz, double mk, double x, double y, double z, double& sx, double& sy,
+                             double& sz) {
+  constexpr int K = SCHEME + 2;
+  int ix[K], iy[K], iz[K];
+  double wx[K], wy[K], wz[K], dx[K], dy[K], dz[K];
+  const double hx = static_cast<double>(P.mesh[0]) / P.L[0];
+  const double hy = static_cast<double>(P.mesh[1]) / P.L[1];
+  const double hz = static_cast<double>(P.mesh[2]) / P.L[2];
+  stencil64_grad<SCHEME>(wrap_position(x, P.L[0]) * hx, P.mesh[0], ix, wx, dx);
+  stencil64_grad<SCHEME>(wrap_position(y, P.L[1]) * hy, P.mesh[1], iy, wy, dy);
+  stencil64_grad<SCHEME>(wrap_position(z, P.L[2]) * hz, P.mesh[2], iz, wz, dz);
+  sx = sy = sz = 0.0;
+#pragma unroll
+  for (int a = 0; a < K; ++a)
+#pragma unroll
+    for (int b = 0; b < K; ++b) {
+      const std::size_t row = (static_cast<std::size_t>(ix[a]) * P.mesh[1] + iy[b]) * P.mesh_row;
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const std::size_t i = row + iz[c];
+        const double u = lx * fx[i] + ly * fy[i] + lz * fz[i] + mk * nu[i];
+        sx += dx[a] * wy[b] * wz[c] * u;
+        sy += wx[a] * dy[b] * wz[c] * u;
+        sz += wx[a] * wy[b] * dz[c] * u;
+      }
+    }
+  sx *= hx;
+  sy *= hy;
+  sz *= hz;
 }
 
 // Stencil from a 32-bit fixed-point coordinate on a power-of-two mesh: the
@@ -378,6 +457,9 @@ struct WalkData {
   const uint32_t *tpos, *tslot;
   const double *f64x, *f64y, *f64z;
   const float *f32x, *f32y, *f32z;
+  // force VJP: tree-ordered lam (x, y, z, unused) and the adjoint potential
+  const double4* lam64;
+  const double* nu64;
 };
 
 __device__ __forceinline__ uint32_t next_node(uint32_t n, uint32_t nxt) {
@@ -385,7 +467,12 @@ __device__ __forceinline__ uint32_t next_node(uint32_t n, uint32_t nxt) {
 }
 
 // ---- FP64 walk (reference arithmetic) ---------------------------------------------
-template <int SCHEME>
+// JVP: instead of the acceleration, each target k accumulates the transpose of
+// the acceleration Jacobian applied to lam (see vjp_gather64 for the PM part).
+// For the residual pairs, with r = x_j - x_k, t = r^2 and the pair force
+// G m_j F(t) r, the contribution is G (F w + 2 F'(t) (r . w) r) with
+// w = m_k lam_j - m_j lam_k; the sum runs over the same interaction lists.
+template <int SCHEME, bool JVP>
 __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, WalkData D, WalkOutputs out,
                                                                unsigned long long* __restrict__ counters) {
   __shared__ uint32_t queue[kWalkWaves][kQueue][kWave];
@@ -400,12 +487,21 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
   uint32_t nxt = valid ? 0u : P.node_count;
   double px = 0.0, py = 0.0, pz = 0.0;
   double gx = 0.0, gy = 0.0, gz = 0.0;
+  double4 lk_ = make_double4(0.0, 0.0, 0.0, 0.0);  // JVP: lam of the target
+  double mk = 0.0;                                  // JVP: mass of the target
   if (valid) {
     const double4 t = D.src64[tp];
     px = t.x;
     py = t.y;
     pz = t.z;
-    if (P.pm_enabled) interp64<SCHEME>(P, D.f64x, D.f64y, D.f64z, px, py, pz, gx, gy, gz);
+    if constexpr (JVP) {
+      lk_ = D.lam64[tp];
+      mk = D.uniform_mass ? P.uniform_mass : t.w;
+      if (P.pm_enabled)
+        vjp_gather64<SCHEME>(P, D.f64x, D.f64y, D.f64z, D.nu64, lk_.x, lk_.y, lk_.z, mk, px, py, pz, gx, gy, gz);
+    } else {
+      if (P.pm_enabled) interp64<SCHEME>(P, D.f64x, D.f64y, D.f64z, px, py, pz, gx, gy, gz);
+    }
   }
   double ax = 0.0, ay = 0.0, az = 0.0;
   const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
@@ -420,9 +516,18 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
         const double r2 = qx * qx + qy * qy + qz * qz;
         const double mj = D.uniform_mass ? P.uniform_mass : sj.w;
         const double f = residual_inv_r3(r2, P.eps, P.a) * P.G;
-        ax += f * mj * qx;
-        ay += f * mj * qy;
-        az += f * mj * qz;
+        if constexpr (JVP) {
+          const double4 lj = D.lam64[j];
+          const double wx = mk * lj.x - mj * lk_.x, wy = mk * lj.y - mj * lk_.y, wz = mk * lj.z - mj * lk_.z;
+          const double c = 2.0 * residual_inv_r3_dt(r2, P.eps, P.a) * P.G * (qx * wx + qy * wy + qz * wz);
+          ax += f * wx + c * qx;
+          ay += f * wy + c * qy;
+          az += f * wz + c * qz;
+        } else {
+          ax += f * mj * qx;
+          ay += f * mj * qy;
+          az += f * mj * qz;
+        }
       }
     }
     queued += qlen;
@@ -464,7 +569,9 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
             const double qx = mi64(sj.x - px, L0), qy = mi64(sj.y - py, L1), qz = mi64(sj.z - pz, L2);
             const double r2 = qx * qx + qy * qy + qz * qz;
             if (keep && j != tp && r2 <= P.rcut2) {
-              if (qx == 0.0 && qy == 0.0 && qz == 0.0) {
+              // a coincident pair has a zero force but, in the JVP, the
+              // Jacobian F(0) I
+              if (!JVP && qx == 0.0 && qy == 0.0 && qz == 0.0) {
                 ++cnt[2];  // counted coincident pair, zero vector
               } else {
                 queue[w][qlen][lane] = j;
@@ -855,10 +962,10 @@ double gaussian_h_slope(double t) {
   return (0.5 * c * std::exp(-t) - 1.5 * gaussian_h_exact(t)) / t;
 }
 
-template <int SCHEME>
+template <int SCHEME, bool JVP = false>
 void launch64(const WalkParams& p, const WalkData& d, const WalkOutputs& out, unsigned long long* counters,
               hipStream_t s) {
-  walk64_kernel<SCHEME><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(p, d, out, counters);
+  walk64_kernel<SCHEME, JVP><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(p, d, out, counters);
   CHUI_LAUNCH_CHECK("walk64_kernel");
 }
 template <int SCHEME, bool CUBIC>
@@ -914,6 +1021,27 @@ void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int unifo
   d.f64z = fz;
   if (p.assignment == 0) launch64<0>(p, d, out, counters, s);
   else launch64<1>(p, d, out, counters, s);
+}
+
+void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,
+                   const double* fx, const double* fy, const double* fz, const double* nu, const WalkOutputs& out,
+                   unsigned long long* counters, hipStream_t s) {
+  if (p.n_targets == 0) return;
+  WalkData d{};
+  d.link = t.link.as<NodeLink>();
+  d.geo64 = t.geo64.as<NodeGeo64>();
+  d.mom64 = t.mom64.as<NodeMom64>();
+  d.quad64 = t.quad64.as<NodeQuad64>();
+  d.order = t.order.as<uint32_t>();
+  d.src64 = src;
+  d.uniform_mass = uniform_mass;
+  d.f64x = fx;
+  d.f64y = fy;
+  d.f64z = fz;
+  d.lam64 = lam;
+  d.nu64 = nu;
+  if (p.assignment == 0) launch64<0, true>(p, d, out, counters, s);
+  else launch64<1, true>(p, d, out, counters, s);
 }
 
 void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int uniform_mass, const float4* htab,

@@ -76,6 +76,12 @@ void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int unifo
                const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
                const double* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s);
 
+// Transpose of the total-force Jacobian applied to lam (FP64, all targets, tree
+// order identity). src and lam are tree-ordered; fx..fz the forward force
+// meshes and nu the adjoint potential on the same mesh; multipole order 0.
+void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,
+                   const double* fx, const double* fy, const double* fz, const double* nu, const WalkOutputs& out,
+                   unsigned long long* counters, hipStream_t s);
 void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int uniform_mass, const float4* htab,
                 const uint32_t* target_pos, const uint32_t* target_slot, const float* fx, const float* fy,
                 const float* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s);

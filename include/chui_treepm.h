@@ -238,6 +238,35 @@ chui_status chui_treepm_mass_momentum_device(chui_treepm_context* ctx, const dou
                                              const double* vel_x, const double* vel_y, const double* vel_z,
                                              uint64_t n, double out_host[4]);
 
+/* ---- differentiation (no reference counterpart) --------------------------- */
+/* The reference has no adjoint (no autograd or adjoint code in src/); these
+ * entry points serve the differentiable forward of python/cosmosim/adjoint.py.
+ *
+ * Transpose of the Jacobian of the total acceleration a (PM + short-range
+ * residual, as chui_treepm_solve_device computes it with every source active)
+ * with respect to the source positions, applied to lam:
+ *   out_k = sum_i (d a_i / d x_k)^T lam_i,  all arrays device FP64 in caller order.
+ * The tree and both PM fields are rebuilt at (x, y, z). Requires FP64 precision
+ * and multipole_order 0: with the strict policy no node is then accepted
+ * (tree_pm_acceptance.hpp:62-66), so the short-range force is the exact pair
+ * sum within r_cut; otherwise CHUI_ERR_UNSUPPORTED. */
+chui_status chui_treepm_force_vjp_device(chui_treepm_context* ctx, const double* pos_x, const double* pos_y,
+                                         const double* pos_z, const double* mass, uint64_t n,
+                                         const double* lam_x, const double* lam_y, const double* lam_z,
+                                         double* out_x, double* out_y, double* out_z,
+                                         const chui_treepm_options* options);
+/* CIC density contrast of n unit-weight particles on a periodic mesh^3 grid of
+ * side box (row-major, no padding): delta = count * mesh^3 / n - 1, with the
+ * CIC stencil of pm_solver.cpp:925-936. */
+chui_status chui_treepm_cic_density_device(chui_treepm_context* ctx, const double* pos_x, const double* pos_y,
+                                          const double* pos_z, uint64_t n, double box, int32_t mesh,
+                                          double* delta);
+/* Its transpose: out_k = d/dx_k sum_c grad_delta(c) delta(c). */
+chui_status chui_treepm_cic_density_vjp_device(chui_treepm_context* ctx, const double* pos_x,
+                                              const double* pos_y, const double* pos_z, uint64_t n, double box,
+                                              int32_t mesh, const double* grad_delta, double* out_x,
+                                              double* out_y, double* out_z);
+
 /* ---- parity / inspection entry points ----------------------------------- */
 /* Periodic unwrap of one axis (device arrays): out = unwrapped coordinate, returns anchor. */
 chui_status chui_treepm_unwrap_device(chui_treepm_context* ctx, const double* in, uint64_t n,
