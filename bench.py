@@ -55,6 +55,10 @@ def main():
     ap.add_argument("--ewald-samples", type=int, default=8192)
     ap.add_argument("--cpu-n-side", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grad-n-side", type=int, default=64,
+                    help="particles per side of the differentiable KDK workload (0 skips the grad/fwd figure)")
+    ap.add_argument("--grad-directions", type=int, default=16,
+                    help="random unit directions of the central-difference check of the gradient")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="one plain device-to-device copy of this size before the timed region; its HBM counters "
                          "scale the walk's FETCH_SIZE/WRITE_SIZE readings under rocprofv3 --pmc")
@@ -154,6 +158,11 @@ def main():
     path_bytes = 185 * n + 64 * M
     step_ms = 1e3 * t_mixed / args.steps
 
+    grad = None
+    if args.grad_n_side > 0:
+        grad = measure_gradient(args.grad_n_side, args.grad_directions)
+        log(f"grad/fwd {grad}")
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -188,7 +197,7 @@ def main():
                    "parallelism": "single GPU"},
         "fp64_gpu": fp64,
         "force_error": {"mixed_vs_fp64": err_mix_vs_fp64, "ewald": ewald},
-        "grad_fwd_ratio": None,
+        "grad_fwd_ratio": grad,
         "stages_ms_mixed": {k: v / max(prof_mixed["solves"], 1) for k, v in prof_mixed.items() if k.endswith("_ms")},
         "roofline": {"bound": "hbm", "kernel": "walkmix_kernel", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(w.name),
@@ -206,6 +215,66 @@ def main():
     }
     coord.close()
     print(json.dumps(line), flush=True)
+
+
+def measure_gradient(n_side, directions):
+    """grad/fwd ratio of the differentiable workload: n_side^3 particles, box
+    50 (n_side/64) Mpc/h, mesh 2 n_side, ten KDK steps a = 0.05 -> 0.5 with
+    checkpoints every two steps, loss = sum (delta_final - delta_target)^2 on an
+    n_side^3 CIC mesh; FP64 throughout (the force VJP is FP64). Also reports
+    central differences of the loss along random unit directions against the
+    gradient, at the 1e-3 spacing step and at 1e-6 spacings."""
+    import torch
+
+    from cosmosim import adjoint
+
+    prob, psi, target = adjoint.adjoint_problem(n_side=n_side, steps=10, a_begin=0.05, a_end=0.5,
+                                                checkpoint_every=2)
+    try:
+        psi64 = psi.to(torch.float64)
+
+        def run(with_grad):
+            p = psi64.clone().requires_grad_(with_grad)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            loss = adjoint.density_loss(p, target, prob)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            grad = None
+            if with_grad:
+                (grad,) = torch.autograd.grad(loss, p)
+                torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            return float(loss), grad, t1 - t0, t2 - t1
+
+        run(True)  # warm-up: buffers and tables
+        prob.force_evaluations = prob.vjp_evaluations = 0
+        loss, grad, t_fwd_g, t_bwd = run(True)
+        evals = {"forward_force_evaluations": prob.force_evaluations, "force_vjps": prob.vjp_evaluations}
+        _, _, t_fwd, _ = run(False)
+        rng = np.random.default_rng(20261202)
+        spacing = prob.box / n_side
+        fd = {}
+        for label, h in (("step_1e-3_spacing", 1e-3 * spacing), ("step_1e-6_spacing", 1e-6 * spacing)):
+            errs = []
+            for _ in range(directions):
+                d = rng.standard_normal(psi.shape)
+                d /= np.linalg.norm(d)
+                dt = torch.tensor(d, dtype=torch.float64, device=psi.device)
+                with torch.no_grad():
+                    lp = float(adjoint.density_loss(psi64 + h * dt, target, prob))
+                    lm = float(adjoint.density_loss(psi64 - h * dt, target, prob))
+                an = float((grad * dt).sum())
+                errs.append(abs((lp - lm) / (2 * h) - an) / max(abs(an), 1e-300))
+            fd[label] = {"directions": directions, "median_relative_error": float(np.median(errs)),
+                         "max_relative_error": float(np.max(errs))}
+        return {"value": (t_fwd_g + t_bwd) / t_fwd, "backward_over_forward": t_bwd / t_fwd,
+                "forward_ms": 1e3 * t_fwd, "gradient_ms": 1e3 * (t_fwd_g + t_bwd), "loss": loss,
+                "workload": f"adjoint_{n_side}^3_box{prob.box:g}_mesh{prob.mesh}, 10 KDK steps a 0.05->0.5, "
+                            f"checkpoints every 2, FP64",
+                **evals, "central_differences": fd}
+    finally:
+        prob.close()
 
 
 def measured_traffic(workload):
