@@ -128,6 +128,41 @@ class Diagnostics(C.Structure):
         return out
 
 
+ESTIMATOR_RESIDUAL_DERIVATIVE = 0
+ESTIMATOR_RELATIVE_MONOPOLE = 1
+CUTOFF_CONTAIN_NODE = 0
+CUTOFF_COM_DISTANCE = 1
+GROUP_WALK_STATUS = {0: "not_requested", 1: "active", 2: "no_local_tree", 3: "incompatible_policy",
+                     6: "cutoff_outside_table", 8: "lookup_scaling_unrepresentable"}
+
+
+class AdaptiveOptions(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("error_estimator", C.c_int32),
+        ("maximum_opening_angle", C.c_double),
+        ("cutoff_node_policy", C.c_int32),
+        ("gaussian_pair_lookup", C.c_int32),
+        ("radial_coefficients", C.c_int32),
+        ("group_walk", C.c_int32),
+        ("group_target_size", C.c_uint32),
+        ("reserved0", C.c_uint32),
+        ("group_max_extent_cutoff_fraction", C.c_double),
+    ]
+
+
+class WalkCounters(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("group_walk_status", C.c_uint32)] + [(n, C.c_uint64) for n in (
+        "visited_nodes", "cutoff_pruned_nodes", "direct_pair_evaluations", "accepted_internal_multipoles",
+        "accepted_leaves", "group_walk_groups", "group_walk_grouped_targets", "group_walk_target_group_size_max",
+        "group_walk_fallback_targets", "group_walk_fallback_history", "group_walk_fallback_geometry",
+        "group_walk_candidate_leaves", "group_walk_culled_leaves", "group_walk_candidate_particles",
+        "group_walk_list_entries_max")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "struct_size"}
+
+
 class KdkFactors(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "scale_factor_begin", "scale_factor_midpoint", "scale_factor_end", "drift_factor",
@@ -159,6 +194,11 @@ SIGNATURES = {
                           C.POINTER(Options), _I32, C.POINTER(Diagnostics)],
     "chui_treepm_solve_device": [_P, _P, _P, _P, _P, _U64, _P, _U64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                  C.POINTER(Options), _I32, C.POINTER(Diagnostics)],
+    "chui_treepm_default_adaptive_options": [C.POINTER(AdaptiveOptions)],
+    "chui_treepm_adaptive_options_set": [C.POINTER(AdaptiveOptions), C.c_char_p, C.c_char_p],
+    "chui_treepm_solve_adaptive_device": [_P, _P, _P, _P, _P, _U64, _P, _U64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                          _P, C.POINTER(Options), C.POINTER(AdaptiveOptions), _I32,
+                                          C.POINTER(WalkCounters)],
     "chui_treepm_profiling": [_P, _I32],
     "chui_treepm_profile_read": [_P, C.POINTER(_D), C.POINTER(_U64)],
     "chui_treepm_kdk_factors": [_D, _D, _D, _D, _D, C.POINTER(KdkFactors)],
@@ -218,6 +258,20 @@ def default_options(**overrides) -> Options:
         else:
             setattr(opts, key, value)
     return opts
+
+
+def default_adaptive_options(**overrides) -> AdaptiveOptions:
+    ad = AdaptiveOptions()
+    check(load().chui_treepm_default_adaptive_options(C.byref(ad)))
+    for key, value in overrides.items():
+        setattr(ad, key, value)
+    return ad
+
+
+def set_adaptive_param(ad: AdaptiveOptions, key: str, value) -> None:
+    if isinstance(value, bool):
+        value = "true" if value else "false"
+    check(load().chui_treepm_adaptive_options_set(C.byref(ad), key.encode(), str(value).encode()))
 
 
 def set_param(opts: Options, key: str, value) -> None:

@@ -34,6 +34,8 @@ class ForceAccumulator:
     active: object = None
     long_range: tuple | None = None
     short_range: tuple | None = None
+    # previous_acceleration_magnitude_code: |a_prev| per active slot (adaptive policy)
+    previous_acceleration: object = None
 
 
 @dataclass
@@ -82,7 +84,8 @@ class TreePmCoordinator:
 
     def solve_active_set_with_pm_cadence(self, pos_x, pos_y, pos_z, mass, accumulator: ForceAccumulator,
                                          options: _lib.Options, refresh_long_range_field: bool = True,
-                                         want_diagnostics: bool = False) -> SolveResult:
+                                         want_diagnostics: bool = False,
+                                         adaptive_options: _lib.AdaptiveOptions | None = None) -> SolveResult:
         n = int(pos_x.shape[0])
         active = accumulator.active
         n_active = int(active.shape[0]) if active is not None else n
@@ -93,6 +96,19 @@ class TreePmCoordinator:
             diag = _lib.Diagnostics()
             diag.struct_size = C.sizeof(_lib.Diagnostics)
         on_device = not isinstance(pos_x, np.ndarray)
+        if adaptive_options is not None:
+            # adaptive acceptance / group walk (chui_treepm_solve_adaptive_device)
+            if not on_device:
+                raise ValueError("the adaptive entry point takes device arrays")
+            counters = _lib.WalkCounters()
+            counters.struct_size = C.sizeof(_lib.WalkCounters)
+            _lib.check(self._lib.chui_treepm_solve_adaptive_device(
+                self._h, _ptr(pos_x), _ptr(pos_y), _ptr(pos_z), _ptr(mass), n, _ptr(active), n_active,
+                _ptr(accumulator.accel[0]), _ptr(accumulator.accel[1]), _ptr(accumulator.accel[2]),
+                _ptr(lr[0]), _ptr(lr[1]), _ptr(lr[2]), _ptr(sr[0]), _ptr(sr[1]), _ptr(sr[2]),
+                _ptr(accumulator.previous_acceleration), C.byref(options), C.byref(adaptive_options),
+                1 if refresh_long_range_field else 0, C.byref(counters) if want_diagnostics else None))
+            return SolveResult(diagnostics=counters.as_dict() if want_diagnostics else {})
         fn = self._lib.chui_treepm_solve_device if on_device else self._lib.chui_treepm_solve
         _lib.check(fn(self._h, _ptr(pos_x), _ptr(pos_y), _ptr(pos_z), _ptr(mass), n, _ptr(active), n_active,
                       _ptr(accumulator.accel[0]), _ptr(accumulator.accel[1]), _ptr(accumulator.accel[2]),
@@ -101,9 +117,10 @@ class TreePmCoordinator:
                       C.byref(diag) if diag is not None else None))
         return SolveResult(diagnostics=diag.as_dict() if diag is not None else {})
 
-    def solve_active_set(self, pos_x, pos_y, pos_z, mass, accumulator, options, want_diagnostics=False):
+    def solve_active_set(self, pos_x, pos_y, pos_z, mass, accumulator, options, want_diagnostics=False,
+                         adaptive_options=None):
         return self.solve_active_set_with_pm_cadence(pos_x, pos_y, pos_z, mass, accumulator, options, True,
-                                                     want_diagnostics)
+                                                     want_diagnostics, adaptive_options)
 
     def profiling(self, enable: bool):
         _lib.check(self._lib.chui_treepm_profiling(self._h, 1 if enable else 0))

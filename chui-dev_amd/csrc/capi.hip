@@ -50,6 +50,9 @@ struct chui_treepm_context {
   DeviceBuffer hx, hy, hz, hm, hactive, hout[9];
   DeviceBuffer tpos, tslot, tkeys_alt, tvals_alt, tsort_scratch, flag;
   DeviceBuffer counters, red_scratch, ewald_scratch;
+  // adaptive policy: Gaussian Hermite knots, leaf of each tree position,
+  // group-walk fallback targets and counters, source bounds
+  DeviceBuffer gtab, leaf_of_pos, fb_pos, fb_slot, gcounters, bounds;
   hipEvent_t ev[kEvCount] = {};
   bool events = false;
   // profiling mode: events recorded every solve, harvested at the next solve's
@@ -159,8 +162,13 @@ void validate_options(const chui_treepm_options* o, const int mesh[3]) {
   const double lmin = std::min({o->box_size[0], o->box_size[1], o->box_size[2]});
   if (s.rcut >= std::nextafter(0.5 * lmin, 0.0))
     throw std::invalid_argument("periodic TreePM requires r_cut < half the shortest box axis");
-  if (o->acceptance_policy != CHUI_ACCEPT_STRICT)
-    throw chui::UnsupportedError("adaptive acceptance policy is not built on the GPU path yet");
+  if (o->acceptance_policy != CHUI_ACCEPT_STRICT && o->acceptance_policy != CHUI_ACCEPT_ADAPTIVE)
+    throw std::invalid_argument("Unknown TreePM acceptance policy");
+  if (o->acceptance_policy == CHUI_ACCEPT_ADAPTIVE && o->precision != CHUI_PRECISION_FP64)
+    throw chui::UnsupportedError("the adaptive acceptance policy is built for FP64 precision only");
+  if (o->acceptance_policy == CHUI_ACCEPT_ADAPTIVE &&
+      !(std::isfinite(o->relative_force_tolerance) && o->relative_force_tolerance > 0.0))
+    throw std::invalid_argument("numerics.treepm_tree_relative_force_tolerance must be finite and > 0");
   if (o->opening_criterion == CHUI_OPEN_RELATIVE && o->multipole_order == 2) {
     // Without an acceleration history the relative criterion falls back to the
     // COM-distance MAC (tree_interaction_common.hpp:60-62); handled in the walk.
@@ -310,13 +318,68 @@ chui::WalkParams base_walk_params(const Ctx* c, const chui_treepm_options* o, co
   return wp;
 }
 
+// TreePmOptions defaults of the adaptive/group-walk fields (tree_pm_coupling.hpp:248-270).
+chui_treepm_adaptive_options adaptive_defaults() {
+  chui_treepm_adaptive_options a{};
+  a.struct_size = sizeof(a);
+  a.error_estimator = CHUI_ESTIMATOR_RESIDUAL_DERIVATIVE;
+  a.maximum_opening_angle = 0.25;
+  a.cutoff_node_policy = CHUI_CUTOFF_CONTAIN_NODE;
+  a.group_max_extent_cutoff_fraction = 0.5;
+  return a;
+}
+
+void validate_adaptive(const chui_treepm_adaptive_options& a) {
+  if (a.struct_size != sizeof(chui_treepm_adaptive_options))
+    throw std::invalid_argument("chui_treepm_adaptive_options.struct_size does not match this library");
+  if (!(std::isfinite(a.maximum_opening_angle) && a.maximum_opening_angle > 0.0))
+    throw std::invalid_argument("numerics.treepm_adaptive_maximum_opening_angle must be finite and > 0");
+  if (a.error_estimator != CHUI_ESTIMATOR_RESIDUAL_DERIVATIVE && a.error_estimator != CHUI_ESTIMATOR_RELATIVE_MONOPOLE)
+    throw std::invalid_argument("Unknown TreePM adaptive error estimator");
+  if (a.cutoff_node_policy != CHUI_CUTOFF_CONTAIN_NODE && a.cutoff_node_policy != CHUI_CUTOFF_COM_DISTANCE)
+    throw std::invalid_argument("Unknown TreePM cutoff node policy");
+  if (!(std::isfinite(a.group_max_extent_cutoff_fraction) && a.group_max_extent_cutoff_fraction > 0.0))
+    throw std::invalid_argument("numerics.treepm_group_walk_max_extent_cutoff_fraction must be finite and > 0");
+}
+
+// Per-axis min/max of the caller's source coordinates (the bounds of the
+// bounded periodic metric, tree_pm_geometry.hpp:27-32), as order keys.
+__global__ void source_bounds_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                     const double* __restrict__ z, std::size_t n,
+                                     unsigned long long* __restrict__ keys) {
+  const double* src[3] = {x, y, z};
+  uint64_t lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0, 0, 0};
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    for (int a = 0; a < 3; ++a) {
+      const uint64_t k = chui::order_key(src[a][i]);
+      lo[a] = k < lo[a] ? k : lo[a];
+      hi[a] = k > hi[a] ? k : hi[a];
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    const uint64_t l = chui::wave_min(lo[a]), h = chui::wave_max(hi[a]);
+    if (chui::lane_id() == 0) {
+      atomicMin(&keys[a], static_cast<unsigned long long>(l));
+      atomicMax(&keys[3 + a], static_cast<unsigned long long>(h));
+    }
+  }
+}
+
 // Core force evaluation on device arrays (tree_pm_coupling.cpp:2282-3519 for
 // one rank: PM refresh/reuse, PM interpolation, unwrap + build, residual walk).
 void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z, const double* m, uint64_t n,
                        const uint32_t* active, uint64_t n_active, void* ax, void* ay, void* az, void* pmx,
                        void* pmy, void* pmz, void* srx, void* sry, void* srz, const chui_treepm_options* o,
-                       int refresh, chui_treepm_diagnostics* diag) {
+                       int refresh, chui_treepm_diagnostics* diag, const double* prev = nullptr,
+                       const chui_treepm_adaptive_options* adaptive = nullptr,
+                       chui_treepm_walk_counters* wcount = nullptr) {
   validate_options(o, c->mesh);
+  const bool adaptive_policy = o->acceptance_policy == CHUI_ACCEPT_ADAPTIVE;
+  const chui_treepm_adaptive_options ad = adaptive ? *adaptive : adaptive_defaults();
+  validate_adaptive(ad);
+  if (wcount && wcount->struct_size != sizeof(chui_treepm_walk_counters))
+    throw std::invalid_argument("chui_treepm_walk_counters.struct_size does not match this library");
   if (n > 0xFFFFFFFEull) throw std::overflow_error("Tree gravity source count exceeds the 32-bit tree-index contract");
   if (active == nullptr) n_active = n;
   if (n_active > 0 && (!ax || !ay || !az)) throw std::invalid_argument("TreePM accumulator output lanes are null");
@@ -441,12 +504,102 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   chui::WalkParams wp = base_walk_params(c, o, sp, g.row);
   wp.n_targets = static_cast<uint32_t>(n_active);
   wp.identity_targets = identity;
-  wp.count = diag != nullptr ? 1 : 0;
+  wp.count = (diag != nullptr || wcount != nullptr) ? 1 : 0;
   wp.uniform_mass = um;
   wp.uniform_mass_f = static_cast<float>(um);
   wp.pm_enabled = 1;
   chui::WalkOutputs out{ax, ay, az, pmx, pmy, pmz, srx, sry, srz};
-  if (n_active > 0 && n > 0) {
+  uint32_t group_status = ad.group_walk ? CHUI_GROUP_WALK_INCOMPATIBLE_POLICY : CHUI_GROUP_WALK_NOT_REQUESTED;
+  const bool group_counts = adaptive_policy || wcount != nullptr;
+  if (group_counts) {
+    c->gcounters.reserve(chui::kGroupCounters * sizeof(unsigned long long));
+    chui::fill_u64(c->gcounters.as<uint64_t>(), 0ull, chui::kGroupCounters, s);
+  }
+  if (n_active > 0 && n > 0 && adaptive_policy) {
+    // Adaptive acceptance (tree_pm_acceptance.hpp:54-139) and, when eligible,
+    // the group walk (eligibility tree_pm_coupling.cpp:3839-3869, :4282-4286).
+    wp.estimator = ad.error_estimator;
+    wp.com_cut = ad.cutoff_node_policy == CHUI_CUTOFF_COM_DISTANCE ? 1 : 0;
+    wp.lookup = ad.gaussian_pair_lookup ? 1 : 0;
+    wp.radial = ad.radial_coefficients ? 1 : 0;
+    wp.amax = ad.maximum_opening_angle;
+    wp.alpha = o->relative_force_tolerance;
+    wp.accel_floor = o->relative_force_acceleration_floor;
+    wp.inv2a2 = (0.25 / sp.a) / sp.a;
+    wp.inv8a3 = ((0.125 / sp.a) / sp.a) / sp.a;
+    if (!c->gtab.ptr) {
+      const std::vector<double> knots = chui::gaussian_hermite_knots();
+      c->gtab.reserve(knots.size() * sizeof(double));
+      CHUI_HIP(hipMemcpyAsync(c->gtab.ptr, knots.data(), knots.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      CHUI_HIP(hipStreamSynchronize(s));
+    }
+    const double t_cut = (0.25 * wp.rcut2 / sp.a) / sp.a;
+    if (!ad.group_walk) group_status = CHUI_GROUP_WALK_NOT_REQUESTED;
+    else if (c->tree.node_count == 0) group_status = CHUI_GROUP_WALK_NO_LOCAL_TREE;
+    else if (ad.error_estimator != CHUI_ESTIMATOR_RELATIVE_MONOPOLE || !ad.gaussian_pair_lookup)
+      group_status = CHUI_GROUP_WALK_INCOMPATIBLE_POLICY;
+    else if (!(std::isfinite(t_cut) && t_cut < 63.0)) group_status = CHUI_GROUP_WALK_CUTOFF_OUTSIDE_TABLE;
+    else if (!(std::isnormal(wp.inv2a2) && std::isnormal(wp.inv8a3)))
+      group_status = CHUI_GROUP_WALK_LOOKUP_SCALING_UNREPRESENTABLE;
+    else group_status = CHUI_GROUP_WALK_ACTIVE;
+    const double2* gtab = c->gtab.as<double2>();
+    const double* f0 = static_cast<const double*>(pw.force[0]);
+    const double* f1 = static_cast<const double*>(pw.force[1]);
+    const double* f2 = static_cast<const double*>(pw.force[2]);
+    auto* wcnt = reinterpret_cast<unsigned long long*>(c->counters.ptr);
+    if (group_status == CHUI_GROUP_WALK_ACTIVE) {
+      c->leaf_of_pos.reserve(n * sizeof(uint32_t));
+      chui::build_leaf_of_pos(c->tree, c->leaf_of_pos.as<uint32_t>(), s);
+      c->fb_pos.reserve(n_active * sizeof(uint32_t));
+      c->fb_slot.reserve(n_active * sizeof(uint32_t));
+      c->bounds.reserve(8 * sizeof(unsigned long long));
+      const unsigned long long init[8] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+      CHUI_HIP(hipMemcpyAsync(c->bounds.ptr, init, sizeof(init), hipMemcpyHostToDevice, s));
+      auto* bkeys = c->bounds.as<unsigned long long>();
+      source_bounds_kernel<<<std::min<unsigned>(chui::grid_for(n, 256), 1024u), 256, 0, s>>>(x, y, z, n, bkeys);
+      CHUI_LAUNCH_CHECK("source_bounds_kernel");
+      unsigned long long keys[8];
+      CHUI_HIP(hipMemcpyAsync(keys, bkeys, sizeof(keys), hipMemcpyDeviceToHost, s));
+      CHUI_HIP(hipStreamSynchronize(s));
+      chui::GroupParams gp{};
+      gp.block = static_cast<uint32_t>(o->residual_block_size);
+      gp.group_size = ad.group_target_size;
+      gp.max_extent = ad.group_max_extent_cutoff_fraction * std::sqrt(wp.rcut2);
+      gp.tolg = o->relative_force_tolerance / o->gravitational_constant;
+      gp.amax2 = ad.maximum_opening_angle * ad.maximum_opening_angle;
+      gp.contain = wp.com_cut ? 0 : 1;
+      for (int a = 0; a < 3; ++a) {
+        uint64_t lo = keys[a], hi = keys[3 + a];
+        auto decode = [](uint64_t k) {
+          const uint64_t b = (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+          double d;
+          std::memcpy(&d, &b, sizeof(d));
+          return d;
+        };
+        gp.lo[a] = decode(lo);
+        gp.hi[a] = decode(hi);
+      }
+      auto* fb_count = reinterpret_cast<uint32_t*>(bkeys + 6);  // zeroed above
+      chui::walk_fp64_group(c->tree, wp, gp, c->src64.as<double4>(), uniform ? 1 : 0, gtab, prev,
+                            c->leaf_of_pos.as<uint32_t>(), target_pos, target_slot, f0, f1, f2, out,
+                            c->fb_pos.as<uint32_t>(), c->fb_slot.as<uint32_t>(), fb_count,
+                            reinterpret_cast<unsigned long long*>(c->gcounters.ptr), s);
+      uint32_t nfb = 0;
+      CHUI_HIP(hipMemcpyAsync(&nfb, fb_count, sizeof(nfb), hipMemcpyDeviceToHost, s));
+      CHUI_HIP(hipStreamSynchronize(s));
+      if (nfb > n_active) throw std::logic_error("TreePM group walk fallback list overflow");
+      if (nfb > 0) {
+        chui::WalkParams fw = wp;
+        fw.n_targets = nfb;
+        fw.identity_targets = 0;
+        chui::walk_fp64_adaptive(c->tree, fw, c->src64.as<double4>(), uniform ? 1 : 0, gtab, prev,
+                                 c->fb_pos.as<uint32_t>(), c->fb_slot.as<uint32_t>(), f0, f1, f2, out, wcnt, s);
+      }
+    } else {
+      chui::walk_fp64_adaptive(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, gtab, prev, target_pos,
+                               target_slot, f0, f1, f2, out, wcnt, s);
+    }
+  } else if (n_active > 0 && n > 0) {
     if (!mixed) {
       chui::walk_fp64(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, target_pos, target_slot,
                       static_cast<const double*>(pw.force[0]), static_cast<const double*>(pw.force[1]),
@@ -482,6 +635,33 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     }
   }
   record(c, kEvWalk);
+  if (wcount) {
+    unsigned long long wc[chui::kWalkCounters + 1] = {};
+    unsigned long long gc[chui::kGroupCounters] = {};
+    CHUI_HIP(hipMemcpyAsync(wc, c->counters.ptr, sizeof(wc), hipMemcpyDeviceToHost, s));
+    CHUI_HIP(hipMemcpyAsync(gc, c->gcounters.ptr, sizeof(gc), hipMemcpyDeviceToHost, s));
+    CHUI_HIP(hipStreamSynchronize(s));
+    if (wc[chui::kWalkCounters]) throw std::logic_error("TreePM walk lost progress (wave reduction selected no pending node)");
+    chui_treepm_walk_counters r{};
+    r.struct_size = sizeof(r);
+    r.group_walk_status = group_status;
+    r.visited_nodes = wc[0] + gc[0];
+    r.cutoff_pruned_nodes = wc[1] + gc[1];
+    r.direct_pair_evaluations = wc[2] + gc[2];
+    r.accepted_internal_multipoles = wc[3] + gc[3];
+    r.accepted_leaves = wc[4] + gc[4];
+    r.group_walk_groups = gc[5];
+    r.group_walk_grouped_targets = gc[6];
+    r.group_walk_target_group_size_max = gc[7];
+    r.group_walk_fallback_targets = gc[8];
+    r.group_walk_fallback_history = gc[9];
+    r.group_walk_fallback_geometry = gc[10];
+    r.group_walk_candidate_leaves = gc[11];
+    r.group_walk_culled_leaves = gc[12];
+    r.group_walk_candidate_particles = gc[13];
+    r.group_walk_list_entries_max = gc[14];
+    *wcount = r;
+  }
   if (check_walk_progress_) {
     unsigned long long bad = 0;
     CHUI_HIP(hipMemcpyAsync(&bad, reinterpret_cast<unsigned long long*>(c->counters.ptr) + chui::kWalkCounters,
@@ -816,6 +996,79 @@ chui_status chui_treepm_solve_device(chui_treepm_context* c, const double* x, co
     set_device(c);
     solve_device_impl(c, x, y, z, m, n, active, n_active, ax, ay, az, pmx, pmy, pmz, srx, sry, srz, o, refresh,
                       diag);
+  });
+}
+
+chui_status chui_treepm_default_adaptive_options(chui_treepm_adaptive_options* a) {
+  return guarded([&] {
+    if (!a) throw std::invalid_argument("TreePM adaptive options pointer is null");
+    *a = adaptive_defaults();
+  });
+}
+
+chui_status chui_treepm_adaptive_options_set(chui_treepm_adaptive_options* a, const char* key_c,
+                                             const char* value_c) {
+  return guarded([&] {
+    if (!a || !key_c || !value_c) throw std::invalid_argument("null argument");
+    std::string key(key_c), value(value_c);
+    if (key.rfind("numerics.", 0) == 0) key = key.substr(9);
+    auto num = [&]() {
+      char* end = nullptr;
+      const double v = std::strtod(value.c_str(), &end);
+      if (end == value.c_str()) throw std::invalid_argument("value of " + key + " is not a number: " + value);
+      return v;
+    };
+    auto boolean = [&]() {
+      if (value == "true" || value == "1" || value == "yes" || value == "on") return 1;
+      if (value == "false" || value == "0" || value == "no" || value == "off") return 0;
+      throw std::invalid_argument("value of " + key + " is not a boolean: " + value);
+    };
+    // config.cpp:1068-1094 (estimator and cutoff policy names), :1741-1754 (keys)
+    if (key == "treepm_adaptive_maximum_opening_angle") {
+      a->maximum_opening_angle = num();
+    } else if (key == "treepm_adaptive_error_estimator") {
+      if (value == "residual_derivative") a->error_estimator = CHUI_ESTIMATOR_RESIDUAL_DERIVATIVE;
+      else if (value == "relative_monopole") a->error_estimator = CHUI_ESTIMATOR_RELATIVE_MONOPOLE;
+      else throw std::invalid_argument("key 'numerics.treepm_adaptive_error_estimator': invalid value '" + value +
+                                       "' (supported: residual_derivative, relative_monopole)");
+    } else if (key == "treepm_cutoff_node_policy") {
+      if (value == "contain") a->cutoff_node_policy = CHUI_CUTOFF_CONTAIN_NODE;
+      else if (value == "com_distance") a->cutoff_node_policy = CHUI_CUTOFF_COM_DISTANCE;
+      else throw std::invalid_argument("key 'numerics.treepm_cutoff_node_policy': invalid value '" + value +
+                                       "' (supported: contain, com_distance)");
+    } else if (key == "treepm_group_walk_enabled") {
+      a->group_walk = boolean();
+    } else if (key == "treepm_group_walk_target_size") {
+      const double v = num();
+      if (!(v >= 0.0 && v <= 256.0) || v == 1.0 || v != std::floor(v))
+        throw std::invalid_argument(
+            "numerics.treepm_group_walk_target_size must be 0 (one source leaf) or in [2, treepm_residual_block_size]");
+      a->group_target_size = static_cast<uint32_t>(v);
+    } else if (key == "treepm_gaussian_pair_lookup_enabled") {
+      a->gaussian_pair_lookup = boolean();
+    } else if (key == "treepm_radial_coefficients_enabled") {
+      a->radial_coefficients = boolean();
+    } else {
+      throw std::invalid_argument("unknown TreePM adaptive option key: " + key);
+    }
+  });
+}
+
+chui_status chui_treepm_solve_adaptive_device(chui_treepm_context* c, const double* x, const double* y,
+                                              const double* z, const double* m, uint64_t n, const uint32_t* active,
+                                              uint64_t n_active, double* ax, double* ay, double* az, double* pmx,
+                                              double* pmy, double* pmz, double* srx, double* sry, double* srz,
+                                              const double* previous_acceleration, const chui_treepm_options* o,
+                                              const chui_treepm_adaptive_options* adaptive, int32_t refresh,
+                                              chui_treepm_walk_counters* counters) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    if (!adaptive) throw std::invalid_argument("TreePM adaptive options pointer is null");
+    if (o && o->precision != CHUI_PRECISION_FP64)
+      throw chui::UnsupportedError("chui_treepm_solve_adaptive_device is built for FP64 precision only");
+    solve_device_impl(c, x, y, z, m, n, active, n_active, ax, ay, az, pmx, pmy, pmz, srx, sry, srz, o, refresh,
+                      nullptr, previous_acceleration, adaptive, counters);
   });
 }
 

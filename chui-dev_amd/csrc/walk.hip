@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
 #include <vector>
 
 namespace chui {
@@ -174,6 +175,158 @@ __device__ __forceinline__ bool accept64(double h, double offset, bool inside, b
     else mac = ((2.0 * h + offset) / sqrt(r2 + 1.0e-30)) < P.theta;
   }
   return mac && env_ok && contained && !inside && isfinite(r2);
+}
+
+// ---- adaptive acceptance pieces (FP64) -------------------------------------------
+// Gaussian Hermite table (tree_pm_split_kernel.hpp:130-183): knots (H, step dH/dt)
+// at t = i / 64, i = 0..4096.
+constexpr int kGaussCells = 4096;
+__device__ __forceinline__ double gauss_hermite(const double2* __restrict__ tab, double t) {
+  const double cell = t * 64.0;
+  const int i = static_cast<int>(cell);
+  const double u = cell - static_cast<double>(i), u2 = u * u, u3 = u2 * u;
+  const double2 k0 = tab[i], k1 = tab[i + 1];
+  return (2 * u3 - 3 * u2 + 1) * k0.x + (u3 - 2 * u2 + u) * k0.y + (-2 * u3 + 3 * u2) * k1.x + (u3 - u2) * k1.y;
+}
+// dimensionlessCoefficient: series below t = 1/64, analytic from t = 64
+__device__ __forceinline__ double gauss_coefficient(const double2* __restrict__ tab, double t) {
+  if (t < 0.015625) return series_h(t);
+  if (!(t < 64.0)) return long_inv_r3(sqrt(t), 0.5);
+  return gauss_hermite(tab, t);
+}
+// Leaf-pair residual of the per-target walk with the Gaussian lookup
+// (tree_pm_coupling.cpp:4216-4227), r2 > 0.
+__device__ __forceinline__ double table_residual64(const double2* __restrict__ tab, double r2, const WalkParams& P) {
+  const double t = r2 * P.inv2a2;
+  double lr;
+  if (t < 64.0) {
+    lr = gauss_coefficient(tab, t) * P.inv8a3;
+  } else {
+    const double tt = 0.25 * (r2 / P.a) / P.a;  // longRangeInvR3 beyond the table
+    lr = (!(tt >= 0.0) || tt >= 64.0) ? long_inv_r3(sqrt(r2), P.a)
+                                      : ((0.125 * gauss_coefficient(tab, tt) / P.a) / P.a) / P.a;
+  }
+  return softened_inv_r3(r2, P.eps) - lr;
+}
+
+// treePmResidualRadialCoefficients (tree_pm_split_kernel.hpp:245-282).
+__device__ __forceinline__ void residual_radial64(double x, double eps, double a, double& value, double& first,
+                                                  double& second) {
+  const double d = x + eps * eps;
+  const double soft = 1.0 / (d * sqrt(d));
+  const double t = (0.25 * x / a) / a;
+  constexpr double c = 2.25675833419102514779231780624309034;
+  double h, dh, ddh;
+  if (t < 0.015625) {
+    constexpr double k[8] = {1.0 / 3.0, -1.0 / 5.0, 1.0 / 14.0, -1.0 / 54.0,
+                             1.0 / 264.0, -1.0 / 1560.0, 1.0 / 10800.0, -1.0 / 85680.0};
+    h = k[7]; dh = 0.0; ddh = 0.0;
+#pragma unroll
+    for (int i = 7; i > 0; --i) {
+      ddh = ddh * t + 2.0 * dh;
+      dh = dh * t + h;
+      h = h * t + k[i - 1];
+    }
+    h *= c; dh *= c; ddh *= c;
+  } else {
+    const double q = sqrt(t);
+    const double e = exp(-t);
+    h = (erf(q) - 0.5 * c * q * e) / (t * q);
+    dh = (0.5 * c * e - 1.5 * h) / t;
+    ddh = (-0.5 * c * e - 2.5 * dh) / t;
+  }
+  const double scale = ((0.125 / a) / a) / a;
+  const double dt_dx = (0.25 / a) / a;
+  value = soft - scale * h;
+  first = -3.0 * soft / d - 2.0 * scale * dt_dx * dh;
+  second = 15.0 * soft / (d * d) - 4.0 * scale * dt_dx * dt_dx * ddh;
+}
+
+// monopolePlusQuadrupoleAccelPeriodic (tree_pm_coupling.cpp:829-894); non-finite
+// coefficients use the reference expansion.
+__device__ __forceinline__ void node_accel_radial64(const NodeMom64& M, const NodeQuad64& Q, double dx, double dy,
+                                                    double dz, const WalkParams& P, double& ax, double& ay,
+                                                    double& az) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double safe_r2 = fmax(r2, 1.0e-30);
+  double v, fo, cr;
+  residual_radial64(safe_r2, P.eps, P.a, v, fo, cr);
+  if (!isfinite(v) || !isfinite(fo) || !isfinite(cr)) {
+    node_accel64(M, Q, dx, dy, dz, P, ax, ay, az);
+    return;
+  }
+  const double pref = P.G;
+  double cx = pref * M.mass * v * dx;
+  double cy = pref * M.mass * v * dy;
+  double cz = pref * M.mass * v * dz;
+  if (P.quad) {
+    const double tr = Q.trace;
+    const double mxx = (Q.qxx + tr) / 3.0, mxy = Q.qxy / 3.0, mxz = Q.qxz / 3.0;
+    const double myy = (Q.qyy + tr) / 3.0, myz = Q.qyz / 3.0, mzz = (Q.qzz + tr) / 3.0;
+    const double mdx = mxx * dx + mxy * dy + mxz * dz;
+    const double mdy = mxy * dx + myy * dy + myz * dz;
+    const double mdz = mxz * dx + myz * dy + mzz * dz;
+    const double dmd = dx * mdx + dy * mdy + dz * mdz;
+    cx += pref * (mdx * fo + 0.5 * tr * dx * fo + 0.5 * dx * dmd * cr);
+    cy += pref * (mdy * fo + 0.5 * tr * dy * fo + 0.5 * dy * dmd * cr);
+    cz += pref * (mdz * fo + 0.5 * tr * dz * fo + 0.5 * dz * dmd * cr);
+  }
+  ax += cx;
+  ay += cy;
+  az += cz;
+}
+
+// residualAccuracyProxy (tree_pm_acceptance.hpp:14-51) for uniform softening.
+__device__ __forceinline__ double residual_proxy64(double h, double offset, double mass, double r2,
+                                                   const WalkParams& P) {
+  if (!(r2 > 0.0) || !isfinite(r2)) return INFINITY;
+  const double a = P.a;
+  const double r = sqrt(r2);
+  const double d = r2 + P.eps * P.eps;
+  const double inv_d5 = 1.0 / (d * d * sqrt(d));
+  const double inv_r3 = 1.0 / (r2 * r);
+  const double inv_r4 = 1.0 / (r2 * r2);
+  const double lf = long_factor(r, a);
+  const double q = r / (2.0 * a);
+  const double e = exp(-q * q);
+  const double a3 = a * a * a;
+  const double s1 = -0.5 * kInvSqrtPi * r2 * e / a3;
+  const double s2 = kInvSqrtPi * e * (-r / a3 + 0.25 * r * r2 / (a3 * a * a));
+  const double first = -3.0 * r * inv_d5 + s1 * inv_r3 - 3.0 * (-lf) * inv_r4;
+  const double second = -3.0 * inv_d5 + 15.0 * r2 * inv_d5 / d + s2 * inv_r3 - 6.0 * s1 * inv_r4 +
+                        12.0 * (-lf) * inv_r4 / r;
+  const double width = 2.0 * h;
+  const double rho = 1.7320508075688772935274463415058723 * h + offset;
+  const double proxy = P.G * mass * fmax(width * width * inv_r4,
+                                         0.5 * rho * rho * (3.0 * fabs(first) + fabs(r * second - first)));
+  return isfinite(proxy) && proxy >= 0.0 ? proxy : INFINITY;
+}
+
+// acceptTreePmNode (tree_pm_acceptance.hpp:54-139) under the adaptive policy for
+// an internal node that neither holds the target nor leaves the cutoff
+// container (those are opened first, tree_pm_coupling.cpp:4094-4104).
+__device__ __forceinline__ bool accept_adaptive64(double h, double offset, double mass, double r2, bool history,
+                                                  double prev, const WalkParams& P) {
+  const double r = sqrt(fmax(r2, 1.0e-30));
+  const bool env_ok = (2.0 * h / r) < P.amax;
+  bool mac;
+  if (history && isfinite(prev) && prev > P.accel_floor) {
+    const double allowance = P.alpha * prev;
+    const double width = 2.0 * h;
+    const double inv_r4 = 1.0 / (r2 * r2);
+    const double lower = P.G * mass * (width * width * inv_r4);
+    if (isfinite(allowance) && isfinite(lower) && lower > allowance) {
+      mac = false;
+    } else if (P.estimator == 1) {  // relative monopole
+      mac = isfinite(allowance) && isfinite(lower);
+    } else {
+      const double est = residual_proxy64(h, offset, mass, r2, P);
+      mac = isfinite(est) && isfinite(allowance) && est <= allowance;
+    }
+  } else {
+    mac = ((2.0 * h + offset) / sqrt(r2 + 1.0e-30)) < P.theta;
+  }
+  return mac && env_ok && isfinite(r2);
 }
 
 // ---- PM interpolation (pm_solver.cpp:925-951, :5900-5957) --------------------
@@ -460,6 +613,9 @@ struct WalkData {
   // force VJP: tree-ordered lam (x, y, z, unused) and the adjoint potential
   const double4* lam64;
   const double* nu64;
+  // adaptive policy: Gaussian Hermite knots and |a_prev| per output slot (nullable)
+  const double2* gtab;
+  const double* prev;
 };
 
 __device__ __forceinline__ uint32_t next_node(uint32_t n, uint32_t nxt) {
@@ -472,7 +628,11 @@ __device__ __forceinline__ uint32_t next_node(uint32_t n, uint32_t nxt) {
 // For the residual pairs, with r = x_j - x_k, t = r^2 and the pair force
 // G m_j F(t) r, the contribution is G (F w + 2 F'(t) (r . w) r) with
 // w = m_k lam_j - m_j lam_k; the sum runs over the same interaction lists.
-template <int SCHEME, bool JVP>
+// ADAPT: the adaptive acceptance policy with the target's acceleration history,
+// the cutoff node policy, optional Gaussian-table leaf pairs and the optional
+// radial-coefficient node expansion (tree_pm_coupling.cpp:4026-4272 with
+// TreePmAcceptancePolicy::kAdaptiveRelative).
+template <int SCHEME, bool JVP, bool ADAPT = false>
 __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, WalkData D, WalkOutputs out,
                                                                unsigned long long* __restrict__ counters) {
   __shared__ uint32_t queue[kWalkWaves][kQueue][kWave];
@@ -505,6 +665,15 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
   }
   double ax = 0.0, ay = 0.0, az = 0.0;
   const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
+  bool history = false;
+  double prev = 0.0;
+  if constexpr (ADAPT) {
+    if (valid && D.prev != nullptr) {
+      prev = D.prev[slot];
+      history = isfinite(prev);
+      if (!history) prev = 0.0;
+    }
+  }
 
   auto drain = [&]() {
     const uint32_t m = wave_max_u32(qlen);
@@ -515,7 +684,12 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
         const double qx = mi64(sj.x - px, L0), qy = mi64(sj.y - py, L1), qz = mi64(sj.z - pz, L2);
         const double r2 = qx * qx + qy * qy + qz * qz;
         const double mj = D.uniform_mass ? P.uniform_mass : sj.w;
-        const double f = residual_inv_r3(r2, P.eps, P.a) * P.G;
+        double f;
+        if constexpr (ADAPT) {
+          f = (P.lookup ? table_residual64(D.gtab, r2, P) : residual_inv_r3(r2, P.eps, P.a)) * P.G;
+        } else {
+          f = residual_inv_r3(r2, P.eps, P.a) * P.G;
+        }
         if constexpr (JVP) {
           const double4 lj = D.lam64[j];
           const double wx = mk * lj.x - mj * lk_.x, wy = mk * lj.y - mj * lk_.y, wz = mk * lj.z - mj * lk_.z;
@@ -588,20 +762,41 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
       if (keep) {
         const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
         const double mx = ax_ + h, my = ay_ + h, mz = az_ + h;
-        const bool contained = within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
-        const NodeMom64 M = D.mom64[n];
-        mdx = mi64(M.comx - px, L0);
-        mdy = mi64(M.comy - py, L1);
-        mdz = mi64(M.comz - pz, L2);
-        const double r2 = mdx * mdx + mdy * mdy + mdz * mdz;
-        take_mp = accept64(h, D.quad64[n].offset, inside, contained, r2, P);
-        nxt = take_mp ? n + lk.skip : n + 1;
+        if constexpr (ADAPT) {
+          const bool contained = P.com_cut || within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
+          bool cut = false;
+          if (!inside && contained) {
+            const NodeMom64 M = D.mom64[n];
+            mdx = mi64(M.comx - px, L0);
+            mdy = mi64(M.comy - py, L1);
+            mdz = mi64(M.comz - pz, L2);
+            const double r2 = mdx * mdx + mdy * mdy + mdz * mdz;
+            take_mp = accept_adaptive64(h, D.quad64[n].offset, M.mass, r2, history, prev, P);
+            // kComDistance: an accepted node whose COM lies beyond r_cut is pruned
+            if (take_mp && P.com_cut && r2 > P.rcut2) {
+              take_mp = false;
+              cut = true;
+              ++cnt[1];
+            }
+          }
+          nxt = (take_mp || cut) ? n + lk.skip : n + 1;
+        } else {
+          const bool contained = within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
+          const NodeMom64 M = D.mom64[n];
+          mdx = mi64(M.comx - px, L0);
+          mdy = mi64(M.comy - py, L1);
+          mdz = mi64(M.comz - pz, L2);
+          const double r2 = mdx * mdx + mdy * mdy + mdz * mdz;
+          take_mp = accept64(h, D.quad64[n].offset, inside, contained, r2, P);
+          nxt = take_mp ? n + lk.skip : n + 1;
+        }
       }
       if (__ballot(take_mp)) {
         drain();
         if (take_mp) {
           ++cnt[3];
-          node_accel64(D.mom64[n], D.quad64[n], mdx, mdy, mdz, P, ax, ay, az);
+          if (ADAPT && P.radial) node_accel_radial64(D.mom64[n], D.quad64[n], mdx, mdy, mdz, P, ax, ay, az);
+          else node_accel64(D.mom64[n], D.quad64[n], mdx, mdy, mdz, P, ax, ay, az);
         }
       }
     }
@@ -624,6 +819,379 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
     static_cast<double*>(out.sr_y)[slot] = ay;
     static_cast<double*>(out.sr_z)[slot] = az;
   }
+}
+
+// ---- group walk (FP64) -----------------------------------------------------------
+// One wave per residual block of visit-ordered targets (tree_pm_coupling.cpp:
+// 4451-4546). The wave splits its block into target groups (the targets of one
+// leaf, or up to group_size consecutive targets whose box stays within
+// max_extent per axis), then per group:
+//   1. checks every target's history and bounded metric; a failing group is
+//      appended to the per-target fallback list (tree_pm_coupling.cpp:4341-4370);
+//   2. walks the tree once for the group box (buildTreePmGroupInteractionList,
+//      tree_pm_group_walk.hpp:78-163). The walk is the stackless preorder walk
+//      of the per-target kernel made wave-cooperative: the 64 lanes decide 64
+//      consecutive preorder nodes at once, and the nodes actually visited are
+//      the chain p -> next(p) -> ... through them (next = q + 1 to open, q + skip
+//      otherwise). Listed leaves and accepted nodes are appended in preorder to
+//      an LDS buffer;
+//   3. whenever the buffer fills (and at the end) every target evaluates the
+//      buffered list (accumulateTreePmGroupCulledLeaves, :290-341, and the
+//      accepted-node expansion): a group of up to 16 targets uses four lanes per
+//      target, up to 32 two, each lane taking every S-th source of a leaf and
+//      every S-th node; the lanes' sums are added at the end.
+// Groups above 64 targets are evaluated 64 targets at a time, repeating the
+// (identical) tree walk for each.
+constexpr int kGroupLeafCap = 192;
+constexpr int kGroupNodeCap = 128;
+
+struct GroupLeaf {
+  double ox, oy, oz, h;  // cell centre relative to the group centre, cell half size
+  uint32_t b, e;         // tree-order source range
+};
+
+__device__ __forceinline__ double nearest_image(double d, double L) { return d - L * rint(d / L); }
+__device__ __forceinline__ double wrap_half(double d, double L) {
+  const double hl = 0.5 * L;
+  return d > hl ? d - L : (d < -hl ? d + L : d);
+}
+template <class T> __device__ __forceinline__ T scan_min(T v, int lane) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const T o = __shfl_up(v, off, kWave);
+    if (lane >= off) v = o < v ? o : v;
+  }
+  return v;
+}
+template <class T> __device__ __forceinline__ T scan_max(T v, int lane) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const T o = __shfl_up(v, off, kWave);
+    if (lane >= off) v = o > v ? o : v;
+  }
+  return v;
+}
+
+template <int SCHEME>
+__global__ __launch_bounds__(kWave) void group_walk64_kernel(WalkParams P, GroupParams G, WalkData D,
+                                                              const uint32_t* __restrict__ leaf_of_pos,
+                                                              WalkOutputs out, uint32_t* __restrict__ fb_pos,
+                                                              uint32_t* __restrict__ fb_slot,
+                                                              uint32_t* __restrict__ fb_count,
+                                                              unsigned long long* __restrict__ counters) {
+  __shared__ GroupLeaf leaves[kGroupLeafCap];
+  __shared__ uint32_t nodes[kGroupNodeCap];
+  const int lane = threadIdx.x;
+  const uint32_t bb = blockIdx.x * G.block;
+  const uint32_t be = min(bb + G.block, P.n_targets);
+  const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
+  const double cull2 = P.rcut2 * (1.0 + 1.0e-9);
+  const double tlim = 0x1.fffffffffffffp+5;  // largest double below 64
+  const double eps2 = P.eps * P.eps;
+  uint32_t cnt[kGroupCounters] = {};
+  unsigned long long size_max = 0, entries_max = 0;
+  auto tpos = [&](uint32_t v) { return P.identity_targets ? v : D.tpos[v]; };
+  auto tslot = [&](uint32_t v) { return P.identity_targets ? D.order[v] : D.tslot[v]; };
+
+  uint32_t v = bb;
+  while (v < be) {
+    const double4 p0 = D.src64[tpos(v)];
+    // ---- the group [v, ge) (tree_pm_coupling.cpp:4491-4523)
+    uint32_t ge;
+    if (G.group_size == 0) {
+      const uint32_t leaf0 = leaf_of_pos[tpos(v)];
+      ge = be;
+      for (uint32_t c = v + 1; c < be; c += kWave) {
+        const uint32_t k = c + lane;
+        const unsigned long long m = __ballot(k < be && leaf_of_pos[tpos(k)] != leaf0);
+        if (m) {
+          ge = c + static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+          break;
+        }
+      }
+    } else {
+      const uint32_t lim = min(be, v + G.group_size);
+      double lo0 = 0.0, lo1 = 0.0, lo2 = 0.0, hi0 = 0.0, hi1 = 0.0, hi2 = 0.0;
+      ge = lim;
+      for (uint32_t c = v + 1; c < lim; c += kWave) {
+        const uint32_t k = c + lane;
+        const bool in = k < lim;
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;  // 0 is neutral: every running box holds 0
+        if (in) {
+          const double4 q = D.src64[tpos(k)];
+          d0 = nearest_image(q.x - p0.x, L0);
+          d1 = nearest_image(q.y - p0.y, L1);
+          d2 = nearest_image(q.z - p0.z, L2);
+        }
+        const double a0 = scan_min(fmin(lo0, d0), lane), a1 = scan_min(fmin(lo1, d1), lane),
+                     a2 = scan_min(fmin(lo2, d2), lane);
+        const double b0 = scan_max(fmax(hi0, d0), lane), b1 = scan_max(fmax(hi1, d1), lane),
+                     b2 = scan_max(fmax(hi2, d2), lane);
+        const bool within = b0 - a0 <= G.max_extent && b1 - a1 <= G.max_extent && b2 - a2 <= G.max_extent;
+        const unsigned long long bad = __ballot(in && !within);
+        if (bad) {
+          ge = c + static_cast<uint32_t>(__ffsll(static_cast<long long>(bad)) - 1);
+          break;
+        }
+        lo0 = readlane_f64(a0, 63); lo1 = readlane_f64(a1, 63); lo2 = readlane_f64(a2, 63);
+        hi0 = readlane_f64(b0, 63); hi1 = readlane_f64(b1, 63); hi2 = readlane_f64(b2, 63);
+      }
+    }
+    const uint32_t count = ge - v;
+
+    // ---- history / bounded metric, minimum |a_prev| and the group box (:4333-4370)
+    int fall = 0;  // 1 history, 2 geometry (first failing target decides)
+    double amin = INFINITY, lo0 = 0.0, lo1 = 0.0, lo2 = 0.0, hi0 = 0.0, hi1 = 0.0, hi2 = 0.0;
+    for (uint32_t c = v; c < ge; c += kWave) {
+      const uint32_t k = c + lane;
+      const bool in = k < ge;
+      double pv = INFINITY, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+      bool hist = true, bounded = true;
+      if (in) {
+        pv = D.prev != nullptr ? D.prev[tslot(k)] : NAN;
+        hist = isfinite(pv) && pv > P.accel_floor;
+        const double4 q = D.src64[tpos(k)];
+        bounded = G.lo[0] - q.x >= -L0 && G.hi[0] - q.x <= L0 && G.lo[1] - q.y >= -L1 && G.hi[1] - q.y <= L1 &&
+                  G.lo[2] - q.z >= -L2 && G.hi[2] - q.z <= L2;
+        d0 = nearest_image(q.x - p0.x, L0);
+        d1 = nearest_image(q.y - p0.y, L1);
+        d2 = nearest_image(q.z - p0.z, L2);
+      }
+      const unsigned long long bad = __ballot(in && !(hist && bounded));
+      if (bad) {
+        const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(bad)) - 1);
+        fall = readlane_u32(hist ? 2u : 1u, l);
+        break;
+      }
+      amin = fmin(amin, wave_min(in ? pv : INFINITY));
+      lo0 = fmin(lo0, wave_min(d0)); lo1 = fmin(lo1, wave_min(d1)); lo2 = fmin(lo2, wave_min(d2));
+      hi0 = fmax(hi0, wave_max(d0)); hi1 = fmax(hi1, wave_max(d1)); hi2 = fmax(hi2, wave_max(d2));
+    }
+    if (fall) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(fb_count, count);
+      base = readlane_u32(base, 0);
+      for (uint32_t c = v; c < ge; c += kWave) {
+        const uint32_t k = c + lane;
+        if (k < ge) {
+          fb_pos[base + (k - v)] = tpos(k);
+          fb_slot[base + (k - v)] = tslot(k);
+        }
+      }
+      if (lane == 0) {
+        cnt[8] += count;
+        cnt[fall == 1 ? 9 : 10] += count;
+      }
+      v = ge;
+      continue;
+    }
+    const double c0x = p0.x + 0.5 * (lo0 + hi0), c0y = p0.y + 0.5 * (lo1 + hi1), c0z = p0.z + 0.5 * (lo2 + hi2);
+    const double hg0 = 0.5 * (hi0 - lo0), hg1 = 0.5 * (hi1 - lo1), hg2 = 0.5 * (hi2 - lo2);
+    if (lane == 0) {
+      cnt[5] += 1;
+      cnt[6] += count;
+    }
+    size_max = count > size_max ? count : size_max;
+
+    // ---- targets, 64 at a time
+    for (uint32_t t0 = v; t0 < ge; t0 += kWave) {
+      const uint32_t T = min(ge - t0, static_cast<uint32_t>(kWave));
+      const int S = T <= 16 ? 4 : (T <= 32 ? 2 : 1);
+      const int TT = kWave / S;
+      const int t = lane & (TT - 1), s = lane / TT;
+      const bool valid = static_cast<uint32_t>(t) < T;
+      const bool first = t0 == v;  // count the (repeated) tree walk once per group
+      uint32_t tp = 0, slot = 0;
+      double px = 0.0, py = 0.0, pz = 0.0, ox = 0.0, oy = 0.0, oz = 0.0;
+      if (valid) {
+        tp = tpos(t0 + t);
+        slot = tslot(t0 + t);
+        const double4 q = D.src64[tp];
+        px = q.x;
+        py = q.y;
+        pz = q.z;
+        ox = nearest_image(px - c0x, L0);
+        oy = nearest_image(py - c0y, L1);
+        oz = nearest_image(pz - c0z, L2);
+      }
+      double ax = 0.0, ay = 0.0, az = 0.0;
+      uint32_t listed = 0, nl = 0, nm = 0, p = 0;
+
+      auto evaluate = [&]() {
+        __syncthreads();
+        for (uint32_t e = 0; e < nl; ++e) {
+          const GroupLeaf Lf = leaves[e];
+          if (!valid) continue;
+          const double gx = fmax(0.0, fabs(wrap_half(Lf.ox - ox, L0)) - Lf.h);
+          const double gy = fmax(0.0, fabs(wrap_half(Lf.oy - oy, L1)) - Lf.h);
+          const double gz = fmax(0.0, fabs(wrap_half(Lf.oz - oz, L2)) - Lf.h);
+          const bool cull = gx * gx + gy * gy + gz * gz > cull2;
+          if (s == 0) {
+            ++cnt[11];
+            if (cull) ++cnt[12];
+            else cnt[13] += Lf.e - Lf.b;
+          }
+          if (cull) continue;
+          double sx = 0.0, sy = 0.0, sz = 0.0;
+          for (uint32_t j = Lf.b + s; j < Lf.e; j += S) {
+            const double4 sj = D.src64[j];
+            const double dx = wrap_half(sj.x - px, L0), dy = wrap_half(sj.y - py, L1), dz = wrap_half(sj.z - pz, L2);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const bool inside = j != tp && r2 <= P.rcut2;
+            const double lr = gauss_hermite(D.gtab, fmin(r2 * P.inv2a2, tlim)) * P.inv8a3;
+            const double dd = r2 + eps2;
+            const double soft = 1.0 / (dd * sqrt(dd));
+            const double mj = D.uniform_mass ? P.uniform_mass : sj.w;
+            const double f = inside && r2 > 0.0 ? (soft - lr) * P.G * mj : 0.0;
+            sx += f * dx;
+            sy += f * dy;
+            sz += f * dz;
+            cnt[2] += inside ? 1u : 0u;
+          }
+          ax += sx;
+          ay += sy;
+          az += sz;
+        }
+        for (uint32_t e = s; e < nm; e += S) {
+          if (!valid) continue;
+          const uint32_t nd = nodes[e];
+          const NodeMom64 M = D.mom64[nd];
+          const double dx = mi64(M.comx - px, L0), dy = mi64(M.comy - py, L1), dz = mi64(M.comz - pz, L2);
+          if (P.com_cut && dx * dx + dy * dy + dz * dz > P.rcut2) {
+            ++cnt[1];
+            continue;
+          }
+          if (P.radial) node_accel_radial64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
+          else node_accel64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
+        }
+        __syncthreads();
+        nl = 0;
+        nm = 0;
+      };
+
+      while (true) {
+        const bool done = p >= P.node_count;
+        if (done || nl > static_cast<uint32_t>(kGroupLeafCap - kWave) ||
+            nm > static_cast<uint32_t>(kGroupNodeCap - kWave))
+          evaluate();
+        if (done) break;
+        // decide preorder nodes p .. p + 63 for the group box
+        const uint32_t q = p + lane;
+        uint32_t next = q + 1, lb = 0, le = 0;
+        int kind = 3;  // 0 pruned, 1 listed leaf, 2 accepted node, 3 opened
+        double dc0 = 0.0, dc1 = 0.0, dc2 = 0.0, h = 0.0;
+        if (q < P.node_count) {
+          const NodeLink lk = D.link[q];
+          const NodeGeo64 g = D.geo64[q];
+          h = g.half;
+          dc0 = nearest_image(g.cx - c0x, L0);
+          dc1 = nearest_image(g.cy - c0y, L1);
+          dc2 = nearest_image(g.cz - c0z, L2);
+          const double v0 = fabs(dc0), v1 = fabs(dc1), v2 = fabs(dc2);
+          const double g0 = fmax(0.0, v0 - h - hg0), g1 = fmax(0.0, v1 - h - hg1), g2 = fmax(0.0, v2 - h - hg2);
+          const double f0 = v0 + h + hg0, f1 = v1 + h + hg1, f2 = v2 + h + hg2;
+          const double gap2 = g0 * g0 + g1 * g1 + g2 * g2;
+          const double far2 = f0 * f0 + f1 * f1 + f2 * f2;
+          if (gap2 > P.rcut2) {
+            kind = 0;
+          } else if (lk.child_count == 0) {
+            kind = 1;
+            lb = lk.begin;
+            le = lk.begin + lk.count;
+          } else {
+            bool acc = gap2 > 0.0 && (!G.contain || far2 <= P.rcut2);
+            if (acc) {
+              const NodeMom64 M = D.mom64[q];
+              const double m0 = fmax(0.0, fabs(nearest_image(M.comx - c0x, L0)) - hg0);
+              const double m1 = fmax(0.0, fabs(nearest_image(M.comy - c0y, L1)) - hg1);
+              const double m2 = fmax(0.0, fabs(nearest_image(M.comz - c0z, L2)) - hg2);
+              const double rmin2 = m0 * m0 + m1 * m1 + m2 * m2;
+              const double width2 = 4.0 * h * h;
+              acc = rmin2 > 0.0 && width2 < G.amax2 * rmin2 && M.mass * width2 <= G.tolg * amin * rmin2 * rmin2;
+            }
+            kind = acc ? 2 : 3;
+          }
+          next = kind == 3 ? q + 1 : q + lk.skip;
+        }
+        // the visited chain through the window
+        unsigned long long chain = 0;
+        uint32_t cur = p;
+        while (cur < p + kWave && cur < P.node_count) {
+          const uint32_t l = cur - p;
+          chain |= 1ull << l;
+          cur = readlane_u32(next, l);
+        }
+        p = cur;
+        const bool on = (chain >> lane) & 1ull;
+        const unsigned long long lm = __ballot(on && kind == 1);
+        const unsigned long long mm = __ballot(on && kind == 2);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (on && kind == 1) {
+          GroupLeaf& Lf = leaves[nl + __popcll(lm & below)];
+          Lf.ox = dc0;
+          Lf.oy = dc1;
+          Lf.oz = dc2;
+          Lf.h = h;
+          Lf.b = lb;
+          Lf.e = le;
+        }
+        if (on && kind == 2) nodes[nm + __popcll(mm & below)] = q;
+        const unsigned long long pm = __ballot(on && kind == 0);
+        nl += __popcll(lm);
+        nm += __popcll(mm);
+        listed += __popcll(lm) + __popcll(mm);
+        if (first && lane == 0) {
+          cnt[0] += __popcll(chain);
+          cnt[1] += __popcll(pm);
+          cnt[3] += __popcll(mm);
+          cnt[4] += __popcll(lm);
+        }
+      }
+      entries_max = listed > entries_max ? listed : entries_max;
+      // sums of a target's lanes
+      for (int off = TT; off < kWave; off <<= 1) {
+        ax += __shfl_xor(ax, off, kWave);
+        ay += __shfl_xor(ay, off, kWave);
+        az += __shfl_xor(az, off, kWave);
+      }
+      if (valid && s == 0) {
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        if (P.pm_enabled) interp64<SCHEME>(P, D.f64x, D.f64y, D.f64z, px, py, pz, gx, gy, gz);
+        static_cast<double*>(out.ax)[slot] = gx + ax;
+        static_cast<double*>(out.ay)[slot] = gy + ay;
+        static_cast<double*>(out.az)[slot] = gz + az;
+        if (out.pm_x) {
+          static_cast<double*>(out.pm_x)[slot] = gx;
+          static_cast<double*>(out.pm_y)[slot] = gy;
+          static_cast<double*>(out.pm_z)[slot] = gz;
+        }
+        if (out.sr_x) {
+          static_cast<double*>(out.sr_x)[slot] = ax;
+          static_cast<double*>(out.sr_y)[slot] = ay;
+          static_cast<double*>(out.sr_z)[slot] = az;
+        }
+      }
+    }
+    v = ge;
+  }
+  for (int i = 0; i < kGroupCounters; ++i) {
+    if (i == 7 || i == 14) continue;
+    const unsigned long long sum = wave_sum(static_cast<unsigned long long>(cnt[i]));
+    if (lane == 0 && sum) atomicAdd(&counters[i], sum);
+  }
+  if (lane == 0) {
+    if (size_max) atomicMax(&counters[7], size_max);
+    if (entries_max) atomicMax(&counters[14], entries_max);
+  }
+}
+
+__global__ void leaf_of_pos_kernel(const NodeLink* __restrict__ link, std::size_t nodes,
+                                   uint32_t* __restrict__ leaf_of_pos) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nodes) return;
+  const NodeLink lk = link[i];
+  if (lk.child_count != 0) return;
+  for (uint32_t p = lk.begin; p < lk.begin + lk.count; ++p) leaf_of_pos[p] = static_cast<uint32_t>(i);
 }
 
 // ---- mixed walk -----------------------------------------------------------------
@@ -1021,6 +1589,95 @@ void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int unifo
   d.f64z = fz;
   if (p.assignment == 0) launch64<0>(p, d, out, counters, s);
   else launch64<1>(p, d, out, counters, s);
+}
+
+void walk_fp64_adaptive(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
+                        const double2* gtab, const double* prev, const uint32_t* target_pos,
+                        const uint32_t* target_slot, const double* fx, const double* fy, const double* fz,
+                        const WalkOutputs& out, unsigned long long* counters, hipStream_t s) {
+  if (p.n_targets == 0) return;
+  WalkData d{};
+  d.link = t.link.as<NodeLink>();
+  d.geo64 = t.geo64.as<NodeGeo64>();
+  d.mom64 = t.mom64.as<NodeMom64>();
+  d.quad64 = t.quad64.as<NodeQuad64>();
+  d.order = t.order.as<uint32_t>();
+  d.src64 = src;
+  d.uniform_mass = uniform_mass;
+  d.tpos = target_pos;
+  d.tslot = target_slot;
+  d.f64x = fx;
+  d.f64y = fy;
+  d.f64z = fz;
+  d.gtab = gtab;
+  d.prev = prev;
+  const unsigned grid = grid_for(p.n_targets, kWalkThreads);
+  if (p.assignment == 0) walk64_kernel<0, false, true><<<grid, kWalkThreads, 0, s>>>(p, d, out, counters);
+  else walk64_kernel<1, false, true><<<grid, kWalkThreads, 0, s>>>(p, d, out, counters);
+  CHUI_LAUNCH_CHECK("walk64_kernel (adaptive)");
+}
+
+void walk_fp64_group(const Tree& t, const WalkParams& p, const GroupParams& g, const double4* src,
+                     int uniform_mass, const double2* gtab, const double* prev, const uint32_t* leaf_of_pos,
+                     const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
+                     const double* fz, const WalkOutputs& out, uint32_t* fallback_pos, uint32_t* fallback_slot,
+                     uint32_t* fallback_count, unsigned long long* counters, hipStream_t s) {
+  if (p.n_targets == 0) return;
+  if (g.block == 0 || g.block > 256) throw std::invalid_argument("group walk residual block must be in [1, 256]");
+  WalkData d{};
+  d.link = t.link.as<NodeLink>();
+  d.geo64 = t.geo64.as<NodeGeo64>();
+  d.mom64 = t.mom64.as<NodeMom64>();
+  d.quad64 = t.quad64.as<NodeQuad64>();
+  d.order = t.order.as<uint32_t>();
+  d.src64 = src;
+  d.uniform_mass = uniform_mass;
+  d.tpos = target_pos;
+  d.tslot = target_slot;
+  d.f64x = fx;
+  d.f64y = fy;
+  d.f64z = fz;
+  d.gtab = gtab;
+  d.prev = prev;
+  const unsigned blocks = static_cast<unsigned>((static_cast<std::size_t>(p.n_targets) + g.block - 1) / g.block);
+  if (p.assignment == 0)
+    group_walk64_kernel<0><<<blocks, kWave, 0, s>>>(p, g, d, leaf_of_pos, out, fallback_pos, fallback_slot,
+                                                    fallback_count, counters);
+  else
+    group_walk64_kernel<1><<<blocks, kWave, 0, s>>>(p, g, d, leaf_of_pos, out, fallback_pos, fallback_slot,
+                                                    fallback_count, counters);
+  CHUI_LAUNCH_CHECK("group_walk64_kernel");
+}
+
+void build_leaf_of_pos(const Tree& t, uint32_t* leaf_of_pos, hipStream_t s) {
+  if (t.node_count == 0) return;
+  leaf_of_pos_kernel<<<grid_for(t.node_count, 256), 256, 0, s>>>(t.link.as<NodeLink>(), t.node_count, leaf_of_pos);
+  CHUI_LAUNCH_CHECK("leaf_of_pos_kernel");
+}
+
+std::vector<double> gaussian_hermite_knots() {
+  // tree_pm_split_kernel.hpp:159-170: H(t_i) from the closed form (series at
+  // t = 0) and step * dH/dt(t_i), t_i = i / 64, interleaved (value, slope).
+  constexpr double c = 2.25675833419102514779231780624309034;  // 4 / sqrt(pi)
+  constexpr double step = 64.0 / kGaussCells;
+  std::vector<double> k(2 * (kGaussCells + 1));
+  for (int i = 0; i <= kGaussCells; ++i) {
+    const double t = static_cast<double>(i) * step;
+    const double q = std::sqrt(t);
+    double h, dh;
+    if (i == 0) {
+      h = c * (1.0 / 3.0);
+      dh = -4.0 * kInvSqrtPi / 5.0;
+    } else {
+      const double qq = (0.5 * q) / 0.5;  // L(q) at split scale 1/2
+      const double lf = std::erf(qq) - 2.0 * kInvSqrtPi * qq * std::exp(-qq * qq);
+      h = lf / (t * q);
+      dh = (2.0 * kInvSqrtPi * std::exp(-t) - 1.5 * h) / t;
+    }
+    k[2 * i] = h;
+    k[2 * i + 1] = step * dh;
+  }
+  return k;
 }
 
 void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,

@@ -52,7 +52,37 @@ struct WalkParams {
   float tab_scale[3];
   float rcut2_tab, eps2_tab, c8_tab, force_scale;
   float accept_half_max;
+  // Adaptive acceptance (TreePmAcceptancePolicy::kAdaptiveRelative; FP64 walk):
+  // estimator, com-distance cutoff policy, Gaussian-table leaf pairs, radial-
+  // coefficient node expansion, maximum opening angle, alpha, acceleration
+  // floor, and 1/(2a)^2, 1/(8a^3) of the table lookup.
+  int estimator;
+  int com_cut;
+  int lookup;
+  int radial;
+  double amax, alpha, accel_floor;
+  double inv2a2, inv8a3;
 };
+
+// Group walk parameters (tree_pm_group_walk.hpp:78-341): residual block and
+// target group sizes, largest group extent, alpha / G, the squared maximum
+// opening angle, the cutoff containment rule and the caller's source
+// coordinate bounds for the bounded-metric check.
+struct GroupParams {
+  uint32_t block;        // residual_block_size (visit positions per block)
+  uint32_t group_size;   // 0: the active targets of one leaf
+  double max_extent;
+  double tolg;
+  double amax2;
+  int contain;           // kContainNode
+  double lo[3], hi[3];
+};
+
+// Group walk counters on the device (chui_treepm_walk_counters order after
+// its header): visited, pruned, pairs, multipoles, leaves, groups, grouped
+// targets, largest group, fallback targets, fallback history, fallback
+// geometry, candidate leaves, culled leaves, candidate particles, largest list.
+constexpr int kGroupCounters = 15;
 
 struct WalkOutputs {
   void* ax;
@@ -82,6 +112,31 @@ void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int unifo
 void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,
                    const double* fx, const double* fy, const double* fz, const double* nu, const WalkOutputs& out,
                    unsigned long long* counters, hipStream_t s);
+// Per-target FP64 walk under the adaptive acceptance. gtab = Gaussian Hermite
+// knots (value, slope) x 4097 (required when p.lookup), prev = previous
+// acceleration magnitude per output slot (nullable).
+void walk_fp64_adaptive(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
+                        const double2* gtab, const double* prev, const uint32_t* target_pos,
+                        const uint32_t* target_slot, const double* fx, const double* fy, const double* fz,
+                        const WalkOutputs& out, unsigned long long* counters, hipStream_t s);
+
+// FP64 group walk over the residual blocks of the visit-ordered targets.
+// leaf_of_pos = leaf node per tree position. Targets of groups that fall back
+// are appended to (fallback_pos, fallback_slot) with their count in
+// *fallback_count (device) for walk_fp64_adaptive.
+void walk_fp64_group(const Tree& t, const WalkParams& p, const GroupParams& g, const double4* src,
+                     int uniform_mass, const double2* gtab, const double* prev, const uint32_t* leaf_of_pos,
+                     const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
+                     const double* fz, const WalkOutputs& out, uint32_t* fallback_pos, uint32_t* fallback_slot,
+                     uint32_t* fallback_count, unsigned long long* counters, hipStream_t s);
+
+// Host FP64 Gaussian Hermite knots of tree_pm_split_kernel.hpp:130-183:
+// 4097 (H(t_i), step * dH/dt(t_i)) pairs, t_i = i / 64.
+std::vector<double> gaussian_hermite_knots();
+
+// Leaf node of every tree position of the last built tree.
+void build_leaf_of_pos(const Tree& t, uint32_t* leaf_of_pos, hipStream_t s);
+
 void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int uniform_mass, const float4* htab,
                 const uint32_t* target_pos, const uint32_t* target_slot, const float* fx, const float* fy,
                 const float* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s);

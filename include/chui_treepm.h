@@ -14,6 +14,10 @@
  *       TreePmCoordinator::solveActiveSetWithPmCadence(...)
  *       include/cosmosim/gravity/tree_pm_coupling.hpp:563-573
  *       (solveActiveSet, :575-584, is the same call with refresh = 1)
+ *   chui_treepm_solve_adaptive_device / chui_treepm_default_adaptive_options
+ *       the same call under TreePmAcceptancePolicy::kAdaptiveRelative with the
+ *       accumulator's previous_acceleration_magnitude_code lane and the group
+ *       walk options (tree_pm_coupling.hpp:127-165, 243-276)
  *   chui_treepm_options_*  TreePmOptions / PmSolveOptions / TreeGravityOptions /
  *       TreePmSplitPolicy (tree_pm_coupling.hpp:243-332, pm_solver.hpp:223-254,
  *       tree_gravity.hpp:30-43, tree_pm_split_kernel.hpp:20-27) and the
@@ -105,7 +109,8 @@ typedef struct chui_treepm_options {
   double relative_force_acceleration_floor;
   /* TreePmOptions */
   int32_t acceptance_policy;         /* chui_acceptance_policy */
-  int32_t residual_block_size;       /* accepted for compatibility (16..256); GPU walks 64-wide */
+  int32_t residual_block_size;       /* 16..256: visit positions per residual block, the span of
+                                        group-walk target groups; the per-target walks run 64-wide */
   /* this library */
   int32_t precision;                 /* chui_precision */
   int32_t reserved1;
@@ -196,6 +201,104 @@ chui_status chui_treepm_solve_device(chui_treepm_context* ctx,
                                      void* short_accel_x, void* short_accel_y, void* short_accel_z,
                                      const chui_treepm_options* options, int32_t refresh_long_range,
                                      chui_treepm_diagnostics* diagnostics);
+
+/* ---- adaptive acceptance and the group walk ------------------------------- */
+typedef enum chui_adaptive_estimator {
+  CHUI_ESTIMATOR_RESIDUAL_DERIVATIVE = 0, /* kResidualDerivative (reference default) */
+  CHUI_ESTIMATOR_RELATIVE_MONOPOLE = 1    /* kRelativeMonopole */
+} chui_adaptive_estimator;
+
+typedef enum chui_cutoff_node_policy {
+  CHUI_CUTOFF_CONTAIN_NODE = 0, /* kContainNode (reference default) */
+  CHUI_CUTOFF_COM_DISTANCE = 1  /* kComDistance */
+} chui_cutoff_node_policy;
+
+/* The TreePmOptions fields of the adaptive acceptance policy and the group walk
+ * (tree_pm_coupling.hpp:248-276). chui_treepm_default_adaptive_options gives the
+ * reference's defaults. The policy constants alpha (relative_force_tolerance),
+ * the acceleration floor and theta stay in chui_treepm_options. */
+typedef struct chui_treepm_adaptive_options {
+  uint32_t struct_size;                     /* = sizeof(chui_treepm_adaptive_options) */
+  int32_t error_estimator;                  /* chui_adaptive_estimator */
+  double maximum_opening_angle;             /* adaptive_maximum_opening_angle (0.25) */
+  int32_t cutoff_node_policy;               /* chui_cutoff_node_policy */
+  int32_t gaussian_pair_lookup;             /* gaussian_pair_lookup_enabled: Hermite table for leaf pairs */
+  int32_t radial_coefficients;              /* radial_coefficients_enabled: accepted-node derivatives */
+  int32_t group_walk;                       /* group_walk_enabled */
+  uint32_t group_target_size;               /* group_walk_target_group_size (0 = the active targets of one leaf) */
+  uint32_t reserved0;
+  double group_max_extent_cutoff_fraction;  /* group_walk_max_extent_cutoff_fraction (0.5) */
+} chui_treepm_adaptive_options;
+
+/* Group walk status of a solve (TreePmGroupWalkStatus, tree_pm_coupling.hpp:215-227). */
+typedef enum chui_group_walk_status {
+  CHUI_GROUP_WALK_NOT_REQUESTED = 0,
+  CHUI_GROUP_WALK_ACTIVE = 1,
+  CHUI_GROUP_WALK_NO_LOCAL_TREE = 2,
+  CHUI_GROUP_WALK_INCOMPATIBLE_POLICY = 3,
+  CHUI_GROUP_WALK_CUTOFF_OUTSIDE_TABLE = 6,
+  CHUI_GROUP_WALK_LOOKUP_SCALING_UNREPRESENTABLE = 8
+} chui_group_walk_status;
+
+/* Traversal counters of an adaptive solve (the TreePmTraversalCounters fields of
+ * tree_pm_coupling.hpp:40-90 that the group walk and the per-target walk share).
+ * Per-target walk: visited / pruned / pairs / multipoles / leaves per target.
+ * Group walk: visited nodes, pruned nodes, listed leaves and listed multipoles
+ * per group; pairs and com-distance prunes per target. */
+typedef struct chui_treepm_walk_counters {
+  uint32_t struct_size;                 /* = sizeof(chui_treepm_walk_counters) */
+  uint32_t group_walk_status;           /* chui_group_walk_status */
+  uint64_t visited_nodes;
+  uint64_t cutoff_pruned_nodes;
+  uint64_t direct_pair_evaluations;
+  uint64_t accepted_internal_multipoles;
+  uint64_t accepted_leaves;
+  uint64_t group_walk_groups;
+  uint64_t group_walk_grouped_targets;
+  uint64_t group_walk_target_group_size_max;
+  uint64_t group_walk_fallback_targets;
+  uint64_t group_walk_fallback_history;
+  uint64_t group_walk_fallback_geometry;
+  uint64_t group_walk_candidate_leaves;
+  uint64_t group_walk_culled_leaves;
+  uint64_t group_walk_candidate_particles;
+  uint64_t group_walk_list_entries_max;
+} chui_treepm_walk_counters;
+
+chui_status chui_treepm_default_adaptive_options(chui_treepm_adaptive_options* adaptive);
+/* Applies one [numerics] .param.txt key of these fields (config.cpp:1741-1754):
+ * treepm_adaptive_maximum_opening_angle, treepm_adaptive_error_estimator
+ * (residual_derivative | relative_monopole), treepm_cutoff_node_policy
+ * (contain | com_distance), treepm_group_walk_enabled, treepm_group_walk_target_size,
+ * treepm_gaussian_pair_lookup_enabled, treepm_radial_coefficients_enabled.
+ * treepm_adaptive_acceptance_enabled selects the policy in chui_treepm_options_set. */
+chui_status chui_treepm_adaptive_options_set(chui_treepm_adaptive_options* adaptive, const char* key,
+                                             const char* value);
+
+/* solveActiveSetWithPmCadence with options.acceptance_policy = CHUI_ACCEPT_ADAPTIVE
+ * (tree_pm_coupling.cpp:4026-4546): the per-target walk with the adaptive
+ * acceptance (tree_pm_acceptance.hpp:54-139) and, when adaptive->group_walk is
+ * set and the policy allows it (tree_pm_coupling.cpp:3839-3869), the group walk
+ * of tree_pm_group_walk.hpp:78-341 over groups of active targets inside each
+ * residual block of options.residual_block_size visit positions; a group whose
+ * targets lack a usable history or leave the bounded periodic metric falls back
+ * to the per-target walk. previous_acceleration = double[n_active] magnitudes of
+ * the previous total acceleration per active slot
+ * (TreePmForceAccumulatorView::previous_acceleration_magnitude_code,
+ * tree_pm_coupling.hpp:127-165), or NULL for none. Device arrays as in
+ * chui_treepm_solve_device; FP64 precision only. counters may be NULL (no
+ * synchronisation then). */
+chui_status chui_treepm_solve_adaptive_device(chui_treepm_context* ctx,
+                                              const double* pos_x, const double* pos_y, const double* pos_z,
+                                              const double* mass, uint64_t n,
+                                              const uint32_t* active, uint64_t n_active,
+                                              double* accel_x, double* accel_y, double* accel_z,
+                                              double* pm_accel_x, double* pm_accel_y, double* pm_accel_z,
+                                              double* short_accel_x, double* short_accel_y, double* short_accel_z,
+                                              const double* previous_acceleration,
+                                              const chui_treepm_options* options,
+                                              const chui_treepm_adaptive_options* adaptive,
+                                              int32_t refresh_long_range, chui_treepm_walk_counters* counters);
 
 /* Per-stage device timing accumulated over solves with HIP events on the
  * context stream (no extra synchronisation inside a solve).

@@ -204,3 +204,86 @@ def cluster_node_accel(x, y, z, m, target, G, eps, a, quad=True):
     _check(load().oracle_cluster_node_accel(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), t, C.c_double(G),
                                             C.c_double(eps), C.c_double(a), C.c_int(1 if quad else 0), out))
     return np.array(out[:])
+
+
+_WALK_COUNTER_NAMES = ("visited_nodes", "cutoff_pruned_nodes", "direct_pair_evaluations",
+                       "accepted_internal_multipoles", "accepted_leaves", "group_walk_groups",
+                       "group_walk_grouped_targets", "group_walk_target_group_size_max",
+                       "group_walk_fallback_targets", "group_walk_fallback_history", "group_walk_fallback_geometry",
+                       "group_walk_candidate_leaves", "group_walk_culled_leaves", "group_walk_candidate_particles",
+                       "group_walk_list_entries_max")
+
+
+def short_range_adaptive(x, y, z, m, options, adaptive, mesh, active=None, previous=None):
+    """Adaptive-policy residual per active slot -> ([ax, ay, az], counters dict).
+
+    options / adaptive are ctypes structures with the chui_treepm_options and
+    chui_treepm_adaptive_options layouts; previous = |a_prev| per active slot."""
+    x, y, z, m = _f64(x), _f64(y), _f64(z), _f64(m)
+    act = None if active is None else np.ascontiguousarray(active, dtype=np.uint32)
+    na = x.size if act is None else act.size
+    prev = None if previous is None else _f64(previous)
+    if prev is not None and prev.size != na:
+        raise ValueError("previous acceleration lane must have one entry per active slot")
+    a = [np.empty(na) for _ in range(3)]
+    raw = np.zeros(16, np.uint64)
+    _check(load().oracle_short_range_adaptive(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), _p(act),
+                                              C.c_uint64(na), _p(prev), C.byref(options), C.byref(adaptive),
+                                              _mesh(mesh), _p(a[0]), _p(a[1]), _p(a[2]), _p(raw)))
+    counters = dict(zip(_WALK_COUNTER_NAMES, raw[1:].tolist()))
+    counters["group_walk_status"] = int(raw[0] >> np.uint64(32))
+    return a, counters
+
+
+def gauss_table(t, want_knots=False):
+    """(H(t) as dimensionlessCoefficient, clamped Hermite of the group kernel[, knots (2, 4097)])."""
+    t = _f64(t)
+    coef, herm = np.empty_like(t), np.empty_like(t)
+    knots = np.empty((2, 4097)) if want_knots else None
+    _check(load().oracle_gauss_table(_p(t), C.c_uint64(t.size), _p(coef), _p(herm), _p(knots)))
+    return (coef, herm, knots) if want_knots else (coef, herm)
+
+
+def residual_radial(x, eps, a):
+    """(value, f'(r)/r, (f'' - f'/r)/r^2) of the residual kernel at x = r^2, shape (n, 3)."""
+    x = _f64(x)
+    out = np.empty((x.size, 3))
+    _check(load().oracle_residual_radial(_p(x), C.c_uint64(x.size), C.c_double(eps), C.c_double(a), _p(out)))
+    return out
+
+
+def cluster_node_accel_radial(x, y, z, m, target, G, eps, a, quad=True, radial=True):
+    x, y, z, m = _f64(x), _f64(y), _f64(z), _f64(m)
+    t = (C.c_double * 3)(*target)
+    out = (C.c_double * 3)()
+    _check(load().oracle_cluster_node_accel_radial(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), t,
+                                                   C.c_double(G), C.c_double(eps), C.c_double(a),
+                                                   C.c_int(1 if quad else 0), C.c_int(1 if radial else 0), out))
+    return np.array(out[:])
+
+
+def group_list(x, y, z, m, rows, previous, options, adaptive, mesh, cap=1 << 16):
+    """Interaction list of one target group -> (accepted nodes, listed leaves, (centre, half)).
+
+    The tree it was built on is available from group_tree()."""
+    x, y, z, m = _f64(x), _f64(y), _f64(z), _f64(m)
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    prev = _f64(previous)
+    nodes = np.zeros(cap, np.uint32)
+    leaves = np.zeros(cap, np.uint32)
+    sizes = np.zeros(2, np.uint64)
+    box = np.zeros(6)
+    _check(load().oracle_group_list(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), _p(rows), C.c_uint64(rows.size),
+                                    _p(prev), C.byref(options), C.byref(adaptive), _mesh(mesh), C.c_uint64(cap),
+                                    _p(nodes), _p(leaves), _p(sizes), _p(box)))
+    nn, nl = int(sizes[0]), int(sizes[1])
+    if max(nn, nl) > cap:
+        raise ValueError("interaction list exceeds cap")
+    return nodes[:nn].copy(), leaves[:nl].copy(), (box[:3].copy(), box[3:].copy())
+
+
+def group_tree(n):
+    """Tree built by the last group_list call (node_count from its geometry)."""
+    count = C.c_uint64()
+    _check(load().oracle_last_tree_size(C.byref(count)))
+    return last_tree(count.value, n)

@@ -489,6 +489,405 @@ std::array<double, 3> walk_target(const Tree& t, const double* sx, const double*
 }
 
 // ---------------------------------------------------------------------------
+// Gaussian coefficient table: tree_pm_split_kernel.hpp:130-183. H(t) =
+// 8 a^3 L(r)/r^3 at t = r^2/(4a^2) as a cubic Hermite interpolant on 4096 equal
+// cells of [0, 64]; knot values from the closed form (series at t = 0), knot
+// slopes dH/dt = (2 e^{-t}/sqrt(pi) - 1.5 H)/t pre-multiplied by the cell width.
+struct GaussTable {
+  static constexpr size_t kCells = 4096;
+  static constexpr double kStep = 64.0 / 4096.0;
+  std::vector<double> value, slope;
+  GaussTable() : value(kCells + 1), slope(kCells + 1) {
+    for (size_t i = 0; i <= kCells; ++i) {
+      const double t = static_cast<double>(i) * kStep;
+      const double q = std::sqrt(t);
+      value[i] = i == 0 ? series_h(0.0) : long_range_factor(q, 0.5) / (t * q);
+      const double dh = i == 0 ? -4.0 * kInvSqrtPi / 5.0 : (2.0 * kInvSqrtPi * std::exp(-t) - 1.5 * value[i]) / t;
+      slope[i] = kStep * dh;
+    }
+  }
+  double hermite(double t) const {
+    const double cell = t / kStep;
+    const size_t i = static_cast<size_t>(cell);
+    const double u = cell - static_cast<double>(i), u2 = u * u, u3 = u2 * u;
+    return (2 * u3 - 3 * u2 + 1) * value[i] + (u3 - 2 * u2 + u) * slope[i] + (-2 * u3 + 3 * u2) * value[i + 1] +
+           (u3 - u2) * slope[i + 1];
+  }
+  // dimensionlessCoefficient: series below t = 1/64, analytic from t = 64
+  double coefficient(double t) const {
+    if (t < 0.015625) return series_h(t);
+    if (!(t < 64.0)) return long_range_inv_r3(std::sqrt(t), 0.5);
+    return hermite(t);
+  }
+  // longRangeInvR3
+  double long_inv_r3(double r2, double a) const {
+    const double t = 0.25 * (r2 / a) / a;
+    if (!(t >= 0.0) || t >= 64.0) return long_range_inv_r3(std::sqrt(r2), a);
+    return ((0.125 * coefficient(t) / a) / a) / a;
+  }
+};
+
+const GaussTable& gauss_table() {
+  static const GaussTable table;
+  return table;
+}
+
+// treePmResidualRadialCoefficients (tree_pm_split_kernel.hpp:245-282): for the
+// residual f(x) = (x + eps^2)^{-3/2} - H(t)/(8a^3), x = r^2, t = x/(4a^2):
+// value f, f'(r)/r = 2 df/dx and (f'' - f'/r)/r^2 = 4 d^2f/dx^2.
+struct Radial { double value, first_over_r, contracted_second; };
+Radial residual_radial(double x, double eps, double a) {
+  const double d = x + eps * eps;
+  const double soft = 1.0 / (d * std::sqrt(d));
+  const double t = (0.25 * x / a) / a;
+  constexpr double c = 2.25675833419102514779231780624309034;
+  double h, dh, ddh;
+  if (t < 0.015625) {
+    static const double coef[8] = {1.0 / 3.0, -1.0 / 5.0, 1.0 / 14.0, -1.0 / 54.0,
+                                   1.0 / 264.0, -1.0 / 1560.0, 1.0 / 10800.0, -1.0 / 85680.0};
+    h = coef[7]; dh = 0.0; ddh = 0.0;
+    for (int i = 7; i > 0; --i) {
+      ddh = ddh * t + 2.0 * dh;
+      dh = dh * t + h;
+      h = h * t + coef[i - 1];
+    }
+    h *= c; dh *= c; ddh *= c;
+  } else {
+    const double q = std::sqrt(t);
+    const double e = std::exp(-t);
+    h = (std::erf(q) - 0.5 * c * q * e) / (t * q);
+    dh = (0.5 * c * e - 1.5 * h) / t;
+    ddh = (-0.5 * c * e - 2.5 * dh) / t;
+  }
+  const double scale = ((0.125 / a) / a) / a;
+  const double dt_dx = (0.25 / a) / a;
+  return {soft - scale * h, -3.0 * soft / d - 2.0 * scale * dt_dx * dh,
+          15.0 * soft / (d * d) - 4.0 * scale * dt_dx * dt_dx * ddh};
+}
+
+// monopolePlusQuadrupoleAccelPeriodic (tree_pm_coupling.cpp:829-894): the same
+// expansion with the shared radial coefficients; non-finite coefficients use
+// node_accel.
+std::array<double, 3> node_accel_radial(const Tree& t, uint32_t n, double dx, double dy, double dz,
+                                        const WalkParams& P) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double safe_r2 = std::max(r2, 1.0e-30);
+  const double pe = std::max(t.smax[n], P.eps);
+  const Radial R = residual_radial(safe_r2, pe, P.a);
+  if (!std::isfinite(R.value) || !std::isfinite(R.first_over_r) || !std::isfinite(R.contracted_second))
+    return node_accel(t, n, dx, dy, dz, P);
+  const double pref = P.G;
+  double ax = pref * t.mass[n] * R.value * dx;
+  double ay = pref * t.mass[n] * R.value * dy;
+  double az = pref * t.mass[n] * R.value * dz;
+  if (!P.quad) return {ax, ay, az};
+  const double tr = t.trace[n];
+  const double mxx = (t.qxx[n] + tr) / 3.0, mxy = t.qxy[n] / 3.0, mxz = t.qxz[n] / 3.0;
+  const double myy = (t.qyy[n] + tr) / 3.0, myz = t.qyz[n] / 3.0, mzz = (t.qzz[n] + tr) / 3.0;
+  const double mdx = mxx * dx + mxy * dy + mxz * dz;
+  const double mdy = mxy * dx + myy * dy + myz * dz;
+  const double mdz = mxz * dx + myz * dy + mzz * dz;
+  const double dmd = dx * mdx + dy * mdy + dz * mdz;
+  const double fo = R.first_over_r, cr = R.contracted_second;
+  ax += pref * (mdx * fo + 0.5 * tr * dx * fo + 0.5 * dx * dmd * cr);
+  ay += pref * (mdy * fo + 0.5 * tr * dy * fo + 0.5 * dy * dmd * cr);
+  az += pref * (mdz * fo + 0.5 * tr * dz * fo + 0.5 * dz * dmd * cr);
+  return {ax, ay, az};
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive acceptance (TreePmAcceptancePolicy::kAdaptiveRelative).
+struct AdaptiveParams {
+  WalkParams P;
+  double amax;       // adaptive_maximum_opening_angle
+  double alpha;      // relative_force_tolerance
+  double floor;      // relative_force_acceleration_floor_code
+  int estimator;     // chui_adaptive_estimator
+  bool com_cut;      // cutoff_node_policy == kComDistance
+  bool lookup;       // gaussian_pair_lookup_enabled
+  bool radial;       // radial_coefficients_enabled
+  double inv2a2, inv8a3;
+};
+
+// residualAccuracyProxy (tree_pm_acceptance.hpp:14-51).
+double residual_proxy(double h, double offset, double M, double r2, double eps, double a, double G) {
+  if (!(r2 > 0.0) || !std::isfinite(r2)) return INFINITY;
+  const double r = std::sqrt(r2);
+  const double d = r2 + eps * eps;
+  const double inv_d5 = 1.0 / (d * d * std::sqrt(d));
+  const double inv_r3 = 1.0 / (r2 * r);
+  const double inv_r4 = 1.0 / (r2 * r2);
+  const double lf = long_range_factor(r, a);
+  const double q = r / (2.0 * a);
+  const double e = std::exp(-q * q);
+  const double a3 = a * a * a;
+  const double s1 = -0.5 * kInvSqrtPi * r2 * e / a3;
+  const double s2 = kInvSqrtPi * e * (-r / a3 + 0.25 * r * r2 / (a3 * a * a));
+  const double first = -3.0 * r * inv_d5 + s1 * inv_r3 - 3.0 * (-lf) * inv_r4;
+  const double second = -3.0 * inv_d5 + 15.0 * r2 * inv_d5 / d + s2 * inv_r3 - 6.0 * s1 * inv_r4 +
+                         12.0 * (-lf) * inv_r4 / r;
+  const double width = 2.0 * h;
+  const double rho = std::sqrt(3.0) * h + offset;
+  const double proxy = G * M * std::max(width * width * inv_r4,
+                                        0.5 * rho * rho * (3.0 * std::abs(first) + std::abs(r * second - first)));
+  return std::isfinite(proxy) && proxy >= 0.0 ? proxy : INFINITY;
+}
+
+// acceptTreePmNode (tree_pm_acceptance.hpp:54-139) under the adaptive policy for
+// a node that is neither around the target nor outside the cutoff container
+// (those are opened before the MAC, tree_pm_coupling.cpp:4094-4104).
+static inline bool accept_adaptive(const Tree& t, uint32_t n, double r2, bool history, double prev,
+                                   const AdaptiveParams& A) {
+  const double h = t.half[n];
+  const double r = std::sqrt(std::max(r2, 1.0e-30));
+  bool soft_ok = true;
+  if (!(t.smax[n] - t.smin[n] <= 1.0e-12)) soft_ok = r > (2.0 * h + 2.0 * std::max(t.smax[n], A.P.eps));
+  const bool env_ok = (2.0 * h / r) < A.amax;
+  const double ox = t.cx[n] - t.comx[n], oy = t.cy[n] - t.comy[n], oz = t.cz[n] - t.comz[n];
+  const double offset = std::sqrt(ox * ox + oy * oy + oz * oz);
+  bool mac;
+  if (history && std::isfinite(prev) && prev > A.floor) {
+    const double allowance = A.alpha * prev;
+    const double width = 2.0 * h;
+    const double inv_r4 = 1.0 / (r2 * r2);
+    const double lower = A.P.G * t.mass[n] * (width * width * inv_r4);
+    if (std::isfinite(allowance) && std::isfinite(lower) && lower > allowance) {
+      mac = false;
+    } else if (A.estimator == CHUI_ESTIMATOR_RELATIVE_MONOPOLE) {
+      mac = std::isfinite(allowance) && std::isfinite(lower);
+    } else {
+      const double est = residual_proxy(h, offset, t.mass[n], r2, std::max(t.smax[n], A.P.eps), A.P.a, A.P.G);
+      mac = std::isfinite(est) && std::isfinite(allowance) && est <= allowance;
+    }
+  } else {
+    mac = ((2.0 * h + offset) / std::sqrt(r2 + 1.0e-30)) < A.P.theta;
+  }
+  return mac && env_ok && soft_ok && std::isfinite(r2);
+}
+
+// Leaf-pair residual coefficient of the per-target walk
+// (tree_pm_coupling.cpp:4216-4229): Gaussian table when enabled, else analytic.
+static inline double pair_residual(double r2, const AdaptiveParams& A) {
+  if (A.lookup && r2 > 0.0) {
+    const double t = r2 * A.inv2a2;
+    const GaussTable& T = gauss_table();
+    const double lr = t < 64.0 ? T.coefficient(t) * A.inv8a3 : T.long_inv_r3(r2, A.P.a);
+    return softened_inv_r3(r2, A.P.eps) - lr;
+  }
+  return residual_inv_r3(r2, A.P.eps, A.P.a);
+}
+
+// Per-target walk under the adaptive policy (tree_pm_coupling.cpp:4026-4272).
+std::array<double, 3> walk_target_adaptive(const Tree& t, const double* sx, const double* sy, const double* sz,
+                                           const double* sm, double px, double py, double pz, uint32_t self,
+                                           bool history, double prev, const AdaptiveParams& A,
+                                           std::vector<uint32_t>& stack, WalkCounters& C) {
+  const WalkParams& P = A.P;
+  double ax = 0, ay = 0, az = 0;
+  if (t.size() == 0) return {0, 0, 0};
+  stack.clear();
+  stack.push_back(0);
+  while (!stack.empty()) {
+    const uint32_t n = stack.back();
+    stack.pop_back();
+    ++C.visited;
+    const double h = t.half[n];
+    const double ndx = min_image(t.cx[n] - px, P.L[0]);
+    const double ndy = min_image(t.cy[n] - py, P.L[1]);
+    const double ndz = min_image(t.cz[n] - pz, P.L[2]);
+    const double ax_ = std::abs(ndx), ay_ = std::abs(ndy), az_ = std::abs(ndz);
+    const double ex = std::max(0.0, ax_ - h), ey = std::max(0.0, ay_ - h), ez = std::max(0.0, az_ - h);
+    if (!within_cutoff(ex * ex + ey * ey + ez * ez, P.rcut, P.rcut2)) { ++C.pruned; continue; }
+    if (t.child_count[n] == 0) {
+      ++C.leaves;
+      const uint32_t b = t.begin[n], e = b + t.count[n];
+      for (uint32_t i = b; i < e; ++i) {
+        if (t.order[i] == self) continue;
+        const double qx = min_image(sx[i] - px, P.L[0]);
+        const double qy = min_image(sy[i] - py, P.L[1]);
+        const double qz = min_image(sz[i] - pz, P.L[2]);
+        const double r2 = qx * qx + qy * qy + qz * qz;
+        if (r2 > P.rcut2) { ++C.skipped_pairs; continue; }
+        ++C.pairs;
+        if (qx == 0.0 && qy == 0.0 && qz == 0.0) continue;
+        const double f = pair_residual(r2, A) * P.G;
+        ax += f * sm[i] * qx; ay += f * sm[i] * qy; az += f * sm[i] * qz;
+      }
+      continue;
+    }
+    const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+    const double mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+    const bool contained = A.com_cut || within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
+    bool accept = false;
+    double dx = 0, dy = 0, dz = 0, r2 = 0;
+    if (!inside && contained) {
+      dx = min_image(t.comx[n] - px, P.L[0]);
+      dy = min_image(t.comy[n] - py, P.L[1]);
+      dz = min_image(t.comz[n] - pz, P.L[2]);
+      r2 = dx * dx + dy * dy + dz * dz;
+      accept = accept_adaptive(t, n, r2, history, prev, A);
+    }
+    if (accept && A.com_cut && r2 > P.rcut2) { ++C.pruned; continue; }
+    if (accept) {
+      ++C.multipoles;
+      const auto c = A.radial ? node_accel_radial(t, n, dx, dy, dz, P) : node_accel(t, n, dx, dy, dz, P);
+      ax += c[0]; ay += c[1]; az += c[2];
+    } else {
+      const size_t co = static_cast<size_t>(n) * 8;
+      for (int o = 8; o > 0; --o) {
+        const uint32_t c = t.child_index[co + o - 1];
+        if (c != kInvalid) stack.push_back(c);
+      }
+    }
+  }
+  return {ax, ay, az};
+}
+
+// ---------------------------------------------------------------------------
+// Group walk: tree_pm_group_walk.hpp:78-341 and its driver
+// tree_pm_coupling.cpp:4318-4450.
+struct GroupList {
+  std::vector<uint32_t> nodes, begin, end;
+  std::vector<double> ox, oy, oz, half;
+};
+
+struct GroupCounters {
+  uint64_t groups = 0, grouped = 0, size_max = 0, fallback = 0, fallback_history = 0, fallback_geometry = 0;
+  uint64_t candidate_leaves = 0, culled = 0, candidate_particles = 0, entries_max = 0;
+};
+
+// buildTreePmGroupInteractionList: box-box prune, leaves listed with their cell
+// geometry relative to the group centre, internal nodes accepted by the
+// group-conservative relative-monopole criterion.
+void group_list(const Tree& t, const double c[3], const double hg[3], double amin, double tolg, double amax2,
+                bool contain, const WalkParams& P, std::vector<uint32_t>& stack, GroupList& out, WalkCounters& C) {
+  out = GroupList{};
+  if (t.size() == 0) return;
+  stack.clear();
+  stack.push_back(0);
+  while (!stack.empty()) {
+    const uint32_t n = stack.back();
+    stack.pop_back();
+    ++C.visited;
+    const double h = t.half[n];
+    const double dc[3] = {min_image(t.cx[n] - c[0], P.L[0]), min_image(t.cy[n] - c[1], P.L[1]),
+                          min_image(t.cz[n] - c[2], P.L[2])};
+    double gap2 = 0.0, far2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double v = std::abs(dc[a]);
+      const double gap = std::max(0.0, v - h - hg[a]);
+      const double far = v + h + hg[a];
+      gap2 += gap * gap;
+      far2 += far * far;
+    }
+    if (gap2 > P.rcut2) { ++C.pruned; continue; }
+    if (t.child_count[n] == 0) {
+      ++C.leaves;
+      out.begin.push_back(t.begin[n]);
+      out.end.push_back(t.begin[n] + t.count[n]);
+      out.ox.push_back(dc[0]); out.oy.push_back(dc[1]); out.oz.push_back(dc[2]);
+      out.half.push_back(h);
+      continue;
+    }
+    bool accept = gap2 > 0.0 && (!contain || far2 <= P.rcut2);
+    if (accept) {
+      const double dm[3] = {min_image(t.comx[n] - c[0], P.L[0]), min_image(t.comy[n] - c[1], P.L[1]),
+                            min_image(t.comz[n] - c[2], P.L[2])};
+      double rmin2 = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        const double gap = std::max(0.0, std::abs(dm[a]) - hg[a]);
+        rmin2 += gap * gap;
+      }
+      const double width2 = 4.0 * h * h;
+      accept = rmin2 > 0.0 && width2 < amax2 * rmin2 && t.mass[n] * width2 <= tolg * amin * rmin2 * rmin2;
+    }
+    if (accept) {
+      ++C.multipoles;
+      out.nodes.push_back(n);
+    } else {
+      const size_t co = static_cast<size_t>(n) * 8;
+      for (int o = 8; o > 0; --o) {
+        const uint32_t ch = t.child_index[co + o - 1];
+        if (ch != kInvalid) stack.push_back(ch);
+      }
+    }
+  }
+}
+
+// accumulateTreePmGroupPairsImpl: Hermite-table residual over tree-order
+// sources [b, e), one conditional periodic shift per axis, the target's own
+// source excluded by identity (distinct coincident sources count as pairs).
+static void group_pairs(const double* sx, const double* sy, const double* sz, const double* sm,
+                        const uint32_t* order, uint32_t self, double px, double py, double pz, uint32_t b,
+                        uint32_t e, const AdaptiveParams& A, double& ax, double& ay, double& az, uint64_t& pairs) {
+  const WalkParams& P = A.P;
+  const GaussTable& T = gauss_table();
+  const double tlim = std::nextafter(64.0, 0.0);
+  const double eps2 = P.eps * P.eps;
+  double s[3] = {0, 0, 0};
+  for (uint32_t j = b; j < e; ++j) {
+    double d[3] = {sx[j] - px, sy[j] - py, sz[j] - pz};
+    for (int a = 0; a < 3; ++a) {
+      const double hl = 0.5 * P.L[a];
+      d[a] = d[a] > hl ? d[a] - P.L[a] : (d[a] < -hl ? d[a] + P.L[a] : d[a]);
+    }
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const bool inside = order[j] != self && r2 <= P.rcut2;
+    const double tt = std::min(r2 * A.inv2a2, tlim);
+    const double lr = T.hermite(tt) * A.inv8a3;
+    const double dd = r2 + eps2;
+    const double soft = 1.0 / (dd * std::sqrt(dd));
+    const double f = inside && r2 > 0.0 ? (soft - lr) * P.G * sm[j] : 0.0;
+    for (int a = 0; a < 3; ++a) s[a] += f * d[a];
+    pairs += inside ? 1 : 0;
+  }
+  ax += s[0]; ay += s[1]; az += s[2];
+}
+
+// accumulateTreePmGroupCulledLeaves + the accepted multipoles of the list for one
+// target: leaves whose cell lies beyond r_cut (1 + 1e-9) of the target are
+// skipped, contiguous survivors are summed as one range.
+std::array<double, 3> group_target(const Tree& t, const GroupList& L, const double c[3], const double* sx,
+                                   const double* sy, const double* sz, const double* sm, uint32_t self, double px,
+                                   double py, double pz, const AdaptiveParams& A, WalkCounters& C,
+                                   GroupCounters& G) {
+  const WalkParams& P = A.P;
+  const double to[3] = {min_image(px - c[0], P.L[0]), min_image(py - c[1], P.L[1]), min_image(pz - c[2], P.L[2])};
+  const double cull2 = P.rcut2 * (1.0 + 1.0e-9);
+  auto wrap = [&](double d, int a) {
+    const double hl = 0.5 * P.L[a];
+    return d > hl ? d - P.L[a] : (d < -hl ? d + P.L[a] : d);
+  };
+  double ax = 0, ay = 0, az = 0;
+  uint64_t pairs = 0;
+  bool open = false;
+  uint32_t rb = 0, re = 0;
+  for (size_t l = 0; l < L.begin.size(); ++l) {
+    const double h = L.half[l];
+    const double gx = std::max(0.0, std::abs(wrap(L.ox[l] - to[0], 0)) - h);
+    const double gy = std::max(0.0, std::abs(wrap(L.oy[l] - to[1], 1)) - h);
+    const double gz = std::max(0.0, std::abs(wrap(L.oz[l] - to[2], 2)) - h);
+    if (gx * gx + gy * gy + gz * gz > cull2) { ++G.culled; continue; }
+    G.candidate_particles += L.end[l] - L.begin[l];
+    if (open && re == L.begin[l]) { re = L.end[l]; continue; }
+    if (open) group_pairs(sx, sy, sz, sm, t.order.data(), self, px, py, pz, rb, re, A, ax, ay, az, pairs);
+    rb = L.begin[l]; re = L.end[l]; open = true;
+  }
+  if (open) group_pairs(sx, sy, sz, sm, t.order.data(), self, px, py, pz, rb, re, A, ax, ay, az, pairs);
+  G.candidate_leaves += L.begin.size();
+  C.pairs += pairs;
+  for (const uint32_t n : L.nodes) {
+    const double dx = min_image(t.comx[n] - px, P.L[0]);
+    const double dy = min_image(t.comy[n] - py, P.L[1]);
+    const double dz = min_image(t.comz[n] - pz, P.L[2]);
+    if (A.com_cut && dx * dx + dy * dy + dz * dz > P.rcut2) { ++C.pruned; continue; }
+    const auto q = A.radial ? node_accel_radial(t, n, dx, dy, dz, P) : node_accel(t, n, dx, dy, dz, P);
+    ax += q[0]; ay += q[1]; az += q[2];
+  }
+  return {ax, ay, az};
+}
+
+// ---------------------------------------------------------------------------
 // FFT (FP64, own implementation; FFTW is absent). Radix-2 iterative for powers
 // of two, direct DFT otherwise. sign = -1 forward (FFTW_FORWARD), +1 backward.
 void fft1d(std::complex<double>* d, size_t n, int sign) {
@@ -782,6 +1181,194 @@ void short_range(const double* x, const double* y, const double* z, const double
   }
 }
 
+// Adaptive-policy residual over active slots (tree_pm_coupling.cpp:3839-3869
+// eligibility, :4451-4546 residual blocks and target groups, :4333-4450 group
+// evaluation with per-target fallback). prev = previous total-acceleration
+// magnitude per active slot (nullable). Outputs per active slot.
+struct AdaptiveOut { WalkCounters counters; GroupCounters group; uint32_t status = 0; };
+
+void adaptive_short_range(const double* x, const double* y, const double* z, const double* m, size_t n,
+                          const uint32_t* active, size_t na, const double* prev, const chui_treepm_options& o,
+                          const chui_treepm_adaptive_options& ad, const int mesh[3], double* ax, double* ay,
+                          double* az, AdaptiveOut& out) {
+  const double L[3] = {o.box_size[0], o.box_size[1], o.box_size[2]};
+  const double* src[3] = {x, y, z};
+  std::vector<double> u[3];
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = INFINITY; hi[a] = -INFINITY;
+    for (size_t i = 0; i < n; ++i) { lo[a] = std::min(lo[a], src[a][i]); hi[a] = std::max(hi[a], src[a][i]); }
+    u[a].resize(n);
+    unwrap_axis(src[a], n, L[a], u[a].data(), lo[a] >= 0.0 && hi[a] < L[a]);
+  }
+  Tree t;
+  build_tree(u[0].data(), u[1].data(), u[2].data(), m, n, static_cast<size_t>(o.max_leaf_size),
+             o.multipole_order == 2, o.softening_epsilon, t);
+  std::vector<double> sx(n), sy(n), sz(n), sm(n);
+  std::vector<uint32_t> pos_of_row(n), leaf_of_pos(n, kInvalid);
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t r = t.order[i];
+    sx[i] = x[r]; sy[i] = y[r]; sz[i] = z[r]; sm[i] = m[r];
+    pos_of_row[r] = static_cast<uint32_t>(i);
+  }
+  for (size_t nd = 0; nd < t.size(); ++nd)
+    if (t.child_count[nd] == 0)
+      for (uint32_t p = t.begin[nd]; p < t.begin[nd] + t.count[nd]; ++p) leaf_of_pos[p] = static_cast<uint32_t>(nd);
+  const double h = mesh_spacing(o, mesh);
+  AdaptiveParams A{};
+  for (int a = 0; a < 3; ++a) A.P.L[a] = L[a];
+  A.P.G = o.gravitational_constant;
+  A.P.a = o.asmth_cells * h;
+  A.P.rcut = o.rcut_cells * h;
+  A.P.rcut2 = A.P.rcut * A.P.rcut;
+  A.P.theta = o.opening_theta;
+  A.P.eps = o.softening_epsilon;
+  A.P.criterion = o.opening_criterion;
+  A.P.quad = o.multipole_order == 2;
+  A.amax = ad.maximum_opening_angle;
+  A.alpha = o.relative_force_tolerance;
+  A.floor = o.relative_force_acceleration_floor;
+  A.estimator = ad.error_estimator;
+  A.com_cut = ad.cutoff_node_policy == CHUI_CUTOFF_COM_DISTANCE;
+  A.lookup = ad.gaussian_pair_lookup != 0;
+  A.radial = ad.radial_coefficients != 0;
+  A.inv2a2 = (0.25 / A.P.a) / A.P.a;
+  A.inv8a3 = ((0.125 / A.P.a) / A.P.a) / A.P.a;
+  // group walk eligibility
+  const double t_cut = (0.25 * A.P.rcut2 / A.P.a) / A.P.a;
+  uint32_t status = CHUI_GROUP_WALK_ACTIVE;
+  if (!ad.group_walk) status = CHUI_GROUP_WALK_NOT_REQUESTED;
+  else if (t.size() == 0) status = CHUI_GROUP_WALK_NO_LOCAL_TREE;
+  else if (ad.error_estimator != CHUI_ESTIMATOR_RELATIVE_MONOPOLE || !A.lookup)
+    status = CHUI_GROUP_WALK_INCOMPATIBLE_POLICY;
+  else if (!(std::isfinite(t_cut) && t_cut < 63.0)) status = CHUI_GROUP_WALK_CUTOFF_OUTSIDE_TABLE;
+  else if (!(std::isnormal(A.inv2a2) && std::isnormal(A.inv8a3))) status = CHUI_GROUP_WALK_LOOKUP_SCALING_UNREPRESENTABLE;
+  out.status = status;
+  const bool grouped = status == CHUI_GROUP_WALK_ACTIVE;
+  // visit order: active slots by tree position
+  std::vector<uint32_t> row(na), visit(na);
+  for (size_t s = 0; s < na; ++s) {
+    row[s] = active ? active[s] : static_cast<uint32_t>(s);
+    if (row[s] >= n) throw std::invalid_argument("TreePM active particle index out of range");
+    visit[s] = static_cast<uint32_t>(s);
+  }
+  std::stable_sort(visit.begin(), visit.end(),
+                   [&](uint32_t a, uint32_t b) { return pos_of_row[row[a]] < pos_of_row[row[b]]; });
+  const size_t B = static_cast<size_t>(o.residual_block_size);
+  const double tolg = o.relative_force_tolerance / o.gravitational_constant;
+  const double amax2 = ad.maximum_opening_angle * ad.maximum_opening_angle;
+  const double max_extent = ad.group_max_extent_cutoff_fraction * std::sqrt(A.P.rcut2);
+  const size_t nblocks = (na + B - 1) / B;
+  std::vector<WalkCounters> wc(nblocks);
+  std::vector<GroupCounters> gc(nblocks);
+#pragma omp parallel
+  {
+    std::vector<uint32_t> stack;
+    stack.reserve(1 + 7 * 65);
+    GroupList list;
+#pragma omp for schedule(dynamic, 1)
+    for (size_t blk = 0; blk < nblocks; ++blk) {
+      WalkCounters& C = wc[blk];
+      GroupCounters& G = gc[blk];
+      const size_t vb = blk * B, ve = std::min(vb + B, na);
+      auto per_target = [&](size_t v) {
+        const uint32_t s = visit[v];
+        const bool hist = prev != nullptr && std::isfinite(prev[s]);
+        const auto a = walk_target_adaptive(t, sx.data(), sy.data(), sz.data(), sm.data(), x[row[s]], y[row[s]],
+                                            z[row[s]], row[s], hist, hist ? prev[s] : 0.0, A, stack, C);
+        ax[s] = a[0]; ay[s] = a[1]; az[s] = a[2];
+      };
+      if (!grouped) {
+        for (size_t v = vb; v < ve; ++v) per_target(v);
+        continue;
+      }
+      size_t v = vb;
+      while (v < ve) {
+        const uint32_t leaf = leaf_of_pos[pos_of_row[row[visit[v]]]];
+        size_t ge = v + 1;
+        if (ad.group_target_size == 0) {
+          while (ge < ve && leaf_of_pos[pos_of_row[row[visit[ge]]]] == leaf) ++ge;
+        } else {
+          const uint32_t r0 = row[visit[v]];
+          const double ref[3] = {x[r0], y[r0], z[r0]};
+          double glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+          while (ge < ve && ge - v < ad.group_target_size) {
+            const uint32_t r = row[visit[ge]];
+            const double p[3] = {x[r], y[r], z[r]};
+            double nlo[3], nhi[3];
+            bool within = true;
+            for (int a = 0; a < 3; ++a) {
+              const double d = min_image(p[a] - ref[a], L[a]);
+              nlo[a] = std::min(glo[a], d);
+              nhi[a] = std::max(ghi[a], d);
+              within = within && nhi[a] - nlo[a] <= max_extent;
+            }
+            if (!within) break;
+            for (int a = 0; a < 3; ++a) { glo[a] = nlo[a]; ghi[a] = nhi[a]; }
+            ++ge;
+          }
+        }
+        const size_t count = ge - v;
+        // evaluate_target_group: history and bounded-metric checks, group box
+        int fall = 0;  // 1 history, 2 geometry
+        double amin = INFINITY, ref[3] = {0, 0, 0}, glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+        for (size_t i = 0; i < count && !fall; ++i) {
+          const uint32_t s = visit[v + i];
+          const double pa = prev ? prev[s] : NAN;
+          if (!(std::isfinite(pa) && pa > o.relative_force_acceleration_floor)) { fall = 1; break; }
+          amin = std::min(amin, pa);
+          const double p[3] = {x[row[s]], y[row[s]], z[row[s]]};
+          for (int a = 0; a < 3; ++a) {
+            if (!(lo[a] - p[a] >= -L[a] && hi[a] - p[a] <= L[a])) { fall = 2; break; }
+            if (i == 0) { ref[a] = p[a]; continue; }
+            const double d = min_image(p[a] - ref[a], L[a]);
+            glo[a] = std::min(glo[a], d);
+            ghi[a] = std::max(ghi[a], d);
+          }
+        }
+        if (fall) {
+          G.fallback += count;
+          (fall == 1 ? G.fallback_history : G.fallback_geometry) += count;
+          for (size_t i = v; i < ge; ++i) per_target(i);
+          v = ge;
+          continue;
+        }
+        double c[3], hg[3];
+        for (int a = 0; a < 3; ++a) {
+          c[a] = ref[a] + 0.5 * (glo[a] + ghi[a]);
+          hg[a] = 0.5 * (ghi[a] - glo[a]);
+        }
+        group_list(t, c, hg, amin, tolg, amax2, !A.com_cut, A.P, stack, list, C);
+        ++G.groups;
+        G.grouped += count;
+        G.size_max = std::max<uint64_t>(G.size_max, count);
+        G.entries_max = std::max<uint64_t>(G.entries_max, list.begin.size() + list.nodes.size());
+        for (size_t i = v; i < ge; ++i) {
+          const uint32_t s = visit[i];
+          const auto a = group_target(t, list, c, sx.data(), sy.data(), sz.data(), sm.data(), row[s], x[row[s]],
+                                      y[row[s]], z[row[s]], A, C, G);
+          ax[s] = a[0]; ay[s] = a[1]; az[s] = a[2];
+        }
+        v = ge;
+      }
+    }
+  }
+  for (size_t b = 0; b < nblocks; ++b) {
+    const WalkCounters& c = wc[b];
+    out.counters.visited += c.visited; out.counters.pruned += c.pruned; out.counters.pairs += c.pairs;
+    out.counters.multipoles += c.multipoles; out.counters.leaves += c.leaves;
+    out.counters.skipped_pairs += c.skipped_pairs;
+    const GroupCounters& g = gc[b];
+    out.group.groups += g.groups; out.group.grouped += g.grouped;
+    out.group.size_max = std::max(out.group.size_max, g.size_max);
+    out.group.fallback += g.fallback; out.group.fallback_history += g.fallback_history;
+    out.group.fallback_geometry += g.fallback_geometry;
+    out.group.candidate_leaves += g.candidate_leaves; out.group.culled += g.culled;
+    out.group.candidate_particles += g.candidate_particles;
+    out.group.entries_max = std::max(out.group.entries_max, g.entries_max);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Ewald: tests/support/periodic_ewald_reference.cpp:202-400 (real + reciprocal
 // sums, optional Plummer lattice correction), compensated accumulation.
@@ -914,6 +1501,10 @@ ORACLE_API int oracle_build_tree(const double* x, const double* y, const double*
 }
 
 // Copies the last built tree (layout identical to chui_treepm_tree_copy).
+ORACLE_API int oracle_last_tree_size(uint64_t* nodes) {
+  ORACLE_TRY(*nodes = g_tree.size())
+}
+
 ORACLE_API int oracle_tree_copy(double* geometry, double* moments, uint32_t* begin, uint32_t* count,
                                 uint32_t* child_index, uint32_t* order, uint32_t* subtree) {
   const Tree& t = g_tree;
@@ -1026,6 +1617,154 @@ ORACLE_API int oracle_short_range(const double* x, const double* y, const double
       counters[3] = out.counters.multipoles; counters[4] = out.counters.leaves; counters[5] = out.counters.skipped_pairs;
     }
     if (anchors) for (int a = 0; a < 3; ++a) anchors[a] = out.anchor[a];
+  })
+}
+
+// Adaptive-policy short-range residual per active slot (active may be NULL for
+// all rows, prev may be NULL); counters in the chui_treepm_walk_counters layout.
+ORACLE_API int oracle_short_range_adaptive(const double* x, const double* y, const double* z, const double* m,
+                                           uint64_t n, const uint32_t* active, uint64_t n_active, const double* prev,
+                                           const chui_treepm_options* o, const chui_treepm_adaptive_options* ad,
+                                           const int32_t mesh[3], double* ax, double* ay, double* az,
+                                           chui_treepm_walk_counters* counters) {
+  ORACLE_TRY({
+    const int msh[3] = {mesh[0], mesh[1], mesh[2]};
+    AdaptiveOut out;
+    adaptive_short_range(x, y, z, m, n, active, active ? n_active : n, prev, *o, *ad, msh, ax, ay, az, out);
+    if (counters) {
+      chui_treepm_walk_counters c{};
+      c.struct_size = sizeof(c);
+      c.group_walk_status = out.status;
+      c.visited_nodes = out.counters.visited;
+      c.cutoff_pruned_nodes = out.counters.pruned;
+      c.direct_pair_evaluations = out.counters.pairs;
+      c.accepted_internal_multipoles = out.counters.multipoles;
+      c.accepted_leaves = out.counters.leaves;
+      c.group_walk_groups = out.group.groups;
+      c.group_walk_grouped_targets = out.group.grouped;
+      c.group_walk_target_group_size_max = out.group.size_max;
+      c.group_walk_fallback_targets = out.group.fallback;
+      c.group_walk_fallback_history = out.group.fallback_history;
+      c.group_walk_fallback_geometry = out.group.fallback_geometry;
+      c.group_walk_candidate_leaves = out.group.candidate_leaves;
+      c.group_walk_culled_leaves = out.group.culled;
+      c.group_walk_candidate_particles = out.group.candidate_particles;
+      c.group_walk_list_entries_max = out.group.entries_max;
+      *counters = c;
+    }
+  })
+}
+
+// Interaction list of one group of source rows (all with history prev[i]) in
+// the tree of all n sources: accepted nodes (geometry/moments via
+// oracle_tree_copy) and listed leaves. Returns the group box in box[6]
+// (centre, half) and the list sizes in sizes[2]; nodes / leaves get at most
+// cap entries each.
+ORACLE_API int oracle_group_list(const double* x, const double* y, const double* z, const double* m, uint64_t n,
+                                 const uint32_t* rows, uint64_t k, const double* prev,
+                                 const chui_treepm_options* o, const chui_treepm_adaptive_options* ad,
+                                 const int32_t mesh[3], uint64_t cap, uint32_t* nodes, uint32_t* leaves,
+                                 uint64_t* sizes, double* box) {
+  ORACLE_TRY({
+    if (k == 0) throw std::invalid_argument("empty group");
+    const int msh[3] = {mesh[0], mesh[1], mesh[2]};
+    const double L[3] = {o->box_size[0], o->box_size[1], o->box_size[2]};
+    const double* src[3] = {x, y, z};
+    std::vector<double> u[3];
+    for (int a = 0; a < 3; ++a) {
+      double lo = INFINITY, hi = -INFINITY;
+      for (size_t i = 0; i < n; ++i) { lo = std::min(lo, src[a][i]); hi = std::max(hi, src[a][i]); }
+      u[a].resize(n);
+      unwrap_axis(src[a], n, L[a], u[a].data(), lo >= 0.0 && hi < L[a]);
+    }
+    g_tree = Tree{};
+    build_tree(u[0].data(), u[1].data(), u[2].data(), m, n, static_cast<size_t>(o->max_leaf_size),
+               o->multipole_order == 2, o->softening_epsilon, g_tree);
+    double amin = INFINITY, ref[3] = {0, 0, 0}, glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+    for (uint64_t i = 0; i < k; ++i) {
+      amin = std::min(amin, prev[i]);
+      const double p[3] = {x[rows[i]], y[rows[i]], z[rows[i]]};
+      for (int a = 0; a < 3; ++a) {
+        if (i == 0) { ref[a] = p[a]; continue; }
+        const double d = min_image(p[a] - ref[a], L[a]);
+        glo[a] = std::min(glo[a], d);
+        ghi[a] = std::max(ghi[a], d);
+      }
+    }
+    double c[3], hg[3];
+    for (int a = 0; a < 3; ++a) {
+      c[a] = ref[a] + 0.5 * (glo[a] + ghi[a]);
+      hg[a] = 0.5 * (ghi[a] - glo[a]);
+      box[a] = c[a];
+      box[3 + a] = hg[a];
+    }
+    WalkParams P{};
+    for (int a = 0; a < 3; ++a) P.L[a] = L[a];
+    const double h = mesh_spacing(*o, msh);
+    P.rcut = o->rcut_cells * h;
+    P.rcut2 = P.rcut * P.rcut;
+    std::vector<uint32_t> stack;
+    GroupList list;
+    WalkCounters C;
+    group_list(g_tree, c, hg, amin, o->relative_force_tolerance / o->gravitational_constant,
+               ad->maximum_opening_angle * ad->maximum_opening_angle, ad->cutoff_node_policy != CHUI_CUTOFF_COM_DISTANCE,
+               P, stack, list, C);
+    sizes[0] = list.nodes.size();
+    sizes[1] = list.begin.size();
+    for (size_t i = 0; i < list.nodes.size() && i < cap; ++i) nodes[i] = list.nodes[i];
+    // listed leaves as node indices: the leaf whose particle range starts at begin
+    size_t j = 0;
+    for (size_t nd = 0; nd < g_tree.size() && j < list.begin.size(); ++nd)
+      if (g_tree.child_count[nd] == 0 && g_tree.begin[nd] == list.begin[j]) {
+        if (j < cap) leaves[j] = static_cast<uint32_t>(nd);
+        ++j;
+      }
+  })
+}
+
+// Gaussian table H(t) (dimensionlessCoefficient) and the group kernel's
+// clamped Hermite form at n points; the knots when knots != NULL (2 x 4097).
+ORACLE_API int oracle_gauss_table(const double* t, uint64_t n, double* coefficient, double* hermite,
+                                  double* knots) {
+  ORACLE_TRY({
+    const GaussTable& T = gauss_table();
+    const double tlim = std::nextafter(64.0, 0.0);
+    for (uint64_t i = 0; i < n; ++i) {
+      coefficient[i] = T.coefficient(t[i]);
+      hermite[i] = T.hermite(std::min(t[i], tlim));
+    }
+    if (knots)
+      for (size_t i = 0; i <= GaussTable::kCells; ++i) {
+        knots[i] = T.value[i];
+        knots[GaussTable::kCells + 1 + i] = T.slope[i];
+      }
+  })
+}
+
+// Radial residual coefficients (value, f'/r, (f'' - f'/r)/r^2) at x = r^2.
+ORACLE_API int oracle_residual_radial(const double* x, uint64_t n, double eps, double a, double* out3) {
+  ORACLE_TRY({
+    for (uint64_t i = 0; i < n; ++i) {
+      const Radial R = residual_radial(x[i], eps, a);
+      out3[3 * i] = R.value; out3[3 * i + 1] = R.first_over_r; out3[3 * i + 2] = R.contracted_second;
+    }
+  })
+}
+
+// One source cluster evaluated as one accepted node with the radial-coefficient
+// expansion (radial = 1) or the reference expansion (radial = 0).
+ORACLE_API int oracle_cluster_node_accel_radial(const double* x, const double* y, const double* z,
+                                                const double* m, uint64_t n, const double t[3], double G,
+                                                double eps, double a, int quad, int radial, double out[3]) {
+  ORACLE_TRY({
+    if (n == 0) throw std::invalid_argument("empty cluster");
+    Tree tr;
+    build_tree(x, y, z, m, n, n, quad != 0, eps, tr);
+    WalkParams P{};
+    P.G = G; P.a = a; P.eps = eps; P.quad = quad != 0;
+    const double dx = tr.comx[0] - t[0], dy = tr.comy[0] - t[1], dz = tr.comz[0] - t[2];
+    const auto c = radial ? node_accel_radial(tr, 0, dx, dy, dz, P) : node_accel(tr, 0, dx, dy, dz, P);
+    out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
   })
 }
 

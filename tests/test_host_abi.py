@@ -118,7 +118,8 @@ MESH = (C.c_int32 * 3)(64, 64, 64)
     ("gravitational_constant", -1.0, L.CHUI_ERR_OVERFLOW),
     ("assignment_scheme", 7, L.CHUI_ERR_INVALID_ARGUMENT),
     ("multipole_order", 1, L.CHUI_ERR_INVALID_ARGUMENT),
-    ("acceptance_policy", 1, L.CHUI_ERR_UNSUPPORTED),
+    ("acceptance_policy", 1, L.CHUI_OK),                        # adaptive (FP64)
+    ("acceptance_policy", 2, L.CHUI_ERR_INVALID_ARGUMENT),
     ("struct_size", 8, L.CHUI_ERR_INVALID_ARGUMENT),
 ])
 def test_options_validate(lib, field, value, status):
@@ -193,3 +194,56 @@ def test_python_mirror_raises_reference_exception_types():
     with pytest.raises(NotImplementedError):
         L.check(L.CHUI_ERR_UNSUPPORTED)
     assert np.dtype(np.float64).itemsize == 8
+
+
+def test_adaptive_policy_is_fp64_only(lib):
+    o = valid_options()
+    o.acceptance_policy = 1
+    o.precision = L.PRECISION_MIXED
+    assert lib.chui_treepm_options_validate(C.byref(o), MESH) == L.CHUI_ERR_UNSUPPORTED
+
+
+def test_adaptive_options_mirror_reference_defaults(lib):
+    # TreePmOptions (tree_pm_coupling.hpp:248-270)
+    assert C.sizeof(L.AdaptiveOptions) == 48 and C.sizeof(L.WalkCounters) == 128
+    ad = L.default_adaptive_options()
+    assert ad.struct_size == C.sizeof(L.AdaptiveOptions)
+    assert ad.maximum_opening_angle == 0.25
+    assert ad.error_estimator == L.ESTIMATOR_RESIDUAL_DERIVATIVE
+    assert ad.cutoff_node_policy == L.CUTOFF_CONTAIN_NODE
+    assert ad.gaussian_pair_lookup == 0 and ad.radial_coefficients == 0 and ad.group_walk == 0
+    assert ad.group_target_size == 0 and ad.group_max_extent_cutoff_fraction == 0.5
+
+
+def test_adaptive_options_accept_the_fast_deck_keys(lib):
+    # configs/treepm_evolved_snapshot_fast.param.txt:63-73
+    ad = L.default_adaptive_options()
+    for key, value in [("treepm_adaptive_maximum_opening_angle", "1.0"),
+                       ("treepm_adaptive_error_estimator", "relative_monopole"),
+                       ("numerics.treepm_cutoff_node_policy", "com_distance"),
+                       ("treepm_group_walk_enabled", "true"),
+                       ("treepm_gaussian_pair_lookup_enabled", "true"),
+                       ("treepm_radial_coefficients_enabled", "false"),
+                       ("treepm_group_walk_target_size", "32")]:
+        L.set_adaptive_param(ad, key, value)
+    assert ad.maximum_opening_angle == 1.0 and ad.error_estimator == L.ESTIMATOR_RELATIVE_MONOPOLE
+    assert ad.cutoff_node_policy == L.CUTOFF_COM_DISTANCE and ad.group_walk == 1
+    assert ad.gaussian_pair_lookup == 1 and ad.radial_coefficients == 0 and ad.group_target_size == 32
+    o = valid_options()
+    L.set_param(o, "treepm_adaptive_acceptance_enabled", True)
+    assert o.acceptance_policy == 1
+    for key, value in [("treepm_adaptive_error_estimator", "monopole"), ("treepm_cutoff_node_policy", "x"),
+                       ("treepm_group_walk_target_size", "1"), ("treepm_group_walk_enabled", "maybe"),
+                       ("treepm_unknown", "1")]:
+        assert lib.chui_treepm_adaptive_options_set(C.byref(ad), key.encode(), value.encode()) == \
+            L.CHUI_ERR_INVALID_ARGUMENT
+
+
+def test_adaptive_entry_point_argument_errors(lib):
+    o = valid_options()
+    o.acceptance_policy = 1
+    ad = L.default_adaptive_options()
+    args = [None] * 16
+    st = lib.chui_treepm_solve_adaptive_device(None, *args[:4], 0, None, 0, *args[:9], None, C.byref(o),
+                                               C.byref(ad), 1, None)
+    assert st == L.CHUI_ERR_INVALID_ARGUMENT
