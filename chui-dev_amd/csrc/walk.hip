@@ -276,6 +276,15 @@ __device__ __forceinline__ void node_accel_radial64(const NodeMom64& M, const No
   az += cz;
 }
 
+// Out-of-line forms for kernels whose register budget the inlined expansions
+// would exceed (accepted nodes are rare next to leaf pairs).
+__device__ __attribute__((noinline)) void node_accel64_call(const NodeMom64& M, const NodeQuad64& Q, double dx,
+                                                            double dy, double dz, const WalkParams& P, double& ax,
+                                                            double& ay, double& az) {
+  if (P.radial) node_accel_radial64(M, Q, dx, dy, dz, P, ax, ay, az);
+  else node_accel64(M, Q, dx, dy, dz, P, ax, ay, az);
+}
+
 // residualAccuracyProxy (tree_pm_acceptance.hpp:14-51) for uniform softening.
 __device__ __forceinline__ double residual_proxy64(double h, double offset, double mass, double r2,
                                                    const WalkParams& P) {
@@ -837,9 +846,10 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
 //      an LDS buffer;
 //   3. whenever the buffer fills (and at the end) every target evaluates the
 //      buffered list (accumulateTreePmGroupCulledLeaves, :290-341, and the
-//      accepted-node expansion): a group of up to 16 targets uses four lanes per
-//      target, up to 32 two, each lane taking every S-th source of a leaf and
-//      every S-th node; the lanes' sums are added at the end.
+//      accepted-node expansion): a group of T targets gives each target S lanes
+//      (the largest power of two with S T <= 64), each lane taking every S-th
+//      listed leaf and every S-th accepted node; the lanes' sums are added at
+//      the end.
 // Groups above 64 targets are evaluated 64 targets at a time, repeating the
 // (identical) tree walk for each.
 constexpr int kGroupLeafCap = 192;
@@ -873,7 +883,7 @@ template <class T> __device__ __forceinline__ T scan_max(T v, int lane) {
 }
 
 template <int SCHEME>
-__global__ __launch_bounds__(kWave) void group_walk64_kernel(WalkParams P, GroupParams G, WalkData D,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void group_walk64_kernel(WalkParams P, GroupParams G, WalkData D,
                                                               const uint32_t* __restrict__ leaf_of_pos,
                                                               WalkOutputs out, uint32_t* __restrict__ fb_pos,
                                                               uint32_t* __restrict__ fb_slot,
@@ -985,8 +995,12 @@ __global__ __launch_bounds__(kWave) void group_walk64_kernel(WalkParams P, Group
       v = ge;
       continue;
     }
-    const double c0x = p0.x + 0.5 * (lo0 + hi0), c0y = p0.y + 0.5 * (lo1 + hi1), c0z = p0.z + 0.5 * (lo2 + hi2);
-    const double hg0 = 0.5 * (hi0 - lo0), hg1 = 0.5 * (hi1 - lo1), hg2 = 0.5 * (hi2 - lo2);
+    // wave-uniform group box and minimum |a_prev| (scalar registers)
+    amin = readlane_f64(amin, 0);
+    const double c0x = readlane_f64(p0.x + 0.5 * (lo0 + hi0), 0), c0y = readlane_f64(p0.y + 0.5 * (lo1 + hi1), 0),
+                 c0z = readlane_f64(p0.z + 0.5 * (lo2 + hi2), 0);
+    const double hg0 = readlane_f64(0.5 * (hi0 - lo0), 0), hg1 = readlane_f64(0.5 * (hi1 - lo1), 0),
+                 hg2 = readlane_f64(0.5 * (hi2 - lo2), 0);
     if (lane == 0) {
       cnt[5] += 1;
       cnt[6] += count;
@@ -996,7 +1010,8 @@ __global__ __launch_bounds__(kWave) void group_walk64_kernel(WalkParams P, Group
     // ---- targets, 64 at a time
     for (uint32_t t0 = v; t0 < ge; t0 += kWave) {
       const uint32_t T = min(ge - t0, static_cast<uint32_t>(kWave));
-      const int S = T <= 16 ? 4 : (T <= 32 ? 2 : 1);
+      int S = 1;  // lanes per target: the largest power of two with S * T <= 64
+      while (S < kWave && static_cast<uint32_t>(2 * S) * T <= static_cast<uint32_t>(kWave)) S *= 2;
       const int TT = kWave / S;
       const int t = lane & (TT - 1), s = lane / TT;
       const bool valid = static_cast<uint32_t>(t) < T;
@@ -1019,21 +1034,19 @@ __global__ __launch_bounds__(kWave) void group_walk64_kernel(WalkParams P, Group
 
       auto evaluate = [&]() {
         __syncthreads();
-        for (uint32_t e = 0; e < nl; ++e) {
-          const GroupLeaf Lf = leaves[e];
+        for (uint32_t e = s; e < nl; e += S) {
           if (!valid) continue;
+          const GroupLeaf Lf = leaves[e];
           const double gx = fmax(0.0, fabs(wrap_half(Lf.ox - ox, L0)) - Lf.h);
           const double gy = fmax(0.0, fabs(wrap_half(Lf.oy - oy, L1)) - Lf.h);
           const double gz = fmax(0.0, fabs(wrap_half(Lf.oz - oz, L2)) - Lf.h);
           const bool cull = gx * gx + gy * gy + gz * gz > cull2;
-          if (s == 0) {
-            ++cnt[11];
-            if (cull) ++cnt[12];
-            else cnt[13] += Lf.e - Lf.b;
-          }
+          ++cnt[11];
+          if (cull) ++cnt[12];
+          else cnt[13] += Lf.e - Lf.b;
           if (cull) continue;
           double sx = 0.0, sy = 0.0, sz = 0.0;
-          for (uint32_t j = Lf.b + s; j < Lf.e; j += S) {
+          for (uint32_t j = Lf.b; j < Lf.e; ++j) {
             const double4 sj = D.src64[j];
             const double dx = wrap_half(sj.x - px, L0), dy = wrap_half(sj.y - py, L1), dz = wrap_half(sj.z - pz, L2);
             const double r2 = dx * dx + dy * dy + dz * dz;
@@ -1061,8 +1074,7 @@ __global__ __launch_bounds__(kWave) void group_walk64_kernel(WalkParams P, Group
             ++cnt[1];
             continue;
           }
-          if (P.radial) node_accel_radial64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
-          else node_accel64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
+          node_accel64_call(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
         }
         __syncthreads();
         nl = 0;
