@@ -30,6 +30,8 @@ for p in (str(ROOT / "chui-dev_amd"), str(ROOT)):
 HBM_PEAK = 8.0e12          # B/s, MI355X
 FP32_VALU_PEAK = 157.3e12  # FLOP/s, MI355X vector FP32
 FLOP_PER_PAIR = 40         # accounting constant per residual pair
+FP64_VALU_PEAK = 78.6e12   # FLOP/s, MI355X vector FP64
+GROUP_FLOP_PER_PAIR = 60   # FP64 Hermite-table pair of the group walk (accounting constant)
 
 
 def log(msg):
@@ -59,6 +61,9 @@ def main():
                     help="particles per side of the differentiable KDK workload (0 skips the grad/fwd figure)")
     ap.add_argument("--grad-directions", type=int, default=16,
                     help="random unit directions of the central-difference check of the gradient")
+    ap.add_argument("--adaptive-steps", type=int, default=2,
+                    help="timed FP64 solves of the adaptive policy with the group walk (the reference's fast deck "
+                         "settings), history from the FP64 strict forces; 0 skips")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="one plain device-to-device copy of this size before the timed region; its HBM counters "
                          "scale the walk's FETCH_SIZE/WRITE_SIZE readings under rocprofv3 --pmc")
@@ -131,6 +136,18 @@ def main():
         log(f"mixed {value:.4g} p/s, fp64 {value_fp64:.4g} p/s, ratio {value / value_fp64:.3f}, "
             f"mixed-vs-fp64 {err_mix_vs_fp64}")
 
+    # Adaptive policy + group walk (FP64) with the fast deck's TreePM settings
+    # (configs/treepm_evolved_snapshot_fast.param.txt:61-73): relative_monopole,
+    # alpha 0.0025, maximum angle 1.0, com_distance cutoff, Gaussian pair lookup;
+    # |a_prev| = |a| of the FP64 strict solve (or the mixed one).
+    adaptive = None
+    acc_adaptive = None
+    if args.adaptive_steps > 0:
+        adaptive, acc_adaptive = measure_adaptive(L, coord, dev, w, acc_fp64 if acc_fp64 is not None else acc_mixed,
+                                                  args.adaptive_steps,
+                                                  "fp64 strict" if acc_fp64 is not None else "mixed strict")
+        log(f"adaptive {adaptive}")
+
     # FP64 Ewald on sample targets (periodic_ewald_reference.cpp semantics).
     ewald = None
     if args.ewald_samples > 0:
@@ -144,7 +161,9 @@ def main():
         ref = np.stack([o.cpu().numpy() for o in out])
         ewald = {"samples": int(tgt.size), "alpha_box": 11.8, "modes": 24, "seconds": round(time.time() - te, 2),
                  "fp64_vs_ewald": error_metrics(acc_fp64[:, tgt], ref) if acc_fp64 is not None else None,
-                 "mixed_vs_ewald": error_metrics(acc_mixed[:, tgt], ref)}
+                 "mixed_vs_ewald": error_metrics(acc_mixed[:, tgt], ref),
+                 "adaptive_group_vs_ewald": error_metrics(acc_adaptive[:, tgt], ref) if acc_adaptive is not None
+                 else None}
         log(f"ewald: {ewald}")
 
     # Roofline of the dominant kernel (the short-range walk with fused PM
@@ -198,6 +217,7 @@ def main():
         "fp64_gpu": fp64,
         "force_error": {"mixed_vs_fp64": err_mix_vs_fp64, "ewald": ewald},
         "grad_fwd_ratio": grad,
+        "adaptive_group_walk_fp64": adaptive,
         "stages_ms_mixed": {k: v / max(prof_mixed["solves"], 1) for k, v in prof_mixed.items() if k.endswith("_ms")},
         "roofline": {"bound": "hbm", "kernel": "walkmix_kernel", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(w.name),
@@ -215,6 +235,61 @@ def main():
     }
     coord.close()
     print(json.dumps(line), flush=True)
+
+
+def measure_adaptive(L, coord, dev, w, acc_prev, steps, prev_name):
+    """FP64 adaptive-policy solves with the group walk (and one per-target
+    adaptive solve for comparison); returns (summary, group-walk accelerations)."""
+    import torch
+
+    from cosmosim import ForceAccumulator, ics
+
+    n = w.x.size
+    o = L.default_options(box_size=w.box, gravitational_constant=ics.G_CODE, opening_theta=0.7,
+                          rcut_cells=4.5 * 1.25, asmth_cells=1.25, softening_epsilon=w.softening, max_leaf_size=16)
+    o.acceptance_policy = 1
+    o.relative_force_tolerance = 0.0025
+    prev = torch.from_numpy(np.sqrt((acc_prev ** 2).sum(axis=0))).to("cuda:0")
+    acc = tuple(torch.zeros(n, dtype=torch.float64, device="cuda:0") for _ in range(3))
+    fa = ForceAccumulator(acc, previous_acceleration=prev)
+    out = {"settings": {"acceptance": "adaptive relative_monopole", "alpha": 0.0025, "maximum_opening_angle": 1.0,
+                        "cutoff_node_policy": "com_distance", "gaussian_pair_lookup": True, "group_walk": True,
+                        "history": f"|a| of the {prev_name} solve"}}
+    result = None
+    for group in (1, 0):
+        ad = L.default_adaptive_options(error_estimator=L.ESTIMATOR_RELATIVE_MONOPOLE, maximum_opening_angle=1.0,
+                                        cutoff_node_policy=L.CUTOFF_COM_DISTANCE, gaussian_pair_lookup=1,
+                                        group_walk=group)
+        reps = steps if group else 1
+        counters = coord.solve_active_set(*dev, fa, o, want_diagnostics=True, adaptive_options=ad).diagnostics
+        coord.profiling(True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            coord.solve_active_set(*dev, fa, o, adaptive_options=ad)
+        torch.cuda.synchronize()
+        elapsed = (time.perf_counter() - t) / reps
+        prof = coord.profile_read()
+        coord.profiling(False)
+        walk_ms = prof["walk_ms"] / max(prof["solves"], 1)
+        key = "group_walk" if group else "per_target_walk"
+        out[key] = {"value": n / elapsed, "unit": "particles/s", "ms_per_step": 1e3 * elapsed, "walk_ms": walk_ms,
+                    "pairs_per_target": counters["direct_pair_evaluations"] / n,
+                    "visited_nodes_per_target": counters["visited_nodes"] / n,
+                    "accepted_multipoles": counters["accepted_internal_multipoles"]}
+        if group:
+            acc_group = np.stack([a.cpu().numpy() for a in acc])
+            pairs = counters["direct_pair_evaluations"]
+            out[key].update({"groups": counters["group_walk_groups"],
+                             "fallback_targets": counters["group_walk_fallback_targets"],
+                             "list_entries_max": counters["group_walk_list_entries_max"],
+                             "valu": {"bound": "fp64 valu", "pairs": pairs, "flop_per_pair": GROUP_FLOP_PER_PAIR,
+                                      "achieved_tflops": pairs * GROUP_FLOP_PER_PAIR / (walk_ms * 1e-3) / 1e12,
+                                      "peak_tflops": FP64_VALU_PEAK / 1e12}})
+            out["group_vs_strict"] = {"reference": prev_name, **error_metrics(acc_group, acc_prev)}
+            result = acc_group
+    out["group_over_per_target"] = out["group_walk"]["value"] / out["per_target_walk"]["value"]
+    return out, result
 
 
 def measure_gradient(n_side, directions):

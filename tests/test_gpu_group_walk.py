@@ -149,3 +149,17 @@ def test_inputs_outside_the_bounded_metric_fall_back(dense):
     o, ad = seam_cluster_options(max_angle=1.0, alpha=0.01, group=True)
     _, c = check_against_oracle(x, y, z, m, o, ad, previous=prev)
     assert c["group_walk_fallback_geometry"] > 0
+
+
+def test_plain_solve_with_adaptive_policy_is_the_historyless_adaptive_walk(seam):
+    x, y, z, m = seam
+    o, _ = seam_cluster_options(group=False)
+    ad = L.default_adaptive_options()
+    acc, pm, sr, _ = gpu_adaptive(x, y, z, m, o, ad, previous=None)
+    dx, dy, dz, dm = to_dev(x, y, z, m)
+    out = [empty_dev(x.size, torch.float64) for _ in range(3)]
+    with TreePmCoordinator(MESH) as coord:
+        coord.solve_active_set(dx, dy, dz, dm, ForceAccumulator(tuple(out)), o)
+        torch.cuda.synchronize()
+    for a, b in zip(acc, out):
+        assert np.array_equal(a, b.cpu().numpy())
