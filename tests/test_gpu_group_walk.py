@@ -161,5 +161,5 @@ def test_plain_solve_with_adaptive_policy_is_the_historyless_adaptive_walk(seam)
     with TreePmCoordinator(MESH) as coord:
         coord.solve_active_set(dx, dy, dz, dm, ForceAccumulator(tuple(out)), o)
         torch.cuda.synchronize()
-    for a, b in zip(acc, out):
-        assert np.array_equal(a, b.cpu().numpy())
+    # the PM deposit's atomics leave round-off differences between two solves
+    assert rel_l2(acc, [b.cpu().numpy() for b in out]) < 1e-12
