@@ -829,8 +829,10 @@ chui_status chui_treepm_create(const int32_t mesh[3], int32_t device, chui_treep
     *out = nullptr;
     for (int a = 0; a < 3; ++a) {
       if (mesh[a] <= 0) throw std::invalid_argument("numerics.treepm_pm_grid_n{xyz} must all be > 0");
-      if (ilog2_exact(mesh[a]) < 1)
-        throw chui::UnsupportedError("PM mesh axes must be powers of two >= 2 on the GPU path");
+      int m = mesh[a];
+      while (m % 3 == 0) m /= 3;
+      if (mesh[a] % 2 != 0 || ilog2_exact(m) < 1)
+        throw chui::UnsupportedError("PM mesh axes must be even and of the form 2^a 3^b on the GPU path");
     }
     int count = 0;
     CHUI_HIP(hipGetDeviceCount(&count));
@@ -839,7 +841,7 @@ chui_status chui_treepm_create(const int32_t mesh[3], int32_t device, chui_treep
     c->device = device;
     for (int a = 0; a < 3; ++a) {
       c->mesh[a] = mesh[a];
-      c->mesh_lg[a] = ilog2_exact(mesh[a]);
+      c->mesh_lg[a] = ilog2_exact(mesh[a]);  // -1: not a power of two (mixed-radix line passes)
     }
     try {
       CHUI_HIP(hipSetDevice(device));

@@ -163,14 +163,135 @@ __device__ void lds_fft_dir(typename C2<T>::type* X, int nlines, int lg, int str
   if (tail == 1) stockham_pass<T, 2, INV>(X, nlines, lg, stride, tw, done);
 }
 
+// ---- lines of n = 3^b 2^a points (meshes that are not powers of two) -----------
+// The same Stockham pass with integer index arithmetic and a full-circle
+// twiddle table tw[m] = exp(-2 pi i m / n), m < n. Radix-3 butterflies:
+// y0 = a0 + a1 + a2, y1,2 = a0 - (a1 + a2)/2 -+ i (sqrt 3 / 2)(a1 - a2) (forward).
+template <class V, bool INV> __device__ __forceinline__ void dft3(V* v) {
+  using T = decltype(v[0].x);
+  const T s3 = T(0.86602540378443864676372317075293618);
+  const V sum = cadd(v[1], v[2]);
+  const V t1 = mk<V, T>(v[0].x - T(0.5) * sum.x, v[0].y - T(0.5) * sum.y);
+  const V d = rot_mi<V, INV>(csub(v[1], v[2]));
+  const V t2 = mk<V, T>(s3 * d.x, s3 * d.y);
+  v[0] = cadd(v[0], sum);
+  v[1] = cadd(t1, t2);
+  v[2] = csub(t1, t2);
+}
+
+template <class T, int R, bool INV>
+__device__ __attribute__((noinline)) void stockham_pass_gen(typename C2<T>::type* X, int nlines, int n, int stride,
+                                                  const typename C2<T>::type* tw, int Ns) {
+  using V = typename C2<T>::type;
+  constexpr int MAXB = fft_max_butterflies<T>();  // the host sizes blocks so lines * n / R <= MAXB * kFftThreads
+  const int q = n / R;
+  const int total = nlines * q;
+  const int step = n / (Ns * R);  // twiddle index of exp(-2 pi i / (Ns R))
+  V v[MAXB][R];
+  int dst[MAXB];
+#pragma unroll
+  for (int u = 0; u < MAXB; ++u) {
+    const int b = threadIdx.x + u * kFftThreads;
+    dst[u] = -1;
+    if (b < total) {
+      const int line = b / q, j = b - line * q;
+      const V* L = X + line * stride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[u][r] = L[j + r * q];
+      const int jg = j / Ns;
+      const int jm = j - jg * Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          V w = tw[jm * r * step];
+          if (INV) w.y = -w.y;
+          v[u][r] = cmul(v[u][r], w);
+        }
+      }
+      if constexpr (R == 3) dft3<V, INV>(v[u]);
+      else dft<V, R, INV>(v[u]);
+      dst[u] = line * stride + jg * Ns * R + jm;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < MAXB; ++u) {
+    if (dst[u] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) X[dst[u] + r * Ns] = v[u][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, bool INV>
+__device__ void lds_fft_gen_dir(typename C2<T>::type* X, int nlines, int n, int stride,
+                                const typename C2<T>::type* tw) {
+  // plan: radix-3 passes for 3^b, then radix 8 / 4 / 2 for 2^a as in lds_fft_dir
+  int Ns = 1, m = n;
+#pragma nounroll
+  while (m % 3 == 0) {
+    stockham_pass_gen<T, 3, INV>(X, nlines, n, stride, tw, Ns);
+    Ns *= 3;
+    m /= 3;
+  }
+  int lg = 0;
+  while ((1 << lg) < m) ++lg;
+  const int n8 = (lg % 3 == 1 && lg >= 4) ? (lg - 4) / 3 : lg / 3;
+  const int tail = lg - 3 * n8;
+#pragma nounroll
+  for (int i = 0; i < n8; ++i) {
+    stockham_pass_gen<T, 8, INV>(X, nlines, n, stride, tw, Ns);
+    Ns *= 8;
+  }
+  if (tail >= 2) {
+    stockham_pass_gen<T, 4, INV>(X, nlines, n, stride, tw, Ns);
+    Ns *= 4;
+  }
+  if (tail == 4) stockham_pass_gen<T, 4, INV>(X, nlines, n, stride, tw, Ns);
+  if (tail == 1) stockham_pass_gen<T, 2, INV>(X, nlines, n, stride, tw, Ns);
+}
+
+// Smallest radix of the pass plan for a line of n points (lg < 0: not a power of two).
+inline int fft_plan_min_radix(int n) {
+  int m = n, r = 1 << 30;
+  while (m % 3 == 0) {
+    m /= 3;
+    r = 3;
+  }
+  int lg = 0;
+  while ((1 << lg) < m) ++lg;
+  if (lg > 0) r = std::min(r, fft_min_radix(lg));
+  return r;
+}
+
+// Twiddle entries a kernel keeps in LDS: half a circle for power-of-two lines,
+// the full circle otherwise.
+__host__ __device__ __forceinline__ int fft_tw_len(int n, int lg) { return lg >= 0 ? n / 2 : n; }
+
 // In-place FFT of nlines LDS lines (natural order in and out); blockDim must
-// be kFftThreads and nlines * n <= 8192.
+// be kFftThreads. lg = log2(n), or -1 when n = 3^b 2^a is not a power of two.
 template <class T>
 __device__ void lds_fft(typename C2<T>::type* X, int nlines, int n, int lg, int stride,
                         const typename C2<T>::type* tw, bool inverse) {
-  (void)n;
+  if (lg < 0) {
+    if (inverse) lds_fft_gen_dir<T, true>(X, nlines, n, stride, tw);
+    else lds_fft_gen_dir<T, false>(X, nlines, n, stride, tw);
+    return;
+  }
   if (inverse) lds_fft_dir<T, true>(X, nlines, lg, stride, tw);
   else lds_fft_dir<T, false>(X, nlines, lg, stride, tw);
+}
+
+// idx -> (idx / n, idx % n) for n = 2^lg (lg >= 0) or any n.
+__device__ __forceinline__ void split_index(int idx, int n, int lg, int& hi, int& lo) {
+  if (lg >= 0) {
+    hi = idx >> lg;
+    lo = idx & (n - 1);
+  } else {
+    hi = idx / n;
+    lo = idx - hi * n;
+  }
 }
 
 // ---- deposit ------------------------------------------------------------------
@@ -224,24 +345,35 @@ __device__ __forceinline__ void stencil(const DepositArgs& a, const double4& sp,
 }
 __device__ __forceinline__ double source_mass(const double4& sp) { return sp.w; }
 
+// Index i in [-n, 2n) wrapped into [0, n).
+__device__ __forceinline__ int wrap_once(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Fixed-point coordinate q (x = q L / 2^32) on a mesh of n cells: q n / 2^32 =
+// cell + t exactly in 64-bit integers (for n = 2^lg: the top lg bits of q).
+__device__ __forceinline__ void fixed_cell(uint32_t q, int n, int& cell, float& t) {
+  const uint64_t p = static_cast<uint64_t>(q) * static_cast<uint32_t>(n);
+  cell = static_cast<int>(p >> 32);
+  t = static_cast<float>(static_cast<uint32_t>(p)) * 0x1p-32f;
+}
+
 __device__ __forceinline__ void stencil(const DepositArgs& a, const uint4& sp, int idx[3][3], float w[3][3]) {
   const uint32_t q[3] = {sp.x, sp.y, sp.z};
   for (int ax = 0; ax < 3; ++ax) {
-    const int sh = 32 - a.lg[ax];
     const int nn = a.n[ax];
-    const int cell = static_cast<int>(q[ax] >> sh);
-    const float t = static_cast<float>(q[ax] & ((1u << sh) - 1u)) * __builtin_ldexpf(1.0f, -sh);
+    int cell;
+    float t;
+    fixed_cell(q[ax], nn, cell, t);
     if (a.scheme == 0) {
       idx[ax][0] = cell;
-      idx[ax][1] = (cell + 1) & (nn - 1);
+      idx[ax][1] = wrap_once(cell + 1, nn);
       w[ax][0] = 1.0f - t;
       w[ax][1] = t;
     } else {
       const int c = t >= 0.5f ? cell + 1 : cell;
       const float d = t >= 0.5f ? t - 1.0f : t;
-      idx[ax][0] = (c - 1) & (nn - 1);
-      idx[ax][1] = c & (nn - 1);
-      idx[ax][2] = (c + 1) & (nn - 1);
+      idx[ax][0] = wrap_once(c - 1, nn);
+      idx[ax][1] = wrap_once(c, nn);
+      idx[ax][2] = wrap_once(c + 1, nn);
       w[ax][0] = 0.5f * (0.5f - d) * (0.5f - d);
       w[ax][1] = 0.75f - d * d;
       w[ax][2] = 0.5f * (0.5f + d) * (0.5f + d);
@@ -290,7 +422,7 @@ __global__ __launch_bounds__(256) void deposit_kernel(DepositArgs a, const S* __
     int o[3];
     bool in = tiled;
     for (int ax = 0; ax < 3; ++ax) {
-      o[ax] = (idx[ax][0] - u.lo[ax]) & (a.n[ax] - 1);
+      o[ax] = wrap_once(idx[ax][0] - u.lo[ax], a.n[ax]);
       in = in && o[ax] + cnt <= u.dims[ax];
     }
     for (int i = 0; i < cnt; ++i)
@@ -312,10 +444,11 @@ __global__ __launch_bounds__(256) void deposit_kernel(DepositArgs a, const S* __
   for (int r = wv; r < rows; r += blockDim.x / kWave) {
     const int ix = r / dy, iy = r - ix * dy;
     const std::size_t base =
-        (static_cast<std::size_t>((u.lo[0] + ix) & (a.n[0] - 1)) * a.n[1] + ((u.lo[1] + iy) & (a.n[1] - 1))) * a.row;
+        (static_cast<std::size_t>(wrap_once(u.lo[0] + ix, a.n[0])) * a.n[1] + wrap_once(u.lo[1] + iy, a.n[1])) *
+        a.row;
     for (int iz = lane; iz < dz; iz += kWave) {
       const T v = tile[r * dz + iz];
-      if (v != T(0)) atomicAdd(&rho[base + ((u.lo[2] + iz) & (a.n[2] - 1))], v);
+      if (v != T(0)) atomicAdd(&rho[base + wrap_once(u.lo[2] + iz, a.n[2])], v);
     }
   }
 }
@@ -328,9 +461,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
   using V = typename C2<T>::type;
   extern __shared__ unsigned char smem_raw[];
   V* tw = reinterpret_cast<V*>(smem_raw);
-  V* X = tw + (nz >> 1);
+  const int twn = fft_tw_len(nz, lg);
+  V* X = tw + twn;
   const int stride = nz + 1;
-  for (int i = threadIdx.x; i < (nz >> 1); i += blockDim.x) tw[i] = twz[i];
+  for (int i = threadIdx.x; i < twn; i += blockDim.x) tw[i] = twz[i];
   const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
   const int npts = pairs_per_block * nz;
   for (int base = threadIdx.x; base < npts; base += kFftLoadBatch * kFftThreads) {
@@ -339,7 +473,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
     for (int u = 0; u < kFftLoadBatch; ++u) {  // all loads in flight before the LDS stores
       const int idx = base + u * kFftThreads;
       v[u] = mk<V, T>(T(0), T(0));
-      const int pp = idx >> lg, i = idx & (nz - 1);
+      int pp, i;
+      split_index(idx, nz, lg, pp, i);
       const std::size_t pr = pair0 + pp;
       if (idx < npts && 2 * pr < rows) {
         const T* ra = data + (2 * pr) * row;
@@ -350,7 +485,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
 #pragma unroll
     for (int u = 0; u < kFftLoadBatch; ++u) {
       const int idx = base + u * kFftThreads;
-      if (idx < npts) X[(idx >> lg) * stride + (idx & (nz - 1))] = v[u];
+      if (idx < npts) {
+        int pp, i;
+        split_index(idx, nz, lg, pp, i);
+        X[pp * stride + i] = v[u];
+      }
     }
   }
   __syncthreads();
@@ -361,7 +500,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
     const std::size_t pr = pair0 + pp;
     if (2 * pr >= rows) continue;
     const V zk = X[pp * stride + (k == 0 ? 0 : k)];
-    const V zm = X[pp * stride + ((nz - k) & (nz - 1))];
+    const V zm = X[pp * stride + (k == 0 ? 0 : nz - k)];
     const V zmc = mk<V, T>(zm.x, -zm.y);
     const V s = cadd(zk, zmc), d = csub(zk, zmc);
     V* ca = reinterpret_cast<V*>(data + (2 * pr) * row);
@@ -381,10 +520,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::si
   using V = typename C2<T>::type;
   extern __shared__ unsigned char smem_raw[];
   V* tw = reinterpret_cast<V*>(smem_raw);
-  V* X = tw + (nz >> 1);
+  const int twn = fft_tw_len(nz, lg);
+  V* X = tw + twn;
   const int stride = nz + 1;
   const int nzc = (nz >> 1) + 1;
-  for (int i = threadIdx.x; i < (nz >> 1); i += blockDim.x) tw[i] = twz[i];
+  for (int i = threadIdx.x; i < twn; i += blockDim.x) tw[i] = twz[i];
   const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
   const int npts = pairs_per_block * nzc;
   for (int base = threadIdx.x; base < npts; base += kFftLoadBatch * kFftThreads) {
@@ -413,7 +553,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::si
         xb = mk<V, T>(xb.y * kk, -xb.x * kk);
       }
       if (k == 0 || k == nz / 2) {
-        X[pp * stride + (k & (nz - 1))] = mk<V, T>(xa.x, xb.x);
+        X[pp * stride + k] = mk<V, T>(xa.x, xb.x);
       } else {
         X[pp * stride + k] = mk<V, T>(xa.x - xb.y, xa.y + xb.x);          // xa + i xb
         X[pp * stride + (nz - k)] = mk<V, T>(xa.x + xb.y, -xa.y + xb.x);  // conj(xa) + i conj(xb)
@@ -459,9 +599,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_lines_kernel(const typename C
   using V = typename C2<T>::type;
   extern __shared__ unsigned char smem_raw[];
   V* tw = reinterpret_cast<V*>(smem_raw);
-  V* X = tw + (a.n >> 1);
+  const int twn = fft_tw_len(a.n, a.lg);
+  V* X = tw + twn;
   const int stride = a.n + 1;
-  for (int i = threadIdx.x; i < (a.n >> 1); i += blockDim.x) tw[i] = twl[i];
+  for (int i = threadIdx.x; i < twn; i += blockDim.x) tw[i] = twl[i];
   const std::size_t j0 = static_cast<std::size_t>(blockIdx.x) * a.tj;
   const std::size_t obase = static_cast<std::size_t>(blockIdx.y) * a.outer_stride;
   const int tjm = a.tj - 1;
@@ -573,7 +714,7 @@ Tables<T> upload_tables(PmWorkspace& w, const PmGeometry& g, const PmOptions& o,
   for (int ax = 0; ax < 3; ++ax) {
     toff[ax] = tws.size();
     const int n = g.n[ax];
-    for (int k = 0; k < std::max(n / 2, 1); ++k) {
+    for (int k = 0; k < n; ++k) {  // full circle; power-of-two lines use the first half
       const double ang = -2.0 * kPi * static_cast<double>(k) / static_cast<double>(n);
       V v;
       v.x = static_cast<T>(std::cos(ang));
@@ -606,17 +747,23 @@ std::size_t fft_lds_limit() {
   return limit;
 }
 
+int line_log2(int n) {
+  int lg = 0;
+  while ((1 << lg) < n) ++lg;
+  return (1 << lg) == n ? lg : -1;
+}
+
 template <class T>
 int lines_per_block(int n) {
   using V = typename C2<T>::type;
-  int lg = 0;
-  while ((1 << lg) < n) ++lg;
-  const int cap = fft_max_butterflies<T>() * kFftThreads * fft_min_radix(lg);  // points per block
+  const int lg = line_log2(n);
+  const int cap = fft_max_butterflies<T>() * kFftThreads * fft_plan_min_radix(n);  // points per block
   const std::size_t lim = fft_lds_limit();
+  const std::size_t twb = static_cast<std::size_t>(fft_tw_len(n, lg)) * sizeof(V);
   int tj = 16;
-  while (tj > 1 && (static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + (n / 2) * sizeof(V) > lim ||
-                    tj * n > cap))
-    tj >>= 1;
+  while (tj > 1 && (static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + twb > lim || tj * n > cap)) tj >>= 1;
+  if (tj * n > cap || static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + twb > lim)
+    throw UnsupportedError("PM mesh axis too long for one LDS line");
   return tj;
 }
 
@@ -625,7 +772,8 @@ void launch_lines(const typename C2<T>::type* in, typename C2<T>::type* out, Lin
                   const typename C2<T>::type* tw, const T* dk, const Tables<T>& tb, hipStream_t s) {
   using V = typename C2<T>::type;
   a.tj = lines_per_block<T>(a.n);
-  const std::size_t smem = static_cast<std::size_t>(a.tj) * (a.n + 1) * sizeof(V) + (a.n / 2) * sizeof(V);
+  const std::size_t smem =
+      static_cast<std::size_t>(a.tj) * (a.n + 1) * sizeof(V) + fft_tw_len(a.n, a.lg) * sizeof(V);
   if (smem > 64 * 1024) {
     static std::size_t granted = 0;
     if (smem > granted) {
@@ -642,13 +790,13 @@ void launch_lines(const typename C2<T>::type* in, typename C2<T>::type* out, Lin
 template <class T>
 int pairs_per_block(int nz) {
   using V = typename C2<T>::type;
-  int lg = 0;
-  while ((1 << lg) < nz) ++lg;
-  const int cap = fft_max_butterflies<T>() * kFftThreads * fft_min_radix(lg);  // points per block
+  const int lg = line_log2(nz);
+  const int cap = fft_max_butterflies<T>() * kFftThreads * fft_plan_min_radix(nz);  // points per block
+  const std::size_t twb = static_cast<std::size_t>(fft_tw_len(nz, lg)) * sizeof(V);
   int pp = 8;
-  while (pp > 1 && (static_cast<std::size_t>(pp) * (nz + 1) * sizeof(V) + (nz / 2) * sizeof(V) > 48 * 1024 ||
-                    pp * nz > cap))
-    pp >>= 1;
+  while (pp > 1 && (static_cast<std::size_t>(pp) * (nz + 1) * sizeof(V) + twb > 48 * 1024 || pp * nz > cap)) pp >>= 1;
+  if (pp * nz > cap || static_cast<std::size_t>(pp) * (nz + 1) * sizeof(V) + twb > 64 * 1024)
+    throw UnsupportedError("PM mesh z axis too long for one LDS line");
   return pp;
 }
 
@@ -658,7 +806,8 @@ void z_r2c(T* data, const PmGeometry& g, const Tables<T>& tb, hipStream_t s) {
   const std::size_t rows = static_cast<std::size_t>(g.n[0]) * g.n[1];
   const int pp = pairs_per_block<T>(g.n[2]);
   const std::size_t pairs = (rows + 1) / 2;
-  const std::size_t smem = static_cast<std::size_t>(pp) * (g.n[2] + 1) * sizeof(V) + (g.n[2] / 2) * sizeof(V);
+  const std::size_t smem =
+      static_cast<std::size_t>(pp) * (g.n[2] + 1) * sizeof(V) + fft_tw_len(g.n[2], g.lg[2]) * sizeof(V);
   fft_z_r2c_kernel<T><<<static_cast<unsigned>((pairs + pp - 1) / pp), kFftThreads, smem, s>>>(
       data, rows, g.n[2], g.lg[2], g.row, tb.twz, pp);
   CHUI_LAUNCH_CHECK("fft_z_r2c_kernel");
@@ -670,7 +819,8 @@ void z_c2r(T* data, const PmGeometry& g, const Tables<T>& tb, bool premul, hipSt
   const std::size_t rows = static_cast<std::size_t>(g.n[0]) * g.n[1];
   const int pp = pairs_per_block<T>(g.n[2]);
   const std::size_t pairs = (rows + 1) / 2;
-  const std::size_t smem = static_cast<std::size_t>(pp) * (g.n[2] + 1) * sizeof(V) + (g.n[2] / 2) * sizeof(V);
+  const std::size_t smem =
+      static_cast<std::size_t>(pp) * (g.n[2] + 1) * sizeof(V) + fft_tw_len(g.n[2], g.lg[2]) * sizeof(V);
   fft_z_c2r_kernel<T><<<static_cast<unsigned>((pairs + pp - 1) / pp), kFftThreads, smem, s>>>(
       data, rows, g.n[2], g.lg[2], g.row, tb.twz, tb.dkz, premul ? 1 : 0, pp);
   CHUI_LAUNCH_CHECK("fft_z_c2r_kernel");

@@ -457,24 +457,28 @@ __device__ void vjp_gather64(const WalkParams& P, const double* __restrict__ fx,
   sz *= hz;
 }
 
-// Stencil from a 32-bit fixed-point coordinate on a power-of-two mesh: the
-// cell index is the top log2(n) bits, the fraction the remaining bits.
+// Stencil from a 32-bit fixed-point coordinate q (x = q L / 2^32) on a mesh of
+// n cells: q n / 2^32 = cell + t exactly in 64-bit integers (for n = 2^lg the
+// cell is the top lg bits and t the remaining bits).
+__device__ __forceinline__ int wrap_once(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
 template <int SCHEME>
 __device__ __forceinline__ void stencil32(uint32_t q, int n, int lg, int (&idx)[SCHEME + 2], float (&w)[SCHEME + 2]) {
-  const int sh = 32 - lg;
-  const int cell = static_cast<int>(q >> sh);
-  const float t = static_cast<float>(q & ((1u << sh) - 1u)) * __builtin_ldexpf(1.0f, -sh);
+  (void)lg;
+  const uint64_t p = static_cast<uint64_t>(q) * static_cast<uint32_t>(n);
+  const int cell = static_cast<int>(p >> 32);
+  const float t = static_cast<float>(static_cast<uint32_t>(p)) * 0x1p-32f;
   if constexpr (SCHEME == 0) {
     idx[0] = cell;
-    idx[1] = (cell + 1) & (n - 1);
+    idx[1] = wrap_once(cell + 1, n);
     w[0] = 1.0f - t;
     w[1] = t;
   } else {
     const int c = t >= 0.5f ? cell + 1 : cell;
     const float d = t >= 0.5f ? t - 1.0f : t;
-    idx[0] = (c - 1) & (n - 1);
-    idx[1] = c & (n - 1);
-    idx[2] = (c + 1) & (n - 1);
+    idx[0] = wrap_once(c - 1, n);
+    idx[1] = wrap_once(c, n);
+    idx[2] = wrap_once(c + 1, n);
     w[0] = 0.5f * (0.5f - d) * (0.5f - d);
     w[1] = 0.75f - d * d;
     w[2] = 0.5f * (0.5f + d) * (0.5f + d);

@@ -175,8 +175,9 @@ def test_kdk_factors_reject_bad_arguments(lib):
 def test_context_argument_errors_before_any_device_call(lib):
     assert lib.chui_treepm_destroy(None) == L.CHUI_OK
     ctx = C.c_void_p()
-    bad = (C.c_int32 * 3)(48, 64, 64)
-    assert lib.chui_treepm_create(bad, 0, C.byref(ctx)) == L.CHUI_ERR_UNSUPPORTED
+    for bad_axis in (40, 45, 3):  # a factor 5, odd, odd: not an even 2^a 3^b mesh
+        bad = (C.c_int32 * 3)(bad_axis, 64, 64)
+        assert lib.chui_treepm_create(bad, 0, C.byref(ctx)) == L.CHUI_ERR_UNSUPPORTED
     zero = (C.c_int32 * 3)(0, 64, 64)
     assert lib.chui_treepm_create(zero, 0, C.byref(ctx)) == L.CHUI_ERR_INVALID_ARGUMENT
     o = valid_options()
