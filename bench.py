@@ -64,6 +64,10 @@ def main():
     ap.add_argument("--adaptive-steps", type=int, default=2,
                     help="timed FP64 solves of the adaptive policy with the group walk (the reference's fast deck "
                          "settings), history from the FP64 strict forces; 0 skips")
+    ap.add_argument("--ensemble-steps", type=int, default=5,
+                    help="KDK steps of the 48-box ensemble (16 x 32^3, 16 x 64^3, 12 x 96^3, 4 x 128^3); 0 skips")
+    ap.add_argument("--ensemble-gradients", type=int, default=1,
+                    help="1: also time the FP64 gradients of every ensemble box")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="one plain device-to-device copy of this size before the timed region; its HBM counters "
                          "scale the walk's FETCH_SIZE/WRITE_SIZE readings under rocprofv3 --pmc")
@@ -182,6 +186,11 @@ def main():
         grad = measure_gradient(args.grad_n_side, args.grad_directions)
         log(f"grad/fwd {grad}")
 
+    ensemble = None
+    if args.ensemble_steps > 0:
+        ensemble = measure_ensemble(args.ensemble_steps, args.ensemble_gradients != 0)
+        log(f"ensemble {ensemble}")
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -218,6 +227,7 @@ def main():
         "force_error": {"mixed_vs_fp64": err_mix_vs_fp64, "ewald": ewald},
         "grad_fwd_ratio": grad,
         "adaptive_group_walk_fp64": adaptive,
+        "ensemble": ensemble,
         "stages_ms_mixed": {k: v / max(prof_mixed["solves"], 1) for k, v in prof_mixed.items() if k.endswith("_ms")},
         "roofline": {"bound": "hbm", "kernel": "walkmix_kernel", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(w.name),
@@ -235,6 +245,53 @@ def main():
     }
     coord.close()
     print(json.dumps(line), flush=True)
+
+
+def measure_ensemble(steps, gradients, workers=8):
+    """The 48-box ensemble: leaf size per box size from the autotuner, one
+    batched mixed-precision KDK forward of `steps` steps (initial forces plus
+    one batched solve per step), the same forward from one host worker, and the
+    FP64 gradients of every box."""
+    import torch
+
+    import cosmosim._lib as L
+    from cosmosim import ensemble as E
+
+    specs = E.ensemble_specs()
+    tuned = {}
+    for n_side in sorted({s.n_side for s in specs}):
+        spec = next(s for s in specs if s.n_side == n_side)
+        best, times = E.autotune_leaf(spec)
+        tuned[n_side] = {"leaf": best, "ms_by_leaf": {str(k): round(v, 3) for k, v in times.items()}}
+    leaves = {k: v["leaf"] for k, v in tuned.items()}
+    particles = sum(s.n for s in specs)
+    out = {"boxes": len(specs), "particles": particles, "kdk_steps": steps, "precision": "mixed",
+           "autotuned_leaf": tuned, "meshes": "2 n_side per axis (64, 128, 192, 256)"}
+    for workers_ in (workers, 1):
+        with E.Ensemble(specs, workers=workers_) as ens:
+            fwd = E.EnsembleForward(ens, steps, L.PRECISION_MIXED, leaves)
+            ens.solve(fwd.x, fwd.m, fwd.g, fwd.options)  # warm-up: sizes every box's buffers
+            for v in ens.views:
+                v.synchronize()
+            t = time.perf_counter()
+            fwd.run()
+            el = time.perf_counter() - t
+        key = "forward" if workers_ == workers else "forward_one_worker"
+        out[key] = {"workers": workers_, "seconds": el, "force_evaluations_per_box": steps + 1,
+                    "value": particles * (steps + 1) / el, "unit": "particle force evaluations/s"}
+        log(f"ensemble forward with {workers_} workers: {el:.3f} s")
+    out["worker_speedup"] = out["forward"]["value"] / out["forward_one_worker"]["value"]
+    if gradients:
+        t = time.perf_counter()
+        nf = nv = 0
+        for s in specs:
+            _, _, f, v = E.box_gradient(s, steps)
+            nf += f
+            nv += v
+        torch.cuda.synchronize()
+        out["gradients"] = {"seconds": time.perf_counter() - t, "precision": "fp64", "force_evaluations": nf,
+                            "force_vjps": nv, "checkpoint_every": 2}
+    return out
 
 
 def measure_adaptive(L, coord, dev, w, acc_prev, steps, prev_name):

@@ -163,6 +163,13 @@ class WalkCounters(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "struct_size"}
 
 
+class Box(C.Structure):
+    """chui_treepm_box: one box of a batched ensemble solve (device pointers)."""
+    _fields_ = [("pos_x", C.c_void_p), ("pos_y", C.c_void_p), ("pos_z", C.c_void_p), ("mass", C.c_void_p),
+                ("n", C.c_uint64), ("accel_x", C.c_void_p), ("accel_y", C.c_void_p), ("accel_z", C.c_void_p),
+                ("options", C.POINTER(Options))]
+
+
 class KdkFactors(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "scale_factor_begin", "scale_factor_midpoint", "scale_factor_end", "drift_factor",
@@ -199,6 +206,10 @@ SIGNATURES = {
     "chui_treepm_solve_adaptive_device": [_P, _P, _P, _P, _P, _U64, _P, _U64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                           _P, C.POINTER(Options), C.POINTER(AdaptiveOptions), _I32,
                                           C.POINTER(WalkCounters)],
+    "chui_treepm_ensemble_create": [_I32, C.POINTER(_I32), _I32, _I32, C.POINTER(_P)],
+    "chui_treepm_ensemble_destroy": [_P],
+    "chui_treepm_ensemble_box": [_P, _I32, C.POINTER(_P)],
+    "chui_treepm_ensemble_solve_device": [_P, C.POINTER(Box), _I32, _I32],
     "chui_treepm_profiling": [_P, _I32],
     "chui_treepm_profile_read": [_P, C.POINTER(_D), C.POINTER(_U64)],
     "chui_treepm_kdk_factors": [_D, _D, _D, _D, _D, C.POINTER(KdkFactors)],

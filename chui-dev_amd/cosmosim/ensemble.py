@@ -1,0 +1,229 @@
+"""Ensembles of independent periodic boxes: workload law, batched KDK forward,
+per-box gradients and the leaf-size autotuner.
+
+The reference evolves one box per run; ensembles have no counterpart there.
+Each box follows the single-box path (TreePmCoordinator::solveActiveSet,
+tree_pm_coupling.hpp:563-584, and the KDK pieces of time_integration.cpp:
+448-450, 3530-3556). The batched forward issues every box's force evaluation
+through chui_treepm_ensemble_solve_device (host worker threads, one stream per
+box); kicks and drifts run on each box's stream between the batched solves.
+Gradients come from the differentiable forward of cosmosim.adjoint (FP64, one
+box at a time).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib, adjoint, ics
+from .treepm import TreePmCoordinator, kdk_factors
+
+
+@dataclass
+class BoxSpec:
+    n_side: int
+    box: float
+    mesh: int
+    spectral_index: float
+    a_begin: float
+    a_end: float
+    seed: int
+
+    @property
+    def n(self) -> int:
+        return self.n_side**3
+
+
+def ensemble_specs(counts=((32, 16), (64, 16), (96, 12), (128, 4)), seed: int = ics.BASE_SEED + 4) -> list:
+    """The ensemble law: for each (n_side, count), count boxes of n_side^3
+    particles in 50 n_side / 64 Mpc/h (the density of the 64^3 / 50 Mpc/h
+    adjoint workload), mesh 2 n_side, spectral index uniform in [-2.8, -2.0],
+    a_begin uniform in [0.05, 0.1] and a_end = a_begin + U[0.25, 0.45]; every
+    box has its own seed."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for n_side, count in counts:
+        for _ in range(count):
+            a0 = float(rng.uniform(0.05, 0.1))
+            specs.append(BoxSpec(n_side=n_side, box=50.0 * n_side / 64, mesh=2 * n_side,
+                                 spectral_index=float(rng.uniform(-2.8, -2.0)), a_begin=a0,
+                                 a_end=a0 + float(rng.uniform(0.25, 0.45)), seed=int(rng.integers(1 << 31))))
+    return specs
+
+
+def box_initial_conditions(spec: BoxSpec, rms_spacing: float = 0.1):
+    """Lattice + Zel'dovich displacement psi (3, N) of the box's own field."""
+    rng = np.random.default_rng(spec.seed)
+    psi = ics.zeldovich_displacement(spec.n_side, spec.box, spec.spectral_index, rms_spacing, rng)
+    spacing = spec.box / spec.n_side
+    idx = (np.arange(spec.n_side, dtype=np.float64) + 0.5) * spacing
+    gx, gy, gz = np.meshgrid(idx, idx, idx, indexing="ij")
+    lattice = np.stack([gx.reshape(-1), gy.reshape(-1), gz.reshape(-1)])
+    return lattice, psi
+
+
+class Ensemble:
+    """One context per box (chui_treepm_ensemble_create)."""
+
+    def __init__(self, specs, device: int = 0, workers: int = 8):
+        self._lib = _lib.load()
+        self.specs = list(specs)
+        meshes = (C.c_int32 * (3 * len(self.specs)))(*[s.mesh for s in self.specs for _ in range(3)])
+        handle = C.c_void_p()
+        _lib.check(self._lib.chui_treepm_ensemble_create(len(self.specs), meshes, int(device), int(workers),
+                                                         C.byref(handle)))
+        self._h = handle
+        self.views = []
+        for b, s in enumerate(self.specs):
+            ctx = C.c_void_p()
+            _lib.check(self._lib.chui_treepm_ensemble_box(self._h, b, C.byref(ctx)))
+            self.views.append(TreePmCoordinator.borrowed(ctx, (s.mesh,) * 3))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for v in self.views:
+                v.close()
+            _lib.check(self._lib.chui_treepm_ensemble_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, positions, masses, accels, options, refresh: bool = True):
+        """positions[b] = (x, y, z), accels[b] = (ax, ay, az): device tensors;
+        options[b]: cosmosim._lib.Options (kept alive by the caller)."""
+        boxes = (_lib.Box * len(self.specs))()
+        for b in range(len(self.specs)):
+            x, y, z = positions[b]
+            ax, ay, az = accels[b]
+            boxes[b] = _lib.Box(x.data_ptr(), y.data_ptr(), z.data_ptr(), masses[b].data_ptr(), int(x.shape[0]),
+                                ax.data_ptr(), ay.data_ptr(), az.data_ptr(), C.pointer(options[b]))
+        _lib.check(self._lib.chui_treepm_ensemble_solve_device(self._h, boxes, len(self.specs), 1 if refresh else 0))
+
+
+def box_options(spec: BoxSpec, precision: int, leaf: int = 16):
+    spacing = spec.box / spec.n_side
+    return _lib.default_options(box_size=spec.box, gravitational_constant=ics.G_CODE, opening_theta=0.5,
+                                asmth_cells=1.25, rcut_cells=4.5 * 1.25, softening_epsilon=spacing / 30.0,
+                                max_leaf_size=leaf, precision=precision)
+
+
+def autotune_leaf(spec: BoxSpec, candidates=(8, 12, 16, 24, 32), repeats: int = 3, device: str = "cuda:0"):
+    """Leaf size with the fastest mixed-precision force evaluation of one box
+    of this size (its own initial conditions), among candidates. Returns
+    (best leaf, {leaf: ms})."""
+    lattice, psi = box_initial_conditions(spec)
+    pos = np.mod(lattice + psi, spec.box)
+    pos[pos >= spec.box] = 0.0
+    dev = [torch.from_numpy(np.ascontiguousarray(p)).to(device) for p in pos]
+    rho_crit = 3.0 * ics.H0_CODE**2 / (8.0 * np.pi * ics.G_CODE)
+    m = torch.full((spec.n,), rho_crit * ics.OMEGA_M * spec.box**3 / spec.n, dtype=torch.float64, device=device)
+    acc = tuple(torch.empty(spec.n, dtype=torch.float32, device=device) for _ in range(3))
+    torch.cuda.synchronize(device)  # the library runs on its own stream
+    from .treepm import ForceAccumulator
+
+    times = {}
+    with TreePmCoordinator(spec.mesh) as coord:
+        for leaf in candidates:
+            o = box_options(spec, _lib.PRECISION_MIXED, leaf)
+            coord.solve_active_set(*dev, m, ForceAccumulator(acc), o)
+            coord.synchronize()
+            t = time.perf_counter()
+            for _ in range(repeats):
+                coord.solve_active_set(*dev, m, ForceAccumulator(acc), o)
+            coord.synchronize()
+            times[leaf] = 1e3 * (time.perf_counter() - t) / repeats
+    best = min(times, key=times.get)
+    return best, times
+
+
+class EnsembleForward:
+    """Batched KDK forward of every box (steps uniform in a from a_begin to a_end)."""
+
+    def __init__(self, ens: Ensemble, steps: int, precision: int, leaf_by_n_side=None, device: str = "cuda:0"):
+        self.ens = ens
+        self.steps = steps
+        self.precision = precision
+        self.device = device
+        leaf_by_n_side = leaf_by_n_side or {}
+        self.options = [box_options(s, precision, leaf_by_n_side.get(s.n_side, 16)) for s in ens.specs]
+        self.factors = []
+        for s in ens.specs:
+            a = np.linspace(s.a_begin, s.a_end, steps + 1)
+            self.factors.append([kdk_factors(a0, a1, ics.H0_CODE, ics.OMEGA_M, 1.0 - ics.OMEGA_M)
+                                 for a0, a1 in zip(a[:-1], a[1:])])
+        adt = torch.float32 if precision == _lib.PRECISION_MIXED else torch.float64
+        self.x, self.v, self.m, self.g = [], [], [], []
+        rho_crit = 3.0 * ics.H0_CODE**2 / (8.0 * np.pi * ics.G_CODE)
+        for s in ens.specs:
+            lattice, psi = box_initial_conditions(s)
+            pos = np.mod(lattice + psi, s.box)
+            pos[pos >= s.box] = 0.0
+            vf = s.a_begin * ics.hubble(s.a_begin) * ics.growth_rate(s.a_begin)
+            self.x.append(tuple(torch.from_numpy(np.ascontiguousarray(p)).to(device) for p in pos))
+            self.v.append(tuple(torch.from_numpy(np.ascontiguousarray(vf * p)).to(device) for p in psi))
+            self.m.append(torch.full((s.n,), rho_crit * ics.OMEGA_M * s.box**3 / s.n, dtype=torch.float64,
+                                     device=device))
+            self.g.append(tuple(torch.empty(s.n, dtype=adt, device=device) for _ in range(3)))
+        torch.cuda.synchronize(device)
+
+    @property
+    def particles(self) -> int:
+        return sum(s.n for s in self.ens.specs)
+
+    def run(self):
+        """Initial forces and every KDK step; returns when all boxes are done."""
+        ens = self.ens
+        ens.solve(self.x, self.m, self.g, self.options)
+        for k in range(self.steps):
+            for b, s in enumerate(ens.specs):
+                f = self.factors[b][k]
+                view = ens.views[b]
+                view.kick(*self.v[b], *self.g[b], f["first_kick_factor"], f["first_hubble_drag"])
+                view.drift(*self.x[b], *self.v[b], f["drift_factor"], (s.box,) * 3)
+            ens.solve(self.x, self.m, self.g, self.options)
+            for b in range(len(ens.specs)):
+                f = self.factors[b][k]
+                ens.views[b].kick(*self.v[b], *self.g[b], f["second_kick_factor"], f["second_hubble_drag"])
+        for v in ens.views:
+            v.synchronize()
+
+
+def box_gradient(spec: BoxSpec, steps: int, device: str = "cuda:0", checkpoint_every: int = 2):
+    """d loss / d psi of one box: loss = sum of squared differences between the
+    final CIC density contrast (n_side^3 grid) and the CIC contrast of a second
+    seeded displacement of ten times the RMS. FP64 forward and adjoint.
+    Returns (loss, gradient (3, N) on the device, force evaluations, VJPs)."""
+    lattice, psi = box_initial_conditions(spec)
+    del lattice
+    prob = adjoint.KdkProblem(n_side=spec.n_side, box=spec.box, mesh=spec.mesh,
+                              scale_factors=list(np.linspace(spec.a_begin, spec.a_end, steps + 1)),
+                              softening=spec.box / spec.n_side / 30.0, loss_mesh=spec.n_side,
+                              checkpoint_every=checkpoint_every, device=device)
+    try:
+        tdisp = ics.zeldovich_displacement(spec.n_side, spec.box, spec.spectral_index, 1.0,
+                                           np.random.default_rng(spec.seed + 1))
+        target = prob.cic_density(prob.wrap(prob.lattice + torch.tensor(tdisp, dtype=torch.float64, device=device)))
+        p = torch.tensor(psi, dtype=torch.float64, device=device).requires_grad_(True)
+        loss = adjoint.density_loss(p, target, prob)
+        (grad,) = torch.autograd.grad(loss, p)
+        return float(loss.detach()), grad.detach(), prob.force_evaluations, prob.vjp_evaluations
+    finally:
+        prob.close()
+
+
+__all__ = ["BoxSpec", "Ensemble", "EnsembleForward", "autotune_leaf", "box_gradient", "box_initial_conditions",
+           "box_options", "ensemble_specs"]

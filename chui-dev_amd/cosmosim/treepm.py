@@ -54,10 +54,22 @@ class TreePmCoordinator:
         _lib.check(lib.chui_treepm_create(mesh, int(device), C.byref(handle)))
         self._h = handle
         self._lib = lib
+        self._owned = True
+
+    @classmethod
+    def borrowed(cls, handle: C.c_void_p, pm_shape) -> "TreePmCoordinator":
+        """A view of a context another owner (an ensemble) creates and destroys."""
+        self = cls.__new__(cls)
+        self.pm_shape = tuple(int(v) for v in pm_shape)
+        self._h = handle
+        self._lib = _lib.load()
+        self._owned = False
+        return self
 
     def close(self):
         if getattr(self, "_h", None):
-            _lib.check(self._lib.chui_treepm_destroy(self._h))
+            if getattr(self, "_owned", True):
+                _lib.check(self._lib.chui_treepm_destroy(self._h))
             self._h = None
 
     def __del__(self):

@@ -9,12 +9,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/chui_treepm.h"
@@ -864,6 +866,128 @@ chui_status chui_treepm_destroy(chui_treepm_context* c) {
       if (e) (void)hipEventDestroy(e);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+  });
+}
+
+// ---- ensembles ------------------------------------------------------------------
+// Independent periodic boxes, one context (own mesh, tree, stream) per box.
+// A batched solve hands the boxes, largest cost first, to host worker threads;
+// each worker runs whole solves on its boxes' streams, so small boxes overlap
+// on the device instead of queueing behind one another.
+struct chui_treepm_ensemble {
+  int device = 0;
+  int workers = 1;
+  std::vector<chui_treepm_context*> boxes;
+};
+
+chui_status chui_treepm_ensemble_create(int32_t box_count, const int32_t* meshes, int32_t device, int32_t workers,
+                                        chui_treepm_ensemble** out) {
+  return guarded([&] {
+    if (!meshes || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    if (box_count <= 0) throw std::invalid_argument("an ensemble needs at least one box");
+    if (workers < 1 || workers > 64) throw std::invalid_argument("ensemble workers must be in [1, 64]");
+    auto* e = new chui_treepm_ensemble();
+    e->device = device;
+    e->workers = workers;
+    try {
+      for (int32_t b = 0; b < box_count; ++b) {
+        chui_treepm_context* c = nullptr;
+        const chui_status st = chui_treepm_create(meshes + 3 * b, device, &c);
+        if (st != CHUI_OK) {
+          const std::string msg = "box " + std::to_string(b) + ": " + g_last_error;
+          if (st == CHUI_ERR_UNSUPPORTED) throw chui::UnsupportedError(msg);
+          if (st == CHUI_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+          throw std::runtime_error(msg);
+        }
+        e->boxes.push_back(c);
+      }
+    } catch (...) {
+      for (auto* c : e->boxes) chui_treepm_destroy(c);
+      delete e;
+      throw;
+    }
+    *out = e;
+  });
+}
+
+chui_status chui_treepm_ensemble_destroy(chui_treepm_ensemble* e) {
+  return guarded([&] {
+    if (!e) return;
+    for (auto* c : e->boxes) chui_treepm_destroy(c);
+    delete e;
+  });
+}
+
+chui_status chui_treepm_ensemble_box(chui_treepm_ensemble* e, int32_t index, chui_treepm_context** out) {
+  return guarded([&] {
+    if (!e || !out) throw std::invalid_argument("null argument");
+    if (index < 0 || static_cast<std::size_t>(index) >= e->boxes.size())
+      throw std::invalid_argument("ensemble box index out of range");
+    *out = e->boxes[static_cast<std::size_t>(index)];
+  });
+}
+
+chui_status chui_treepm_ensemble_solve_device(chui_treepm_ensemble* e, const chui_treepm_box* boxes,
+                                              int32_t box_count, int32_t refresh_long_range) {
+  return guarded([&] {
+    if (!e || !boxes) throw std::invalid_argument("null argument");
+    if (box_count != static_cast<int32_t>(e->boxes.size()))
+      throw std::invalid_argument("ensemble solve needs one chui_treepm_box per ensemble box");
+    // largest cost first (N log N + M log M): a greedy list schedule over the workers
+    std::vector<int> order(static_cast<std::size_t>(box_count));
+    std::vector<double> cost(order.size());
+    for (int b = 0; b < box_count; ++b) {
+      order[b] = b;
+      const chui_treepm_context* c = e->boxes[b];
+      const double n = static_cast<double>(boxes[b].n) + 1.0;
+      const double m = static_cast<double>(c->mesh[0]) * c->mesh[1] * c->mesh[2];
+      cost[b] = n * std::log2(n) + m * std::log2(m);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::vector<chui_status> status(order.size(), CHUI_OK);
+    std::vector<std::string> message(order.size());
+    auto work = [&]() {
+      (void)hipSetDevice(e->device);
+      for (;;) {
+        const int k = next.fetch_add(1);
+        if (k >= box_count || failed.load()) return;
+        const int b = order[k];
+        const chui_treepm_box& bx = boxes[b];
+        chui_treepm_context* c = e->boxes[b];
+        status[b] = guarded([&] {
+          if (!bx.options) throw std::invalid_argument("ensemble box options are null");
+          solve_device_impl(c, bx.pos_x, bx.pos_y, bx.pos_z, bx.mass, bx.n, nullptr, bx.n, bx.accel_x, bx.accel_y,
+                            bx.accel_z, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, bx.options,
+                            refresh_long_range, nullptr);
+          CHUI_HIP(hipStreamSynchronize(c->stream));
+        });
+        if (status[b] != CHUI_OK) {
+          message[b] = g_last_error;
+          failed.store(true);
+        }
+      }
+    };
+    const int nw = std::min<int>(e->workers, box_count);
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nw; ++w) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (int b = 0; b < box_count; ++b) {
+      if (status[b] == CHUI_OK) continue;
+      const std::string msg = "ensemble box " + std::to_string(b) + ": " + message[b];
+      switch (status[b]) {
+        case CHUI_ERR_INVALID_ARGUMENT: throw std::invalid_argument(msg);
+        case CHUI_ERR_UNSUPPORTED: throw chui::UnsupportedError(msg);
+        case CHUI_ERR_OVERFLOW: throw std::overflow_error(msg);
+        case CHUI_ERR_OUT_OF_MEMORY: throw chui::OutOfMemoryError(msg);
+        case CHUI_ERR_DEVICE: throw chui::DeviceError(msg);
+        case CHUI_ERR_LOGIC: throw std::logic_error(msg);
+        default: throw std::runtime_error(msg);
+      }
+    }
   });
 }
 

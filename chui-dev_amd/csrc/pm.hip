@@ -11,6 +11,8 @@
 // inverse FFT is folded into the Green's function.
 #include "pm.hpp"
 
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
@@ -775,7 +777,10 @@ void launch_lines(const typename C2<T>::type* in, typename C2<T>::type* out, Lin
   const std::size_t smem =
       static_cast<std::size_t>(a.tj) * (a.n + 1) * sizeof(V) + fft_tw_len(a.n, a.lg) * sizeof(V);
   if (smem > 64 * 1024) {
+    // contexts of an ensemble launch from several host threads
+    static std::mutex mu;
     static std::size_t granted = 0;
+    std::lock_guard<std::mutex> lock(mu);
     if (smem > granted) {
       CHUI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lines_kernel<T>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem)));

@@ -300,6 +300,39 @@ chui_status chui_treepm_solve_adaptive_device(chui_treepm_context* ctx,
                                               const chui_treepm_adaptive_options* adaptive,
                                               int32_t refresh_long_range, chui_treepm_walk_counters* counters);
 
+/* ---- ensembles of independent boxes ---------------------------------------
+ * No counterpart in the reference (one box per run there). An ensemble holds
+ * one context per box (own mesh, tree, workspaces and stream). */
+typedef struct chui_treepm_ensemble chui_treepm_ensemble;
+
+/* One box of a batched solve: device arrays, all n rows active, FP64 (double
+ * accelerations) or mixed (float accelerations) by options->precision. */
+typedef struct chui_treepm_box {
+  const double* pos_x;
+  const double* pos_y;
+  const double* pos_z;
+  const double* mass;
+  uint64_t n;
+  void* accel_x;
+  void* accel_y;
+  void* accel_z;
+  const chui_treepm_options* options;
+} chui_treepm_box;
+
+/* meshes = int32[3 * box_count]; workers = host threads that issue the boxes'
+ * solves (1..64). */
+chui_status chui_treepm_ensemble_create(int32_t box_count, const int32_t* meshes, int32_t device, int32_t workers,
+                                        chui_treepm_ensemble** out);
+chui_status chui_treepm_ensemble_destroy(chui_treepm_ensemble* ensemble);
+/* The context of one box (owned by the ensemble): kick, drift, adjoint and
+ * parity entry points take it like any other context. */
+chui_status chui_treepm_ensemble_box(chui_treepm_ensemble* ensemble, int32_t index, chui_treepm_context** box);
+/* solveActiveSet for every box (all rows active), the boxes handed to the
+ * workers largest cost (N log N + M log M) first; each box's solve runs on
+ * its context's stream. Returns when every box is done. */
+chui_status chui_treepm_ensemble_solve_device(chui_treepm_ensemble* ensemble, const chui_treepm_box* boxes,
+                                              int32_t box_count, int32_t refresh_long_range);
+
 /* Per-stage device timing accumulated over solves with HIP events on the
  * context stream (no extra synchronisation inside a solve).
  * chui_treepm_profiling(ctx, 1) resets and enables, (ctx, 0) disables.
