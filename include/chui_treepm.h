@@ -164,7 +164,8 @@ chui_status chui_treepm_options_set(chui_treepm_options* options, const char* ke
 chui_status chui_treepm_options_validate(const chui_treepm_options* options, const int32_t mesh[3]);
 
 /* ---- context ----------------------------------------------------------- */
-/* mesh = {nx, ny, nz}; device = HIP ordinal. Mesh axes must be powers of two. */
+/* mesh = {nx, ny, nz}; device = HIP ordinal. Each axis must be even with no prime
+ * factor other than 2 and 3 (2^a 3^b, a >= 1); others return CHUI_ERR_UNSUPPORTED. */
 chui_status chui_treepm_create(const int32_t mesh[3], int32_t device, chui_treepm_context** out);
 chui_status chui_treepm_destroy(chui_treepm_context* ctx);
 /* Stream the context enqueues on (hipStream_t; NULL = library-owned stream). */
