@@ -204,7 +204,7 @@ SIGNATURES = {
     "chui_treepm_default_adaptive_options": [C.POINTER(AdaptiveOptions)],
     "chui_treepm_adaptive_options_set": [C.POINTER(AdaptiveOptions), C.c_char_p, C.c_char_p],
     "chui_treepm_solve_adaptive_device": [_P, _P, _P, _P, _P, _U64, _P, _U64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                          _P, C.POINTER(Options), C.POINTER(AdaptiveOptions), _I32,
+                                          _P, _P, C.POINTER(Options), C.POINTER(AdaptiveOptions), _I32,
                                           C.POINTER(WalkCounters)],
     "chui_treepm_ensemble_create": [_I32, C.POINTER(_I32), _I32, _I32, C.POINTER(_P)],
     "chui_treepm_ensemble_destroy": [_P],

@@ -36,6 +36,8 @@ class ForceAccumulator:
     short_range: tuple | None = None
     # previous_acceleration_magnitude_code: |a_prev| per active slot (adaptive policy)
     previous_acceleration: object = None
+    # short_range_nearest_pair_r2_comoving: source-row indexed encounter lane
+    nearest_pair_r2: object = None
 
 
 @dataclass
@@ -108,6 +110,8 @@ class TreePmCoordinator:
             diag = _lib.Diagnostics()
             diag.struct_size = C.sizeof(_lib.Diagnostics)
         on_device = not isinstance(pos_x, np.ndarray)
+        if adaptive_options is None and accumulator.nearest_pair_r2 is not None:
+            adaptive_options = _lib.default_adaptive_options()
         if adaptive_options is not None:
             # adaptive acceptance / group walk (chui_treepm_solve_adaptive_device)
             if not on_device:
@@ -118,7 +122,8 @@ class TreePmCoordinator:
                 self._h, _ptr(pos_x), _ptr(pos_y), _ptr(pos_z), _ptr(mass), n, _ptr(active), n_active,
                 _ptr(accumulator.accel[0]), _ptr(accumulator.accel[1]), _ptr(accumulator.accel[2]),
                 _ptr(lr[0]), _ptr(lr[1]), _ptr(lr[2]), _ptr(sr[0]), _ptr(sr[1]), _ptr(sr[2]),
-                _ptr(accumulator.previous_acceleration), C.byref(options), C.byref(adaptive_options),
+                _ptr(accumulator.nearest_pair_r2), _ptr(accumulator.previous_acceleration), C.byref(options),
+                C.byref(adaptive_options),
                 1 if refresh_long_range_field else 0, C.byref(counters) if want_diagnostics else None))
             return SolveResult(diagnostics=counters.as_dict() if want_diagnostics else {})
         fn = self._lib.chui_treepm_solve_device if on_device else self._lib.chui_treepm_solve

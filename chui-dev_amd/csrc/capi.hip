@@ -375,7 +375,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
                        void* pmy, void* pmz, void* srx, void* sry, void* srz, const chui_treepm_options* o,
                        int refresh, chui_treepm_diagnostics* diag, const double* prev = nullptr,
                        const chui_treepm_adaptive_options* adaptive = nullptr,
-                       chui_treepm_walk_counters* wcount = nullptr) {
+                       chui_treepm_walk_counters* wcount = nullptr, void* near2 = nullptr) {
   validate_options(o, c->mesh);
   const bool adaptive_policy = o->acceptance_policy == CHUI_ACCEPT_ADAPTIVE;
   const chui_treepm_adaptive_options ad = adaptive ? *adaptive : adaptive_defaults();
@@ -506,11 +506,12 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   chui::WalkParams wp = base_walk_params(c, o, sp, g.row);
   wp.n_targets = static_cast<uint32_t>(n_active);
   wp.identity_targets = identity;
-  wp.count = (diag != nullptr || wcount != nullptr) ? 1 : 0;
+  // the mixed walk's counting variant also produces the nearest-pair lane
+  wp.count = (diag != nullptr || wcount != nullptr || (mixed && near2 != nullptr)) ? 1 : 0;
   wp.uniform_mass = um;
   wp.uniform_mass_f = static_cast<float>(um);
   wp.pm_enabled = 1;
-  chui::WalkOutputs out{ax, ay, az, pmx, pmy, pmz, srx, sry, srz};
+  chui::WalkOutputs out{ax, ay, az, pmx, pmy, pmz, srx, sry, srz, near2};
   uint32_t group_status = ad.group_walk ? CHUI_GROUP_WALK_INCOMPATIBLE_POLICY : CHUI_GROUP_WALK_NOT_REQUESTED;
   const bool group_counts = adaptive_policy || wcount != nullptr;
   if (group_counts) {
@@ -622,6 +623,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       wp.c8_tab = static_cast<float>(0.125 / (sp.a * sp.a * sp.a) / (t2c * sq));
       // with uniform masses the pair kernel leaves the mass to this factor
       wp.force_scale = static_cast<float>(wp.G * t2c * (uniform ? um : 1.0));
+      wp.tab_to_len2 = static_cast<float>(1.0 / t2c);
       wp.accept_half_max = static_cast<float>(0.041 * sp.rcut);
       if (c->htab_key[0] != t_max || c->htab_key[1] != sp.a || !c->htab.ptr) {
         c->htab.reserve(tab.size() * sizeof(float));
@@ -1182,19 +1184,18 @@ chui_status chui_treepm_adaptive_options_set(chui_treepm_adaptive_options* a, co
 
 chui_status chui_treepm_solve_adaptive_device(chui_treepm_context* c, const double* x, const double* y,
                                               const double* z, const double* m, uint64_t n, const uint32_t* active,
-                                              uint64_t n_active, double* ax, double* ay, double* az, double* pmx,
-                                              double* pmy, double* pmz, double* srx, double* sry, double* srz,
-                                              const double* previous_acceleration, const chui_treepm_options* o,
+                                              uint64_t n_active, void* ax, void* ay, void* az, void* pmx,
+                                              void* pmy, void* pmz, void* srx, void* sry, void* srz,
+                                              void* nearest_pair_r2, const double* previous_acceleration,
+                                              const chui_treepm_options* o,
                                               const chui_treepm_adaptive_options* adaptive, int32_t refresh,
                                               chui_treepm_walk_counters* counters) {
   return guarded([&] {
     require_ctx(c);
     set_device(c);
     if (!adaptive) throw std::invalid_argument("TreePM adaptive options pointer is null");
-    if (o && o->precision != CHUI_PRECISION_FP64)
-      throw chui::UnsupportedError("chui_treepm_solve_adaptive_device is built for FP64 precision only");
     solve_device_impl(c, x, y, z, m, n, active, n_active, ax, ay, az, pmx, pmy, pmz, srx, sry, srz, o, refresh,
-                      nullptr, previous_acceleration, adaptive, counters);
+                      nullptr, previous_acceleration, adaptive, counters, nearest_pair_r2);
   });
 }
 
@@ -1227,24 +1228,28 @@ chui_status chui_treepm_solve(chui_treepm_context* c, const double* x, const dou
     }
     const bool mixed = o->precision == CHUI_PRECISION_MIXED;
     const std::size_t eb = mixed ? 4 : 8;
-    for (auto& b : c->hout) b.reserve(std::max<uint64_t>(na, 1) * eb);
+    // lanes 0-5 per active slot, 6-8 (short range) per source row
+    for (int i = 0; i < 9; ++i) c->hout[i].reserve(std::max<uint64_t>(i < 6 ? na : n, 1) * eb);
     double* hosts[9] = {ax, ay, az, pmx, pmy, pmz, srx, sry, srz};
     void* dev[9];
     for (int i = 0; i < 9; ++i) dev[i] = hosts[i] ? c->hout[i].ptr : nullptr;
     solve_device_impl(c, c->hx.as<double>(), c->hy.as<double>(), c->hz.as<double>(), c->hm.as<double>(), n,
                       active ? c->hactive.as<uint32_t>() : nullptr, na, dev[0], dev[1], dev[2], dev[3], dev[4],
                       dev[5], dev[6], dev[7], dev[8], o, refresh, diag);
-    if (!mixed) {
-      for (int i = 0; i < 9; ++i)
-        if (hosts[i] && na) CHUI_HIP(hipMemcpyAsync(hosts[i], dev[i], na * 8, hipMemcpyDeviceToHost, s));
+    std::vector<double> tmp64;
+    for (int i = 0; i < 9; ++i) {
+      if (!hosts[i] || !na) continue;
+      const uint64_t len = i < 6 ? na : n;
+      if (mixed) c->host_float.resize(len);
+      else tmp64.resize(len);
+      CHUI_HIP(hipMemcpyAsync(mixed ? static_cast<void*>(c->host_float.data()) : static_cast<void*>(tmp64.data()),
+                              dev[i], len * eb, hipMemcpyDeviceToHost, s));
       CHUI_HIP(hipStreamSynchronize(s));
-    } else {
-      c->host_float.resize(na);
-      for (int i = 0; i < 9; ++i) {
-        if (!hosts[i] || !na) continue;
-        CHUI_HIP(hipMemcpyAsync(c->host_float.data(), dev[i], na * 4, hipMemcpyDeviceToHost, s));
-        CHUI_HIP(hipStreamSynchronize(s));
-        for (uint64_t k = 0; k < na; ++k) hosts[i][k] = c->host_float[k];
+      auto value = [&](uint64_t k) { return mixed ? static_cast<double>(c->host_float[k]) : tmp64[k]; };
+      if (i < 6 || !active) {
+        for (uint64_t k = 0; k < len; ++k) hosts[i][k] = value(k);
+      } else {  // only the active rows of a source-row lane are written
+        for (uint64_t k = 0; k < na; ++k) hosts[i][active[k]] = value(active[k]);
       }
     }
   });

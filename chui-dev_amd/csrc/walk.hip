@@ -658,6 +658,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
   uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t qlen = 0, queued = 0;
   uint32_t nxt = valid ? 0u : P.node_count;
+  double nearest = INFINITY;  // r^2 of the nearest distinct direct-summed source inside r_cut
   double px = 0.0, py = 0.0, pz = 0.0;
   double gx = 0.0, gy = 0.0, gz = 0.0;
   double4 lk_ = make_double4(0.0, 0.0, 0.0, 0.0);  // JVP: lam of the target
@@ -756,6 +757,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
             const double qx = mi64(sj.x - px, L0), qy = mi64(sj.y - py, L1), qz = mi64(sj.z - pz, L2);
             const double r2 = qx * qx + qy * qy + qz * qz;
             if (keep && j != tp && r2 <= P.rcut2) {
+              nearest = fmin(nearest, r2);
               // a coincident pair has a zero force but, in the JVP, the
               // Jacobian F(0) I
               if (!JVP && qx == 0.0 && qy == 0.0 && qz == 0.0) {
@@ -827,11 +829,13 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
     static_cast<double*>(out.pm_y)[slot] = gy;
     static_cast<double*>(out.pm_z)[slot] = gz;
   }
+  const uint32_t row = D.order[tp];
   if (out.sr_x) {
-    static_cast<double*>(out.sr_x)[slot] = ax;
-    static_cast<double*>(out.sr_y)[slot] = ay;
-    static_cast<double*>(out.sr_z)[slot] = az;
+    static_cast<double*>(out.sr_x)[row] = ax;
+    static_cast<double*>(out.sr_y)[row] = ay;
+    static_cast<double*>(out.sr_z)[row] = az;
   }
+  if (out.near2) static_cast<double*>(out.near2)[row] = nearest;
 }
 
 // ---- group walk (FP64) -----------------------------------------------------------
@@ -1033,7 +1037,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
         oy = nearest_image(py - c0y, L1);
         oz = nearest_image(pz - c0z, L2);
       }
-      double ax = 0.0, ay = 0.0, az = 0.0;
+      double ax = 0.0, ay = 0.0, az = 0.0, nearest = INFINITY;
       uint32_t listed = 0, nl = 0, nm = 0, p = 0;
 
       auto evaluate = [&]() {
@@ -1064,6 +1068,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             sy += f * dy;
             sz += f * dz;
             cnt[2] += inside ? 1u : 0u;
+            nearest = inside ? fmin(nearest, r2) : nearest;
           }
           ax += sx;
           ay += sy;
@@ -1169,6 +1174,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
         ax += __shfl_xor(ax, off, kWave);
         ay += __shfl_xor(ay, off, kWave);
         az += __shfl_xor(az, off, kWave);
+        nearest = fmin(nearest, __shfl_xor(nearest, off, kWave));
       }
       if (valid && s == 0) {
         double gx = 0.0, gy = 0.0, gz = 0.0;
@@ -1181,11 +1187,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
           static_cast<double*>(out.pm_y)[slot] = gy;
           static_cast<double*>(out.pm_z)[slot] = gz;
         }
+        const uint32_t row = D.order[tp];
         if (out.sr_x) {
-          static_cast<double*>(out.sr_x)[slot] = ax;
-          static_cast<double*>(out.sr_y)[slot] = ay;
-          static_cast<double*>(out.sr_z)[slot] = az;
+          static_cast<double*>(out.sr_x)[row] = ax;
+          static_cast<double*>(out.sr_y)[row] = ay;
+          static_cast<double*>(out.sr_z)[row] = az;
         }
+        if (out.near2) static_cast<double*>(out.near2)[row] = nearest;
       }
     }
     v = ge;
@@ -1258,6 +1266,7 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
   uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t nxt = valid ? 0u : P.node_count;
+  float nearest = INFINITY;  // counting variant: nearest distinct pair, table units
   uint32_t px = 0, py = 0, pz = 0;
   float gx = 0.0f, gy = 0.0f, gz = 0.0f;
   if (valid) {
@@ -1364,7 +1373,11 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
       // against a finite softened value), so no self test is needed. The
       // counting variant counts coincident sources as the reference does.
       const bool pair = r2 <= lim;
-      if constexpr (COUNT) cnt[2] += pair && (r2 != 0.0f || (c0 + si) != tp);
+      if constexpr (COUNT) {
+        const bool distinct = pair && (r2 != 0.0f || (c0 + si) != tp);
+        cnt[2] += distinct;
+        nearest = distinct ? fminf(nearest, r2) : nearest;
+      }
       const float f = pair ? fr : 0.0f;
       lx = fmaf(f, dx, lx);
       ly = fmaf(f, dy, ly);
@@ -1514,11 +1527,13 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
     static_cast<float*>(out.pm_y)[slot] = gy;
     static_cast<float*>(out.pm_z)[slot] = gz;
   }
+  const uint32_t row = D.order[tp];
   if (out.sr_x) {
-    static_cast<float*>(out.sr_x)[slot] = ax;
-    static_cast<float*>(out.sr_y)[slot] = ay;
-    static_cast<float*>(out.sr_z)[slot] = az;
+    static_cast<float*>(out.sr_x)[row] = ax;
+    static_cast<float*>(out.sr_y)[row] = ay;
+    static_cast<float*>(out.sr_z)[row] = az;
   }
+  if (COUNT && out.near2) static_cast<float*>(out.near2)[row] = nearest * P.tab_to_len2;
 }
 
 // H(t) = L(sqrt t) / t^{3/2} (tree_pm_split_kernel.hpp:109-128).

@@ -51,6 +51,7 @@ struct WalkParams {
   // possible (0.041 r_cut).
   float tab_scale[3];
   float rcut2_tab, eps2_tab, c8_tab, force_scale;
+  float tab_to_len2;   // table-unit r^2 -> length^2 (nearest-pair lane)
   float accept_half_max;
   // Adaptive acceptance (TreePmAcceptancePolicy::kAdaptiveRelative; FP64 walk):
   // estimator, com-distance cutoff policy, Gaussian-table leaf pairs, radial-
@@ -84,6 +85,9 @@ struct GroupParams {
 // geometry, candidate leaves, culled leaves, candidate particles, largest list.
 constexpr int kGroupCounters = 15;
 
+// Totals and PM lanes are per active slot; the short-range lanes and the
+// nearest-pair lane are indexed by source row (TreePmForceAccumulatorView,
+// tree_pm_coupling.hpp:127-165, :1957-1976).
 struct WalkOutputs {
   void* ax;
   void* ay;
@@ -91,9 +95,10 @@ struct WalkOutputs {
   void* pm_x;   // nullable
   void* pm_y;
   void* pm_z;
-  void* sr_x;   // nullable
+  void* sr_x;   // nullable, source-row indexed
   void* sr_y;
   void* sr_z;
+  void* near2 = nullptr;  // nullable, source-row indexed: r^2 of the nearest direct pair in r_cut
 };
 
 // Counters accumulated on the device: visited, pruned, pairs, multipoles,

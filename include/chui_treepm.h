@@ -176,8 +176,11 @@ chui_status chui_treepm_synchronize(chui_treepm_context* ctx);
 /* Host-pointer drop-in for solveActiveSetWithPmCadence. Positions/masses are
  * double[n]; active = uint32[n_active] source rows or NULL for all rows.
  * accel_* = double[n_active] total (PM + short-range residual) accelerations.
- * pm_accel_* (nullable) capture the PM-only part; short_accel_* (nullable) the
- * tree-only residual, both per active slot. refresh_long_range = 0 reuses the
+ * pm_accel_* (nullable) capture the PM-only part per active slot;
+ * short_accel_* (nullable) the tree-only residual, indexed by source row
+ * (double[n]): the active rows are written, the others left untouched, as the
+ * accumulator view's source-indexed lanes (tree_pm_coupling.hpp:149-153,
+ * tree_pm_coupling.cpp:1957-1966). refresh_long_range = 0 reuses the
  * previous PM mesh field (cadence) when the mesh/split/box are unchanged. */
 chui_status chui_treepm_solve(chui_treepm_context* ctx,
                               const double* pos_x, const double* pos_y, const double* pos_z,
@@ -286,16 +289,23 @@ chui_status chui_treepm_adaptive_options_set(chui_treepm_adaptive_options* adapt
  * to the per-target walk. previous_acceleration = double[n_active] magnitudes of
  * the previous total acceleration per active slot
  * (TreePmForceAccumulatorView::previous_acceleration_magnitude_code,
- * tree_pm_coupling.hpp:127-165), or NULL for none. Device arrays as in
- * chui_treepm_solve_device; FP64 precision only. counters may be NULL (no
- * synchronisation then). */
+ * tree_pm_coupling.hpp:127-165), or NULL for none. short_nearest_pair_r2
+ * (nullable, source-row indexed like the short-range lanes) receives the
+ * squared separation of the nearest distinct direct-summed source inside
+ * r_cut, +infinity when none (short_range_nearest_pair_r2_comoving,
+ * tree_pm_coupling.hpp:154-159). Device arrays and element types as in
+ * chui_treepm_solve_device. The adaptive policy is FP64 only; with
+ * acceptance_policy = CHUI_ACCEPT_STRICT this is the strict solve in either
+ * precision plus the nearest-pair lane (mixed: FP32, from the counting walk).
+ * counters may be NULL (no synchronisation then). */
 chui_status chui_treepm_solve_adaptive_device(chui_treepm_context* ctx,
                                               const double* pos_x, const double* pos_y, const double* pos_z,
                                               const double* mass, uint64_t n,
                                               const uint32_t* active, uint64_t n_active,
-                                              double* accel_x, double* accel_y, double* accel_z,
-                                              double* pm_accel_x, double* pm_accel_y, double* pm_accel_z,
-                                              double* short_accel_x, double* short_accel_y, double* short_accel_z,
+                                              void* accel_x, void* accel_y, void* accel_z,
+                                              void* pm_accel_x, void* pm_accel_y, void* pm_accel_z,
+                                              void* short_accel_x, void* short_accel_y, void* short_accel_z,
+                                              void* short_nearest_pair_r2,
                                               const double* previous_acceleration,
                                               const chui_treepm_options* options,
                                               const chui_treepm_adaptive_options* adaptive,

@@ -126,7 +126,7 @@ def reference_gradient(prob, psi64_cpu, target_cpu):
                         o.gravitational_constant, rs, rcut, o.softening_epsilon, prob.factors, prob.vel_factor,
                         prob.loss_mesh)
     (grad,) = torch.autograd.grad(loss, psi)
-    return float(loss), grad
+    return float(loss.detach()), grad
 
 
 def reference_accel(prob, x_cpu):

@@ -40,7 +40,7 @@ def gpu_adaptive(x, y, z, m, o, ad, previous=None, active=None, mesh=MESH):
     na = x.size if active is None else len(active)
     acc = [empty_dev(na, torch.float64) for _ in range(3)]
     pm = [empty_dev(na, torch.float64) for _ in range(3)]
-    sr = [empty_dev(na, torch.float64) for _ in range(3)]
+    sr = [empty_dev(x.size, torch.float64) for _ in range(3)]  # source-row indexed
     act = to_dev(np.asarray(active, np.uint32).view(np.int32))[0] if active is not None else None
     prev = to_dev(np.asarray(previous, np.float64))[0] if previous is not None else None
     with TreePmCoordinator(mesh) as coord:
@@ -48,7 +48,8 @@ def gpu_adaptive(x, y, z, m, o, ad, previous=None, active=None, mesh=MESH):
                                      o, want_diagnostics=True, adaptive_options=ad)
         torch.cuda.synchronize()
     host = lambda ts: [t.cpu().numpy() for t in ts]  # noqa: E731
-    return host(acc), host(pm), host(sr), res.diagnostics
+    rows = np.arange(x.size) if active is None else np.asarray(active)
+    return host(acc), host(pm), [a[rows] for a in host(sr)], res.diagnostics
 
 
 def check_against_oracle(x, y, z, m, o, ad, previous=None, active=None, tol=1e-12):

@@ -42,7 +42,7 @@ def gpu_solve(w_x, w_y, w_z, m, opts, mesh, precision=0, active=None, capture=Tr
     na = len(w_x) if active is None else len(active)
     acc = [empty_dev(na, dt) for _ in range(3)]
     pm = [empty_dev(na, dt) for _ in range(3)] if capture else None
-    sr = [empty_dev(na, dt) for _ in range(3)] if capture else None
+    sr = [empty_dev(len(w_x), dt) for _ in range(3)] if capture else None  # source-row indexed
     act = to_dev(np.asarray(active, np.uint32).view(np.int32))[0] if active is not None else None
     opts.precision = precision
     with TreePmCoordinator(mesh) as coord:
@@ -208,16 +208,18 @@ def test_mixed_waves_spanning_the_box():
 
 def test_active_subset_equals_full_rows():
     # The short-range walk is deterministic per target, so the tree-only lanes
-    # match bit for bit; the PM deposit uses FP64 atomics, so totals agree to
-    # round-off only.
+    # (source-row indexed: only the active rows are written) match bit for
+    # bit; the PM deposit uses FP64 atomics, so totals agree to round-off only.
     w = ics.config(0, n_side=16)
     opts = options_struct(w.box, eps=w.softening)
     full, _, full_sr, _, _ = gpu_solve(w.x, w.y, w.z, w.mass, opts, w.mesh)
     rng = np.random.default_rng(5)
     active = rng.choice(w.x.size, 777, replace=False).astype(np.uint32)
     sub, _, sub_sr, _, _ = gpu_solve(w.x, w.y, w.z, w.mass, opts, w.mesh, active=active)
+    inactive = np.setdiff1d(np.arange(w.x.size), active)
     for a in range(3):
-        assert np.array_equal(sub_sr[a], full_sr[a][active])
+        assert np.array_equal(sub_sr[a][active], full_sr[a][active])
+        assert np.all(sub_sr[a][inactive] == 0.0)  # untouched (the lanes start zeroed)
     assert rel_l2(sub, [f[active] for f in full]) < 1e-13
 
 

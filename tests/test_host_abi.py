@@ -245,6 +245,6 @@ def test_adaptive_entry_point_argument_errors(lib):
     o.acceptance_policy = 1
     ad = L.default_adaptive_options()
     args = [None] * 16
-    st = lib.chui_treepm_solve_adaptive_device(None, *args[:4], 0, None, 0, *args[:9], None, C.byref(o),
+    st = lib.chui_treepm_solve_adaptive_device(None, *args[:4], 0, None, 0, *args[:10], None, C.byref(o),
                                                C.byref(ad), 1, None)
     assert st == L.CHUI_ERR_INVALID_ARGUMENT
