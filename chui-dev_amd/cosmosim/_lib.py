@@ -84,7 +84,7 @@ class Options(C.Structure):
         ("acceptance_policy", C.c_int32),
         ("residual_block_size", C.c_int32),
         ("precision", C.c_int32),
-        ("reserved1", C.c_int32),
+        ("short_range_only", C.c_int32),
     ]
 
 

@@ -391,6 +391,10 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   if ((srx || sry || srz) && !(srx && sry && srz))
     throw std::invalid_argument("TreePM short-range lanes require all three lanes");
   const bool mixed = o->precision == CHUI_PRECISION_MIXED;
+  const bool sr_only = o->short_range_only != 0;
+  // tree_pm_coupling.cpp:2508-2511
+  if (sr_only && (refresh || pmx || pmy || pmz))
+    throw std::invalid_argument("short-range split surface cannot request a PM refresh/capture or zoom correction");
   hipStream_t s = c->stream;
   harvest(c);
   c->events = diag != nullptr || c->profiling;
@@ -447,7 +451,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
                          static_cast<double>(o->assignment_scheme), static_cast<double>(o->window_deconvolution),
                          o->scale_factor};
   const bool key_match = pw.valid && std::equal(key, key + 8, pw.key);
-  if (refresh || !key_match) {
+  if (!sr_only && (refresh || !key_match)) {
     if (!refresh)
       throw std::runtime_error("TreePM long-range field reuse requested without a matching field (refresh required)");
     chui::DepositTiling dt{};
@@ -510,7 +514,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   wp.count = (diag != nullptr || wcount != nullptr || (mixed && near2 != nullptr)) ? 1 : 0;
   wp.uniform_mass = um;
   wp.uniform_mass_f = static_cast<float>(um);
-  wp.pm_enabled = 1;
+  wp.pm_enabled = sr_only ? 0 : 1;
   chui::WalkOutputs out{ax, ay, az, pmx, pmy, pmz, srx, sry, srz, near2};
   uint32_t group_status = ad.group_walk ? CHUI_GROUP_WALK_INCOMPATIBLE_POLICY : CHUI_GROUP_WALK_NOT_REQUESTED;
   const bool group_counts = adaptive_policy || wcount != nullptr;

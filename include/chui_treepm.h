@@ -113,7 +113,10 @@ typedef struct chui_treepm_options {
                                         group-walk target groups; the per-target walks run 64-wide */
   /* this library */
   int32_t precision;                 /* chui_precision */
-  int32_t reserved1;
+  /* TreePmOptions::short_range_only (tree_pm_coupling.hpp:288-291): evaluate
+   * only the short-range residual, with no PM solve, reuse or interpolation
+   * (refresh_long_range must be 0 and the PM lanes NULL) */
+  int32_t short_range_only;
 } chui_treepm_options;
 
 /* Per-solve counters and timings. Counter names follow TreePmTraversalCounters
