@@ -68,6 +68,8 @@ def main():
                     help="KDK steps of the 48-box ensemble (16 x 32^3, 16 x 64^3, 12 x 96^3, 4 x 128^3); 0 skips")
     ap.add_argument("--ensemble-gradients", type=int, default=1,
                     help="1: also time the FP64 gradients of every ensemble box")
+    ap.add_argument("--hierarchical-rungs", type=int, default=3,
+                    help="one hierarchical KDK block with this many rungs on the 128^3 workload (mixed); 0 skips")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="one plain device-to-device copy of this size before the timed region; its HBM counters "
                          "scale the walk's FETCH_SIZE/WRITE_SIZE readings under rocprofv3 --pmc")
@@ -186,6 +188,11 @@ def main():
         grad = measure_gradient(args.grad_n_side, args.grad_directions)
         log(f"grad/fwd {grad}")
 
+    hierarchical = None
+    if args.hierarchical_rungs > 0:
+        hierarchical = measure_hierarchical(args.hierarchical_rungs)
+        log(f"hierarchical {hierarchical}")
+
     ensemble = None
     if args.ensemble_steps > 0:
         ensemble = measure_ensemble(args.ensemble_steps, args.ensemble_gradients != 0)
@@ -228,6 +235,7 @@ def main():
         "grad_fwd_ratio": grad,
         "adaptive_group_walk_fp64": adaptive,
         "ensemble": ensemble,
+        "hierarchical_kdk": hierarchical,
         "stages_ms_mixed": {k: v / max(prof_mixed["solves"], 1) for k, v in prof_mixed.items() if k.endswith("_ms")},
         "roofline": {"bound": "hbm", "kernel": "walkmix_kernel", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(w.name),
@@ -245,6 +253,58 @@ def main():
     }
     coord.close()
     print(json.dumps(line), flush=True)
+
+
+def measure_hierarchical(max_bin, n_side=128):
+    """One hierarchical KDK block (mixed precision) on the 128^3 workload law
+    at a = 0.5 (velocities from its linear field), block span 2^M times the
+    smallest gravity interval: wall time, fine-tick solves and the active
+    fraction, beside the cost of 2^M global steps at the quantum."""
+    import torch
+
+    import cosmosim._lib as L
+    from cosmosim import TreePmCoordinator, ForceAccumulator, hierarchical as H, ics
+
+    w = ics.config(1, n_side=n_side)
+    cosmo = H.Cosmology(ics.H0_CODE, ics.OMEGA_M, 1.0 - ics.OMEGA_M)
+    a0 = w.scale_factor
+    o = L.default_options(box_size=w.box, gravitational_constant=ics.G_CODE, opening_theta=0.5,
+                          rcut_cells=4.5 * 1.25, asmth_cells=1.25, softening_epsilon=w.softening,
+                          max_leaf_size=16, precision=L.PRECISION_MIXED)
+    dev = lambda a: [torch.tensor(c, dtype=torch.float64, device="cuda:0") for c in a]  # noqa: E731
+    x = dev([w.x, w.y, w.z])
+    u = dev([w.vx / a0, w.vy / a0, w.vz / a0])
+    m = torch.tensor(w.mass, device="cuda:0")
+    kdk = H.HierarchicalKdk(w.mesh, x, u, m, o, cosmo, max_bin)
+    try:
+        first = kdk.block(a0)  # warm-up: sizes the buffers and advances the state one block
+        solves0, sr0 = kdk.solves, kdk.short_range_solves
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan = kdk.block(first.scale[-1])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+    finally:
+        kdk.close()
+    counts = np.bincount(plan.bins, minlength=max_bin + 1)
+    periods = 1 << max_bin
+    active_targets = sum(int(sum(counts[b] for b in plan.closing[k])) for k in range(periods - 1))
+    # cost of one full mixed solve for comparison (2^M global steps at the quantum)
+    acc = tuple(torch.empty(w.x.size, dtype=torch.float32, device="cuda:0") for _ in range(3))
+    o.short_range_only = 0
+    with TreePmCoordinator(w.mesh) as coord:
+        coord.solve_active_set(*x, m, ForceAccumulator(acc), o)
+        coord.synchronize()
+        t = time.perf_counter()
+        coord.solve_active_set(*x, m, ForceAccumulator(acc), o)
+        coord.synchronize()
+        full = time.perf_counter() - t
+    return {"workload": w.name, "particles": int(w.x.size), "max_bin": max_bin, "block_seconds": el,
+            "bins": counts.tolist(), "quantum_code_time": plan.quantum,
+            "synchronized_solves": kdk.solves - solves0, "short_range_solves": kdk.short_range_solves - sr0,
+            "fine_tick_targets": active_targets, "full_solve_seconds": full,
+            "global_steps_at_quantum_seconds": periods * full,
+            "speedup_vs_global_at_quantum": periods * full / el}
 
 
 def measure_ensemble(steps, gradients, workers=8):
