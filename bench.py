@@ -286,7 +286,7 @@ def measure_hierarchical(max_bin, n_side=128):
         el = time.perf_counter() - t
     finally:
         kdk.close()
-    counts = np.bincount(plan.bins, minlength=max_bin + 1)
+    counts = np.bincount(plan.bins.cpu().numpy(), minlength=max_bin + 1)
     periods = 1 << max_bin
     active_targets = sum(int(sum(counts[b] for b in plan.closing[k])) for k in range(periods - 1))
     # cost of one full mixed solve for comparison (2^M global steps at the quantum)

@@ -25,6 +25,10 @@ CHUI_ERR_UNSUPPORTED = 7
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 
+TARGETS_AUTO = 0
+TARGETS_PER_LANE = 1
+TARGETS_PER_WAVE = 2
+
 
 class ChuiError(RuntimeError):
     """Base class; .status carries the chui_status code."""
@@ -77,7 +81,7 @@ class Options(C.Structure):
         ("opening_criterion", C.c_int32),
         ("multipole_order", C.c_int32),
         ("max_leaf_size", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("target_mapping", C.c_int32),
         ("softening_epsilon", C.c_double),
         ("relative_force_tolerance", C.c_double),
         ("relative_force_acceleration_floor", C.c_double),

@@ -43,44 +43,60 @@ class Cosmology:
     omega_m: float
     omega_lambda: float
 
-    def hubble(self, a: float) -> float:
+    def hubble(self, a):
         ok = 1.0 - self.omega_m - self.omega_lambda
-        return self.hubble0 * math.sqrt(self.omega_m / a**3 + ok / a**2 + self.omega_lambda)
+        return self.hubble0 * np.sqrt(self.omega_m / a**3 + ok / a**2 + self.omega_lambda)
 
-    def _midpoint(self, f, b: float, e: float, samples: int = 64) -> float:
+    def _midpoint(self, f, b, e, samples: int = 64):
+        """Midpoint rule in a over [b, e] (elementwise over array endpoints);
+        the samples are summed pairwise."""
+        b = np.asarray(b, dtype=np.float64)
+        e = np.asarray(e, dtype=np.float64)
         da = (e - b) / samples
-        return sum(f(b + (i + 0.5) * da) for i in range(samples)) * da
+        a = b[..., None] + (np.arange(samples) + 0.5) * da[..., None]
+        r = f(a).sum(axis=-1) * da
+        return float(r) if r.ndim == 0 else r
 
-    def cosmic_time(self, b: float, e: float, samples: int = 64) -> float:
+    def cosmic_time(self, b, e, samples: int = 64):
         """int dt = int da / (a H), midpoint rule in a."""
         return self._midpoint(lambda a: 1.0 / (a * self.hubble(a)), b, e, samples)
 
-    def drift_like(self, b: float, e: float) -> float:
+    def drift_like(self, b, e):
         """int dt / a = int da / (a^2 H) (computeComovingDriftFactor)."""
         return self._midpoint(lambda a: 1.0 / (a * a * self.hubble(a)), b, e)
 
-    def position_drift(self, b: float, e: float) -> float:
+    def position_drift(self, b, e):
         """int dt / a^2 = int da / (a^3 H)."""
         return self._midpoint(lambda a: 1.0 / (a * a * a * self.hubble(a)), b, e)
 
-    def advance(self, a: float, dt: float, samples: int = 64) -> float:
-        """Scale factor a cosmic time dt after a: bracket by doubling from
-        1.001 a, then 96 bisections (advanceScaleFactorByCosmicTime)."""
-        if dt == 0.0:
-            return a
-        lo, hi = a, a * 1.001
+    def advance_many(self, a: float, dts, samples: int = 64) -> np.ndarray:
+        """Scale factors cosmic times dts after a, each bracketed by doubling
+        from 1.001 a and bisected 96 times (advanceScaleFactorByCosmicTime),
+        all at once. Bisection stops early once no bracket changes any more:
+        when mid rounds to lo or hi the bracket is a fixed point, so the
+        result equals the 96-step one."""
+        dts = np.asarray(dts, dtype=np.float64)
+        base = np.full(dts.shape, float(a))
+        hi = base * 1.001
         for _ in range(256):
-            if self.cosmic_time(a, hi, samples) >= dt:
+            short = self.cosmic_time(base, hi, samples) < dts
+            if not short.any():
                 break
-            lo, hi = hi, hi * 2.0
-        lo = a
+            hi = np.where(short, hi * 2.0, hi)
+        lo = base.copy()
         for _ in range(96):
             mid = 0.5 * (lo + hi)
-            if self.cosmic_time(a, mid, samples) < dt:
-                lo = mid
-            else:
-                hi = mid
-        return 0.5 * (lo + hi)
+            below = self.cosmic_time(base, mid, samples) < dts
+            nlo = np.where(below, mid, lo)
+            nhi = np.where(below, hi, mid)
+            if np.array_equal(nlo, lo) and np.array_equal(nhi, hi):
+                break
+            lo, hi = nlo, nhi
+        return np.where(dts == 0.0, base, 0.5 * (lo + hi))
+
+    def advance(self, a: float, dt: float, samples: int = 64) -> float:
+        """Scale factor a cosmic time dt after a (advance_many of one)."""
+        return float(self.advance_many(a, [dt], samples)[0])
 
 
 @dataclass
@@ -90,7 +106,7 @@ class BlockPlan:
     max_bin: int
     quantum: float
     scale: list
-    bins: np.ndarray
+    bins: object  # int64 bin per particle: numpy array, or a device tensor when planned from one
     opening: list = field(default_factory=list)   # per tick: {bin: factor}
     closing: list = field(default_factory=list)   # per tick: {bin: factor}
     drift: list = field(default_factory=list)     # per tick: a_k int dt/a^2
@@ -99,18 +115,24 @@ class BlockPlan:
     pm_close: float = 0.0
 
 
-def plan_block(cosmo: Cosmology, a_begin: float, accel_norm: np.ndarray, softening: float, max_bin: int,
-               eta: float = 0.2, max_span: float | None = None, bins: np.ndarray | None = None) -> BlockPlan:
+def plan_block(cosmo: Cosmology, a_begin: float, accel_norm, softening: float, max_bin: int,
+               eta: float = 0.2, max_span: float | None = None, bins=None) -> BlockPlan:
     """Bins and factors of one block (time_integration.cpp:1356-1600;
     docs/time_integration.md:392-470). accel_norm = |A| per particle at the
-    block start; bins, when given, replaces the criterion's assignment."""
+    block start, a numpy array or a device tensor (the bins are then computed
+    on the device and plan.bins is a device int64 tensor); bins, when given,
+    replaces the criterion's assignment."""
     if not 0 <= max_bin <= 12:
         raise ValueError("hierarchical_max_rung must be in [0, 12]")
     a = a_begin
-    with np.errstate(divide="ignore"):
-        dt = eta * np.sqrt(softening / (np.asarray(accel_norm, dtype=np.float64) / a**3))
-    dt_min = float(np.min(dt)) if dt.size else math.inf
     periods = 1 << max_bin
+    if torch.is_tensor(accel_norm):
+        dt = eta * torch.sqrt(softening / (accel_norm.double() / a**3))
+        dt_min = float(dt.min()) if dt.numel() else math.inf
+    else:
+        with np.errstate(divide="ignore"):
+            dt = eta * np.sqrt(softening / (np.asarray(accel_norm, dtype=np.float64) / a**3))
+        dt_min = float(np.min(dt)) if dt.size else math.inf
     span = periods * dt_min
     if max_span is not None:
         span = min(span, max_span)
@@ -118,10 +140,15 @@ def plan_block(cosmo: Cosmology, a_begin: float, accel_norm: np.ndarray, softeni
         raise ValueError("hierarchical block span is not finite positive (no force or no explicit span)")
     quantum = span / periods
     if bins is None:
-        with np.errstate(divide="ignore"):
-            b = np.floor(np.log2(dt / quantum))
-        bins = np.clip(np.nan_to_num(b, nan=0.0, posinf=max_bin, neginf=0.0), 0, max_bin).astype(np.int64)
-    scale = [a_begin] + [cosmo.advance(a_begin, 0.5 * h * quantum) for h in range(1, 2 * periods + 1)]
+        if torch.is_tensor(dt):
+            b = torch.floor(torch.log2(dt / quantum))
+            bins = torch.clip(torch.nan_to_num(b, nan=0.0, posinf=max_bin, neginf=0.0), 0, max_bin).to(torch.int64)
+        else:
+            with np.errstate(divide="ignore"):
+                b = np.floor(np.log2(dt / quantum))
+            bins = np.clip(np.nan_to_num(b, nan=0.0, posinf=max_bin, neginf=0.0), 0, max_bin).astype(np.int64)
+    halves = cosmo.advance_many(a_begin, 0.5 * quantum * np.arange(1, 2 * periods + 1))
+    scale = [a_begin] + [float(v) for v in halves]
     plan = BlockPlan(max_bin, quantum, scale, bins)
 
     def kick(begin: int, end: int, a_now: float) -> float:
@@ -193,13 +220,16 @@ class HierarchicalKdk:
             mask |= bins_dev == b
         return torch.nonzero(mask).reshape(-1).to(torch.int32)
 
-    def block(self, a_begin: float, max_span: float | None = None, bins: np.ndarray | None = None) -> BlockPlan:
+    def block(self, a_begin: float, max_span: float | None = None, bins=None) -> BlockPlan:
+        """One block from a_begin; the returned plan's bins are a device tensor."""
         # synchronized Tree+PM solve at the block start (the mandatory bootstrap)
         acc, pm, sr = self._solve(None, True)
-        norm = torch.sqrt(acc[0].double() ** 2 + acc[1].double() ** 2 + acc[2].double() ** 2).cpu().numpy()
+        norm = torch.sqrt(acc[0].double() ** 2 + acc[1].double() ** 2 + acc[2].double() ** 2)
+        if bins is not None and not torch.is_tensor(bins):
+            bins = torch.from_numpy(np.asarray(bins, dtype=np.int64)).to(self.device)
         plan = plan_block(self.cosmo, a_begin, norm, self.options.softening_epsilon, self.max_bin, self.eta, max_span,
                           bins)
-        bins = torch.from_numpy(plan.bins).to(self.device)
+        bins = plan.bins
         tree = sr
         periods = 1 << plan.max_bin
         all_rows = None

@@ -583,6 +583,39 @@ __device__ __forceinline__ void node_accel32(const NodeMom32& M, const NodeQuad3
   az = cz;
 }
 
+// Node-test units of the mixed walk: per-axis scale of fixed-point
+// differences (1 for a cubic box, whose tests run in fixed-point units),
+// separation unit -> length and back, the containment radius^2 and theta^2.
+struct MixFrame {
+  float s0, s1, s2, to_len, inv_len, contain2, theta2;
+};
+
+// Strict acceptance of internal node n (tree_pm_acceptance.hpp:54-66: the
+// node's cell inside r_cut, the opening criterion and 2h/r < 0.08 with
+// quadrupoles) for the target at fixed point (px, py, pz), with the COM
+// separation in lengths left in mf* for the multipole evaluation.
+__device__ __forceinline__ bool strict_accept32(const WalkParams& P, const MixFrame& F, const NodeGeo32& g,
+                                                const NodeMom32& M, const NodeQuad32* __restrict__ node_quad,
+                                                uint32_t n, uint32_t px, uint32_t py, uint32_t pz, float& mfx,
+                                                float& mfy, float& mfz) {
+  const float h = g.half * F.inv_len;
+  const float ax_ = fabsf(fx_delta(g.cx, px, F.s0)), ay_ = fabsf(fx_delta(g.cy, py, F.s1)),
+              az_ = fabsf(fx_delta(g.cz, pz, F.s2));
+  const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+  const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+  const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= F.contain2;
+  mfx = fx_delta(M.comx, px, F.s0) * F.to_len;
+  mfy = fx_delta(M.comy, py, F.s1) * F.to_len;
+  mfz = fx_delta(M.comz, pz, F.s2) * F.to_len;
+  const float r2 = mfx * mfx + mfy * mfy + mfz * mfz;
+  const float w2 = 2.0f * g.half;
+  // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
+  const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
+  const float lead = P.criterion == 0 ? w2 : w2 + node_quad[n].offset;
+  const bool mac = !inside && lead * lead < F.theta2 * (r2 + 1.0e-30f);
+  return mac && env_ok && contained;
+}
+
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   const float y = v - c;
   const float t = s + y;
@@ -1392,24 +1425,9 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   // this lane, with the COM separation it needs left in mf*. Nodes with
   // h >= 0.041 r_cut are never accepted (2h/r < 0.08 with r inside r_cut), so
   // the caller skips the test for them.
+  const MixFrame F{s0, s1, s2, to_len, inv_len, contain2, theta2};
   auto accepts = [&](uint32_t n, const NodeGeo32& g, float& mfx, float& mfy, float& mfz) -> bool {
-    const float h = g.half * inv_len;
-    const float ax_ = fabsf(fx_delta(g.cx, px, s0)), ay_ = fabsf(fx_delta(g.cy, py, s1)),
-                az_ = fabsf(fx_delta(g.cz, pz, s2));
-    const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
-    const float mx = ax_ + h, my = ay_ + h, mz = az_ + h;
-    const bool contained = fmaf(mx, mx, fmaf(my, my, mz * mz)) <= contain2;
-    const NodeMom32 M = node_mom[n];
-    mfx = fx_delta(M.comx, px, s0) * to_len;
-    mfy = fx_delta(M.comy, py, s1) * to_len;
-    mfz = fx_delta(M.comz, pz, s2) * to_len;
-    const float r2 = mfx * mfx + mfy * mfy + mfz * mfz;
-    const float w2 = 2.0f * g.half;
-    // 2h/r < 0.08 and (2h [+ |c - COM|]) / r < theta, squared (no sqrt/div)
-    const bool env_ok = P.quad && w2 * w2 < 0.0064f * fmaxf(r2, 1.0e-30f);
-    const float lead = P.criterion == 0 ? w2 : w2 + node_quad[n].offset;
-    const bool mac = !inside && lead * lead < theta2 * (r2 + 1.0e-30f);
-    return mac && env_ok && contained;
+    return strict_accept32(P, F, g, node_mom[n], node_quad, n, px, py, pz, mfx, mfy, mfz);
   };
 
   // The evaluating walk prunes against each node's tight particle box. Every
@@ -1536,6 +1554,166 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   if (COUNT && out.near2) static_cast<float*>(out.near2)[row] = nearest * P.tab_to_len2;
 }
 
+// ---- mixed walk, one target per wave --------------------------------------------
+// For sparse active sets the 64 targets of a walkmix_kernel wave lie far
+// apart, and the wave walks the union of 64 distant neighbourhoods. Here a
+// wave serves one target. Its lanes classify 64 consecutive preorder nodes at
+// a time with walkmix_kernel's tests (the cutoff prune against the tight
+// particle boxes, strict_accept32 for internal nodes whose cell is small
+// enough), follow the visited chain through that window, list the kept
+// leaves' sources and the accepted nodes in LDS and evaluate the lists 64
+// entries at a time with walkmix_kernel's pair and multipole arithmetic. The
+// pairs and multipoles are walkmix_kernel's; per-lane partial sums are
+// reduced in FP64 at the end.
+constexpr int kTargetList = 1024;  // listed sources per wave before an evaluation
+constexpr int kTargetNodes = 128;  // listed multipoles per wave before an evaluation
+
+template <int SCHEME, bool CUBIC>
+__global__ __launch_bounds__(kWalkThreads) void walkmix_target_kernel(WalkParams P, WalkData D, WalkOutputs out) {
+  __shared__ float4 htab[kHTableCells];
+  __shared__ uint32_t list[kWalkWaves][kTargetList];
+  __shared__ uint32_t mps[kWalkWaves][kTargetNodes];
+  for (int i = threadIdx.x; i < kHTableCells; i += kWalkThreads) htab[i] = D.htab[i];
+  __syncthreads();
+  const int w = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const uint32_t k = xcd_logical_block() * kWalkWaves + w;
+  if (k >= P.n_targets) return;  // wave-uniform; no block barrier follows
+  const uint32_t tp = P.identity_targets ? k : D.tpos[k];
+  const uint32_t slot = P.identity_targets ? D.order[k] : D.tslot[k];
+  const uint4 t = D.src32[tp];
+  const uint32_t px = t.x, py = t.y, pz = t.z;
+  const float s0 = CUBIC ? 1.0f : P.fx_to_len[0];
+  const float s1 = CUBIC ? 1.0f : P.fx_to_len[1];
+  const float s2 = CUBIC ? 1.0f : P.fx_to_len[2];
+  const float to_len = CUBIC ? P.fx_to_len[0] : 1.0f;
+  const float inv_len = CUBIC ? 1.0f / P.fx_to_len[0] : 1.0f;
+  const MixFrame F{s0, s1, s2, to_len, inv_len, P.contain2_sep, static_cast<float>(P.theta * P.theta)};
+  const float prune2 = P.prune2_sep;
+  const float u0 = P.tab_scale[0], u1 = P.tab_scale[1], u2 = P.tab_scale[2];
+  const float rc2 = P.rcut2_tab, eps2 = P.eps2_tab, c8 = P.c8_tab;
+  const unsigned long long below = (1ull << lane) - 1ull;
+
+  float lx = 0.0f, ly = 0.0f, lz = 0.0f;  // pair sums (force-scale units)
+  float mx = 0.0f, my = 0.0f, mz = 0.0f;  // multipole accelerations
+  uint32_t nl = 0, nm = 0;
+  auto evaluate = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t e = lane; e < nl; e += kWave) {
+      const uint4 sv = D.src32[list[w][e]];
+      const float dx = fx_delta(sv.x, px, u0), dy = fx_delta(sv.y, py, u1), dz = fx_delta(sv.z, pz, u2);
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));  // = H-table coordinate
+      const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
+      const float soft = rinv * rinv * rinv;
+      const int ci = static_cast<int>(fminf(r2, static_cast<float>(kHTableCells - 1)));
+      const float fu = __builtin_amdgcn_fractf(r2);
+      const float4 c = htab[ci];
+      const float hcoef = fmaf(fmaf(fmaf(c.w, fu, c.z), fu, c.y), fu, c.x);
+      float fr = fmaf(-hcoef, c8, soft);
+      if (!D.uniform_mass) fr *= __uint_as_float(sv.w);
+      const float f = r2 <= rc2 ? fr : 0.0f;  // the target itself: dx = dy = dz = 0
+      lx = fmaf(f, dx, lx);
+      ly = fmaf(f, dy, ly);
+      lz = fmaf(f, dz, lz);
+    }
+    for (uint32_t e = lane; e < nm; e += kWave) {
+      const uint32_t n = mps[w][e];
+      const NodeMom32 M = D.mom32[n];
+      const float dx = fx_delta(M.comx, px, s0) * to_len, dy = fx_delta(M.comy, py, s1) * to_len,
+                  dz = fx_delta(M.comz, pz, s2) * to_len;
+      float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+      node_accel32(M, D.quad32[n], dx, dy, dz, P, cx, cy, cz);
+      mx += cx;
+      my += cy;
+      mz += cz;
+    }
+    __builtin_amdgcn_wave_barrier();
+    nl = 0;
+    nm = 0;
+  };
+
+  uint32_t p = 0;
+  while (p < P.node_count) {
+    // classify preorder nodes p .. p + 63 for this target
+    const uint32_t q = p + lane;
+    uint32_t next = q + 1, lb = 0, lc = 0;
+    int kind = 3;  // 0 pruned, 1 kept leaf, 2 accepted node, 3 opened
+    if (q < P.node_count) {
+      const NodeLink lk = D.link[q];
+      const NodeBox32 b = D.box32[q];
+      const float ax_ = fabsf(fx_delta(b.cx, px, s0)), ay_ = fabsf(fx_delta(b.cy, py, s1)),
+                  az_ = fabsf(fx_delta(b.cz, pz, s2));
+      const float ex = fmaxf(0.0f, ax_ - b.hx), ey = fmaxf(0.0f, ay_ - b.hy), ez = fmaxf(0.0f, az_ - b.hz);
+      const bool keep = !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
+      if (lk.child_count == 0) {
+        kind = keep ? 1 : 0;
+        lb = lk.begin;
+        lc = lk.count;
+      } else if (!keep) {
+        kind = 0;
+        next = q + lk.skip;
+      } else {
+        bool take = false;
+        if (b.cell_half < P.accept_half_max) {
+          float mfx, mfy, mfz;
+          take = strict_accept32(P, F, D.geo32[q], D.mom32[q], D.quad32, q, px, py, pz, mfx, mfy, mfz);
+        }
+        kind = take ? 2 : 3;
+        next = take ? q + lk.skip : q + 1;
+      }
+    }
+    // the visited chain through the window
+    unsigned long long chain = 0;
+    uint32_t cur = p;
+    while (cur < p + kWave && cur < P.node_count) {
+      const uint32_t l = cur - p;
+      chain |= 1ull << l;
+      cur = readlane_u32(next, l);
+    }
+    p = cur;
+    const bool on = (chain >> lane) & 1ull;
+    const unsigned long long mm = __ballot(on && kind == 2);
+    if (nm + static_cast<uint32_t>(__popcll(mm)) > static_cast<uint32_t>(kTargetNodes)) evaluate();
+    if (on && kind == 2) mps[w][nm + __popcll(mm & below)] = q;
+    nm += __popcll(mm);
+    unsigned long long rest = __ballot(on && kind == 1);
+    while (rest) {
+      const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(rest)) - 1);
+      rest &= rest - 1ull;
+      const uint32_t b0 = readlane_u32(lb, l), c = readlane_u32(lc, l);
+      for (uint32_t c0 = 0; c0 < c; c0 += kWave) {
+        const uint32_t cn = c - c0 < static_cast<uint32_t>(kWave) ? c - c0 : static_cast<uint32_t>(kWave);
+        if (nl + cn > static_cast<uint32_t>(kTargetList)) evaluate();
+        if (static_cast<uint32_t>(lane) < cn) list[w][nl + lane] = b0 + c0 + lane;
+        nl += cn;
+      }
+    }
+  }
+  evaluate();
+  const double gm = P.force_scale;
+  const double sx = gm * wave_sum(static_cast<double>(lx)) + wave_sum(static_cast<double>(mx));
+  const double sy = gm * wave_sum(static_cast<double>(ly)) + wave_sum(static_cast<double>(my));
+  const double sz = gm * wave_sum(static_cast<double>(lz)) + wave_sum(static_cast<double>(mz));
+  if (lane != 0) return;
+  const float ax = static_cast<float>(sx), ay = static_cast<float>(sy), az = static_cast<float>(sz);
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  if (P.pm_enabled) interp32<SCHEME>(P, D.f32x, D.f32y, D.f32z, px, py, pz, gx, gy, gz);
+  static_cast<float*>(out.ax)[slot] = gx + ax;
+  static_cast<float*>(out.ay)[slot] = gy + ay;
+  static_cast<float*>(out.az)[slot] = gz + az;
+  if (out.pm_x) {
+    static_cast<float*>(out.pm_x)[slot] = gx;
+    static_cast<float*>(out.pm_y)[slot] = gy;
+    static_cast<float*>(out.pm_z)[slot] = gz;
+  }
+  const uint32_t row = D.order[tp];
+  if (out.sr_x) {
+    static_cast<float*>(out.sr_x)[row] = ax;
+    static_cast<float*>(out.sr_y)[row] = ay;
+    static_cast<float*>(out.sr_z)[row] = az;
+  }
+}
+
 // H(t) = L(sqrt t) / t^{3/2} (tree_pm_split_kernel.hpp:109-128).
 double gaussian_h_exact(double t) {
   if (t < 0.015625) {
@@ -1573,6 +1751,8 @@ void launchmix(const WalkParams& p, const WalkData& d, const WalkOutputs& out, u
   if (p.count)
     walkmix_kernel<SCHEME, CUBIC, true><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
         p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
+  else if (p.target_per_wave)
+    walkmix_target_kernel<SCHEME, CUBIC><<<grid_for(p.n_targets, kWalkWaves), kWalkThreads, 0, s>>>(p, d, out);
   else
     walkmix_kernel<SCHEME, CUBIC, false><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
         p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);

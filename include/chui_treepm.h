@@ -85,6 +85,16 @@ typedef enum chui_acceptance_policy {
   CHUI_ACCEPT_ADAPTIVE = 1  /* kAdaptiveRelative */
 } chui_acceptance_policy;
 
+/* How the mixed-precision strict walk maps targets onto 64-wide waves (this
+ * library; no reference counterpart: the reference walks targets one by one).
+ * The forces are the same pairs and multipoles either way, summed in another
+ * order. Other walks always run one target per lane. */
+typedef enum chui_target_mapping {
+  CHUI_TARGETS_AUTO = 0,      /* per wave when the active set is sparse, else per lane */
+  CHUI_TARGETS_PER_LANE = 1,  /* 64 visit-ordered targets per wave sharing one walk */
+  CHUI_TARGETS_PER_WAVE = 2   /* one target per wave: lanes split its nodes and pairs */
+} chui_target_mapping;
+
 /* Flat options: the fields of TreePmOptions that change a force evaluation.
  * Defaults (chui_treepm_default_options) equal the reference's struct defaults. */
 typedef struct chui_treepm_options {
@@ -103,7 +113,7 @@ typedef struct chui_treepm_options {
   int32_t opening_criterion;         /* chui_opening_criterion */
   int32_t multipole_order;           /* 0 monopole, 2 quadrupole */
   int32_t max_leaf_size;
-  int32_t reserved0;
+  int32_t target_mapping;            /* chui_target_mapping (this library) */
   double softening_epsilon;          /* Plummer epsilon, comoving */
   double relative_force_tolerance;
   double relative_force_acceleration_floor;

@@ -42,7 +42,7 @@ def test_block_matches_cpu_restatement(n_side, max_bin, precision):
         plan = kdk.block(0.04, max_span=2e-3, bins=None if precision == L.PRECISION_FP64 else plan_r.bins)
         xg = np.stack([c.cpu().numpy() for c in kdk.x])
         ug = np.stack([c.cpu().numpy() for c in kdk.u])
-        assert np.array_equal(plan.bins, plan_r.bins)
+        assert np.array_equal(plan.bins.cpu().numpy(), plan_r.bins)
         assert kdk.short_range_solves == n_sr and kdk.solves == 2
     finally:
         kdk.close()

@@ -120,6 +120,9 @@ MESH = (C.c_int32 * 3)(64, 64, 64)
     ("multipole_order", 1, L.CHUI_ERR_INVALID_ARGUMENT),
     ("acceptance_policy", 1, L.CHUI_OK),                        # adaptive (FP64)
     ("acceptance_policy", 2, L.CHUI_ERR_INVALID_ARGUMENT),
+    ("target_mapping", L.TARGETS_PER_WAVE, L.CHUI_OK),
+    ("target_mapping", 3, L.CHUI_ERR_INVALID_ARGUMENT),
+    ("target_mapping", -1, L.CHUI_ERR_INVALID_ARGUMENT),
     ("struct_size", 8, L.CHUI_ERR_INVALID_ARGUMENT),
 ])
 def test_options_validate(lib, field, value, status):
