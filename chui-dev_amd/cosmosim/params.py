@@ -212,10 +212,13 @@ def _box_mpc(entries, length_unit: str) -> tuple:
     return (v, v, v)
 
 
-def treepm_deck(source, precision: int = _lib.PRECISION_FP64) -> TreePmDeck:
-    """Options of the TreePM force path from a deck (path or text)."""
-    text = Path(source).read_text() if isinstance(source, Path) or (
-        isinstance(source, str) and "\n" not in source and Path(source).is_file()) else str(source)
+def load_treepm_deck(path, precision: int = _lib.PRECISION_FP64) -> TreePmDeck:
+    """Options of the TreePM force path from a deck file."""
+    return treepm_deck(Path(path).read_text(), precision)
+
+
+def treepm_deck(text: str, precision: int = _lib.PRECISION_FP64) -> TreePmDeck:
+    """Options of the TreePM force path from the text of a deck."""
     entries = parse_entries(text)
     units = {u: entries.get(f"units.{u}_unit", (d, 0))[0]
              for u, d in (("length", "mpc"), ("mass", "msun"), ("velocity", "km_s"))}
@@ -259,5 +262,5 @@ def treepm_deck(source, precision: int = _lib.PRECISION_FP64) -> TreePmDeck:
     return deck
 
 
-__all__ = ["ParamError", "TreePmDeck", "gravitational_constant_code", "length_mpc", "parse_entries",
-           "treepm_deck"]
+__all__ = ["ParamError", "TreePmDeck", "gravitational_constant_code", "length_mpc", "load_treepm_deck",
+           "parse_entries", "treepm_deck"]

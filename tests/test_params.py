@@ -111,6 +111,13 @@ def test_fast_deck_fills_every_field():
     assert lib.chui_treepm_options_validate(C.byref(o), mesh) == L.CHUI_OK
 
 
+def test_deck_file(tmp_path):
+    f = tmp_path / "fast.param.txt"
+    f.write_text(FAST)
+    d = P.load_treepm_deck(f)
+    assert d.mesh == (128, 128, 128) and d.adaptive.group_walk == 1
+
+
 def test_empty_deck_takes_the_registry_defaults():
     d = P.treepm_deck("[numerics]\n", precision=L.PRECISION_MIXED)
     o = d.options
