@@ -54,7 +54,7 @@ struct chui_treepm_context {
   DeviceBuffer counters, red_scratch, ewald_scratch;
   // adaptive policy: Gaussian Hermite knots, leaf of each tree position,
   // group-walk fallback targets and counters, source bounds
-  DeviceBuffer gtab, leaf_of_pos, fb_pos, fb_slot, gcounters, bounds;
+  DeviceBuffer gtab, gcoef, leaf_of_pos, fb_pos, fb_slot, gcounters, bounds;
   hipEvent_t ev[kEvCount] = {};
   bool events = false;
   // profiling mode: events recorded every solve, harvested at the next solve's
@@ -548,8 +548,11 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     wp.inv8a3 = ((0.125 / sp.a) / sp.a) / sp.a;
     if (!c->gtab.ptr) {
       const std::vector<double> knots = chui::gaussian_hermite_knots();
+      const std::vector<double> mono = chui::gaussian_hermite_monomials(knots);
       c->gtab.reserve(knots.size() * sizeof(double));
+      c->gcoef.reserve(mono.size() * sizeof(double));
       CHUI_HIP(hipMemcpyAsync(c->gtab.ptr, knots.data(), knots.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      CHUI_HIP(hipMemcpyAsync(c->gcoef.ptr, mono.data(), mono.size() * sizeof(double), hipMemcpyHostToDevice, s));
       CHUI_HIP(hipStreamSynchronize(s));
     }
     const double t_cut = (0.25 * wp.rcut2 / sp.a) / sp.a;
@@ -599,7 +602,8 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
         gp.hi[a] = decode(hi);
       }
       auto* fb_count = reinterpret_cast<uint32_t*>(bkeys + 6);  // zeroed above
-      chui::walk_fp64_group(c->tree, wp, gp, c->src64.as<double4>(), uniform ? 1 : 0, gtab, prev,
+      chui::walk_fp64_group(c->tree, wp, gp, c->src64.as<double4>(), uniform ? 1 : 0, gtab,
+                            c->gcoef.as<double4>(), prev,
                             c->leaf_of_pos.as<uint32_t>(), target_pos, target_slot, f0, f1, f2, out,
                             c->fb_pos.as<uint32_t>(), c->fb_slot.as<uint32_t>(), fb_count,
                             reinterpret_cast<unsigned long long*>(c->gcounters.ptr), s);

@@ -662,6 +662,8 @@ struct WalkData {
   // adaptive policy: Gaussian Hermite knots and |a_prev| per output slot (nullable)
   const double2* gtab;
   const double* prev;
+  // group walk: the same Gaussian cells as monomial coefficients (a0..a3)
+  const double4* gcoef;
 };
 
 __device__ __forceinline__ uint32_t next_node(uint32_t n, uint32_t nxt) {
@@ -888,13 +890,17 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
 //   3. whenever the buffer fills (and at the end) every target evaluates the
 //      buffered list (accumulateTreePmGroupCulledLeaves, :290-341, and the
 //      accepted-node expansion): a group of T targets gives each target S lanes
-//      (the largest power of two with S T <= 64), each lane taking every S-th
-//      listed leaf and every S-th accepted node; the lanes' sums are added at
-//      the end.
+//      (the largest power of two with S T <= 64). The listed leaves' sources
+//      are staged in LDS 64 at a time, each lane taking every S-th staged
+//      source and every S-th accepted node; the lanes' sums are added at the
+//      end. The per-target leaf cull is evaluated for its counters only: a
+//      culled leaf lies beyond r_cut of the target, so its pairs are zero, and
+//      evaluating every listed source keeps all S lanes of a target busy.
 // Groups above 64 targets are evaluated 64 targets at a time, repeating the
 // (identical) tree walk for each.
 constexpr int kGroupLeafCap = 192;
 constexpr int kGroupNodeCap = 128;
+constexpr int kGroupStage = 64;  // listed sources staged in LDS per evaluation pass
 
 struct GroupLeaf {
   double ox, oy, oz, h;  // cell centre relative to the group centre, cell half size
@@ -902,6 +908,11 @@ struct GroupLeaf {
 };
 
 __device__ __forceinline__ double nearest_image(double d, double L) { return d - L * rint(d / L); }
+// The same with 1/L precomputed: rint(d * iL) equals rint(d / L) unless d / L
+// lies within an ulp of a half-integer, where both images are equally near.
+__device__ __forceinline__ double nearest_image_inv(double d, double L, double iL) {
+  return d - L * rint(d * iL);
+}
 __device__ __forceinline__ double wrap_half(double d, double L) {
   const double hl = 0.5 * L;
   return d > hl ? d - L : (d < -hl ? d + L : d);
@@ -932,12 +943,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                                                               unsigned long long* __restrict__ counters) {
   __shared__ GroupLeaf leaves[kGroupLeafCap];
   __shared__ uint32_t nodes[kGroupNodeCap];
+  __shared__ uint32_t leaf_pre[kGroupLeafCap + 1];
+  __shared__ double4 stage[kGroupStage];
+  __shared__ uint32_t stage_j[kGroupStage];
   const int lane = threadIdx.x;
   const uint32_t bb = blockIdx.x * G.block;
   const uint32_t be = min(bb + G.block, P.n_targets);
   const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
   const double cull2 = P.rcut2 * (1.0 + 1.0e-9);
   const double tlim = 0x1.fffffffffffffp+5;  // largest double below 64
+  const double iL0 = 1.0 / P.L[0], iL1 = 1.0 / P.L[1], iL2 = 1.0 / P.L[2];
   const double eps2 = P.eps * P.eps;
   uint32_t cnt[kGroupCounters] = {};
   unsigned long long size_max = 0, entries_max = 0;
@@ -1075,6 +1090,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
 
       auto evaluate = [&]() {
         __syncthreads();
+        // candidate and culled leaves per target (the counters of
+        // accumulateTreePmGroupCulledLeaves): a culled leaf lies beyond r_cut
+        // of the target, so its sources add nothing to the pair sums below
         for (uint32_t e = s; e < nl; e += S) {
           if (!valid) continue;
           const GroupLeaf Lf = leaves[e];
@@ -1085,27 +1103,71 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
           ++cnt[11];
           if (cull) ++cnt[12];
           else cnt[13] += Lf.e - Lf.b;
-          if (cull) continue;
-          double sx = 0.0, sy = 0.0, sz = 0.0;
-          for (uint32_t j = Lf.b; j < Lf.e; ++j) {
-            const double4 sj = D.src64[j];
-            const double dx = wrap_half(sj.x - px, L0), dy = wrap_half(sj.y - py, L1), dz = wrap_half(sj.z - pz, L2);
+        }
+        // exclusive prefix of the listed leaves' source counts
+        uint32_t total = 0;
+        for (uint32_t c = 0; c < nl; c += kWave) {
+          const uint32_t e = c + lane;
+          const uint32_t own = e < nl ? leaves[e].e - leaves[e].b : 0u;
+          uint32_t incl = own;
+          for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += o;
+          }
+          if (e < nl) leaf_pre[e] = total + incl - own;
+          total += readlane_u32(incl, kWave - 1);
+        }
+        if (lane == 0) leaf_pre[nl] = total;
+        // the listed sources, kGroupStage at a time: staged once in LDS and
+        // evaluated by each target's S lanes (lane s takes every S-th source)
+        for (uint32_t g0 = 0; g0 < total; g0 += kGroupStage) {
+          const uint32_t gn = min(total - g0, static_cast<uint32_t>(kGroupStage));
+          __syncthreads();
+          if (static_cast<uint32_t>(lane) < gn) {
+            const uint32_t g = g0 + lane;
+            uint32_t lo = 0, hi = nl;  // the leaf e with leaf_pre[e] <= g < leaf_pre[e + 1]
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (leaf_pre[mid] <= g) lo = mid;
+              else hi = mid;
+            }
+            const uint32_t j = leaves[lo].b + (g - leaf_pre[lo]);
+            stage[lane] = D.src64[j];
+            stage_j[lane] = j;
+          }
+          __syncthreads();
+          if (!valid) continue;
+          for (uint32_t q = s; q < gn; q += S) {
+            const double4 sj = stage[q];
+            const uint32_t j = stage_j[q];
+            // minimum image by rounding: the same separations as wrap_half
+            // wherever |d| <= L/2 or d - L is exact (every pair inside r_cut)
+            const double ex = sj.x - px, ey = sj.y - py, ez = sj.z - pz;
+            const double dx = fma(-L0, rint(ex * iL0), ex), dy = fma(-L1, rint(ey * iL1), ey),
+                         dz = fma(-L2, rint(ez * iL2), ez);
             const double r2 = dx * dx + dy * dy + dz * dz;
             const bool inside = j != tp && r2 <= P.rcut2;
-            const double lr = gauss_hermite(D.gtab, fmin(r2 * P.inv2a2, tlim)) * P.inv8a3;
+            // the Hermite cell as a cubic in u (Horner with fused steps)
+            const double cell = fmin(r2 * P.inv2a2, tlim) * 64.0;
+            const int ci = static_cast<int>(cell);
+            const double u = cell - static_cast<double>(ci);
+            const double4 hc = D.gcoef[ci];
+            const double lr = fma(fma(fma(hc.w, u, hc.z), u, hc.y), u, hc.x) * P.inv8a3;
+            // (r^2 + eps^2)^(-3/2): hardware reciprocal square root and two
+            // Newton steps y <- y + y (1 - dd y^2) / 2 (r2 = 0 is masked below)
             const double dd = r2 + eps2;
-            const double soft = 1.0 / (dd * sqrt(dd));
+            double y = __builtin_amdgcn_rsq(dd);
+            y = fma(0.5 * y, fma(-dd * y, y, 1.0), y);
+            y = fma(0.5 * y, fma(-dd * y, y, 1.0), y);
+            const double soft = y * y * y;
             const double mj = D.uniform_mass ? P.uniform_mass : sj.w;
             const double f = inside && r2 > 0.0 ? (soft - lr) * P.G * mj : 0.0;
-            sx += f * dx;
-            sy += f * dy;
-            sz += f * dz;
+            ax += f * dx;
+            ay += f * dy;
+            az += f * dz;
             cnt[2] += inside ? 1u : 0u;
             nearest = inside ? fmin(nearest, r2) : nearest;
           }
-          ax += sx;
-          ay += sy;
-          az += sz;
         }
         for (uint32_t e = s; e < nm; e += S) {
           if (!valid) continue;
@@ -1138,9 +1200,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
           const NodeLink lk = D.link[q];
           const NodeGeo64 g = D.geo64[q];
           h = g.half;
-          dc0 = nearest_image(g.cx - c0x, L0);
-          dc1 = nearest_image(g.cy - c0y, L1);
-          dc2 = nearest_image(g.cz - c0z, L2);
+          dc0 = nearest_image_inv(g.cx - c0x, L0, iL0);
+          dc1 = nearest_image_inv(g.cy - c0y, L1, iL1);
+          dc2 = nearest_image_inv(g.cz - c0z, L2, iL2);
           const double v0 = fabs(dc0), v1 = fabs(dc1), v2 = fabs(dc2);
           const double g0 = fmax(0.0, v0 - h - hg0), g1 = fmax(0.0, v1 - h - hg1), g2 = fmax(0.0, v2 - h - hg2);
           const double f0 = v0 + h + hg0, f1 = v1 + h + hg1, f2 = v2 + h + hg2;
@@ -1156,9 +1218,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             bool acc = gap2 > 0.0 && (!G.contain || far2 <= P.rcut2);
             if (acc) {
               const NodeMom64 M = D.mom64[q];
-              const double m0 = fmax(0.0, fabs(nearest_image(M.comx - c0x, L0)) - hg0);
-              const double m1 = fmax(0.0, fabs(nearest_image(M.comy - c0y, L1)) - hg1);
-              const double m2 = fmax(0.0, fabs(nearest_image(M.comz - c0z, L2)) - hg2);
+              const double m0 = fmax(0.0, fabs(nearest_image_inv(M.comx - c0x, L0, iL0)) - hg0);
+              const double m1 = fmax(0.0, fabs(nearest_image_inv(M.comy - c0y, L1, iL1)) - hg1);
+              const double m2 = fmax(0.0, fabs(nearest_image_inv(M.comz - c0z, L2, iL2)) - hg2);
               const double rmin2 = m0 * m0 + m1 * m1 + m2 * m2;
               const double width2 = 4.0 * h * h;
               acc = rmin2 > 0.0 && width2 < G.amax2 * rmin2 && M.mass * width2 <= G.tolg * amin * rmin2 * rmin2;
@@ -1829,7 +1891,8 @@ void walk_fp64_adaptive(const Tree& t, const WalkParams& p, const double4* src, 
 }
 
 void walk_fp64_group(const Tree& t, const WalkParams& p, const GroupParams& g, const double4* src,
-                     int uniform_mass, const double2* gtab, const double* prev, const uint32_t* leaf_of_pos,
+                     int uniform_mass, const double2* gtab, const double4* gcoef, const double* prev,
+                     const uint32_t* leaf_of_pos,
                      const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
                      const double* fz, const WalkOutputs& out, uint32_t* fallback_pos, uint32_t* fallback_slot,
                      uint32_t* fallback_count, unsigned long long* counters, hipStream_t s) {
@@ -1849,6 +1912,7 @@ void walk_fp64_group(const Tree& t, const WalkParams& p, const GroupParams& g, c
   d.f64y = fy;
   d.f64z = fz;
   d.gtab = gtab;
+  d.gcoef = gcoef;
   d.prev = prev;
   const unsigned blocks = static_cast<unsigned>((static_cast<std::size_t>(p.n_targets) + g.block - 1) / g.block);
   if (p.assignment == 0)
@@ -1889,6 +1953,19 @@ std::vector<double> gaussian_hermite_knots() {
     k[2 * i + 1] = step * dh;
   }
   return k;
+}
+
+std::vector<double> gaussian_hermite_monomials(const std::vector<double>& knots) {
+  // the cubic Hermite cell i on u in [0, 1) as a0 + u (a1 + u (a2 + u a3))
+  std::vector<double> a(4 * kGaussCells);
+  for (int i = 0; i < kGaussCells; ++i) {
+    const double v0 = knots[2 * i], s0 = knots[2 * i + 1], v1 = knots[2 * i + 2], s1 = knots[2 * i + 3];
+    a[4 * i] = v0;
+    a[4 * i + 1] = s0;
+    a[4 * i + 2] = 3.0 * (v1 - v0) - 2.0 * s0 - s1;
+    a[4 * i + 3] = 2.0 * (v0 - v1) + s0 + s1;
+  }
+  return a;
 }
 
 void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,

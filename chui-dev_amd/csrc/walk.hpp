@@ -131,7 +131,8 @@ void walk_fp64_adaptive(const Tree& t, const WalkParams& p, const double4* src, 
 // are appended to (fallback_pos, fallback_slot) with their count in
 // *fallback_count (device) for walk_fp64_adaptive.
 void walk_fp64_group(const Tree& t, const WalkParams& p, const GroupParams& g, const double4* src,
-                     int uniform_mass, const double2* gtab, const double* prev, const uint32_t* leaf_of_pos,
+                     int uniform_mass, const double2* gtab, const double4* gcoef, const double* prev,
+                     const uint32_t* leaf_of_pos,
                      const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
                      const double* fz, const WalkOutputs& out, uint32_t* fallback_pos, uint32_t* fallback_slot,
                      uint32_t* fallback_count, unsigned long long* counters, hipStream_t s);
@@ -139,6 +140,9 @@ void walk_fp64_group(const Tree& t, const WalkParams& p, const GroupParams& g, c
 // Host FP64 Gaussian Hermite knots of tree_pm_split_kernel.hpp:130-183:
 // 4097 (H(t_i), step * dH/dt(t_i)) pairs, t_i = i / 64.
 std::vector<double> gaussian_hermite_knots();
+// The same 4096 cubic cells as monomials in the cell coordinate u:
+// (a0, a1, a2, a3) per cell (the group walk's pair kernel).
+std::vector<double> gaussian_hermite_monomials(const std::vector<double>& knots);
 
 // Leaf node of every tree position of the last built tree.
 void build_leaf_of_pos(const Tree& t, uint32_t* leaf_of_pos, hipStream_t s);
