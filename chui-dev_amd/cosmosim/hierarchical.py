@@ -187,11 +187,16 @@ class HierarchicalKdk:
         self.box = tuple(options.box_size)
         self.solves = 0
         self.short_range_solves = 0
+        # torch's gathers and the library's solves, kicks and drifts share one
+        # stream (default priority), so a block needs no host synchronisation
+        # between them
+        self.stream = torch.cuda.Stream(device=device)
+        self.coord.set_stream(self.stream.cuda_stream)
 
     def close(self):
         self.coord.close()
 
-    # -- device pieces -----------------------------------------------------------
+    # -- device pieces (enqueued on self.stream) ----------------------------------
     def _solve(self, active, refresh: bool):
         """Short-range lanes per source row; PM lanes per active slot when refreshing."""
         na = self.n if active is None else int(active.shape[0])
@@ -199,10 +204,8 @@ class HierarchicalKdk:
         sr = tuple(torch.zeros(self.n, dtype=self.dtype, device=self.device) for _ in range(3))
         pm = tuple(torch.empty(na, dtype=self.dtype, device=self.device) for _ in range(3)) if refresh else None
         self.options.short_range_only = 0 if refresh else 1
-        torch.cuda.current_stream(self.device).synchronize()
         self.coord.solve_active_set_with_pm_cadence(*self.x, self.m, ForceAccumulator(acc, active, pm, sr),
                                                     self.options, refresh)
-        self.coord.synchronize()
         if refresh:
             self.solves += 1
         else:
@@ -211,17 +214,17 @@ class HierarchicalKdk:
 
     def _kick(self, accel, rows, factor: float, drag: float = 1.0):
         """u[rows] = drag u[rows] + factor accel (accel per listed row)."""
-        torch.cuda.current_stream(self.device).synchronize()  # accel may come from torch gathers
         self.coord.kick(*self.u, *accel, factor, drag, rows)
-
-    def _rows_of_bins(self, bins_dev, wanted):
-        mask = torch.zeros_like(bins_dev, dtype=torch.bool)
-        for b in wanted:
-            mask |= bins_dev == b
-        return torch.nonzero(mask).reshape(-1).to(torch.int32)
 
     def block(self, a_begin: float, max_span: float | None = None, bins=None) -> BlockPlan:
         """One block from a_begin; the returned plan's bins are a device tensor."""
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(self.stream):
+            plan = self._block(a_begin, max_span, bins)
+        self.stream.synchronize()
+        return plan
+
+    def _block(self, a_begin, max_span, bins) -> BlockPlan:
         # synchronized Tree+PM solve at the block start (the mandatory bootstrap)
         acc, pm, sr = self._solve(None, True)
         norm = torch.sqrt(acc[0].double() ** 2 + acc[1].double() ** 2 + acc[2].double() ** 2)
@@ -230,37 +233,39 @@ class HierarchicalKdk:
         plan = plan_block(self.cosmo, a_begin, norm, self.options.softening_epsilon, self.max_bin, self.eta, max_span,
                           bins)
         bins = plan.bins
+        # rows of each bin, once per block (the bins are fixed within it)
+        rows_of = [torch.nonzero(bins == b).reshape(-1).to(torch.int32) for b in range(plan.max_bin + 1)]
+        zeros = tuple(torch.zeros_like(t) for t in pm)
         tree = sr
         periods = 1 << plan.max_bin
         all_rows = None
         for off in range(periods):
             # opening half-kicks (tree force of the interval's start) and the PM opening kick
             for b, f in plan.opening[off].items():
-                rows = self._rows_of_bins(bins, [b])
+                rows = rows_of[b]
                 if rows.numel():
                     self._kick(tuple(t.index_select(0, rows.long()) for t in tree), rows, f)
             if off == 0:
                 self._kick(pm, all_rows, plan.pm_open)
             # drift of every particle with u at the tick start, then the Hubble drag of u
             self.coord.drift(*self.x, *self.u, plan.drift[off], self.box)
-            self._kick(tuple(torch.zeros_like(t) for t in pm), all_rows, 0.0, plan.drag[off])
+            self._kick(zeros, all_rows, 0.0, plan.drag[off])
             # closing forces and half-kicks of the bins whose interval ends
             closing = list(plan.closing[off].keys())
             last = off + 1 == periods
             if last:
                 acc, pm, tree = self._solve(None, True)
             else:
-                rows = self._rows_of_bins(bins, closing)
+                rows = torch.cat([rows_of[b] for b in closing])
                 if rows.numel():
                     _, _, sr = self._solve(rows, False)
                     tree = tuple(torch.where(self._member(bins, closing), s, t) for s, t in zip(sr, tree))
             for b, f in plan.closing[off].items():
-                rows = self._rows_of_bins(bins, [b])
+                rows = rows_of[b]
                 if rows.numel():
                     self._kick(tuple(t.index_select(0, rows.long()) for t in tree), rows, f)
             if last:
                 self._kick(pm, all_rows, plan.pm_close)
-        self.coord.synchronize()
         return plan
 
     @staticmethod
