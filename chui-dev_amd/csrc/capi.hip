@@ -514,14 +514,15 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   wp.identity_targets = identity;
   // the mixed walk's counting variant also produces the nearest-pair lane
   wp.count = (diag != nullptr || wcount != nullptr || (mixed && near2 != nullptr)) ? 1 : 0;
-  // one target per wave (mixed strict walk without counters) when asked, or
-  // when at most 1/64 of the sources are active: 64 visit-ordered targets
-  // then lie far apart and a shared walk covers the union of their
-  // neighbourhoods (measured crossover: 2-3% active at 256^3, about 35% at
-  // 64^3, where the per-wave walk's latency-bound traversal costs less)
-  constexpr uint64_t kSparseTargets = 64;
-  const bool sparse = n_active * kSparseTargets <= n;
-  wp.target_per_wave = mixed && !wp.count &&
+  // one target per wave when asked, or when few sources are active (at most
+  // 1/64 in mixed precision, 1/16 in FP64): 64 visit-ordered targets then
+  // lie far apart and a shared walk covers the union of their
+  // neighbourhoods. Measured crossovers: mixed 2-3% active at 256^3 and
+  // about 35% at 64^3; FP64, whose pairs cost more, about 15% at 128^3.
+  const uint64_t sparse_ratio = mixed ? 64 : 16;
+  const bool sparse = n_active * sparse_ratio <= n;
+  // (strict walks without counters; FP64 also without the nearest-pair lane)
+  wp.target_per_wave = !wp.count && (mixed || near2 == nullptr) &&
                        (o->target_mapping == CHUI_TARGETS_PER_WAVE ||
                         (o->target_mapping == CHUI_TARGETS_AUTO && sparse)) ? 1 : 0;
   wp.uniform_mass = um;

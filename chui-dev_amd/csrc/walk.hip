@@ -1776,6 +1776,137 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_target_kernel(WalkParams
   }
 }
 
+// ---- FP64 walk, one target per wave ----------------------------------------------
+// walkmix_target_kernel's mapping for the FP64 strict walk: the lanes classify
+// 64 consecutive preorder nodes with walk64_kernel's tests (the cutoff prune
+// against the octree cell, accept64 for internal nodes), the visited chain
+// lists kept leaves' sources and accepted nodes in LDS, and the lists are
+// evaluated 64 entries at a time with walk64_kernel's pair and node
+// arithmetic. The pairs and multipoles are walk64_kernel's; the sum runs in
+// another order (per-lane partial sums, then a wave reduction).
+template <int SCHEME>
+__global__ __launch_bounds__(kWalkThreads) void walk64_target_kernel(WalkParams P, WalkData D, WalkOutputs out) {
+  __shared__ uint32_t list[kWalkWaves][kTargetList];
+  __shared__ uint32_t mps[kWalkWaves][kTargetNodes];
+  const int w = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const uint32_t k = xcd_logical_block() * kWalkWaves + w;
+  if (k >= P.n_targets) return;  // wave-uniform
+  const uint32_t tp = P.identity_targets ? k : D.tpos[k];
+  const uint32_t slot = P.identity_targets ? D.order[k] : D.tslot[k];
+  const double4 t = D.src64[tp];
+  const double px = t.x, py = t.y, pz = t.z;
+  const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
+  const unsigned long long below = (1ull << lane) - 1ull;
+
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  uint32_t nl = 0, nm = 0;
+  auto evaluate = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t e = lane; e < nl; e += kWave) {
+      const uint32_t j = list[w][e];
+      const double4 sj = D.src64[j];
+      const double qx = mi64(sj.x - px, L0), qy = mi64(sj.y - py, L1), qz = mi64(sj.z - pz, L2);
+      const double r2 = qx * qx + qy * qy + qz * qz;
+      // coincident pairs have a zero vector; the target itself is skipped
+      if (j != tp && r2 <= P.rcut2 && !(qx == 0.0 && qy == 0.0 && qz == 0.0)) {
+        const double f = residual_inv_r3(r2, P.eps, P.a) * P.G;
+        const double mj = D.uniform_mass ? P.uniform_mass : sj.w;
+        ax += f * mj * qx;
+        ay += f * mj * qy;
+        az += f * mj * qz;
+      }
+    }
+    for (uint32_t e = lane; e < nm; e += kWave) {
+      const uint32_t n = mps[w][e];
+      const NodeMom64 M = D.mom64[n];
+      const double dx = mi64(M.comx - px, L0), dy = mi64(M.comy - py, L1), dz = mi64(M.comz - pz, L2);
+      node_accel64_call(M, D.quad64[n], dx, dy, dz, P, ax, ay, az);  // out of line: register pressure
+    }
+    __builtin_amdgcn_wave_barrier();
+    nl = 0;
+    nm = 0;
+  };
+
+  uint32_t p = 0;
+  while (p < P.node_count) {
+    const uint32_t q = p + lane;
+    uint32_t next = q + 1, lb = 0, lc = 0;
+    int kind = 3;  // 0 pruned, 1 kept leaf, 2 accepted node, 3 opened
+    if (q < P.node_count) {
+      const NodeLink lk = D.link[q];
+      const NodeGeo64 g = D.geo64[q];
+      const double h = g.half;
+      const double ax_ = fabs(mi64(g.cx - px, L0)), ay_ = fabs(mi64(g.cy - py, L1)), az_ = fabs(mi64(g.cz - pz, L2));
+      const double ex = fmax(0.0, ax_ - h), ey = fmax(0.0, ay_ - h), ez = fmax(0.0, az_ - h);
+      const bool keep = within_cutoff(ex * ex + ey * ey + ez * ez, P.rcut, P.rcut2);
+      if (lk.child_count == 0) {
+        kind = keep ? 1 : 0;
+        lb = lk.begin;
+        lc = lk.count;
+      } else if (!keep) {
+        kind = 0;
+        next = q + lk.skip;
+      } else {
+        const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+        const double mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+        const bool contained = within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
+        const NodeMom64 M = D.mom64[q];
+        const double mdx = mi64(M.comx - px, L0), mdy = mi64(M.comy - py, L1), mdz = mi64(M.comz - pz, L2);
+        const bool take = accept64(h, D.quad64[q].offset, inside, contained, mdx * mdx + mdy * mdy + mdz * mdz, P);
+        kind = take ? 2 : 3;
+        next = take ? q + lk.skip : q + 1;
+      }
+    }
+    unsigned long long chain = 0;
+    uint32_t cur = p;
+    while (cur < p + kWave && cur < P.node_count) {
+      const uint32_t l = cur - p;
+      chain |= 1ull << l;
+      cur = readlane_u32(next, l);
+    }
+    p = cur;
+    const bool on = (chain >> lane) & 1ull;
+    const unsigned long long mm = __ballot(on && kind == 2);
+    if (nm + static_cast<uint32_t>(__popcll(mm)) > static_cast<uint32_t>(kTargetNodes)) evaluate();
+    if (on && kind == 2) mps[w][nm + __popcll(mm & below)] = q;
+    nm += __popcll(mm);
+    unsigned long long rest = __ballot(on && kind == 1);
+    while (rest) {
+      const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(rest)) - 1);
+      rest &= rest - 1ull;
+      const uint32_t b0 = readlane_u32(lb, l), c = readlane_u32(lc, l);
+      for (uint32_t c0 = 0; c0 < c; c0 += kWave) {
+        const uint32_t cn = c - c0 < static_cast<uint32_t>(kWave) ? c - c0 : static_cast<uint32_t>(kWave);
+        if (nl + cn > static_cast<uint32_t>(kTargetList)) evaluate();
+        if (static_cast<uint32_t>(lane) < cn) list[w][nl + lane] = b0 + c0 + lane;
+        nl += cn;
+      }
+    }
+  }
+  evaluate();
+  ax = wave_sum(ax);
+  ay = wave_sum(ay);
+  az = wave_sum(az);
+  if (lane != 0) return;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  if (P.pm_enabled) interp64<SCHEME>(P, D.f64x, D.f64y, D.f64z, px, py, pz, gx, gy, gz);
+  static_cast<double*>(out.ax)[slot] = gx + ax;
+  static_cast<double*>(out.ay)[slot] = gy + ay;
+  static_cast<double*>(out.az)[slot] = gz + az;
+  if (out.pm_x) {
+    static_cast<double*>(out.pm_x)[slot] = gx;
+    static_cast<double*>(out.pm_y)[slot] = gy;
+    static_cast<double*>(out.pm_z)[slot] = gz;
+  }
+  const uint32_t row = D.order[tp];
+  if (out.sr_x) {
+    static_cast<double*>(out.sr_x)[row] = ax;
+    static_cast<double*>(out.sr_y)[row] = ay;
+    static_cast<double*>(out.sr_z)[row] = az;
+  }
+}
+
 // H(t) = L(sqrt t) / t^{3/2} (tree_pm_split_kernel.hpp:109-128).
 double gaussian_h_exact(double t) {
   if (t < 0.015625) {
@@ -1860,6 +1991,14 @@ void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int unifo
   d.f64x = fx;
   d.f64y = fy;
   d.f64z = fz;
+  if (p.target_per_wave) {
+    if (p.assignment == 0)
+      walk64_target_kernel<0><<<grid_for(p.n_targets, kWalkWaves), kWalkThreads, 0, s>>>(p, d, out);
+    else
+      walk64_target_kernel<1><<<grid_for(p.n_targets, kWalkWaves), kWalkThreads, 0, s>>>(p, d, out);
+    CHUI_LAUNCH_CHECK("walk64_target_kernel");
+    return;
+  }
   if (p.assignment == 0) launch64<0>(p, d, out, counters, s);
   else launch64<1>(p, d, out, counters, s);
 }

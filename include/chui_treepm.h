@@ -85,12 +85,14 @@ typedef enum chui_acceptance_policy {
   CHUI_ACCEPT_ADAPTIVE = 1  /* kAdaptiveRelative */
 } chui_acceptance_policy;
 
-/* How the mixed-precision strict walk maps targets onto 64-wide waves (this
- * library; no reference counterpart: the reference walks targets one by one).
- * The forces are the same pairs and multipoles either way, summed in another
- * order. Other walks always run one target per lane. */
+/* How the strict walks map targets onto 64-wide waves (this library; no
+ * reference counterpart: the reference walks targets one by one). The forces
+ * are the same pairs and multipoles either way, summed in another order.
+ * Walks with counters or (FP64) the nearest-pair lane, and the adaptive
+ * walks, always run one target per lane. */
 typedef enum chui_target_mapping {
-  CHUI_TARGETS_AUTO = 0,      /* per wave when the active set is sparse, else per lane */
+  CHUI_TARGETS_AUTO = 0,      /* per wave when at most 1/64 (mixed) or 1/16 (FP64) of the
+                                 sources are active, else per lane */
   CHUI_TARGETS_PER_LANE = 1,  /* 64 visit-ordered targets per wave sharing one walk */
   CHUI_TARGETS_PER_WAVE = 2   /* one target per wave: lanes split its nodes and pairs */
 } chui_target_mapping;

@@ -1,4 +1,4 @@
-"""GPU parity of the mixed walk's two target mappings.
+"""GPU parity of the two target mappings of the strict walks.
 
 CHUI_TARGETS_PER_WAVE (one target per wave, walkmix_target_kernel) must give
 the pairs and multipoles of CHUI_TARGETS_PER_LANE (64 targets per wave,
@@ -10,7 +10,9 @@ below 1e-3) against the oracle's FP64 short-range residual
 quadrupole acceptance (accepted multipoles), both opening criteria, a
 rectangular box, per-particle masses, coincident sources, leaves of up to
 200 sources (several 64-source chunks, full source lists), sparse and dense active
-sets, the PM interpolation fused in (TSC) and short-range-only solves.
+sets, the PM interpolation fused in (TSC) and short-range-only solves. The
+FP64 strict walk's per-wave kernel (walk64_target_kernel) matches its per-lane
+walk and the oracle to 1e-12.
 """
 from __future__ import annotations
 
@@ -31,15 +33,16 @@ from cosmosim import _lib as L  # noqa: E402
 from test_oracle_pins import clustered  # noqa: E402
 
 
-def solve_mixed(x, y, z, m, o, mesh, mapping, active=None, sr_only=False):
-    """Mixed solve -> (totals per active slot, short-range lanes per source row)."""
-    o.precision = L.PRECISION_MIXED
+def solve_mixed(x, y, z, m, o, mesh, mapping, active=None, sr_only=False, precision=L.PRECISION_MIXED):
+    """Solve -> (totals per active slot, short-range lanes per source row)."""
+    o.precision = precision
     o.target_mapping = mapping
     o.short_range_only = 1 if sr_only else 0
     dx, dy, dz, dm = to_dev(x, y, z, m)
     na = x.size if active is None else len(active)
-    acc = tuple(empty_dev(na, torch.float32) for _ in range(3))
-    sr = tuple(torch.zeros(x.size, dtype=torch.float32, device="cuda:0") for _ in range(3))
+    dt = torch.float32 if precision == L.PRECISION_MIXED else torch.float64
+    acc = tuple(empty_dev(na, dt) for _ in range(3))
+    sr = tuple(torch.zeros(x.size, dtype=dt, device="cuda:0") for _ in range(3))
     act = to_dev(np.asarray(active, np.uint32).view(np.int32))[0] if active is not None else None
     torch.cuda.synchronize()
     with TreePmCoordinator(mesh) as coord:
@@ -47,6 +50,7 @@ def solve_mixed(x, y, z, m, o, mesh, mapping, active=None, sr_only=False):
                                                not sr_only)
         coord.synchronize()
     o.short_range_only = 0
+    o.precision = L.PRECISION_MIXED
     return np.stack([a.double().cpu().numpy() for a in acc]), np.stack([s.double().cpu().numpy() for s in sr])
 
 
@@ -105,3 +109,24 @@ def test_auto_mapping_short_range_only(case):
         assert np.array_equal(auto_acc, ref_acc)
         # short-range only: the totals are the short-range lanes of the active rows
         assert np.array_equal(auto_acc, auto_sr[:, active.astype(np.int64)])
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("fraction", [0.02, 1.0])
+def test_fp64_per_wave_matches_per_lane_and_oracle(case, fraction):
+    """The FP64 strict walk with one target per wave (walk64_target_kernel):
+    the same pairs and multipoles as the per-lane walk, summed in another
+    order, so both agree with each other and with the oracle to 1e-12."""
+    _, (x, y, z, m), o, mesh = case
+    n = x.size
+    active = None
+    if fraction < 1.0:
+        active = np.sort(np.random.default_rng(6).choice(n, int(fraction * n), replace=False)).astype(np.uint32)
+    fp64 = L.PRECISION_FP64
+    wave_acc, wave_sr = solve_mixed(x, y, z, m, o, mesh, L.TARGETS_PER_WAVE, active, precision=fp64)
+    lane_acc, lane_sr = solve_mixed(x, y, z, m, o, mesh, L.TARGETS_PER_LANE, active, precision=fp64)
+    rows = np.arange(n) if active is None else active.astype(np.int64)
+    assert rel_l2(wave_sr[:, rows], lane_sr[:, rows]) < 1e-13
+    assert rel_l2(wave_acc, lane_acc) < 1e-12
+    ref, _, _ = O.short_range(x, y, z, m, o, mesh)
+    assert rel_l2(wave_sr[:, rows], np.stack(ref)[:, rows]) < 1e-12
