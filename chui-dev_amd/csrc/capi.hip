@@ -493,8 +493,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
                                                                      c->flag.as<uint32_t>());
     CHUI_LAUNCH_CHECK("target_keys_kernel");
     uint32_t bad = 0;
-    CHUI_HIP(hipMemcpyAsync(&bad, c->flag.ptr, 4, hipMemcpyDeviceToHost, s));
-    CHUI_HIP(hipStreamSynchronize(s));
+    c->tree.readback.read(&bad, c->flag.ptr, 4, s);
     if (bad) throw std::invalid_argument("TreePM active particle index out of range");
     // Visit targets in tree order (tree_pm_coupling.cpp:3801-3838).
     chui::radix_sort_pairs32(c->tpos.as<uint32_t>(), c->tslot.as<uint32_t>(), c->tkeys_alt.as<uint32_t>(),
@@ -582,8 +581,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       source_bounds_kernel<<<std::min<unsigned>(chui::grid_for(n, 256), 1024u), 256, 0, s>>>(x, y, z, n, bkeys);
       CHUI_LAUNCH_CHECK("source_bounds_kernel");
       unsigned long long keys[8];
-      CHUI_HIP(hipMemcpyAsync(keys, bkeys, sizeof(keys), hipMemcpyDeviceToHost, s));
-      CHUI_HIP(hipStreamSynchronize(s));
+      c->tree.readback.read(keys, bkeys, sizeof(keys), s);
       chui::GroupParams gp{};
       gp.block = static_cast<uint32_t>(o->residual_block_size);
       gp.group_size = ad.group_target_size;
@@ -609,8 +607,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
                             c->fb_pos.as<uint32_t>(), c->fb_slot.as<uint32_t>(), fb_count,
                             reinterpret_cast<unsigned long long*>(c->gcounters.ptr), s);
       uint32_t nfb = 0;
-      CHUI_HIP(hipMemcpyAsync(&nfb, fb_count, sizeof(nfb), hipMemcpyDeviceToHost, s));
-      CHUI_HIP(hipStreamSynchronize(s));
+      c->tree.readback.read(&nfb, fb_count, sizeof(nfb), s);
       if (nfb > n_active) throw std::logic_error("TreePM group walk fallback list overflow");
       if (nfb > 0) {
         chui::WalkParams fw = wp;

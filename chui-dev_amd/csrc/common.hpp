@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 
@@ -52,6 +53,28 @@ struct DeviceBuffer {
     bytes = need;
   }
   template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Small device-to-host readbacks (counts, flags, bounds) through a pinned host
+// block: the copy lands directly in host memory instead of going through the
+// runtime's staging pass for pageable destinations.
+struct PinnedReadback {
+  static constexpr std::size_t kBytes = 256;
+  void* ptr = nullptr;
+  PinnedReadback() = default;
+  PinnedReadback(const PinnedReadback&) = delete;
+  PinnedReadback& operator=(const PinnedReadback&) = delete;
+  ~PinnedReadback() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+  // copies bytes (<= kBytes) from dev to host after the stream's prior work
+  void read(void* host, const void* dev, std::size_t bytes, hipStream_t s) {
+    if (bytes > kBytes) throw std::logic_error("pinned readback larger than its block");
+    if (!ptr) CHUI_HIP(hipHostMalloc(&ptr, kBytes, hipHostMallocDefault));
+    CHUI_HIP(hipMemcpyAsync(ptr, dev, bytes, hipMemcpyDeviceToHost, s));
+    CHUI_HIP(hipStreamSynchronize(s));
+    std::memcpy(host, ptr, bytes);
+  }
 };
 
 // ---- device helpers -------------------------------------------------------

@@ -1037,8 +1037,7 @@ SourceStats compute_stats(Tree& t, Axes3 p, const double* m, std::size_t n, hipS
     CHUI_LAUNCH_CHECK("source_stats_kernel");
   }
   SourceStats h{};
-  CHUI_HIP(hipMemcpyAsync(&h, st, sizeof(SourceStats), hipMemcpyDeviceToHost, s));
-  CHUI_HIP(hipStreamSynchronize(s));
+  t.readback.read(&h, st, sizeof(SourceStats), s);
   return h;
 }
 
@@ -1115,8 +1114,7 @@ void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, co
   t.bfs_scan.reserve(scan_scratch_bytes(NW) + 64);
   exclusive_scan_u32(popc, rank_base, NW, marked, t.bfs_scan.ptr, s);
   uint32_t hm = 0;
-  CHUI_HIP(hipMemcpyAsync(&hm, marked, sizeof(hm), hipMemcpyDeviceToHost, s));
-  CHUI_HIP(hipStreamSynchronize(s));
+  t.readback.read(&hm, marked, sizeof(hm), s);
   t.bucket_max.reserve(2 * std::max<uint64_t>(hm, 1) * sizeof(uint64_t));
   uint64_t* cmin = t.bucket_max.as<uint64_t>();
   uint64_t* cmax = cmin + std::max<uint64_t>(hm, 1);
@@ -1131,8 +1129,7 @@ void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, co
   gap_final_kernel<<<naxes, kGapThreads, 0, s>>>(best, nblk, dres);
   CHUI_LAUNCH_CHECK("gap_final_kernel");
   UnwrapResult h{};
-  CHUI_HIP(hipMemcpyAsync(&h, dres, sizeof(UnwrapResult), hipMemcpyDeviceToHost, s));
-  CHUI_HIP(hipStreamSynchronize(s));
+  t.readback.read(&h, dres, sizeof(UnwrapResult), s);
   for (int ax = 0; ax < naxes; ++ax) {
     res.anchor[ax] = h.anchor[ax];
     res.umin[ax] = h.umin[ax];
@@ -1160,8 +1157,7 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
   if (n == 0) return;
   t.mass_uniform = st.mass_nonuniform == 0;
   t.uniform_mass = 0.0;
-  CHUI_HIP(hipMemcpyAsync(&t.uniform_mass, m, sizeof(double), hipMemcpyDeviceToHost, s));
-  CHUI_HIP(hipStreamSynchronize(s));
+  t.readback.read(&t.uniform_mass, m, sizeof(double), s);
   int canonical[3];
   for (int a = 0; a < 3; ++a) {
     const double lo = host_order_key_inv(st.lo[a]);
@@ -1247,8 +1243,7 @@ restart_discovery:
     uint32_t* cbase = t.level_scratch.as<uint32_t>();
     exclusive_scan_u32(t.bfs_nchild.as<uint32_t>() + lb, cbase, cnt, t.flags.as<uint32_t>() + 1, t.bfs_scan.ptr, s);
     uint32_t hflags[2] = {0, 0};
-    CHUI_HIP(hipMemcpyAsync(hflags, t.flags.ptr, sizeof(hflags), hipMemcpyDeviceToHost, s));
-    CHUI_HIP(hipStreamSynchronize(s));
+    t.readback.read(hflags, t.flags.ptr, sizeof(hflags), s);
     if (hflags[0]) {
       throw UnsupportedError(
           "octree deeper than 21 levels (more than max_leaf_size coincident or near-coincident particles) "
@@ -1358,8 +1353,7 @@ restart_discovery:
                                                                                       t.flags.as<uint32_t>() + 2);
   CHUI_LAUNCH_CHECK("count_leaves_kernel");
   uint32_t hleaf = 0;
-  CHUI_HIP(hipMemcpyAsync(&hleaf, t.flags.as<uint32_t>() + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  CHUI_HIP(hipStreamSynchronize(s));
+  t.readback.read(&hleaf, t.flags.as<uint32_t>() + 2, sizeof(uint32_t), s);
   t.leaf_count = hleaf;
 }
 
@@ -1452,11 +1446,9 @@ DepositUnits build_deposit_units(Tree& t, const DepositTiling& d, hipStream_t s)
       t.dep_blocks.as<uint32_t>());
   CHUI_LAUNCH_CHECK("deposit_emit_kernel");
   uint32_t host[2] = {0, 0};
-  CHUI_HIP(hipMemcpyAsync(host, totals, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  CHUI_HIP(hipStreamSynchronize(s));
+  t.readback.read(host, totals, sizeof(uint32_t), s);
   exclusive_scan_u32(t.dep_blocks.as<uint32_t>(), t.dep_blocks.as<uint32_t>(), host[0], totals + 1, t.dep_scan.ptr, s);
-  CHUI_HIP(hipMemcpyAsync(host + 1, totals + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  CHUI_HIP(hipStreamSynchronize(s));
+  t.readback.read(host + 1, totals + 1, sizeof(uint32_t), s);
   r.units = t.dep_units.as<DepositUnit>();
   r.first_block = t.dep_blocks.as<uint32_t>();
   r.count = host[0];

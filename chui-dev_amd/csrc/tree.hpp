@@ -76,6 +76,7 @@ struct Tree {
   DeviceBuffer dep_flag, dep_index, dep_units, dep_blocks, dep_scan, dep_total;
   std::size_t bfs_capacity = 0;
   std::vector<std::size_t> level_start;  // BFS level ranges (host copy)
+  PinnedReadback readback;               // counts and flags read back during a build
 };
 
 // Bounds/canonical check, periodic unwrap on all three axes, Morton/path keys,
