@@ -10,10 +10,12 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cctype>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -780,14 +782,32 @@ chui_status chui_treepm_options_set(chui_treepm_options* o, const char* key_c, c
       if (value == "false" || value == "0" || value == "no" || value == "off") return 0;
       throw std::invalid_argument("value of " + key + " is not a boolean: " + value);
     };
+    // "<magnitude> [mpc|kpc]" (config.cpp:300-362). A bare magnitude is in
+    // Mpc, the reference's default units.length_unit; decks with another
+    // length unit go through the deck reader, which resolves it.
+    auto length_mpc = [&]() {
+      std::istringstream in(value);
+      double mag = 0.0;
+      in >> mag;
+      if (in.fail() || !std::isfinite(mag))
+        throw std::invalid_argument("invalid value '" + value + "': expected finite numeric magnitude");
+      std::string unit;
+      in >> unit;
+      in >> std::ws;
+      if (!in.eof()) throw std::invalid_argument("invalid value '" + value + "': unexpected trailing token(s)");
+      for (char& ch : unit) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+      if (unit.empty() || unit == "mpc") return mag;
+      if (unit == "kpc") return mag * 1.0e-3;
+      throw std::invalid_argument("key '" + key + "': unsupported unit in value '" + value + "'");
+    };
     if (key == "box_size" || key == "box_size_mpc_comoving") {
-      o->box_size[0] = o->box_size[1] = o->box_size[2] = num();
+      o->box_size[0] = o->box_size[1] = o->box_size[2] = length_mpc();
     } else if (key == "box_size_x") {
-      o->box_size[0] = num();
+      o->box_size[0] = length_mpc();
     } else if (key == "box_size_y") {
-      o->box_size[1] = num();
+      o->box_size[1] = length_mpc();
     } else if (key == "box_size_z") {
-      o->box_size[2] = num();
+      o->box_size[2] = length_mpc();
     } else if (key == "treepm_asmth_cells") {
       o->asmth_cells = num();
     } else if (key == "treepm_rcut_cells") {
@@ -816,8 +836,8 @@ chui_status chui_treepm_options_set(chui_treepm_options* o, const char* key_c, c
     } else if (key == "treepm_residual_block_size") {
       o->residual_block_size = static_cast<int32_t>(num());
     } else if (key == "gravity_softening") {
-      // "<value> kpc" -> comoving Mpc (gravity_runtime.cpp:566-567)
-      o->softening_epsilon = num() * 1.0e-3;
+      // comoving length, handed to the force path in Mpc (gravity_runtime.cpp:566-572)
+      o->softening_epsilon = length_mpc();
     } else if (key == "gravity_softening_mpc") {
       o->softening_epsilon = num();
     } else if (key == "gravitational_constant") {

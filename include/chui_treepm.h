@@ -172,7 +172,9 @@ typedef struct chui_treepm_context chui_treepm_context;
 /* ---- options ----------------------------------------------------------- */
 chui_status chui_treepm_default_options(chui_treepm_options* options);
 /* Applies one [numerics]/[cosmology] .param.txt key (config.cpp:1723-1766 names,
- * e.g. "treepm_asmth_cells", "treepm_tree_opening_theta", "box_size_mpc_comoving"). */
+ * e.g. "treepm_asmth_cells", "treepm_tree_opening_theta", "box_size_mpc_comoving").
+ * Lengths (box sizes, gravity_softening) read "<magnitude> [mpc|kpc]", a bare
+ * magnitude in Mpc (config.cpp:300-362 with the default units.length_unit). */
 chui_status chui_treepm_options_set(chui_treepm_options* options, const char* key, const char* value);
 /* Same validation rules as the reference: residual_block_size in [16,256], leaf in
  * [1,1024] (config.cpp:3370-3372), eps/r_s <= 0.2 (:3669), r_cut < L_min/2 (:3682-3692). */

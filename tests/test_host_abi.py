@@ -78,7 +78,7 @@ def test_options_set_accepts_param_keys(lib):
                        ("numerics.treepm_tree_opening_criterion", "geometric"),
                        ("numerics.treepm_tree_max_leaf_size", "8"), ("numerics.treepm_assignment_scheme", "tsc"),
                        ("numerics.treepm_enable_window_deconvolution", "true"), ("box_size", "100"),
-                       ("gravity_softening", "20"), ("treepm_precision", "mixed")):
+                       ("gravity_softening", "20 kpc"), ("treepm_precision", "mixed")):
         assert lib.chui_treepm_options_set(C.byref(o), key.encode(), value.encode()) == L.CHUI_OK
     assert o.asmth_cells == 1.5 and o.rcut_cells == 5.0 and o.opening_theta == 0.5
     assert o.opening_criterion == 0 and o.max_leaf_size == 8 and o.assignment_scheme == 1
@@ -87,7 +87,23 @@ def test_options_set_accepts_param_keys(lib):
     assert o.precision == L.PRECISION_MIXED
 
 
+def test_options_set_lengths_take_units():
+    # config.cpp:300-362: "<magnitude> [mpc|kpc]", a bare magnitude in the
+    # default length unit (Mpc)
+    o = L.default_options()
+    for key, value, field, want in (("gravity_softening", "0.015", "softening_epsilon", 0.015),
+                                    ("gravity_softening", "15 KPC", "softening_epsilon", 0.015),
+                                    ("box_size", "14761.89 kpc", "box_size", 14.76189),
+                                    ("box_size_y", "12 mpc", "box_size", 12.0)):
+        assert L.load().chui_treepm_options_set(C.byref(o), key.encode(), value.encode()) == L.CHUI_OK
+        got = o.softening_epsilon if field == "softening_epsilon" else o.box_size[1]
+        assert got == pytest.approx(want, rel=1e-14)
+
+
 @pytest.mark.parametrize("key,value,status", [
+    ("gravity_softening", "1 pc", L.CHUI_ERR_INVALID_ARGUMENT),
+    ("box_size", "10 mpc extra", L.CHUI_ERR_INVALID_ARGUMENT),
+    ("box_size", "inf", L.CHUI_ERR_INVALID_ARGUMENT),
     ("numerics.treepm_no_such_key", "1", L.CHUI_ERR_INVALID_ARGUMENT),
     ("numerics.treepm_asmth_cells", "abc", L.CHUI_ERR_INVALID_ARGUMENT),
     ("numerics.treepm_enable_window_deconvolution", "maybe", L.CHUI_ERR_INVALID_ARGUMENT),
