@@ -130,3 +130,22 @@ def test_fp64_per_wave_matches_per_lane_and_oracle(case, fraction):
     assert rel_l2(wave_acc, lane_acc) < 1e-12
     ref, _, _ = O.short_range(x, y, z, m, o, mesh)
     assert rel_l2(wave_sr[:, rows], np.stack(ref)[:, rows]) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [L.PRECISION_FP64, L.PRECISION_MIXED])
+@pytest.mark.parametrize("n", [1, 2, 3, 17])
+def test_per_wave_tiny_sets(precision, n):
+    """Trees of one leaf or a few nodes: the per-wave walk's window and chain
+    handle a node count below one wave."""
+    rng = np.random.default_rng(40 + n)
+    x, y, z = rng.uniform(0.0, 10.0, (3, n))
+    x[0], y[0], z[0] = 9.99, 0.01, 5.0  # a pair across the periodic seam when n > 1
+    if n > 1:
+        x[1], y[1], z[1] = 0.02, 9.98, 5.1
+    m = rng.uniform(0.5, 1.5, n)
+    o = options_struct(10.0, eps=0.02)
+    wave_acc, wave_sr = solve_mixed(x, y, z, m, o, 16, L.TARGETS_PER_WAVE, precision=precision)
+    lane_acc, lane_sr = solve_mixed(x, y, z, m, o, 16, L.TARGETS_PER_LANE, precision=precision)
+    tol = 1e-13 if precision == L.PRECISION_FP64 else 1e-6
+    assert np.all(np.isfinite(wave_sr)) and rel_l2(wave_sr, lane_sr) <= tol
+    assert rel_l2(wave_acc, lane_acc) <= (1e-12 if precision == L.PRECISION_FP64 else 1e-5)
