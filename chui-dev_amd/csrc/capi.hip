@@ -515,13 +515,16 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   wp.identity_targets = identity;
   // the mixed walk's counting variant also produces the nearest-pair lane
   wp.count = (diag != nullptr || wcount != nullptr || (mixed && near2 != nullptr)) ? 1 : 0;
-  // one target per wave when asked, or when few sources are active (at most
-  // 1/64 in mixed precision, 1/16 in FP64): 64 visit-ordered targets then
-  // lie far apart and a shared walk covers the union of their
-  // neighbourhoods. Measured crossovers: mixed 2-3% active at 256^3 and
-  // about 35% at 64^3; FP64, whose pairs cost more, about 15% at 128^3.
-  const uint64_t sparse_ratio = mixed ? 64 : 16;
-  const bool sparse = n_active * sparse_ratio <= n;
+  // one target per wave when asked, or when few sources are active: 64
+  // visit-ordered targets then lie far apart and a shared walk covers the
+  // union of their neighbourhoods, costing about a full walk whatever the
+  // active count. The measured short-range-only crossovers fall with the
+  // source count: mixed 2-3% active at 256^3 (clustered, 1065 pairs per
+  // target), 12% at 128^3 and 35% at 64^3; FP64 15% at 128^3. Auto switches
+  // at f(n) = min(0.25, f0 (2^24 / n)^0.75), f0 = 0.025 (mixed) or 0.03 (FP64).
+  const double f0 = mixed ? 0.025 : 0.03;
+  const double f_switch = std::min(0.25, f0 * std::pow(16777216.0 / static_cast<double>(std::max<uint64_t>(n, 1)), 0.75));
+  const bool sparse = static_cast<double>(n_active) <= f_switch * static_cast<double>(n);
   // (strict walks without counters; FP64 also without the nearest-pair lane)
   wp.target_per_wave = !wp.count && (mixed || near2 == nullptr) &&
                        (o->target_mapping == CHUI_TARGETS_PER_WAVE ||

@@ -91,8 +91,9 @@ typedef enum chui_acceptance_policy {
  * Walks with counters or (FP64) the nearest-pair lane, and the adaptive
  * walks, always run one target per lane. */
 typedef enum chui_target_mapping {
-  CHUI_TARGETS_AUTO = 0,      /* per wave when at most 1/64 (mixed) or 1/16 (FP64) of the
-                                 sources are active, else per lane */
+  CHUI_TARGETS_AUTO = 0,      /* per wave when at most a fraction min(0.25, f0 (2^24/n)^0.75)
+                                 of the n sources is active (f0 = 0.025 mixed, 0.03 FP64),
+                                 else per lane */
   CHUI_TARGETS_PER_LANE = 1,  /* 64 visit-ordered targets per wave sharing one walk */
   CHUI_TARGETS_PER_WAVE = 2   /* one target per wave: lanes split its nodes and pairs */
 } chui_target_mapping;

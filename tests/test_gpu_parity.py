@@ -210,8 +210,10 @@ def test_active_subset_equals_full_rows():
     # The short-range walk is deterministic per target, so the tree-only lanes
     # (source-row indexed: only the active rows are written) match bit for
     # bit; the PM deposit uses FP64 atomics, so totals agree to round-off only.
+    # (the per-lane mapping on both sides: a per-wave walk sums in another order)
     w = ics.config(0, n_side=16)
     opts = options_struct(w.box, eps=w.softening)
+    opts.target_mapping = L.TARGETS_PER_LANE
     full, _, full_sr, _, _ = gpu_solve(w.x, w.y, w.z, w.mass, opts, w.mesh)
     rng = np.random.default_rng(5)
     active = rng.choice(w.x.size, 777, replace=False).astype(np.uint32)
