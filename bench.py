@@ -256,10 +256,11 @@ def main():
 
 
 def measure_hierarchical(max_bin, n_side=128):
-    """One hierarchical KDK block (mixed precision) on the 128^3 workload law
+    """Hierarchical KDK blocks (mixed precision) on the 128^3 workload law
     at a = 0.5 (velocities from its linear field), block span 2^M times the
-    smallest gravity interval: wall time, fine-tick solves and the active
-    fraction, beside the cost of 2^M global steps at the quantum."""
+    smallest gravity interval: median wall time of three consecutive blocks
+    after two warm-up blocks, fine-tick solves and the active fraction of the
+    last, beside the cost of 2^M global steps at the quantum (full solves)."""
     import torch
 
     import cosmosim._lib as L
@@ -276,14 +277,19 @@ def measure_hierarchical(max_bin, n_side=128):
     u = dev([w.vx / a0, w.vy / a0, w.vz / a0])
     m = torch.tensor(w.mass, device="cuda:0")
     kdk = H.HierarchicalKdk(w.mesh, x, u, m, o, cosmo, max_bin)
+    timed = 3
     try:
-        first = kdk.block(a0)  # warm-up: sizes the buffers and advances the state one block
+        plan = kdk.block(a0)  # warm-up blocks: size the buffers and advance the state
+        plan = kdk.block(plan.scale[-1])
         solves0, sr0 = kdk.solves, kdk.short_range_solves
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        plan = kdk.block(first.scale[-1])
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t
+        times = []
+        for _ in range(timed):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            plan = kdk.block(plan.scale[-1])
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        el = float(np.median(times))  # a block that first grows a buffer pays hipMalloc once
     finally:
         kdk.close()
     counts = np.bincount(plan.bins.cpu().numpy(), minlength=max_bin + 1)
@@ -296,12 +302,15 @@ def measure_hierarchical(max_bin, n_side=128):
         coord.solve_active_set(*x, m, ForceAccumulator(acc), o)
         coord.synchronize()
         t = time.perf_counter()
-        coord.solve_active_set(*x, m, ForceAccumulator(acc), o)
+        for _ in range(timed):
+            coord.solve_active_set(*x, m, ForceAccumulator(acc), o)
         coord.synchronize()
-        full = time.perf_counter() - t
+        full = (time.perf_counter() - t) / timed
     return {"workload": w.name, "particles": int(w.x.size), "max_bin": max_bin, "block_seconds": el,
+            "blocks_timed": timed, "block_seconds_each": times,
             "bins": counts.tolist(), "quantum_code_time": plan.quantum,
-            "synchronized_solves": kdk.solves - solves0, "short_range_solves": kdk.short_range_solves - sr0,
+            "synchronized_solves": (kdk.solves - solves0) // timed,
+            "short_range_solves": (kdk.short_range_solves - sr0) // timed,
             "fine_tick_targets": active_targets, "full_solve_seconds": full,
             "global_steps_at_quantum_seconds": periods * full,
             "speedup_vs_global_at_quantum": periods * full / el}
