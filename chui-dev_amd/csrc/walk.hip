@@ -1630,6 +1630,54 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
 constexpr int kTargetList = 1024;  // listed sources per wave before an evaluation
 constexpr int kTargetNodes = 128;  // listed multipoles per wave before an evaluation
 
+// The tree walk of the per-wave kernels: the lanes classify preorder nodes
+// p .. p + 63 (classify(q, next, lb, lc) returns 0 pruned, 1 kept leaf with
+// sources [lb, lb + lc), 2 accepted node or 3 opened, and sets next to q + 1
+// to descend or q + skip to move past the subtree), a scalar loop follows the
+// visited chain through the window with readlane, and the chain's kept
+// leaves' sources and accepted nodes are appended to the wave's LDS lists
+// (list: kTargetList entries, mps: kTargetNodes). evaluate() consumes the
+// lists and resets nl and nm; it runs whenever a list would overflow and at
+// the end.
+template <class Classify, class Evaluate>
+__device__ __forceinline__ void walk_target_wave(uint32_t node_count, int lane, uint32_t* list, uint32_t* mps,
+                                                 uint32_t& nl, uint32_t& nm, Classify&& classify,
+                                                 Evaluate&& evaluate) {
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t p = 0;
+  while (p < node_count) {
+    const uint32_t q = p + lane;
+    uint32_t next = q + 1, lb = 0, lc = 0;
+    const int kind = q < node_count ? classify(q, next, lb, lc) : 3;
+    unsigned long long chain = 0;
+    uint32_t cur = p;
+    while (cur < p + kWave && cur < node_count) {
+      const uint32_t l = cur - p;
+      chain |= 1ull << l;
+      cur = readlane_u32(next, l);
+    }
+    p = cur;
+    const bool on = (chain >> lane) & 1ull;
+    const unsigned long long mm = __ballot(on && kind == 2);
+    if (nm + static_cast<uint32_t>(__popcll(mm)) > static_cast<uint32_t>(kTargetNodes)) evaluate();
+    if (on && kind == 2) mps[nm + __popcll(mm & below)] = q;
+    nm += __popcll(mm);
+    unsigned long long rest = __ballot(on && kind == 1);
+    while (rest) {
+      const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(rest)) - 1);
+      rest &= rest - 1ull;
+      const uint32_t b0 = readlane_u32(lb, l), c = readlane_u32(lc, l);
+      for (uint32_t c0 = 0; c0 < c; c0 += kWave) {
+        const uint32_t cn = c - c0 < static_cast<uint32_t>(kWave) ? c - c0 : static_cast<uint32_t>(kWave);
+        if (nl + cn > static_cast<uint32_t>(kTargetList)) evaluate();
+        if (static_cast<uint32_t>(lane) < cn) list[nl + lane] = b0 + c0 + lane;
+        nl += cn;
+      }
+    }
+  }
+  evaluate();
+}
+
 template <int SCHEME, bool CUBIC>
 __global__ __launch_bounds__(kWalkThreads) void walkmix_target_kernel(WalkParams P, WalkData D, WalkOutputs out) {
   __shared__ float4 htab[kHTableCells];
@@ -1654,7 +1702,6 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_target_kernel(WalkParams
   const float prune2 = P.prune2_sep;
   const float u0 = P.tab_scale[0], u1 = P.tab_scale[1], u2 = P.tab_scale[2];
   const float rc2 = P.rcut2_tab, eps2 = P.eps2_tab, c8 = P.c8_tab;
-  const unsigned long long below = (1ull << lane) - 1ull;
 
   float lx = 0.0f, ly = 0.0f, lz = 0.0f;  // pair sums (force-scale units)
   float mx = 0.0f, my = 0.0f, mz = 0.0f;  // multipole accelerations
@@ -1694,64 +1741,33 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_target_kernel(WalkParams
     nm = 0;
   };
 
-  uint32_t p = 0;
-  while (p < P.node_count) {
-    // classify preorder nodes p .. p + 63 for this target
-    const uint32_t q = p + lane;
-    uint32_t next = q + 1, lb = 0, lc = 0;
-    int kind = 3;  // 0 pruned, 1 kept leaf, 2 accepted node, 3 opened
-    if (q < P.node_count) {
-      const NodeLink lk = D.link[q];
-      const NodeBox32 b = D.box32[q];
-      const float ax_ = fabsf(fx_delta(b.cx, px, s0)), ay_ = fabsf(fx_delta(b.cy, py, s1)),
-                  az_ = fabsf(fx_delta(b.cz, pz, s2));
-      const float ex = fmaxf(0.0f, ax_ - b.hx), ey = fmaxf(0.0f, ay_ - b.hy), ez = fmaxf(0.0f, az_ - b.hz);
-      const bool keep = !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
-      if (lk.child_count == 0) {
-        kind = keep ? 1 : 0;
-        lb = lk.begin;
-        lc = lk.count;
-      } else if (!keep) {
-        kind = 0;
-        next = q + lk.skip;
-      } else {
-        bool take = false;
-        if (b.cell_half < P.accept_half_max) {
-          float mfx, mfy, mfz;
-          take = strict_accept32(P, F, D.geo32[q], D.mom32[q], D.quad32, q, px, py, pz, mfx, mfy, mfz);
-        }
-        kind = take ? 2 : 3;
-        next = take ? q + lk.skip : q + 1;
-      }
+  // walkmix_kernel's tests: the prune against the tight particle box and the
+  // strict acceptance of small internal nodes
+  auto classify = [&](uint32_t q, uint32_t& next, uint32_t& lb, uint32_t& lc) -> int {
+    const NodeLink lk = D.link[q];
+    const NodeBox32 b = D.box32[q];
+    const float ax_ = fabsf(fx_delta(b.cx, px, s0)), ay_ = fabsf(fx_delta(b.cy, py, s1)),
+                az_ = fabsf(fx_delta(b.cz, pz, s2));
+    const float ex = fmaxf(0.0f, ax_ - b.hx), ey = fmaxf(0.0f, ay_ - b.hy), ez = fmaxf(0.0f, az_ - b.hz);
+    const bool keep = !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
+    if (lk.child_count == 0) {
+      lb = lk.begin;
+      lc = lk.count;
+      return keep ? 1 : 0;
     }
-    // the visited chain through the window
-    unsigned long long chain = 0;
-    uint32_t cur = p;
-    while (cur < p + kWave && cur < P.node_count) {
-      const uint32_t l = cur - p;
-      chain |= 1ull << l;
-      cur = readlane_u32(next, l);
+    if (!keep) {
+      next = q + lk.skip;
+      return 0;
     }
-    p = cur;
-    const bool on = (chain >> lane) & 1ull;
-    const unsigned long long mm = __ballot(on && kind == 2);
-    if (nm + static_cast<uint32_t>(__popcll(mm)) > static_cast<uint32_t>(kTargetNodes)) evaluate();
-    if (on && kind == 2) mps[w][nm + __popcll(mm & below)] = q;
-    nm += __popcll(mm);
-    unsigned long long rest = __ballot(on && kind == 1);
-    while (rest) {
-      const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(rest)) - 1);
-      rest &= rest - 1ull;
-      const uint32_t b0 = readlane_u32(lb, l), c = readlane_u32(lc, l);
-      for (uint32_t c0 = 0; c0 < c; c0 += kWave) {
-        const uint32_t cn = c - c0 < static_cast<uint32_t>(kWave) ? c - c0 : static_cast<uint32_t>(kWave);
-        if (nl + cn > static_cast<uint32_t>(kTargetList)) evaluate();
-        if (static_cast<uint32_t>(lane) < cn) list[w][nl + lane] = b0 + c0 + lane;
-        nl += cn;
-      }
+    bool take = false;
+    if (b.cell_half < P.accept_half_max) {
+      float mfx, mfy, mfz;
+      take = strict_accept32(P, F, D.geo32[q], D.mom32[q], D.quad32, q, px, py, pz, mfx, mfy, mfz);
     }
-  }
-  evaluate();
+    next = take ? q + lk.skip : q + 1;
+    return take ? 2 : 3;
+  };
+  walk_target_wave(P.node_count, lane, list[w], mps[w], nl, nm, classify, evaluate);
   const double gm = P.force_scale;
   const double sx = gm * wave_sum(static_cast<double>(lx)) + wave_sum(static_cast<double>(mx));
   const double sy = gm * wave_sum(static_cast<double>(ly)) + wave_sum(static_cast<double>(my));
@@ -1797,7 +1813,6 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_target_kernel(WalkParams 
   const double4 t = D.src64[tp];
   const double px = t.x, py = t.y, pz = t.z;
   const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
-  const unsigned long long below = (1ull << lane) - 1ull;
 
   double ax = 0.0, ay = 0.0, az = 0.0;
   uint32_t nl = 0, nm = 0;
@@ -1828,63 +1843,33 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_target_kernel(WalkParams 
     nm = 0;
   };
 
-  uint32_t p = 0;
-  while (p < P.node_count) {
-    const uint32_t q = p + lane;
-    uint32_t next = q + 1, lb = 0, lc = 0;
-    int kind = 3;  // 0 pruned, 1 kept leaf, 2 accepted node, 3 opened
-    if (q < P.node_count) {
-      const NodeLink lk = D.link[q];
-      const NodeGeo64 g = D.geo64[q];
-      const double h = g.half;
-      const double ax_ = fabs(mi64(g.cx - px, L0)), ay_ = fabs(mi64(g.cy - py, L1)), az_ = fabs(mi64(g.cz - pz, L2));
-      const double ex = fmax(0.0, ax_ - h), ey = fmax(0.0, ay_ - h), ez = fmax(0.0, az_ - h);
-      const bool keep = within_cutoff(ex * ex + ey * ey + ez * ez, P.rcut, P.rcut2);
-      if (lk.child_count == 0) {
-        kind = keep ? 1 : 0;
-        lb = lk.begin;
-        lc = lk.count;
-      } else if (!keep) {
-        kind = 0;
-        next = q + lk.skip;
-      } else {
-        const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
-        const double mx = ax_ + h, my = ay_ + h, mz = az_ + h;
-        const bool contained = within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
-        const NodeMom64 M = D.mom64[q];
-        const double mdx = mi64(M.comx - px, L0), mdy = mi64(M.comy - py, L1), mdz = mi64(M.comz - pz, L2);
-        const bool take = accept64(h, D.quad64[q].offset, inside, contained, mdx * mdx + mdy * mdy + mdz * mdz, P);
-        kind = take ? 2 : 3;
-        next = take ? q + lk.skip : q + 1;
-      }
+  // walk64_kernel's tests: the prune against the octree cell and accept64
+  auto classify = [&](uint32_t q, uint32_t& next, uint32_t& lb, uint32_t& lc) -> int {
+    const NodeLink lk = D.link[q];
+    const NodeGeo64 g = D.geo64[q];
+    const double h = g.half;
+    const double ax_ = fabs(mi64(g.cx - px, L0)), ay_ = fabs(mi64(g.cy - py, L1)), az_ = fabs(mi64(g.cz - pz, L2));
+    const double ex = fmax(0.0, ax_ - h), ey = fmax(0.0, ay_ - h), ez = fmax(0.0, az_ - h);
+    const bool keep = within_cutoff(ex * ex + ey * ey + ez * ez, P.rcut, P.rcut2);
+    if (lk.child_count == 0) {
+      lb = lk.begin;
+      lc = lk.count;
+      return keep ? 1 : 0;
     }
-    unsigned long long chain = 0;
-    uint32_t cur = p;
-    while (cur < p + kWave && cur < P.node_count) {
-      const uint32_t l = cur - p;
-      chain |= 1ull << l;
-      cur = readlane_u32(next, l);
+    if (!keep) {
+      next = q + lk.skip;
+      return 0;
     }
-    p = cur;
-    const bool on = (chain >> lane) & 1ull;
-    const unsigned long long mm = __ballot(on && kind == 2);
-    if (nm + static_cast<uint32_t>(__popcll(mm)) > static_cast<uint32_t>(kTargetNodes)) evaluate();
-    if (on && kind == 2) mps[w][nm + __popcll(mm & below)] = q;
-    nm += __popcll(mm);
-    unsigned long long rest = __ballot(on && kind == 1);
-    while (rest) {
-      const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(rest)) - 1);
-      rest &= rest - 1ull;
-      const uint32_t b0 = readlane_u32(lb, l), c = readlane_u32(lc, l);
-      for (uint32_t c0 = 0; c0 < c; c0 += kWave) {
-        const uint32_t cn = c - c0 < static_cast<uint32_t>(kWave) ? c - c0 : static_cast<uint32_t>(kWave);
-        if (nl + cn > static_cast<uint32_t>(kTargetList)) evaluate();
-        if (static_cast<uint32_t>(lane) < cn) list[w][nl + lane] = b0 + c0 + lane;
-        nl += cn;
-      }
-    }
-  }
-  evaluate();
+    const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+    const double mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+    const bool contained = within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
+    const NodeMom64 M = D.mom64[q];
+    const double mdx = mi64(M.comx - px, L0), mdy = mi64(M.comy - py, L1), mdz = mi64(M.comz - pz, L2);
+    const bool take = accept64(h, D.quad64[q].offset, inside, contained, mdx * mdx + mdy * mdy + mdz * mdz, P);
+    next = take ? q + lk.skip : q + 1;
+    return take ? 2 : 3;
+  };
+  walk_target_wave(P.node_count, lane, list[w], mps[w], nl, nm, classify, evaluate);
   ax = wave_sum(ax);
   ay = wave_sum(ay);
   az = wave_sum(az);
