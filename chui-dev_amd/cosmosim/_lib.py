@@ -54,6 +54,14 @@ class ChuiDeviceError(ChuiError):
     pass
 
 
+class ChuiRuntimeError(ChuiError):
+    """std::runtime_error of the reference (cadence / field identity violations)."""
+
+
+class ChuiLogicError(ChuiError):
+    """std::logic_error of the reference (a broken internal invariant)."""
+
+
 class ChuiOutOfMemory(ChuiError, MemoryError):
     pass
 
@@ -62,6 +70,8 @@ _ERRORS = {
     CHUI_ERR_INVALID_ARGUMENT: ChuiInvalidArgument,
     CHUI_ERR_OVERFLOW: ChuiOverflow,
     CHUI_ERR_UNSUPPORTED: ChuiUnsupported,
+    CHUI_ERR_RUNTIME: ChuiRuntimeError,
+    CHUI_ERR_LOGIC: ChuiLogicError,
     CHUI_ERR_DEVICE: ChuiDeviceError,
     CHUI_ERR_OUT_OF_MEMORY: ChuiOutOfMemory,
 }
@@ -89,6 +99,12 @@ class Options(C.Structure):
         ("residual_block_size", C.c_int32),
         ("precision", C.c_int32),
         ("short_range_only", C.c_int32),
+        ("source_generation", C.c_uint64),
+        ("pm_field_version", C.c_uint64),
+        ("force_epoch_sequence", C.c_uint64),
+        ("force_epoch_scale_factor", C.c_double),
+        ("allow_stale_long_range_field", C.c_int32),
+        ("reserved1", C.c_int32),
     ]
 
 
@@ -230,6 +246,7 @@ SIGNATURES = {
     "chui_treepm_ewald_device": [_P, _P, _P, _P, _P, _U64, _P, _U64, C.POINTER(_D), _D, _D, _I32, _I32, _D,
                                  _P, _P, _P],
     "chui_treepm_split_diagnostics": [_D, _D, _D, C.POINTER(SplitDiagnostics)],
+    "chui_treepm_debug_inject_walk_fault": [_P],
     "chui_treepm_last_error": [],
     "chui_treepm_abi_version": [],
 }

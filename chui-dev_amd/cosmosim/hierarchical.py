@@ -177,7 +177,9 @@ class HierarchicalKdk:
         self.x = [t.contiguous() for t in x]
         self.u = [t.contiguous() for t in u]
         self.m = mass.contiguous()
-        self.options = options
+        # a private copy: the fine ticks switch short_range_only on it, and the
+        # caller's options must stay as given even when a solve raises
+        self.options = _lib.Options.from_buffer_copy(options)
         self.cosmo = cosmo
         self.max_bin = max_bin
         self.eta = eta

@@ -159,8 +159,7 @@ ADAPTIVE_DEFAULTS = {
 # distributed exchange sizes, caller-side cadence)
 IGNORED = {
     "numerics.treepm_full_mac_diagnostics", "numerics.treepm_leaf_simd_enabled",
-    "numerics.treepm_identical_source_tree_reuse_enabled", "numerics.treepm_update_cadence_steps",
-    "numerics.treepm_pm_reuse_policy", "numerics.treepm_multirate_rollback_memory_policy",
+    "numerics.treepm_identical_source_tree_reuse_enabled", "numerics.treepm_multirate_rollback_memory_policy",
     "numerics.treepm_allow_diagnostic_naive_dft", "numerics.treepm_pm_decomposition_mode",
     "numerics.treepm_tree_exchange_batch_bytes", "numerics.treepm_pm_exchange_batch_bytes",
 }
@@ -179,6 +178,11 @@ class TreePmDeck:
     gravitational_constant: float
     applied: list = field(default_factory=list)   # deck keys that set a field
     ignored: list = field(default_factory=list)   # numerics.treepm_* keys with no effect on the forces
+    # PM cadence of the caller's integrator (numerics.treepm_update_cadence_steps)
+    # and the reuse policy; stale_within_cadence sets
+    # options.allow_stale_long_range_field
+    update_cadence_steps: int = 1
+    pm_reuse_policy: str = "exact"
 
 
 def _mesh(entries) -> tuple:
@@ -210,6 +214,35 @@ def _box_mpc(entries, length_unit: str) -> tuple:
         return tuple(length_mpc(entries[k][0], length_unit, k) for k in axes)
     v = length_mpc(entries.get("cosmology.box_size", ("50.0", 0))[0], length_unit, "cosmology.box_size")
     return (v, v, v)
+
+
+def _int_key(entries, key: str, default: str) -> int:
+    value = entries.get(key, (default, 0))[0]
+    try:
+        return int(value)
+    except ValueError:
+        raise ParamError(f"key '{key}': expected an integer, got '{value}'") from None
+
+
+def _pm_cadence(entries) -> tuple:
+    """(policy, cadence) with the reference's checks (config.cpp:987-996, 3410-3434)."""
+    policy = entries.get("numerics.treepm_pm_reuse_policy", ("exact", 0))[0].strip().lower()
+    if policy not in ("exact", "stale_within_cadence", "multirate_outer_kick"):
+        raise ParamError(f"key 'numerics.treepm_pm_reuse_policy': invalid value '{policy}' "
+                         "(supported: exact, stale_within_cadence, multirate_outer_kick)")
+    cadence = _int_key(entries, "numerics.treepm_update_cadence_steps", "1")
+    max_rung = _int_key(entries, "numerics.hierarchical_max_rung", "0")
+    if policy == "exact" and cadence != 1:
+        raise ParamError("numerics.treepm_update_cadence_steps must be exactly 1 with treepm_pm_reuse_policy=exact; "
+                         "select treepm_pm_reuse_policy=stale_within_cadence for experimental PM reuse")
+    if policy == "stale_within_cadence" and (cadence < 2 or cadence > 64 or max_rung != 0):
+        raise ParamError("numerics.treepm_pm_reuse_policy=stale_within_cadence requires treepm_update_cadence_steps "
+                         "in [2,64] and global KDK (hierarchical_max_rung=0; hierarchical KDK already applies PM at "
+                         "coarse endpoints)")
+    if policy == "multirate_outer_kick" and (cadence < 2 or cadence > 64 or max_rung <= 0):
+        raise ParamError("numerics.treepm_pm_reuse_policy=multirate_outer_kick requires treepm_update_cadence_steps "
+                         "in [2,64] (coarse blocks per PM cycle) and hierarchical KDK (hierarchical_max_rung > 0)")
+    return policy, cadence
 
 
 def load_treepm_deck(path, precision: int = _lib.PRECISION_FP64) -> TreePmDeck:
@@ -249,6 +282,10 @@ def treepm_deck(text: str, precision: int = _lib.PRECISION_FP64) -> TreePmDeck:
         _lib.set_adaptive_param(ad, key, value)
         if key in entries:
             deck.applied.append(key)
+    deck.pm_reuse_policy, deck.update_cadence_steps = _pm_cadence(entries)
+    _lib.set_param(opts, "treepm_pm_reuse_policy", deck.pm_reuse_policy)
+    deck.applied += [k for k in ("numerics.treepm_pm_reuse_policy", "numerics.treepm_update_cadence_steps")
+                     if k in entries]
     for key, default in UNSUPPORTED.items():
         if key in entries and entries[key][0].lower() != default:
             raise _lib.ChuiUnsupported(_lib.CHUI_ERR_UNSUPPORTED,

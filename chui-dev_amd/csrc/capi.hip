@@ -66,6 +66,13 @@ struct chui_treepm_context {
   double prof_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t prof_solves = 0;
   std::vector<float> host_float;
+  // Walk traversal fault: the walks set counters[kWalkCounters] (sticky: the
+  // per-solve reset leaves it) when a wave's lanes all left the tree early.
+  // Every solve copies it to pinned host memory behind fault_ev; it is
+  // checked at the next synchronisation through the library or the next solve.
+  uint64_t* fault_host = nullptr;
+  hipEvent_t fault_ev = nullptr;
+  bool fault_pending = false;
 };
 
 namespace {
@@ -262,6 +269,51 @@ void record(Ctx* c, Stage st) {
   if (c->events) CHUI_HIP(hipEventRecord(c->ev[st], c->stream));
 }
 
+// Walk counters with the sticky traversal-fault word after them; allocated and
+// zeroed once, so the fault word survives the per-solve counter resets.
+unsigned long long* walk_counters(Ctx* c, hipStream_t s) {
+  if (!c->counters.ptr) {
+    c->counters.reserve((chui::kWalkCounters + 1) * sizeof(unsigned long long));
+    chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounters + 1, s);
+  }
+  chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounters, s);
+  return reinterpret_cast<unsigned long long*>(c->counters.ptr);
+}
+
+// Copies the fault word to pinned host memory after the stream's prior work.
+void post_fault_word(Ctx* c, hipStream_t s) {
+  if (!c->fault_host) {
+    void* p = nullptr;
+    CHUI_HIP(hipHostMalloc(&p, 64, hipHostMallocDefault));
+    c->fault_host = static_cast<uint64_t*>(p);
+    *c->fault_host = 0;
+  }
+  if (!c->fault_ev) CHUI_HIP(hipEventCreateWithFlags(&c->fault_ev, hipEventDisableTiming));
+  CHUI_HIP(hipMemcpyAsync(c->fault_host, c->counters.as<unsigned long long>() + chui::kWalkCounters,
+                          sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CHUI_HIP(hipEventRecord(c->fault_ev, s));
+  c->fault_pending = true;
+}
+
+// Reports a posted traversal fault as std::logic_error (CHUI_ERR_LOGIC). With
+// wait = false a copy still in flight stays pending for a later check.
+void check_fault(Ctx* c, bool wait) {
+  if (!c->fault_pending) return;
+  if (wait) {
+    CHUI_HIP(hipEventSynchronize(c->fault_ev));
+  } else {
+    const hipError_t q = hipEventQuery(c->fault_ev);
+    if (q == hipErrorNotReady) return;
+    CHUI_HIP(q);
+  }
+  c->fault_pending = false;
+  if (*c->fault_host != 0) {
+    *c->fault_host = 0;
+    chui::fill_u64(c->counters.as<uint64_t>() + chui::kWalkCounters, 0ull, 1, c->stream);
+    throw std::logic_error("TreePM walk lost progress (wave reduction selected no pending node)");
+  }
+}
+
 float elapsed(Ctx* c, Stage a, Stage b) {
   float ms = 0.0f;
   CHUI_HIP(hipEventElapsedTime(&ms, c->ev[a], c->ev[b]));
@@ -401,6 +453,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     throw std::invalid_argument("short-range split surface cannot request a PM refresh/capture or zoom correction");
   hipStream_t s = c->stream;
   harvest(c);
+  check_fault(c, false);
   c->events = diag != nullptr || c->profiling;
   if (c->events && !c->ev[0])
     for (auto& e : c->ev) CHUI_HIP(hipEventCreate(&e));
@@ -454,10 +507,20 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   const double key[8] = {o->box_size[0], o->box_size[1], o->box_size[2], o->gravitational_constant, sp.a,
                          static_cast<double>(o->assignment_scheme), static_cast<double>(o->window_deconvolution),
                          o->scale_factor};
-  const bool key_match = pw.valid && std::equal(key, key + 8, pw.key);
-  if (!sr_only && (refresh || !key_match)) {
-    if (!refresh)
-      throw std::runtime_error("TreePM long-range field reuse requested without a matching field (refresh required)");
+  // tree_pm_coupling.cpp:2483-2506: exact reuse needs the same source state
+  // (generation and force epoch); the stale window admits any earlier one.
+  // The field version and the mesh-level options must match either way.
+  const bool source_admitted =
+      o->allow_stale_long_range_field != 0 ||
+      (pw.source_generation == o->source_generation && pw.epoch_sequence == o->force_epoch_sequence &&
+       pw.epoch_scale_factor == o->force_epoch_scale_factor);
+  const bool key_match =
+      pw.valid && source_admitted && pw.field_version == o->pm_field_version && std::equal(key, key + 8, pw.key);
+  if (!sr_only && !refresh && !key_match) {
+    if (!pw.valid) throw std::runtime_error("TreePM long-range mesh field is unavailable for reuse");
+    throw std::runtime_error("TreePM long-range reuse requested without a compatible PM field");
+  }
+  if (!sr_only && refresh) {
     chui::DepositTiling dt{};
     for (int a = 0; a < 3; ++a) {
       dt.n[a] = g.n[a];
@@ -473,6 +536,10 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       chui::pm_solve_mixed(pw, g, po, c->src32.as<uint4>(), uniform ? 1 : 0, static_cast<float>(um), n, units, s);
     }
     std::copy(key, key + 8, pw.key);
+    pw.source_generation = o->source_generation;
+    pw.field_version = o->pm_field_version;
+    pw.epoch_sequence = o->force_epoch_sequence;
+    pw.epoch_scale_factor = o->force_epoch_scale_factor;
   }
   record(c, kEvPm);
 
@@ -507,9 +574,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   record(c, kEvTargets);
 
   // --- walk (+ fused PM interpolation) ---
-  c->counters.reserve((chui::kWalkCounters + 1) * sizeof(unsigned long long));
-  chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounters + 1, s);
-  const bool check_walk_progress_ = diag != nullptr;
+  unsigned long long* const walk_cnt = walk_counters(c, s);
   chui::WalkParams wp = base_walk_params(c, o, sp, g.row);
   wp.n_targets = static_cast<uint32_t>(n_active);
   wp.identity_targets = identity;
@@ -573,7 +638,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     const double* f0 = static_cast<const double*>(pw.force[0]);
     const double* f1 = static_cast<const double*>(pw.force[1]);
     const double* f2 = static_cast<const double*>(pw.force[2]);
-    auto* wcnt = reinterpret_cast<unsigned long long*>(c->counters.ptr);
+    auto* wcnt = walk_cnt;
     if (group_status == CHUI_GROUP_WALK_ACTIVE) {
       c->leaf_of_pos.reserve(n * sizeof(uint32_t));
       chui::build_leaf_of_pos(c->tree, c->leaf_of_pos.as<uint32_t>(), s);
@@ -629,8 +694,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     if (!mixed) {
       chui::walk_fp64(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, target_pos, target_slot,
                       static_cast<const double*>(pw.force[0]), static_cast<const double*>(pw.force[1]),
-                      static_cast<const double*>(pw.force[2]), out,
-                      reinterpret_cast<unsigned long long*>(c->counters.ptr), s);
+                      static_cast<const double*>(pw.force[2]), out, walk_cnt, s);
     } else {
       const double t_max = wp.rcut2 * (0.25 / (sp.a * sp.a)) * (1.0 + 1.0e-3);
       std::vector<float> tab = chui::gaussian_coefficient_table(wp, t_max);
@@ -657,18 +721,18 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       }
       chui::walk_mixed(c->tree, wp, c->src32.as<uint4>(), uniform ? 1 : 0, c->htab.as<float4>(), target_pos,
                        target_slot, static_cast<const float*>(pw.force[0]), static_cast<const float*>(pw.force[1]),
-                       static_cast<const float*>(pw.force[2]), out,
-                       reinterpret_cast<unsigned long long*>(c->counters.ptr), s);
+                       static_cast<const float*>(pw.force[2]), out, walk_cnt, s);
     }
   }
   record(c, kEvWalk);
+  post_fault_word(c, s);
+  if (wcount || diag) check_fault(c, true);
   if (wcount) {
     unsigned long long wc[chui::kWalkCounters + 1] = {};
     unsigned long long gc[chui::kGroupCounters] = {};
     CHUI_HIP(hipMemcpyAsync(wc, c->counters.ptr, sizeof(wc), hipMemcpyDeviceToHost, s));
     CHUI_HIP(hipMemcpyAsync(gc, c->gcounters.ptr, sizeof(gc), hipMemcpyDeviceToHost, s));
     CHUI_HIP(hipStreamSynchronize(s));
-    if (wc[chui::kWalkCounters]) throw std::logic_error("TreePM walk lost progress (wave reduction selected no pending node)");
     chui_treepm_walk_counters r{};
     r.struct_size = sizeof(r);
     r.group_walk_status = group_status;
@@ -689,14 +753,6 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     r.group_walk_list_entries_max = gc[14];
     *wcount = r;
   }
-  if (check_walk_progress_) {
-    unsigned long long bad = 0;
-    CHUI_HIP(hipMemcpyAsync(&bad, reinterpret_cast<unsigned long long*>(c->counters.ptr) + chui::kWalkCounters,
-                            sizeof(bad), hipMemcpyDeviceToHost, s));
-    CHUI_HIP(hipStreamSynchronize(s));
-    if (bad) throw std::logic_error("TreePM walk lost progress (wave reduction selected no pending node)");
-  }
-
   if (diag) {
     if (diag->struct_size != sizeof(chui_treepm_diagnostics))
       throw std::invalid_argument("chui_treepm_diagnostics.struct_size does not match this library");
@@ -765,6 +821,7 @@ chui_status chui_treepm_default_options(chui_treepm_options* o) {
     o->acceptance_policy = CHUI_ACCEPT_STRICT;
     o->residual_block_size = 64;
     o->precision = CHUI_PRECISION_FP64;
+    o->force_epoch_scale_factor = 1.0;            // ForceEvaluationEpoch default
   });
 }
 
@@ -849,6 +906,14 @@ chui_status chui_treepm_options_set(chui_treepm_options* o, const char* key_c, c
       if (value == "fp64") o->precision = CHUI_PRECISION_FP64;
       else if (value == "mixed") o->precision = CHUI_PRECISION_MIXED;
       else throw std::invalid_argument("unknown treepm_precision: " + value);
+    } else if (key == "treepm_pm_reuse_policy") {
+      // config.cpp:987-996; gravity_runtime.cpp:564-565 admits stale fields
+      // only under stale_within_cadence
+      if (value == "exact" || value == "multirate_outer_kick") o->allow_stale_long_range_field = 0;
+      else if (value == "stale_within_cadence") o->allow_stale_long_range_field = 1;
+      else
+        throw std::invalid_argument("key 'numerics.treepm_pm_reuse_policy': invalid value '" + value +
+                                    "' (supported: exact, stale_within_cadence, multirate_outer_kick)");
     } else if (key == "treepm_pm_force_gradient") {
       if (value != "spectral") throw chui::UnsupportedError("only the spectral PM gradient is built");
     } else if (key == "treepm_pm_grid" || key == "treepm_pm_grid_nx" || key == "treepm_pm_grid_ny" ||
@@ -907,6 +972,8 @@ chui_status chui_treepm_destroy(chui_treepm_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev)
       if (e) (void)hipEventDestroy(e);
+    if (c->fault_ev) (void)hipEventDestroy(c->fault_ev);
+    if (c->fault_host) (void)hipHostFree(c->fault_host);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
   });
@@ -1006,6 +1073,7 @@ chui_status chui_treepm_ensemble_solve_device(chui_treepm_ensemble* e, const chu
                             bx.accel_z, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, bx.options,
                             refresh_long_range, nullptr);
           CHUI_HIP(hipStreamSynchronize(c->stream));
+          check_fault(c, true);
         });
         if (status[b] != CHUI_OK) {
           message[b] = g_last_error;
@@ -1055,6 +1123,7 @@ chui_status chui_treepm_synchronize(chui_treepm_context* c) {
     require_ctx(c);
     set_device(c);
     CHUI_HIP(hipStreamSynchronize(c->stream));
+    check_fault(c, true);
   });
 }
 
@@ -1078,6 +1147,7 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
   if (n == 0) return;
   hipStream_t s = c->stream;
   harvest(c);
+  check_fault(c, false);
   const Split sp = split_of(*o, c->mesh);
   chui::TreeParams tp;
   tp.n = n;
@@ -1127,8 +1197,7 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
                             c->vjp_comp[2].as<double4>()};
   chui::pm_adjoint_fp64(c->vjp_adj, g, po, comp, n, units, s);
 
-  c->counters.reserve((chui::kWalkCounters + 1) * sizeof(unsigned long long));
-  chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounters + 1, s);
+  unsigned long long* const walk_cnt = walk_counters(c, s);
   chui::WalkParams wp = base_walk_params(c, o, sp, g.row);
   wp.quad = 0;
   wp.n_targets = static_cast<uint32_t>(n);
@@ -1141,7 +1210,8 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
   chui::walk_fp64_vjp(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, c->vjp_lam.as<double4>(),
                       static_cast<const double*>(c->vjp_fwd.force[0]), static_cast<const double*>(c->vjp_fwd.force[1]),
                       static_cast<const double*>(c->vjp_fwd.force[2]), c->vjp_adj.buf[0].as<double>(), out,
-                      reinterpret_cast<unsigned long long*>(c->counters.ptr), s);
+                      walk_cnt, s);
+  post_fault_word(c, s);
 }
 
 chui_status chui_treepm_force_vjp_device(chui_treepm_context* c, const double* x, const double* y, const double* z,
@@ -1293,6 +1363,7 @@ chui_status chui_treepm_solve(chui_treepm_context* c, const double* x, const dou
         for (uint64_t k = 0; k < na; ++k) hosts[i][active[k]] = value(active[k]);
       }
     }
+    check_fault(c, true);
   });
 }
 
@@ -1314,6 +1385,7 @@ chui_status chui_treepm_profile_read(chui_treepm_context* c, double stage_ms[8],
     set_device(c);
     CHUI_HIP(hipStreamSynchronize(c->stream));
     harvest(c);
+    check_fault(c, true);
     if (stage_ms) std::copy(c->prof_ms, c->prof_ms + 8, stage_ms);
     if (solves) *solves = c->prof_solves;
   });
@@ -1540,6 +1612,15 @@ chui_status chui_treepm_split_diagnostics(double asmth, double rcut_cells, doubl
       worst = std::max(worst, std::abs(S(r) + Lf(r) - 1.0));
     }
     out->max_relative_composition_error = worst;
+  });
+}
+
+chui_status chui_treepm_debug_inject_walk_fault(chui_treepm_context* c) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    walk_counters(c, c->stream);
+    chui::fill_u64(c->counters.as<uint64_t>() + chui::kWalkCounters, 1ull, 1, c->stream);
   });
 }
 

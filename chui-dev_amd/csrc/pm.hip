@@ -944,6 +944,7 @@ DepositArgs deposit_args(const PmGeometry& g, const PmOptions& o) {
 
 void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const double4* src, int uniform_mass,
                    double um, std::size_t n, const DepositUnits& units, hipStream_t s) {
+  w.valid = false;  // the force meshes are overwritten from here on
   reserve<double>(w, g);
   const Tables<double> tb = upload_tables<double>(w, g, o, s);
   double* rho = w.buf[0].as<double>();
@@ -994,6 +995,7 @@ void pm_adjoint_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, co
 
 void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, const uint4* src, int uniform_mass,
                     float um, std::size_t n, const DepositUnits& units, hipStream_t s) {
+  w.valid = false;  // the force meshes are overwritten from here on
   reserve<float>(w, g);
   const Tables<float> tb = upload_tables<float>(w, g, o, s);
   float* rho = w.buf[0].as<float>();

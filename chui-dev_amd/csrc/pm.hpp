@@ -33,8 +33,12 @@ struct PmWorkspace {
   // After a solve: real force meshes (padded stride row) per axis.
   void* force[3] = {nullptr, nullptr, nullptr};
   bool valid = false;
-  // identity of the field (cadence reuse)
+  // identity of the field (cadence reuse): mesh-level options, then the
+  // caller's source generation, field version and force epoch (sequence,
+  // scale factor)
   double key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t source_generation = 0, field_version = 0, epoch_sequence = 0;
+  double epoch_scale_factor = 0.0;
 };
 
 // Mesh-deposit work: subtrees whose particles (contiguous in tree order) all

@@ -130,6 +130,22 @@ typedef struct chui_treepm_options {
    * only the short-range residual, with no PM solve, reuse or interpolation
    * (refresh_long_range must be 0 and the PM lanes NULL) */
   int32_t short_range_only;
+  /* Identity of the long-range field (TreePmOptions::source_generation,
+   * pm_field_version, force_epoch and allow_stale_long_range_field,
+   * tree_pm_coupling.hpp:296-329). A refresh stores these tokens with the
+   * field. A reuse (refresh_long_range = 0) is admitted only when the stored
+   * field's pm_field_version, mesh and physical options match and, unless
+   * allow_stale_long_range_field is set, its source_generation and force
+   * epoch (sequence, scale factor) match too (tree_pm_coupling.cpp:2483-2506);
+   * otherwise the solve fails with CHUI_ERR_RUNTIME. The caller owns the
+   * tokens and must advance source_generation whenever positions, masses or
+   * membership change. */
+  uint64_t source_generation;
+  uint64_t pm_field_version;
+  uint64_t force_epoch_sequence;
+  double force_epoch_scale_factor;
+  int32_t allow_stale_long_range_field;  /* 0/1: core::TreePmPmReusePolicy::kStaleWithinCadence */
+  int32_t reserved1;
 } chui_treepm_options;
 
 /* Per-solve counters and timings. Counter names follow TreePmTraversalCounters
@@ -199,7 +215,10 @@ chui_status chui_treepm_synchronize(chui_treepm_context* ctx);
  * (double[n]): the active rows are written, the others left untouched, as the
  * accumulator view's source-indexed lanes (tree_pm_coupling.hpp:149-153,
  * tree_pm_coupling.cpp:1957-1966). refresh_long_range = 0 reuses the
- * previous PM mesh field (cadence) when the mesh/split/box are unchanged. */
+ * previous PM mesh field (cadence) when its identity matches (see the identity
+ * fields of chui_treepm_options), else returns CHUI_ERR_RUNTIME.
+ * This call synchronises, so a walk that broke its traversal invariant is
+ * reported here as CHUI_ERR_LOGIC. */
 chui_status chui_treepm_solve(chui_treepm_context* ctx,
                               const double* pos_x, const double* pos_y, const double* pos_z,
                               const double* mass, uint64_t n,
@@ -213,7 +232,13 @@ chui_status chui_treepm_solve(chui_treepm_context* ctx,
 /* Device-pointer form: every array is device memory already resident in HBM.
  * Accelerations are double in FP64 precision and float in MIXED precision
  * (accel_* then point to float[n_active]). Work is enqueued on the context
- * stream; the call returns after enqueue unless diagnostics != NULL. */
+ * stream; the call returns after enqueue unless diagnostics != NULL.
+ * Every walk records whether it kept its traversal invariant (a wave whose
+ * lanes all left the tree early); the flag is copied to pinned host memory at
+ * the end of the solve and checked when the caller next synchronises through
+ * the library (chui_treepm_synchronize, chui_treepm_profile_read) or starts
+ * the next solve on this context, which then return CHUI_ERR_LOGIC. A solve
+ * with diagnostics or counters checks it before it returns. */
 chui_status chui_treepm_solve_device(chui_treepm_context* ctx,
                                      const double* pos_x, const double* pos_y, const double* pos_z,
                                      const double* mass, uint64_t n,
@@ -468,6 +493,13 @@ typedef struct chui_split_diagnostics {
 } chui_split_diagnostics;
 chui_status chui_treepm_split_diagnostics(double asmth_cells, double rcut_cells, double mesh_spacing,
                                           chui_split_diagnostics* out);
+
+/* ---- testing --------------------------------------------------------------- */
+/* No reference counterpart. Sets the context's walk traversal-fault word on
+ * the device (enqueued on the context stream), as a walk whose wave lost its
+ * traversal invariant would; the next synchronising call then reports
+ * CHUI_ERR_LOGIC. Lets tests exercise that report without a faulty walk. */
+chui_status chui_treepm_debug_inject_walk_fault(chui_treepm_context* ctx);
 
 /* ---- misc ----------------------------------------------------------------- */
 const char* chui_treepm_last_error(void);

@@ -287,3 +287,32 @@ def group_tree(n):
     count = C.c_uint64()
     _check(load().oracle_last_tree_size(C.byref(count)))
     return last_tree(count.value, n)
+
+
+def probe_walk(x, y, z, m, options, mesh, target, cap=1 << 16):
+    """One strict-policy walk for the point target (not a source) over the
+    tree of the given sources: residual acceleration and the visit-order
+    lists of accepted internal nodes, opened internal nodes, summed leaves and
+    pruned nodes (preorder node numbers of last_tree())."""
+    x, y, z, m = (_f64(a) for a in (x, y, z, m))
+    acc = np.zeros(3)
+    lists = [np.zeros(cap, np.uint32) for _ in range(4)]
+    sizes = np.zeros(4, np.uint64)
+    _check(load().oracle_probe_walk(_p(x), _p(y), _p(z), _p(m), C.c_uint64(x.size), C.byref(options), _mesh(mesh),
+                                    C.c_double(target[0]), C.c_double(target[1]), C.c_double(target[2]), _p(acc),
+                                    _p(lists[0]), _p(lists[1]), _p(lists[2]), _p(lists[3]), C.c_uint64(cap),
+                                    _p(sizes)))
+    if int(sizes.max()) > cap:
+        raise RuntimeError("probe walk lists exceed their capacity")
+    names = ("accepted", "opened", "leaves", "pruned")
+    return acc, {k: lists[i][:int(sizes[i])].copy() for i, k in enumerate(names)}
+
+
+def residual_proxy(h, offset, mass, r2, eps, a, G):
+    """residualAccuracyProxy (tree_pm_acceptance.hpp:14-51) elementwise."""
+    h, offset, mass, r2 = np.broadcast_arrays(*(np.asarray(v, np.float64) for v in (h, offset, mass, r2)))
+    h, offset, mass, r2 = (np.ascontiguousarray(v) for v in (h, offset, mass, r2))
+    out = np.zeros(h.size)
+    _check(load().oracle_residual_proxy(_p(h), _p(offset), _p(mass), _p(r2), C.c_uint64(h.size), C.c_double(eps),
+                                        C.c_double(a), C.c_double(G), _p(out)))
+    return out.reshape(h.shape)

@@ -403,6 +403,13 @@ struct WalkCounters {
   uint64_t visited = 0, pruned = 0, pairs = 0, multipoles = 0, leaves = 0, skipped_pairs = 0;
 };
 
+// Optional record of one target's walk, in visit order: internal nodes it
+// accepted as multipoles, internal nodes it opened, leaves it summed and nodes
+// the cutoff pruned (test infrastructure: the node-by-node validator).
+struct WalkTrace {
+  std::vector<uint32_t> accepted, opened, leaves, pruned;
+};
+
 // Strict-reference acceptance of an internal node (tree_pm_acceptance.hpp:54-139
 // with full_mac_diagnostics = true, no acceleration history).
 static inline bool accept_internal(const Tree& t, uint32_t n, bool inside, bool contained, double r2,
@@ -428,7 +435,8 @@ static inline bool accept_internal(const Tree& t, uint32_t n, bool inside, bool 
 
 std::array<double, 3> walk_target(const Tree& t, const double* sx, const double* sy, const double* sz,
                                   const double* sm, double px, double py, double pz, uint32_t self,
-                                  const WalkParams& P, std::vector<uint32_t>& stack, WalkCounters& C) {
+                                  const WalkParams& P, std::vector<uint32_t>& stack, WalkCounters& C,
+                                  WalkTrace* trace = nullptr) {
   double ax = 0, ay = 0, az = 0;
   if (t.size() == 0) return {0, 0, 0};
   stack.clear();
@@ -443,7 +451,11 @@ std::array<double, 3> walk_target(const Tree& t, const double* sx, const double*
     const double ndz = min_image(t.cz[n] - pz, P.L[2]);
     const double ax_ = std::abs(ndx), ay_ = std::abs(ndy), az_ = std::abs(ndz);
     const double ex = std::max(0.0, ax_ - h), ey = std::max(0.0, ay_ - h), ez = std::max(0.0, az_ - h);
-    if (!within_cutoff(ex * ex + ey * ey + ez * ez, P.rcut, P.rcut2)) { ++C.pruned; continue; }
+    if (!within_cutoff(ex * ex + ey * ey + ez * ez, P.rcut, P.rcut2)) {
+      ++C.pruned;
+      if (trace) trace->pruned.push_back(n);
+      continue;
+    }
     const bool leaf = t.child_count[n] == 0;
     double dx = 0, dy = 0, dz = 0;
     bool accept = leaf;
@@ -456,6 +468,7 @@ std::array<double, 3> walk_target(const Tree& t, const double* sx, const double*
       dz = min_image(t.comz[n] - pz, P.L[2]);
       accept = accept_internal(t, n, inside, contained, dx * dx + dy * dy + dz * dz, P);
     }
+    if (trace) (leaf ? trace->leaves : accept ? trace->accepted : trace->opened).push_back(n);
     if (accept) {
       if (leaf) {
         ++C.leaves;
@@ -1622,6 +1635,67 @@ ORACLE_API int oracle_short_range(const double* x, const double* y, const double
 
 // Adaptive-policy short-range residual per active slot (active may be NULL for
 // all rows, prev may be NULL); counters in the chui_treepm_walk_counters layout.
+// One walk of the strict policy for an arbitrary target point (px, py, pz),
+// not a source (no self exclusion), over the tree of the given sources:
+// short_range()'s tree, source copy and parameters. Writes the residual
+// acceleration and the trace lists (node numbers in preorder numbering, each
+// list truncated to cap entries; sizes[] receives the full lengths in the
+// order accepted, opened, leaves, pruned).
+ORACLE_API int oracle_probe_walk(const double* x, const double* y, const double* z, const double* m, uint64_t n,
+                                 const chui_treepm_options* o, const int32_t mesh[3], double px, double py,
+                                 double pz, double* acc3, uint32_t* accepted, uint32_t* opened, uint32_t* leaves,
+                                 uint32_t* pruned, uint64_t cap, uint64_t* sizes) {
+  ORACLE_TRY({
+    const int msh[3] = {mesh[0], mesh[1], mesh[2]};
+    const double L[3] = {o->box_size[0], o->box_size[1], o->box_size[2]};
+    const double* src[3] = {x, y, z};
+    std::vector<double> u[3];
+    for (int a = 0; a < 3; ++a) {
+      double lo = INFINITY, hi = -INFINITY;
+      for (size_t i = 0; i < n; ++i) { lo = std::min(lo, src[a][i]); hi = std::max(hi, src[a][i]); }
+      u[a].resize(n);
+      unwrap_axis(src[a], n, L[a], u[a].data(), lo >= 0.0 && hi < L[a]);
+    }
+    build_tree(u[0].data(), u[1].data(), u[2].data(), m, n, static_cast<size_t>(o->max_leaf_size),
+               o->multipole_order == 2, o->softening_epsilon, g_tree);
+    const Tree& t = g_tree;
+    std::vector<double> sx(n), sy(n), sz(n), sm(n);
+    for (size_t i = 0; i < n; ++i) {
+      const uint32_t r = t.order[i];
+      sx[i] = x[r]; sy[i] = y[r]; sz[i] = z[r]; sm[i] = m[r];
+    }
+    const double h = mesh_spacing(*o, msh);
+    WalkParams P{};
+    for (int a = 0; a < 3; ++a) P.L[a] = L[a];
+    P.G = o->gravitational_constant;
+    P.a = o->asmth_cells * h;
+    P.rcut = o->rcut_cells * h;
+    P.rcut2 = P.rcut * P.rcut;
+    P.theta = o->opening_theta;
+    P.eps = o->softening_epsilon;
+    P.criterion = o->opening_criterion;
+    P.quad = o->multipole_order == 2;
+    std::vector<uint32_t> stack;
+    WalkCounters C;
+    WalkTrace T;
+    const auto a = walk_target(t, sx.data(), sy.data(), sz.data(), sm.data(), px, py, pz, kInvalid, P, stack, C, &T);
+    acc3[0] = a[0]; acc3[1] = a[1]; acc3[2] = a[2];
+    const std::vector<uint32_t>* lists[4] = {&T.accepted, &T.opened, &T.leaves, &T.pruned};
+    uint32_t* outs[4] = {accepted, opened, leaves, pruned};
+    for (int k = 0; k < 4; ++k) {
+      sizes[k] = lists[k]->size();
+      if (outs[k]) std::copy_n(lists[k]->begin(), std::min<size_t>(cap, lists[k]->size()), outs[k]);
+    }
+  });
+}
+
+// residualAccuracyProxy of the adaptive walk (residual_proxy) on arrays of
+// (h, offset, M, r^2) with one softening, split scale and G.
+ORACLE_API int oracle_residual_proxy(const double* h, const double* offset, const double* mass, const double* r2,
+                                     uint64_t n, double eps, double a, double G, double* out) {
+  ORACLE_TRY(for (size_t i = 0; i < n; ++i) out[i] = residual_proxy(h[i], offset[i], mass[i], r2[i], eps, a, G));
+}
+
 ORACLE_API int oracle_short_range_adaptive(const double* x, const double* y, const double* z, const double* m,
                                            uint64_t n, const uint32_t* active, uint64_t n_active, const double* prev,
                                            const chui_treepm_options* o, const chui_treepm_adaptive_options* ad,

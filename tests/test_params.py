@@ -155,3 +155,45 @@ def test_kpc_code_units_and_rectangular_box():
 def test_deck_errors(text, exc):
     with pytest.raises(exc):
         P.treepm_deck(text)
+
+
+def test_pm_reuse_policy_and_cadence():
+    # config.cpp:3410-3434: cadence 1 under 'exact'; the stale window needs a
+    # cadence in [2, 64] with global KDK and turns on the stale-field option
+    # (gravity_runtime.cpp:564-565); the multirate cycle needs hierarchical KDK
+    d = P.treepm_deck("[numerics]\n")
+    assert (d.pm_reuse_policy, d.update_cadence_steps) == ("exact", 1)
+    assert d.options.allow_stale_long_range_field == 0
+    d = P.treepm_deck("[numerics]\ntreepm_pm_reuse_policy = stale_within_cadence\ntreepm_update_cadence_steps = 4\n")
+    assert (d.pm_reuse_policy, d.update_cadence_steps) == ("stale_within_cadence", 4)
+    assert d.options.allow_stale_long_range_field == 1
+    assert "numerics.treepm_update_cadence_steps" in d.applied
+    d = P.treepm_deck("[numerics]\ntreepm_pm_reuse_policy = multirate_outer_kick\ntreepm_update_cadence_steps = 2\n"
+                      "hierarchical_max_rung = 3\n")
+    assert d.options.allow_stale_long_range_field == 0 and d.update_cadence_steps == 2
+
+
+@pytest.mark.parametrize("text", [
+    "[numerics]\ntreepm_update_cadence_steps = 2\n",
+    "[numerics]\ntreepm_pm_reuse_policy = stale_within_cadence\n",
+    "[numerics]\ntreepm_pm_reuse_policy = stale_within_cadence\ntreepm_update_cadence_steps = 65\n",
+    "[numerics]\ntreepm_pm_reuse_policy = stale_within_cadence\ntreepm_update_cadence_steps = 4\n"
+    "hierarchical_max_rung = 2\n",
+    "[numerics]\ntreepm_pm_reuse_policy = multirate_outer_kick\ntreepm_update_cadence_steps = 4\n",
+    "[numerics]\ntreepm_pm_reuse_policy = sometimes\n",
+    "[numerics]\ntreepm_update_cadence_steps = two\n",
+])
+def test_pm_cadence_rejected(text):
+    with pytest.raises(P.ParamError):
+        P.treepm_deck(text)
+
+
+def test_options_set_reuse_policy():
+    o = L.default_options()
+    assert o.allow_stale_long_range_field == 0 and o.force_epoch_scale_factor == 1.0
+    L.set_param(o, "treepm_pm_reuse_policy", "stale_within_cadence")
+    assert o.allow_stale_long_range_field == 1
+    L.set_param(o, "numerics.treepm_pm_reuse_policy", "exact")
+    assert o.allow_stale_long_range_field == 0
+    with pytest.raises(L.ChuiInvalidArgument):
+        L.set_param(o, "treepm_pm_reuse_policy", "never")
