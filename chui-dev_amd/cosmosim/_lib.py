@@ -236,6 +236,7 @@ SIGNATURES = {
     "chui_treepm_kick_device": [_P, _P, _P, _P, _P, _P, _P, _I32, _P, _U64, _D, _D],
     "chui_treepm_drift_device": [_P, _P, _P, _P, _P, _P, _P, _U64, _D, C.POINTER(_D)],
     "chui_treepm_mass_momentum_device": [_P, _P, _P, _P, _P, _U64, C.POINTER(_D)],
+    "chui_treepm_mass_momentum_noalloc_device": [_P, _P, _P, _P, _P, _U64, _P],
     "chui_treepm_force_vjp_device": [_P, _P, _P, _P, _P, _U64, _P, _P, _P, _P, _P, _P, C.POINTER(Options)],
     "chui_treepm_cic_density_device": [_P, _P, _P, _P, _U64, _D, _I32, _P],
     "chui_treepm_cic_density_vjp_device": [_P, _P, _P, _P, _U64, _D, _I32, _P, _P, _P, _P],

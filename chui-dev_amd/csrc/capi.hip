@@ -957,6 +957,7 @@ chui_status chui_treepm_create(const int32_t mesh[3], int32_t device, chui_treep
       CHUI_HIP(hipSetDevice(device));
       CHUI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       c->own_stream = true;
+      chui::noalloc_scratch_init(device);
     } catch (...) {
       delete c;
       throw;
@@ -1494,6 +1495,18 @@ chui_status chui_treepm_mass_momentum_device(chui_treepm_context* c, const doubl
     set_device(c);
     c->red_scratch.reserve((4 * 1024 + 8) * sizeof(double));
     chui::mass_momentum_device(m, vx, vy, vz, n, c->red_scratch.as<double>(), out, c->stream);
+  });
+}
+
+chui_status chui_treepm_mass_momentum_noalloc_device(chui_treepm_context* c, const double* m, const double* vx,
+                                                     const double* vy, const double* vz, uint64_t n,
+                                                     double* out_device) {
+  return guarded([&] {
+    require_ctx(c);
+    if (!out_device) throw std::invalid_argument("reduction output pointer is null");
+    if (n > 0 && (!m || !vx || !vy || !vz)) throw std::invalid_argument("reduction input lanes are null");
+    set_device(c);
+    chui::mass_momentum_noalloc_device(c->device, m, vx, vy, vz, n, out_device, c->stream);
   });
 }
 

@@ -11,6 +11,7 @@
 #include "kdk.hpp"
 
 #include <cmath>
+#include <mutex>
 
 namespace chui {
 namespace {
@@ -329,6 +330,52 @@ void mass_momentum_device(const double* m, const double* vx, const double* vy, c
   CHUI_LAUNCH_CHECK("mass_momentum_final");
   CHUI_HIP(hipMemcpyAsync(out_host, partial_scratch + 4 * nb, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
   CHUI_HIP(hipStreamSynchronize(s));
+}
+
+// Allocation-free reduction: the partial sums live in a file-scope device
+// array, one copy per GPU, instead of a buffer the context allocates. Host code
+// never passes the array's name to a kernel (on the host it is only a stand-in
+// for the symbol); it passes the device address hipGetSymbolAddress returns,
+// looked up once per device with that device current.
+constexpr int kNoAllocPartials = 4 * kRedBlocks;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("HIP device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return;
+  int current = 0;
+  CHUI_HIP(hipGetDevice(&current));
+  if (current != device) throw std::logic_error("reduction scratch looked up with another device current");
+  double* ptr = nullptr;
+  CHUI_HIP(hipGetSymbolAddress(reinterpret_cast<void**>(&ptr), HIP_SYMBOL(scratch)));
+  if (!ptr) throw DeviceError("hipGetSymbolAddress returned a null address for the reduction scratch");
+  g_scratch_ptr[device] = ptr;
+}
+
+std::size_t noalloc_scratch_bytes() { return sizeof(double) * kNoAllocPartials; }
+
+void mass_momentum_noalloc_device(int device, const double* m, const double* vx, const double* vy, const double* vz,
+                                  std::size_t n, double* out_device, hipStream_t s) {
+  double* partial = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device >= 0 && device < kMaxDevices) partial = g_scratch_ptr[device];
+  }
+  if (!partial) throw std::logic_error("reduction scratch of this device was not set up");
+  // Same grid and summation tree as mass_momentum_device; every partial slot
+  // is written before it is read, so the array needs no clearing.
+  static_assert(4 * kRedBlocks <= kNoAllocPartials, "partials of the fixed grid must fit the scratch array");
+  mass_momentum_partial<<<kRedBlocks, kRedThreads, 0, s>>>(m, vx, vy, vz, n, partial);
+  CHUI_LAUNCH_CHECK("mass_momentum_partial");
+  mass_momentum_final<<<1, kRedThreads, 0, s>>>(partial, kRedBlocks, out_device);
+  CHUI_LAUNCH_CHECK("mass_momentum_final");
 }
 
 std::size_t ewald_scratch_bytes(int modes) {

@@ -14,6 +14,14 @@ void drift_device(double* x, double* y, double* z, const double* vx, const doubl
 void mass_momentum_device(const double* m, const double* vx, const double* vy, const double* vz, std::size_t n,
                           double* partial_scratch, double out_host[4], hipStream_t s);
 
+// Allocation-free form: partial sums in the library's per-GPU device array
+// (noalloc_scratch_init must have run once with `device` current), result in
+// out_device[0..3] (device memory), stream-ordered, no synchronisation.
+void noalloc_scratch_init(int device);
+std::size_t noalloc_scratch_bytes();
+void mass_momentum_noalloc_device(int device, const double* m, const double* vx, const double* vy, const double* vz,
+                                  std::size_t n, double* out_device, hipStream_t s);
+
 std::size_t ewald_scratch_bytes(int modes);
 // delta(c) = count(c) * n^3 / np - 1 on an n^3 mesh (row-major, no padding),
 // CIC weights, unit-weight particles.

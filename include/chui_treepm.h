@@ -427,6 +427,19 @@ chui_status chui_treepm_drift_device(chui_treepm_context* ctx, double* pos_x, do
 chui_status chui_treepm_mass_momentum_device(chui_treepm_context* ctx, const double* mass,
                                              const double* vel_x, const double* vel_y, const double* vel_z,
                                              uint64_t n, double out_host[4]);
+/* Allocation-free form of chui_treepm_mass_momentum_device. It has no
+ * counterpart in the reference; it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). Same sums, same
+ * fixed grid and summation tree, so the result is bit-identical; it goes to
+ * out_device[0..3] (device memory the caller owns) in the context's stream
+ * order and the call returns without synchronising. No device memory is
+ * allocated or freed: the partial sums live in a 32 KiB array the library
+ * holds on each GPU (4 x 1024 FP64 partials, one per block of the fixed grid,
+ * so every n fits it). Every context on one GPU shares that array: calls that
+ * overlap on one GPU (several contexts, streams or host threads) race. */
+chui_status chui_treepm_mass_momentum_noalloc_device(chui_treepm_context* ctx, const double* mass,
+                                                     const double* vel_x, const double* vel_y,
+                                                     const double* vel_z, uint64_t n, double* out_device);
 
 /* ---- differentiation (no reference counterpart) --------------------------- */
 /* The reference has no adjoint (no autograd or adjoint code in src/); these
