@@ -136,6 +136,7 @@ class Diagnostics(C.Structure):
         ("wave_node_visits", C.c_uint64),
         ("wave_leaf_source_slots", C.c_uint64),
         ("wave_sum_flushes", C.c_uint64),
+        ("pair_set_checksum", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

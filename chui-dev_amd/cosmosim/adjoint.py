@@ -56,6 +56,11 @@ class KdkProblem:
     vel_factor: float = 0.0
     force_evaluations: int = 0
     vjp_evaluations: int = 0
+    # When set, every force evaluation also returns the walk's diagnostics and
+    # appends (direct pair count, pair-set checksum) to pair_sets: two
+    # trajectories whose lists agree summed the same pairs inside r_cut.
+    record_pair_sets: bool = False
+    pair_sets: list = field(default_factory=list)
 
     def __post_init__(self):
         n = self.n_side**3
@@ -98,8 +103,15 @@ class KdkProblem:
 
     def forces(self, x: torch.Tensor) -> torch.Tensor:
         acc = torch.empty_like(x)
-        self._call(self.coord.solve_active_set, x[0], x[1], x[2], self.mass,
-                   ForceAccumulator((acc[0], acc[1], acc[2])), self.options)
+        if self.record_pair_sets:
+            torch.cuda.current_stream(self.device).synchronize()
+            res = self.coord.solve_active_set(x[0], x[1], x[2], self.mass, ForceAccumulator((acc[0], acc[1], acc[2])),
+                                              self.options, want_diagnostics=True)
+            d = res.diagnostics
+            self.pair_sets.append((d["direct_pair_evaluations"], d["pair_set_checksum"]))
+        else:
+            self._call(self.coord.solve_active_set, x[0], x[1], x[2], self.mass,
+                       ForceAccumulator((acc[0], acc[1], acc[2])), self.options)
         self.force_evaluations += 1
         return acc
 
@@ -200,6 +212,42 @@ def density_loss(psi: torch.Tensor, target: torch.Tensor, prob: KdkProblem) -> t
     x = KdkTrajectory.apply(psi.to(torch.float64), prob)
     delta = CicDensity.apply(x, prob)
     return ((delta - target) ** 2).sum()
+
+
+def central_differences(prob: KdkProblem, psi: torch.Tensor, target: torch.Tensor, grad: torch.Tensor,
+                        directions: int, step: float, seed: int = ics.BASE_SEED) -> list:
+    """Central differences of density_loss along `directions` random unit
+    directions u of the (3, N) displacement field, (L(psi + step u) -
+    L(psi - step u)) / (2 step), against grad . u. A direction is flagged
+    crosses_cutoff when the two trajectories did not sum the same pairs inside
+    r_cut in every force evaluation (pair counts and pair-set checksums of the
+    FP64 walk): the loss jumps there, and the difference quotient measures the
+    jump, not the derivative."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    saved = prob.record_pair_sets
+    prob.record_pair_sets = True
+    try:
+        for _ in range(directions):
+            d = rng.standard_normal(tuple(psi.shape))
+            d /= np.linalg.norm(d)
+            u = torch.tensor(d, dtype=torch.float64, device=psi.device)
+            sets = []
+            values = []
+            for sign in (1.0, -1.0):
+                prob.pair_sets = []
+                with torch.no_grad():
+                    values.append(float(density_loss(psi + sign * step * u, target, prob)))
+                sets.append(list(prob.pair_sets))
+            fd = (values[0] - values[1]) / (2.0 * step)
+            an = float((grad * u).sum())
+            rows.append({"finite_difference": fd, "analytic": an,
+                         "relative_error": abs(fd - an) / max(abs(an), 1e-300),
+                         "crosses_cutoff": sets[0] != sets[1]})
+    finally:
+        prob.record_pair_sets = saved
+        prob.pair_sets = []
+    return rows
 
 
 def adjoint_problem(n_side: int = 64, steps: int = 10, a_begin: float = 0.05, a_end: float = 0.5,

@@ -51,6 +51,14 @@ struct chui_treepm_context {
   DeviceBuffer vjp_lam, vjp_comp[3];       // force VJP: tree-ordered lam and deposit lanes
   DeviceBuffer src64, src32, htab;  // tree-ordered packed sources; H(t) table
   double htab_key[2] = {0.0, 0.0};
+  DeviceBuffer h64tab;              // FP64 box walk: degree-5 cells of H(t)
+  double h64_key[2] = {0.0, 0.0};
+  // mixed timed walk: bit-indexed residual cells, keyed by (scale, eps, r_s)
+  DeviceBuffer ltab;
+  std::vector<float> ltab_host;
+  double ltab_key[3] = {0.0, -1.0, 0.0};
+  uint32_t ltab_cells = 0;
+  int32_t ltab_first_octave = 0;
   DeviceBuffer hx, hy, hz, hm, hactive, hout[9];
   DeviceBuffer tpos, tslot, tkeys_alt, tvals_alt, tsort_scratch, flag;
   DeviceBuffer counters, red_scratch, ewald_scratch;
@@ -66,7 +74,7 @@ struct chui_treepm_context {
   double prof_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t prof_solves = 0;
   std::vector<float> host_float;
-  // Walk traversal fault: the walks set counters[kWalkCounters] (sticky: the
+  // Walk traversal fault: the walks set counters[kFaultSlot] (sticky: the
   // per-solve reset leaves it) when a wave's lanes all left the tree early.
   // Every solve copies it to pinned host memory behind fault_ev; it is
   // checked at the next synchronisation through the library or the next solve.
@@ -273,10 +281,11 @@ void record(Ctx* c, Stage st) {
 // zeroed once, so the fault word survives the per-solve counter resets.
 unsigned long long* walk_counters(Ctx* c, hipStream_t s) {
   if (!c->counters.ptr) {
-    c->counters.reserve((chui::kWalkCounters + 1) * sizeof(unsigned long long));
-    chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounters + 1, s);
+    c->counters.reserve(chui::kWalkCounterWords * sizeof(unsigned long long));
+    chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounterWords, s);
   }
   chui::fill_u64(c->counters.as<uint64_t>(), 0ull, chui::kWalkCounters, s);
+  chui::fill_u64(c->counters.as<uint64_t>() + chui::kPairChecksumSlot, 0ull, 1, s);
   return reinterpret_cast<unsigned long long*>(c->counters.ptr);
 }
 
@@ -289,7 +298,7 @@ void post_fault_word(Ctx* c, hipStream_t s) {
     *c->fault_host = 0;
   }
   if (!c->fault_ev) CHUI_HIP(hipEventCreateWithFlags(&c->fault_ev, hipEventDisableTiming));
-  CHUI_HIP(hipMemcpyAsync(c->fault_host, c->counters.as<unsigned long long>() + chui::kWalkCounters,
+  CHUI_HIP(hipMemcpyAsync(c->fault_host, c->counters.as<unsigned long long>() + chui::kFaultSlot,
                           sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   CHUI_HIP(hipEventRecord(c->fault_ev, s));
   c->fault_pending = true;
@@ -309,7 +318,7 @@ void check_fault(Ctx* c, bool wait) {
   c->fault_pending = false;
   if (*c->fault_host != 0) {
     *c->fault_host = 0;
-    chui::fill_u64(c->counters.as<uint64_t>() + chui::kWalkCounters, 0ull, 1, c->stream);
+    chui::fill_u64(c->counters.as<uint64_t>() + chui::kFaultSlot, 0ull, 1, c->stream);
     throw std::logic_error("TreePM walk lost progress (wave reduction selected no pending node)");
   }
 }
@@ -364,6 +373,14 @@ chui::WalkParams base_walk_params(const Ctx* c, const chui_treepm_options* o, co
     wp.contain2_sep = static_cast<float>(r2 * (1.0 - 1.0e-5));
   }
   for (int a = 0; a < 3; ++a) {
+    // box-classified walk for waves whose targets lie within +-2 r_cut of the
+    // box centre per axis. At 256^3 a bound of +-r_cut leaves 79% of the
+    // waves to the lane-by-lane walk; +-2 r_cut measured 13% faster in total,
+    // +-3 and +-4 r_cut slower again (more sources per wider box).
+    const double sep = wp.cubic ? o->box_size[a] / 4294967296.0 : 1.0;
+    wp.box_half_max[a] = static_cast<float>(2.0 * sp.rcut / sep);
+  }
+  for (int a = 0; a < 3; ++a) {
     // float offsets from a wave origin: targets within L/16 (precision) and
     // within L/2 - r_cut (no wrap of the origin's frame inside a pair), with a
     // margin of 2^24 units for the r_cut conversion
@@ -374,6 +391,25 @@ chui::WalkParams base_walk_params(const Ctx* c, const chui_treepm_options* o, co
   wp.node_count = static_cast<uint32_t>(c->tree.node_count);
   wp.mesh_row = mesh_row;
   return wp;
+}
+
+// The FP64 box walk's table of H on [0, t_max] (t_max just above r_cut^2 / 4a^2)
+// and the parameters that go with it.
+void prepare_h64(Ctx* c, chui::WalkParams& wp, const Split& sp, hipStream_t s) {
+  const double t_max = wp.rcut2 * (0.25 / (sp.a * sp.a)) * (1.0 + 1.0e-3);
+  const int cells = static_cast<int>(std::min<double>(chui::kH64MaxCells, std::max(16.0, std::ceil(t_max / 0.04))));
+  if (c->h64_key[0] != t_max || c->h64_key[1] != cells || !c->h64tab.ptr) {
+    const std::vector<double> tab = chui::gaussian_quintic_table(t_max, cells);
+    c->h64tab.reserve(tab.size() * sizeof(double));
+    CHUI_HIP(hipMemcpyAsync(c->h64tab.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    CHUI_HIP(hipStreamSynchronize(s));
+    c->h64_key[0] = t_max;
+    c->h64_key[1] = cells;
+  }
+  wp.h64_cells = cells;
+  wp.h64_scale = (0.25 / (sp.a * sp.a)) / (t_max / cells);
+  wp.eps2_64 = wp.eps * wp.eps;
+  wp.c8_64 = ((0.125 / sp.a) / sp.a) / sp.a;
 }
 
 // TreePmOptions defaults of the adaptive/group-walk fields (tree_pm_coupling.hpp:248-270).
@@ -692,9 +728,13 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     }
   } else if (n_active > 0 && n > 0) {
     if (!mixed) {
+      // solves without counters or the nearest-pair lane take the
+      // box-classified walk with the FP64 polynomial table of H
+      wp.box64 = (!wp.count && !wp.target_per_wave && near2 == nullptr) ? 1 : 0;
+      if (wp.box64) prepare_h64(c, wp, sp, s);
       chui::walk_fp64(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, target_pos, target_slot,
                       static_cast<const double*>(pw.force[0]), static_cast<const double*>(pw.force[1]),
-                      static_cast<const double*>(pw.force[2]), out, walk_cnt, s);
+                      static_cast<const double*>(pw.force[2]), out, walk_cnt, c->h64tab.as<double2>(), s);
     } else {
       const double t_max = wp.rcut2 * (0.25 / (sp.a * sp.a)) * (1.0 + 1.0e-3);
       std::vector<float> tab = chui::gaussian_coefficient_table(wp, t_max);
@@ -719,9 +759,42 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
         c->htab_key[0] = t_max;
         c->htab_key[1] = sp.a;
       }
-      chui::walk_mixed(c->tree, wp, c->src32.as<uint4>(), uniform ? 1 : 0, c->htab.as<float4>(), target_pos,
-                       target_slot, static_cast<const float*>(pw.force[0]), static_cast<const float*>(pw.force[1]),
-                       static_cast<const float*>(pw.force[2]), out, walk_cnt, s);
+      // The timed walk (no counters, 64 targets per wave) takes the residual
+      // table indexed by the float bits of s = r^2 + eps^2, in units where
+      // r_cut^2 + eps^2 = 128; the counting and per-wave walks keep the
+      // reciprocal square root and the Gaussian table above.
+      if (!wp.count && !wp.target_per_wave) {
+        const double scale = 128.0 / (wp.rcut2 + wp.eps * wp.eps);
+        const double key[3] = {scale, wp.eps, sp.a};
+        if (!(c->ltab_key[0] == key[0] && c->ltab_key[1] == key[1] && c->ltab_key[2] == key[2])) {
+          int32_t e0 = 0;
+          uint32_t cells = 0;
+          c->ltab_host = chui::log_residual_table(scale, wp.eps, sp.a, e0, cells);
+          c->ltab_cells = cells;
+          c->ltab_first_octave = e0;
+          if (cells > 0) {
+            c->ltab.reserve(c->ltab_host.size() * sizeof(float));
+            CHUI_HIP(hipMemcpyAsync(c->ltab.ptr, c->ltab_host.data(), c->ltab_host.size() * sizeof(float),
+                                    hipMemcpyHostToDevice, s));
+            CHUI_HIP(hipStreamSynchronize(s));
+          }
+          std::copy(key, key + 3, c->ltab_key);
+        }
+        if (c->ltab_cells > 0) {
+          const double sq = std::sqrt(scale);
+          for (int a = 0; a < 3; ++a) wp.tab_scale[a] = static_cast<float>(o->box_size[a] / 4294967296.0 * sq);
+          wp.rcut2_tab = static_cast<float>(wp.rcut2 * scale);
+          wp.eps2_tab = static_cast<float>(wp.eps * wp.eps * scale);
+          wp.force_scale = static_cast<float>(wp.G * scale * (uniform ? um : 1.0));
+          wp.log_cells = c->ltab_cells;
+          wp.log_base = static_cast<uint32_t>(c->ltab_first_octave + 127) << chui::kLogCellBits;
+          wp.log_zero = c->ltab_cells - 1;
+        }
+      }
+      chui::walk_mixed(c->tree, wp, c->src32.as<uint4>(), uniform ? 1 : 0, c->htab.as<float4>(),
+                       c->ltab.as<float4>(), target_pos, target_slot, static_cast<const float*>(pw.force[0]),
+                       static_cast<const float*>(pw.force[1]), static_cast<const float*>(pw.force[2]), out, walk_cnt,
+                       s);
     }
   }
   record(c, kEvWalk);
@@ -756,7 +829,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   if (diag) {
     if (diag->struct_size != sizeof(chui_treepm_diagnostics))
       throw std::invalid_argument("chui_treepm_diagnostics.struct_size does not match this library");
-    unsigned long long cnt[chui::kWalkCounters] = {};
+    unsigned long long cnt[chui::kWalkCounterWords] = {};
     CHUI_HIP(hipMemcpyAsync(cnt, c->counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, s));
     CHUI_HIP(hipStreamSynchronize(s));
     diag->node_count = c->tree.node_count;
@@ -770,6 +843,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     diag->wave_node_visits = cnt[5];
     diag->wave_leaf_source_slots = cnt[6];
     diag->wave_sum_flushes = cnt[7];
+    diag->pair_set_checksum = cnt[chui::kPairChecksumSlot];
     diag->split_scale_comoving = sp.a;
     diag->cutoff_radius_comoving = sp.rcut;
     diag->mesh_spacing_comoving = sp.h;
@@ -1208,10 +1282,12 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
   wp.uniform_mass_f = static_cast<float>(um);
   wp.pm_enabled = 1;
   chui::WalkOutputs out{ox, oy, oz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  wp.box64 = 1;
+  prepare_h64(c, wp, sp, s);
   chui::walk_fp64_vjp(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, c->vjp_lam.as<double4>(),
                       static_cast<const double*>(c->vjp_fwd.force[0]), static_cast<const double*>(c->vjp_fwd.force[1]),
                       static_cast<const double*>(c->vjp_fwd.force[2]), c->vjp_adj.buf[0].as<double>(), out,
-                      walk_cnt, s);
+                      walk_cnt, c->h64tab.as<double2>(), s);
   post_fault_word(c, s);
 }
 
@@ -1633,7 +1709,7 @@ chui_status chui_treepm_debug_inject_walk_fault(chui_treepm_context* c) {
     require_ctx(c);
     set_device(c);
     walk_counters(c, c->stream);
-    chui::fill_u64(c->counters.as<uint64_t>() + chui::kWalkCounters, 1ull, 1, c->stream);
+    chui::fill_u64(c->counters.as<uint64_t>() + chui::kFaultSlot, 1ull, 1, c->stream);
   });
 }
 

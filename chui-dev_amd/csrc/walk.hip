@@ -664,7 +664,23 @@ struct WalkData {
   const double* prev;
   // group walk: the same Gaussian cells as monomial coefficients (a0..a3)
   const double4* gcoef;
+  // FP64 box walk: degree-5 cells of H (a0..a5 as three double2 per cell)
+  const double2* h64tab;
+  // mixed timed walk: residual cells indexed by the float bits of r^2 + eps^2
+  const float4* ltab;
+  // preorder index of each node's child per octant (kInvalid for none)
+  const uint32_t* child;
 };
+
+// splitmix64 finaliser of the pair (target row, source row): the pair-set
+// checksum adds it over every direct pair inside r_cut.
+__device__ __forceinline__ unsigned long long pair_mix(uint32_t target_row, uint32_t source_row) {
+  unsigned long long z = (static_cast<unsigned long long>(target_row) << 32) | source_row;
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
 
 __device__ __forceinline__ uint32_t next_node(uint32_t n, uint32_t nxt) {
   return __ballot(nxt == n + 1) ? n + 1 : wave_min_u32(nxt);
@@ -694,6 +710,8 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
   uint32_t qlen = 0, queued = 0;
   uint32_t nxt = valid ? 0u : P.node_count;
   double nearest = INFINITY;  // r^2 of the nearest distinct direct-summed source inside r_cut
+  const uint32_t target_row = valid ? D.order[tp] : 0u;
+  unsigned long long pair_sum = 0;  // pair-set checksum (wrapping)
   double px = 0.0, py = 0.0, pz = 0.0;
   double gx = 0.0, gy = 0.0, gz = 0.0;
   double4 lk_ = make_double4(0.0, 0.0, 0.0, 0.0);  // JVP: lam of the target
@@ -764,7 +782,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
     const NodeGeo64 g = D.geo64[n];
     const bool act = nxt == n;
     if (!__ballot(act)) {  // the chosen node must be some lane's next node
-      if (lane == 0) atomicOr(&counters[kWalkCounters], 1ull);
+      if (lane == 0) atomicOr(&counters[kFaultSlot], 1ull);
       break;
     }
     cnt[0] += act;
@@ -793,6 +811,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
             const double r2 = qx * qx + qy * qy + qz * qz;
             if (keep && j != tp && r2 <= P.rcut2) {
               nearest = fmin(nearest, r2);
+              pair_sum += pair_mix(target_row, D.order[j]);
               // a coincident pair has a zero force but, in the JVP, the
               // Jacobian F(0) I
               if (!JVP && qx == 0.0 && qy == 0.0 && qz == 0.0) {
@@ -855,6 +874,10 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
   drain();
   cnt[2] += queued;
   flush_counters(counters, cnt);
+  {
+    const unsigned long long v = wave_sum(pair_sum);
+    if (lane == 0 && v) atomicAdd(&counters[kPairChecksumSlot], v);
+  }
   if (!valid) return;
   static_cast<double*>(out.ax)[slot] = gx + ax;
   static_cast<double*>(out.ay)[slot] = gy + ay;
@@ -1336,9 +1359,38 @@ __global__ void leaf_of_pos_kernel(const NodeLink* __restrict__ link, std::size_
 // of O (per axis), a pair inside r_cut cannot straddle the wrap of O's frame.
 // Waves that straddle a jump of the space-filling order (targets far apart)
 // store raw fixed-point coordinates and form int32(q_s - q_t) per pair.
+//
+// Box-classified traversal (BOX, the timed walk). A wave whose targets lie in
+// a box of half-extent at most box_half_max (2 r_cut) per axis walks the tree
+// once for that box instead of node by node for its 64 lanes: the lanes classify 64
+// consecutive preorder nodes at a time against the box (pruned when the
+// node's tight particle box lies beyond r_cut of the target box, i.e. beyond
+// r_cut of every target), a scalar loop follows the visited chain through the
+// window, and the kept leaves' sources go to a 128-entry ring in LDS that is
+// evaluated one 64-source window at a time with the pair kernel above (each
+// lane keeps only its own pairs inside r_cut). Only internal nodes of cells
+// small enough for the strict acceptance (h < 0.041 r_cut) and not within
+// 25 h of every target (where 2h/r < 0.08 fails for all of them) can be
+// accepted by some lane; their subtrees are walked lane by lane as above.
+// The pairs and multipoles are therefore those of the lane-by-lane walk;
+// only the summation order differs.
 constexpr int kWindow = 64;
+constexpr int kBoxRing = 2 * kWindow;
 
-template <int SCHEME, bool CUBIC, bool COUNT>
+//
+// Residual table indexed by the float bits (LOG, the timed walk): the whole
+// pair coefficient F(s) = s^{-3/2} - c8 H((s/scale - eps^2) / 4a^2) as a
+// function of s = r^2 + eps^2 in table units, whose scale puts r_cut^2 + eps^2
+// at s = 128. The cell of s is its exponent and top 5 mantissa bits (32 cubic
+// Hermite cells per octave, from the octave of eps^2 up to s = 128, then one
+// zero cell), the position in the cell its low 18 mantissa bits as an integer
+// (the coefficients carry the 2^-18 powers). So a pair costs an add, a shift,
+// a subtract, a min, an LDS read and three FMAs instead of a reciprocal square
+// root, the Gaussian table and a cutoff select; pairs beyond r_cut land in the
+// zero cell. Cubic Hermite cells of relative width 1/32 to 1/64 keep F within
+// 1.5e-7 of its value.
+
+template <int SCHEME, bool CUBIC, bool COUNT, bool BOX = false, bool LOG = false>
 __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : 8)))
 void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long long* __restrict__ counters,
                     const NodeLink* __restrict__ node_link, const NodeGeo32* __restrict__ node_geo,
@@ -1347,11 +1399,16 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   // node records are read-only for the kernel's lifetime and indexed by the
   // wave-uniform node number, so they come through the scalar cache
   __shared__ float4 win[kWalkWaves][kWindow];
-  __shared__ float4 htab[kHTableCells];
-  // per lane: force totals [0..2], their Kahan compensations [3..5] and the
-  // partial pair sums since the last fold [6..8]
-  __shared__ float ksum[kWalkWaves][9][kWave];
-  for (int i = threadIdx.x; i < kHTableCells; i += kWalkThreads) htab[i] = D.htab[i];
+  __shared__ float4 htab[LOG ? 1 : kHTableCells];
+  __shared__ uint32_t ring[kWalkWaves][kBoxRing];
+  // per lane: force totals [0..2] and their Kahan compensations [3..5]
+  __shared__ float ksum[kWalkWaves][6][kWave];
+  extern __shared__ float4 ltab[];  // LOG: P.log_cells cells
+  if constexpr (LOG) {
+    for (int i = threadIdx.x; i < static_cast<int>(P.log_cells); i += kWalkThreads) ltab[i] = D.ltab[i];
+  } else {
+    for (int i = threadIdx.x; i < kHTableCells; i += kWalkThreads) htab[i] = D.htab[i];
+  }
   __syncthreads();
   const int w = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
@@ -1388,23 +1445,28 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   // wave origin: centre of the valid targets' bounding box in fixed point
   uint32_t org[3];
   bool near = true;
+  float box_half[3];  // the box's half-extents, separation units (covers every target)
   {
     const uint32_t pp[3] = {px, py, pz};
+    const float sa[3] = {s0, s1, s2};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const uint32_t o0 = readlane_u32(pp[a], 0);
       const int32_t off = valid ? static_cast<int32_t>(pp[a] - o0) : 0;
-      const int32_t lo = wave_min(off), hi = wave_max(off);
+      // wave-uniform through readlane, so the box lives in scalar registers
+      const int32_t lo = static_cast<int32_t>(readlane_u32(static_cast<uint32_t>(wave_min(off)), 0));
+      const int32_t hi = static_cast<int32_t>(readlane_u32(static_cast<uint32_t>(wave_max(off)), 0));
       const int64_t mid = (static_cast<int64_t>(lo) + static_cast<int64_t>(hi)) / 2;
       org[a] = o0 + static_cast<uint32_t>(static_cast<int32_t>(mid));
       const int64_t half_extent = (static_cast<int64_t>(hi) - static_cast<int64_t>(lo)) / 2 + 1;
       near = near && half_extent <= static_cast<int64_t>(P.fast_half_extent[a]);
+      box_half[a] = static_cast<float>(half_extent + 1) * sa[a];
     }
   }
   const uint32_t ox = org[0], oy = org[1], oz = org[2];
   const float tx = fx_delta(px, ox, u0), ty = fx_delta(py, oy, u1), tz = fx_delta(pz, oz, u2);
 #pragma unroll
-  for (int c = 0; c < 9; ++c) ksum[w][c][lane] = 0.0f;
+  for (int c = 0; c < 6; ++c) ksum[w][c][lane] = 0.0f;
   auto accumulate = [&](float vx, float vy, float vz) {
     const float v[3] = {vx, vy, vz};
 #pragma unroll
@@ -1436,51 +1498,65 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   };
 
   uint32_t pend = 0;  // wave-uniform count of sources in the partial sums
+  float psx = 0.0f, psy = 0.0f, psz = 0.0f;  // partial pair sums since the last fold
   auto fold = [&]() {
     const float gm = P.force_scale;
-    accumulate(ksum[w][6][lane] * gm, ksum[w][7][lane] * gm, ksum[w][8][lane] * gm);
-#pragma unroll
-    for (int c = 6; c < 9; ++c) ksum[w][c][lane] = 0.0f;
+    accumulate(psx * gm, psy * gm, psz * gm);
+    psx = psy = psz = 0.0f;
     pend = 0;
     if (COUNT && lane == 0) ++cnt[7];
   };
 
   // the residual pair kernel against the cn staged sources; UMASS: every
-  // source has the uniform mass, which the force scale already carries
-  auto pairs = [&]<bool NEAR, bool UMASS>(uint32_t c0, uint32_t cn, float lim) {
+  // source has the uniform mass, which the force scale already carries; SEL:
+  // lanes may carry a negative cutoff (they pruned the leaf or accepted an
+  // ancestor), so their terms are dropped explicitly
+  const uint32_t log_base = P.log_base, log_zero = P.log_zero;
+  auto pairs = [&]<bool NEAR, bool UMASS, bool SEL = true>(uint32_t c0, uint32_t cn, float lim) {
     float lx = 0.0f, ly = 0.0f, lz = 0.0f;
 #pragma unroll 4
     for (uint32_t si = 0; si < cn; ++si) {
       const float4 sj = win[w][si];
       float dx, dy, dz;
       separation.template operator()<NEAR>(sj, dx, dy, dz);
-      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));  // = H-table coordinate
-      const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
-      const float soft = rinv * rinv * rinv;
-      const int ci = static_cast<int>(fminf(r2, static_cast<float>(kHTableCells - 1)));
-      const float fu = __builtin_amdgcn_fractf(r2);
-      const float4 c = htab[ci];
-      const float hcoef = fmaf(fmaf(fmaf(c.w, fu, c.z), fu, c.y), fu, c.x);
-      float fr = fmaf(-hcoef, c8, soft);
-      if constexpr (!UMASS) fr *= sj.w;
-      // The target itself has r2 == 0 exactly (same fixed-point value), as
-      // does any coincident source: its term is exactly zero (dx = dy = dz = 0
-      // against a finite softened value), so no self test is needed. The
-      // counting variant counts coincident sources as the reference does.
-      const bool pair = r2 <= lim;
-      if constexpr (COUNT) {
-        const bool distinct = pair && (r2 != 0.0f || (c0 + si) != tp);
-        cnt[2] += distinct;
-        nearest = distinct ? fminf(nearest, r2) : nearest;
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));  // = table coordinate
+      float f;
+      if constexpr (LOG) {
+        const uint32_t bits = __float_as_uint(r2 + eps2);
+        const uint32_t cell = min((bits >> kLogMantissaShift) - log_base, log_zero);
+        const float4 c = ltab[cell];
+        const float u = static_cast<float>(bits & ((1u << kLogMantissaShift) - 1u));
+        float fr = fmaf(fmaf(fmaf(c.w, u, c.z), u, c.y), u, c.x);
+        if constexpr (!UMASS) fr *= sj.w;
+        f = (SEL && lim < 0.0f) ? 0.0f : fr;
+      } else {
+        const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
+        const float soft = rinv * rinv * rinv;
+        const int ci = static_cast<int>(fminf(r2, static_cast<float>(kHTableCells - 1)));
+        const float fu = __builtin_amdgcn_fractf(r2);
+        const float4 c = htab[ci];
+        const float hcoef = fmaf(fmaf(fmaf(c.w, fu, c.z), fu, c.y), fu, c.x);
+        float fr = fmaf(-hcoef, c8, soft);
+        if constexpr (!UMASS) fr *= sj.w;
+        // The target itself has r2 == 0 exactly (same fixed-point value), as
+        // does any coincident source: its term is exactly zero (dx = dy = dz = 0
+        // against a finite softened value), so no self test is needed. The
+        // counting variant counts coincident sources as the reference does.
+        const bool pair = r2 <= lim;
+        if constexpr (COUNT) {
+          const bool distinct = pair && (r2 != 0.0f || (c0 + si) != tp);
+          cnt[2] += distinct;
+          nearest = distinct ? fminf(nearest, r2) : nearest;
+        }
+        f = pair ? fr : 0.0f;
       }
-      const float f = pair ? fr : 0.0f;
       lx = fmaf(f, dx, lx);
       ly = fmaf(f, dy, ly);
       lz = fmaf(f, dz, lz);
     }
-    ksum[w][6][lane] += lx;
-    ksum[w][7][lane] += ly;
-    ksum[w][8][lane] += lz;
+    psx += lx;
+    psy += ly;
+    psz += lz;
   };
 
   // Strict acceptance of internal node n (cell centre and half from g) for
@@ -1497,15 +1573,15 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   // accepts as a multipole (its whole cell inside r_cut) is always kept, so
   // the pairs and multipoles are the reference's; only nodes whose particles
   // all lie beyond r_cut are skipped earlier than with the octree cells.
-  auto walk = [&]<bool NEAR>() {
+  auto walk = [&]<bool NEAR>(uint32_t end) {
     uint32_t n = wave_min_u32(nxt);
-    while (n < P.node_count) {
+    while (n < end) {
       if (COUNT && lane == 0) ++cnt[5];
       const NodeLink lk = node_link[n];
       const NodeBox32 b = node_box[n];
       const bool act = nxt == n;
       if (!__ballot(act)) {
-        if (lane == 0) atomicOr(&counters[kWalkCounters], 1ull);
+        if (lane == 0) atomicOr(&counters[kFaultSlot], 1ull);
         break;
       }
       const float ax_ = fabsf(fx_delta(b.cx, px, s0)), ay_ = fabsf(fx_delta(b.cy, py, s1)),
@@ -1554,10 +1630,129 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
       }
       n = next_node(n, nxt);
     }
-    fold();
   };
-  if (near) walk.template operator()<true>();
-  else walk.template operator()<false>();
+
+  // Box classification of node q against the wave's target box: 0 pruned,
+  // 1 kept leaf (sources [lb, lb + lc)), 3 opened, 4 a subtree some lane may
+  // accept a multipole in (walked lane by lane); next is q + 1 to descend,
+  // q + skip to move past the subtree.
+  auto box_classify = [&](uint32_t q, uint32_t& next, uint32_t& lb, uint32_t& lc, uint32_t& span,
+                          uint32_t& nchild) -> int {
+    const NodeLink lk = node_link[q];
+    const NodeBox32 b = node_box[q];
+    span = lk.skip;
+    nchild = lk.child_count;
+    const float ex = fmaxf(0.0f, fabsf(fx_delta(b.cx, ox, s0)) - b.hx - box_half[0]);
+    const float ey = fmaxf(0.0f, fabsf(fx_delta(b.cy, oy, s1)) - b.hy - box_half[1]);
+    const float ez = fmaxf(0.0f, fabsf(fx_delta(b.cz, oz, s2)) - b.hz - box_half[2]);
+    const bool keep = !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > prune2);
+    if (lk.child_count == 0) {
+      lb = lk.begin;
+      lc = lk.count;
+      return keep ? 1 : 0;
+    }
+    if (!keep) {
+      next = q + lk.skip;
+      return 0;
+    }
+    if (b.cell_half < P.accept_half_max) {
+      // every target within 25 h of the COM fails 2h/r < 0.08
+      const NodeMom32 M = node_mom[q];
+      const float dx = (fabsf(fx_delta(M.comx, ox, s0)) + box_half[0]) * to_len;
+      const float dy = (fabsf(fx_delta(M.comy, oy, s1)) + box_half[1]) * to_len;
+      const float dz = (fabsf(fx_delta(M.comz, oz, s2)) + box_half[2]) * to_len;
+      const float far2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      if (!(far2 <= 624.9f * b.cell_half * b.cell_half)) {
+        next = q + lk.skip;
+        return 4;
+      }
+    }
+    return 3;
+  };
+
+  auto box_walk = [&]() {
+    const float lim = valid ? rc2 : -1.0f;
+    uint32_t head = 0, nl = 0;  // ring[(head + i) % kBoxRing], i < nl: listed sources
+    auto eval_window = [&](uint32_t cn) {
+      __builtin_amdgcn_wave_barrier();
+      {
+        const uint32_t e = (head + (static_cast<uint32_t>(lane) < cn ? lane : 0)) % kBoxRing;
+        const uint4 sv = D.src32[ring[w][e]];
+        const float mj = D.uniform_mass ? 1.0f : __uint_as_float(sv.w);
+        if (static_cast<uint32_t>(lane) < cn) win[w][lane] = stage_record.template operator()<true>(sv, mj);
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (pend + cn > static_cast<uint32_t>(kWindow)) fold();
+      pend += cn;
+      // every lane sums every staged source (pairs beyond r_cut are zero);
+      // lanes without a target are never written out
+      if (D.uniform_mass) pairs.template operator()<true, true, false>(0, cn, lim);
+      else pairs.template operator()<true, false, false>(0, cn, lim);
+      __builtin_amdgcn_wave_barrier();
+      head = (head + cn) % kBoxRing;
+      nl -= cn;
+    };
+    // the lanes' classified nodes: kept leaves' sources go to the ring,
+    // subtrees some lane may accept in are walked lane by lane
+    auto take = [&](bool on, int kind, uint32_t q, uint32_t lb, uint32_t lc, uint32_t next) {
+      unsigned long long rest = __ballot(on && kind == 1);
+      while (rest) {
+        const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(rest)) - 1);
+        rest &= rest - 1ull;
+        const uint32_t b0 = readlane_u32(lb, l), c = readlane_u32(lc, l);
+        for (uint32_t c0 = 0; c0 < c; c0 += kWindow) {
+          const uint32_t cn = c - c0 < static_cast<uint32_t>(kWindow) ? c - c0 : static_cast<uint32_t>(kWindow);
+          // nl < kWindow here, so the ring has room for cn <= kWindow more
+          if (static_cast<uint32_t>(lane) < cn) ring[w][(head + nl + lane) % kBoxRing] = b0 + c0 + lane;
+          nl += cn;
+          if (nl >= static_cast<uint32_t>(kWindow)) eval_window(kWindow);
+        }
+      }
+      unsigned long long sub = __ballot(on && kind == 4);
+      while (sub) {
+        const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(sub)) - 1);
+        sub &= sub - 1ull;
+        const uint32_t q4 = readlane_u32(q, l), end = readlane_u32(next, l);
+        nxt = valid ? q4 : end;
+        walk.template operator()<true>(end);
+      }
+    };
+    // windows of 64 consecutive preorder nodes of [start, end), following the
+    // visited chain through each window
+    auto chain_walk = [&](uint32_t start, uint32_t end) {
+      uint32_t p = start;
+      while (p < end) {
+        const uint32_t q = p + lane;
+        uint32_t next = q + 1, lb = 0, lc = 0, span = 1, nch = 0;
+        const int kind = q < end ? box_classify(q, next, lb, lc, span, nch) : 3;
+        unsigned long long chain = 0;
+        uint32_t cur = p;
+        while (cur < p + kWave && cur < end) {
+          const uint32_t l = cur - p;
+          chain |= 1ull << l;
+          cur = readlane_u32(next, l);
+        }
+        p = cur;
+        take((chain >> lane) & 1ull, kind, q, lb, lc, next);
+      }
+    };
+    chain_walk(0, P.node_count);
+    if (nl) eval_window(nl);
+  };
+
+  bool boxed = false;
+  if constexpr (BOX) {
+    boxed = near && box_half[0] <= P.box_half_max[0] && box_half[1] <= P.box_half_max[1] &&
+            box_half[2] <= P.box_half_max[2];
+  }
+  if (boxed) {
+    if constexpr (BOX) box_walk();
+  } else if (near) {
+    walk.template operator()<true>(P.node_count);
+  } else {
+    walk.template operator()<false>(P.node_count);
+  }
+  fold();
 
   // Diagnostic launches also replay the reference's walk over the octree
   // cells (no pair work) for its per-target node counters.
@@ -1892,6 +2087,358 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_target_kernel(WalkParams 
   }
 }
 
+// ---- FP64 strict walk, box-classified (the timed FP64 path) ----------------------
+// walkmix_kernel's box-classified traversal in FP64 for solves without
+// counters: a wave whose 64 targets lie in a box of half-extent at most 2 r_cut
+// (and below L/2 - r_cut, so that every pair inside r_cut has its minimum
+// image relative to the box centre) classifies 64 preorder nodes at a time
+// against the box (octree cells beyond r_cut of the box are pruned; internal
+// nodes that no target can accept are opened), lists the kept leaves'
+// sources in an LDS ring and evaluates them one 64-source window at a time;
+// subtrees some target may accept as a multipole (cells with h < 0.041 r_cut
+// farther than 25 h from some target) and waves that do not fit a box are
+// walked lane by lane with walk64_kernel's tests (cell prune, accept64,
+// node_accel64). Pair term: (r^2 + eps^2)^{-3/2} from v_rsq_f64 and two Newton
+// steps, L(r)/r^3 = H(r^2 / 4a^2) / 8a^3 from a degree-5 polynomial per cell
+// of an LDS table (Chebyshev interpolation, error below 1e-14 of H), in FP64
+// throughout. The pairs and multipoles are walk64_kernel's; the pair
+// arithmetic and the summation order differ at the round-off level.
+constexpr int kH64Cells = kH64MaxCells;
+
+// s^{-1/2} for s in [1e-30, 1e30]: the FP32 reciprocal square root (about
+// 2^-23 relative) refined by two Newton steps in FP64 (error squared each
+// step), so the result is within an ulp or two of FP64.
+__device__ __forceinline__ double rsq64(double s) {
+  double y = static_cast<double>(__builtin_amdgcn_rsqf(static_cast<float>(s)));
+  double e = fma(-s * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-s * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+// JVP: the transpose of the acceleration Jacobian applied to lam, as in
+// walk64_kernel<..., JVP> (pair term G (F w + 2 F'(t) (r . w) r), w = m_k
+// lam_j - m_j lam_k, with F' = dF/d(r^2) from the same table), for
+// multipole_order 0 (no accepted nodes).
+template <int SCHEME, bool JVP = false>
+__global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4))) void walk64_box_kernel(
+    WalkParams P, WalkData D, WalkOutputs out, unsigned long long* __restrict__ counters) {
+  __shared__ double4 win[kWalkWaves][kWindow];
+  __shared__ double4 winl[JVP ? kWalkWaves : 1][JVP ? kWindow : 1];  // JVP: lam of the staged sources
+  __shared__ uint32_t ring[kWalkWaves][kBoxRing];
+  __shared__ double2 htab[3 * kH64Cells];
+  for (int i = threadIdx.x; i < 3 * P.h64_cells; i += kWalkThreads) htab[i] = D.h64tab[i];
+  __syncthreads();
+  const int w = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
+  const bool valid = k < P.n_targets;
+  const uint32_t tp = valid ? (P.identity_targets ? k : D.tpos[k]) : 0u;
+  const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
+  double px = 0.0, py = 0.0, pz = 0.0;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  double4 lk_ = make_double4(0.0, 0.0, 0.0, 0.0);  // JVP: lam of the target
+  double mk = 0.0;                                  // JVP: mass of the target
+  if (valid) {
+    const double4 t = D.src64[tp];
+    px = t.x;
+    py = t.y;
+    pz = t.z;
+    if constexpr (JVP) {
+      lk_ = D.lam64[tp];
+      mk = D.uniform_mass ? P.uniform_mass : t.w;
+      if (P.pm_enabled)
+        vjp_gather64<SCHEME>(P, D.f64x, D.f64y, D.f64z, D.nu64, lk_.x, lk_.y, lk_.z, mk, px, py, pz, gx, gy, gz);
+    } else {
+      if (P.pm_enabled) interp64<SCHEME>(P, D.f64x, D.f64y, D.f64z, px, py, pz, gx, gy, gz);
+    }
+  }
+  const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
+  const double iL0 = 1.0 / L0, iL1 = 1.0 / L1, iL2 = 1.0 / L2;
+  // minimum image by a reciprocal multiply: d - L rint(d / L) up to the
+  // rounding of d / L, which only matters at |d| within an ulp of L/2, far
+  // outside r_cut
+  auto mi_fast = [&](double d, int a) -> double {
+    const double L = a == 0 ? L0 : (a == 1 ? L1 : L2);
+    const double iL = a == 0 ? iL0 : (a == 1 ? iL1 : iL2);
+    return fma(-L, rint(d * iL), d);
+  };
+  const double rc = P.rcut, rc2 = P.rcut2;
+  // target box: offsets from lane 0's target (lane 0 is valid whenever any lane is)
+  const double o0 = readlane_f64(px, 0), o1 = readlane_f64(py, 0), o2 = readlane_f64(pz, 0);
+  const double ux = valid ? mi_fast(px - o0, 0) : 0.0, uy = valid ? mi_fast(py - o1, 1) : 0.0,
+               uz = valid ? mi_fast(pz - o2, 2) : 0.0;
+  // wave-uniform values through readlane, so they live in scalar registers
+  const double lox = readlane_f64(wave_min(ux), 0), hix = readlane_f64(wave_max(ux), 0),
+               loy = readlane_f64(wave_min(uy), 0), hiy = readlane_f64(wave_max(uy), 0),
+               loz = readlane_f64(wave_min(uz), 0), hiz = readlane_f64(wave_max(uz), 0);
+  const double cx = 0.5 * (lox + hix), cy = 0.5 * (loy + hiy), cz = 0.5 * (loz + hiz);
+  const double ex = 0.5 * (hix - lox) * (1.0 + 1.0e-12), ey = 0.5 * (hiy - loy) * (1.0 + 1.0e-12),
+               ez = 0.5 * (hiz - loz) * (1.0 + 1.0e-12);
+  const bool any = __ballot(valid) != 0ull;
+  const double emax = 2.0 * rc;  // the mixed walk's bound on the box half-extent
+  const bool boxed = any && ex <= emax && ey <= emax && ez <= emax && ex + rc < 0.5 * L0 * (1.0 - 1.0e-9) &&
+                     ey + rc < 0.5 * L1 * (1.0 - 1.0e-9) && ez + rc < 0.5 * L2 * (1.0 - 1.0e-9);
+  // box centre in the tree frame. Boxed waves stage each source as its image
+  // nearest the box centre and take the target's image likewise: for waves
+  // away from the seam both are the tree-frame coordinates themselves (the
+  // shift is zero), so r = x_j - x_k is the reference's difference exactly.
+  const double bx = o0 + cx, by = o1 + cy, bz = o2 + cz;
+  auto image = [&](double v, double b, int a) -> double {
+    const double L = a == 0 ? L0 : (a == 1 ? L1 : L2);
+    const double iL = a == 0 ? iL0 : (a == 1 ? iL1 : iL2);
+    return fma(-L, rint((v - b) * iL), v);
+  };
+  const double qx = image(px, bx, 0), qy = image(py, by, 1), qz = image(pz, bz, 2);
+  const double eps2 = P.eps2_64, c8 = P.c8_64, tscale = P.h64_scale;
+  const double cmax = static_cast<double>(P.h64_cells) - 1.0;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+
+  // residual pair kernel in FP64: G m_j [(r^2+eps^2)^{-3/2} - H(r^2/4a^2)/8a^3] r_vec (G m_j applied by the
+  // caller); with the derivative fp = dF/d(r^2) when asked (JVP)
+  auto residual = [&](double r2, double* fp) -> double {
+    const double sft = fmax(r2 + eps2, 1.0e-30);  // coincident unsoftened pairs: finite, times a zero vector
+    const double y = rsq64(sft);
+    const double soft = y * y * y;
+    const double t = fmin(r2 * tscale, cmax + 0.999999999999);
+    const int ci = static_cast<int>(t);
+    const double u = t - static_cast<double>(ci);
+    const double2 a01 = htab[3 * ci], a23 = htab[3 * ci + 1], a45 = htab[3 * ci + 2];
+    const double hv = fma(fma(fma(fma(fma(a45.y, u, a45.x), u, a23.y), u, a23.x), u, a01.y), u, a01.x);
+    if (fp) {
+      const double dh = fma(fma(fma(fma(5.0 * a45.y, u, 4.0 * a45.x), u, 3.0 * a23.y), u, 2.0 * a23.x), u, a01.y);
+      *fp = fma(-1.5 * soft, y * y, -c8 * dh * tscale);
+    }
+    return fma(-hv, c8, soft);
+  };
+  // F at r = 0 (the reference's limit: 0 without softening), for coincident pairs in the JVP
+  const double f_zero = JVP ? residual_inv_r3(0.0, P.eps, P.a) : 0.0;
+  // the staged window against this lane's target: NEAR = images nearest the box
+  // centre (boxed waves), else tree-frame coordinates with a minimum image per pair
+  auto pairs = [&]<bool NEAR>(uint32_t cn, double lim) {
+    double lx = 0.0, ly = 0.0, lz = 0.0;
+#pragma unroll 2
+    for (uint32_t si = 0; si < cn; ++si) {
+      const double4 sj = win[w][si];
+      double dx, dy, dz;
+      if constexpr (NEAR) {
+        dx = sj.x - qx;
+        dy = sj.y - qy;
+        dz = sj.z - qz;
+      } else {
+        dx = mi_fast(sj.x - px, 0);
+        dy = mi_fast(sj.y - py, 1);
+        dz = mi_fast(sj.z - pz, 2);
+      }
+      const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+      if constexpr (JVP) {
+        // the target itself: w = 0; a coincident source: F(0) w
+        double fp = 0.0;
+        double f = residual(r2, &fp);
+        f = r2 > 0.0 ? f : f_zero;
+        const double4 lj = winl[w][si];
+        double wx, wy, wz;
+        if (D.uniform_mass) {  // w / m (the mass goes into the final scale)
+          wx = lj.x - lk_.x;
+          wy = lj.y - lk_.y;
+          wz = lj.z - lk_.z;
+        } else {
+          wx = mk * lj.x - sj.w * lk_.x;
+          wy = mk * lj.y - sj.w * lk_.y;
+          wz = mk * lj.z - sj.w * lk_.z;
+        }
+        const bool in = r2 <= lim;
+        f = in ? f : 0.0;
+        const double cr = in ? 2.0 * fp * fma(dx, wx, fma(dy, wy, dz * wz)) : 0.0;
+        lx = fma(f, wx, fma(cr, dx, lx));
+        ly = fma(f, wy, fma(cr, dy, ly));
+        lz = fma(f, wz, fma(cr, dz, lz));
+      } else {
+        double f = residual(r2, nullptr);
+        if (!D.uniform_mass) f *= sj.w;
+        f = r2 <= lim ? f : 0.0;  // the target itself and coincident sources: zero vector
+        lx = fma(f, dx, lx);
+        ly = fma(f, dy, ly);
+        lz = fma(f, dz, lz);
+      }
+    }
+    const double gm = D.uniform_mass ? P.G * P.uniform_mass : P.G;
+    ax = fma(gm, lx, ax);
+    ay = fma(gm, ly, ay);
+    az = fma(gm, lz, az);
+  };
+  auto stage = [&]<bool NEAR>(uint32_t j, bool on) {
+    const double4 sv = D.src64[j];
+    if constexpr (JVP) {
+      const double4 lv = D.lam64[j];
+      if (on) winl[w][lane] = lv;
+    }
+    if (on) {
+      if constexpr (NEAR)
+        win[w][lane] = make_double4(image(sv.x, bx, 0), image(sv.y, by, 1), image(sv.z, bz, 2), sv.w);
+      else
+        win[w][lane] = sv;
+    }
+  };
+
+  // lane-by-lane walk of [nxt, end) with walk64_kernel's tests
+  uint32_t nxt = valid ? 0u : P.node_count;
+  auto walk = [&]<bool NEAR>(uint32_t end) {
+    uint32_t n = wave_min_u32(nxt);
+    while (n < end) {
+      const NodeLink lk = D.link[n];
+      const NodeGeo64 g = D.geo64[n];
+      const bool act = nxt == n;
+      if (!__ballot(act)) {
+        if (lane == 0) atomicOr(&counters[kFaultSlot], 1ull);
+        break;
+      }
+      const double h = g.half;
+      const double ndx = mi_fast(g.cx - px, 0), ndy = mi_fast(g.cy - py, 1), ndz = mi_fast(g.cz - pz, 2);
+      const double ax_ = fabs(ndx), ay_ = fabs(ndy), az_ = fabs(ndz);
+      const double fx_ = fmax(0.0, ax_ - h), fy_ = fmax(0.0, ay_ - h), fz_ = fmax(0.0, az_ - h);
+      const bool keep = act && within_cutoff(fx_ * fx_ + fy_ * fy_ + fz_ * fz_, P.rcut, P.rcut2);
+      if (act && !keep) nxt = n + lk.skip;
+      if (lk.child_count == 0) {
+        if (keep) nxt = n + 1;
+        if (__ballot(keep)) {
+          const double lim = keep ? rc2 : -1.0;
+          const uint32_t e0 = lk.begin + lk.count;
+          for (uint32_t c0 = lk.begin; c0 < e0; c0 += kWindow) {
+            const uint32_t cn = e0 - c0 < static_cast<uint32_t>(kWindow) ? e0 - c0 : static_cast<uint32_t>(kWindow);
+            __builtin_amdgcn_wave_barrier();
+            stage.template operator()<NEAR>(c0 + (static_cast<uint32_t>(lane) < cn ? lane : 0),
+                                            static_cast<uint32_t>(lane) < cn);
+            __builtin_amdgcn_wave_barrier();
+            pairs.template operator()<NEAR>(cn, lim);
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      } else if (__ballot(keep)) {
+        if constexpr (JVP) {
+          if (keep) nxt = n + 1;  // monopoles only: every kept internal node is opened
+        } else {
+          bool take_mp = false;
+          double mdx = 0.0, mdy = 0.0, mdz = 0.0;
+          if (keep) {
+            const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+            const double mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+            const bool contained = within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
+            const NodeMom64 M = D.mom64[n];
+            mdx = mi_fast(M.comx - px, 0);
+            mdy = mi_fast(M.comy - py, 1);
+            mdz = mi_fast(M.comz - pz, 2);
+            take_mp = accept64(h, D.quad64[n].offset, inside, contained, mdx * mdx + mdy * mdy + mdz * mdz, P);
+            nxt = take_mp ? n + lk.skip : n + 1;
+          }
+          if (__ballot(take_mp) && take_mp) node_accel64(D.mom64[n], D.quad64[n], mdx, mdy, mdz, P, ax, ay, az);
+        }
+      }
+      n = next_node(n, nxt);
+    }
+  };
+
+  if (boxed) {
+    // box classification of node q: 0 pruned, 1 kept leaf, 3 opened, 4 a
+    // subtree walked lane by lane
+    auto classify = [&](uint32_t q, uint32_t& next, uint32_t& lb, uint32_t& lc) -> int {
+      const NodeLink lk = D.link[q];
+      const NodeGeo64 g = D.geo64[q];
+      const double h = g.half;
+      const double dx = fmax(0.0, fabs(mi_fast(g.cx - bx, 0)) - h - ex);
+      const double dy = fmax(0.0, fabs(mi_fast(g.cy - by, 1)) - h - ey);
+      const double dz = fmax(0.0, fabs(mi_fast(g.cz - bz, 2)) - h - ez);
+      const bool keep = dx * dx + dy * dy + dz * dz <= rc2 * (1.0 + 1.0e-9);
+      if (lk.child_count == 0) {
+        lb = lk.begin;
+        lc = lk.count;
+        return keep ? 1 : 0;
+      }
+      if (!keep) {
+        next = q + lk.skip;
+        return 0;
+      }
+      if (!JVP && P.quad && h < 0.041 * rc) {
+        // accept64 needs 2h/r < 0.08: every target within 25 h of the COM fails it
+        const NodeMom64 M = D.mom64[q];
+        const double fx_ = fabs(mi_fast(M.comx - bx, 0)) + ex, fy_ = fabs(mi_fast(M.comy - by, 1)) + ey,
+                     fz_ = fabs(mi_fast(M.comz - bz, 2)) + ez;
+        if (!(fx_ * fx_ + fy_ * fy_ + fz_ * fz_ <= 625.0 * h * h * (1.0 - 1.0e-9))) {
+          next = q + lk.skip;
+          return 4;
+        }
+      }
+      return 3;
+    };
+    const double lim = valid ? rc2 : -1.0;
+    uint32_t head = 0, nl = 0;
+    auto eval_window = [&](uint32_t cn) {
+      __builtin_amdgcn_wave_barrier();
+      stage.template operator()<true>(ring[w][(head + (static_cast<uint32_t>(lane) < cn ? lane : 0)) % kBoxRing],
+                                      static_cast<uint32_t>(lane) < cn);
+      __builtin_amdgcn_wave_barrier();
+      pairs.template operator()<true>(cn, lim);
+      __builtin_amdgcn_wave_barrier();
+      head = (head + cn) % kBoxRing;
+      nl -= cn;
+    };
+    uint32_t p = 0;
+    while (p < P.node_count) {
+      const uint32_t q = p + lane;
+      uint32_t next = q + 1, lb = 0, lc = 0;
+      const int kind = q < P.node_count ? classify(q, next, lb, lc) : 3;
+      unsigned long long chain = 0;
+      uint32_t cur = p;
+      while (cur < p + kWave && cur < P.node_count) {
+        const uint32_t l = cur - p;
+        chain |= 1ull << l;
+        cur = readlane_u32(next, l);
+      }
+      const uint32_t base = p;
+      p = cur;
+      const bool on = (chain >> lane) & 1ull;
+      unsigned long long rest = __ballot(on && kind == 1);
+      while (rest) {
+        const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(rest)) - 1);
+        rest &= rest - 1ull;
+        const uint32_t b0 = readlane_u32(lb, l), c = readlane_u32(lc, l);
+        for (uint32_t c0 = 0; c0 < c; c0 += kWindow) {
+          const uint32_t cn = c - c0 < static_cast<uint32_t>(kWindow) ? c - c0 : static_cast<uint32_t>(kWindow);
+          if (static_cast<uint32_t>(lane) < cn) ring[w][(head + nl + lane) % kBoxRing] = b0 + c0 + lane;
+          nl += cn;
+          if (nl >= static_cast<uint32_t>(kWindow)) eval_window(kWindow);
+        }
+      }
+      unsigned long long sub = __ballot(on && kind == 4);
+      while (sub) {
+        const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(sub)) - 1);
+        sub &= sub - 1ull;
+        const uint32_t end = readlane_u32(next, l);
+        nxt = valid ? base + l : end;
+        walk.template operator()<true>(end);
+      }
+    }
+    if (nl) eval_window(nl);
+  } else {
+    walk.template operator()<false>(P.node_count);
+  }
+  if (!valid) return;
+  static_cast<double*>(out.ax)[slot] = gx + ax;
+  static_cast<double*>(out.ay)[slot] = gy + ay;
+  static_cast<double*>(out.az)[slot] = gz + az;
+  if (out.pm_x) {
+    static_cast<double*>(out.pm_x)[slot] = gx;
+    static_cast<double*>(out.pm_y)[slot] = gy;
+    static_cast<double*>(out.pm_z)[slot] = gz;
+  }
+  const uint32_t row = D.order[tp];
+  if (out.sr_x) {
+    static_cast<double*>(out.sr_x)[row] = ax;
+    static_cast<double*>(out.sr_y)[row] = ay;
+    static_cast<double*>(out.sr_z)[row] = az;
+  }
+}
+
 // H(t) = L(sqrt t) / t^{3/2} (tree_pm_split_kernel.hpp:109-128).
 double gaussian_h_exact(double t) {
   if (t < 0.015625) {
@@ -1931,13 +2478,94 @@ void launchmix(const WalkParams& p, const WalkData& d, const WalkOutputs& out, u
         p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
   else if (p.target_per_wave)
     walkmix_target_kernel<SCHEME, CUBIC><<<grid_for(p.n_targets, kWalkWaves), kWalkThreads, 0, s>>>(p, d, out);
+  else if (p.log_cells > 0)
+    walkmix_kernel<SCHEME, CUBIC, false, true, true>
+        <<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, p.log_cells * sizeof(float4), s>>>(
+            p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
   else
-    walkmix_kernel<SCHEME, CUBIC, false><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
+    walkmix_kernel<SCHEME, CUBIC, false, true><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
         p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
   CHUI_LAUNCH_CHECK("walkmix_kernel");
 }
 
 }  // namespace
+
+std::vector<float> log_residual_table(double scale, double eps, double a, int32_t& first_octave,
+                                      uint32_t& cells) {
+  // s = (r^2 + eps^2) * scale; F(s) = s^{-3/2} - c H(x(s)) with
+  // x(s) = (s / scale - eps^2) / (4 a^2) and c = scale^{-3/2} / (8 a^3).
+  cells = 0;
+  const float eps2_tab = static_cast<float>(eps * eps * scale);
+  if (!(eps2_tab > 0.0f) || !std::isfinite(eps2_tab)) return {};
+  const int e0 = std::ilogb(eps2_tab);
+  const int octaves = 7 - e0;
+  if (octaves < 1 || octaves > kLogMaxOctaves) return {};
+  const double c = std::pow(scale, -1.5) / (8.0 * a * a * a);
+  const double dxds = 1.0 / (scale * 4.0 * a * a);
+  auto F = [&](double sv) { return std::pow(sv, -1.5) - c * gaussian_h_exact((sv / scale - eps * eps) / (4.0 * a * a)); };
+  auto dF = [&](double sv) {
+    return -1.5 * std::pow(sv, -2.5) - c * gaussian_h_slope((sv / scale - eps * eps) / (4.0 * a * a)) * dxds;
+  };
+  const int K = octaves * kLogCellsPerOctave;
+  std::vector<float> tab(4 * static_cast<std::size_t>(K + 1), 0.0f);
+  const double u1 = std::ldexp(1.0, -kLogMantissaShift), u2 = u1 * u1, u3 = u2 * u1;
+  for (int o = 0; o < octaves; ++o) {
+    const double base = std::ldexp(1.0, e0 + o);
+    for (int j = 0; j < kLogCellsPerOctave; ++j) {
+      const double s0 = base * (1.0 + static_cast<double>(j) / kLogCellsPerOctave);
+      const double s1 = base * (1.0 + static_cast<double>(j + 1) / kLogCellsPerOctave);
+      const double h = s1 - s0;
+      const double f0 = F(s0), f1 = F(s1), d0 = h * dF(s0), d1 = h * dF(s1);
+      float* t = &tab[4 * static_cast<std::size_t>(o * kLogCellsPerOctave + j)];
+      t[0] = static_cast<float>(f0);
+      t[1] = static_cast<float>(d0 * u1);
+      t[2] = static_cast<float>((3.0 * (f1 - f0) - 2.0 * d0 - d1) * u2);
+      t[3] = static_cast<float>((2.0 * (f0 - f1) + d0 + d1) * u3);
+    }
+  }
+  first_octave = e0;
+  cells = static_cast<uint32_t>(K + 1);  // the last cell is zero: s >= 128, beyond r_cut
+  return tab;
+}
+
+std::vector<double> gaussian_quintic_table(double t_max, int cells) {
+  // Interpolation at the Chebyshev points of each cell: error below
+  // 2 (h/4)^6 max|H^(6)| / 6!, under 1e-16 for cells of width 0.04 and
+  // about 1e-13 at width 0.2 (t_max = 50 on 256 cells). The 6 x 6
+  // Vandermonde system in u is solved in long double.
+  const int K = cells;
+  const double h = t_max / K;
+  std::vector<double> tab(6 * static_cast<std::size_t>(K));
+  constexpr int D = 6;
+  long double un[D];
+  for (int j = 0; j < D; ++j)
+    un[j] = 0.5L + 0.5L * std::cos(static_cast<long double>(3.14159265358979323846264338327950288L) *
+                                   (2 * j + 1) / (2 * D));
+  for (int i = 0; i < K; ++i) {
+    long double A[D][D + 1];
+    for (int j = 0; j < D; ++j) {
+      long double pw = 1.0L;
+      for (int c = 0; c < D; ++c) {
+        A[j][c] = pw;
+        pw *= un[j];
+      }
+      A[j][D] = gaussian_h_exact((i + static_cast<double>(un[j])) * h);
+    }
+    for (int c = 0; c < D; ++c) {  // Gaussian elimination with partial pivoting
+      int piv = c;
+      for (int r = c + 1; r < D; ++r)
+        if (std::fabs(static_cast<double>(A[r][c])) > std::fabs(static_cast<double>(A[piv][c]))) piv = r;
+      for (int q = 0; q <= D; ++q) std::swap(A[c][q], A[piv][q]);
+      for (int r = 0; r < D; ++r) {
+        if (r == c) continue;
+        const long double f = A[r][c] / A[c][c];
+        for (int q = c; q <= D; ++q) A[r][q] -= f * A[c][q];
+      }
+    }
+    for (int c = 0; c < D; ++c) tab[6 * static_cast<std::size_t>(i) + c] = static_cast<double>(A[c][D] / A[c][c]);
+  }
+  return tab;
+}
 
 std::vector<float> gaussian_coefficient_table(WalkParams& p, double t_max) {
   // Cubic Hermite cells on [0, t_max]: value c0 + c1 u + c2 u^2 + c3 u^3 with u
@@ -1961,7 +2589,8 @@ std::vector<float> gaussian_coefficient_table(WalkParams& p, double t_max) {
 
 void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
                const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
-               const double* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s) {
+               const double* fz, const WalkOutputs& out, unsigned long long* counters, const double2* h64tab,
+               hipStream_t s) {
   if (p.n_targets == 0) return;
   WalkData d{};
   d.link = t.link.as<NodeLink>();
@@ -1982,6 +2611,14 @@ void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int unifo
     else
       walk64_target_kernel<1><<<grid_for(p.n_targets, kWalkWaves), kWalkThreads, 0, s>>>(p, d, out);
     CHUI_LAUNCH_CHECK("walk64_target_kernel");
+    return;
+  }
+  if (p.box64) {
+    d.h64tab = h64tab;
+    const unsigned grid = grid_for(p.n_targets, kWalkThreads);
+    if (p.assignment == 0) walk64_box_kernel<0><<<grid, kWalkThreads, 0, s>>>(p, d, out, counters);
+    else walk64_box_kernel<1><<<grid, kWalkThreads, 0, s>>>(p, d, out, counters);
+    CHUI_LAUNCH_CHECK("walk64_box_kernel");
     return;
   }
   if (p.assignment == 0) launch64<0>(p, d, out, counters, s);
@@ -2094,7 +2731,7 @@ std::vector<double> gaussian_hermite_monomials(const std::vector<double>& knots)
 
 void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,
                    const double* fx, const double* fy, const double* fz, const double* nu, const WalkOutputs& out,
-                   unsigned long long* counters, hipStream_t s) {
+                   unsigned long long* counters, const double2* h64tab, hipStream_t s) {
   if (p.n_targets == 0) return;
   WalkData d{};
   d.link = t.link.as<NodeLink>();
@@ -2109,13 +2746,22 @@ void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int u
   d.f64z = fz;
   d.lam64 = lam;
   d.nu64 = nu;
+  if (p.box64) {
+    d.h64tab = h64tab;
+    const unsigned grid = grid_for(p.n_targets, kWalkThreads);
+    if (p.assignment == 0) walk64_box_kernel<0, true><<<grid, kWalkThreads, 0, s>>>(p, d, out, counters);
+    else walk64_box_kernel<1, true><<<grid, kWalkThreads, 0, s>>>(p, d, out, counters);
+    CHUI_LAUNCH_CHECK("walk64_box_kernel (JVP)");
+    return;
+  }
   if (p.assignment == 0) launch64<0, true>(p, d, out, counters, s);
   else launch64<1, true>(p, d, out, counters, s);
 }
 
 void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int uniform_mass, const float4* htab,
-                const uint32_t* target_pos, const uint32_t* target_slot, const float* fx, const float* fy,
-                const float* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s) {
+                const float4* ltab, const uint32_t* target_pos, const uint32_t* target_slot, const float* fx,
+                const float* fy, const float* fz, const WalkOutputs& out, unsigned long long* counters,
+                hipStream_t s) {
   if (p.n_targets == 0) return;
   WalkData d{};
   d.link = t.link.as<NodeLink>();
@@ -2126,6 +2772,8 @@ void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int unifor
   d.order = t.order.as<uint32_t>();
   d.src32 = src;
   d.htab = htab;
+  d.ltab = ltab;
+  d.child = t.child_index.as<uint32_t>();
   d.uniform_mass = uniform_mass;
   d.tpos = target_pos;
   d.tslot = target_slot;

@@ -54,6 +54,21 @@ struct WalkParams {
   float rcut2_tab, eps2_tab, c8_tab, force_scale;
   float tab_to_len2;   // table-unit r^2 -> length^2 (nearest-pair lane)
   float accept_half_max;
+  // Box-classified mixed walk: largest half-extent of a wave's target box
+  // (separation units, per axis) for which the wave walks the tree once for
+  // its box instead of lane by lane.
+  float box_half_max[3];
+  // FP64 box-classified walk (solves without counters or nearest-pair lane):
+  // degree-5 cells of H on [0, t_max] (h64_cells of them, t = r^2 / 4a^2),
+  // r^2 -> cell coordinate, eps^2 and 1/(8 a^3).
+  // Mixed timed walk with the residual table indexed by the float bits of
+  // s = r^2 + eps^2 (log_cells > 0): cells, the (exponent, top-5-mantissa)
+  // field of the first cell and the index of the zero cell.
+  uint32_t log_cells, log_base, log_zero;
+  int box64;
+  int h64_cells;
+  double h64_scale;
+  double eps2_64, c8_64;
   // Adaptive acceptance (TreePmAcceptancePolicy::kAdaptiveRelative; FP64 walk):
   // estimator, com-distance cutoff policy, Gaussian-table leaf pairs, radial-
   // coefficient node expansion, maximum opening angle, alpha, acceleration
@@ -106,18 +121,28 @@ struct WalkOutputs {
 // leaves (per target), wave node visits, wave leaf-source slots and wave
 // pair-kernel iterations (per wave).
 constexpr int kWalkCounters = 8;
+// Counter words after the per-solve counters: the sticky traversal-fault word,
+// then the FP64 per-lane walk's checksum of its direct-pair set (a wrapping
+// sum of a 64-bit mix of (target row, source row) over every pair inside
+// r_cut, so it does not depend on the summation order).
+constexpr int kFaultSlot = kWalkCounters;
+constexpr int kPairChecksumSlot = kWalkCounters + 1;
+constexpr int kWalkCounterWords = kWalkCounters + 2;
 
 // Tree-ordered sources: FP64 (x, y, z, m); mixed (qx, qy, qz, float bits of m).
+// p.box64 selects the box-classified FP64 walk (h64tab: its table, device).
 void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
                const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
-               const double* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s);
+               const double* fz, const WalkOutputs& out, unsigned long long* counters, const double2* h64tab,
+               hipStream_t s);
 
 // Transpose of the total-force Jacobian applied to lam (FP64, all targets, tree
 // order identity). src and lam are tree-ordered; fx..fz the forward force
 // meshes and nu the adjoint potential on the same mesh; multipole order 0.
+// p.box64 selects the box-classified walk with the polynomial table (h64tab).
 void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,
                    const double* fx, const double* fy, const double* fz, const double* nu, const WalkOutputs& out,
-                   unsigned long long* counters, hipStream_t s);
+                   unsigned long long* counters, const double2* h64tab, hipStream_t s);
 // Per-target FP64 walk under the adaptive acceptance. gtab = Gaussian Hermite
 // knots (value, slope) x 4097 (required when p.lookup), prev = previous
 // acceleration magnitude per output slot (nullable).
@@ -147,12 +172,29 @@ std::vector<double> gaussian_hermite_monomials(const std::vector<double>& knots)
 // Leaf node of every tree position of the last built tree.
 void build_leaf_of_pos(const Tree& t, uint32_t* leaf_of_pos, hipStream_t s);
 
+// ltab: the bit-indexed residual table of the timed walk (used when p.log_cells > 0).
 void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int uniform_mass, const float4* htab,
-                const uint32_t* target_pos, const uint32_t* target_slot, const float* fx, const float* fy,
-                const float* fz, const WalkOutputs& out, unsigned long long* counters, hipStream_t s);
+                const float4* ltab, const uint32_t* target_pos, const uint32_t* target_slot, const float* fx,
+                const float* fy, const float* fz, const WalkOutputs& out, unsigned long long* counters,
+                hipStream_t s);
 
 // Host FP64 table of H(t) on [0, t_max] (kHTableCells cubic cells, 4 floats
 // each); sets p.t_to_cell.
 std::vector<float> gaussian_coefficient_table(WalkParams& p, double t_max);
+// Host table of the FP64 box walk: H(t) on [0, t_max] in `cells` cells, per
+// cell the monomial coefficients a0..a5 of the degree-5 polynomial in the cell
+// coordinate u in [0, 1) through the 6 Chebyshev points of the cell.
+std::vector<double> gaussian_quintic_table(double t_max, int cells);
+// Host table of the mixed timed walk: the residual coefficient as a function
+// of s = (r^2 + eps^2) * scale on 32 cubic cells per octave from the octave of
+// eps^2 * scale up to s = 128, plus one zero cell; 4 floats per cell (the
+// polynomial in the integer position of s in its cell). Empty when eps is 0
+// or more than kLogMaxOctaves octaves would be needed.
+constexpr int kLogMaxOctaves = 24;
+constexpr int kLogCellBits = 5;  // 32 cells per octave
+constexpr int kLogCellsPerOctave = 1 << kLogCellBits;
+constexpr int kLogMantissaShift = 23 - kLogCellBits;
+std::vector<float> log_residual_table(double scale, double eps, double a, int32_t& first_octave, uint32_t& cells);
+constexpr int kH64MaxCells = 256;
 
 }  // namespace chui

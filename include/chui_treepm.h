@@ -182,6 +182,12 @@ typedef struct chui_treepm_diagnostics {
   /* mixed precision: times a wave folded its register pair sums (at most 64
    * sources since the previous fold) into the lanes' compensated totals */
   uint64_t wave_sum_flushes;
+  /* FP64 per-lane walk (the walk of every FP64 solve with diagnostics): a
+   * wrapping sum over its direct pairs inside r_cut of a 64-bit mix of
+   * (target row, source row). Equal checksums mean, up to a 2^-64 chance, the
+   * same pair set; a gradient check uses it to find finite-difference steps
+   * that move a pair across r_cut. 0 for the other walks. */
+  uint64_t pair_set_checksum;
 } chui_treepm_diagnostics;
 
 typedef struct chui_treepm_context chui_treepm_context;

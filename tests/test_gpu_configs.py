@@ -1,0 +1,190 @@
+"""The benchmark workloads at their stated sizes on the GPU.
+
+- 128^3 particles (10% in 200 NFW halos), box 100 Mpc/h, mesh 256^3, leaf 16:
+  the FP64 solve equals the oracle (tree_pm_coupling.cpp:2433-4682 restated)
+  on every particle: short-range lanes to 1e-12 with identical walk counters,
+  totals to the 1e-5 relative RMS parity bar. The mixed solve stays within its
+  budget of the FP64 solve.
+- 256^3 particles (20% in 1000 halos), box 200 Mpc/h, mesh 512^3: too large
+  for the oracle in a test, so size-independent properties: mixed within its
+  budget of FP64 (relative L2 < 1e-4, p99 < 1e-3), net force sum m a small
+  against sum m |a|, and FP64 against the GPU FP64 Ewald sum on 512 sample
+  particles within the algorithm's own error (CIC, r_cut = 4.5 r_s).
+- 64^3 particles, box 50, mesh 128^3, ten KDK steps a 0.05 -> 0.5 with
+  checkpoints every two: the gradient of the density loss against central
+  differences at 1e-6 mean spacings along 16 random unit directions. A step
+  that moves a pair across r_cut makes the loss jump (the truncated residual
+  is 1.8e-2 of Newtonian at r_cut), so the pass rule detects those steps: the
+  FP64 walk's pair-set checksum over every force evaluation of the +h and -h
+  trajectories (chui_treepm_diagnostics.pair_set_checksum and the pair
+  counts). Directions whose pair sets agree must match to 1e-4 (median 1e-5);
+  at least a quarter of them must keep their pair sets (about one pair in
+  10^9 lies within a step of r_cut, so roughly half the directions cross).
+- The 48-box ensemble (16 x 32^3, 16 x 64^3, 12 x 96^3, 4 x 128^3 particles,
+  meshes 2 n_side): the batched solve gives every box the accelerations of a
+  separate single-box solve.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import error_metrics, options_struct, rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - GPU-only module
+    pytest.skip("no HIP device", allow_module_level=True)
+
+from cosmosim import ForceAccumulator, TreePmCoordinator, ics  # noqa: E402
+from cosmosim import _lib as L  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def _solve(w, opts, precision, coord, diagnostics=False):
+    opts.precision = precision
+    dt = torch.float64 if precision == L.PRECISION_FP64 else torch.float32
+    dev = to_dev(w.x, w.y, w.z, w.mass)
+    n = w.x.size
+    acc = tuple(torch.zeros(n, dtype=dt, device="cuda:0") for _ in range(3))
+    sr = tuple(torch.zeros(n, dtype=dt, device="cuda:0") for _ in range(3))
+    torch.cuda.synchronize()
+    res = coord.solve_active_set(*dev, ForceAccumulator(acc, None, None, sr), opts, want_diagnostics=diagnostics)
+    coord.synchronize()
+    out = np.stack([a.double().cpu().numpy() for a in acc]), np.stack([a.double().cpu().numpy() for a in sr])
+    return out[0], out[1], res.diagnostics
+
+
+@pytest.fixture(scope="module")
+def config2():
+    w = ics.config(1)
+    assert w.x.size == 128**3 and w.mesh == 256
+    opts = options_struct(w.box, eps=w.softening, leaf=16)
+    with TreePmCoordinator(w.mesh) as coord:
+        acc64, sr64, diag = _solve(w, opts, L.PRECISION_FP64, coord, diagnostics=True)
+        accmx, _, _ = _solve(w, opts, L.PRECISION_MIXED, coord)
+    return w, opts, acc64, sr64, diag, accmx
+
+
+def test_config2_fp64_equals_the_oracle(config2):
+    w, opts, acc64, sr64, diag, _ = config2
+    O.set_threads(16)
+    ref_sr, cnt, _ = O.short_range(w.x, w.y, w.z, w.mass, opts, w.mesh)
+    assert rel_l2(sr64, ref_sr) < 1e-12
+    assert diag["direct_pair_evaluations"] == cnt["pairs"]
+    assert diag["accepted_internal_multipoles"] == cnt["multipoles"]
+    assert diag["cutoff_pruned_nodes"] == cnt["pruned"]
+    assert diag["visited_nodes"] == cnt["visited"]
+    assert diag["accepted_leaves"] == cnt["leaves"]
+    ref_pm = O.pm_accel(w.x, w.y, w.z, w.mass, opts, w.mesh)
+    ref = np.stack(ref_pm) + np.stack(ref_sr)
+    assert rel_l2(acc64, ref) < 1e-5
+    print(f"128^3: pairs {cnt['pairs']}, multipoles {cnt['multipoles']}, "
+          f"total rel L2 {rel_l2(acc64, ref):.3e}")
+
+
+def test_config2_mixed_within_budget(config2):
+    _, _, acc64, _, _, accmx = config2
+    m = error_metrics(accmx, acc64)
+    print("128^3 mixed vs fp64:", m)
+    assert m["relative_l2"] < 1e-4
+    assert m["p99"] < 1e-3
+
+
+@pytest.fixture(scope="module")
+def config3():
+    w = ics.config(2)
+    assert w.x.size == 256**3 and w.mesh == 512
+    opts = options_struct(w.box, eps=w.softening, leaf=16)
+    with TreePmCoordinator(w.mesh) as coord:
+        acc64, _, _ = _solve(w, opts, L.PRECISION_FP64, coord)
+        accmx, _, _ = _solve(w, opts, L.PRECISION_MIXED, coord)
+        yield w, opts, acc64, accmx, coord
+
+
+def test_config3_mixed_within_budget(config3):
+    _, _, acc64, accmx, _ = config3
+    m = error_metrics(accmx, acc64)
+    print("256^3 mixed vs fp64:", m)
+    assert m["relative_l2"] < 1e-4
+    assert m["p99"] < 1e-3
+
+
+def test_config3_net_force(config3):
+    # Pair forces cancel in pairs; the PM force of a periodic mesh sums to
+    # zero up to the assignment's aliasing; accepted quadrupoles are not
+    # symmetric. The sum stays a small fraction of sum m |a|.
+    w, _, acc64, accmx, _ = config3
+    for acc in (acc64, accmx):
+        net = np.abs((w.mass[None, :] * acc).sum(axis=1))
+        scale = float(np.sum(w.mass * np.linalg.norm(acc, axis=0)))
+        print("256^3 net force / sum m|a|:", net / scale)
+        assert np.all(net < 1e-4 * scale)
+
+
+def test_config3_fp64_against_ewald(config3):
+    w, _, acc64, _, coord = config3
+    rng = np.random.default_rng(ics.BASE_SEED + 2)
+    samples = np.sort(rng.choice(w.x.size, 512, replace=False)).astype(np.uint32)
+    dx, dy, dz, dm = to_dev(w.x, w.y, w.z, w.mass)
+    dt = torch.from_numpy(samples).to("cuda:0")
+    out = tuple(torch.zeros(samples.size, dtype=torch.float64, device="cuda:0") for _ in range(3))
+    torch.cuda.synchronize()
+    coord.ewald(dx, dy, dz, dm, dt, (w.box,) * 3, ics.G_CODE, 11.8 / w.box, 0, 24, w.softening, *out)
+    coord.synchronize()
+    ref = np.stack([a.cpu().numpy() for a in out])
+    m = error_metrics(acc64[:, samples], ref)
+    print("256^3 fp64 vs Ewald (512 samples):", m)
+    # the algorithm's own error with CIC and r_cut = 4.5 r_s: 1.4e-2 relative
+    # L2 and 3.8e-2 p99 on 8192 samples
+    assert m["relative_l2"] < 2.5e-2
+    assert m["p99"] < 8e-2
+
+
+def test_config4_gradient_against_central_differences():
+    from cosmosim import adjoint
+
+    prob, psi, target = adjoint.adjoint_problem(n_side=64, steps=10, a_begin=0.05, a_end=0.5, checkpoint_every=2)
+    try:
+        assert prob.mesh == 128 and abs(prob.box - 50.0) < 1e-12
+        p = psi.to(torch.float64).clone().requires_grad_(True)
+        loss = adjoint.density_loss(p, target, prob)
+        (grad,) = torch.autograd.grad(loss, p)
+        step = 1e-6 * prob.box / prob.n_side
+        rows = adjoint.central_differences(prob, psi.to(torch.float64), target, grad, directions=16, step=step)
+    finally:
+        prob.close()
+    kept = [r for r in rows if not r["crosses_cutoff"]]
+    errs = np.array([r["relative_error"] for r in kept])
+    print(f"64^3 gradient: {len(kept)} of {len(rows)} directions keep their pair sets; "
+          f"relative errors {np.round(errs, 8).tolist()}; excluded "
+          f"{[round(r['relative_error'], 4) for r in rows if r['crosses_cutoff']]}")
+    assert len(kept) >= len(rows) // 4
+    assert np.median(errs) < 1e-5
+    assert errs.max() < 1e-4
+
+
+def test_config5_48_box_ensemble_batched_equals_separate():
+    from cosmosim import ensemble as E
+
+    specs = E.ensemble_specs()
+    assert len(specs) == 48 and sum(s.n for s in specs) == 23_724_032
+    assert sorted({s.mesh for s in specs}) == [64, 128, 192, 256]
+    opts = [E.box_options(s, L.PRECISION_MIXED) for s in specs]
+    with E.Ensemble(specs, workers=8) as ens:
+        state = E.EnsembleForward(ens, steps=1, precision=L.PRECISION_MIXED)
+        ens.solve(state.x, state.m, state.g, opts)
+        torch.cuda.synchronize()
+        worst = 0.0
+        for b, s in enumerate(specs):
+            batched = np.stack([a.double().cpu().numpy() for a in state.g[b]])
+            acc = tuple(torch.empty(s.n, dtype=torch.float32, device="cuda:0") for _ in range(3))
+            with TreePmCoordinator(s.mesh) as coord:
+                coord.solve_active_set(*state.x[b], state.m[b], ForceAccumulator(acc), opts[b])
+                coord.synchronize()
+            ref = np.stack([a.double().cpu().numpy() for a in acc])
+            err = rel_l2(batched, ref)
+            worst = max(worst, err)
+            assert err < 1e-5, (b, s)
+    print(f"48 boxes: worst batched-vs-separate relative L2 {worst:.2e}")

@@ -227,15 +227,23 @@ def test_active_subset_equals_full_rows():
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_walk_without_diagnostics_matches_counted_walk(precision):
-    # Diagnostic solves run a walk variant that also counts; the plain variant
-    # must produce the same short-range accelerations bit for bit.
+    # Diagnostic solves run a walk variant that also counts. The plain walks
+    # classify nodes against each wave's target box and sum the same pairs and
+    # multipoles in another order (FP64: with a polynomial table of the
+    # Gaussian term instead of erf/exp), so they agree to the round-off of the
+    # sums: 1e-12 in FP64, FP32 rounding in mixed precision.
     w = ics.config(1, n_side=32)
     opts = options_struct(w.box, eps=w.softening)
     _, _, sr_d, diag, _ = gpu_solve(w.x, w.y, w.z, w.mass, opts, w.mesh, precision=precision)
     _, _, sr_p, none, _ = gpu_solve(w.x, w.y, w.z, w.mass, opts, w.mesh, precision=precision, diagnostics=False)
     assert diag and not none
-    for a in range(3):
-        assert np.array_equal(sr_d[a], sr_p[a])
+    if precision == 0:
+        assert rel_l2(sr_p, sr_d) < 1e-12
+    else:
+        assert rel_l2(sr_p, sr_d) < 2e-6
+        per = np.linalg.norm(np.stack(sr_p) - np.stack(sr_d), axis=0) / np.maximum(
+            np.linalg.norm(np.stack(sr_d), axis=0), 1e-30)
+        assert per.max() < 1e-4
 
 
 @pytest.mark.parametrize("precision", [0, 1])

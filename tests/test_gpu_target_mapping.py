@@ -76,6 +76,40 @@ CASES = list(cases())
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_box_walk_matches_counted_lane_walk(case):
+    # The timed mixed walk classifies nodes against each wave's target box and
+    # walks lane by lane only the subtrees some lane may accept; the counting
+    # (diagnostic) walk goes lane by lane everywhere. Same pairs and
+    # multipoles, other summation order: FP32 rounding of the sums apart, the
+    # short-range lanes agree; a missed or doubled multipole or leaf would
+    # show at the 1e-3 level.
+    _, (x, y, z, m), o, mesh = case
+    o.precision = L.PRECISION_MIXED
+    o.target_mapping = L.TARGETS_PER_LANE
+    o.short_range_only = 1
+    dx, dy, dz, dm = to_dev(x, y, z, m)
+    lanes = []
+    for diagnostics in (True, False):
+        acc = tuple(empty_dev(x.size, torch.float32) for _ in range(3))
+        sr = tuple(torch.zeros(x.size, dtype=torch.float32, device="cuda:0") for _ in range(3))
+        torch.cuda.synchronize()
+        with TreePmCoordinator(mesh) as coord:
+            res = coord.solve_active_set_with_pm_cadence(dx, dy, dz, dm, ForceAccumulator(acc, None, None, sr), o,
+                                                         False, want_diagnostics=diagnostics)
+            coord.synchronize()
+        if diagnostics:
+            print(case[0], "multipoles", res.diagnostics["accepted_internal_multipoles"],
+                  "pairs", res.diagnostics["direct_pair_evaluations"])
+        lanes.append(np.stack([s_.double().cpu().numpy() for s_ in sr]))
+    o.short_range_only = 0
+    o.target_mapping = L.TARGETS_AUTO
+    counted, boxed = lanes
+    assert rel_l2(boxed, counted) < 2e-6
+    per = np.linalg.norm(boxed - counted, axis=0) / np.maximum(np.linalg.norm(counted, axis=0), 1e-30)
+    assert per.max() < 1e-4
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("fraction", [0.02, 0.3, 1.0])
 def test_per_wave_matches_per_lane(case, fraction):
     _, (x, y, z, m), o, mesh = case
@@ -115,8 +149,10 @@ def test_auto_mapping_short_range_only(case):
 @pytest.mark.parametrize("fraction", [0.02, 1.0])
 def test_fp64_per_wave_matches_per_lane_and_oracle(case, fraction):
     """The FP64 strict walk with one target per wave (walk64_target_kernel):
-    the same pairs and multipoles as the per-lane walk, summed in another
-    order, so both agree with each other and with the oracle to 1e-12."""
+    the same pairs and multipoles as the per-lane walk (the box-classified
+    walk with the polynomial H table when no counters are asked for), summed
+    in another order, so both agree with each other and with the oracle to
+    1e-12."""
     _, (x, y, z, m), o, mesh = case
     n = x.size
     active = None
@@ -126,10 +162,41 @@ def test_fp64_per_wave_matches_per_lane_and_oracle(case, fraction):
     wave_acc, wave_sr = solve_mixed(x, y, z, m, o, mesh, L.TARGETS_PER_WAVE, active, precision=fp64)
     lane_acc, lane_sr = solve_mixed(x, y, z, m, o, mesh, L.TARGETS_PER_LANE, active, precision=fp64)
     rows = np.arange(n) if active is None else active.astype(np.int64)
-    assert rel_l2(wave_sr[:, rows], lane_sr[:, rows]) < 1e-13
+    assert rel_l2(wave_sr[:, rows], lane_sr[:, rows]) < 1e-12
     assert rel_l2(wave_acc, lane_acc) < 1e-12
     ref, _, _ = O.short_range(x, y, z, m, o, mesh)
     assert rel_l2(wave_sr[:, rows], np.stack(ref)[:, rows]) < 1e-12
+    assert rel_l2(lane_sr[:, rows], np.stack(ref)[:, rows]) < 1e-12
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fp64_box_walk_matches_counted_walk(case):
+    """The timed FP64 walk (box classification, Chebyshev H table, v_rsq_f64
+    with Newton steps) against the counting FP64 walk (the reference's
+    arithmetic: erf/exp residual, exact minimum image): same pairs and
+    multipoles, so they agree to 1e-12 relative L2 and 1e-10 per particle."""
+    _, (x, y, z, m), o, mesh = case
+    o.precision = L.PRECISION_FP64
+    o.target_mapping = L.TARGETS_PER_LANE
+    o.short_range_only = 1
+    dx, dy, dz, dm = to_dev(x, y, z, m)
+    lanes = []
+    for diagnostics in (True, False):
+        acc = tuple(empty_dev(x.size, torch.float64) for _ in range(3))
+        sr = tuple(torch.zeros(x.size, dtype=torch.float64, device="cuda:0") for _ in range(3))
+        torch.cuda.synchronize()
+        with TreePmCoordinator(mesh) as coord:
+            coord.solve_active_set_with_pm_cadence(dx, dy, dz, dm, ForceAccumulator(acc, None, None, sr), o, False,
+                                                   want_diagnostics=diagnostics)
+            coord.synchronize()
+        lanes.append(np.stack([s_.cpu().numpy() for s_ in sr]))
+    o.short_range_only = 0
+    o.target_mapping = L.TARGETS_AUTO
+    o.precision = L.PRECISION_MIXED
+    counted, boxed = lanes
+    assert rel_l2(boxed, counted) < 1e-12
+    per = np.linalg.norm(boxed - counted, axis=0) / np.maximum(np.linalg.norm(counted, axis=0), 1e-300)
+    assert per.max() < 1e-10
 
 
 @pytest.mark.parametrize("precision", [L.PRECISION_FP64, L.PRECISION_MIXED])
@@ -146,6 +213,6 @@ def test_per_wave_tiny_sets(precision, n):
     o = options_struct(10.0, eps=0.02)
     wave_acc, wave_sr = solve_mixed(x, y, z, m, o, 16, L.TARGETS_PER_WAVE, precision=precision)
     lane_acc, lane_sr = solve_mixed(x, y, z, m, o, 16, L.TARGETS_PER_LANE, precision=precision)
-    tol = 1e-13 if precision == L.PRECISION_FP64 else 1e-6
+    tol = 1e-12 if precision == L.PRECISION_FP64 else 1e-6
     assert np.all(np.isfinite(wave_sr)) and rel_l2(wave_sr, lane_sr) <= tol
     assert rel_l2(wave_acc, lane_acc) <= (1e-12 if precision == L.PRECISION_FP64 else 1e-5)
