@@ -761,10 +761,10 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       }
       // The timed walk (no counters, 64 targets per wave) takes the residual
       // table indexed by the float bits of s = r^2 + eps^2, in units where
-      // r_cut^2 + eps^2 = 128; the counting and per-wave walks keep the
+      // r_cut^2 + eps^2 = 1; the counting and per-wave walks keep the
       // reciprocal square root and the Gaussian table above.
       if (!wp.count && !wp.target_per_wave) {
-        const double scale = 128.0 / (wp.rcut2 + wp.eps * wp.eps);
+        const double scale = 1.0 / (wp.rcut2 + wp.eps * wp.eps);
         const double key[3] = {scale, wp.eps, sp.a};
         if (!(c->ltab_key[0] == key[0] && c->ltab_key[1] == key[1] && c->ltab_key[2] == key[2])) {
           int32_t e0 = 0;
@@ -788,7 +788,6 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
           wp.force_scale = static_cast<float>(wp.G * scale * (uniform ? um : 1.0));
           wp.log_cells = c->ltab_cells;
           wp.log_base = static_cast<uint32_t>(c->ltab_first_octave + 127) << chui::kLogCellBits;
-          wp.log_zero = c->ltab_cells - 1;
         }
       }
       chui::walk_mixed(c->tree, wp, c->src32.as<uint4>(), uniform ? 1 : 0, c->htab.as<float4>(),

@@ -1381,26 +1381,54 @@ constexpr int kBoxRing = 2 * kWindow;
 // Residual table indexed by the float bits (LOG, the timed walk): the whole
 // pair coefficient F(s) = s^{-3/2} - c8 H((s/scale - eps^2) / 4a^2) as a
 // function of s = r^2 + eps^2 in table units, whose scale puts r_cut^2 + eps^2
-// at s = 128. The cell of s is its exponent and top 5 mantissa bits (32 cubic
-// Hermite cells per octave, from the octave of eps^2 up to s = 128, then one
+// at s = 1. The cell of s is its exponent and top 5 mantissa bits (32 cubic
+// Hermite cells per octave, from the octave of eps^2 up to s = 1, then one
 // zero cell), the position in the cell its low 18 mantissa bits as an integer
-// (the coefficients carry the 2^-18 powers). So a pair costs an add, a shift,
-// a subtract, a min, an LDS read and three FMAs instead of a reciprocal square
-// root, the Gaussian table and a cutoff select; pairs beyond r_cut land in the
-// zero cell. Cubic Hermite cells of relative width 1/32 to 1/64 keep F within
-// 1.5e-7 of its value.
+// (the coefficients carry the 2^-18 powers). eps^2 enters the FMA chain of
+// r^2, the chain's last FMA clamps s to [0, 1] (so every pair beyond r_cut
+// lands in the zero cell, the cell of s = 1), and the octave base is folded
+// into the table's LDS address. A pair then costs a shift, an address add, an
+// LDS read, the position (and, convert) and three FMAs instead of a
+// reciprocal square root, the Gaussian table and a cutoff select. Cubic
+// Hermite cells of relative width 1/32 to 1/64 keep F within 1.5e-7 of its
+// value.
+//
+// The box walk stages the sources of its kept leaves 64 at a time and drops
+// those farther than r_cut from the target box (they would all land in the
+// zero cell) before they enter the LDS window: the kept records are compacted
+// with a ballot and a lane prefix count, and the window is evaluated whenever
+// 64 records are in it.
 
+// 16-byte cell `cell` of an LDS table at LDS address `base`: the address is
+// one shift-add (the compiler folds a constant shift of a shifted value into
+// shift, and, add). Inline asm counts as convergent, so a loop holding it is
+// unrolled only with a constant trip count: the box walk's full windows use
+// it, the lane-by-lane walk (variable leaf counts) indexes the table plainly.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+using lds_f32x4 = __attribute__((address_space(3))) const f32x4;
+__device__ __forceinline__ float4 lds_cell(uint32_t base, uint32_t cell) {
+  uint32_t a;
+  asm("v_lshl_add_u32 %0, %1, 4, %2" : "=v"(a) : "v"(cell), "s"(base));
+  const f32x4 v = *(lds_f32x4*)static_cast<uintptr_t>(a);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// The box walk's last, partial window has a variable trip count, so its pair
+// loop stays rolled (see lds_cell); the unroller's warning about it is muted.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wpass-failed"
 template <int SCHEME, bool CUBIC, bool COUNT, bool BOX = false, bool LOG = false>
-__global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : 8)))
+__global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : 7)))
 void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long long* __restrict__ counters,
                     const NodeLink* __restrict__ node_link, const NodeGeo32* __restrict__ node_geo,
                     const NodeMom32* __restrict__ node_mom, const NodeQuad32* __restrict__ node_quad,
                     const NodeBox32* __restrict__ node_box) {
   // node records are read-only for the kernel's lifetime and indexed by the
   // wave-uniform node number, so they come through the scalar cache
-  __shared__ float4 win[kWalkWaves][kWindow];
+  // the box walk's window holds up to 127 compacted records between evaluations
+  __shared__ float4 win[kWalkWaves][BOX ? 2 * kWindow : kWindow];
   __shared__ float4 htab[LOG ? 1 : kHTableCells];
-  __shared__ uint32_t ring[kWalkWaves][kBoxRing];
+  __shared__ uint32_t list[kWalkWaves][BOX ? kWindow : 1];  // box walk: listed source indices
   // per lane: force totals [0..2] and their Kahan compensations [3..5]
   __shared__ float ksum[kWalkWaves][6][kWave];
   extern __shared__ float4 ltab[];  // LOG: P.log_cells cells
@@ -1446,6 +1474,7 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   uint32_t org[3];
   bool near = true;
   float box_half[3];  // the box's half-extents, separation units (covers every target)
+  float box_tab[3];   // the same in table units
   {
     const uint32_t pp[3] = {px, py, pz};
     const float sa[3] = {s0, s1, s2};
@@ -1461,6 +1490,7 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
       const int64_t half_extent = (static_cast<int64_t>(hi) - static_cast<int64_t>(lo)) / 2 + 1;
       near = near && half_extent <= static_cast<int64_t>(P.fast_half_extent[a]);
       box_half[a] = static_cast<float>(half_extent + 1) * sa[a];
+      box_tab[a] = static_cast<float>(half_extent + 1) * P.tab_scale[a];
     }
   }
   const uint32_t ox = org[0], oy = org[1], oz = org[2];
@@ -1511,7 +1541,10 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   // source has the uniform mass, which the force scale already carries; SEL:
   // lanes may carry a negative cutoff (they pruned the leaf or accepted an
   // ancestor), so their terms are dropped explicitly
-  const uint32_t log_base = P.log_base, log_zero = P.log_zero;
+  // LOG: cell (bits of s) >> 18 of the table is lcell[cell], at LDS address
+  // lbase + 16 * cell
+  const float4* const lcell = ltab - P.log_base;
+  const uint32_t lbase = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_f32x4*)ltab - P.log_base));
   auto pairs = [&]<bool NEAR, bool UMASS, bool SEL = true>(uint32_t c0, uint32_t cn, float lim) {
     float lx = 0.0f, ly = 0.0f, lz = 0.0f;
 #pragma unroll 4
@@ -1519,17 +1552,21 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
       const float4 sj = win[w][si];
       float dx, dy, dz;
       separation.template operator()<NEAR>(sj, dx, dy, dz);
-      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));  // = table coordinate
       float f;
       if constexpr (LOG) {
-        const uint32_t bits = __float_as_uint(r2 + eps2);
-        const uint32_t cell = min((bits >> kLogMantissaShift) - log_base, log_zero);
-        const float4 c = ltab[cell];
+        // s = r^2 + eps^2 clamped to [0, 1]: s >= eps^2 >= 2^first_octave, and
+        // s = 1 (r_cut and beyond) is the zero cell
+        const float sc = __builtin_amdgcn_fmed3f(fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2))), 0.0f, 1.0f);
+        const uint32_t bits = __float_as_uint(sc);
+        float4 c;
+        if constexpr (SEL) c = lcell[bits >> kLogMantissaShift];
+        else c = lds_cell(lbase, bits >> kLogMantissaShift);
         const float u = static_cast<float>(bits & ((1u << kLogMantissaShift) - 1u));
         float fr = fmaf(fmaf(fmaf(c.w, u, c.z), u, c.y), u, c.x);
         if constexpr (!UMASS) fr *= sj.w;
         f = (SEL && lim < 0.0f) ? 0.0f : fr;
       } else {
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));  // = table coordinate
         const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
         const float soft = rinv * rinv * rinv;
         const int ci = static_cast<int>(fminf(r2, static_cast<float>(kHTableCells - 1)));
@@ -1672,15 +1709,14 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
 
   auto box_walk = [&]() {
     const float lim = valid ? rc2 : -1.0f;
-    uint32_t head = 0, nl = 0;  // ring[(head + i) % kBoxRing], i < nl: listed sources
-    auto eval_window = [&](uint32_t cn) {
-      __builtin_amdgcn_wave_barrier();
-      {
-        const uint32_t e = (head + (static_cast<uint32_t>(lane) < cn ? lane : 0)) % kBoxRing;
-        const uint4 sv = D.src32[ring[w][e]];
-        const float mj = D.uniform_mass ? 1.0f : __uint_as_float(sv.w);
-        if (static_cast<uint32_t>(lane) < cn) win[w][lane] = stage_record.template operator()<true>(sv, mj);
-      }
+    // a source whose offset from the box centre lies beyond r_cut of the box
+    // (outside half-extent + 1 fixed-point unit per axis) is beyond r_cut of
+    // every target; the 1e-5 margin covers the rounding of the offsets, so
+    // every dropped source is one whose s reaches the zero cell
+    const float cull2 = rc2 * 1.00001f;
+    uint32_t nl = 0;            // list[w][0, nl): listed sources, staged 64 at a time
+    uint32_t wn = 0;            // compacted records in win[w][0, wn)
+    auto evaluate = [&](uint32_t cn) {
       __builtin_amdgcn_wave_barrier();
       if (pend + cn > static_cast<uint32_t>(kWindow)) fold();
       pend += cn;
@@ -1689,23 +1725,52 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
       if (D.uniform_mass) pairs.template operator()<true, true, false>(0, cn, lim);
       else pairs.template operator()<true, false, false>(0, cn, lim);
       __builtin_amdgcn_wave_barrier();
-      head = (head + cn) % kBoxRing;
-      nl -= cn;
     };
-    // the lanes' classified nodes: kept leaves' sources go to the ring,
-    // subtrees some lane may accept in are walked lane by lane
+    // stage the next cn listed sources, keep those within r_cut of the box
+    auto eval_window = [&](uint32_t cn) {
+      __builtin_amdgcn_wave_barrier();
+      const bool on = static_cast<uint32_t>(lane) < cn;
+      const uint4 sv = D.src32[list[w][on ? lane : 0]];
+      const float mj = D.uniform_mass ? 1.0f : __uint_as_float(sv.w);
+      const float4 rec = stage_record.template operator()<true>(sv, mj);
+      const float ex = fmaxf(fabsf(rec.x) - box_tab[0], 0.0f);
+      const float ey = fmaxf(fabsf(rec.y) - box_tab[1], 0.0f);
+      const float ez = fmaxf(fabsf(rec.z) - box_tab[2], 0.0f);
+      const bool keep = on && !(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > cull2);
+      const unsigned long long km = __ballot(keep);
+      const uint32_t at =
+          wn + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(km >> 32),
+                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(km), 0u));
+      if (keep) win[w][at] = rec;
+      wn += static_cast<uint32_t>(__popcll(km));
+      nl = 0;
+      if (wn >= static_cast<uint32_t>(kWindow)) {
+        evaluate(kWindow);
+        wn -= kWindow;
+        // wn < 64: move the records past the window to its front
+        if (static_cast<uint32_t>(lane) < wn) {
+          const float4 t = win[w][kWindow + lane];
+          win[w][lane] = t;
+        }
+      }
+    };
+    // the lanes' classified nodes: kept leaves' sources go to the list (a
+    // leaf may continue in the next batch), subtrees some lane may accept in
+    // are walked lane by lane
     auto take = [&](bool on, int kind, uint32_t q, uint32_t lb, uint32_t lc, uint32_t next) {
       unsigned long long rest = __ballot(on && kind == 1);
       while (rest) {
         const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(rest)) - 1);
         rest &= rest - 1ull;
         const uint32_t b0 = readlane_u32(lb, l), c = readlane_u32(lc, l);
-        for (uint32_t c0 = 0; c0 < c; c0 += kWindow) {
-          const uint32_t cn = c - c0 < static_cast<uint32_t>(kWindow) ? c - c0 : static_cast<uint32_t>(kWindow);
-          // nl < kWindow here, so the ring has room for cn <= kWindow more
-          if (static_cast<uint32_t>(lane) < cn) ring[w][(head + nl + lane) % kBoxRing] = b0 + c0 + lane;
+        for (uint32_t c0 = 0; c0 < c;) {
+          // nl < kWindow here: fill the list up to kWindow entries
+          const uint32_t room = static_cast<uint32_t>(kWindow) - nl;
+          const uint32_t cn = c - c0 < room ? c - c0 : room;
+          if (static_cast<uint32_t>(lane) < cn) list[w][nl + lane] = b0 + c0 + lane;
           nl += cn;
-          if (nl >= static_cast<uint32_t>(kWindow)) eval_window(kWindow);
+          c0 += cn;
+          if (nl == static_cast<uint32_t>(kWindow)) eval_window(kWindow);
         }
       }
       unsigned long long sub = __ballot(on && kind == 4);
@@ -1713,6 +1778,11 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
         const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(sub)) - 1);
         sub &= sub - 1ull;
         const uint32_t q4 = readlane_u32(q, l), end = readlane_u32(next, l);
+        // the lane-by-lane walk stages its leaves in the window's front
+        if (wn) {
+          evaluate(wn);
+          wn = 0;
+        }
         nxt = valid ? q4 : end;
         walk.template operator()<true>(end);
       }
@@ -1738,6 +1808,7 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
     };
     chain_walk(0, P.node_count);
     if (nl) eval_window(nl);
+    if (wn) evaluate(wn);
   };
 
   bool boxed = false;
@@ -1810,6 +1881,8 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   }
   if (COUNT && out.near2) static_cast<float*>(out.near2)[row] = nearest * P.tab_to_len2;
 }
+
+#pragma clang diagnostic pop
 
 // ---- mixed walk, one target per wave --------------------------------------------
 // For sparse active sets the 64 targets of a walkmix_kernel wave lie far
@@ -2498,7 +2571,7 @@ std::vector<float> log_residual_table(double scale, double eps, double a, int32_
   const float eps2_tab = static_cast<float>(eps * eps * scale);
   if (!(eps2_tab > 0.0f) || !std::isfinite(eps2_tab)) return {};
   const int e0 = std::ilogb(eps2_tab);
-  const int octaves = 7 - e0;
+  const int octaves = -e0;
   if (octaves < 1 || octaves > kLogMaxOctaves) return {};
   const double c = std::pow(scale, -1.5) / (8.0 * a * a * a);
   const double dxds = 1.0 / (scale * 4.0 * a * a);
@@ -2524,7 +2597,7 @@ std::vector<float> log_residual_table(double scale, double eps, double a, int32_
     }
   }
   first_octave = e0;
-  cells = static_cast<uint32_t>(K + 1);  // the last cell is zero: s >= 128, beyond r_cut
+  cells = static_cast<uint32_t>(K + 1);  // the last cell is zero: s >= 1, beyond r_cut
   return tab;
 }
 

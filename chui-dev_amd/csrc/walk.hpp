@@ -62,9 +62,9 @@ struct WalkParams {
   // degree-5 cells of H on [0, t_max] (h64_cells of them, t = r^2 / 4a^2),
   // r^2 -> cell coordinate, eps^2 and 1/(8 a^3).
   // Mixed timed walk with the residual table indexed by the float bits of
-  // s = r^2 + eps^2 (log_cells > 0): cells, the (exponent, top-5-mantissa)
-  // field of the first cell and the index of the zero cell.
-  uint32_t log_cells, log_base, log_zero;
+  // s = r^2 + eps^2 (log_cells > 0): cells (the last one, at s = 1, is zero)
+  // and the (exponent, top-5-mantissa) field of the first cell.
+  uint32_t log_cells, log_base;
   int box64;
   int h64_cells;
   double h64_scale;
@@ -187,7 +187,7 @@ std::vector<float> gaussian_coefficient_table(WalkParams& p, double t_max);
 std::vector<double> gaussian_quintic_table(double t_max, int cells);
 // Host table of the mixed timed walk: the residual coefficient as a function
 // of s = (r^2 + eps^2) * scale on 32 cubic cells per octave from the octave of
-// eps^2 * scale up to s = 128, plus one zero cell; 4 floats per cell (the
+// eps^2 * scale up to s = 1, plus one zero cell; 4 floats per cell (the
 // polynomial in the integer position of s in its cell). Empty when eps is 0
 // or more than kLogMaxOctaves octaves would be needed.
 constexpr int kLogMaxOctaves = 24;
