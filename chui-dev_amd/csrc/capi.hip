@@ -1623,6 +1623,7 @@ chui_status chui_treepm_tree_copy(chui_treepm_context* c, double* geometry, doub
     const std::size_t nn = c->tree.node_count;
     hipStream_t s = c->stream;
     if (nn > 0) {
+      chui::ensure_records64(c->tree, s);
       std::vector<chui::NodeLink> link(nn);
       std::vector<chui::NodeGeo64> geo(nn);
       std::vector<chui::NodeMom64> mom(nn);

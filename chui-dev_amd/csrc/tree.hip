@@ -518,13 +518,19 @@ struct PathArgs {
 };
 
 // 21-level octant path of every particle against the reference's FP64 centres.
-__global__ __launch_bounds__(256) void path_key_kernel(Axes3 p, std::size_t n, PathArgs a, uint64_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ vals) {
+// Also writes the sources in input order as canonical coordinates and mass
+// (double4), so that the tree-order gather reads one 32-byte record per source.
+__global__ __launch_bounds__(256) void path_key_kernel(Axes3 p, const double* __restrict__ m, std::size_t n,
+                                                       PathArgs a, uint64_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals, double4* __restrict__ packed) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double ux = unwrap_coord(p.p[0][i], a.u.L[0], a.u.canonical[0], a.u.anchor[0]);
-  const double uy = unwrap_coord(p.p[1][i], a.u.L[1], a.u.canonical[1], a.u.anchor[1]);
-  const double uz = unwrap_coord(p.p[2][i], a.u.L[2], a.u.canonical[2], a.u.anchor[2]);
+  const double x = p.p[0][i], y = p.p[1][i], z = p.p[2][i];
+  packed[i] = make_double4(wrap_coord(x, a.u.L[0], a.u.canonical[0]), wrap_coord(y, a.u.L[1], a.u.canonical[1]),
+                           wrap_coord(z, a.u.L[2], a.u.canonical[2]), m[i]);
+  const double ux = unwrap_coord(x, a.u.L[0], a.u.canonical[0], a.u.anchor[0]);
+  const double uy = unwrap_coord(y, a.u.L[1], a.u.canonical[1], a.u.anchor[1]);
+  const double uz = unwrap_coord(z, a.u.L[2], a.u.canonical[2], a.u.anchor[2]);
   double cx = a.c0[0], cy = a.c0[1], cz = a.c0[2], h = a.h0;
   uint64_t key = 0;
   for (int lvl = 0; lvl < kPathLevels; ++lvl) {
@@ -548,17 +554,15 @@ __global__ __launch_bounds__(256) void morton_keys_kernel(Axes3 p, std::size_t n
   vals[i] = static_cast<uint32_t>(i);
 }
 
-// Morton key of the particle at each tree position (unwrapped coordinates).
 // Tree-order copy of the sources: canonical coordinates (wrap_coord) and mass.
-// One gather through the sort order; everything after it reads the copy.
-__global__ __launch_bounds__(256) void gather_sources_kernel(Axes3 p, const double* __restrict__ m, std::size_t n,
-                                                             UnwrapArgs u, const uint32_t* __restrict__ order,
+// One gather through the sort order of path_key_kernel's input-order records;
+// everything after it reads the copy.
+__global__ __launch_bounds__(256) void gather_sources_kernel(const double4* __restrict__ packed, std::size_t n,
+                                                             const uint32_t* __restrict__ order,
                                                              double4* __restrict__ pk) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t r = order[i];
-  pk[i] = make_double4(wrap_coord(p.p[0][r], u.L[0], u.canonical[0]), wrap_coord(p.p[1][r], u.L[1], u.canonical[1]),
-                       wrap_coord(p.p[2][r], u.L[2], u.canonical[2]), m[r]);
+  pk[i] = packed[order[i]];
 }
 
 // unwrap_coord of a canonical coordinate
@@ -588,100 +592,190 @@ __device__ __forceinline__ uint32_t lower_bound_digit(const uint64_t* __restrict
   return b;
 }
 
-struct LevelArgs {
-  uint32_t lvl_begin, lvl_count, next_begin, leaf_size;
+// ---- level discovery driven from the device ---------------------------------
+// One level of the octree per three launches, with the level's node range read
+// from the device (lv[d], lv[d + 1]), so the host issues every level without
+// reading anything back and reads the level starts and flags once at the end.
+// A level's nodes are split into kLevelBlocks contiguous chunks, one per
+// block. Count: eight threads per node, thread o of a splitting node
+// binary-searches the first particle of octant o (the searches are
+// independent, so a level costs one search depth rather than seven); the
+// octant boundaries are kept for the emit pass and the group counts the
+// non-empty octants. Offsets: one block turns the block sums into offsets and
+// the next level's start. Emit: each block numbers its children in node order,
+// so children are in BFS order and each node's children are contiguous in
+// octant order.
+constexpr int kLevelBlocks = 2048;
+constexpr int kMaxLevels = kPathLevels + 1;            // depths 0..21
+constexpr int kLvTooDeep = kMaxLevels + 2;              // lv[0, kMaxLevels + 2): level starts
+constexpr int kLvOverflow = kLvTooDeep + 1;             // children beyond the node capacity
+constexpr int kLvDeepSplit = kLvTooDeep + 2;            // a node below the partial sort's depth splits
+constexpr int kLvLeaves = kLvTooDeep + 3;               // nodes without children
+constexpr int kLvWords = kLvTooDeep + 4;
+
+struct LevelDevArgs {
+  uint32_t d, leaf_size, capacity, sorted_levels;
 };
 
-// Eight threads per node: thread o of a splitting node binary-searches the
-// first particle of octant o (the searches are independent, so a level costs
-// one search depth rather than seven), the octant boundaries are kept for the
-// emit pass and the group counts the non-empty octants.
-__global__ __launch_bounds__(256) void level_count_kernel(LevelArgs a, const uint32_t* __restrict__ nb,
-                                                          const uint32_t* __restrict__ nc, uint32_t* __restrict__ meta,
-                                                          const double4* __restrict__ geo,
-                                                          const uint64_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ nchild, uint32_t* __restrict__ bounds,
-                                                          uint32_t* __restrict__ flags) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = t >> 3, o = t & 7u;
-  const bool in = k < a.lvl_count;
-  const uint32_t node = a.lvl_begin + (in ? k : 0u);
-  const uint32_t cnt = in ? nc[node] : 0u;
-  const double half = in ? geo[node].w : 0.0;
-  const uint32_t depth = in ? (meta[node] & 0xFFu) : 0u;
-  const bool leaf = cnt <= a.leaf_size || half <= 1.0e-12;  // tree_gravity.cpp:1133
-  const bool too_deep = !leaf && depth >= static_cast<uint32_t>(kPathLevels);
-  const bool split = in && !leaf && !too_deep;
-  const uint32_t b = in ? nb[node] : 0u, e = b + cnt;
-  uint32_t lo = b;
-  if (split && o > 0) lo = lower_bound_digit(keys, b, e, 3 * (kPathLevels - 1 - static_cast<int>(depth)), o);
-  // octant o covers [lo, hi) with hi the next thread's lo (e for octant 7)
-  const uint32_t nxt = __shfl_down(lo, 1, 8);
-  const uint32_t hi = o == 7u ? e : nxt;
-  uint32_t c = split && hi > lo ? 1u : 0u;
-  c += __shfl_xor(c, 1, 8);
-  c += __shfl_xor(c, 2, 8);
-  c += __shfl_xor(c, 4, 8);
-  if (!in) return;
-  if (split) bounds[8ull * k + o] = lo;
-  if (o != 0u) return;
-  if (split) {
-    nchild[node] = c;
-    return;
+// exclusive prefix sum over the block (blockDim.x a multiple of the wave); total to *total
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+  const int lane = lane_id(), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  uint32_t inc = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t t = __shfl_up(inc, off, kWave);
+    if (lane >= off) inc += t;
   }
-  nchild[node] = 0;
-  meta[node] |= 1u << 16;
-  if (too_deep) atomicOr(&flags[0], 1u);
+  if (lane == kWave - 1) wsum[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int i = 0; i < nw; ++i) {
+      const uint32_t t = wsum[i];
+      wsum[i] = run;
+      run += t;
+    }
+    *total = run;
+  }
+  __syncthreads();
+  const uint32_t r = wsum[w] + inc - v;
+  __syncthreads();
+  return r;
 }
 
-// Number of leaves (nodes without children), one atomic per wave.
-__global__ __launch_bounds__(256) void count_leaves_kernel(const uint32_t* __restrict__ nchild, std::size_t total,
-                                                           uint32_t* __restrict__ out) {
-  uint32_t c = 0;
-  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
-       i += static_cast<std::size_t>(gridDim.x) * blockDim.x)
-    c += nchild[i] == 0 ? 1u : 0u;
-  const unsigned long long w = wave_sum(static_cast<unsigned long long>(c));
-  if (lane_id() == 0 && w) atomicAdd(out, static_cast<uint32_t>(w));
-}
-
-__global__ __launch_bounds__(256) void level_emit_kernel(LevelArgs a, uint32_t* __restrict__ nb,
-                                                         uint32_t* __restrict__ nc, uint32_t* __restrict__ meta,
-                                                         double4* __restrict__ geo,
-                                                         const uint32_t* __restrict__ bounds,
-                                                         const uint32_t* __restrict__ nchild,
-                                                         const uint32_t* __restrict__ cbase,
-                                                         uint32_t* __restrict__ first_child) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.lvl_count) return;
-  const uint32_t node = a.lvl_begin + k;
-  if (nchild[node] == 0) {
-    first_child[node] = kInvalidIndex;
-    return;
-  }
-  const uint32_t depth = meta[node] & 0xFFu;
-  const uint32_t e = nb[node] + nc[node];
-  const double4 g = geo[node];
-  const uint32_t fc = a.next_begin + cbase[k];
-  first_child[node] = fc;
-  uint32_t j = 0;
-  for (uint32_t o = 0; o < 8; ++o) {
-    const uint32_t lo = bounds[8ull * k + o];
-    const uint32_t hi = o == 7 ? e : bounds[8ull * k + o + 1];
-    if (hi > lo) {
-      const uint32_t c = fc + j++;
-      nb[c] = lo;
-      nc[c] = hi - lo;
-      meta[c] = (depth + 1u) | (o << 8);
-      const double ch = 0.5 * g.w;  // tree_gravity.cpp:1180-1184
-      double4 cg;
-      cg.x = g.x + ((o & 4u) ? ch : -ch);
-      cg.y = g.y + ((o & 2u) ? ch : -ch);
-      cg.z = g.z + ((o & 1u) ? ch : -ch);
-      cg.w = ch;
-      geo[c] = cg;
+__global__ __launch_bounds__(256) void level_count_dev_kernel(LevelDevArgs a, uint32_t* __restrict__ lv,
+                                                              const uint32_t* __restrict__ nb,
+                                                              const uint32_t* __restrict__ nc,
+                                                              uint32_t* __restrict__ meta,
+                                                              const double4* __restrict__ geo,
+                                                              const uint64_t* __restrict__ keys,
+                                                              uint32_t* __restrict__ nchild,
+                                                              uint32_t* __restrict__ bounds,
+                                                              uint32_t* __restrict__ bsum) {
+  __shared__ uint32_t wsum[256 / kWave];
+  __shared__ uint32_t tot;
+  const uint32_t lb = lv[a.d], cnt = lv[a.d + 1] - lb;
+  const uint32_t chunk = (cnt + gridDim.x - 1) / gridDim.x;
+  const uint32_t k0 = min(cnt, blockIdx.x * chunk), k1 = min(cnt, k0 + chunk);
+  uint32_t sum = 0, leaves = 0;
+  bool deep = false, too = false;
+  for (uint32_t base = k0; base < k1; base += blockDim.x / 8) {  // the same trip count for the whole block
+    const uint32_t k = base + threadIdx.x / 8, o = threadIdx.x & 7u;
+    const bool in = k < k1;
+    const uint32_t node = lb + (in ? k : k0);
+    const uint32_t count = in ? nc[node] : 0u;
+    const double half = in ? geo[node].w : 0.0;
+    const uint32_t depth = in ? (meta[node] & 0xFFu) : 0u;
+    const bool leaf = count <= a.leaf_size || half <= 1.0e-12;  // tree_gravity.cpp:1133
+    const bool too_deep = in && !leaf && depth >= static_cast<uint32_t>(kPathLevels);
+    const bool split = in && !leaf && !too_deep;
+    const uint32_t b = in ? nb[node] : 0u, e = b + count;
+    uint32_t lo = b;
+    if (split && o > 0) lo = lower_bound_digit(keys, b, e, 3 * (kPathLevels - 1 - static_cast<int>(depth)), o);
+    const uint32_t nxt = __shfl_down(lo, 1, 8);
+    const uint32_t hi = o == 7u ? e : nxt;
+    uint32_t c = split && hi > lo ? 1u : 0u;
+    c += __shfl_xor(c, 1, 8);
+    c += __shfl_xor(c, 2, 8);
+    c += __shfl_xor(c, 4, 8);
+    if (split) bounds[8ull * k + o] = lo;
+    if (in && o == 0u) {
+      if (split) {
+        nchild[node] = c;
+        sum += c;
+        deep = deep || depth + 1u > a.sorted_levels;
+      } else {
+        nchild[node] = 0;
+        meta[node] |= 1u << 16;
+        too = too || too_deep;
+        ++leaves;
+      }
     }
   }
+  if (deep) atomicOr(&lv[kLvDeepSplit], 1u);
+  if (too) atomicOr(&lv[kLvTooDeep], 1u);
+  (void)block_exclusive_scan(leaves, wsum, &tot);
+  if (threadIdx.x == 0 && tot) atomicAdd(&lv[kLvLeaves], tot);
+  (void)block_exclusive_scan(sum, wsum, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// One block of kLevelBlocks / 2 threads: block sums -> exclusive offsets and
+// the next level's start (or the overflow flag).
+__global__ __launch_bounds__(kLevelBlocks / 2) void level_offsets_dev_kernel(LevelDevArgs a, uint32_t* __restrict__ lv,
+                                                                            uint32_t* __restrict__ bsum) {
+  __shared__ uint32_t wsum[kLevelBlocks / 2 / kWave];
+  __shared__ uint32_t tot;
+  const uint32_t i = 2 * threadIdx.x;
+  const uint32_t v0 = bsum[i], v1 = bsum[i + 1];
+  const uint32_t off = block_exclusive_scan(v0 + v1, wsum, &tot);
+  bsum[i] = off;
+  bsum[i + 1] = off + v0;
+  if (threadIdx.x == 0) {
+    const uint32_t le = lv[a.d + 1];
+    const uint64_t next = static_cast<uint64_t>(le) + tot;
+    const bool over = lv[kLvOverflow] != 0u || next > a.capacity;
+    if (over) lv[kLvOverflow] = 1u;
+    lv[a.d + 2] = over ? le : static_cast<uint32_t>(next);
+  }
+}
+
+__global__ __launch_bounds__(256) void level_emit_dev_kernel(LevelDevArgs a, const uint32_t* __restrict__ lv,
+                                                             uint32_t* __restrict__ nb, uint32_t* __restrict__ nc,
+                                                             uint32_t* __restrict__ meta, double4* __restrict__ geo,
+                                                             const uint32_t* __restrict__ bounds,
+                                                             const uint32_t* __restrict__ nchild,
+                                                             const uint32_t* __restrict__ bsum,
+                                                             uint32_t* __restrict__ first_child) {
+  __shared__ uint32_t wsum[256 / kWave];
+  __shared__ uint32_t tot;
+  if (lv[kLvOverflow]) return;  // block-uniform
+  const uint32_t lb = lv[a.d], le = lv[a.d + 1], cnt = le - lb;
+  const uint32_t chunk = (cnt + gridDim.x - 1) / gridDim.x;
+  const uint32_t k0 = min(cnt, blockIdx.x * chunk), k1 = min(cnt, k0 + chunk);
+  uint32_t run = le + bsum[blockIdx.x];
+  for (uint32_t base = k0; base < k1; base += blockDim.x) {
+    const uint32_t k = base + threadIdx.x;
+    const bool in = k < k1;
+    const uint32_t node = lb + (in ? k : k0);
+    const uint32_t c = in ? nchild[node] : 0u;
+    const uint32_t pre = block_exclusive_scan(c, wsum, &tot);
+    if (in) {
+      if (c == 0) {
+        first_child[node] = kInvalidIndex;
+      } else {
+        const uint32_t depth = meta[node] & 0xFFu;
+        const uint32_t e = nb[node] + nc[node];
+        const double4 g = geo[node];
+        const uint32_t fc = run + pre;
+        first_child[node] = fc;
+        uint32_t j = 0;
+        for (uint32_t o = 0; o < 8; ++o) {
+          const uint32_t lo = bounds[8ull * k + o];
+          const uint32_t hi = o == 7 ? e : bounds[8ull * k + o + 1];
+          if (hi > lo) {
+            const uint32_t ch = fc + j++;
+            nb[ch] = lo;
+            nc[ch] = hi - lo;
+            meta[ch] = (depth + 1u) | (o << 8);
+            const double chh = 0.5 * g.w;  // tree_gravity.cpp:1180-1184
+            double4 cg;
+            cg.x = g.x + ((o & 4u) ? chh : -chh);
+            cg.y = g.y + ((o & 2u) ? chh : -chh);
+            cg.z = g.z + ((o & 1u) ? chh : -chh);
+            cg.w = chh;
+            geo[ch] = cg;
+          }
+        }
+      }
+    }
+    run += tot;
+  }
+}
+
+__global__ void level_dev_init_kernel(uint32_t* lv) {
+  const int i = threadIdx.x;
+  if (i < kLvWords) lv[i] = i == 1 ? 1u : 0u;
 }
 
 __global__ void root_init_kernel(uint32_t* nb, uint32_t* nc, uint32_t* meta, double4* geo, uint32_t n, double cx,
@@ -924,7 +1018,8 @@ __device__ __forceinline__ uint32_t to_fixed(double c, double scale) {
 
 struct RecordArgs {
   uint32_t total;
-  int mixed;
+  int fp64;   // FP64 records and child indices
+  int mixed;  // fixed-point/FP32 records and tight boxes
   int cubic;  // separation units of the mixed walk: fixed point (cubic box) or lengths
   double fscale[3];
 };
@@ -935,7 +1030,8 @@ __device__ __forceinline__ float float_up(double x) {
   return static_cast<double>(f) < x ? nextafterf(f, INFINITY) : f;
 }
 
-// BFS -> preorder records (FP64 always, fixed-point/FP32 in mixed mode).
+// BFS -> preorder records: links always, FP64 records and/or the mixed walk's
+// fixed-point/FP32 records.
 __global__ __launch_bounds__(256) void records_kernel(RecordArgs a, const uint32_t* __restrict__ pre,
                                                       const uint32_t* __restrict__ nb, const uint32_t* __restrict__ nc,
                                                       const uint32_t* __restrict__ nchild,
@@ -960,18 +1056,20 @@ __global__ __launch_bounds__(256) void records_kernel(RecordArgs a, const uint32
   const double4 g = geo[node];
   const double4 M = mom[node];
   const double* q = quad + 8ull * node;
-  geo64[p] = NodeGeo64{g.x, g.y, g.z, g.w};
-  mom64[p] = NodeMom64{M.x, M.y, M.z, M.w};
-  quad64[p] = NodeQuad64{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]};
-  uint32_t ci[8];
+  if (a.fp64) {
+    geo64[p] = NodeGeo64{g.x, g.y, g.z, g.w};
+    mom64[p] = NodeMom64{M.x, M.y, M.z, M.w};
+    quad64[p] = NodeQuad64{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]};
+    uint32_t ci[8];
 #pragma unroll
-  for (int o = 0; o < 8; ++o) ci[o] = kInvalidIndex;
-  for (uint32_t j = 0; j < c; ++j) {
-    const uint32_t ch = first_child[node] + j;
-    ci[(meta[ch] >> 8) & 7u] = pre[ch];
+    for (int o = 0; o < 8; ++o) ci[o] = kInvalidIndex;
+    for (uint32_t j = 0; j < c; ++j) {
+      const uint32_t ch = first_child[node] + j;
+      ci[(meta[ch] >> 8) & 7u] = pre[ch];
+    }
+#pragma unroll
+    for (int o = 0; o < 8; ++o) child_index[8ull * p + o] = ci[o];
   }
-#pragma unroll
-  for (int o = 0; o < 8; ++o) child_index[8ull * p + o] = ci[o];
   if (a.mixed) {
     geo32[p] = NodeGeo32{to_fixed(g.x, a.fscale[0]), to_fixed(g.y, a.fscale[1]), to_fixed(g.z, a.fscale[2]),
                          static_cast<float>(g.w)};
@@ -1137,7 +1235,39 @@ void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, co
   }
 }
 
+void launch_records(Tree& t, bool fp64, bool mixed, hipStream_t s) {
+  const std::size_t total = t.node_count;
+  if (fp64) {
+    t.geo64.reserve(total * sizeof(NodeGeo64));
+    t.mom64.reserve(total * sizeof(NodeMom64));
+    t.quad64.reserve(total * sizeof(NodeQuad64));
+    t.child_index.reserve(total * 8 * sizeof(uint32_t));
+  }
+  RecordArgs ra{};
+  ra.total = static_cast<uint32_t>(total);
+  ra.fp64 = fp64 ? 1 : 0;
+  ra.mixed = mixed ? 1 : 0;
+  ra.cubic = t.cubic ? 1 : 0;
+  for (int a = 0; a < 3; ++a) ra.fscale[a] = t.fscale[a];
+  records_kernel<<<grid_for(total, 256), 256, 0, s>>>(
+      ra, t.bfs_pre.as<uint32_t>(), t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(),
+      t.bfs_nchild.as<uint32_t>(), t.bfs_first_child.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
+      t.bfs_subtree.as<uint32_t>(), t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>(),
+      t.link.as<NodeLink>(), fp64 ? t.geo64.as<NodeGeo64>() : nullptr, fp64 ? t.mom64.as<NodeMom64>() : nullptr,
+      fp64 ? t.quad64.as<NodeQuad64>() : nullptr, fp64 ? t.child_index.as<uint32_t>() : nullptr,
+      mixed ? t.geo32.as<NodeGeo32>() : nullptr, mixed ? t.mom32.as<NodeMom32>() : nullptr,
+      mixed ? t.quad32.as<NodeQuad32>() : nullptr, mixed ? t.bfs_box.as<double>() : nullptr,
+      mixed ? t.box32.as<NodeBox32>() : nullptr);
+  CHUI_LAUNCH_CHECK("records_kernel");
+  if (fp64) t.records64 = true;
+}
+
 }  // namespace
+
+void ensure_records64(Tree& t, hipStream_t s) {
+  if (t.records64 || t.node_count == 0) return;
+  launch_records(t, true, false, s);
+}
 
 // ---------------------------------------------------------------------------
 void build_tree(Tree& t, const double* x, const double* y, const double* z, const double* m, const TreeParams& P,
@@ -1192,7 +1322,9 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
   pa.h0 = t.root_half;
   uint64_t* keys = t.keys.as<uint64_t>();
   uint32_t* order = t.order.as<uint32_t>();
-  path_key_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, pa, keys, order);
+  t.pk.reserve(n * sizeof(double4));
+  t.pk_in.reserve(n * sizeof(double4));
+  path_key_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, m, n, pa, keys, order, t.pk_in.as<double4>());
   CHUI_LAUNCH_CHECK("path_key_kernel");
   // Sort by the bits of levels 1..16 only (6 digit passes instead of 8). This
   // is the reference's partition order for every node up to depth 16, and the
@@ -1205,7 +1337,10 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
                    3 * kPathLevels, t.sort_scratch.ptr, s);
   bool partial = true;
 
-  // Level-synchronous discovery.
+  // Level-synchronous discovery. Every level is issued from the host without a
+  // read-back (the kernels read each level's range from lv); the level starts
+  // and flags come back once. A node capacity too small for the tree, or a
+  // depth-16 node that splits while only levels 1..16 are sorted, restarts it.
   const std::size_t initial_cap = std::max<std::size_t>(1024, n / 4 + 64);
   auto ensure = [&](std::size_t need, std::size_t used) {
     if (need <= t.bfs_capacity && t.bfs_begin.ptr) return;
@@ -1219,55 +1354,59 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
     t.bfs_capacity = nc;
   };
   ensure(initial_cap, 0);
-  t.flags.reserve(16);
-restart_discovery:
-  fill_u32(t.flags.as<uint32_t>(), 0u, 4, s);
-  root_init_kernel<<<1, 1, 0, s>>>(t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
-                                   t.bfs_geo.as<double4>(), static_cast<uint32_t>(n), t.root_centre[0],
-                                   t.root_centre[1], t.root_centre[2], t.root_half);
-  CHUI_LAUNCH_CHECK("root_init_kernel");
-  t.level_start = {0, 1};
-  t.max_depth = 0;
-  for (int d = 0;; ++d) {
-    const std::size_t lb = t.level_start[d], le = t.level_start[d + 1];
-    const uint32_t cnt = static_cast<uint32_t>(le - lb);
-    if (cnt == 0) break;
-    t.level_scratch.reserve((cnt + 4) * sizeof(uint32_t));
-    t.level_bounds.reserve(8ull * cnt * sizeof(uint32_t));
-    t.bfs_scan.reserve(scan_scratch_bytes(cnt) + 64);
-    LevelArgs la{static_cast<uint32_t>(lb), cnt, static_cast<uint32_t>(le), P.leaf_size};
-    level_count_kernel<<<grid_for(8ull * cnt, 256), 256, 0, s>>>(
-        la, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(),
-        keys, t.bfs_nchild.as<uint32_t>(), t.level_bounds.as<uint32_t>(), t.flags.as<uint32_t>());
-    CHUI_LAUNCH_CHECK("level_count_kernel");
-    uint32_t* cbase = t.level_scratch.as<uint32_t>();
-    exclusive_scan_u32(t.bfs_nchild.as<uint32_t>() + lb, cbase, cnt, t.flags.as<uint32_t>() + 1, t.bfs_scan.ptr, s);
-    uint32_t hflags[2] = {0, 0};
-    t.readback.read(hflags, t.flags.ptr, sizeof(hflags), s);
-    if (hflags[0]) {
-      throw UnsupportedError(
-          "octree deeper than 21 levels (more than max_leaf_size coincident or near-coincident particles) "
-          "is not built by the GPU path yet");
+  t.level_dev.reserve(kLvWords * sizeof(uint32_t));
+  t.level_scratch.reserve(kLevelBlocks * sizeof(uint32_t));
+  uint32_t* lv = t.level_dev.as<uint32_t>();
+  uint32_t* bsum = t.level_scratch.as<uint32_t>();
+  for (;;) {
+    const std::size_t cap = std::min<std::size_t>(t.bfs_capacity, 0xFFFFFFFFull);
+    t.level_bounds.reserve(8ull * cap * sizeof(uint32_t));
+    root_init_kernel<<<1, 1, 0, s>>>(t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(),
+                                     t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(), static_cast<uint32_t>(n),
+                                     t.root_centre[0], t.root_centre[1], t.root_centre[2], t.root_half);
+    CHUI_LAUNCH_CHECK("root_init_kernel");
+    level_dev_init_kernel<<<1, 64, 0, s>>>(lv);
+    CHUI_LAUNCH_CHECK("level_dev_init_kernel");
+    for (int d = 0; d < kMaxLevels; ++d) {
+      const LevelDevArgs la{static_cast<uint32_t>(d), P.leaf_size, static_cast<uint32_t>(cap),
+                            static_cast<uint32_t>(partial ? kPartialSortLevels : kMaxLevels)};
+      level_count_dev_kernel<<<kLevelBlocks, 256, 0, s>>>(
+          la, lv, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
+          t.bfs_geo.as<double4>(), keys, t.bfs_nchild.as<uint32_t>(), t.level_bounds.as<uint32_t>(), bsum);
+      CHUI_LAUNCH_CHECK("level_count_dev_kernel");
+      level_offsets_dev_kernel<<<1, kLevelBlocks / 2, 0, s>>>(la, lv, bsum);
+      CHUI_LAUNCH_CHECK("level_offsets_dev_kernel");
+      level_emit_dev_kernel<<<kLevelBlocks, 256, 0, s>>>(
+          la, lv, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
+          t.bfs_geo.as<double4>(), t.level_bounds.as<uint32_t>(), t.bfs_nchild.as<uint32_t>(), bsum,
+          t.bfs_first_child.as<uint32_t>());
+      CHUI_LAUNCH_CHECK("level_emit_dev_kernel");
     }
-    const uint32_t total_children = hflags[1];
-    if (total_children == 0) break;
-    if (partial && d + 1 > kPartialSortLevels) {
+    uint32_t h[kLvWords];
+    t.readback.read(h, lv, sizeof(h), s);
+    if (partial && h[kLvDeepSplit]) {
       // A depth-16 node splits (whether it does depends on its count, not on
       // the unsorted low digits). Sort by every level and start over; the sort
       // is stable, so equal keys keep their row order.
       radix_sort_pairs(keys, order, t.keys_alt.as<uint64_t>(), t.vals_alt.as<uint32_t>(), n, 0, 3 * kPathLevels,
                        t.sort_scratch.ptr, s);
       partial = false;
-      goto restart_discovery;
+      continue;
     }
-    ensure(le + total_children, le);
-    level_emit_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(la, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(),
-                                                         t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(),
-                                                         t.level_bounds.as<uint32_t>(), t.bfs_nchild.as<uint32_t>(),
-                                                         cbase, t.bfs_first_child.as<uint32_t>());
-    CHUI_LAUNCH_CHECK("level_emit_kernel");
-    t.level_start.push_back(le + total_children);
-    t.max_depth = static_cast<uint32_t>(d + 1);
+    if (h[kLvOverflow]) {
+      ensure(2 * t.bfs_capacity, 0);
+      continue;
+    }
+    if (h[kLvTooDeep]) {
+      throw UnsupportedError(
+          "octree deeper than 21 levels (more than max_leaf_size coincident or near-coincident particles) "
+          "is not built by the GPU path yet");
+    }
+    t.level_start = {h[0], h[1]};
+    for (int d = 0; d + 1 < kMaxLevels && h[d + 2] > h[d + 1]; ++d) t.level_start.push_back(h[d + 2]);
+    t.max_depth = static_cast<uint32_t>(t.level_start.size() - 2);
+    t.leaf_count = h[kLvLeaves];
+    break;
   }
   const std::size_t total = t.level_start.back();
   const int nlevels = static_cast<int>(t.level_start.size()) - 1;
@@ -1301,9 +1440,8 @@ restart_discovery:
   }
   ma.extent = std::max(mext, 1.0e-12);
   uint64_t* mk = t.keys_alt.as<uint64_t>();
-  t.pk.reserve(n * sizeof(double4));
   double4* pk = t.pk.as<double4>();
-  gather_sources_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, m, n, ua, order, pk);
+  gather_sources_kernel<<<grid_for(n, 256), 256, 0, s>>>(t.pk_in.as<double4>(), n, order, pk);
   CHUI_LAUNCH_CHECK("gather_sources_kernel");
   leaf_morton_kernel<<<grid_for(n, 256), 256, 0, s>>>(pk, n, ua, ma, mk);
   CHUI_LAUNCH_CHECK("leaf_morton_kernel");
@@ -1324,37 +1462,19 @@ restart_discovery:
         t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>(), t.bfs_box.as<double>());
     CHUI_LAUNCH_CHECK("multipole_kernel");
   }
-  // Preorder records.
+  // Preorder records: links and, for mixed builds, the fixed-point records;
+  // the FP64 records for FP64 builds (or later, on demand).
   t.link.reserve(total * sizeof(NodeLink));
-  t.geo64.reserve(total * sizeof(NodeGeo64));
-  t.mom64.reserve(total * sizeof(NodeMom64));
-  t.quad64.reserve(total * sizeof(NodeQuad64));
-  t.child_index.reserve(total * 8 * sizeof(uint32_t));
   if (P.mixed) {
     t.geo32.reserve(total * sizeof(NodeGeo32));
     t.mom32.reserve(total * sizeof(NodeMom32));
     t.quad32.reserve(total * sizeof(NodeQuad32));
     t.box32.reserve(total * sizeof(NodeBox32));
   }
-  RecordArgs ra{};
-  ra.total = static_cast<uint32_t>(total);
-  ra.mixed = P.mixed ? 1 : 0;
-  ra.cubic = (P.L[0] == P.L[1] && P.L[0] == P.L[2]) ? 1 : 0;
-  for (int a = 0; a < 3; ++a) ra.fscale[a] = 4294967296.0 / P.L[a];
-  records_kernel<<<grid_for(total, 256), 256, 0, s>>>(
-      ra, t.bfs_pre.as<uint32_t>(), t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), nchild, fchild,
-      t.bfs_meta.as<uint32_t>(), t.bfs_subtree.as<uint32_t>(), t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(),
-      t.bfs_quad.as<double>(), t.link.as<NodeLink>(), t.geo64.as<NodeGeo64>(), t.mom64.as<NodeMom64>(),
-      t.quad64.as<NodeQuad64>(), t.child_index.as<uint32_t>(), P.mixed ? t.geo32.as<NodeGeo32>() : nullptr,
-      P.mixed ? t.mom32.as<NodeMom32>() : nullptr, P.mixed ? t.quad32.as<NodeQuad32>() : nullptr,
-      P.mixed ? t.bfs_box.as<double>() : nullptr, P.mixed ? t.box32.as<NodeBox32>() : nullptr);
-  CHUI_LAUNCH_CHECK("records_kernel");
-  count_leaves_kernel<<<std::min<unsigned>(grid_for(total, 256), 1024u), 256, 0, s>>>(nchild, total,
-                                                                                      t.flags.as<uint32_t>() + 2);
-  CHUI_LAUNCH_CHECK("count_leaves_kernel");
-  uint32_t hleaf = 0;
-  t.readback.read(&hleaf, t.flags.as<uint32_t>() + 2, sizeof(uint32_t), s);
-  t.leaf_count = hleaf;
+  t.records64 = false;
+  t.cubic = (P.L[0] == P.L[1] && P.L[0] == P.L[2]);
+  for (int a = 0; a < 3; ++a) t.fscale[a] = 4294967296.0 / P.L[a];
+  launch_records(t, !P.mixed, P.mixed, s);
 }
 
 // ---- mesh-deposit units ---------------------------------------------------------

@@ -47,7 +47,7 @@ struct TreeParams {
   double L[3] = {0, 0, 0};
   uint32_t leaf_size = 16;
   bool quadrupole = true;
-  bool mixed = false;  // also emit the fixed-point records
+  bool mixed = false;  // emit the fixed-point records; the FP64 records follow on demand
 };
 
 // Device state of one built tree. Buffers are grow-only and owned by Tree.
@@ -64,15 +64,19 @@ struct Tree {
   // particle level
   DeviceBuffer order;      // uint32[n]: tree position -> source row
   DeviceBuffer pk;         // double4[n]: tree-order canonical coordinates and mass
+  DeviceBuffer pk_in;      // double4[n]: the same in input order (build scratch)
   DeviceBuffer pos_of_row; // uint32[n]: source row -> tree position (built on demand)
   // preorder node level
   DeviceBuffer link, geo64, mom64, quad64, child_index;  // child_index: uint32[8*nodes]
   DeviceBuffer geo32, mom32, quad32, box32;
+  bool records64 = false;  // geo64, mom64, quad64 and child_index are current
+  bool cubic = false;      // equal axes: the mixed records' separation unit is fixed point
+  double fscale[3] = {0, 0, 0};  // 2^32 / L per axis
   // construction scratch
   DeviceBuffer keys, keys_alt, vals_alt, sort_scratch;
   DeviceBuffer bucket_min, bucket_max, block_best, block_max;
   DeviceBuffer bfs_begin, bfs_count, bfs_first_child, bfs_nchild, bfs_meta, bfs_geo, bfs_subtree, bfs_pre,
-      bfs_mom, bfs_quad, bfs_box, bfs_scan, level_scratch, level_bounds, flags, bounds;
+      bfs_mom, bfs_quad, bfs_box, bfs_scan, level_scratch, level_bounds, level_dev, flags, bounds;
   DeviceBuffer dep_flag, dep_index, dep_units, dep_blocks, dep_scan, dep_total;
   std::size_t bfs_capacity = 0;
   std::vector<std::size_t> level_start;  // BFS level ranges (host copy)
@@ -84,6 +88,11 @@ struct Tree {
 // order and multipoles. Inputs are device pointers (caller positions/masses).
 void build_tree(Tree& t, const double* x, const double* y, const double* z, const double* m,
                 const TreeParams& p, double softening, hipStream_t s);
+
+// FP64 preorder records (geo64, mom64, quad64, child_index) of the last built
+// tree. A mixed-precision build leaves them out (its walks read the
+// fixed-point records); they are written from the BFS arrays on first need.
+void ensure_records64(Tree& t, hipStream_t s);
 
 // Mesh-deposit units of the last built tree: the shallowest subtrees whose
 // particles span at most d.max_cells cells (or leaves that never do). Reads

@@ -1375,7 +1375,7 @@ __global__ void leaf_of_pos_kernel(const NodeLink* __restrict__ link, std::size_
 // The pairs and multipoles are therefore those of the lane-by-lane walk;
 // only the summation order differs.
 constexpr int kWindow = 64;
-constexpr int kBoxRing = 2 * kWindow;
+constexpr int kPairGroup = 2;  // box walk: table reads in flight together
 
 //
 // Residual table indexed by the float bits (LOG, the timed walk): the whole
@@ -1595,6 +1595,42 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
     psy += ly;
     psz += lz;
   };
+  // A full window of the box walk (LOG): sources in groups of kPairGroup whose
+  // table reads are in flight together before their polynomials are evaluated.
+  auto window_pairs = [&]<bool UMASS>() {
+    constexpr int G = kPairGroup;
+    float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+#pragma unroll 2
+    for (int s0 = 0; s0 < kWindow; s0 += G) {
+      float dx[G], dy[G], dz[G], mj[G];
+      uint32_t bits[G];
+      float4 c[G];
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        const float4 sj = win[w][s0 + k];
+        dx[k] = sj.x - tx;
+        dy[k] = sj.y - ty;
+        dz[k] = sj.z - tz;
+        mj[k] = sj.w;
+        bits[k] = __float_as_uint(
+            __builtin_amdgcn_fmed3f(fmaf(dx[k], dx[k], fmaf(dy[k], dy[k], fmaf(dz[k], dz[k], eps2))), 0.0f, 1.0f));
+      }
+#pragma unroll
+      for (int k = 0; k < G; ++k) c[k] = lds_cell(lbase, bits[k] >> kLogMantissaShift);
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        const float u = static_cast<float>(bits[k] & ((1u << kLogMantissaShift) - 1u));
+        float fr = fmaf(fmaf(fmaf(c[k].w, u, c[k].z), u, c[k].y), u, c[k].x);
+        if constexpr (!UMASS) fr *= mj[k];
+        lx = fmaf(fr, dx[k], lx);
+        ly = fmaf(fr, dy[k], ly);
+        lz = fmaf(fr, dz[k], lz);
+      }
+    }
+    psx += lx;
+    psy += ly;
+    psz += lz;
+  };
 
   // Strict acceptance of internal node n (cell centre and half from g) for
   // this lane, with the COM separation it needs left in mf*. Nodes with
@@ -1722,8 +1758,14 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
       pend += cn;
       // every lane sums every staged source (pairs beyond r_cut are zero);
       // lanes without a target are never written out
-      if (D.uniform_mass) pairs.template operator()<true, true, false>(0, cn, lim);
-      else pairs.template operator()<true, false, false>(0, cn, lim);
+      if (LOG && cn == static_cast<uint32_t>(kWindow)) {
+        if (D.uniform_mass) window_pairs.template operator()<true>();
+        else window_pairs.template operator()<false>();
+      } else if (D.uniform_mass) {
+        pairs.template operator()<true, true, false>(0, cn, lim);
+      } else {
+        pairs.template operator()<true, false, false>(0, cn, lim);
+      }
       __builtin_amdgcn_wave_barrier();
     };
     // stage the next cn listed sources, keep those within r_cut of the box
@@ -2196,9 +2238,10 @@ __device__ __forceinline__ double rsq64(double s) {
 template <int SCHEME, bool JVP = false>
 __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4))) void walk64_box_kernel(
     WalkParams P, WalkData D, WalkOutputs out, unsigned long long* __restrict__ counters) {
-  __shared__ double4 win[kWalkWaves][kWindow];
+  // forward: the box walk's window holds up to 127 compacted records
+  __shared__ double4 win[kWalkWaves][JVP ? kWindow : 2 * kWindow];
   __shared__ double4 winl[JVP ? kWalkWaves : 1][JVP ? kWindow : 1];  // JVP: lam of the staged sources
-  __shared__ uint32_t ring[kWalkWaves][kBoxRing];
+  __shared__ uint32_t list[kWalkWaves][kWindow];                       // box walk: listed source indices
   __shared__ double2 htab[3 * kH64Cells];
   for (int i = threadIdx.x; i < 3 * P.h64_cells; i += kWalkThreads) htab[i] = D.h64tab[i];
   __syncthreads();
@@ -2444,16 +2487,46 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
       return 3;
     };
     const double lim = valid ? rc2 : -1.0;
-    uint32_t head = 0, nl = 0;
-    auto eval_window = [&](uint32_t cn) {
-      __builtin_amdgcn_wave_barrier();
-      stage.template operator()<true>(ring[w][(head + (static_cast<uint32_t>(lane) < cn ? lane : 0)) % kBoxRing],
-                                      static_cast<uint32_t>(lane) < cn);
+    // forward: staged sources beyond r_cut of the target box (every pair with
+    // them lies beyond r_cut, so their terms are zero) are dropped before the
+    // window, as in walkmix_kernel's box walk
+    const double cull2 = rc2 * (1.0 + 1.0e-9);
+    uint32_t nl = 0;  // list[w][0, nl): listed sources, staged 64 at a time
+    uint32_t wn = 0;  // forward: compacted records in win[w][0, wn)
+    auto evaluate = [&](uint32_t cn) {
       __builtin_amdgcn_wave_barrier();
       pairs.template operator()<true>(cn, lim);
       __builtin_amdgcn_wave_barrier();
-      head = (head + cn) % kBoxRing;
-      nl -= cn;
+    };
+    auto eval_window = [&](uint32_t cn) {
+      __builtin_amdgcn_wave_barrier();
+      const bool on = static_cast<uint32_t>(lane) < cn;
+      const uint32_t j = list[w][on ? lane : 0];
+      nl = 0;
+      if constexpr (JVP) {
+        stage.template operator()<true>(j, on);
+        evaluate(cn);
+      } else {
+        const double4 sv = D.src64[j];
+        const double4 rec = make_double4(image(sv.x, bx, 0), image(sv.y, by, 1), image(sv.z, bz, 2), sv.w);
+        const double fx_ = fmax(0.0, fabs(rec.x - bx) - ex), fy_ = fmax(0.0, fabs(rec.y - by) - ey),
+                     fz_ = fmax(0.0, fabs(rec.z - bz) - ez);
+        const bool keep = on && fx_ * fx_ + fy_ * fy_ + fz_ * fz_ <= cull2;
+        const unsigned long long km = __ballot(keep);
+        const uint32_t at =
+            wn + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(km >> 32),
+                                           __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(km), 0u));
+        if (keep) win[w][at] = rec;
+        wn += static_cast<uint32_t>(__popcll(km));
+        if (wn >= static_cast<uint32_t>(kWindow)) {
+          evaluate(kWindow);
+          wn -= kWindow;
+          if (static_cast<uint32_t>(lane) < wn) {
+            const double4 t = win[w][kWindow + lane];
+            win[w][lane] = t;
+          }
+        }
+      }
     };
     uint32_t p = 0;
     while (p < P.node_count) {
@@ -2475,11 +2548,13 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
         const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(rest)) - 1);
         rest &= rest - 1ull;
         const uint32_t b0 = readlane_u32(lb, l), c = readlane_u32(lc, l);
-        for (uint32_t c0 = 0; c0 < c; c0 += kWindow) {
-          const uint32_t cn = c - c0 < static_cast<uint32_t>(kWindow) ? c - c0 : static_cast<uint32_t>(kWindow);
-          if (static_cast<uint32_t>(lane) < cn) ring[w][(head + nl + lane) % kBoxRing] = b0 + c0 + lane;
+        for (uint32_t c0 = 0; c0 < c;) {
+          const uint32_t room = static_cast<uint32_t>(kWindow) - nl;
+          const uint32_t cn = c - c0 < room ? c - c0 : room;
+          if (static_cast<uint32_t>(lane) < cn) list[w][nl + lane] = b0 + c0 + lane;
           nl += cn;
-          if (nl >= static_cast<uint32_t>(kWindow)) eval_window(kWindow);
+          c0 += cn;
+          if (nl == static_cast<uint32_t>(kWindow)) eval_window(kWindow);
         }
       }
       unsigned long long sub = __ballot(on && kind == 4);
@@ -2487,11 +2562,17 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
         const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<long long>(sub)) - 1);
         sub &= sub - 1ull;
         const uint32_t end = readlane_u32(next, l);
+        // the lane-by-lane walk stages its leaves in the window's front
+        if (wn) {
+          evaluate(wn);
+          wn = 0;
+        }
         nxt = valid ? base + l : end;
         walk.template operator()<true>(end);
       }
     }
     if (nl) eval_window(nl);
+    if (wn) evaluate(wn);
   } else {
     walk.template operator()<false>(P.node_count);
   }
@@ -2664,6 +2745,7 @@ void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int unifo
                const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
                const double* fz, const WalkOutputs& out, unsigned long long* counters, const double2* h64tab,
                hipStream_t s) {
+  if (!t.records64 && t.node_count > 0) throw std::logic_error("FP64 walk on a tree without FP64 records");
   if (p.n_targets == 0) return;
   WalkData d{};
   d.link = t.link.as<NodeLink>();
@@ -2702,6 +2784,7 @@ void walk_fp64_adaptive(const Tree& t, const WalkParams& p, const double4* src, 
                         const double2* gtab, const double* prev, const uint32_t* target_pos,
                         const uint32_t* target_slot, const double* fx, const double* fy, const double* fz,
                         const WalkOutputs& out, unsigned long long* counters, hipStream_t s) {
+  if (!t.records64 && t.node_count > 0) throw std::logic_error("FP64 walk on a tree without FP64 records");
   if (p.n_targets == 0) return;
   WalkData d{};
   d.link = t.link.as<NodeLink>();
@@ -2730,6 +2813,7 @@ void walk_fp64_group(const Tree& t, const WalkParams& p, const GroupParams& g, c
                      const uint32_t* target_pos, const uint32_t* target_slot, const double* fx, const double* fy,
                      const double* fz, const WalkOutputs& out, uint32_t* fallback_pos, uint32_t* fallback_slot,
                      uint32_t* fallback_count, unsigned long long* counters, hipStream_t s) {
+  if (!t.records64 && t.node_count > 0) throw std::logic_error("FP64 walk on a tree without FP64 records");
   if (p.n_targets == 0) return;
   if (g.block == 0 || g.block > 256) throw std::invalid_argument("group walk residual block must be in [1, 256]");
   WalkData d{};
@@ -2805,6 +2889,7 @@ std::vector<double> gaussian_hermite_monomials(const std::vector<double>& knots)
 void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,
                    const double* fx, const double* fy, const double* fz, const double* nu, const WalkOutputs& out,
                    unsigned long long* counters, const double2* h64tab, hipStream_t s) {
+  if (!t.records64 && t.node_count > 0) throw std::logic_error("FP64 walk on a tree without FP64 records");
   if (p.n_targets == 0) return;
   WalkData d{};
   d.link = t.link.as<NodeLink>();
