@@ -79,6 +79,10 @@ def main():
                     help="1: also time the FP64 gradients of every ensemble box")
     ap.add_argument("--hierarchical-rungs", type=int, default=3,
                     help="one hierarchical KDK block with this many rungs on the 128^3 workload (mixed); 0 skips")
+    ap.add_argument("--profile", default="",
+                    help="run the timed solves with one of the accuracy profiles (e.g. tsc_deconv_rs1.25_rcut5rs, "
+                         "TSC + deconvolution, r_cut = 6.25 cells) instead of the headline's CIC settings; for "
+                         "kernel traces of the setting that meets the metric's 1e-3 force error")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="one plain device-to-device copy of this size before the timed region; its HBM counters "
                          "scale the walk's FETCH_SIZE/WRITE_SIZE readings under rocprofv3 --pmc")
@@ -102,6 +106,19 @@ def main():
     opts = L.default_options(box_size=w.box, gravitational_constant=ics.G_CODE, opening_theta=0.5,
                              rcut_cells=4.5 * 1.25, asmth_cells=1.25, softening_epsilon=w.softening,
                              max_leaf_size=16)
+    asmth_used, rcut_rs_used = 1.25, 4.5
+    if args.profile:
+        prof = {p["name"]: p for p in ACCURACY_PROFILES}.get(args.profile)
+        if prof is None:
+            print(f"unknown --profile {args.profile}; one of {[p['name'] for p in ACCURACY_PROFILES]}",
+                  file=sys.stderr)
+            sys.exit(2)
+        opts = L.default_options(box_size=w.box, gravitational_constant=ics.G_CODE, opening_theta=0.5,
+                                 asmth_cells=prof["asmth"], rcut_cells=prof["rcut_over_rs"] * prof["asmth"],
+                                 softening_epsilon=w.softening, max_leaf_size=16,
+                                 assignment_scheme=prof["assignment"], window_deconvolution=prof["deconv"])
+        asmth_used, rcut_rs_used = prof["asmth"], prof["rcut_over_rs"]
+        log(f"timed solves use accuracy profile {args.profile}")
     coord = TreePmCoordinator(w.mesh)
 
     def run(precision, steps, warmup):
@@ -243,8 +260,8 @@ def main():
         "data": "synthetic (seeded Zel'dovich lattice + NFW halos)",
         "config": {"workload": w.name, "particles": n, "mesh": w.mesh, "box_mpc_h": w.box,
                    "halo_fraction": w.halo_fraction, "halos": w.halo_count, "theta": 0.5, "leaf": 16,
-                   "asmth_cells": 1.25, "rcut_over_rs": 4.5, "softening": w.softening,
-                   "parallelism": "single GPU"},
+                   "asmth_cells": asmth_used, "rcut_over_rs": rcut_rs_used, "softening": w.softening,
+                   "parallelism": "single GPU", **({"profile": args.profile} if args.profile else {})},
         "fp64_gpu": fp64,
         "force_error": {"mixed_vs_fp64": err_mix_vs_fp64, "ewald": ewald},
         "accuracy_profiles": accuracy,

@@ -109,6 +109,16 @@ template <class V, bool INV> __device__ __forceinline__ V twiddle(const V* tw, i
   return w;
 }
 
+// LDS slot of point i of a power-of-two line: one padding slot after every 8
+// points. The radix-8 passes store each butterfly's outputs 8 points (first
+// pass) or 64 points (second pass) apart; unpadded, the 64 lanes of such a
+// store fall into 4 or 8 of the 64 LDS banks (16- and 8-way conflicts), padded
+// into 32 (2-way). Lines of other lengths are not padded.
+__host__ __device__ __forceinline__ int lds_slot(int i, bool pad) { return pad ? i + (i >> 3) : i; }
+// Slots between consecutive LDS lines: odd, so that the lines of a block
+// start in different banks.
+__host__ __device__ __forceinline__ int lds_line_stride(int n, bool pad) { return pad ? n + (n >> 3) + 1 : n + 1; }
+
 template <class T, int R, bool INV>
 __device__ __attribute__((noinline)) void stockham_pass(typename C2<T>::type* X, int nlines, int lgn, int stride,
                                               const typename C2<T>::type* tw, int lgNs) {
@@ -122,23 +132,25 @@ __device__ __attribute__((noinline)) void stockham_pass(typename C2<T>::type* X,
   const int halfn = 1 << (lgn - 1);
   const int sh = lgn - lgNs - LR;
   V v[MAXB][R];
-  int dst[MAXB];
+  int base[MAXB], dst[MAXB];
 #pragma unroll
   for (int u = 0; u < MAXB; ++u) {
     const int b = threadIdx.x + u * kFftThreads;
     dst[u] = -1;
+    base[u] = 0;
     if (b < total) {
       const int line = b >> lgq, j = b & (q - 1);
       const V* L = X + line * stride;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[u][r] = L[j + r * q];
+      for (int r = 0; r < R; ++r) v[u][r] = L[lds_slot(j + r * q, true)];
       const int jm = j & (Ns - 1);
       if (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) v[u][r] = cmul(v[u][r], twiddle<V, INV>(tw, (jm * r) << sh, halfn));
       }
       dft<V, R, INV>(v[u]);
-      dst[u] = line * stride + ((j >> lgNs) << (lgNs + LR)) + jm;
+      base[u] = line * stride;
+      dst[u] = ((j >> lgNs) << (lgNs + LR)) + jm;
     }
   }
   __syncthreads();
@@ -146,7 +158,7 @@ __device__ __attribute__((noinline)) void stockham_pass(typename C2<T>::type* X,
   for (int u = 0; u < MAXB; ++u) {
     if (dst[u] >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) X[dst[u] + r * Ns] = v[u][r];
+      for (int r = 0; r < R; ++r) X[base[u] + lds_slot(dst[u] + r * Ns, true)] = v[u][r];
     }
   }
   __syncthreads();
@@ -465,7 +477,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
   V* tw = reinterpret_cast<V*>(smem_raw);
   const int twn = fft_tw_len(nz, lg);
   V* X = tw + twn;
-  const int stride = nz + 1;
+  const bool pad = lg >= 0;
+  const int stride = lds_line_stride(nz, pad);
   for (int i = threadIdx.x; i < twn; i += blockDim.x) tw[i] = twz[i];
   const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
   const int npts = pairs_per_block * nz;
@@ -490,7 +503,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
       if (idx < npts) {
         int pp, i;
         split_index(idx, nz, lg, pp, i);
-        X[pp * stride + i] = v[u];
+        X[pp * stride + lds_slot(i, pad)] = v[u];
       }
     }
   }
@@ -501,8 +514,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
     const int pp = idx / nzc, k = idx - pp * nzc;
     const std::size_t pr = pair0 + pp;
     if (2 * pr >= rows) continue;
-    const V zk = X[pp * stride + (k == 0 ? 0 : k)];
-    const V zm = X[pp * stride + (k == 0 ? 0 : nz - k)];
+    const V zk = X[pp * stride + lds_slot(k == 0 ? 0 : k, pad)];
+    const V zm = X[pp * stride + lds_slot(k == 0 ? 0 : nz - k, pad)];
     const V zmc = mk<V, T>(zm.x, -zm.y);
     const V s = cadd(zk, zmc), d = csub(zk, zmc);
     V* ca = reinterpret_cast<V*>(data + (2 * pr) * row);
@@ -524,7 +537,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::si
   V* tw = reinterpret_cast<V*>(smem_raw);
   const int twn = fft_tw_len(nz, lg);
   V* X = tw + twn;
-  const int stride = nz + 1;
+  const bool pad = lg >= 0;
+  const int stride = lds_line_stride(nz, pad);
   const int nzc = (nz >> 1) + 1;
   for (int i = threadIdx.x; i < twn; i += blockDim.x) tw[i] = twz[i];
   const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
@@ -555,10 +569,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::si
         xb = mk<V, T>(xb.y * kk, -xb.x * kk);
       }
       if (k == 0 || k == nz / 2) {
-        X[pp * stride + k] = mk<V, T>(xa.x, xb.x);
+        X[pp * stride + lds_slot(k, pad)] = mk<V, T>(xa.x, xb.x);
       } else {
-        X[pp * stride + k] = mk<V, T>(xa.x - xb.y, xa.y + xb.x);          // xa + i xb
-        X[pp * stride + (nz - k)] = mk<V, T>(xa.x + xb.y, -xa.y + xb.x);  // conj(xa) + i conj(xb)
+        X[pp * stride + lds_slot(k, pad)] = mk<V, T>(xa.x - xb.y, xa.y + xb.x);          // xa + i xb
+        X[pp * stride + lds_slot(nz - k, pad)] = mk<V, T>(xa.x + xb.y, -xa.y + xb.x);  // conj(xa) + i conj(xb)
       }
     }
   }
@@ -568,7 +582,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::si
     const int pp = idx / nz, i = idx - pp * nz;
     const std::size_t pr = pair0 + pp;
     if (2 * pr >= rows) continue;
-    const V v = X[pp * stride + i];
+    const V v = X[pp * stride + lds_slot(i, pad)];
     data[(2 * pr) * row + i] = v.x;
     if (2 * pr + 1 < rows) data[(2 * pr + 1) * row + i] = v.y;
   }
@@ -603,7 +617,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_lines_kernel(const typename C
   V* tw = reinterpret_cast<V*>(smem_raw);
   const int twn = fft_tw_len(a.n, a.lg);
   V* X = tw + twn;
-  const int stride = a.n + 1;
+  const bool pad = a.lg >= 0;
+  const int stride = lds_line_stride(a.n, pad);
   for (int i = threadIdx.x; i < twn; i += blockDim.x) tw[i] = twl[i];
   const std::size_t j0 = static_cast<std::size_t>(blockIdx.x) * a.tj;
   const std::size_t obase = static_cast<std::size_t>(blockIdx.y) * a.outer_stride;
@@ -631,7 +646,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_lines_kernel(const typename C
         const T kk = dk[i];
         vv = mk<V, T>(vv.y * kk, -vv.x * kk);
       }
-      X[jj * stride + i] = vv;
+      X[jj * stride + lds_slot(i, pad)] = vv;
     }
   }
   __syncthreads();
@@ -640,7 +655,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_lines_kernel(const typename C
     const int jj = idx & tjm, i = idx >> tjl;
     const std::size_t j = j0 + jj;
     if (j >= a.inner_count) continue;
-    V v = X[jj * stride + i];
+    V v = X[jj * stride + lds_slot(i, pad)];
     if (a.green) {
       const std::size_t iy = j / a.nzc, iz = j - iy * a.nzc;
       const T k2 = kx[i] * kx[i] + ky[iy] * ky[iy] + kz[iz] * kz[iz];
@@ -763,8 +778,9 @@ int lines_per_block(int n) {
   const std::size_t lim = fft_lds_limit();
   const std::size_t twb = static_cast<std::size_t>(fft_tw_len(n, lg)) * sizeof(V);
   int tj = 16;
-  while (tj > 1 && (static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + twb > lim || tj * n > cap)) tj >>= 1;
-  if (tj * n > cap || static_cast<std::size_t>(tj) * (n + 1) * sizeof(V) + twb > lim)
+  const std::size_t ls = static_cast<std::size_t>(lds_line_stride(n, lg >= 0));
+  while (tj > 1 && (static_cast<std::size_t>(tj) * ls * sizeof(V) + twb > lim || tj * n > cap)) tj >>= 1;
+  if (tj * n > cap || static_cast<std::size_t>(tj) * ls * sizeof(V) + twb > lim)
     throw UnsupportedError("PM mesh axis too long for one LDS line");
   return tj;
 }
@@ -775,7 +791,7 @@ void launch_lines(const typename C2<T>::type* in, typename C2<T>::type* out, Lin
   using V = typename C2<T>::type;
   a.tj = lines_per_block<T>(a.n);
   const std::size_t smem =
-      static_cast<std::size_t>(a.tj) * (a.n + 1) * sizeof(V) + fft_tw_len(a.n, a.lg) * sizeof(V);
+      static_cast<std::size_t>(a.tj) * lds_line_stride(a.n, a.lg >= 0) * sizeof(V) + fft_tw_len(a.n, a.lg) * sizeof(V);
   if (smem > 64 * 1024) {
     // contexts of an ensemble launch from several host threads
     static std::mutex mu;
@@ -799,8 +815,9 @@ int pairs_per_block(int nz) {
   const int cap = fft_max_butterflies<T>() * kFftThreads * fft_plan_min_radix(nz);  // points per block
   const std::size_t twb = static_cast<std::size_t>(fft_tw_len(nz, lg)) * sizeof(V);
   int pp = 8;
-  while (pp > 1 && (static_cast<std::size_t>(pp) * (nz + 1) * sizeof(V) + twb > 48 * 1024 || pp * nz > cap)) pp >>= 1;
-  if (pp * nz > cap || static_cast<std::size_t>(pp) * (nz + 1) * sizeof(V) + twb > 64 * 1024)
+  const std::size_t ls = static_cast<std::size_t>(lds_line_stride(nz, lg >= 0));
+  while (pp > 1 && (static_cast<std::size_t>(pp) * ls * sizeof(V) + twb > 48 * 1024 || pp * nz > cap)) pp >>= 1;
+  if (pp * nz > cap || static_cast<std::size_t>(pp) * ls * sizeof(V) + twb > 64 * 1024)
     throw UnsupportedError("PM mesh z axis too long for one LDS line");
   return pp;
 }
@@ -812,7 +829,8 @@ void z_r2c(T* data, const PmGeometry& g, const Tables<T>& tb, hipStream_t s) {
   const int pp = pairs_per_block<T>(g.n[2]);
   const std::size_t pairs = (rows + 1) / 2;
   const std::size_t smem =
-      static_cast<std::size_t>(pp) * (g.n[2] + 1) * sizeof(V) + fft_tw_len(g.n[2], g.lg[2]) * sizeof(V);
+      static_cast<std::size_t>(pp) * lds_line_stride(g.n[2], g.lg[2] >= 0) * sizeof(V) +
+      fft_tw_len(g.n[2], g.lg[2]) * sizeof(V);
   fft_z_r2c_kernel<T><<<static_cast<unsigned>((pairs + pp - 1) / pp), kFftThreads, smem, s>>>(
       data, rows, g.n[2], g.lg[2], g.row, tb.twz, pp);
   CHUI_LAUNCH_CHECK("fft_z_r2c_kernel");
@@ -825,7 +843,8 @@ void z_c2r(T* data, const PmGeometry& g, const Tables<T>& tb, bool premul, hipSt
   const int pp = pairs_per_block<T>(g.n[2]);
   const std::size_t pairs = (rows + 1) / 2;
   const std::size_t smem =
-      static_cast<std::size_t>(pp) * (g.n[2] + 1) * sizeof(V) + fft_tw_len(g.n[2], g.lg[2]) * sizeof(V);
+      static_cast<std::size_t>(pp) * lds_line_stride(g.n[2], g.lg[2] >= 0) * sizeof(V) +
+      fft_tw_len(g.n[2], g.lg[2]) * sizeof(V);
   fft_z_c2r_kernel<T><<<static_cast<unsigned>((pairs + pp - 1) / pp), kFftThreads, smem, s>>>(
       data, rows, g.n[2], g.lg[2], g.row, tb.twz, tb.dkz, premul ? 1 : 0, pp);
   CHUI_LAUNCH_CHECK("fft_z_c2r_kernel");

@@ -2504,8 +2504,31 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
       const uint32_t j = list[w][on ? lane : 0];
       nl = 0;
       if constexpr (JVP) {
-        stage.template operator()<true>(j, on);
-        evaluate(cn);
+        // the same drop without a second window: kept sources beyond the
+        // window's free room go back to the front of the list (as indices)
+        // and are staged again with the next batch
+        const double4 sv = D.src64[j];
+        const double4 rec = make_double4(image(sv.x, bx, 0), image(sv.y, by, 1), image(sv.z, bz, 2), sv.w);
+        const double fx_ = fmax(0.0, fabs(rec.x - bx) - ex), fy_ = fmax(0.0, fabs(rec.y - by) - ey),
+                     fz_ = fmax(0.0, fabs(rec.z - bz) - ez);
+        const bool keep = on && fx_ * fx_ + fy_ * fy_ + fz_ * fz_ <= cull2;
+        const unsigned long long km = __ballot(keep);
+        const uint32_t pos = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(km >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(km), 0u));
+        const uint32_t room = static_cast<uint32_t>(kWindow) - wn;
+        const uint32_t kept = static_cast<uint32_t>(__popcll(km));
+        if (keep && pos < room) {
+          win[w][wn + pos] = rec;
+          winl[w][wn + pos] = D.lam64[j];
+        }
+        if (keep && pos >= room) list[w][pos - room] = j;
+        const uint32_t taken = kept < room ? kept : room;
+        wn += taken;
+        nl = kept - taken;
+        if (wn == static_cast<uint32_t>(kWindow)) {
+          evaluate(kWindow);
+          wn = 0;
+        }
       } else {
         const double4 sv = D.src64[j];
         const double4 rec = make_double4(image(sv.x, bx, 0), image(sv.y, by, 1), image(sv.z, bz, 2), sv.w);
@@ -2571,7 +2594,7 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
         walk.template operator()<true>(end);
       }
     }
-    if (nl) eval_window(nl);
+    while (nl) eval_window(nl);  // JVP: each call takes at least one source or evaluates a full window
     if (wn) evaluate(wn);
   } else {
     walk.template operator()<false>(P.node_count);
