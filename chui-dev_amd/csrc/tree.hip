@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void source_stats_kernel(Axes3 p, const double
 // occupied bucket i. If e = i - p - 1 empty buckets lie between them, the gap
 // is within (e w, (e + 2) w) up to rounding of the bucket computation, so only
 // runs with e >= R - 2 (R the longest empty run) can hold the largest gap.
-// Steps per axis: an occupancy bitmap (one 32-bit OR per value); R from a scan
+// Steps per axis: an occupancy bitmap (one byte store per value, packed); R from a scan
 // of the bitmap; the end buckets of candidate runs and their predecessors are
 // marked; exact FP64 min/max (IEEE bits; non-negative doubles order like their
 // bits) are gathered only for marked buckets, ranked by a popcount scan; the
@@ -180,32 +180,33 @@ __device__ __forceinline__ uint64_t bucket_of(double v, const BucketArgs& a, int
   return b >= a.B ? a.B - 1 : b;
 }
 
-// Neighbouring input values often share a bitmap word (lattice-ordered inputs
-// do on two of three axes): the lanes whose word equals lane 0's combine their
-// bits with one DPP reduction and lane 0 sets them with a single atomic.
-__global__ __launch_bounds__(256) void occupancy_kernel(Axes3 p, std::size_t n, BucketArgs a,
-                                                        uint32_t* __restrict__ occ) {
-  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
-  for (std::size_t base = static_cast<std::size_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~(kWave - 1));
-       base < n; base += stride) {  // wave-uniform loop: all lanes take part in the reductions
-    const std::size_t i = base + lane_id();
-    const bool valid = i < n;  // valid lanes form a prefix, so lane 0 is valid
-    for (int ax = 0; ax < a.naxes; ++ax) {
-      uint64_t wi = 0;
-      uint32_t bit = 0;
-      if (valid) {
-        const uint64_t b = bucket_of(wrap_coord(p.p[ax][i], a.L[ax], a.canonical[ax]), a, ax);
-        wi = ax * a.W + (b >> 5);
-        bit = 1u << (b & 31);
-      }
-      const uint64_t w0 = (static_cast<uint64_t>(readlane_u32(static_cast<uint32_t>(wi >> 32), 0)) << 32) |
-                          readlane_u32(static_cast<uint32_t>(wi), 0);
-      const bool same = valid && wi == w0;
-      const uint32_t grp = wave_or_u32(same ? bit : 0u);
-      if (lane_id() == 0) atomicOr(&occ[w0], grp);
-      if (valid && !same) atomicOr(&occ[wi], bit);
-    }
+// Occupancy as one byte per bucket: every value stores 1 into its bucket's
+// byte. The stores are idempotent, so they need no atomics (a device-scope
+// atomic per value and axis ran at about 1e10 per second, scattered); a
+// coalesced pass then packs 32 bytes into each bitmap word.
+__global__ __launch_bounds__(256) void occupancy_bytes_kernel(Axes3 p, std::size_t n, BucketArgs a,
+                                                              uint8_t* __restrict__ occb) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    for (int ax = 0; ax < a.naxes; ++ax)
+      occb[ax * a.B + bucket_of(wrap_coord(p.p[ax][i], a.L[ax], a.canonical[ax]), a, ax)] = 1u;
   }
+}
+
+__global__ __launch_bounds__(256) void occupancy_pack_kernel(const uint8_t* __restrict__ occb, std::size_t words,
+                                                             uint32_t* __restrict__ occ) {
+  const std::size_t wi = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (wi >= words) return;
+  const uint4* src = reinterpret_cast<const uint4*>(occb + 32 * wi);
+  const uint4 b0 = src[0], b1 = src[1];
+  const uint32_t q[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+  uint32_t bits = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bits |= ((q[k] >> (8 * c)) & 1u) << (4 * k + c);
+  }
+  occ[wi] = bits;
 }
 
 // Exact min/max (IEEE bits, max stored +1 so 0 = empty) of the marked buckets.
@@ -1186,8 +1187,14 @@ void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, co
   fill_u32(occ, 0u, 3 * NW, s);
   fill_u64(total, 0ull, 8, s);
   const unsigned pgrid = std::min<unsigned>(grid_for(n, 256), 8192u);
-  occupancy_kernel<<<pgrid, 256, 0, s>>>(p, n, a, occ);
-  CHUI_LAUNCH_CHECK("occupancy_kernel");
+  // bytes per bucket (B per axis, a multiple of 32), then packed into occ
+  t.occ_bytes.reserve(naxes * B);
+  uint8_t* occb = t.occ_bytes.as<uint8_t>();
+  fill_u32(reinterpret_cast<uint32_t*>(occb), 0u, naxes * B / 4, s);
+  occupancy_bytes_kernel<<<pgrid, 256, 0, s>>>(p, n, a, occb);
+  CHUI_LAUNCH_CHECK("occupancy_bytes_kernel");
+  occupancy_pack_kernel<<<grid_for(NW, 256), 256, 0, s>>>(occb, NW, occ);
+  CHUI_LAUNCH_CHECK("occupancy_pack_kernel");
   occ_blockmax_kernel<<<dim3(static_cast<unsigned>(nblk), naxes), kGapThreads, 0, s>>>(occ, W, blockmax, nblk);
   CHUI_LAUNCH_CHECK("occ_blockmax_kernel");
   blockmax_scan_kernel<<<naxes, kGapThreads, 0, s>>>(blockmax, nblk, total);

@@ -74,7 +74,7 @@ struct Tree {
   double fscale[3] = {0, 0, 0};  // 2^32 / L per axis
   // construction scratch
   DeviceBuffer keys, keys_alt, vals_alt, sort_scratch;
-  DeviceBuffer bucket_min, bucket_max, block_best, block_max;
+  DeviceBuffer bucket_min, bucket_max, block_best, block_max, occ_bytes;
   DeviceBuffer bfs_begin, bfs_count, bfs_first_child, bfs_nchild, bfs_meta, bfs_geo, bfs_subtree, bfs_pre,
       bfs_mom, bfs_quad, bfs_box, bfs_scan, level_scratch, level_bounds, level_dev, flags, bounds;
   DeviceBuffer dep_flag, dep_index, dep_units, dep_blocks, dep_scan, dep_total;
