@@ -361,7 +361,7 @@ def measure_accuracy_profiles(L, coord, dev, w, tgt, ref, steps, headline_value)
 
 def measure_other_configs(L, ics):
     """The 128^3 workload timed in both precisions, and the Ewald force error
-    of the 32^3 (FP64), 128^3, 64^3 adjoint (FP64, monopoles, its initial
+    of the 32^3 (FP64), 128^3, 64^3 adjoint (FP64, quadrupoles, its initial
     state) and ensemble (mixed, one box per size) workloads."""
     import torch
 
@@ -402,7 +402,7 @@ def measure_other_configs(L, ics):
             softening = prob.softening
 
         tgt, ref = ewald_reference(prob.coord, dev4, _W, 4096, ics.BASE_SEED + 93, ics.G_CODE)
-        out["adjoint_64^3_box50_mesh128"] = {"precision": "fp64, monopoles (the differentiated forward)",
+        out["adjoint_64^3_box50_mesh128"] = {"precision": "fp64, quadrupoles (the differentiated forward)",
                                             "fp64_vs_ewald": error_metrics(a4[:, tgt], ref),
                                             "ewald_samples": int(tgt.size)}
     finally:
@@ -687,6 +687,13 @@ def measure_gradient(n_side, directions):
         loss, grad, t_fwd_g, t_bwd = run(True)
         evals = {"forward_force_evaluations": prob.force_evaluations, "force_vjps": prob.vjp_evaluations}
         _, _, t_fwd, _ = run(False)
+        prob.record_pair_sets = True
+        prob.pair_sets = []
+        with torch.no_grad():
+            adjoint.density_loss(psi64, target, prob)
+        accepted = int(sum(rec[2] for rec in prob.pair_sets))
+        prob.record_pair_sets = False
+        prob.pair_sets = []
         spacing = prob.box / n_side
         fd = {}
         for label, h in (("step_1e-3_spacing", 1e-3 * spacing), ("step_1e-6_spacing", 1e-6 * spacing)):
@@ -698,13 +705,16 @@ def measure_gradient(n_side, directions):
                          "directions_keeping_pair_sets": int(kept.sum()),
                          "kept_median_relative_error": float(np.median(errs[kept])) if kept.any() else None,
                          "kept_max_relative_error": float(np.max(errs[kept])) if kept.any() else None,
-                         "rule": "a direction whose +h and -h trajectories sum different pairs inside r_cut in any "
-                                 "force evaluation (pair counts and pair-set checksums of the FP64 walk) moves a "
-                                 "pair across the cutoff, where the loss jumps; it is excluded"}
+                         "rule": "a direction whose +h and -h trajectories sum different pairs inside r_cut or "
+                                 "accept a different number of nodes in any force evaluation (pair counts, pair-set "
+                                 "checksums and accepted-node counts of the FP64 walk) moves a pair across the "
+                                 "cutoff or flips an acceptance, where the loss jumps; it is excluded"}
         return {"value": (t_fwd_g + t_bwd) / t_fwd, "backward_over_forward": t_bwd / t_fwd,
                 "forward_ms": 1e3 * t_fwd, "gradient_ms": 1e3 * (t_fwd_g + t_bwd), "loss": loss,
                 "workload": f"adjoint_{n_side}^3_box{prob.box:g}_mesh{prob.mesh}, 10 KDK steps a 0.05->0.5, "
                             f"checkpoints every 2, FP64",
+                "forward": "strict walk with quadrupole moments (multipole_order 2)",
+                "accepted_nodes_over_trajectory": accepted,
                 **evals, "central_differences": fd}
     finally:
         prob.close()

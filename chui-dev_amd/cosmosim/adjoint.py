@@ -20,9 +20,14 @@ and finally lam_x += J(x_0)^T c_0, d loss / d psi = lam_x + a_0 H f lam_v. The
 wrap has unit derivative. Positions are checkpointed every `checkpoint_every`
 steps; the backward pass recomputes the positions inside each segment.
 
-The force VJP requires multipole_order 0: with the strict acceptance policy no
-node is accepted (tree_pm_acceptance.hpp:62-66), so the short-range force is
-the exact pair sum within r_cut and its Jacobian is the pairwise one.
+The forward is the strict solve with quadrupole moments (multipole_order 2,
+the benchmarked forward). The force VJP takes the walk's acceptance decisions:
+pairs contribute their pairwise Jacobian, and each accepted node its
+expansion's Jacobian through the target position and the node's centre of
+mass and quadrupole (tree_pm_coupling.cpp:743-827). On the near-uniform
+lattice workloads no node is accepted (2h/r < 0.08 needs cells far smaller
+than the particle spacing), so the short-range force there is the exact pair
+sum within r_cut (tree_pm_acceptance.hpp:62-66).
 """
 from __future__ import annotations
 
@@ -57,8 +62,9 @@ class KdkProblem:
     force_evaluations: int = 0
     vjp_evaluations: int = 0
     # When set, every force evaluation also returns the walk's diagnostics and
-    # appends (direct pair count, pair-set checksum) to pair_sets: two
-    # trajectories whose lists agree summed the same pairs inside r_cut.
+    # appends (direct pair count, pair-set checksum, accepted nodes) to
+    # pair_sets: two trajectories whose lists agree summed the same pairs
+    # inside r_cut and accepted as many nodes.
     record_pair_sets: bool = False
     pair_sets: list = field(default_factory=list)
 
@@ -73,7 +79,7 @@ class KdkProblem:
                                device=self.device)
         self.options = _lib.default_options(
             box_size=self.box, gravitational_constant=ics.G_CODE, opening_theta=0.5, asmth_cells=1.25,
-            rcut_cells=4.5 * 1.25, softening_epsilon=self.softening, multipole_order=0,
+            rcut_cells=4.5 * 1.25, softening_epsilon=self.softening, multipole_order=2,
             precision=_lib.PRECISION_FP64)
         self.coord = TreePmCoordinator(self.mesh)
         self.factors = [kdk_factors(a0, a1, ics.H0_CODE, self.omega_m, 1.0 - self.omega_m)
@@ -108,7 +114,8 @@ class KdkProblem:
             res = self.coord.solve_active_set(x[0], x[1], x[2], self.mass, ForceAccumulator((acc[0], acc[1], acc[2])),
                                               self.options, want_diagnostics=True)
             d = res.diagnostics
-            self.pair_sets.append((d["direct_pair_evaluations"], d["pair_set_checksum"]))
+            self.pair_sets.append((d["direct_pair_evaluations"], d["pair_set_checksum"],
+                                   d["accepted_internal_multipoles"]))
         else:
             self._call(self.coord.solve_active_set, x[0], x[1], x[2], self.mass,
                        ForceAccumulator((acc[0], acc[1], acc[2])), self.options)

@@ -197,7 +197,8 @@ class TreePmCoordinator:
 
     def force_vjp(self, x, y, z, m, lam, out, options):
         """out_k = sum_i (d a_i / d x_k)^T lam_i for every source (FP64 device
-        arrays in caller order; options must be FP64 with multipole_order 0)."""
+        arrays in caller order; options must be FP64 with the strict acceptance
+        policy, multipole_order 0 or 2)."""
         _lib.check(self._lib.chui_treepm_force_vjp_device(
             self._h, _ptr(x), _ptr(y), _ptr(z), _ptr(m), int(x.shape[0]), _ptr(lam[0]), _ptr(lam[1]),
             _ptr(lam[2]), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), C.byref(options)))

@@ -49,6 +49,10 @@ struct chui_treepm_context {
   chui::PmWorkspace pm64, pm32;
   chui::PmWorkspace vjp_fwd, vjp_adj;      // force VJP: forward field and adjoint potential
   DeviceBuffer vjp_lam, vjp_comp[3];       // force VJP: tree-ordered lam and deposit lanes
+  // force VJP with accepted quadrupole nodes: node accumulators, the accepted
+  // list (and sort scratch), its count, per-source corrections
+  DeviceBuffer vjp_nacc, vjp_rec, vjp_rec_alt, vjp_vals, vjp_vals_alt, vjp_sort, vjp_count, vjp_corr;
+  std::size_t vjp_accepted_cap = 0;
   DeviceBuffer src64, src32, htab;  // tree-ordered packed sources; H(t) table
   double htab_key[2] = {0.0, 0.0};
   DeviceBuffer h64tab;              // FP64 box walk: degree-5 cells of H(t)
@@ -1203,18 +1207,23 @@ chui_status chui_treepm_synchronize(chui_treepm_context* c) {
 
 // Transpose of the Jacobian of the total acceleration with respect to the
 // source positions, applied to lam: out_k = sum_i (d a_i / d x_k)^T lam_i over
-// all N sources (all active). FP64 only, and monopole moments only: with the
-// strict policy no node is then accepted (tree_pm_acceptance.hpp:62-66), the
-// short-range force is the exact pair sum within r_cut and its Jacobian is the
-// pairwise one. The tree and both PM fields are rebuilt at x.
+// all N sources (all active). FP64 only, for the strict policy with monopole
+// (multipole_order 0: no node is accepted, tree_pm_acceptance.hpp:62-66, and
+// the short-range force is the exact pair sum within r_cut) or quadrupole
+// moments (multipole_order 2: the walk takes the forward's acceptance
+// decisions, and each accepted node's expansion, tree_pm_coupling.cpp:743-827,
+// is differentiated with respect to the target and to the node's particles
+// through its centre of mass and quadrupole). The tree and both PM fields are
+// rebuilt at x.
 void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, const double* m, uint64_t n,
                     const double* lx, const double* ly, const double* lz, double* ox, double* oy, double* oz,
                     const chui_treepm_options* o) {
   validate_options(o, c->mesh);
   if (o->precision != CHUI_PRECISION_FP64)
     throw chui::UnsupportedError("the force VJP is built for FP64 precision only");
-  if (o->multipole_order != 0)
-    throw chui::UnsupportedError("the force VJP needs multipole_order 0 (exact pair sums within r_cut)");
+  if (o->acceptance_policy != CHUI_ACCEPT_STRICT)
+    throw chui::UnsupportedError("the force VJP differentiates the strict acceptance policy only");
+  const bool quad = o->multipole_order == 2;
   if (n > 0xFFFFFFFEull) throw std::overflow_error("Tree gravity source count exceeds the 32-bit tree-index contract");
   if (n > 0 && (!x || !y || !z || !m || !lx || !ly || !lz || !ox || !oy || !oz))
     throw std::invalid_argument("TreePM force VJP lanes are null");
@@ -1227,7 +1236,7 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
   tp.n = n;
   for (int a = 0; a < 3; ++a) tp.L[a] = o->box_size[a];
   tp.leaf_size = static_cast<uint32_t>(o->max_leaf_size);
-  tp.quadrupole = false;
+  tp.quadrupole = quad;
   tp.mixed = false;
   chui::build_tree(c->tree, x, y, z, m, tp, o->softening_epsilon, s);
   const bool uniform = c->tree.mass_uniform;
@@ -1273,7 +1282,7 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
 
   unsigned long long* const walk_cnt = walk_counters(c, s);
   chui::WalkParams wp = base_walk_params(c, o, sp, g.row);
-  wp.quad = 0;
+  wp.quad = quad ? 1 : 0;
   wp.n_targets = static_cast<uint32_t>(n);
   wp.identity_targets = 1;
   wp.count = 0;
@@ -1283,10 +1292,54 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
   chui::WalkOutputs out{ox, oy, oz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   wp.box64 = 1;
   prepare_h64(c, wp, sp, s);
-  chui::walk_fp64_vjp(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, c->vjp_lam.as<double4>(),
-                      static_cast<const double*>(c->vjp_fwd.force[0]), static_cast<const double*>(c->vjp_fwd.force[1]),
-                      static_cast<const double*>(c->vjp_fwd.force[2]), c->vjp_adj.buf[0].as<double>(), out,
-                      walk_cnt, c->h64tab.as<double2>(), s);
+  // accepted-node terms: a list sized 4 per source, or the last VJP's count
+  // if larger; a walk that accepts more runs again with the list it needs
+  chui::VjpNodeTerms nt;
+  std::size_t cap = std::max<std::size_t>(4 * n + 1024, c->vjp_accepted_cap);
+  for (int attempt = 0;; ++attempt) {
+    if (quad) {
+      const std::size_t nodes = c->tree.node_count;
+      cap = std::min<std::size_t>(cap, 0xFFFFFFFFull);
+      c->vjp_nacc.reserve(std::max<std::size_t>(nodes, 1) * 10 * sizeof(double));
+      c->vjp_rec.reserve(cap * sizeof(uint64_t));
+      c->vjp_rec_alt.reserve(cap * sizeof(uint64_t));
+      c->vjp_vals.reserve(cap * sizeof(uint32_t));
+      c->vjp_vals_alt.reserve(cap * sizeof(uint32_t));
+      c->vjp_sort.reserve(chui::radix_sort_scratch_bytes(cap));
+      c->vjp_count.reserve(sizeof(uint32_t));
+      c->vjp_corr.reserve(n * 3 * sizeof(double));
+      chui::fill_f64(c->vjp_nacc.as<double>(), 0.0, nodes * 10, s);
+      chui::fill_u32(c->vjp_count.as<uint32_t>(), 0u, 1, s);
+      chui::fill_f64(c->vjp_corr.as<double>(), 0.0, n * 3, s);
+      nt.node_acc = c->vjp_nacc.as<double>();
+      nt.accepted = c->vjp_rec.as<unsigned long long>();
+      nt.accepted_alt = c->vjp_rec_alt.as<unsigned long long>();
+      nt.vals = c->vjp_vals.as<uint32_t>();
+      nt.vals_alt = c->vjp_vals_alt.as<uint32_t>();
+      nt.sort_scratch = c->vjp_sort.ptr;
+      nt.count = c->vjp_count.as<unsigned int>();
+      nt.cap = static_cast<unsigned int>(cap);
+      nt.corr = c->vjp_corr.as<double>();
+    }
+    chui::walk_fp64_vjp(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, c->vjp_lam.as<double4>(),
+                        static_cast<const double*>(c->vjp_fwd.force[0]),
+                        static_cast<const double*>(c->vjp_fwd.force[1]),
+                        static_cast<const double*>(c->vjp_fwd.force[2]), c->vjp_adj.buf[0].as<double>(), out,
+                        walk_cnt, c->h64tab.as<double2>(), quad ? &nt : nullptr, s);
+    if (!quad) break;
+    uint32_t count = 0;
+    CHUI_HIP(hipMemcpyAsync(&count, nt.count, sizeof(count), hipMemcpyDeviceToHost, s));
+    CHUI_HIP(hipStreamSynchronize(s));
+    if (count > nt.cap) {
+      if (attempt > 0) throw std::logic_error("force VJP: accepted-node count changed between identical walks");
+      cap = static_cast<std::size_t>(count) + n;
+      c->vjp_accepted_cap = cap;
+      continue;
+    }
+    chui::vjp_node_corrections(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, um, c->vjp_lam.as<double4>(), nt,
+                               count, out, s);
+    break;
+  }
   post_fault_word(c, s);
 }
 

@@ -19,6 +19,8 @@
 //                  folded into a Kahan-compensated FP32 accumulator.
 #include "walk.hpp"
 
+#include "primitives.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -117,50 +119,145 @@ __device__ __forceinline__ bool within_cutoff(double d2, double c, double c2) {
   return d2 <= c2;
 }
 
-// monopolePlusQuadrupoleAccelPeriodicReference (tree_pm_coupling.cpp:743-827)
-__device__ __forceinline__ void node_accel64(const NodeMom64& M, const NodeQuad64& Q, double dx, double dy, double dz,
-                             const WalkParams& P, double& ax, double& ay, double& az) {
-  const double r2 = dx * dx + dy * dy + dz * dz;
-  const double safe_r2 = fmax(r2, 1.0e-30);
-  const double r = sqrt(safe_r2);
+// Forward-mode derivative of a scalar with respect to the node offset d
+// (three partials), for the Jacobian of the node expansion in the force VJP.
+struct D3 {
+  double v, x, y, z;
+};
+__device__ __forceinline__ D3 d3c(double c) { return D3{c, 0.0, 0.0, 0.0}; }
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return D3{a.v + b.v, a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return D3{a.v - b.v, a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 operator-(D3 a) { return D3{-a.v, -a.x, -a.y, -a.z}; }
+__device__ __forceinline__ D3 operator*(D3 a, D3 b) {
+  return D3{a.v * b.v, a.x * b.v + a.v * b.x, a.y * b.v + a.v * b.y, a.z * b.v + a.v * b.z};
+}
+__device__ __forceinline__ D3 operator*(D3 a, double c) { return D3{a.v * c, a.x * c, a.y * c, a.z * c}; }
+__device__ __forceinline__ D3 operator*(double c, D3 a) { return a * c; }
+__device__ __forceinline__ D3 operator+(D3 a, double c) { return D3{a.v + c, a.x, a.y, a.z}; }
+__device__ __forceinline__ D3 operator+(double c, D3 a) { return a + c; }
+__device__ __forceinline__ D3 operator-(D3 a, double c) { return D3{a.v - c, a.x, a.y, a.z}; }
+__device__ __forceinline__ D3 operator-(double c, D3 a) { return D3{c - a.v, -a.x, -a.y, -a.z}; }
+__device__ __forceinline__ D3 operator/(D3 a, D3 b) {
+  const double q = a.v / b.v, ib = 1.0 / b.v;
+  return D3{q, (a.x - q * b.x) * ib, (a.y - q * b.y) * ib, (a.z - q * b.z) * ib};
+}
+__device__ __forceinline__ D3 operator/(D3 a, double c) { return D3{a.v / c, a.x / c, a.y / c, a.z / c}; }
+__device__ __forceinline__ D3 operator/(double c, D3 b) { return d3c(c) / b; }
+__device__ __forceinline__ D3 d_scale(D3 a, double v, double dv) { return D3{v, a.x * dv, a.y * dv, a.z * dv}; }
+__device__ __forceinline__ double f_max(double a, double c) { return fmax(a, c); }
+__device__ __forceinline__ D3 f_max(D3 a, double c) { return a.v > c ? a : d3c(fmax(a.v, c)); }
+__device__ __forceinline__ double f_sqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ D3 f_sqrt(D3 a) {
+  const double v = sqrt(a.v);
+  return d_scale(a, v, 0.5 / v);
+}
+__device__ __forceinline__ double f_exp(double a) { return exp(a); }
+__device__ __forceinline__ D3 f_exp(D3 a) {
+  const double v = exp(a.v);
+  return d_scale(a, v, v);
+}
+// short_factor of a dual r: d/dr [erfc(q) + 2 q exp(-q^2)/sqrt(pi)] = -2 q^2 exp(-q^2) / (sqrt(pi) a), q = r/2a
+__device__ __forceinline__ double f_short(double r, double a) { return short_factor(r, a); }
+__device__ __forceinline__ D3 f_short(D3 r, double a) {
+  const double q = (0.5 * r.v) / a;
+  const double dv = (r.v <= 0.0 || isinf(q)) ? 0.0 : -2.0 * kInvSqrtPi * q * q * exp(-q * q) / a;
+  return d_scale(r, short_factor(r.v, a), dv);
+}
+
+// monopolePlusQuadrupoleAccelPeriodicReference (tree_pm_coupling.cpp:743-827),
+// on double (the walks) or on D3 (its Jacobian with respect to the offset).
+template <class T>
+__device__ __forceinline__ void node_accel_t(double mass, const NodeQuad64& Q, T dx, T dy, T dz,
+                                             const WalkParams& P, T& cx, T& cy, T& cz) {
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  const T safe_r2 = f_max(r2, 1.0e-30);
+  const T r = f_sqrt(safe_r2);
   const double pe = P.eps;  // uniform softening: max(node eps, target eps) = eps
   const double eps2 = pe * pe;
-  const double denom = fmax(r2 + eps2, 1.0e-30);
-  const double soft = 1.0 / (denom * sqrt(denom));
-  const double split = short_factor(r, P.a);
-  const double newton = 1.0 / (safe_r2 * r);
-  const double radial = soft + (split - 1.0) * newton;
+  const T denom = f_max(r2 + eps2, 1.0e-30);
+  const T soft = 1.0 / (denom * f_sqrt(denom));
+  const T split = f_short(r, P.a);
+  const T newton = 1.0 / (safe_r2 * r);
+  const T radial = soft + (split - 1.0) * newton;
   const double pref = P.G;
-  double cx = pref * M.mass * radial * dx;
-  double cy = pref * M.mass * radial * dy;
-  double cz = pref * M.mass * radial * dz;
+  cx = pref * mass * radial * dx;
+  cy = pref * mass * radial * dy;
+  cz = pref * mass * radial * dz;
   if (P.quad) {
     const double tr = Q.trace;
     const double mxx = (Q.qxx + tr) / 3.0, mxy = Q.qxy / 3.0, mxz = Q.qxz / 3.0;
     const double myy = (Q.qyy + tr) / 3.0, myz = Q.qyz / 3.0, mzz = (Q.qzz + tr) / 3.0;
-    const double mdx = mxx * dx + mxy * dy + mxz * dz;
-    const double mdy = mxy * dx + myy * dy + myz * dz;
-    const double mdz = mxz * dx + myz * dy + mzz * dz;
-    const double dmd = dx * mdx + dy * mdy + dz * mdz;
+    const T mdx = mxx * dx + mxy * dy + mxz * dz;
+    const T mdy = mxy * dx + myy * dy + myz * dz;
+    const T mdz = mxz * dx + myz * dy + mzz * dz;
+    const T dmd = dx * mdx + dy * mdy + dz * mdz;
     const double a = P.a, a2 = a * a;
-    const double ex = exp(-r2 / (4.0 * a2));
-    const double s1 = -0.5 * kInvSqrtPi * r2 * ex / (a * a2);
-    const double s2 = kInvSqrtPi * ex * (-r / (a * a2) + 0.25 * r * r2 / (a * a2 * a2));
-    const double f1 = -3.0 * r / (denom * denom * sqrt(denom));
-    const double f2 = -3.0 / (denom * denom * sqrt(denom)) + 15.0 * r2 / (denom * denom * denom * sqrt(denom));
-    const double n1 = -3.0 / (safe_r2 * safe_r2);
-    const double n2 = 12.0 / (safe_r2 * safe_r2 * r);
-    const double rf = f1 + s1 * newton + (split - 1.0) * n1;
-    const double rs = f2 + s2 * newton + 2.0 * s1 * n1 + (split - 1.0) * n2;
-    const double fo = rf / r;
-    const double cr = rs / r2 - rf / (r2 * r);
-    cx += pref * (mdx * fo + 0.5 * tr * dx * fo + 0.5 * dx * dmd * cr);
-    cy += pref * (mdy * fo + 0.5 * tr * dy * fo + 0.5 * dy * dmd * cr);
-    cz += pref * (mdz * fo + 0.5 * tr * dz * fo + 0.5 * dz * dmd * cr);
+    const T ex = f_exp(-r2 / (4.0 * a2));
+    const T s1 = -0.5 * kInvSqrtPi * r2 * ex / (a * a2);
+    const T s2 = kInvSqrtPi * ex * (-r / (a * a2) + 0.25 * r * r2 / (a * a2 * a2));
+    const T f1 = -3.0 * r / (denom * denom * f_sqrt(denom));
+    const T f2 = -3.0 / (denom * denom * f_sqrt(denom)) + 15.0 * r2 / (denom * denom * denom * f_sqrt(denom));
+    const T n1 = -3.0 / (safe_r2 * safe_r2);
+    const T n2 = 12.0 / (safe_r2 * safe_r2 * r);
+    const T rf = f1 + s1 * newton + (split - 1.0) * n1;
+    const T rs = f2 + s2 * newton + 2.0 * s1 * n1 + (split - 1.0) * n2;
+    const T fo = rf / r;
+    const T cr = rs / r2 - rf / (r2 * r);
+    cx = cx + pref * (mdx * fo + 0.5 * tr * dx * fo + 0.5 * dx * dmd * cr);
+    cy = cy + pref * (mdy * fo + 0.5 * tr * dy * fo + 0.5 * dy * dmd * cr);
+    cz = cz + pref * (mdz * fo + 0.5 * tr * dz * fo + 0.5 * dz * dmd * cr);
   }
+}
+
+__device__ __forceinline__ void node_accel64(const NodeMom64& M, const NodeQuad64& Q, double dx, double dy, double dz,
+                                             const WalkParams& P, double& ax, double& ay, double& az) {
+  double cx, cy, cz;
+  node_accel_t<double>(M.mass, Q, dx, dy, dz, P, cx, cy, cz);
   ax += cx;
   ay += cy;
   az += cz;
+}
+
+// Force VJP at an accepted node N of target t (offset d = COM_N - x_t, lam of
+// t): the node's force a(d, Q) depends on x_t through d, and on the positions
+// of N's particles through COM_N (= d + x_t) and the quadrupole Q.
+//   g = J^T lam with J = da/dd (forward-mode D3);
+//   s_t -= g (da/dx_t = -J);
+//   U_N += g, V_N[p] += lam . da/dQ_p (a is linear in the seven stored Q
+//   components qxx..qzz, trace: each column is the expansion with mass 0 and
+//   Q = e_p).
+// U_N and V_N are spread over N's particles afterwards (vjp_node_push).
+__device__ __attribute__((noinline)) void jvp_node64(const NodeMom64& M, const NodeQuad64& Q, double dx, double dy,
+                                                     double dz, const WalkParams& P, double lx, double ly,
+                                                     double lz, double* __restrict__ acc, double& sx, double& sy,
+                                                     double& sz) {
+  D3 cx, cy, cz;
+  node_accel_t<D3>(M.mass, Q, D3{dx, 1.0, 0.0, 0.0}, D3{dy, 0.0, 1.0, 0.0}, D3{dz, 0.0, 0.0, 1.0}, P, cx, cy, cz);
+  const double gx = lx * cx.x + ly * cy.x + lz * cz.x;
+  const double gy = lx * cx.y + ly * cy.y + lz * cz.y;
+  const double gz = lx * cx.z + ly * cy.z + lz * cz.z;
+  sx -= gx;
+  sy -= gy;
+  sz -= gz;
+  atomicAdd(acc + 0, gx);
+  atomicAdd(acc + 1, gy);
+  atomicAdd(acc + 2, gz);
+  if (!P.quad) return;
+  for (int q = 0; q < 7; ++q) {
+    NodeQuad64 e{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    switch (q) {
+      case 0: e.qxx = 1.0; break;
+      case 1: e.qxy = 1.0; break;
+      case 2: e.qxz = 1.0; break;
+      case 3: e.qyy = 1.0; break;
+      case 4: e.qyz = 1.0; break;
+      case 5: e.qzz = 1.0; break;
+      default: e.trace = 1.0; break;
+    }
+    double ex, ey, ez;
+    node_accel_t<double>(0.0, e, dx, dy, dz, P, ex, ey, ez);
+    atomicAdd(acc + 3 + q, lx * ex + ly * ey + lz * ez);
+  }
 }
 
 // Strict-reference acceptance of an internal node (tree_pm_acceptance.hpp:54-139,
@@ -670,6 +767,12 @@ struct WalkData {
   const float4* ltab;
   // preorder index of each node's child per octant (kInvalid for none)
   const uint32_t* child;
+  // force VJP with accepted nodes: per preorder node U (3) and V (7)
+  // accumulators, and the (target tree position << 32 | node) list
+  double* vjp_node_acc;
+  unsigned long long* vjp_accepted;
+  unsigned int* vjp_accepted_count;
+  unsigned int vjp_accepted_cap;
 };
 
 // splitmix64 finaliser of the pair (target row, source row): the pair-set
@@ -2432,7 +2535,28 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
         }
       } else if (__ballot(keep)) {
         if constexpr (JVP) {
-          if (keep) nxt = n + 1;  // monopoles only: every kept internal node is opened
+          // the forward's strict acceptance; an accepted node's Jacobian terms
+          // go to the target and to the node's accumulators
+          bool take_mp = false;
+          double mdx = 0.0, mdy = 0.0, mdz = 0.0;
+          if (keep && P.quad) {
+            const bool inside = ax_ <= h && ay_ <= h && az_ <= h;
+            const double mx = ax_ + h, my = ay_ + h, mz = az_ + h;
+            const bool contained = within_cutoff(mx * mx + my * my + mz * mz, P.rcut, P.rcut2);
+            const NodeMom64 M = D.mom64[n];
+            mdx = mi_fast(M.comx - px, 0);
+            mdy = mi_fast(M.comy - py, 1);
+            mdz = mi_fast(M.comz - pz, 2);
+            take_mp = accept64(h, D.quad64[n].offset, inside, contained, mdx * mdx + mdy * mdy + mdz * mdz, P);
+          }
+          if (keep) nxt = take_mp ? n + lk.skip : n + 1;
+          if (__ballot(take_mp) && take_mp) {
+            jvp_node64(D.mom64[n], D.quad64[n], mdx, mdy, mdz, P, lk_.x, lk_.y, lk_.z, D.vjp_node_acc + 10ull * n,
+                       ax, ay, az);
+            const unsigned int slot_i = atomicAdd(D.vjp_accepted_count, 1u);
+            if (slot_i < D.vjp_accepted_cap)
+              D.vjp_accepted[slot_i] = (static_cast<unsigned long long>(tp) << 32) | n;
+          }
         } else {
           bool take_mp = false;
           double mdx = 0.0, mdy = 0.0, mdz = 0.0;
@@ -2474,7 +2598,7 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
         next = q + lk.skip;
         return 0;
       }
-      if (!JVP && P.quad && h < 0.041 * rc) {
+      if (P.quad && h < 0.041 * rc) {
         // accept64 needs 2h/r < 0.08: every target within 25 h of the COM fails it
         const NodeMom64 M = D.mom64[q];
         const double fx_ = fabs(mi_fast(M.comx - bx, 0)) + ex, fy_ = fabs(mi_fast(M.comy - by, 1)) + ey,
@@ -2614,6 +2738,108 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
     static_cast<double*>(out.sr_y)[row] = ay;
     static_cast<double*>(out.sr_z)[row] = az;
   }
+}
+
+// ---- force VJP: accepted-node terms ----------------------------------------
+// For a direct pair (t sees p) the symmetric pair term of p's walk adds
+// M_tp lam_t to s_p, M_tp = d a_t / d x_p = G m_p [F I + 2 F' r r^T]; it is
+// right only if t also sees p directly. For an accepted (t, N) and p in N
+// within r_cut that sees t directly (p accepts no node holding t): remove
+// M_tp lam_t from s_p, and add to s_t the term p's force contributes,
+// M_pt lam_p = G m_t [F lam_p + 2 F' (r . lam_p) r]. rec is sorted.
+__global__ __launch_bounds__(256) void vjp_pair_fix_kernel(const unsigned long long* __restrict__ rec, uint32_t nrec,
+                                                           const NodeLink* __restrict__ link,
+                                                           const double4* __restrict__ src,
+                                                           const double4* __restrict__ lam, int uniform, double um,
+                                                           WalkParams P, double* __restrict__ corr) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrec) return;
+  const uint32_t t = static_cast<uint32_t>(rec[i] >> 32), node = static_cast<uint32_t>(rec[i] & 0xFFFFFFFFull);
+  const double4 xt = src[t], lt = lam[t];
+  const double mt = uniform ? um : xt.w;
+  const NodeLink lk = link[node];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (uint32_t pp = lk.begin; pp < lk.begin + lk.count; ++pp) {
+    const double4 xp = src[pp];
+    const double dx = mi64(xp.x - xt.x, P.L[0]), dy = mi64(xp.y - xt.y, P.L[1]), dz = mi64(xp.z - xt.z, P.L[2]);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 <= P.rcut2) || !(r2 > 0.0)) continue;
+    const unsigned long long key = static_cast<unsigned long long>(pp) << 32;
+    uint32_t lo = 0, hi = nrec;
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      if (rec[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    bool mutual = false;
+    for (uint32_t j = lo; j < nrec && static_cast<uint32_t>(rec[j] >> 32) == pp; ++j) {
+      const NodeLink l2 = link[static_cast<uint32_t>(rec[j] & 0xFFFFFFFFull)];
+      if (t >= l2.begin && t < l2.begin + l2.count) {
+        mutual = true;
+        break;
+      }
+    }
+    if (mutual) continue;
+    const double f = residual_inv_r3(r2, P.eps, P.a), fd = residual_inv_r3_dt(r2, P.eps, P.a);
+    const double4 lp = lam[pp];
+    const double mp = uniform ? um : xp.w;
+    const double rt = dx * lt.x + dy * lt.y + dz * lt.z, rp = dx * lp.x + dy * lp.y + dz * lp.z;
+    const double gp = P.G * mp, gt = P.G * mt;
+    atomicAdd(&corr[3ull * pp + 0], -gp * (f * lt.x + 2.0 * fd * rt * dx));
+    atomicAdd(&corr[3ull * pp + 1], -gp * (f * lt.y + 2.0 * fd * rt * dy));
+    atomicAdd(&corr[3ull * pp + 2], -gp * (f * lt.z + 2.0 * fd * rt * dz));
+    s0 += gt * (f * lp.x + 2.0 * fd * rp * dx);
+    s1 += gt * (f * lp.y + 2.0 * fd * rp * dy);
+    s2 += gt * (f * lp.z + 2.0 * fd * rp * dz);
+  }
+  atomicAdd(&corr[3ull * t + 0], s0);
+  atomicAdd(&corr[3ull * t + 1], s1);
+  atomicAdd(&corr[3ull * t + 2], s2);
+}
+
+// U and V of node N spread over its particles p (d_p = x_p - COM_N):
+// s_p += (m_p / M) U + m_p G_V d_p, with G_V the symmetric matrix of
+// sum_q V_q d Q_q / d d_p for the stored components (Q_q of one particle:
+// m (3 d d^T - |d|^2 I) traceless parts qxx..qzz and the trace m |d|^2).
+__global__ __launch_bounds__(256) void vjp_node_push_kernel(uint32_t nodes, const double* __restrict__ acc,
+                                                            const NodeLink* __restrict__ link,
+                                                            const NodeMom64* __restrict__ mom,
+                                                            const double4* __restrict__ src, int uniform, double um,
+                                                            WalkParams P, double* __restrict__ corr) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nodes) return;
+  const double* a = acc + 10ull * n;
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) any = any || a[k] != 0.0;
+  if (!any) return;
+  const NodeMom64 M = mom[n];
+  if (!(M.mass > 0.0)) return;
+  const double ux = a[0] / M.mass, uy = a[1] / M.mass, uz = a[2] / M.mass;
+  const double v0 = a[3], v1 = a[4], v2 = a[5], v3 = a[6], v4 = a[7], v5 = a[8], v6 = a[9];
+  const double gxx = 4.0 * v0 - 2.0 * v3 - 2.0 * v5 + 2.0 * v6, gxy = 3.0 * v1, gxz = 3.0 * v2;
+  const double gyy = -2.0 * v0 + 4.0 * v3 - 2.0 * v5 + 2.0 * v6, gyz = 3.0 * v4;
+  const double gzz = -2.0 * v0 - 2.0 * v3 + 4.0 * v5 + 2.0 * v6;
+  const NodeLink lk = link[n];
+  for (uint32_t pp = lk.begin; pp < lk.begin + lk.count; ++pp) {
+    const double4 xp = src[pp];
+    const double mp = uniform ? um : xp.w;
+    const double dx = mi64(xp.x - M.comx, P.L[0]), dy = mi64(xp.y - M.comy, P.L[1]), dz = mi64(xp.z - M.comz, P.L[2]);
+    atomicAdd(&corr[3ull * pp + 0], mp * (ux + gxx * dx + gxy * dy + gxz * dz));
+    atomicAdd(&corr[3ull * pp + 1], mp * (uy + gxy * dx + gyy * dy + gyz * dz));
+    atomicAdd(&corr[3ull * pp + 2], mp * (uz + gxz * dx + gyz * dy + gzz * dz));
+  }
+}
+
+__global__ __launch_bounds__(256) void vjp_apply_kernel(uint32_t n, const uint32_t* __restrict__ order,
+                                                        const double* __restrict__ corr, double* __restrict__ ox,
+                                                        double* __restrict__ oy, double* __restrict__ oz) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t row = order[k];
+  ox[row] += corr[3ull * k + 0];
+  oy[row] += corr[3ull * k + 1];
+  oz[row] += corr[3ull * k + 2];
 }
 
 // H(t) = L(sqrt t) / t^{3/2} (tree_pm_split_kernel.hpp:109-128).
@@ -2911,7 +3137,7 @@ std::vector<double> gaussian_hermite_monomials(const std::vector<double>& knots)
 
 void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,
                    const double* fx, const double* fy, const double* fz, const double* nu, const WalkOutputs& out,
-                   unsigned long long* counters, const double2* h64tab, hipStream_t s) {
+                   unsigned long long* counters, const double2* h64tab, const VjpNodeTerms* nt, hipStream_t s) {
   if (!t.records64 && t.node_count > 0) throw std::logic_error("FP64 walk on a tree without FP64 records");
   if (p.n_targets == 0) return;
   WalkData d{};
@@ -2927,6 +3153,13 @@ void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int u
   d.f64z = fz;
   d.lam64 = lam;
   d.nu64 = nu;
+  if (p.quad) {
+    if (!nt || !p.box64) throw std::logic_error("force VJP with accepted nodes needs the node-term buffers");
+    d.vjp_node_acc = nt->node_acc;
+    d.vjp_accepted = nt->accepted;
+    d.vjp_accepted_count = nt->count;
+    d.vjp_accepted_cap = nt->cap;
+  }
   if (p.box64) {
     d.h64tab = h64tab;
     const unsigned grid = grid_for(p.n_targets, kWalkThreads);
@@ -2937,6 +3170,28 @@ void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int u
   }
   if (p.assignment == 0) launch64<0, true>(p, d, out, counters, s);
   else launch64<1, true>(p, d, out, counters, s);
+}
+
+void vjp_node_corrections(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
+                          double uniform_mass_value, const double4* lam, const VjpNodeTerms& nt, unsigned int count,
+                          const WalkOutputs& out, hipStream_t s) {
+  const std::size_t n = t.n;
+  if (n == 0) return;
+  if (count > 0) {
+    radix_sort_pairs(reinterpret_cast<uint64_t*>(nt.accepted), nt.vals, reinterpret_cast<uint64_t*>(nt.accepted_alt),
+                     nt.vals_alt, count, 0, 64, nt.sort_scratch, s);
+    vjp_pair_fix_kernel<<<grid_for(count, 256), 256, 0, s>>>(nt.accepted, count, t.link.as<NodeLink>(), src, lam,
+                                                             uniform_mass, uniform_mass_value, p, nt.corr);
+    CHUI_LAUNCH_CHECK("vjp_pair_fix_kernel");
+    vjp_node_push_kernel<<<grid_for(t.node_count, 256), 256, 0, s>>>(
+        static_cast<uint32_t>(t.node_count), nt.node_acc, t.link.as<NodeLink>(), t.mom64.as<NodeMom64>(), src,
+        uniform_mass, uniform_mass_value, p, nt.corr);
+    CHUI_LAUNCH_CHECK("vjp_node_push_kernel");
+  }
+  vjp_apply_kernel<<<grid_for(n, 256), 256, 0, s>>>(static_cast<uint32_t>(n), t.order.as<uint32_t>(), nt.corr,
+                                                    static_cast<double*>(out.ax), static_cast<double*>(out.ay),
+                                                    static_cast<double*>(out.az));
+  CHUI_LAUNCH_CHECK("vjp_apply_kernel");
 }
 
 void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int uniform_mass, const float4* htab,

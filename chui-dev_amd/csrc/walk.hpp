@@ -136,13 +136,38 @@ void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int unifo
                const double* fz, const WalkOutputs& out, unsigned long long* counters, const double2* h64tab,
                hipStream_t s);
 
+// Device buffers of the force VJP's accepted-node terms (multipole order 2):
+// U (3) and V (7) per preorder node, the accepted (target << 32 | node) list
+// with its count and capacity, sort scratch for it, and per-source
+// corrections (3 per source, tree order).
+struct VjpNodeTerms {
+  double* node_acc = nullptr;
+  unsigned long long* accepted = nullptr;
+  unsigned long long* accepted_alt = nullptr;
+  uint32_t* vals = nullptr;
+  uint32_t* vals_alt = nullptr;
+  void* sort_scratch = nullptr;
+  unsigned int* count = nullptr;
+  unsigned int cap = 0;
+  double* corr = nullptr;
+};
+
 // Transpose of the total-force Jacobian applied to lam (FP64, all targets, tree
 // order identity). src and lam are tree-ordered; fx..fz the forward force
-// meshes and nu the adjoint potential on the same mesh; multipole order 0.
-// p.box64 selects the box-classified walk with the polynomial table (h64tab).
+// meshes and nu the adjoint potential on the same mesh. p.box64 selects the
+// box-classified walk with the polynomial table (h64tab). With p.quad the
+// walk applies the forward's strict acceptance: an accepted node's terms for
+// the target are added in the walk, and nt (required then) collects the rest.
 void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass, const double4* lam,
                    const double* fx, const double* fy, const double* fz, const double* nu, const WalkOutputs& out,
-                   unsigned long long* counters, const double2* h64tab, hipStream_t s);
+                   unsigned long long* counters, const double2* h64tab, const VjpNodeTerms* nt, hipStream_t s);
+// The accepted-node terms after walk_fp64_vjp (count accepted pairs, read back
+// by the caller): per accepted (t, N), the pairs between t and N's particles
+// that the symmetric pair terms counted on one side only; U and V spread over
+// N's particles; the corrections added to out (caller order).
+void vjp_node_corrections(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
+                          double uniform_mass_value, const double4* lam, const VjpNodeTerms& nt, unsigned int count,
+                          const WalkOutputs& out, hipStream_t s);
 // Per-target FP64 walk under the adaptive acceptance. gtab = Gaussian Hermite
 // knots (value, slope) x 4097 (required when p.lookup), prev = previous
 // acceleration magnitude per output slot (nullable).

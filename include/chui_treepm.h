@@ -456,9 +456,12 @@ chui_status chui_treepm_mass_momentum_noalloc_device(chui_treepm_context* ctx, c
  * with respect to the source positions, applied to lam:
  *   out_k = sum_i (d a_i / d x_k)^T lam_i,  all arrays device FP64 in caller order.
  * The tree and both PM fields are rebuilt at (x, y, z). Requires FP64 precision
- * and multipole_order 0: with the strict policy no node is then accepted
- * (tree_pm_acceptance.hpp:62-66), so the short-range force is the exact pair
- * sum within r_cut; otherwise CHUI_ERR_UNSUPPORTED. */
+ * and the strict acceptance policy, with monopole (multipole_order 0: no node
+ * is accepted, tree_pm_acceptance.hpp:62-66) or quadrupole moments
+ * (multipole_order 2: the walk takes the forward's acceptance decisions and
+ * differentiates each accepted node's expansion, tree_pm_coupling.cpp:743-827,
+ * through the target position and the node's centre of mass and quadrupole);
+ * otherwise CHUI_ERR_UNSUPPORTED. */
 chui_status chui_treepm_force_vjp_device(chui_treepm_context* ctx, const double* pos_x, const double* pos_y,
                                          const double* pos_z, const double* mass, uint64_t n,
                                          const double* lam_x, const double* lam_y, const double* lam_z,

@@ -146,7 +146,7 @@ def test_cic_density_vjp_is_the_transpose():
         prob.close()
 
 
-@pytest.mark.parametrize("field,value", [("multipole_order", 2), ("precision", 1)])
+@pytest.mark.parametrize("field,value", [("acceptance_policy", 1), ("precision", 1)])
 def test_force_vjp_rejects_unsupported_options(field, value):
     prob, psi, _ = problem(8, 1)
     try:

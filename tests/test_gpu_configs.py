@@ -11,13 +11,15 @@
   against sum m |a|, and FP64 against the GPU FP64 Ewald sum on 512 sample
   particles within the algorithm's own error (CIC, r_cut = 4.5 r_s).
 - 64^3 particles, box 50, mesh 128^3, ten KDK steps a 0.05 -> 0.5 with
-  checkpoints every two: the gradient of the density loss against central
-  differences at 1e-6 mean spacings along 16 random unit directions. A step
-  that moves a pair across r_cut makes the loss jump (the truncated residual
-  is 1.8e-2 of Newtonian at r_cut), so the pass rule detects those steps: the
-  FP64 walk's pair-set checksum over every force evaluation of the +h and -h
-  trajectories (chui_treepm_diagnostics.pair_set_checksum and the pair
-  counts). Directions whose pair sets agree must match to 1e-4 (median 1e-5);
+  checkpoints every two, the strict forward with quadrupole moments: the
+  gradient of the density loss against central differences at 1e-6 mean
+  spacings along 16 random unit directions. A step that moves a pair across
+  r_cut makes the loss jump (the truncated residual is 1.8e-2 of Newtonian at
+  r_cut), and so does one that flips a node's acceptance, so the pass rule
+  detects those steps: the FP64 walk's pair-set checksum, pair count and
+  accepted-node count over every force evaluation of the +h and -h
+  trajectories (chui_treepm_diagnostics). Directions whose pair sets agree
+  must match to 1e-4 (median 1e-5);
   at least a quarter of them must keep their pair sets (about one pair in
   10^9 lies within a step of r_cut, so roughly half the directions cross).
 - The 48-box ensemble (16 x 32^3, 16 x 64^3, 12 x 96^3, 4 x 128^3 particles,
