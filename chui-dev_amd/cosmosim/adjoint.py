@@ -142,10 +142,14 @@ class KdkProblem:
                    grad_delta.contiguous(), (out[0], out[1], out[2]))
         return out
 
-    def kdk_step(self, n: int, x, v, g):
+    def kdk_step(self, n: int, x, v, g, positions_only: bool = False):
+        """One KDK step; with positions_only the drifted positions alone (no
+        force evaluation, v and g returned as None)."""
         f = self.factors[n]
         vh = f["first_hubble_drag"] * v + f["first_kick_factor"] * g
         x = self.wrap(x + f["drift_factor"] * vh)
+        if positions_only:
+            return x, None, None
         g = self.forces(x)
         v = f["second_hubble_drag"] * vh + f["second_kick_factor"] * g
         return x, v, g
@@ -179,11 +183,13 @@ class KdkTrajectory(torch.autograd.Function):
         starts = sorted(ctx.checkpoints)
         for seg_start in reversed(starts):
             seg_end = min(seg_start + prob.checkpoint_every, S)
-            # positions x_seg_start .. x_seg_end (recomputed inside the segment)
+            # positions x_seg_start .. x_seg_end (recomputed inside the segment);
+            # the reverse sweep needs positions only, so the last recomputed
+            # step drifts without evaluating the force at its new positions
             x, v, g = ctx.checkpoints[seg_start]
             xs = [x]
             for n in range(seg_start, seg_end - 1):
-                x, v, g = prob.kdk_step(n, x, v, g)
+                x, v, g = prob.kdk_step(n, x, v, g, positions_only=n == seg_end - 2)
                 xs.append(x)
             xs.append(ctx.final if seg_end == S else ctx.checkpoints[seg_end][0])
             for n in range(seg_end - 1, seg_start - 1, -1):

@@ -60,16 +60,11 @@ struct SourceStats {
   uint32_t pad;
 };
 
-__global__ void stats_init_kernel(SourceStats* st) {
-  for (int a = 0; a < 3; ++a) { st->lo[a] = kEmpty; st->hi[a] = 0; }
-  st->bad_coord = 0;
-  st->bad_mass = 0;
-  st->mass_nonuniform = 0;
-  st->pad = 0;
-}
-
+// Each block writes its partial statistics to part[blockIdx.x]; one block then
+// folds the partials (same-address atomics from every block would serialise
+// at the L2, about 75 us for 1024 blocks whatever n is).
 __global__ __launch_bounds__(256) void source_stats_kernel(Axes3 p, const double* __restrict__ m, std::size_t n,
-                                                           SourceStats* st) {
+                                                           SourceStats* __restrict__ part) {
   uint64_t lo[3] = {kEmpty, kEmpty, kEmpty}, hi[3] = {0, 0, 0};
   uint32_t bc = 0, bm = 0, nu = 0;
   const double m0 = (m != nullptr && n > 0) ? m[0] : 0.0;
@@ -141,14 +136,70 @@ __global__ __launch_bounds__(256) void source_stats_kernel(Axes3 p, const double
       bm |= sflag[v][1];
       nu |= sflag[v][2];
     }
+    SourceStats r;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (lo[a] != kEmpty) atomicMin(reinterpret_cast<unsigned long long*>(&st->lo[a]), lo[a]);
-      if (hi[a] != 0) atomicMax(reinterpret_cast<unsigned long long*>(&st->hi[a]), hi[a]);
+      r.lo[a] = lo[a];
+      r.hi[a] = hi[a];
     }
-    if (bc) atomicOr(&st->bad_coord, 1u);
-    if (bm) atomicOr(&st->bad_mass, 1u);
-    if (nu) atomicOr(&st->mass_nonuniform, 1u);
+    r.bad_coord = bc;
+    r.bad_mass = bm;
+    r.mass_nonuniform = nu;
+    r.pad = 0u;
+    part[blockIdx.x] = r;
+  }
+}
+
+__global__ __launch_bounds__(256) void stats_final_kernel(const SourceStats* __restrict__ part, uint32_t nb,
+                                                          SourceStats* __restrict__ st) {
+  uint64_t lo[3] = {kEmpty, kEmpty, kEmpty}, hi[3] = {0, 0, 0};
+  uint32_t bc = 0, bm = 0, nu = 0;
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    const SourceStats& q = part[b];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = q.lo[a] < lo[a] ? q.lo[a] : lo[a];
+      hi[a] = q.hi[a] > hi[a] ? q.hi[a] : hi[a];
+    }
+    bc |= q.bad_coord;
+    bm |= q.bad_mass;
+    nu |= q.mass_nonuniform;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = wave_min(lo[a]);
+    hi[a] = wave_max(hi[a]);
+  }
+  bc = wave_max(bc);
+  bm = wave_max(bm);
+  nu = wave_max(nu);
+  __shared__ SourceStats sw[256 / kWave];
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      sw[w].lo[a] = lo[a];
+      sw[w].hi[a] = hi[a];
+    }
+    sw[w].bad_coord = bc;
+    sw[w].bad_mass = bm;
+    sw[w].mass_nonuniform = nu;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    SourceStats r = sw[0];
+    for (int v = 1; v < static_cast<int>(blockDim.x) / kWave; ++v) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        r.lo[a] = sw[v].lo[a] < r.lo[a] ? sw[v].lo[a] : r.lo[a];
+        r.hi[a] = sw[v].hi[a] > r.hi[a] ? sw[v].hi[a] : r.hi[a];
+      }
+      r.bad_coord |= sw[v].bad_coord;
+      r.bad_mass |= sw[v].bad_mass;
+      r.mass_nonuniform |= sw[v].mass_nonuniform;
+    }
+    r.pad = 0u;
+    *st = r;
   }
 }
 
@@ -1127,14 +1178,15 @@ void grow_copy(DeviceBuffer& b, std::size_t old_count, std::size_t new_count, hi
 
 // Source statistics -> host.
 SourceStats compute_stats(Tree& t, Axes3 p, const double* m, std::size_t n, hipStream_t s) {
-  t.bounds.reserve(sizeof(SourceStats));
+  const unsigned nb = n > 0 ? std::min<unsigned>(grid_for(n, 256), 1024u) : 0u;
+  t.bounds.reserve((1 + std::max(nb, 1u)) * sizeof(SourceStats));
   SourceStats* st = t.bounds.as<SourceStats>();
-  stats_init_kernel<<<1, 1, 0, s>>>(st);
-  CHUI_LAUNCH_CHECK("stats_init_kernel");
   if (n > 0) {
-    source_stats_kernel<<<std::min<unsigned>(grid_for(n, 256), 1024u), 256, 0, s>>>(p, m, n, st);
+    source_stats_kernel<<<nb, 256, 0, s>>>(p, m, n, st + 1);
     CHUI_LAUNCH_CHECK("source_stats_kernel");
   }
+  stats_final_kernel<<<1, 256, 0, s>>>(st + 1, nb, st);
+  CHUI_LAUNCH_CHECK("stats_final_kernel");
   SourceStats h{};
   t.readback.read(&h, st, sizeof(SourceStats), s);
   return h;
@@ -1374,23 +1426,39 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
     CHUI_LAUNCH_CHECK("root_init_kernel");
     level_dev_init_kernel<<<1, 64, 0, s>>>(lv);
     CHUI_LAUNCH_CHECK("level_dev_init_kernel");
-    for (int d = 0; d < kMaxLevels; ++d) {
-      const LevelDevArgs la{static_cast<uint32_t>(d), P.leaf_size, static_cast<uint32_t>(cap),
-                            static_cast<uint32_t>(partial ? kPartialSortLevels : kMaxLevels)};
-      level_count_dev_kernel<<<kLevelBlocks, 256, 0, s>>>(
-          la, lv, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
-          t.bfs_geo.as<double4>(), keys, t.bfs_nchild.as<uint32_t>(), t.level_bounds.as<uint32_t>(), bsum);
-      CHUI_LAUNCH_CHECK("level_count_dev_kernel");
-      level_offsets_dev_kernel<<<1, kLevelBlocks / 2, 0, s>>>(la, lv, bsum);
-      CHUI_LAUNCH_CHECK("level_offsets_dev_kernel");
-      level_emit_dev_kernel<<<kLevelBlocks, 256, 0, s>>>(
-          la, lv, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
-          t.bfs_geo.as<double4>(), t.level_bounds.as<uint32_t>(), t.bfs_nchild.as<uint32_t>(), bsum,
-          t.bfs_first_child.as<uint32_t>());
-      CHUI_LAUNCH_CHECK("level_emit_dev_kernel");
-    }
+    auto issue_levels = [&](int d0, int d1) {
+      for (int d = d0; d < d1; ++d) {
+        const LevelDevArgs la{static_cast<uint32_t>(d), P.leaf_size, static_cast<uint32_t>(cap),
+                              static_cast<uint32_t>(partial ? kPartialSortLevels : kMaxLevels)};
+        level_count_dev_kernel<<<kLevelBlocks, 256, 0, s>>>(
+            la, lv, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
+            t.bfs_geo.as<double4>(), keys, t.bfs_nchild.as<uint32_t>(), t.level_bounds.as<uint32_t>(), bsum);
+        CHUI_LAUNCH_CHECK("level_count_dev_kernel");
+        level_offsets_dev_kernel<<<1, kLevelBlocks / 2, 0, s>>>(la, lv, bsum);
+        CHUI_LAUNCH_CHECK("level_offsets_dev_kernel");
+        level_emit_dev_kernel<<<kLevelBlocks, 256, 0, s>>>(
+            la, lv, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
+            t.bfs_geo.as<double4>(), t.level_bounds.as<uint32_t>(), t.bfs_nchild.as<uint32_t>(), bsum,
+            t.bfs_first_child.as<uint32_t>());
+        CHUI_LAUNCH_CHECK("level_emit_dev_kernel");
+      }
+    };
+    // Levels up to the last build's depth + 2 first (an empty level costs
+    // three launches of about 4 us each); the read-back that follows anyway
+    // tells whether the tree went deeper, and only then are the rest issued.
+    int issued = t.level_hint > 0 ? std::min(kMaxLevels, t.level_hint) : kMaxLevels;
+    issue_levels(0, issued);
     uint32_t h[kLvWords];
-    t.readback.read(h, lv, sizeof(h), s);
+    for (;;) {
+      t.readback.read(h, lv, sizeof(h), s);
+      // level `issued` is [h[issued], h[issued + 1]), written by the last pass
+      if (issued < kMaxLevels && !h[kLvOverflow] && !(partial && h[kLvDeepSplit]) && h[issued + 1] > h[issued]) {
+        issue_levels(issued, kMaxLevels);
+        issued = kMaxLevels;
+        continue;
+      }
+      break;
+    }
     if (partial && h[kLvDeepSplit]) {
       // A depth-16 node splits (whether it does depends on its count, not on
       // the unsorted low digits). Sort by every level and start over; the sort
@@ -1413,6 +1481,7 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
     for (int d = 0; d + 1 < kMaxLevels && h[d + 2] > h[d + 1]; ++d) t.level_start.push_back(h[d + 2]);
     t.max_depth = static_cast<uint32_t>(t.level_start.size() - 2);
     t.leaf_count = h[kLvLeaves];
+    t.level_hint = static_cast<int>(t.max_depth) + 3;
     break;
   }
   const std::size_t total = t.level_start.back();

@@ -56,6 +56,7 @@ struct Tree {
   std::size_t node_count = 0;
   std::size_t leaf_count = 0;
   uint32_t max_depth = 0;
+  int level_hint = 0;  // levels the next build issues before its first read-back (0: all)
   UnwrapResult unwrap{};
   bool mass_uniform = false;
   double uniform_mass = 0.0;
