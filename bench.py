@@ -570,20 +570,48 @@ def measure_ensemble(steps, gradients, workers=8):
     particles = sum(s.n for s in specs)
     out = {"boxes": len(specs), "particles": particles, "kdk_steps": steps, "precision": "mixed",
            "autotuned_leaf": tuned, "meshes": "2 n_side per axis (64, 128, 192, 256)"}
+    def timed(fn, start, fwd, repeats=3):
+        els = []
+        for _ in range(repeats):
+            fwd.set_state(start)
+            t = time.perf_counter()
+            fn()
+            fwd.synchronize()
+            els.append(time.perf_counter() - t)
+        return float(np.median(els)), els
+
     for workers_ in (workers, 1):
         with E.Ensemble(specs, workers=workers_) as ens:
             fwd = E.EnsembleForward(ens, steps, L.PRECISION_MIXED, leaves)
+            start = fwd.state()
             ens.solve(fwd.x, fwd.m, fwd.g, fwd.options)  # warm-up: sizes every box's buffers
-            for v in ens.views:
-                v.synchronize()
-            t = time.perf_counter()
-            fwd.run()
-            el = time.perf_counter() - t
-        key = "forward" if workers_ == workers else "forward_one_worker"
-        out[key] = {"workers": workers_, "seconds": el, "force_evaluations_per_box": steps + 1,
-                    "value": particles * (steps + 1) / el, "unit": "particle force evaluations/s"}
-        log(f"ensemble forward with {workers_} workers: {el:.3f} s")
+            fwd.synchronize()
+            el, els = timed(fwd.run, start, fwd)
+            key = "forward" if workers_ == workers else "forward_one_worker"
+            out[key] = {"workers": workers_, "seconds": el, "seconds_each": els,
+                        "force_evaluations_per_box": steps + 1,
+                        "value": particles * (steps + 1) / el, "unit": "particle force evaluations/s"}
+            log(f"ensemble forward with {workers_} workers: {el:.3f} s (median of {len(els)})")
+            if workers_ == workers:
+                # the whole forward of all 48 boxes recorded as one HIP graph
+                # (capture-safe solves, each box a branch), replayed
+                fwd.set_state(start)
+                t = time.perf_counter()
+                graph = fwd.capture()
+                t_capture = time.perf_counter() - t
+                try:
+                    el_g, els_g = timed(graph.launch, start, fwd)
+                finally:
+                    graph.close()
+                out["forward_graph"] = {"seconds": el_g, "seconds_each": els_g, "capture_seconds": t_capture,
+                                        "force_evaluations_per_box": steps + 1,
+                                        "value": particles * (steps + 1) / el_g,
+                                        "unit": "particle force evaluations/s",
+                                        "what": "one hipGraphLaunch of the recorded forward (initial forces, "
+                                                f"{steps} KDK steps of every box), median of {len(els_g)}"}
+                log(f"ensemble forward as one graph: {el_g:.3f} s (capture {t_capture:.2f} s)")
     out["worker_speedup"] = out["forward"]["value"] / out["forward_one_worker"]["value"]
+    out["graph_speedup"] = out["forward_graph"]["value"] / out["forward"]["value"]
     if gradients:
         t = time.perf_counter()
         nf = nv = 0

@@ -231,6 +231,14 @@ SIGNATURES = {
     "chui_treepm_ensemble_destroy": [_P],
     "chui_treepm_ensemble_box": [_P, _I32, C.POINTER(_P)],
     "chui_treepm_ensemble_solve_device": [_P, C.POINTER(Box), _I32, _I32],
+    "chui_treepm_ensemble_set_capture_safe": [_P, _I32],
+    "chui_treepm_ensemble_graph_begin": [_P],
+    "chui_treepm_ensemble_graph_end": [_P, C.POINTER(_P)],
+    "chui_treepm_set_capture_safe": [_P, _I32],
+    "chui_treepm_graph_begin": [_P],
+    "chui_treepm_graph_end": [_P, C.POINTER(_P)],
+    "chui_treepm_graph_launch": [_P],
+    "chui_treepm_graph_destroy": [_P],
     "chui_treepm_profiling": [_P, _I32],
     "chui_treepm_profile_read": [_P, C.POINTER(_D), C.POINTER(_U64)],
     "chui_treepm_kdk_factors": [_D, _D, _D, _D, _D, C.POINTER(KdkFactors)],

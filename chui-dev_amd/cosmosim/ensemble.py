@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib, adjoint, ics
-from .treepm import TreePmCoordinator, kdk_factors
+from .treepm import Graph, TreePmCoordinator, kdk_factors
 
 
 @dataclass
@@ -113,6 +113,20 @@ class Ensemble:
                                 ax.data_ptr(), ay.data_ptr(), az.data_ptr(), C.pointer(options[b]))
         _lib.check(self._lib.chui_treepm_ensemble_solve_device(self._h, boxes, len(self.specs), 1 if refresh else 0))
 
+    def set_capture_safe(self, enable: bool = True):
+        """Capture-safe solves on every box (chui_treepm_ensemble_set_capture_safe)."""
+        _lib.check(self._lib.chui_treepm_ensemble_set_capture_safe(self._h, 1 if enable else 0))
+
+    def graph_begin(self):
+        """Record every box's stream into one HIP graph (box 0's stream records,
+        the others join it as branches)."""
+        _lib.check(self._lib.chui_treepm_ensemble_graph_begin(self._h))
+
+    def graph_end(self) -> Graph:
+        out = C.c_void_p()
+        _lib.check(self._lib.chui_treepm_ensemble_graph_end(self._h, C.byref(out)))
+        return Graph(out)
+
 
 def box_options(spec: BoxSpec, precision: int, leaf: int = 16):
     spacing = spec.box / spec.n_side
@@ -184,8 +198,9 @@ class EnsembleForward:
     def particles(self) -> int:
         return sum(s.n for s in self.ens.specs)
 
-    def run(self):
-        """Initial forces and every KDK step; returns when all boxes are done."""
+    def issue(self):
+        """Initial forces and every KDK step, issued on the boxes' streams
+        without waiting (recordable into a graph)."""
         ens = self.ens
         ens.solve(self.x, self.m, self.g, self.options)
         for k in range(self.steps):
@@ -198,8 +213,45 @@ class EnsembleForward:
             for b in range(len(ens.specs)):
                 f = self.factors[b][k]
                 ens.views[b].kick(*self.v[b], *self.g[b], f["second_kick_factor"], f["second_hubble_drag"])
-        for v in ens.views:
+
+    def run(self):
+        """Initial forces and every KDK step; returns when all boxes are done."""
+        self.issue()
+        self.synchronize()
+
+    def synchronize(self):
+        for v in self.ens.views:
             v.synchronize()
+
+    def state(self):
+        """Copies of the positions and velocities of every box."""
+        return [tuple(t.clone() for t in self.x[b]) + tuple(t.clone() for t in self.v[b])
+                for b in range(len(self.ens.specs))]
+
+    def set_state(self, state):
+        """Positions and velocities back to a state() copy (in place, so a
+        recorded graph sees them)."""
+        for b, st in enumerate(state):
+            for dst, src in zip(self.x[b] + self.v[b], st):
+                dst.copy_(src)
+        torch.cuda.synchronize(self.device)
+
+    def capture(self) -> Graph:
+        """The whole forward (initial forces and every KDK step of every box)
+        as one HIP graph. One eager forward first sizes every box's context
+        for capture-safe solves (the state is restored afterwards); each
+        launch of the graph then advances the boxes from their current state
+        by the same steps, and equals run() bit for bit."""
+        start = self.state()
+        self.ens.set_capture_safe(True)
+        self.run()
+        self.set_state(start)
+        self.ens.graph_begin()
+        try:
+            self.issue()
+        finally:
+            graph = self.ens.graph_end()
+        return graph
 
 
 def box_gradient(spec: BoxSpec, steps: int, device: str = "cuda:0", checkpoint_every: int = 2):

@@ -45,6 +45,30 @@ class SolveResult:
     diagnostics: dict = field(default_factory=dict)
 
 
+class Graph:
+    """A recorded HIP graph of capture-safe solves (chui_treepm_graph)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._lib = _lib.load()
+        self._h = handle
+
+    def launch(self):
+        """Replay on the recording stream after the members' earlier work; the
+        members' later work (and synchronize) waits for it."""
+        _lib.check(self._lib.chui_treepm_graph_launch(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.check(self._lib.chui_treepm_graph_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class TreePmCoordinator:
     def __init__(self, pm_shape, device: int = 0):
         lib = _lib.load()
@@ -95,6 +119,21 @@ class TreePmCoordinator:
 
     def set_stream(self, stream_handle: int | None):
         _lib.check(self._lib.chui_treepm_set_stream(self._h, C.c_void_p(stream_handle) if stream_handle else None))
+
+    # -- capture-safe solves and HIP graphs (no reference counterpart) ---------
+    def set_capture_safe(self, enable: bool = True):
+        """Capture-safe mode: after one solve of a source count, solves of that
+        count read nothing back and allocate nothing (chui_treepm_set_capture_safe)."""
+        _lib.check(self._lib.chui_treepm_set_capture_safe(self._h, 1 if enable else 0))
+
+    def graph_begin(self):
+        """Start recording this context's stream into a HIP graph."""
+        _lib.check(self._lib.chui_treepm_graph_begin(self._h))
+
+    def graph_end(self) -> "Graph":
+        out = C.c_void_p()
+        _lib.check(self._lib.chui_treepm_graph_end(self._h, C.byref(out)))
+        return Graph(out)
 
     def solve_active_set_with_pm_cadence(self, pos_x, pos_y, pos_z, mass, accumulator: ForceAccumulator,
                                          options: _lib.Options, refresh_long_range_field: bool = True,

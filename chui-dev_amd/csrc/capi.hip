@@ -85,6 +85,12 @@ struct chui_treepm_context {
   uint64_t* fault_host = nullptr;
   hipEvent_t fault_ev = nullptr;
   bool fault_pending = false;
+  // Capture-safe solves (chui_treepm_set_capture_safe): mixed strict solves
+  // that read nothing back and allocate nothing once a solve of the same
+  // sources has sized the context, so that they can be recorded into a HIP
+  // graph; `capturing` while the context's stream records one.
+  bool capture_safe = false;
+  bool capturing = false;
 };
 
 namespace {
@@ -264,6 +270,13 @@ __global__ void target_keys_kernel(const uint32_t* __restrict__ active, std::siz
   vals[s] = static_cast<uint32_t>(s);
 }
 
+// A capture-safe solve turns a device flag into fault bits instead of reading
+// it back.
+__global__ void flag_fault_kernel(const uint32_t* __restrict__ flag, unsigned long long* __restrict__ fault,
+                                  unsigned long long bits) {
+  if (threadIdx.x == 0 && *flag) atomicOr(fault, bits);
+}
+
 template <class T>
 __global__ void to_double_kernel(const T* __restrict__ in, double* __restrict__ out, std::size_t n) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -320,9 +333,24 @@ void check_fault(Ctx* c, bool wait) {
     CHUI_HIP(q);
   }
   c->fault_pending = false;
-  if (*c->fault_host != 0) {
+  const uint64_t f = *c->fault_host;
+  if (f != 0) {
     *c->fault_host = 0;
     chui::fill_u64(c->counters.as<uint64_t>() + chui::kFaultSlot, 0ull, 1, c->stream);
+    // capture-safe solves report here what an ordinary solve throws at once
+    if (f & chui::kFaultCoordinates)
+      throw std::invalid_argument("Periodic TreePM tree geometry requires finite source coordinates");
+    if (f & chui::kFaultMasses)
+      throw std::invalid_argument("Tree gravity requires finite coordinates and finite non-negative masses");
+    if (f & chui::kFaultActive) throw std::invalid_argument("TreePM active particle index out of range");
+    if (f & chui::kFaultTooDeep)
+      throw chui::UnsupportedError(
+          "octree deeper than 21 levels (more than max_leaf_size coincident or near-coincident particles) "
+          "is not built by the GPU path yet");
+    if (f & chui::kFaultPrepared)
+      throw std::runtime_error(
+          "capture-safe solve: the sources outgrew the prepared context (node capacity, tree depth or mass "
+          "uniformity); run an ordinary solve of the new sources and capture again");
     throw std::logic_error("TreePM walk lost progress (wave reduction selected no pending node)");
   }
 }
@@ -491,14 +519,32 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   // tree_pm_coupling.cpp:2508-2511
   if (sr_only && (refresh || pmx || pmy || pmz))
     throw std::invalid_argument("short-range split surface cannot request a PM refresh/capture or zoom correction");
+  // Capture-safe mode: the first solve of a source count is an ordinary one
+  // that sizes the context for it (and prepares the node capacity, depth and
+  // mass uniformity); the solves after it read nothing back. Only those can be
+  // recorded into a graph.
+  const bool safe_mode = c->capture_safe;
+  const bool safe = safe_mode && n > 0 && c->tree.prepared_n == n && c->tree.sized_for_capture;
+  if (c->capturing && !safe)
+    throw std::runtime_error(
+        "graph capture of a solve on a context not prepared for these sources: run one capture-safe solve of "
+        "them before capturing");
+  if (safe_mode && (!mixed || adaptive_policy || diag || wcount || near2 || prev))
+    throw chui::UnsupportedError(
+        "capture-safe solves are mixed-precision strict solves without diagnostics, counters, the nearest-pair "
+        "lane or an acceleration history");
   hipStream_t s = c->stream;
-  harvest(c);
-  check_fault(c, false);
-  c->events = diag != nullptr || c->profiling;
+  if (!c->capturing) {
+    harvest(c);
+    check_fault(c, false);
+  }
+  c->events = !c->capturing && (diag != nullptr || c->profiling);
   if (c->events && !c->ev[0])
     for (auto& e : c->ev) CHUI_HIP(hipEventCreate(&e));
   record(c, kEvStart);
   const Split sp = split_of(*o, c->mesh);
+  // the counters' fault word takes a capture-safe build's fault bits
+  unsigned long long* const walk_cnt = walk_counters(c, s);
 
   // --- tree ---
   chui::TreeParams tp;
@@ -507,6 +553,9 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   tp.leaf_size = static_cast<uint32_t>(o->max_leaf_size);
   tp.quadrupole = o->multipole_order == 2;
   tp.mixed = mixed;
+  tp.capture_safe = safe;
+  tp.size_for_capture = safe_mode && !safe;
+  tp.fault = walk_cnt + chui::kFaultSlot;
   chui::build_tree(c->tree, x, y, z, m, tp, o->softening_epsilon, s);
   record(c, kEvTree);
   const bool uniform = n > 0 && c->tree.mass_uniform;
@@ -601,9 +650,14 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
                                                                      c->tpos.as<uint32_t>(), c->tslot.as<uint32_t>(),
                                                                      c->flag.as<uint32_t>());
     CHUI_LAUNCH_CHECK("target_keys_kernel");
-    uint32_t bad = 0;
-    c->tree.readback.read(&bad, c->flag.ptr, 4, s);
-    if (bad) throw std::invalid_argument("TreePM active particle index out of range");
+    if (safe) {
+      flag_fault_kernel<<<1, 64, 0, s>>>(c->flag.as<uint32_t>(), walk_cnt + chui::kFaultSlot, chui::kFaultActive);
+      CHUI_LAUNCH_CHECK("flag_fault_kernel");
+    } else {
+      uint32_t bad = 0;
+      c->tree.readback.read(&bad, c->flag.ptr, 4, s);
+      if (bad) throw std::invalid_argument("TreePM active particle index out of range");
+    }
     // Visit targets in tree order (tree_pm_coupling.cpp:3801-3838).
     chui::radix_sort_pairs32(c->tpos.as<uint32_t>(), c->tslot.as<uint32_t>(), c->tkeys_alt.as<uint32_t>(),
                              c->tvals_alt.as<uint32_t>(), n_active, 0, 32, c->tsort_scratch.ptr, s);
@@ -614,7 +668,6 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   record(c, kEvTargets);
 
   // --- walk (+ fused PM interpolation) ---
-  unsigned long long* const walk_cnt = walk_counters(c, s);
   chui::WalkParams wp = base_walk_params(c, o, sp, g.row);
   wp.n_targets = static_cast<uint32_t>(n_active);
   wp.identity_targets = identity;
@@ -757,6 +810,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       wp.tab_to_len2 = static_cast<float>(1.0 / t2c);
       wp.accept_half_max = static_cast<float>(0.041 * sp.rcut);
       if (c->htab_key[0] != t_max || c->htab_key[1] != sp.a || !c->htab.ptr) {
+        chui::require_not_capturing(s, "the pair table");
         c->htab.reserve(tab.size() * sizeof(float));
         CHUI_HIP(hipMemcpyAsync(c->htab.ptr, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, s));
         CHUI_HIP(hipStreamSynchronize(s));
@@ -777,6 +831,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
           c->ltab_cells = cells;
           c->ltab_first_octave = e0;
           if (cells > 0) {
+            chui::require_not_capturing(s, "the residual table");
             c->ltab.reserve(c->ltab_host.size() * sizeof(float));
             CHUI_HIP(hipMemcpyAsync(c->ltab.ptr, c->ltab_host.data(), c->ltab_host.size() * sizeof(float),
                                     hipMemcpyHostToDevice, s));
@@ -801,7 +856,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     }
   }
   record(c, kEvWalk);
-  post_fault_word(c, s);
+  if (!c->capturing) post_fault_word(c, s);  // a graph launch posts it after the replay
   if (wcount || diag) check_fault(c, true);
   if (wcount) {
     unsigned long long wc[chui::kWalkCounters + 1] = {};
@@ -1066,7 +1121,77 @@ struct chui_treepm_ensemble {
   int device = 0;
   int workers = 1;
   std::vector<chui_treepm_context*> boxes;
+  // graph capture: box 0's stream records, the other boxes' streams join it
+  bool capturing = false;
+  hipEvent_t fork_ev = nullptr;
+  std::vector<hipEvent_t> join_ev;
 };
+
+// A recorded graph of capture-safe solves (and whatever else the caller issued
+// on the member contexts' streams while it was recorded). Launches replay it on
+// the stream it was recorded on, after the members' prior work, and order the
+// members' later work after it.
+struct chui_treepm_graph {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t origin = nullptr;
+  std::vector<chui_treepm_context*> members;
+  std::vector<hipEvent_t> in_ev;  // one per member stream other than origin
+  hipEvent_t out_ev = nullptr;
+  ~chui_treepm_graph() {
+    for (auto e : in_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (out_ev) (void)hipEventDestroy(out_ev);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+
+// Waits for the context's work, reports its faults and checks that a recorded
+// solve would allocate nothing: the walk counters and the fault word's pinned
+// copy exist, and the context is in capture-safe mode.
+void ready_for_capture(Ctx* c) {
+  if (!c->capture_safe)
+    throw std::invalid_argument("graph capture needs capture-safe solves (chui_treepm_set_capture_safe)");
+  if (c->capturing) throw std::logic_error("a graph is already being recorded on this context");
+  CHUI_HIP(hipStreamSynchronize(c->stream));
+  check_fault(c, true);
+  (void)walk_counters(c, c->stream);
+  post_fault_word(c, c->stream);
+  CHUI_HIP(hipStreamSynchronize(c->stream));
+  check_fault(c, true);
+}
+
+chui_treepm_graph* finish_capture(hipStream_t origin, int device, std::vector<Ctx*> members) {
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(origin, &g);
+  for (auto* c : members) c->capturing = false;
+  CHUI_HIP(e);
+  auto* gr = new chui_treepm_graph();
+  gr->graph = g;
+  gr->device = device;
+  gr->origin = origin;
+  gr->members = std::move(members);
+  try {
+    CHUI_HIP(hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0));
+    for (auto* c : gr->members)
+      if (c->stream != origin) {
+        hipEvent_t ev = nullptr;
+        CHUI_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        gr->in_ev.push_back(ev);
+      }
+    CHUI_HIP(hipEventCreateWithFlags(&gr->out_ev, hipEventDisableTiming));
+  } catch (...) {
+    delete gr;
+    throw;
+  }
+  return gr;
+}
+
+}  // namespace
 
 chui_status chui_treepm_ensemble_create(int32_t box_count, const int32_t* meshes, int32_t device, int32_t workers,
                                         chui_treepm_ensemble** out) {
@@ -1116,6 +1241,73 @@ chui_status chui_treepm_ensemble_box(chui_treepm_ensemble* e, int32_t index, chu
   });
 }
 
+chui_status chui_treepm_ensemble_set_capture_safe(chui_treepm_ensemble* e, int32_t enable) {
+  return guarded([&] {
+    if (!e) throw std::invalid_argument("null argument");
+    if (e->capturing) throw std::logic_error("capture-safe mode cannot change while a graph is recorded");
+    for (auto* c : e->boxes) c->capture_safe = enable != 0;
+  });
+}
+
+// Records on box 0's stream; every other box's stream joins the capture by
+// waiting on an event of it, so each box's work forms its own branch of the
+// graph and the boxes still overlap on the device.
+chui_status chui_treepm_ensemble_graph_begin(chui_treepm_ensemble* e) {
+  return guarded([&] {
+    if (!e) throw std::invalid_argument("null argument");
+    if (e->capturing) throw std::logic_error("a graph is already being recorded on this ensemble");
+    CHUI_HIP(hipSetDevice(e->device));
+    for (auto* c : e->boxes) ready_for_capture(c);
+    if (!e->fork_ev) CHUI_HIP(hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming));
+    while (e->join_ev.size() < e->boxes.size()) {
+      hipEvent_t ev = nullptr;
+      CHUI_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      e->join_ev.push_back(ev);
+    }
+    hipStream_t origin = e->boxes[0]->stream;
+    CHUI_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+    e->capturing = true;
+    for (auto* c : e->boxes) c->capturing = true;
+    try {
+      CHUI_HIP(hipEventRecord(e->fork_ev, origin));
+      for (auto* c : e->boxes)
+        if (c->stream != origin) CHUI_HIP(hipStreamWaitEvent(c->stream, e->fork_ev, 0));
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(origin, &g);
+      if (g) (void)hipGraphDestroy(g);
+      e->capturing = false;
+      for (auto* c : e->boxes) c->capturing = false;
+      throw;
+    }
+  });
+}
+
+chui_status chui_treepm_ensemble_graph_end(chui_treepm_ensemble* e, chui_treepm_graph** out) {
+  return guarded([&] {
+    if (!e || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    if (!e->capturing) throw std::logic_error("no graph is being recorded on this ensemble");
+    CHUI_HIP(hipSetDevice(e->device));
+    hipStream_t origin = e->boxes[0]->stream;
+    e->capturing = false;
+    std::size_t k = 0;
+    hipError_t joined = hipSuccess;
+    for (auto* c : e->boxes) {
+      if (c->stream == origin) continue;
+      if (joined == hipSuccess) joined = hipEventRecord(e->join_ev[k], c->stream);
+      if (joined == hipSuccess) joined = hipStreamWaitEvent(origin, e->join_ev[k], 0);
+      ++k;
+    }
+    chui_treepm_graph* g = finish_capture(origin, e->device, e->boxes);
+    if (joined != hipSuccess) {
+      delete g;
+      CHUI_HIP(joined);
+    }
+    *out = g;
+  });
+}
+
 chui_status chui_treepm_ensemble_solve_device(chui_treepm_ensemble* e, const chui_treepm_box* boxes,
                                               int32_t box_count, int32_t refresh_long_range) {
   return guarded([&] {
@@ -1137,6 +1329,17 @@ chui_status chui_treepm_ensemble_solve_device(chui_treepm_ensemble* e, const chu
     std::atomic<bool> failed{false};
     std::vector<chui_status> status(order.size(), CHUI_OK);
     std::vector<std::string> message(order.size());
+    if (e->capturing) {
+      // recording: issue every box from this thread, nothing to wait for
+      for (int b : order) {
+        const chui_treepm_box& bx = boxes[b];
+        if (!bx.options) throw std::invalid_argument("ensemble box options are null");
+        solve_device_impl(e->boxes[b], bx.pos_x, bx.pos_y, bx.pos_z, bx.mass, bx.n, nullptr, bx.n, bx.accel_x,
+                          bx.accel_y, bx.accel_z, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, bx.options,
+                          refresh_long_range, nullptr);
+      }
+      return;
+    }
     auto work = [&]() {
       (void)hipSetDevice(e->device);
       for (;;) {
@@ -1202,6 +1405,67 @@ chui_status chui_treepm_synchronize(chui_treepm_context* c) {
     set_device(c);
     CHUI_HIP(hipStreamSynchronize(c->stream));
     check_fault(c, true);
+  });
+}
+
+// ---- capture-safe solves and HIP graphs ------------------------------------
+chui_status chui_treepm_set_capture_safe(chui_treepm_context* c, int32_t enable) {
+  return guarded([&] {
+    require_ctx(c);
+    if (c->capturing) throw std::logic_error("capture-safe mode cannot change while a graph is recorded");
+    c->capture_safe = enable != 0;
+  });
+}
+
+
+chui_status chui_treepm_graph_begin(chui_treepm_context* c) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    ready_for_capture(c);
+    CHUI_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+  });
+}
+
+chui_status chui_treepm_graph_end(chui_treepm_context* c, chui_treepm_graph** out) {
+  return guarded([&] {
+    require_ctx(c);
+    if (!out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    if (!c->capturing) throw std::logic_error("no graph is being recorded on this context");
+    set_device(c);
+    *out = finish_capture(c->stream, c->device, {c});
+  });
+}
+
+chui_status chui_treepm_graph_launch(chui_treepm_graph* g) {
+  return guarded([&] {
+    if (!g || !g->exec) throw std::invalid_argument("null graph");
+    CHUI_HIP(hipSetDevice(g->device));
+    std::size_t k = 0;
+    for (auto* c : g->members) {
+      if (c->capturing) throw std::logic_error("graph launch while a member context records a graph");
+      if (c->stream == g->origin) continue;
+      CHUI_HIP(hipEventRecord(g->in_ev[k], c->stream));
+      CHUI_HIP(hipStreamWaitEvent(g->origin, g->in_ev[k], 0));
+      ++k;
+    }
+    CHUI_HIP(hipGraphLaunch(g->exec, g->origin));
+    // each member's fault word after the replay, checked at its next synchronisation
+    for (auto* c : g->members) post_fault_word(c, g->origin);
+    CHUI_HIP(hipEventRecord(g->out_ev, g->origin));
+    for (auto* c : g->members)
+      if (c->stream != g->origin) CHUI_HIP(hipStreamWaitEvent(c->stream, g->out_ev, 0));
+  });
+}
+
+chui_status chui_treepm_graph_destroy(chui_treepm_graph* g) {
+  return guarded([&] {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    (void)hipStreamSynchronize(g->origin);
+    delete g;
   });
 }
 
@@ -1663,6 +1927,8 @@ chui_status chui_treepm_morton_device(chui_treepm_context* c, const double* x, c
 chui_status chui_treepm_tree_size(chui_treepm_context* c, uint64_t* nodes, uint64_t* particles) {
   return guarded([&] {
     require_ctx(c);
+    if (c->tree.counts_on_device)
+      throw std::logic_error("the last tree was built by a capture-safe solve; its node count stays on the device");
     if (nodes) *nodes = c->tree.node_count;
     if (particles) *particles = c->tree.n;
   });
@@ -1673,6 +1939,8 @@ chui_status chui_treepm_tree_copy(chui_treepm_context* c, double* geometry, doub
   return guarded([&] {
     require_ctx(c);
     set_device(c);
+    if (c->tree.counts_on_device)
+      throw std::logic_error("the last tree was built by a capture-safe solve; its node count stays on the device");
     const std::size_t nn = c->tree.node_count;
     hipStream_t s = c->stream;
     if (nn > 0) {

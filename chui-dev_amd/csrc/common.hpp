@@ -55,6 +55,18 @@ struct DeviceBuffer {
   template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Uploads of host tables synchronise the stream, which a stream recording a
+// HIP graph does not allow: a capture-safe solve must find its tables built by
+// an earlier solve with the same mesh and options.
+inline void require_not_capturing(hipStream_t s, const char* what) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  CHUI_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    throw std::logic_error(std::string(what) +
+                           " would be uploaded while a graph is recorded: run one solve with the same mesh and "
+                           "options before capturing");
+}
+
 // Small device-to-host readbacks (counts, flags, bounds) through a pinned host
 // block: the copy lands directly in host memory instead of going through the
 // runtime's staging pass for pageable destinations.

@@ -10,6 +10,7 @@
 // constant 1/(cell volume * M) of the reference's density normalisation and
 // inverse FFT is folded into the Green's function.
 #include "pm.hpp"
+#include "primitives.hpp"
 
 #include <mutex>
 
@@ -407,62 +408,70 @@ __global__ __launch_bounds__(256) void deposit_kernel(DepositArgs a, const S* __
                                                       DepositUnits du, T* __restrict__ rho) {
   __shared__ T tile[kDepositTileBytes / sizeof(T)];
   __shared__ uint32_t s_unit;
-  if (threadIdx.x == 0) {
-    uint32_t lo = 0, hi = du.count - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (du.first_block[mid] <= blockIdx.x) lo = mid;
-      else hi = mid - 1;
-    }
-    s_unit = lo;
-  }
-  __syncthreads();
-  const uint32_t ui = s_unit;
-  const DepositUnit u = du.units[ui];
-  const uint32_t p0 = u.begin + (blockIdx.x - du.first_block[ui]) * kDepositChunk;
-  const uint32_t p1 = min(p0 + kDepositChunk, u.begin + u.count);
-  const int cnt = a.scheme == 0 ? 2 : 3;
-  const bool tiled = u.dims[0] > 0;
-  const int dy = u.dims[1], dz = u.dims[2];
-  const int vol = tiled ? u.dims[0] * dy * dz : 0;
-  for (int i = threadIdx.x; i < vol; i += blockDim.x) tile[i] = T(0);
-  __syncthreads();
-  for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const S sp = src[p];
-    int idx[3][3];
-    T w[3][3];
-    stencil(a, sp, idx, w);
-    const T mm = uniform ? um : source_mass(sp);
-    int o[3];
-    bool in = tiled;
-    for (int ax = 0; ax < 3; ++ax) {
-      o[ax] = wrap_once(idx[ax][0] - u.lo[ax], a.n[ax]);
-      in = in && o[ax] + cnt <= u.dims[ax];
-    }
-    for (int i = 0; i < cnt; ++i)
-      for (int j = 0; j < cnt; ++j) {
-        const T mij = mm * w[0][i] * w[1][j];
-        if (in) {
-          T* t = &tile[((o[0] + i) * dy + (o[1] + j)) * dz + o[2]];
-          for (int k = 0; k < cnt; ++k) atomicAdd(t + k, mij * w[2][k]);
-        } else {
-          const std::size_t rb = (static_cast<std::size_t>(idx[0][i]) * a.n[1] + idx[1][j]) * a.row;
-          for (int k = 0; k < cnt; ++k) atomicAdd(&rho[rb + idx[2][k]], mij * w[2][k]);
-        }
+  // Block vb of the unit list; the grid covers every block (one iteration per
+  // workgroup) or, for a capture-safe solve, strides over the block count
+  // read on the device.
+  const uint32_t nunits = du.dev_counts ? du.dev_counts[0] : du.count;
+  const uint32_t nblocks = du.dev_counts ? du.dev_counts[1] : du.blocks;
+  for (uint32_t vb = blockIdx.x; vb < nblocks; vb += gridDim.x) {
+    __syncthreads();  // the previous block's flush is done with tile and s_unit
+    if (threadIdx.x == 0) {
+      uint32_t lo = 0, hi = nunits - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (du.first_block[mid] <= vb) lo = mid;
+        else hi = mid - 1;
       }
-  }
-  if (!tiled) return;  // block-uniform
-  __syncthreads();
-  const int rows = u.dims[0] * dy;
-  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  for (int r = wv; r < rows; r += blockDim.x / kWave) {
-    const int ix = r / dy, iy = r - ix * dy;
-    const std::size_t base =
-        (static_cast<std::size_t>(wrap_once(u.lo[0] + ix, a.n[0])) * a.n[1] + wrap_once(u.lo[1] + iy, a.n[1])) *
-        a.row;
-    for (int iz = lane; iz < dz; iz += kWave) {
-      const T v = tile[r * dz + iz];
-      if (v != T(0)) atomicAdd(&rho[base + wrap_once(u.lo[2] + iz, a.n[2])], v);
+      s_unit = lo;
+    }
+    __syncthreads();
+    const uint32_t ui = s_unit;
+    const DepositUnit u = du.units[ui];
+    const uint32_t p0 = u.begin + (vb - du.first_block[ui]) * kDepositChunk;
+    const uint32_t p1 = min(p0 + kDepositChunk, u.begin + u.count);
+    const int cnt = a.scheme == 0 ? 2 : 3;
+    const bool tiled = u.dims[0] > 0;
+    const int dy = u.dims[1], dz = u.dims[2];
+    const int vol = tiled ? u.dims[0] * dy * dz : 0;
+    for (int i = threadIdx.x; i < vol; i += blockDim.x) tile[i] = T(0);
+    __syncthreads();
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+      const S sp = src[p];
+      int idx[3][3];
+      T w[3][3];
+      stencil(a, sp, idx, w);
+      const T mm = uniform ? um : source_mass(sp);
+      int o[3];
+      bool in = tiled;
+      for (int ax = 0; ax < 3; ++ax) {
+        o[ax] = wrap_once(idx[ax][0] - u.lo[ax], a.n[ax]);
+        in = in && o[ax] + cnt <= u.dims[ax];
+      }
+      for (int i = 0; i < cnt; ++i)
+        for (int j = 0; j < cnt; ++j) {
+          const T mij = mm * w[0][i] * w[1][j];
+          if (in) {
+            T* t = &tile[((o[0] + i) * dy + (o[1] + j)) * dz + o[2]];
+            for (int k = 0; k < cnt; ++k) atomicAdd(t + k, mij * w[2][k]);
+          } else {
+            const std::size_t rb = (static_cast<std::size_t>(idx[0][i]) * a.n[1] + idx[1][j]) * a.row;
+            for (int k = 0; k < cnt; ++k) atomicAdd(&rho[rb + idx[2][k]], mij * w[2][k]);
+          }
+        }
+    }
+    if (!tiled) continue;  // block-uniform
+    __syncthreads();
+    const int rows = u.dims[0] * dy;
+    const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    for (int r = wv; r < rows; r += blockDim.x / kWave) {
+      const int ix = r / dy, iy = r - ix * dy;
+      const std::size_t base =
+          (static_cast<std::size_t>(wrap_once(u.lo[0] + ix, a.n[0])) * a.n[1] + wrap_once(u.lo[1] + iy, a.n[1])) *
+          a.row;
+      for (int iz = lane; iz < dz; iz += kWave) {
+        const T v = tile[r * dz + iz];
+        if (v != T(0)) atomicAdd(&rho[base + wrap_once(u.lo[2] + iz, a.n[2])], v);
+      }
     }
   }
 }
@@ -692,6 +701,16 @@ Tables<T> upload_tables(PmWorkspace& w, const PmGeometry& g, const PmOptions& o,
   using V = typename C2<T>::type;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const std::size_t nzc = g.nzc;
+  const double key[8] = {static_cast<double>(nx), static_cast<double>(ny), static_cast<double>(nz), g.L[0], g.L[1],
+                         g.L[2], static_cast<double>(o.assignment), static_cast<double>(o.deconvolution)};
+  auto tables_at = [&](const std::size_t* off) {
+    T* base = w.tables.as<T>();
+    V* tb = reinterpret_cast<V*>(base + off[9]);
+    return Tables<T>{base + off[0], base + off[1], base + off[2], base + off[3], base + off[4], base + off[5],
+                     base + off[6], base + off[7], base + off[8], tb + off[10], tb + off[10] + nx,
+                     tb + off[10] + nx + ny};
+  };
+  if (w.tables.ptr && std::equal(key, key + 8, w.table_key)) return tables_at(w.table_off);
   std::vector<T> host;
   auto push_k = [&](int n, std::size_t count, double L, bool deriv, bool window, int wexp) {
     const double dk = 2.0 * kPi / L;
@@ -740,15 +759,18 @@ Tables<T> upload_tables(PmWorkspace& w, const PmGeometry& g, const PmOptions& o,
     }
   }
   const std::size_t bytes = scal * sizeof(T) + tws.size() * sizeof(V);
+  require_not_capturing(s, "PM tables");
   w.tables.reserve(bytes);
   CHUI_HIP(hipMemcpyAsync(w.tables.ptr, host.data(), scal * sizeof(T), hipMemcpyHostToDevice, s));
   CHUI_HIP(hipMemcpyAsync(static_cast<char*>(w.tables.ptr) + scal * sizeof(T), tws.data(), tws.size() * sizeof(V),
                           hipMemcpyHostToDevice, s));
   CHUI_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
-  T* base = w.tables.as<T>();
-  V* tb = reinterpret_cast<V*>(base + scal);
-  return Tables<T>{base + off[0], base + off[1], base + off[2], base + off[3], base + off[4], base + off[5],
-                   base + off[6], base + off[7], base + off[8], tb + toff[0], tb + toff[1], tb + toff[2]};
+  for (int i = 0; i < 9; ++i) w.table_off[i] = off[i];
+  w.table_off[9] = scal;
+  w.table_off[10] = toff[0];
+  w.table_off[11] = 0;
+  std::copy(key, key + 8, w.table_key);
+  return tables_at(w.table_off);
 }
 
 // Dynamic LDS a line-pass block may use: 80 KB (two blocks per CU, 16 lines
@@ -967,9 +989,9 @@ void pm_solve_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, cons
   reserve<double>(w, g);
   const Tables<double> tb = upload_tables<double>(w, g, o, s);
   double* rho = w.buf[0].as<double>();
-  CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(double), s));
+  fill_f64(rho, 0.0, g.complex_cells() * 2, s);
   if (n > 0) {
-    if (units.count == 0) throw std::logic_error("PM deposit without deposit units");
+    if (units.count == 0 && !units.dev_counts) throw std::logic_error("PM deposit without deposit units");
     deposit_kernel<double, double4><<<units.blocks, 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, units,
                                                                   rho);
     CHUI_LAUNCH_CHECK("deposit_kernel");
@@ -985,7 +1007,7 @@ void pm_adjoint_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, co
   const Tables<double> tb = upload_tables<double>(w, g, o, s);
   for (int a = 0; a < 3; ++a) {
     double* mu = w.buf[a].as<double>();
-    CHUI_HIP(hipMemsetAsync(mu, 0, g.complex_cells() * 2 * sizeof(double), s));
+    fill_f64(mu, 0.0, g.complex_cells() * 2, s);
     if (n > 0) {
       if (units.count == 0) throw std::logic_error("PM adjoint deposit without deposit units");
       deposit_kernel<double, double4><<<units.blocks, 256, 0, s>>>(deposit_args(g, o), comp[a], 0, 0.0, units, mu);
@@ -1018,9 +1040,9 @@ void pm_solve_mixed(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, con
   reserve<float>(w, g);
   const Tables<float> tb = upload_tables<float>(w, g, o, s);
   float* rho = w.buf[0].as<float>();
-  CHUI_HIP(hipMemsetAsync(rho, 0, g.complex_cells() * 2 * sizeof(float), s));
+  fill_f32(rho, 0.0f, g.complex_cells() * 2, s);
   if (n > 0) {
-    if (units.count == 0) throw std::logic_error("PM deposit without deposit units");
+    if (units.count == 0 && !units.dev_counts) throw std::logic_error("PM deposit without deposit units");
     deposit_kernel<float, uint4><<<units.blocks, 256, 0, s>>>(deposit_args(g, o), src, uniform_mass, um, units, rho);
     CHUI_LAUNCH_CHECK("deposit_kernel");
   }

@@ -28,6 +28,10 @@ struct PmOptions {
 struct PmWorkspace {
   DeviceBuffer buf[3];
   DeviceBuffer tables;   // k, derivative-k, window tables + twiddles
+  // the tables' identity (mesh axes, box lengths, assignment, deconvolution)
+  // and element offsets: they are uploaded again only when it changes
+  double table_key[8] = {0, 0, 0, 0, 0, 0, -1, -1};
+  std::size_t table_off[12] = {};
   std::size_t cells = 0;
   int elem_bytes = 0;
   // After a solve: real force meshes (padded stride row) per axis.
@@ -55,6 +59,9 @@ struct DepositUnits {
   const DepositUnit* units = nullptr;
   const uint32_t* first_block = nullptr;
   uint32_t count = 0, blocks = 0;
+  // Capture-safe: {unit count, block count} on the device; the deposit then
+  // launches `blocks` (a bound) blocks that loop over the real block count.
+  const uint32_t* dev_counts = nullptr;
 };
 // What the unit builder needs to know about the mesh.
 struct DepositTiling {
@@ -65,6 +72,7 @@ struct DepositTiling {
   uint32_t chunk;
 };
 constexpr uint32_t kDepositChunk = 2048;
+constexpr uint32_t kDepositSafeGrid = 2048;  // deposit blocks of a capture-safe solve (strided over the real count)
 constexpr uint32_t kDepositTileBytes = 49152;
 
 // Deposit (tree-ordered particles), forward r2c FFT, Green's function with the

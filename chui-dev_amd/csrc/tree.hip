@@ -58,6 +58,7 @@ struct SourceStats {
   uint32_t bad_mass;
   uint32_t mass_nonuniform;
   uint32_t pad;
+  double mass0;  // m[0] (the uniform mass when every mass equals it)
 };
 
 // Each block writes its partial statistics to part[blockIdx.x]; one block then
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(256) void source_stats_kernel(Axes3 p, const double
 }
 
 __global__ __launch_bounds__(256) void stats_final_kernel(const SourceStats* __restrict__ part, uint32_t nb,
-                                                          SourceStats* __restrict__ st) {
+                                                          const double* __restrict__ m, SourceStats* __restrict__ st) {
   uint64_t lo[3] = {kEmpty, kEmpty, kEmpty}, hi[3] = {0, 0, 0};
   uint32_t bc = 0, bm = 0, nu = 0;
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -199,6 +200,7 @@ __global__ __launch_bounds__(256) void stats_final_kernel(const SourceStats* __r
       r.mass_nonuniform |= sw[v].mass_nonuniform;
     }
     r.pad = 0u;
+    r.mass0 = m ? m[0] : 0.0;
     *st = r;
   }
 }
@@ -226,6 +228,20 @@ struct BucketArgs {
   uint64_t W;       // B / 32 bitmap words per axis
 };
 
+// The per-build values that a capture-safe build keeps on the device instead
+// of reading them back: the canonical flags (from the source statistics), the
+// unwrap anchors, the root cube and the in-leaf Morton frame (from the unwrap
+// result). Kernels given a Frame take these fields from it.
+struct Frame {
+  int canonical[3];
+  int pad;
+  double anchor[3];
+  double centre[3];
+  double half;
+  double mlo[3];
+  double mext;
+};
+
 __device__ __forceinline__ uint64_t bucket_of(double v, const BucketArgs& a, int ax) {
   uint64_t b = static_cast<uint64_t>(v * a.scale[ax]);
   return b >= a.B ? a.B - 1 : b;
@@ -236,7 +252,10 @@ __device__ __forceinline__ uint64_t bucket_of(double v, const BucketArgs& a, int
 // atomic per value and axis ran at about 1e10 per second, scattered); a
 // coalesced pass then packs 32 bytes into each bitmap word.
 __global__ __launch_bounds__(256) void occupancy_bytes_kernel(Axes3 p, std::size_t n, BucketArgs a,
-                                                              uint8_t* __restrict__ occb) {
+                                                              uint8_t* __restrict__ occb,
+                                                              const Frame* __restrict__ fr) {
+  if (fr)
+    for (int ax = 0; ax < 3; ++ax) a.canonical[ax] = fr->canonical[ax];
   for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     for (int ax = 0; ax < a.naxes; ++ax)
@@ -264,7 +283,10 @@ __global__ __launch_bounds__(256) void occupancy_pack_kernel(const uint8_t* __re
 __global__ __launch_bounds__(256) void marked_minmax_kernel(Axes3 p, std::size_t n, BucketArgs a,
                                                             const uint32_t* __restrict__ mark,
                                                             const uint32_t* __restrict__ rank_base,
-                                                            uint64_t* __restrict__ cmin, uint64_t* __restrict__ cmax) {
+                                                            uint64_t* __restrict__ cmin, uint64_t* __restrict__ cmax,
+                                                            const Frame* __restrict__ fr) {
+  if (fr)
+    for (int ax = 0; ax < 3; ++ax) a.canonical[ax] = fr->canonical[ax];
   for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     for (int ax = 0; ax < a.naxes; ++ax) {
@@ -574,10 +596,26 @@ struct PathArgs {
 // (double4), so that the tree-order gather reads one 32-byte record per source.
 __global__ __launch_bounds__(256) void path_key_kernel(Axes3 p, const double* __restrict__ m, std::size_t n,
                                                        PathArgs a, uint64_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ vals, double4* __restrict__ packed) {
+                                                       uint32_t* __restrict__ vals, double4* __restrict__ packed,
+                                                       const Frame* __restrict__ fr) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double x = p.p[0][i], y = p.p[1][i], z = p.p[2][i];
+  if (fr) {
+    for (int ax = 0; ax < 3; ++ax) {
+      a.u.canonical[ax] = fr->canonical[ax];
+      a.u.anchor[ax] = fr->anchor[ax];
+      a.c0[ax] = fr->centre[ax];
+    }
+    a.h0 = fr->half;
+  }
+  double x = p.p[0][i], y = p.p[1][i], z = p.p[2][i];
+  if (fr) {
+    // a capture-safe build reports non-finite coordinates through the fault
+    // word and carries on; zeros keep every later cell index in range
+    x = isfinite(x) ? x : 0.0;
+    y = isfinite(y) ? y : 0.0;
+    z = isfinite(z) ? z : 0.0;
+  }
   packed[i] = make_double4(wrap_coord(x, a.u.L[0], a.u.canonical[0]), wrap_coord(y, a.u.L[1], a.u.canonical[1]),
                            wrap_coord(z, a.u.L[2], a.u.canonical[2]), m[i]);
   const double ux = unwrap_coord(x, a.u.L[0], a.u.canonical[0], a.u.anchor[0]);
@@ -623,9 +661,17 @@ __device__ __forceinline__ double unwrap_canonical(double v, double anchor, doub
 }
 
 __global__ __launch_bounds__(256) void leaf_morton_kernel(const double4* __restrict__ pk, std::size_t n, UnwrapArgs u,
-                                                          MortonArgs a, uint64_t* __restrict__ mk) {
+                                                          MortonArgs a, uint64_t* __restrict__ mk,
+                                                          const Frame* __restrict__ fr) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (fr) {
+    for (int ax = 0; ax < 3; ++ax) {
+      u.anchor[ax] = fr->anchor[ax];
+      a.lo[ax] = fr->mlo[ax];
+    }
+    a.extent = fr->mext;
+  }
   const double4 q = pk[i];
   mk[i] = morton_key(unwrap_canonical(q.x, u.anchor[0], u.L[0]), unwrap_canonical(q.y, u.anchor[1], u.L[1]),
                      unwrap_canonical(q.z, u.anchor[2], u.L[2]), a);
@@ -663,7 +709,8 @@ constexpr int kLvTooDeep = kMaxLevels + 2;              // lv[0, kMaxLevels + 2)
 constexpr int kLvOverflow = kLvTooDeep + 1;             // children beyond the node capacity
 constexpr int kLvDeepSplit = kLvTooDeep + 2;            // a node below the partial sort's depth splits
 constexpr int kLvLeaves = kLvTooDeep + 3;               // nodes without children
-constexpr int kLvWords = kLvTooDeep + 4;
+constexpr int kLvTotal = kLvTooDeep + 4;                // node count (capture-safe builds)
+constexpr int kLvWords = kLvTooDeep + 5;
 
 struct LevelDevArgs {
   uint32_t d, leaf_size, capacity, sorted_levels;
@@ -831,50 +878,69 @@ __global__ void level_dev_init_kernel(uint32_t* lv) {
 }
 
 __global__ void root_init_kernel(uint32_t* nb, uint32_t* nc, uint32_t* meta, double4* geo, uint32_t n, double cx,
-                                 double cy, double cz, double h) {
+                                 double cy, double cz, double h, const Frame* __restrict__ fr) {
   nb[0] = 0;
   nc[0] = n;
   meta[0] = 0;
-  geo[0] = make_double4(cx, cy, cz, h);
+  geo[0] = fr ? make_double4(fr->centre[0], fr->centre[1], fr->centre[2], fr->half) : make_double4(cx, cy, cz, h);
+}
+
+// A level's node range: [lb, lb + cnt) from the host, or [lv[d], lv[d + 1])
+// from the device when lv is given.
+struct LevelRange {
+  uint32_t lb, cnt;
+  const uint32_t* lv;
+  int d;
+};
+__device__ __forceinline__ void level_range(const LevelRange& r, uint32_t& lb, uint32_t& cnt) {
+  if (r.lv) {
+    lb = r.lv[r.d];
+    cnt = r.lv[r.d + 1] - lb;
+  } else {
+    lb = r.lb;
+    cnt = r.cnt;
+  }
 }
 
 // Bottom-up subtree sizes (in nodes), one level per launch.
-__global__ __launch_bounds__(256) void subtree_kernel(uint32_t lb, uint32_t cnt, const uint32_t* __restrict__ nchild,
+__global__ __launch_bounds__(256) void subtree_kernel(LevelRange r, const uint32_t* __restrict__ nchild,
                                                       const uint32_t* __restrict__ first_child,
                                                       uint32_t* __restrict__ subtree) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt) return;
-  const uint32_t node = lb + k;
-  uint32_t s = 1;
-  const uint32_t c = nchild[node];
-  for (uint32_t j = 0; j < c; ++j) s += subtree[first_child[node] + j];
-  subtree[node] = s;
+  uint32_t lb, cnt;
+  level_range(r, lb, cnt);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    const uint32_t node = lb + k;
+    uint32_t s = 1;
+    const uint32_t c = nchild[node];
+    for (uint32_t j = 0; j < c; ++j) s += subtree[first_child[node] + j];
+    subtree[node] = s;
+  }
 }
 
 // Top-down preorder indices, one level per launch.
-__global__ __launch_bounds__(256) void preorder_kernel(uint32_t lb, uint32_t cnt, const uint32_t* __restrict__ nchild,
+__global__ __launch_bounds__(256) void preorder_kernel(LevelRange r, const uint32_t* __restrict__ nchild,
                                                        const uint32_t* __restrict__ first_child,
                                                        const uint32_t* __restrict__ subtree,
                                                        uint32_t* __restrict__ pre) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt) return;
-  const uint32_t node = lb + k;
-  uint32_t run = pre[node] + 1;
-  const uint32_t c = nchild[node];
-  for (uint32_t j = 0; j < c; ++j) {
-    const uint32_t ch = first_child[node] + j;
-    pre[ch] = run;
-    run += subtree[ch];
+  uint32_t lb, cnt;
+  level_range(r, lb, cnt);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    const uint32_t node = lb + k;
+    uint32_t run = pre[node] + 1;
+    const uint32_t c = nchild[node];
+    for (uint32_t j = 0; j < c; ++j) {
+      const uint32_t ch = first_child[node] + j;
+      pre[ch] = run;
+      run += subtree[ch];
+    }
   }
 }
 
 // In-leaf Morton order: insertion sort of (morton key, source row) pairs.
-__global__ __launch_bounds__(256) void leaf_sort_kernel(uint32_t total_nodes, const uint32_t* __restrict__ nchild,
-                                                        const uint32_t* __restrict__ nb, const uint32_t* __restrict__ nc,
-                                                        uint64_t* __restrict__ mk, uint32_t* __restrict__ order,
-                                                        uint32_t* __restrict__ from, double4* __restrict__ pk) {
-  const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= total_nodes || nchild[node] != 0) return;
+__device__ __forceinline__ void leaf_sort_node(uint32_t node, const uint32_t* __restrict__ nb,
+                                               const uint32_t* __restrict__ nc, uint64_t* __restrict__ mk,
+                                               uint32_t* __restrict__ order, uint32_t* __restrict__ from,
+                                               double4* __restrict__ pk) {
   const uint32_t b = nb[node], e = b + nc[node];
   for (uint32_t i = b; i < e; ++i) from[i] = i;
   for (uint32_t i = b + 1; i < e; ++i) {
@@ -912,25 +978,31 @@ __global__ __launch_bounds__(256) void leaf_sort_kernel(uint32_t total_nodes, co
   }
 }
 
+// total_dev (when given) holds the node count of a capture-safe build.
+__global__ __launch_bounds__(256) void leaf_sort_kernel(uint32_t total_nodes, const uint32_t* __restrict__ total_dev,
+                                                        const uint32_t* __restrict__ nchild,
+                                                        const uint32_t* __restrict__ nb, const uint32_t* __restrict__ nc,
+                                                        uint64_t* __restrict__ mk, uint32_t* __restrict__ order,
+                                                        uint32_t* __restrict__ from, double4* __restrict__ pk) {
+  const uint32_t total = total_dev ? *total_dev : total_nodes;
+  for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < total; node += gridDim.x * blockDim.x)
+    if (nchild[node] == 0) leaf_sort_node(node, nb, nc, mk, order, from, pk);
+}
+
 // Multipoles, one BFS level per launch from the deepest level up
 // (tree_gravity.cpp:1208-1380, same operation order).
 struct MomArgs {
   UnwrapArgs u;
-  uint32_t lb, cnt;
+  LevelRange r;
   int quad;
 };
 
-__global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, const double4* __restrict__ pk,
-                                                        const uint32_t* __restrict__ nb,
-                                                        const uint32_t* __restrict__ nc,
-                                                        const uint32_t* __restrict__ nchild,
-                                                        const uint32_t* __restrict__ first_child,
-                                                        const double4* __restrict__ geo, double4* __restrict__ mom,
-                                                        double* __restrict__ quad /* 8 per node */,
-                                                        double* __restrict__ box /* 6 per node or null */) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.cnt) return;
-  const uint32_t node = a.lb + k;
+__device__ __forceinline__ void multipole_node(uint32_t node, const MomArgs& a, const double4* __restrict__ pk,
+                                               const uint32_t* __restrict__ nb, const uint32_t* __restrict__ nc,
+                                               const uint32_t* __restrict__ nchild,
+                                               const uint32_t* __restrict__ first_child,
+                                               const double4* __restrict__ geo, double4* __restrict__ mom,
+                                               double* __restrict__ quad, double* __restrict__ box) {
   const double4 g = geo[node];
   double* q = quad + 8ull * node;
   const uint32_t c = nchild[node];
@@ -1062,6 +1134,23 @@ __global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, const double4
   q[7] = sqrt(ox * ox + oy * oy + oz * oz);
 }
 
+__global__ __launch_bounds__(128) void multipole_kernel(MomArgs a, const double4* __restrict__ pk,
+                                                        const uint32_t* __restrict__ nb,
+                                                        const uint32_t* __restrict__ nc,
+                                                        const uint32_t* __restrict__ nchild,
+                                                        const uint32_t* __restrict__ first_child,
+                                                        const double4* __restrict__ geo, double4* __restrict__ mom,
+                                                        double* __restrict__ quad /* 8 per node */,
+                                                        double* __restrict__ box /* 6 per node or null */,
+                                                        const Frame* __restrict__ fr) {
+  if (fr)
+    for (int ax = 0; ax < 3; ++ax) a.u.anchor[ax] = fr->anchor[ax];
+  uint32_t lb, cnt;
+  level_range(a.r, lb, cnt);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x)
+    multipole_node(lb + k, a, pk, nb, nc, nchild, first_child, geo, mom, quad, box);
+}
+
 __device__ __forceinline__ uint32_t to_fixed(double c, double scale) {
   // scale = 2^32 / L; wraps modulo 2^32 (periodic)
   const double f = floor(c * scale);
@@ -1099,9 +1188,10 @@ __global__ __launch_bounds__(256) void records_kernel(RecordArgs a, const uint32
                                                       NodeMom32* __restrict__ mom32,
                                                       NodeQuad32* __restrict__ quad32,
                                                       const double* __restrict__ box,
-                                                      NodeBox32* __restrict__ box32) {
-  const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= a.total) return;
+                                                      NodeBox32* __restrict__ box32,
+                                                      const uint32_t* __restrict__ total_dev) {
+  const uint32_t total = total_dev ? *total_dev : a.total;
+  for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < total; node += gridDim.x * blockDim.x) {
   const uint32_t p = pre[node];
   const uint32_t c = nchild[node];
   link[p] = NodeLink{nb[node], nc[node], subtree[node], c};
@@ -1152,6 +1242,7 @@ __global__ __launch_bounds__(256) void records_kernel(RecordArgs a, const uint32
     r.pad = 0u;
     box32[p] = r;
   }
+  }
 }
 
 __global__ void unwrap_apply_kernel(const double* in, std::size_t n, double L, int canonical, double anchor,
@@ -1163,6 +1254,80 @@ __global__ void unwrap_apply_kernel(const double* in, std::size_t n, double L, i
 __global__ void pos_of_row_kernel(const uint32_t* __restrict__ order, std::size_t n, uint32_t* __restrict__ pos) {
   const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) pos[order[i]] = static_cast<uint32_t>(i);
+}
+
+// ---- capture-safe build: per-build values kept on the device ------------------
+struct FrameCheck {
+  double L[3];
+  int expect_uniform;  // mass uniformity of the prepared build
+  double expect_mass;  // its uniform mass
+};
+
+// Canonical flags from the statistics (tree_pm_coupling.cpp:3185-3195, as the
+// ordinary build decides them on the host); invalid inputs and a mass
+// uniformity other than the prepared build's set fault bits.
+__global__ void frame_stats_kernel(const SourceStats* __restrict__ st, const double* __restrict__ m, FrameCheck c,
+                                   Frame* __restrict__ fr, unsigned long long* __restrict__ fault) {
+  if (threadIdx.x != 0) return;
+  unsigned long long f = 0;
+  if (st->bad_coord) f |= kFaultCoordinates;
+  if (st->bad_mass) f |= kFaultMasses;
+  const int uniform = st->mass_nonuniform == 0 ? 1 : 0;
+  if (uniform != c.expect_uniform || (uniform && m[0] != c.expect_mass)) f |= kFaultPrepared;
+  for (int a = 0; a < 3; ++a) {
+    const double lo = order_key_inv(st->lo[a]), hi = order_key_inv(st->hi[a]);
+    fr->canonical[a] = (lo >= 0.0 && hi < c.L[a]) ? 1 : 0;
+  }
+  fr->pad = 0;
+  if (f) atomicOr(fault, f);
+}
+
+// Root cube (tree_gravity.cpp:443-452) and the in-leaf Morton frame
+// (tree_ordering.cpp:151) from the unwrap result, with the ordinary build's
+// host arithmetic (max as (a < b) ? b : a, no contraction).
+__global__ void frame_root_kernel(const UnwrapResult* __restrict__ res, Frame* __restrict__ fr) {
+  if (threadIdx.x != 0) return;
+  double ext = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double e = res->umax[a] - res->umin[a];
+    ext = ext < e ? e : ext;
+  }
+  const double max_extent = ext < 1.0e-10 ? 1.0e-10 : ext;
+  for (int a = 0; a < 3; ++a) {
+    fr->anchor[a] = res->anchor[a];
+    fr->centre[a] = 0.5 * (res->umin[a] + res->umax[a]);
+    fr->mlo[a] = res->umin[a];
+  }
+  fr->half = 0.5 * max_extent * (1.0 + 1.0e-8);
+  fr->mext = ext < 1.0e-12 ? 1.0e-12 : ext;
+}
+
+// After a capture-safe level discovery of `issued` levels: the node count,
+// and fault bits for a node capacity or depth beyond the prepared build or a
+// tree deeper than 21 levels.
+__global__ void level_check_kernel(uint32_t* __restrict__ lv, int issued, unsigned long long* __restrict__ fault) {
+  if (threadIdx.x != 0) return;
+  unsigned long long f = 0;
+  if (lv[kLvOverflow]) f |= kFaultPrepared;
+  if (lv[kLvTooDeep]) f |= kFaultTooDeep;
+  if (issued < kMaxLevels && lv[issued + 1] > lv[issued]) f |= kFaultPrepared;
+  if (f) {
+    // The tree is incomplete (children counted but not emitted, or levels left
+    // out): every level and the node count become empty, so the passes after
+    // this one, the deposit and the walk do nothing; the fault word reports it.
+    for (int d = 0; d < kLvTooDeep; ++d) lv[d] = 0;
+    lv[kLvTotal] = 0;
+    atomicOr(fault, f);
+  } else {
+    lv[kLvTotal] = lv[issued];
+  }
+}
+
+// p[0, *count) = v, the count read on the device.
+template <class T>
+__global__ __launch_bounds__(256) void fill_count_kernel(T* __restrict__ p, T v, const uint32_t* __restrict__ count) {
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
 template <class T>
@@ -1185,7 +1350,7 @@ SourceStats compute_stats(Tree& t, Axes3 p, const double* m, std::size_t n, hipS
     source_stats_kernel<<<nb, 256, 0, s>>>(p, m, n, st + 1);
     CHUI_LAUNCH_CHECK("source_stats_kernel");
   }
-  stats_final_kernel<<<1, 256, 0, s>>>(st + 1, nb, st);
+  stats_final_kernel<<<1, 256, 0, s>>>(st + 1, nb, n > 0 ? m : nullptr, st);
   CHUI_LAUNCH_CHECK("stats_final_kernel");
   SourceStats h{};
   t.readback.read(&h, st, sizeof(SourceStats), s);
@@ -1200,15 +1365,19 @@ double host_order_key_inv(uint64_t k) {
 }
 
 // Unwraps naxes axes at once; fills res (host).
-void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, const int* canonical,
-                 UnwrapResult& res, hipStream_t s) {
+// With fr (a capture-safe build) the canonical flags come from the frame, the
+// marked-bucket buffers are sized by their bound (marked buckets are occupied
+// buckets, at most n per axis) and nothing is read back: the result stays in
+// the device UnwrapResult (returned).
+UnwrapResult* unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, const int* canonical,
+                          UnwrapResult& res, hipStream_t s, const Frame* fr = nullptr, bool size_bound = false) {
   for (int ax = 0; ax < naxes; ++ax) {
     res.anchor[ax] = 0.0;
     res.umin[ax] = 0.0;
     res.umax[ax] = 0.0;
     res.canonical[ax] = canonical[ax];
   }
-  if (n == 0) return;
+  if (n == 0) return nullptr;
   BucketArgs a{};
   const uint64_t B = (std::max<uint64_t>(2ull * n, 64ull) + 31) / 32 * 32;
   a.B = B;
@@ -1243,7 +1412,7 @@ void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, co
   t.occ_bytes.reserve(naxes * B);
   uint8_t* occb = t.occ_bytes.as<uint8_t>();
   fill_u32(reinterpret_cast<uint32_t*>(occb), 0u, naxes * B / 4, s);
-  occupancy_bytes_kernel<<<pgrid, 256, 0, s>>>(p, n, a, occb);
+  occupancy_bytes_kernel<<<pgrid, 256, 0, s>>>(p, n, a, occb, fr);
   CHUI_LAUNCH_CHECK("occupancy_bytes_kernel");
   occupancy_pack_kernel<<<grid_for(NW, 256), 256, 0, s>>>(occb, NW, occ);
   CHUI_LAUNCH_CHECK("occupancy_pack_kernel");
@@ -1270,14 +1439,28 @@ void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, co
   CHUI_LAUNCH_CHECK("popc_kernel");
   t.bfs_scan.reserve(scan_scratch_bytes(NW) + 64);
   exclusive_scan_u32(popc, rank_base, NW, marked, t.bfs_scan.ptr, s);
-  uint32_t hm = 0;
-  t.readback.read(&hm, marked, sizeof(hm), s);
-  t.bucket_max.reserve(2 * std::max<uint64_t>(hm, 1) * sizeof(uint64_t));
-  uint64_t* cmin = t.bucket_max.as<uint64_t>();
-  uint64_t* cmax = cmin + std::max<uint64_t>(hm, 1);
-  fill_u64(cmin, kEmpty, hm, s);
-  fill_u64(cmax, 0ull, hm, s);
-  marked_minmax_kernel<<<pgrid, 256, 0, s>>>(p, n, a, mark, rank_base, cmin, cmax);
+  uint64_t* cmin = nullptr;
+  uint64_t* cmax = nullptr;
+  if (fr) {
+    const uint64_t cap = static_cast<uint64_t>(naxes) * n + 64;
+    t.bucket_max.reserve(2 * cap * sizeof(uint64_t));
+    cmin = t.bucket_max.as<uint64_t>();
+    cmax = cmin + cap;
+    fill_count_kernel<uint64_t><<<256, 256, 0, s>>>(cmin, kEmpty, marked);
+    CHUI_LAUNCH_CHECK("fill_count_kernel");
+    fill_count_kernel<uint64_t><<<256, 256, 0, s>>>(cmax, 0ull, marked);
+    CHUI_LAUNCH_CHECK("fill_count_kernel");
+  } else {
+    uint32_t hm = 0;
+    t.readback.read(&hm, marked, sizeof(hm), s);
+    if (size_bound) t.bucket_max.reserve(2 * (static_cast<uint64_t>(naxes) * n + 64) * sizeof(uint64_t));
+    t.bucket_max.reserve(2 * std::max<uint64_t>(hm, 1) * sizeof(uint64_t));
+    cmin = t.bucket_max.as<uint64_t>();
+    cmax = cmin + std::max<uint64_t>(hm, 1);
+    fill_u64(cmin, kEmpty, hm, s);
+    fill_u64(cmax, 0ull, hm, s);
+  }
+  marked_minmax_kernel<<<pgrid, 256, 0, s>>>(p, n, a, mark, rank_base, cmin, cmax, fr);
   CHUI_LAUNCH_CHECK("marked_minmax_kernel");
   ra.cmin = cmin;
   ra.cmax = cmax;
@@ -1285,6 +1468,7 @@ void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, co
   CHUI_LAUNCH_CHECK("runs_kernel<2>");
   gap_final_kernel<<<naxes, kGapThreads, 0, s>>>(best, nblk, dres);
   CHUI_LAUNCH_CHECK("gap_final_kernel");
+  if (fr) return dres;
   UnwrapResult h{};
   t.readback.read(&h, dres, sizeof(UnwrapResult), s);
   for (int ax = 0; ax < naxes; ++ax) {
@@ -1292,6 +1476,7 @@ void unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const double* L, co
     res.umin[ax] = h.umin[ax];
     res.umax[ax] = h.umax[ax];
   }
+  return dres;
 }
 
 void launch_records(Tree& t, bool fp64, bool mixed, hipStream_t s) {
@@ -1316,7 +1501,7 @@ void launch_records(Tree& t, bool fp64, bool mixed, hipStream_t s) {
       fp64 ? t.quad64.as<NodeQuad64>() : nullptr, fp64 ? t.child_index.as<uint32_t>() : nullptr,
       mixed ? t.geo32.as<NodeGeo32>() : nullptr, mixed ? t.mom32.as<NodeMom32>() : nullptr,
       mixed ? t.quad32.as<NodeQuad32>() : nullptr, mixed ? t.bfs_box.as<double>() : nullptr,
-      mixed ? t.box32.as<NodeBox32>() : nullptr);
+      mixed ? t.box32.as<NodeBox32>() : nullptr, device_node_count(t));
   CHUI_LAUNCH_CHECK("records_kernel");
   if (fp64) t.records64 = true;
 }
@@ -1334,41 +1519,76 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
   (void)softening;
   const std::size_t n = P.n;
   if (n > 0xFFFFFFFEull) throw std::overflow_error("Tree gravity source count exceeds the 32-bit tree-index contract");
+  const bool safe = P.capture_safe;
+  if (safe) {
+    if (!P.mixed || !P.fault) throw std::logic_error("capture-safe tree builds are mixed-precision solves");
+    if (n == 0 || n != t.prepared_n || !t.sized_for_capture || t.safe_levels <= 0 || t.bfs_capacity == 0)
+      throw std::runtime_error(
+          "capture-safe solve on a context not prepared for these sources (run an ordinary solve of the same "
+          "source count first)");
+  }
   t.n = n;
   t.node_count = 0;
   t.leaf_count = 0;
-  t.max_depth = 0;
-  t.level_start.clear();
+  t.counts_on_device = false;
+  if (!safe) {
+    t.max_depth = 0;
+    t.level_start.clear();
+  }
   Axes3 p{{x, y, z}};
-  const SourceStats st = compute_stats(t, p, m, n, s);
-  if (st.bad_coord) throw std::invalid_argument("Periodic TreePM tree geometry requires finite source coordinates");
-  if (st.bad_mass) throw std::invalid_argument("Tree gravity requires finite coordinates and finite non-negative masses");
-  if (n == 0) return;
-  t.mass_uniform = st.mass_nonuniform == 0;
-  t.uniform_mass = 0.0;
-  t.readback.read(&t.uniform_mass, m, sizeof(double), s);
-  int canonical[3];
-  for (int a = 0; a < 3; ++a) {
-    const double lo = host_order_key_inv(st.lo[a]);
-    const double hi = host_order_key_inv(st.hi[a]);
-    canonical[a] = (lo >= 0.0 && hi < P.L[a]) ? 1 : 0;  // tree_pm_coupling.cpp:3185-3195
-  }
-  unwrap_axes(t, p, 3, n, P.L, canonical, t.unwrap, s);
-  const UnwrapResult& U = t.unwrap;
-
-  // Root cube (tree_gravity.cpp:443-452) from the unwrapped bounds.
-  double ext = 0.0;
-  for (int a = 0; a < 3; ++a) ext = std::max(ext, U.umax[a] - U.umin[a]);
-  const double max_extent = std::max(ext, 1.0e-10);
-  for (int a = 0; a < 3; ++a) t.root_centre[a] = 0.5 * (U.umin[a] + U.umax[a]);
-  t.root_half = 0.5 * max_extent * (1.0 + 1.0e-8);
-
+  Frame* fr = nullptr;
   UnwrapArgs ua{};
-  for (int a = 0; a < 3; ++a) {
-    ua.L[a] = P.L[a];
-    ua.canonical[a] = U.canonical[a];
-    ua.anchor[a] = U.anchor[a];
+  for (int a = 0; a < 3; ++a) ua.L[a] = P.L[a];
+  if (safe) {
+    // statistics -> canonical flags and checks on the device
+    t.frame.reserve(sizeof(Frame));
+    fr = t.frame.as<Frame>();
+    const unsigned nb = std::min<unsigned>(grid_for(n, 256), 1024u);
+    t.bounds.reserve((1 + nb) * sizeof(SourceStats));
+    SourceStats* st = t.bounds.as<SourceStats>();
+    source_stats_kernel<<<nb, 256, 0, s>>>(p, m, n, st + 1);
+    CHUI_LAUNCH_CHECK("source_stats_kernel");
+    stats_final_kernel<<<1, 256, 0, s>>>(st + 1, nb, m, st);
+    CHUI_LAUNCH_CHECK("stats_final_kernel");
+    FrameCheck fc{};
+    for (int a = 0; a < 3; ++a) fc.L[a] = P.L[a];
+    fc.expect_uniform = t.mass_uniform ? 1 : 0;
+    fc.expect_mass = t.uniform_mass;
+    frame_stats_kernel<<<1, 64, 0, s>>>(st, m, fc, fr, P.fault);
+    CHUI_LAUNCH_CHECK("frame_stats_kernel");
+    int unused[3] = {0, 0, 0};
+    UnwrapResult* dres = unwrap_axes(t, p, 3, n, P.L, unused, t.unwrap, s, fr);
+    frame_root_kernel<<<1, 64, 0, s>>>(dres, fr);
+    CHUI_LAUNCH_CHECK("frame_root_kernel");
+  } else {
+    const SourceStats st = compute_stats(t, p, m, n, s);
+    if (st.bad_coord) throw std::invalid_argument("Periodic TreePM tree geometry requires finite source coordinates");
+    if (st.bad_mass)
+      throw std::invalid_argument("Tree gravity requires finite coordinates and finite non-negative masses");
+    if (n == 0) return;
+    t.mass_uniform = st.mass_nonuniform == 0;
+    t.uniform_mass = st.mass0;
+    int canonical[3];
+    for (int a = 0; a < 3; ++a) {
+      const double lo = host_order_key_inv(st.lo[a]);
+      const double hi = host_order_key_inv(st.hi[a]);
+      canonical[a] = (lo >= 0.0 && hi < P.L[a]) ? 1 : 0;  // tree_pm_coupling.cpp:3185-3195
+    }
+    unwrap_axes(t, p, 3, n, P.L, canonical, t.unwrap, s, nullptr, P.size_for_capture);
+    if (P.size_for_capture) t.frame.reserve(sizeof(Frame));
+    const UnwrapResult& U = t.unwrap;
+    // Root cube (tree_gravity.cpp:443-452) from the unwrapped bounds.
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a) ext = std::max(ext, U.umax[a] - U.umin[a]);
+    const double max_extent = std::max(ext, 1.0e-10);
+    for (int a = 0; a < 3; ++a) t.root_centre[a] = 0.5 * (U.umin[a] + U.umax[a]);
+    t.root_half = 0.5 * max_extent * (1.0 + 1.0e-8);
+    for (int a = 0; a < 3; ++a) {
+      ua.canonical[a] = U.canonical[a];
+      ua.anchor[a] = U.anchor[a];
+    }
   }
+
   // Path keys + stable sort.
   t.keys.reserve(n * sizeof(uint64_t));
   t.keys_alt.reserve(n * sizeof(uint64_t));
@@ -1383,23 +1603,28 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
   uint32_t* order = t.order.as<uint32_t>();
   t.pk.reserve(n * sizeof(double4));
   t.pk_in.reserve(n * sizeof(double4));
-  path_key_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, m, n, pa, keys, order, t.pk_in.as<double4>());
+  path_key_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, m, n, pa, keys, order, t.pk_in.as<double4>(), fr);
   CHUI_LAUNCH_CHECK("path_key_kernel");
   // Sort by the bits of levels 1..16 only (6 digit passes instead of 8). This
   // is the reference's partition order for every node up to depth 16, and the
   // order inside a leaf is rebuilt below by (Morton key, row), so it suffices
   // unless some depth-16 node still splits; the discovery then restarts after
-  // sorting by all 21 levels.
+  // sorting by all 21 levels. A capture-safe build cannot restart: it sorts by
+  // all 21 levels at once (the same partition, and the same order after the
+  // in-leaf sort).
   constexpr int kPartialSortLevels = 16;
   constexpr int kPartialSortLow = 3 * (kPathLevels - kPartialSortLevels);
-  radix_sort_pairs(keys, order, t.keys_alt.as<uint64_t>(), t.vals_alt.as<uint32_t>(), n, kPartialSortLow,
-                   3 * kPathLevels, t.sort_scratch.ptr, s);
-  bool partial = true;
+  bool partial = !safe;
+  radix_sort_pairs(keys, order, t.keys_alt.as<uint64_t>(), t.vals_alt.as<uint32_t>(), n,
+                   partial ? kPartialSortLow : 0, 3 * kPathLevels, t.sort_scratch.ptr, s);
 
   // Level-synchronous discovery. Every level is issued from the host without a
   // read-back (the kernels read each level's range from lv); the level starts
   // and flags come back once. A node capacity too small for the tree, or a
   // depth-16 node that splits while only levels 1..16 are sorted, restarts it.
+  // A capture-safe build issues the prepared depth + 4 levels into the
+  // prepared capacity and reads nothing back; a tree that does not fit sets a
+  // fault bit.
   const std::size_t initial_cap = std::max<std::size_t>(1024, n / 4 + 64);
   auto ensure = [&](std::size_t need, std::size_t used) {
     if (need <= t.bfs_capacity && t.bfs_begin.ptr) return;
@@ -1412,17 +1637,19 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
     grow_copy<double4>(t.bfs_geo, used, nc, s);
     t.bfs_capacity = nc;
   };
-  ensure(initial_cap, 0);
+  if (!safe) ensure(initial_cap, 0);
   t.level_dev.reserve(kLvWords * sizeof(uint32_t));
   t.level_scratch.reserve(kLevelBlocks * sizeof(uint32_t));
   uint32_t* lv = t.level_dev.as<uint32_t>();
   uint32_t* bsum = t.level_scratch.as<uint32_t>();
+  int nlevels = 0;       // BFS levels to run the per-level passes over
+  std::size_t total = 0; // node count, or the capacity for a capture-safe build
   for (;;) {
     const std::size_t cap = std::min<std::size_t>(t.bfs_capacity, 0xFFFFFFFFull);
     t.level_bounds.reserve(8ull * cap * sizeof(uint32_t));
     root_init_kernel<<<1, 1, 0, s>>>(t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(),
                                      t.bfs_meta.as<uint32_t>(), t.bfs_geo.as<double4>(), static_cast<uint32_t>(n),
-                                     t.root_centre[0], t.root_centre[1], t.root_centre[2], t.root_half);
+                                     t.root_centre[0], t.root_centre[1], t.root_centre[2], t.root_half, fr);
     CHUI_LAUNCH_CHECK("root_init_kernel");
     level_dev_init_kernel<<<1, 64, 0, s>>>(lv);
     CHUI_LAUNCH_CHECK("level_dev_init_kernel");
@@ -1443,6 +1670,14 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
         CHUI_LAUNCH_CHECK("level_emit_dev_kernel");
       }
     };
+    if (safe) {
+      nlevels = std::min(kMaxLevels, t.safe_levels);
+      issue_levels(0, nlevels);
+      level_check_kernel<<<1, 64, 0, s>>>(lv, nlevels, P.fault);
+      CHUI_LAUNCH_CHECK("level_check_kernel");
+      total = cap;
+      break;
+    }
     // Levels up to the last build's depth + 2 first (an empty level costs
     // three launches of about 4 us each); the read-back that follows anyway
     // tells whether the tree went deeper, and only then are the rest issued.
@@ -1482,70 +1717,92 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
     t.max_depth = static_cast<uint32_t>(t.level_start.size() - 2);
     t.leaf_count = h[kLvLeaves];
     t.level_hint = static_cast<int>(t.max_depth) + 3;
+    t.safe_levels = std::min(kMaxLevels, static_cast<int>(t.max_depth) + 5);
+    t.prepared_n = n;
+    t.sized_for_capture = P.size_for_capture;
+    nlevels = static_cast<int>(t.level_start.size()) - 1;
+    total = t.level_start.back();
     break;
   }
-  const std::size_t total = t.level_start.back();
-  const int nlevels = static_cast<int>(t.level_start.size()) - 1;
   t.node_count = total;
-  t.bfs_subtree.reserve(total * sizeof(uint32_t));
-  t.bfs_pre.reserve(total * sizeof(uint32_t));
-  t.bfs_mom.reserve(total * sizeof(double4));
-  t.bfs_quad.reserve(total * 8 * sizeof(double));
+  t.counts_on_device = safe;
+  const uint32_t* total_dev = device_node_count(t);
+  // node arrays: the node count, or the capacity when capture-safe builds follow
+  const std::size_t nsz = P.size_for_capture ? std::max(total, t.bfs_capacity) : total;
+  // per-level passes: one launch per level; a capture-safe build reads each
+  // level's range on the device and strides a fixed grid over it
+  auto range_of = [&](int d) {
+    LevelRange r{};
+    if (safe) {
+      r.lv = lv;
+      r.d = d;
+    } else {
+      r.lb = static_cast<uint32_t>(t.level_start[d]);
+      r.cnt = static_cast<uint32_t>(t.level_start[d + 1] - t.level_start[d]);
+    }
+    return r;
+  };
+  auto level_grid = [&](int d, unsigned block) {
+    if (safe) return std::min<unsigned>(grid_for(total, block), 1024u);
+    return grid_for(t.level_start[d + 1] - t.level_start[d], block);
+  };
+  t.bfs_subtree.reserve(nsz * sizeof(uint32_t));
+  t.bfs_pre.reserve(nsz * sizeof(uint32_t));
+  t.bfs_mom.reserve(nsz * sizeof(double4));
+  t.bfs_quad.reserve(nsz * 8 * sizeof(double));
   uint32_t* nchild = t.bfs_nchild.as<uint32_t>();
   uint32_t* fchild = t.bfs_first_child.as<uint32_t>();
   for (int d = nlevels - 1; d >= 0; --d) {
-    const uint32_t lb = static_cast<uint32_t>(t.level_start[d]);
-    const uint32_t cnt = static_cast<uint32_t>(t.level_start[d + 1] - lb);
-    subtree_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(lb, cnt, nchild, fchild, t.bfs_subtree.as<uint32_t>());
+    subtree_kernel<<<level_grid(d, 256), 256, 0, s>>>(range_of(d), nchild, fchild, t.bfs_subtree.as<uint32_t>());
     CHUI_LAUNCH_CHECK("subtree_kernel");
   }
   fill_u32(t.bfs_pre.as<uint32_t>(), 0u, 1, s);
   for (int d = 0; d < nlevels; ++d) {
-    const uint32_t lb = static_cast<uint32_t>(t.level_start[d]);
-    const uint32_t cnt = static_cast<uint32_t>(t.level_start[d + 1] - lb);
-    preorder_kernel<<<grid_for(cnt, 256), 256, 0, s>>>(lb, cnt, nchild, fchild, t.bfs_subtree.as<uint32_t>(),
+    preorder_kernel<<<level_grid(d, 256), 256, 0, s>>>(range_of(d), nchild, fchild, t.bfs_subtree.as<uint32_t>(),
                                                        t.bfs_pre.as<uint32_t>());
     CHUI_LAUNCH_CHECK("preorder_kernel");
   }
   // In-leaf Morton order (tree_ordering.cpp:151 extent with the 1e-12 floor).
   MortonArgs ma{};
-  double mext = 0.0;
-  for (int a = 0; a < 3; ++a) {
-    ma.lo[a] = U.umin[a];
-    mext = std::max(mext, U.umax[a] - U.umin[a]);
+  if (!safe) {
+    const UnwrapResult& U = t.unwrap;
+    double mext = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      ma.lo[a] = U.umin[a];
+      mext = std::max(mext, U.umax[a] - U.umin[a]);
+    }
+    ma.extent = std::max(mext, 1.0e-12);
   }
-  ma.extent = std::max(mext, 1.0e-12);
   uint64_t* mk = t.keys_alt.as<uint64_t>();
   double4* pk = t.pk.as<double4>();
   gather_sources_kernel<<<grid_for(n, 256), 256, 0, s>>>(t.pk_in.as<double4>(), n, order, pk);
   CHUI_LAUNCH_CHECK("gather_sources_kernel");
-  leaf_morton_kernel<<<grid_for(n, 256), 256, 0, s>>>(pk, n, ua, ma, mk);
+  leaf_morton_kernel<<<grid_for(n, 256), 256, 0, s>>>(pk, n, ua, ma, mk, fr);
   CHUI_LAUNCH_CHECK("leaf_morton_kernel");
-  leaf_sort_kernel<<<grid_for(total, 256), 256, 0, s>>>(static_cast<uint32_t>(total), nchild,
+  leaf_sort_kernel<<<grid_for(total, 256), 256, 0, s>>>(static_cast<uint32_t>(total), total_dev, nchild,
                                                         t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), mk,
                                                         order, t.vals_alt.as<uint32_t>(), pk);
   CHUI_LAUNCH_CHECK("leaf_sort_kernel");
   // Multipoles and tight particle boxes bottom-up.
-  t.bfs_box.reserve(total * 6 * sizeof(double));
+  t.bfs_box.reserve(nsz * 6 * sizeof(double));
   for (int d = nlevels - 1; d >= 0; --d) {
     MomArgs ma2{};
     ma2.u = ua;
-    ma2.lb = static_cast<uint32_t>(t.level_start[d]);
-    ma2.cnt = static_cast<uint32_t>(t.level_start[d + 1] - t.level_start[d]);
+    ma2.r = range_of(d);
     ma2.quad = P.quadrupole ? 1 : 0;
-    multipole_kernel<<<grid_for(ma2.cnt, 128), 128, 0, s>>>(
+    multipole_kernel<<<level_grid(d, 128), 128, 0, s>>>(
         ma2, pk, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), nchild, fchild,
-        t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>(), t.bfs_box.as<double>());
+        t.bfs_geo.as<double4>(), t.bfs_mom.as<double4>(), t.bfs_quad.as<double>(), t.bfs_box.as<double>(), fr);
     CHUI_LAUNCH_CHECK("multipole_kernel");
   }
   // Preorder records: links and, for mixed builds, the fixed-point records;
   // the FP64 records for FP64 builds (or later, on demand).
-  t.link.reserve(total * sizeof(NodeLink));
+  t.link.reserve(nsz * sizeof(NodeLink));
   if (P.mixed) {
-    t.geo32.reserve(total * sizeof(NodeGeo32));
-    t.mom32.reserve(total * sizeof(NodeMom32));
-    t.quad32.reserve(total * sizeof(NodeQuad32));
-    t.box32.reserve(total * sizeof(NodeBox32));
+    t.geo32.reserve(nsz * sizeof(NodeGeo32));
+    t.mom32.reserve(nsz * sizeof(NodeMom32));
+    t.quad32.reserve(nsz * sizeof(NodeQuad32));
+    t.box32.reserve(nsz * sizeof(NodeBox32));
   }
   t.records64 = false;
   t.cubic = (P.L[0] == P.L[1] && P.L[0] == P.L[2]);
@@ -1582,74 +1839,98 @@ __device__ bool deposit_tile(const double* box, const DepositTiling& d, int32_t 
 // Units form a cut of the tree: a node is one when it fits (or is a leaf) and
 // its parent does not, so every particle lies in exactly one unit. The parent
 // decides for its children; flags start at zero.
-__global__ __launch_bounds__(256) void deposit_flag_kernel(uint32_t total, const uint32_t* __restrict__ nchild,
+__global__ __launch_bounds__(256) void deposit_flag_kernel(uint32_t total_nodes, const uint32_t* __restrict__ total_dev,
+                                                           const uint32_t* __restrict__ nchild,
                                                            const uint32_t* __restrict__ first_child,
                                                            const double* __restrict__ box, DepositTiling d,
                                                            uint32_t* __restrict__ flag) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= total) return;
-  int32_t lo[3], dims[3];
-  const uint32_t c = nchild[k];
-  const bool fits = deposit_tile(box + 6ull * k, d, lo, dims);
-  if (k == 0) flag[0] = (fits || c == 0) ? 1u : 0u;
-  if (fits || c == 0) return;
-  const uint32_t fc = first_child[k];
-  for (uint32_t j = 0; j < c; ++j) {
-    const uint32_t ch = fc + j;
-    flag[ch] = (nchild[ch] == 0 || deposit_tile(box + 6ull * ch, d, lo, dims)) ? 1u : 0u;
+  const uint32_t total = total_dev ? *total_dev : total_nodes;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+    int32_t lo[3], dims[3];
+    const uint32_t c = nchild[k];
+    const bool fits = deposit_tile(box + 6ull * k, d, lo, dims);
+    if (k == 0) flag[0] = (fits || c == 0) ? 1u : 0u;
+    if (fits || c == 0) continue;
+    const uint32_t fc = first_child[k];
+    for (uint32_t j = 0; j < c; ++j) {
+      const uint32_t ch = fc + j;
+      flag[ch] = (nchild[ch] == 0 || deposit_tile(box + 6ull * ch, d, lo, dims)) ? 1u : 0u;
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void deposit_emit_kernel(uint32_t total, const uint32_t* __restrict__ flag,
+__global__ __launch_bounds__(256) void deposit_emit_kernel(uint32_t total_nodes, const uint32_t* __restrict__ total_dev,
+                                                           const uint32_t* __restrict__ flag,
                                                            const uint32_t* __restrict__ index,
                                                            const uint32_t* __restrict__ nb,
                                                            const uint32_t* __restrict__ nc,
                                                            const double* __restrict__ box, DepositTiling d,
                                                            DepositUnit* __restrict__ units,
                                                            uint32_t* __restrict__ blocks) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= total || !flag[k]) return;
-  DepositUnit u;
-  u.begin = nb[k];
-  u.count = nc[k];
-  if (!deposit_tile(box + 6ull * k, d, u.lo, u.dims)) u.dims[0] = 0;  // leaf spanning too many cells
-  const uint32_t i = index[k];
-  units[i] = u;
-  blocks[i] = (u.count + d.chunk - 1) / d.chunk;
+  const uint32_t total = total_dev ? *total_dev : total_nodes;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+    if (!flag[k]) continue;
+    DepositUnit u;
+    u.begin = nb[k];
+    u.count = nc[k];
+    if (!deposit_tile(box + 6ull * k, d, u.lo, u.dims)) u.dims[0] = 0;  // leaf spanning too many cells
+    const uint32_t i = index[k];
+    units[i] = u;
+    blocks[i] = (u.count + d.chunk - 1) / d.chunk;
+  }
 }
 
 DepositUnits build_deposit_units(Tree& t, const DepositTiling& d, hipStream_t s) {
   DepositUnits r;
   const std::size_t total = t.node_count;
   if (total == 0 || t.n == 0) return r;
-  t.dep_flag.reserve(total * sizeof(uint32_t));
-  t.dep_index.reserve(total * sizeof(uint32_t));
-  t.dep_units.reserve(total * sizeof(DepositUnit));
-  t.dep_blocks.reserve((total + 1) * sizeof(uint32_t));
-  t.dep_scan.reserve(scan_scratch_bytes(total) + 64);
+  const std::size_t nsz = t.sized_for_capture ? std::max(total, t.bfs_capacity) : total;
+  t.dep_flag.reserve(nsz * sizeof(uint32_t));
+  t.dep_index.reserve(nsz * sizeof(uint32_t));
+  t.dep_units.reserve(nsz * sizeof(DepositUnit));
+  t.dep_blocks.reserve((nsz + 1) * sizeof(uint32_t));
+  t.dep_scan.reserve(scan_scratch_bytes(nsz) + 64);
   t.dep_total.reserve(2 * sizeof(uint32_t));
   uint32_t* flag = t.dep_flag.as<uint32_t>();
   uint32_t* totals = t.dep_total.as<uint32_t>();  // [0] units, [1] blocks
+  const uint32_t* total_dev = device_node_count(t);
   fill_u32(flag, 0u, total, s);
-  deposit_flag_kernel<<<grid_for(total, 256), 256, 0, s>>>(static_cast<uint32_t>(total), t.bfs_nchild.as<uint32_t>(),
+  deposit_flag_kernel<<<grid_for(total, 256), 256, 0, s>>>(static_cast<uint32_t>(total), total_dev,
+                                                           t.bfs_nchild.as<uint32_t>(),
                                                            t.bfs_first_child.as<uint32_t>(), t.bfs_box.as<double>(),
                                                            d, flag);
   CHUI_LAUNCH_CHECK("deposit_flag_kernel");
   exclusive_scan_u32(flag, t.dep_index.as<uint32_t>(), total, totals, t.dep_scan.ptr, s);
+  if (total_dev) fill_u32(t.dep_blocks.as<uint32_t>(), 0u, total, s);  // blocks past the last unit stay 0
   deposit_emit_kernel<<<grid_for(total, 256), 256, 0, s>>>(
-      static_cast<uint32_t>(total), flag, t.dep_index.as<uint32_t>(), t.bfs_begin.as<uint32_t>(),
+      static_cast<uint32_t>(total), total_dev, flag, t.dep_index.as<uint32_t>(), t.bfs_begin.as<uint32_t>(),
       t.bfs_count.as<uint32_t>(), t.bfs_box.as<double>(), d, t.dep_units.as<DepositUnit>(),
       t.dep_blocks.as<uint32_t>());
   CHUI_LAUNCH_CHECK("deposit_emit_kernel");
+  r.units = t.dep_units.as<DepositUnit>();
+  r.first_block = t.dep_blocks.as<uint32_t>();
+  if (total_dev) {
+    // capture-safe: scan the blocks of every capacity slot (zero past the last
+    // unit) and leave both counts on the device; the deposit's grid is a fixed
+    // size that strides over the real block count
+    exclusive_scan_u32(t.dep_blocks.as<uint32_t>(), t.dep_blocks.as<uint32_t>(), total, totals + 1, t.dep_scan.ptr,
+                       s);
+    r.dev_counts = totals;
+    r.count = 0;
+    r.blocks = static_cast<uint32_t>(std::min<std::size_t>(total + t.n / d.chunk + 1, kDepositSafeGrid));
+    return r;
+  }
   uint32_t host[2] = {0, 0};
   t.readback.read(host, totals, sizeof(uint32_t), s);
   exclusive_scan_u32(t.dep_blocks.as<uint32_t>(), t.dep_blocks.as<uint32_t>(), host[0], totals + 1, t.dep_scan.ptr, s);
   t.readback.read(host + 1, totals + 1, sizeof(uint32_t), s);
-  r.units = t.dep_units.as<DepositUnit>();
-  r.first_block = t.dep_blocks.as<uint32_t>();
   r.count = host[0];
   r.blocks = host[1];
   return r;
+}
+
+const uint32_t* device_node_count(const Tree& t) {
+  return t.counts_on_device ? t.level_dev.as<uint32_t>() + kLvTotal : nullptr;
 }
 
 void build_pos_of_row(Tree& t, hipStream_t s) {

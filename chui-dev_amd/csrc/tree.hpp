@@ -42,12 +42,31 @@ struct UnwrapResult {
   int canonical[3];
 };
 
+// Bits of the sticky fault word (the walk counters' fault slot, copied to the
+// host after every solve). Walks set kFaultWalk; a capture-safe build sets the
+// others where the ordinary build throws, since it reads nothing back.
+constexpr uint64_t kFaultWalk = 1;         // a walk lost progress (logic error)
+constexpr uint64_t kFaultCoordinates = 2;  // non-finite source coordinates (invalid argument)
+constexpr uint64_t kFaultMasses = 4;       // non-finite or negative masses (invalid argument)
+constexpr uint64_t kFaultPrepared = 8;     // node capacity, depth or mass uniformity beyond the prepared build
+constexpr uint64_t kFaultTooDeep = 16;     // octree deeper than 21 levels (unsupported)
+constexpr uint64_t kFaultActive = 32;      // active index out of range (invalid argument)
+
 struct TreeParams {
   std::size_t n = 0;
   double L[3] = {0, 0, 0};
   uint32_t leaf_size = 16;
   bool quadrupole = true;
   bool mixed = false;  // emit the fixed-point records; the FP64 records follow on demand
+  // Capture-safe build: no read-back and no allocation once prepared; every
+  // per-build value stays on the device and data-dependent failures set bits
+  // of *fault (mixed precision only).
+  bool capture_safe = false;
+  unsigned long long* fault = nullptr;
+  // An ordinary build that sizes every buffer as a capture-safe build of the
+  // same sources will (node arrays by the capacity, unwrap buffers by their
+  // bounds), so that the capture-safe builds after it allocate nothing.
+  bool size_for_capture = false;
 };
 
 // Device state of one built tree. Buffers are grow-only and owned by Tree.
@@ -57,6 +76,15 @@ struct Tree {
   std::size_t leaf_count = 0;
   uint32_t max_depth = 0;
   int level_hint = 0;  // levels the next build issues before its first read-back (0: all)
+  // Capture-safe builds: the node capacity (bfs_capacity), the source count,
+  // the mass uniformity and the levels issued (safe_levels, the prepared
+  // depth + 4) come from the last ordinary build; node_count is then the
+  // capacity and the real count sits on the device (counts_on_device).
+  int safe_levels = 0;
+  std::size_t prepared_n = 0;
+  bool counts_on_device = false;
+  bool sized_for_capture = false;  // the last ordinary build sized its buffers for capture-safe builds
+  DeviceBuffer frame;  // Frame of the last capture-safe build
   UnwrapResult unwrap{};
   bool mass_uniform = false;
   double uniform_mass = 0.0;
@@ -90,6 +118,10 @@ struct Tree {
 void build_tree(Tree& t, const double* x, const double* y, const double* z, const double* m,
                 const TreeParams& p, double softening, hipStream_t s);
 
+// The node count of the last build on the device (capture-safe builds), or
+// nullptr after an ordinary build.
+const uint32_t* device_node_count(const Tree& t);
+
 // FP64 preorder records (geo64, mom64, quad64, child_index) of the last built
 // tree. A mixed-precision build leaves them out (its walks read the
 // fixed-point records); they are written from the BFS arrays on first need.
@@ -97,7 +129,8 @@ void ensure_records64(Tree& t, hipStream_t s);
 
 // Mesh-deposit units of the last built tree: the shallowest subtrees whose
 // particles span at most d.max_cells cells (or leaves that never do). Reads
-// back the unit and block counts (one synchronisation).
+// back the unit and block counts, or after a capture-safe build leaves them on
+// the device (DepositUnits::dev_counts) and sizes the launch by a bound.
 DepositUnits build_deposit_units(Tree& t, const DepositTiling& d, hipStream_t s);
 
 // Morton keys + stable order of given (unwrapped) device coordinates.

@@ -773,6 +773,9 @@ struct WalkData {
   unsigned long long* vjp_accepted;
   unsigned int* vjp_accepted_count;
   unsigned int vjp_accepted_cap;
+  // after a capture-safe build the node count is on the device (P.node_count
+  // is then the capacity); the mixed walks bound their traversal with it
+  const uint32_t* node_count_dev;
 };
 
 // splitmix64 finaliser of the pair (target row, source row): the pair-set
@@ -1541,6 +1544,7 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
     for (int i = threadIdx.x; i < kHTableCells; i += kWalkThreads) htab[i] = D.htab[i];
   }
   __syncthreads();
+  const uint32_t node_count = D.node_count_dev ? *D.node_count_dev : P.node_count;
   const int w = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
@@ -1548,7 +1552,7 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   const uint32_t tp = valid ? (P.identity_targets ? k : D.tpos[k]) : 0u;
   const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
   uint32_t cnt[kWalkCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
-  uint32_t nxt = valid ? 0u : P.node_count;
+  uint32_t nxt = valid ? 0u : node_count;
   float nearest = INFINITY;  // counting variant: nearest distinct pair, table units
   uint32_t px = 0, py = 0, pz = 0;
   float gx = 0.0f, gy = 0.0f, gz = 0.0f;
@@ -1951,7 +1955,7 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
         take((chain >> lane) & 1ull, kind, q, lb, lc, next);
       }
     };
-    chain_walk(0, P.node_count);
+    chain_walk(0, node_count);
     if (nl) eval_window(nl);
     if (wn) evaluate(wn);
   };
@@ -1964,18 +1968,18 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   if (boxed) {
     if constexpr (BOX) box_walk();
   } else if (near) {
-    walk.template operator()<true>(P.node_count);
+    walk.template operator()<true>(node_count);
   } else {
-    walk.template operator()<false>(P.node_count);
+    walk.template operator()<false>(node_count);
   }
   fold();
 
   // Diagnostic launches also replay the reference's walk over the octree
   // cells (no pair work) for its per-target node counters.
   if constexpr (COUNT) {
-    uint32_t cnxt = valid ? 0u : P.node_count;
+    uint32_t cnxt = valid ? 0u : node_count;
     uint32_t n = wave_min_u32(cnxt);
-    while (n < P.node_count) {
+    while (n < node_count) {
       const NodeLink lk = node_link[n];
       const NodeGeo32 g = node_geo[n];
       const bool act = cnxt == n;
@@ -2180,7 +2184,7 @@ __global__ __launch_bounds__(kWalkThreads) void walkmix_target_kernel(WalkParams
     next = take ? q + lk.skip : q + 1;
     return take ? 2 : 3;
   };
-  walk_target_wave(P.node_count, lane, list[w], mps[w], nl, nm, classify, evaluate);
+  walk_target_wave(D.node_count_dev ? *D.node_count_dev : P.node_count, lane, list[w], mps[w], nl, nm, classify, evaluate);
   const double gm = P.force_scale;
   const double sx = gm * wave_sum(static_cast<double>(lx)) + wave_sum(static_cast<double>(mx));
   const double sy = gm * wave_sum(static_cast<double>(ly)) + wave_sum(static_cast<double>(my));
@@ -3216,6 +3220,7 @@ void walk_mixed(const Tree& t, const WalkParams& p, const uint4* src, int unifor
   d.f32x = fx;
   d.f32y = fy;
   d.f32z = fz;
+  d.node_count_dev = device_node_count(t);
   const bool cic = p.assignment == 0;
   if (p.cubic) {
     if (cic) launchmix<0, true>(p, d, out, counters, s);

@@ -212,6 +212,34 @@ chui_status chui_treepm_destroy(chui_treepm_context* ctx);
 chui_status chui_treepm_set_stream(chui_treepm_context* ctx, void* stream);
 chui_status chui_treepm_synchronize(chui_treepm_context* ctx);
 
+/* ---- capture-safe solves and HIP graphs ----------------------------------
+ * No counterpart in the reference (its solves are driven from the host). For
+ * callers that record force evaluations into a HIP graph: ensembles of boxes,
+ * tight step loops. In capture-safe mode the first solve of a source count is
+ * an ordinary one that also sizes the context for it and records the tree's
+ * node capacity, depth and mass uniformity; every later solve of that source
+ * count reads nothing back to the host and allocates nothing, so it can be
+ * recorded. Capture-safe solves are mixed-precision strict solves without
+ * diagnostics, counters, the nearest-pair lane or a history
+ * (CHUI_ERR_UNSUPPORTED otherwise); their results equal the ordinary solve's
+ * bit for bit. What an ordinary solve reports at once (non-finite inputs, an
+ * active index out of range, a tree deeper than 21 levels), and sources that
+ * outgrow the prepared node capacity, depth (prepared depth + 4) or mass
+ * uniformity (CHUI_ERR_RUNTIME: solve once more outside a graph and record
+ * again), comes back at the next synchronisation. */
+typedef struct chui_treepm_graph chui_treepm_graph;
+chui_status chui_treepm_set_capture_safe(chui_treepm_context* ctx, int32_t enable);
+/* Records everything issued on the context's stream (solves, kicks, drifts)
+ * until graph_end into a graph. The context must be capture-safe, and a solve
+ * of each source count to be recorded must have run before. */
+chui_status chui_treepm_graph_begin(chui_treepm_context* ctx);
+chui_status chui_treepm_graph_end(chui_treepm_context* ctx, chui_treepm_graph** out);
+/* Replays the graph on the stream it was recorded on, after the member
+ * contexts' earlier work; their later work waits for it. Faults of the
+ * replayed solves come back at the members' next synchronisation. */
+chui_status chui_treepm_graph_launch(chui_treepm_graph* graph);
+chui_status chui_treepm_graph_destroy(chui_treepm_graph* graph);
+
 /* ---- force evaluation ---------------------------------------------------- */
 /* Host-pointer drop-in for solveActiveSetWithPmCadence. Positions/masses are
  * double[n]; active = uint32[n_active] source rows or NULL for all rows.
@@ -392,6 +420,14 @@ chui_status chui_treepm_ensemble_box(chui_treepm_ensemble* ensemble, int32_t ind
  * its context's stream. Returns when every box is done. */
 chui_status chui_treepm_ensemble_solve_device(chui_treepm_ensemble* ensemble, const chui_treepm_box* boxes,
                                               int32_t box_count, int32_t refresh_long_range);
+/* Capture-safe mode for every box (see chui_treepm_set_capture_safe). */
+chui_status chui_treepm_ensemble_set_capture_safe(chui_treepm_ensemble* ensemble, int32_t enable);
+/* Records the work issued on every box's stream until graph_end (batched
+ * solves, which are then issued from the calling thread, kicks, drifts) into
+ * one graph in which each box is a branch: box 0's stream records, the others
+ * join it. Every box must be capture-safe and prepared. */
+chui_status chui_treepm_ensemble_graph_begin(chui_treepm_ensemble* ensemble);
+chui_status chui_treepm_ensemble_graph_end(chui_treepm_ensemble* ensemble, chui_treepm_graph** out);
 
 /* Per-stage device timing accumulated over solves with HIP events on the
  * context stream (no extra synchronisation inside a solve).
