@@ -564,12 +564,14 @@ def measure_ensemble(steps, gradients, workers=8):
     tuned = {}
     for n_side in sorted({s.n_side for s in specs}):
         spec = next(s for s in specs if s.n_side == n_side)
-        best, times = E.autotune_leaf(spec)
-        tuned[n_side] = {"leaf": best, "ms_by_leaf": {str(k): round(v, 3) for k, v in times.items()}}
+        tuned[n_side] = E.autotune(spec)
+        log(f"autotune {n_side}^3: leaf {tuned[n_side]['leaf']}, fft lines {tuned[n_side]['fft_lines']}, "
+            f"walk waves {tuned[n_side]['walk_waves']}")
     leaves = {k: v["leaf"] for k, v in tuned.items()}
+    shapes = {k: (v["fft_lines"], v["walk_waves"]) for k, v in tuned.items()}
     particles = sum(s.n for s in specs)
     out = {"boxes": len(specs), "particles": particles, "kdk_steps": steps, "precision": "mixed",
-           "autotuned_leaf": tuned, "meshes": "2 n_side per axis (64, 128, 192, 256)"}
+           "autotuned": tuned, "meshes": "2 n_side per axis (64, 128, 192, 256)"}
     def timed(fn, start, fwd, repeats=3):
         els = []
         for _ in range(repeats):
@@ -582,7 +584,7 @@ def measure_ensemble(steps, gradients, workers=8):
 
     for workers_ in (workers, 1):
         with E.Ensemble(specs, workers=workers_) as ens:
-            fwd = E.EnsembleForward(ens, steps, L.PRECISION_MIXED, leaves)
+            fwd = E.EnsembleForward(ens, steps, L.PRECISION_MIXED, leaves, shape_by_n_side=shapes)
             start = fwd.state()
             ens.solve(fwd.x, fwd.m, fwd.g, fwd.options)  # warm-up: sizes every box's buffers
             fwd.synchronize()
@@ -613,6 +615,7 @@ def measure_ensemble(steps, gradients, workers=8):
     out["worker_speedup"] = out["forward"]["value"] / out["forward_one_worker"]["value"]
     out["graph_speedup"] = out["forward_graph"]["value"] / out["forward"]["value"]
     if gradients:
+        E.box_gradient(specs[0], steps, explicit=True)  # warm-up (tables, allocator)
         t = time.perf_counter()
         nf = nv = 0
         for s in specs:
@@ -620,8 +623,16 @@ def measure_ensemble(steps, gradients, workers=8):
             nf += f
             nv += v
         torch.cuda.synchronize()
-        out["gradients"] = {"seconds": time.perf_counter() - t, "precision": "fp64", "force_evaluations": nf,
-                            "force_vjps": nv, "checkpoint_every": 2}
+        serial = time.perf_counter() - t
+        t = time.perf_counter()
+        E.ensemble_gradients(specs, steps, workers=workers)
+        threaded = time.perf_counter() - t
+        out["gradients"] = {"seconds": threaded, "workers": workers, "serial_seconds": serial,
+                            "precision": "fp64", "force_evaluations": nf, "force_vjps": nv, "checkpoint_every": 2,
+                            "what": f"d loss / d psi of all {len(specs)} boxes; boxes on {workers} host threads, "
+                                    "each with its own torch stream and context (explicit adjoint sweeps); "
+                                    "serial_seconds: one box after another through torch.autograd"}
+        log(f"ensemble gradients: {threaded:.2f} s on {workers} threads, {serial:.2f} s serial")
     return out
 
 

@@ -155,19 +155,58 @@ class KdkProblem:
         return x, v, g
 
 
+def trajectory_forward(psi: torch.Tensor, prob: KdkProblem):
+    """The KDK forward from the displacements psi (3, N) FP64: final positions
+    and the checkpoints {step: (x, v, g)} every prob.checkpoint_every steps."""
+    x = prob.wrap(prob.lattice + psi)
+    v = prob.vel_factor * psi
+    g = prob.forces(x)
+    checkpoints = {0: (x, v, g)}
+    for n in range(prob.steps):
+        x, v, g = prob.kdk_step(n, x, v, g)
+        if (n + 1) % prob.checkpoint_every == 0 and n + 1 < prob.steps:
+            checkpoints[n + 1] = (x, v, g)
+    return x, checkpoints
+
+
+def trajectory_backward(prob: KdkProblem, checkpoints: dict, final: torch.Tensor,
+                        lam_final: torch.Tensor) -> torch.Tensor:
+    """The adjoint sweep: d loss / d psi from lam_final = d loss / d x_final."""
+    S = prob.steps
+    lam_x = lam_final.contiguous().clone()
+    lam_v = torch.zeros_like(lam_x)
+    carry = None
+    starts = sorted(checkpoints)
+    for seg_start in reversed(starts):
+        seg_end = min(seg_start + prob.checkpoint_every, S)
+        # positions x_seg_start .. x_seg_end (recomputed inside the segment);
+        # the reverse sweep needs positions only, so the last recomputed
+        # step drifts without evaluating the force at its new positions
+        x, v, g = checkpoints[seg_start]
+        xs = [x]
+        for n in range(seg_start, seg_end - 1):
+            x, v, g = prob.kdk_step(n, x, v, g, positions_only=n == seg_end - 2)
+            xs.append(x)
+        xs.append(final if seg_end == S else checkpoints[seg_end][0])
+        for n in range(seg_end - 1, seg_start - 1, -1):
+            f = prob.factors[n]
+            lam_g = f["second_kick_factor"] * lam_v
+            if carry is not None:
+                lam_g = lam_g + carry
+            lam_x = lam_x + prob.force_vjp(xs[n + 1 - seg_start], lam_g)
+            lam_vh = f["second_hubble_drag"] * lam_v + f["drift_factor"] * lam_x
+            lam_v = f["first_hubble_drag"] * lam_vh
+            carry = f["first_kick_factor"] * lam_vh
+    lam_x = lam_x + prob.force_vjp(checkpoints[0][0], carry)
+    return lam_x + prob.vel_factor * lam_v
+
+
 class KdkTrajectory(torch.autograd.Function):
     """psi (3, N) FP64 initial displacements -> final positions (3, N)."""
 
     @staticmethod
     def forward(ctx, psi: torch.Tensor, prob: KdkProblem):
-        x = prob.wrap(prob.lattice + psi)
-        v = prob.vel_factor * psi
-        g = prob.forces(x)
-        checkpoints = {0: (x, v, g)}
-        for n in range(prob.steps):
-            x, v, g = prob.kdk_step(n, x, v, g)
-            if (n + 1) % prob.checkpoint_every == 0 and n + 1 < prob.steps:
-                checkpoints[n + 1] = (x, v, g)
+        x, checkpoints = trajectory_forward(psi, prob)
         ctx.prob = prob
         ctx.checkpoints = checkpoints
         ctx.final = x
@@ -175,34 +214,7 @@ class KdkTrajectory(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, lam_final: torch.Tensor):
-        prob = ctx.prob
-        S = prob.steps
-        lam_x = lam_final.contiguous().clone()
-        lam_v = torch.zeros_like(lam_x)
-        carry = None
-        starts = sorted(ctx.checkpoints)
-        for seg_start in reversed(starts):
-            seg_end = min(seg_start + prob.checkpoint_every, S)
-            # positions x_seg_start .. x_seg_end (recomputed inside the segment);
-            # the reverse sweep needs positions only, so the last recomputed
-            # step drifts without evaluating the force at its new positions
-            x, v, g = ctx.checkpoints[seg_start]
-            xs = [x]
-            for n in range(seg_start, seg_end - 1):
-                x, v, g = prob.kdk_step(n, x, v, g, positions_only=n == seg_end - 2)
-                xs.append(x)
-            xs.append(ctx.final if seg_end == S else ctx.checkpoints[seg_end][0])
-            for n in range(seg_end - 1, seg_start - 1, -1):
-                f = prob.factors[n]
-                lam_g = f["second_kick_factor"] * lam_v
-                if carry is not None:
-                    lam_g = lam_g + carry
-                lam_x = lam_x + prob.force_vjp(xs[n + 1 - seg_start], lam_g)
-                lam_vh = f["second_hubble_drag"] * lam_v + f["drift_factor"] * lam_x
-                lam_v = f["first_hubble_drag"] * lam_vh
-                carry = f["first_kick_factor"] * lam_vh
-        lam_x = lam_x + prob.force_vjp(ctx.checkpoints[0][0], carry)
-        return lam_x + prob.vel_factor * lam_v, None
+        return trajectory_backward(ctx.prob, ctx.checkpoints, ctx.final, lam_final), None
 
 
 class CicDensity(torch.autograd.Function):
@@ -225,6 +237,20 @@ def density_loss(psi: torch.Tensor, target: torch.Tensor, prob: KdkProblem) -> t
     x = KdkTrajectory.apply(psi.to(torch.float64), prob)
     delta = CicDensity.apply(x, prob)
     return ((delta - target) ** 2).sum()
+
+
+def loss_and_gradient(psi: torch.Tensor, target: torch.Tensor, prob: KdkProblem):
+    """density_loss and d loss / d psi by the same forward and adjoint sweeps
+    as the autograd Functions, run on the calling thread and its current
+    stream without the autograd engine (whose device thread would serialise
+    the backward passes of boxes differentiated from several host threads)."""
+    with torch.no_grad():
+        x, checkpoints = trajectory_forward(psi.to(torch.float64), prob)
+        diff = prob.cic_density(x) - target
+        loss = (diff * diff).sum()
+        lam_final = prob.cic_density_vjp(x, 2.0 * diff)
+        grad = trajectory_backward(prob, checkpoints, x, lam_final)
+    return loss, grad
 
 
 def central_differences(prob: KdkProblem, psi: torch.Tensor, target: torch.Tensor, grad: torch.Tensor,

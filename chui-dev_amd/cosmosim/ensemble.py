@@ -1,5 +1,5 @@
 """Ensembles of independent periodic boxes: workload law, batched KDK forward,
-per-box gradients and the leaf-size autotuner.
+per-box gradients and the autotuner of leaf size and launch shapes.
 
 The reference evolves one box per run; ensembles have no counterpart there.
 Each box follows the single-box path (TreePmCoordinator::solveActiveSet,
@@ -135,10 +135,24 @@ def box_options(spec: BoxSpec, precision: int, leaf: int = 16):
                                 max_leaf_size=leaf, precision=precision)
 
 
-def autotune_leaf(spec: BoxSpec, candidates=(8, 12, 16, 24, 32), repeats: int = 3, device: str = "cuda:0"):
-    """Leaf size with the fastest mixed-precision force evaluation of one box
-    of this size (its own initial conditions), among candidates. Returns
-    (best leaf, {leaf: ms})."""
+def _median(v):
+    return float(np.median(np.asarray(v, dtype=np.float64)))
+
+
+def autotune(spec: BoxSpec, rounds: int = 5, repeats: int = 3, device: str = "cuda:0",
+             leaves=(8, 12, 16, 24, 32), fft_lines=(0, 8, 4), walk_waves=(4, 8, 2)):
+    """Launch parameters of one box of this size (its own initial conditions,
+    mixed-precision force evaluations): leaf size, lines per block of the PM
+    line passes and waves per block of the timed walk, tuned one after the
+    other with the others at their current choice. Each candidate is timed in
+    `rounds` rounds of `repeats` solves, the candidates interleaved within a
+    round so that a drift of the clock hits all of them alike. A candidate
+    replaces the current choice only when its median beats the current
+    choice's median by more than the noise: twice the largest relative spread
+    (median absolute deviation over median) seen among the candidates, and at
+    least 2%. The first entry of each list is the default. Returns
+    {"leaf", "fft_lines", "walk_waves", "steps": [per knob: medians, noise,
+    choice]}."""
     lattice, psi = box_initial_conditions(spec)
     pos = np.mod(lattice + psi, spec.box)
     pos[pos >= spec.box] = 0.0
@@ -149,31 +163,55 @@ def autotune_leaf(spec: BoxSpec, candidates=(8, 12, 16, 24, 32), repeats: int = 
     torch.cuda.synchronize(device)  # the library runs on its own stream
     from .treepm import ForceAccumulator
 
-    times = {}
+    choice = {"leaf": 16 if 16 in leaves else leaves[0], "fft_lines": fft_lines[0], "walk_waves": walk_waves[0]}
+    steps = []
     with TreePmCoordinator(spec.mesh) as coord:
-        for leaf in candidates:
-            o = box_options(spec, _lib.PRECISION_MIXED, leaf)
+        def sample(cfg):
+            coord.set_launch_shape(cfg["fft_lines"], cfg["walk_waves"])
+            o = box_options(spec, _lib.PRECISION_MIXED, cfg["leaf"])
             coord.solve_active_set(*dev, m, ForceAccumulator(acc), o)
             coord.synchronize()
             t = time.perf_counter()
             for _ in range(repeats):
                 coord.solve_active_set(*dev, m, ForceAccumulator(acc), o)
             coord.synchronize()
-            times[leaf] = 1e3 * (time.perf_counter() - t) / repeats
-    best = min(times, key=times.get)
-    return best, times
+            return 1e3 * (time.perf_counter() - t) / repeats
+
+        for knob, cands in (("leaf", leaves), ("fft_lines", fft_lines), ("walk_waves", walk_waves)):
+            cands = list(dict.fromkeys([choice[knob]] + list(cands)))  # the current choice first
+            times = {c: [] for c in cands}
+            for _ in range(rounds):
+                for c in cands:
+                    times[c].append(sample({**choice, knob: c}))
+            med = {c: _median(v) for c, v in times.items()}
+            spread = max(_median(np.abs(np.asarray(v) - med[c])) / med[c] for c, v in times.items())
+            noise = max(0.02, 2.0 * spread)
+            best = min(med, key=med.get)
+            current = choice[knob]
+            if med[best] < med[current] * (1.0 - noise):
+                choice[knob] = best
+            steps.append({"knob": knob, "ms_median": {str(c): round(v, 4) for c, v in med.items()},
+                          "noise": round(noise, 4), "choice": choice[knob]})
+        coord.set_launch_shape(0, 0)
+    return {**choice, "steps": steps}
 
 
 class EnsembleForward:
     """Batched KDK forward of every box (steps uniform in a from a_begin to a_end)."""
 
-    def __init__(self, ens: Ensemble, steps: int, precision: int, leaf_by_n_side=None, device: str = "cuda:0"):
+    def __init__(self, ens: Ensemble, steps: int, precision: int, leaf_by_n_side=None, device: str = "cuda:0",
+                 shape_by_n_side=None):
+        """leaf_by_n_side: {n_side: leaf size}; shape_by_n_side: {n_side:
+        (fft lines per block, walk waves per block)} (autotune's choices)."""
         self.ens = ens
         self.steps = steps
         self.precision = precision
         self.device = device
         leaf_by_n_side = leaf_by_n_side or {}
         self.options = [box_options(s, precision, leaf_by_n_side.get(s.n_side, 16)) for s in ens.specs]
+        for b, s in enumerate(ens.specs):
+            if shape_by_n_side and s.n_side in shape_by_n_side:
+                ens.views[b].set_launch_shape(*shape_by_n_side[s.n_side])
         self.factors = []
         for s in ens.specs:
             a = np.linspace(s.a_begin, s.a_end, steps + 1)
@@ -254,10 +292,12 @@ class EnsembleForward:
         return graph
 
 
-def box_gradient(spec: BoxSpec, steps: int, device: str = "cuda:0", checkpoint_every: int = 2):
+def box_gradient(spec: BoxSpec, steps: int, device: str = "cuda:0", checkpoint_every: int = 2,
+                 explicit: bool = False):
     """d loss / d psi of one box: loss = sum of squared differences between the
     final CIC density contrast (n_side^3 grid) and the CIC contrast of a second
-    seeded displacement of ten times the RMS. FP64 forward and adjoint.
+    seeded displacement of ten times the RMS. FP64 forward and adjoint, through
+    torch.autograd or (explicit) the same sweeps on the calling thread.
     Returns (loss, gradient (3, N) on the device, force evaluations, VJPs)."""
     lattice, psi = box_initial_conditions(spec)
     del lattice
@@ -269,13 +309,42 @@ def box_gradient(spec: BoxSpec, steps: int, device: str = "cuda:0", checkpoint_e
         tdisp = ics.zeldovich_displacement(spec.n_side, spec.box, spec.spectral_index, 1.0,
                                            np.random.default_rng(spec.seed + 1))
         target = prob.cic_density(prob.wrap(prob.lattice + torch.tensor(tdisp, dtype=torch.float64, device=device)))
-        p = torch.tensor(psi, dtype=torch.float64, device=device).requires_grad_(True)
-        loss = adjoint.density_loss(p, target, prob)
-        (grad,) = torch.autograd.grad(loss, p)
+        if explicit:
+            p = torch.tensor(psi, dtype=torch.float64, device=device)
+            loss, grad = adjoint.loss_and_gradient(p, target, prob)
+        else:
+            p = torch.tensor(psi, dtype=torch.float64, device=device).requires_grad_(True)
+            loss = adjoint.density_loss(p, target, prob)
+            (grad,) = torch.autograd.grad(loss, p)
+        torch.cuda.current_stream(device).synchronize()
         return float(loss.detach()), grad.detach(), prob.force_evaluations, prob.vjp_evaluations
     finally:
         prob.close()
 
 
-__all__ = ["BoxSpec", "Ensemble", "EnsembleForward", "autotune_leaf", "box_gradient", "box_initial_conditions",
-           "box_options", "ensemble_specs"]
+def ensemble_gradients(specs, steps: int, workers: int = 8, device: str = "cuda:0", checkpoint_every: int = 2):
+    """box_gradient of every box, the boxes handed to `workers` host threads
+    largest first; each thread differentiates its boxes on a torch stream of
+    its own (default priority) with the explicit adjoint sweeps, so the boxes'
+    force evaluations, VJPs and torch work overlap on the device. Returns the
+    box_gradient results in the order of specs."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    order = sorted(range(len(specs)), key=lambda b: specs[b].n, reverse=True)
+    results = [None] * len(specs)
+
+    def work(b):
+        stream = torch.cuda.Stream(device=device)
+        with torch.cuda.stream(stream):
+            results[b] = box_gradient(specs[b], steps, device=device, checkpoint_every=checkpoint_every,
+                                      explicit=True)
+
+    with ThreadPoolExecutor(max_workers=max(1, int(workers))) as pool:
+        for f in [pool.submit(work, b) for b in order]:
+            f.result()
+    torch.cuda.synchronize(device)
+    return results
+
+
+__all__ = ["BoxSpec", "Ensemble", "EnsembleForward", "autotune", "box_gradient", "box_initial_conditions",
+           "box_options", "ensemble_gradients", "ensemble_specs"]

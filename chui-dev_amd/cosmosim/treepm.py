@@ -120,6 +120,12 @@ class TreePmCoordinator:
     def set_stream(self, stream_handle: int | None):
         _lib.check(self._lib.chui_treepm_set_stream(self._h, C.c_void_p(stream_handle) if stream_handle else None))
 
+    def set_launch_shape(self, fft_lines_per_block: int = 0, walk_waves_per_block: int = 0):
+        """Lines per block of the PM line passes and waves per block of the
+        timed mixed walk (0 = built-in; results do not depend on them)."""
+        _lib.check(self._lib.chui_treepm_set_launch_shape(self._h, int(fft_lines_per_block),
+                                                          int(walk_waves_per_block)))
+
     # -- capture-safe solves and HIP graphs (no reference counterpart) ---------
     def set_capture_safe(self, enable: bool = True):
         """Capture-safe mode: after one solve of a source count, solves of that

@@ -91,6 +91,11 @@ struct chui_treepm_context {
   // graph; `capturing` while the context's stream records one.
   bool capture_safe = false;
   bool capturing = false;
+  // launch shapes (chui_treepm_set_launch_shape, the autotuner's knobs):
+  // lines per block of the PM line passes and waves per block of the timed
+  // mixed walk; 0 = the built-in choice
+  int fft_lines = 0;
+  int walk_waves = 0;
 };
 
 namespace {
@@ -378,6 +383,7 @@ void harvest(Ctx* c) {
 // mixed fields that depend only on the options, the split and the mesh).
 chui::WalkParams base_walk_params(const Ctx* c, const chui_treepm_options* o, const Split& sp, std::size_t mesh_row) {
   chui::WalkParams wp{};
+  wp.walk_waves = c->walk_waves > 0 ? c->walk_waves : 4;
   for (int a = 0; a < 3; ++a) {
     wp.L[a] = o->box_size[a];
     wp.mesh[a] = c->mesh[a];
@@ -592,6 +598,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   po.split_scale = sp.a;
   po.assignment = o->assignment_scheme;
   po.deconvolution = o->window_deconvolution;
+  po.fft_lines = c->fft_lines;
   chui::PmWorkspace& pw = mixed ? c->pm32 : c->pm64;
   const double key[8] = {o->box_size[0], o->box_size[1], o->box_size[2], o->gravitational_constant, sp.a,
                          static_cast<double>(o->assignment_scheme), static_cast<double>(o->window_deconvolution),
@@ -1408,6 +1415,21 @@ chui_status chui_treepm_synchronize(chui_treepm_context* c) {
   });
 }
 
+chui_status chui_treepm_set_launch_shape(chui_treepm_context* c, int32_t fft_lines_per_block,
+                                        int32_t walk_waves_per_block) {
+  return guarded([&] {
+    require_ctx(c);
+    if (c->capturing) throw std::logic_error("launch shapes cannot change while a graph is recorded");
+    const int fl = fft_lines_per_block, ww = walk_waves_per_block;
+    if (!(fl == 0 || fl == 1 || fl == 2 || fl == 4 || fl == 8 || fl == 16))
+      throw std::invalid_argument("fft_lines_per_block must be 0 or a power of two up to 16");
+    if (!(ww == 0 || ww == 2 || ww == 4 || ww == 8))
+      throw std::invalid_argument("walk_waves_per_block must be 0, 2, 4 or 8");
+    c->fft_lines = fl;
+    c->walk_waves = ww;
+  });
+}
+
 // ---- capture-safe solves and HIP graphs ------------------------------------
 chui_status chui_treepm_set_capture_safe(chui_treepm_context* c, int32_t enable) {
   return guarded([&] {
@@ -1529,6 +1551,7 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
   po.split_scale = sp.a;
   po.assignment = o->assignment_scheme;
   po.deconvolution = o->window_deconvolution;
+  po.fft_lines = c->fft_lines;
   chui::DepositTiling dt{};
   for (int a = 0; a < 3; ++a) {
     dt.n[a] = g.n[a];

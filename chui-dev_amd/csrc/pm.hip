@@ -694,6 +694,7 @@ template <class T>
 struct Tables {
   T *kx, *ky, *kz, *dkx, *dky, *dkz, *wx, *wy, *wz;
   typename C2<T>::type *twx, *twy, *twz;
+  int lines = 0;  // lines per block cap of the line passes (PmOptions::fft_lines)
 };
 
 template <class T>
@@ -708,7 +709,7 @@ Tables<T> upload_tables(PmWorkspace& w, const PmGeometry& g, const PmOptions& o,
     V* tb = reinterpret_cast<V*>(base + off[9]);
     return Tables<T>{base + off[0], base + off[1], base + off[2], base + off[3], base + off[4], base + off[5],
                      base + off[6], base + off[7], base + off[8], tb + off[10], tb + off[10] + nx,
-                     tb + off[10] + nx + ny};
+                     tb + off[10] + nx + ny, o.fft_lines};
   };
   if (w.tables.ptr && std::equal(key, key + 8, w.table_key)) return tables_at(w.table_off);
   std::vector<T> host;
@@ -812,6 +813,7 @@ void launch_lines(const typename C2<T>::type* in, typename C2<T>::type* out, Lin
                   const typename C2<T>::type* tw, const T* dk, const Tables<T>& tb, hipStream_t s) {
   using V = typename C2<T>::type;
   a.tj = lines_per_block<T>(a.n);
+  if (tb.lines > 0 && tb.lines < a.tj) a.tj = tb.lines;  // a power of two below the LDS-bound maximum
   const std::size_t smem =
       static_cast<std::size_t>(a.tj) * lds_line_stride(a.n, a.lg >= 0) * sizeof(V) + fft_tw_len(a.n, a.lg) * sizeof(V);
   if (smem > 64 * 1024) {

@@ -21,6 +21,7 @@ struct PmOptions {
   double split_scale = 0.0;
   int assignment = 0;      // 0 CIC, 1 TSC
   int deconvolution = 0;
+  int fft_lines = 0;       // lines per block of the line passes (0: the most the LDS budget allows)
 };
 
 // Device mesh workspace for one precision. Three complex buffers; the real

@@ -1523,8 +1523,8 @@ __device__ __forceinline__ float4 lds_cell(uint32_t base, uint32_t cell) {
 // loop stays rolled (see lds_cell); the unroller's warning about it is muted.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Wpass-failed"
-template <int SCHEME, bool CUBIC, bool COUNT, bool BOX = false, bool LOG = false>
-__global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : 7)))
+template <int SCHEME, bool CUBIC, bool COUNT, bool BOX = false, bool LOG = false, int WAVES = kWalkWaves>
+__global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : 7)))
 void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long long* __restrict__ counters,
                     const NodeLink* __restrict__ node_link, const NodeGeo32* __restrict__ node_geo,
                     const NodeMom32* __restrict__ node_mom, const NodeQuad32* __restrict__ node_quad,
@@ -1532,22 +1532,22 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
   // node records are read-only for the kernel's lifetime and indexed by the
   // wave-uniform node number, so they come through the scalar cache
   // the box walk's window holds up to 127 compacted records between evaluations
-  __shared__ float4 win[kWalkWaves][BOX ? 2 * kWindow : kWindow];
+  __shared__ float4 win[WAVES][BOX ? 2 * kWindow : kWindow];
   __shared__ float4 htab[LOG ? 1 : kHTableCells];
-  __shared__ uint32_t list[kWalkWaves][BOX ? kWindow : 1];  // box walk: listed source indices
+  __shared__ uint32_t list[WAVES][BOX ? kWindow : 1];  // box walk: listed source indices
   // per lane: force totals [0..2] and their Kahan compensations [3..5]
-  __shared__ float ksum[kWalkWaves][6][kWave];
+  __shared__ float ksum[WAVES][6][kWave];
   extern __shared__ float4 ltab[];  // LOG: P.log_cells cells
   if constexpr (LOG) {
-    for (int i = threadIdx.x; i < static_cast<int>(P.log_cells); i += kWalkThreads) ltab[i] = D.ltab[i];
+    for (int i = threadIdx.x; i < static_cast<int>(P.log_cells); i += (WAVES * kWave)) ltab[i] = D.ltab[i];
   } else {
-    for (int i = threadIdx.x; i < kHTableCells; i += kWalkThreads) htab[i] = D.htab[i];
+    for (int i = threadIdx.x; i < kHTableCells; i += (WAVES * kWave)) htab[i] = D.htab[i];
   }
   __syncthreads();
   const uint32_t node_count = D.node_count_dev ? *D.node_count_dev : P.node_count;
   const int w = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
-  const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
+  const uint32_t k = xcd_logical_block() * (WAVES * kWave) + threadIdx.x;
   const bool valid = k < P.n_targets;
   const uint32_t tp = valid ? (P.identity_targets ? k : D.tpos[k]) : 0u;
   const uint32_t slot = valid ? (P.identity_targets ? D.order[k] : D.tslot[k]) : 0u;
@@ -2885,6 +2885,14 @@ void launchmix(const WalkParams& p, const WalkData& d, const WalkOutputs& out, u
         p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
   else if (p.target_per_wave)
     walkmix_target_kernel<SCHEME, CUBIC><<<grid_for(p.n_targets, kWalkWaves), kWalkThreads, 0, s>>>(p, d, out);
+  else if (p.log_cells > 0 && p.walk_waves == 8)
+    walkmix_kernel<SCHEME, CUBIC, false, true, true, 8>
+        <<<grid_for(p.n_targets, 8 * kWave), 8 * kWave, p.log_cells * sizeof(float4), s>>>(
+            p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
+  else if (p.log_cells > 0 && p.walk_waves == 2)
+    walkmix_kernel<SCHEME, CUBIC, false, true, true, 2>
+        <<<grid_for(p.n_targets, 2 * kWave), 2 * kWave, p.log_cells * sizeof(float4), s>>>(
+            p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
   else if (p.log_cells > 0)
     walkmix_kernel<SCHEME, CUBIC, false, true, true>
         <<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, p.log_cells * sizeof(float4), s>>>(

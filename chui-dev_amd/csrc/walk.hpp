@@ -23,6 +23,7 @@ struct WalkParams {
   uint32_t node_count;
   uint32_t n_targets;
   int target_per_wave;   // mixed strict walk: one target per wave (walkmix_target_kernel)
+  int walk_waves;        // waves per block of the timed mixed walk (2, 4 or 8; the autotuner's knob)
   int count;             // accumulate walk counters (diagnostic launches)
   int identity_targets;  // targets are tree positions 0..n-1, output slot = order[k]
   float uniform_mass_f;  // mixed: mass when uniform

@@ -211,6 +211,13 @@ chui_status chui_treepm_destroy(chui_treepm_context* ctx);
 /* Stream the context enqueues on (hipStream_t; NULL = library-owned stream). */
 chui_status chui_treepm_set_stream(chui_treepm_context* ctx, void* stream);
 chui_status chui_treepm_synchronize(chui_treepm_context* ctx);
+/* Launch shapes (no reference counterpart; the knobs of the ensemble's
+ * autotuner): lines per block of the PM line passes (0 = the most the LDS
+ * budget allows, else 1, 2, 4, 8 or 16, capped at that maximum) and waves per
+ * block of the timed mixed walk (0 = 4, else 2, 4 or 8). Results do not
+ * depend on them. */
+chui_status chui_treepm_set_launch_shape(chui_treepm_context* ctx, int32_t fft_lines_per_block,
+                                         int32_t walk_waves_per_block);
 
 /* ---- capture-safe solves and HIP graphs ----------------------------------
  * No counterpart in the reference (its solves are driven from the host). For

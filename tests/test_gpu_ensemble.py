@@ -98,3 +98,63 @@ def test_box_gradient_matches_central_differences():
         assert abs(fd - an) <= 1e-4 * abs(an), (fd, an)
     finally:
         prob.close()
+
+
+def test_explicit_adjoint_sweeps_equal_autograd():
+    # the sweeps without the autograd engine reproduce torch.autograd's
+    # gradient of the same loss (same operations; PM atomics: round-off)
+    s = SPECS[0]
+    loss_a, grad_a, nf_a, nv_a = E.box_gradient(s, 3)
+    loss_e, grad_e, nf_e, nv_e = E.box_gradient(s, 3, explicit=True)
+    assert (nf_a, nv_a) == (nf_e, nv_e) == (4, 4)
+    assert abs(loss_e - loss_a) <= 1e-12 * abs(loss_a)
+    assert rel_l2(grad_e.cpu().numpy(), grad_a.cpu().numpy()) < 1e-11
+
+
+def test_threaded_ensemble_gradients_equal_serial():
+    # boxes differentiated on four host threads, each with its own torch stream
+    serial = [E.box_gradient(s, 2, explicit=True) for s in SPECS]
+    threaded = E.ensemble_gradients(SPECS, 2, workers=4)
+    for (l0, g0, f0, v0), (l1, g1, f1, v1), s in zip(serial, threaded, SPECS):
+        assert (f0, v0) == (f1, v1)
+        assert abs(l1 - l0) <= 1e-12 * abs(l0), s
+        assert rel_l2(g1.cpu().numpy(), g0.cpu().numpy()) < 1e-11, s
+
+
+def test_launch_shapes_leave_results_unchanged():
+    # lines per block of the line passes and waves per block of the timed walk
+    # change how the work is cut, not what is computed
+    s = SPECS[-1]
+    lattice, psi = E.box_initial_conditions(s)
+    pos = np.mod(lattice + psi, s.box)
+    pos[pos >= s.box] = 0.0
+    dev = [torch.from_numpy(np.ascontiguousarray(p)).to("cuda:0") for p in pos]
+    m = torch.full((s.n,), 1.0, dtype=torch.float64, device="cuda:0")
+    o = E.box_options(s, L.PRECISION_MIXED)
+    out = {}
+    with TreePmCoordinator(s.mesh) as coord:
+        for shape in ((0, 0), (4, 2), (8, 8), (1, 4)):
+            coord.set_launch_shape(*shape)
+            acc = tuple(torch.empty(s.n, dtype=torch.float32, device="cuda:0") for _ in range(3))
+            sr = tuple(torch.zeros(s.n, dtype=torch.float32, device="cuda:0") for _ in range(3))
+            coord.solve_active_set(*dev, m, ForceAccumulator(acc, None, None, sr), o)
+            coord.synchronize()
+            out[shape] = (np.stack([a.cpu().numpy() for a in acc]), np.stack([a.cpu().numpy() for a in sr]))
+        with pytest.raises(L.ChuiError):
+            coord.set_launch_shape(3, 0)
+        with pytest.raises(L.ChuiError):
+            coord.set_launch_shape(0, 5)
+    ref_acc, ref_sr = out[(0, 0)]
+    for shape, (acc, sr) in out.items():
+        # per-target walk sums do not depend on the block size; PM by round-off
+        assert np.array_equal(sr, ref_sr), shape
+        assert rel_l2(acc, ref_acc) < 1e-6, shape
+
+
+def test_autotune_picks_from_its_candidates():
+    s = SPECS[0]
+    r = E.autotune(s, rounds=2, repeats=1, leaves=(16, 32), fft_lines=(0, 4), walk_waves=(4, 2))
+    assert r["leaf"] in (16, 32) and r["fft_lines"] in (0, 4) and r["walk_waves"] in (4, 2)
+    assert [st["knob"] for st in r["steps"]] == ["leaf", "fft_lines", "walk_waves"]
+    for st in r["steps"]:
+        assert st["noise"] >= 0.02
