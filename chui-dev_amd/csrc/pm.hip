@@ -27,7 +27,13 @@ constexpr int kFftThreads = 256;
 // Butterflies a thread keeps in registers per pass (float2: 4 x 8 points,
 // double2: 2 x 8 points); blocks are sized so that lines * n / R fits.
 template <class T> constexpr int fft_max_butterflies() { return sizeof(T) == 4 ? 4 : 2; }
-constexpr int kFftLoadBatch = 8;  // global loads a thread issues before its LDS stores
+// Global loads a thread issues before its LDS stores. A block's whole tile is
+// 16 lines of 512 points (FP32) or 8 lines (FP64) over 256 threads, so all of
+// a thread's loads are in flight at once: about 64 KB per CU with two blocks,
+// which HBM3E needs to approach its rate (8 loads per thread reached 2.3 TB/s).
+template <class T> constexpr int fft_load_batch() { return sizeof(T) == 4 ? 32 : 16; }
+// The z passes keep two values per load (real row pairs), so half of that.
+template <class T> constexpr int fft_z_load_batch() { return sizeof(T) == 4 ? 16 : 8; }
 
 // Smallest radix of the pass plan of lds_fft for a line of 2^lg points.
 inline int fft_min_radix(int lg) {
@@ -491,10 +497,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
   for (int i = threadIdx.x; i < twn; i += blockDim.x) tw[i] = twz[i];
   const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
   const int npts = pairs_per_block * nz;
-  for (int base = threadIdx.x; base < npts; base += kFftLoadBatch * kFftThreads) {
-    V v[kFftLoadBatch];
+  for (int base = threadIdx.x; base < npts; base += fft_z_load_batch<T>() * kFftThreads) {
+    V v[fft_z_load_batch<T>()];
 #pragma unroll
-    for (int u = 0; u < kFftLoadBatch; ++u) {  // all loads in flight before the LDS stores
+    for (int u = 0; u < fft_z_load_batch<T>(); ++u) {  // all loads in flight before the LDS stores
       const int idx = base + u * kFftThreads;
       v[u] = mk<V, T>(T(0), T(0));
       int pp, i;
@@ -507,7 +513,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_r2c_kernel(T* data, std::si
       }
     }
 #pragma unroll
-    for (int u = 0; u < kFftLoadBatch; ++u) {
+    for (int u = 0; u < fft_z_load_batch<T>(); ++u) {
       const int idx = base + u * kFftThreads;
       if (idx < npts) {
         int pp, i;
@@ -552,10 +558,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::si
   for (int i = threadIdx.x; i < twn; i += blockDim.x) tw[i] = twz[i];
   const std::size_t pair0 = static_cast<std::size_t>(blockIdx.x) * pairs_per_block;
   const int npts = pairs_per_block * nzc;
-  for (int base = threadIdx.x; base < npts; base += kFftLoadBatch * kFftThreads) {
-    V va[kFftLoadBatch], vb[kFftLoadBatch];
+  for (int base = threadIdx.x; base < npts; base += fft_z_load_batch<T>() * kFftThreads) {
+    V va[fft_z_load_batch<T>()], vb[fft_z_load_batch<T>()];
 #pragma unroll
-    for (int u = 0; u < kFftLoadBatch; ++u) {  // all loads in flight before the LDS stores
+    for (int u = 0; u < fft_z_load_batch<T>(); ++u) {  // all loads in flight before the LDS stores
       const int idx = base + u * kFftThreads;
       va[u] = mk<V, T>(T(0), T(0));
       vb[u] = mk<V, T>(T(0), T(0));
@@ -567,7 +573,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_c2r_kernel(T* data, std::si
       }
     }
 #pragma unroll
-    for (int u = 0; u < kFftLoadBatch; ++u) {
+    for (int u = 0; u < fft_z_load_batch<T>(); ++u) {
       const int idx = base + u * kFftThreads;
       if (idx >= npts) continue;
       const int pp = idx / nzc, k = idx - pp * nzc;
@@ -635,10 +641,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_lines_kernel(const typename C
   int tjl = 0;
   while ((1 << tjl) < a.tj) ++tjl;
   const int npts = a.tj * a.n;
-  for (int base = threadIdx.x; base < npts; base += kFftLoadBatch * kFftThreads) {
-    V v[kFftLoadBatch];
+  for (int base = threadIdx.x; base < npts; base += fft_load_batch<T>() * kFftThreads) {
+    V v[fft_load_batch<T>()];
 #pragma unroll
-    for (int u = 0; u < kFftLoadBatch; ++u) {  // all loads in flight before the LDS stores
+    for (int u = 0; u < fft_load_batch<T>(); ++u) {  // all loads in flight before the LDS stores
       const int idx = base + u * kFftThreads;
       const int jj = idx & tjm, i = idx >> tjl;
       const std::size_t j = j0 + jj;
@@ -646,7 +652,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_lines_kernel(const typename C
       if (idx < npts && j < a.inner_count) v[u] = in[obase + j + static_cast<std::size_t>(i) * a.line_stride];
     }
 #pragma unroll
-    for (int u = 0; u < kFftLoadBatch; ++u) {
+    for (int u = 0; u < fft_load_batch<T>(); ++u) {
       const int idx = base + u * kFftThreads;
       if (idx >= npts) continue;
       const int jj = idx & tjm, i = idx >> tjl;
