@@ -235,6 +235,7 @@ SIGNATURES = {
     "chui_treepm_ensemble_graph_begin": [_P],
     "chui_treepm_ensemble_graph_end": [_P, C.POINTER(_P)],
     "chui_treepm_set_capture_safe": [_P, _I32],
+    "chui_treepm_fault_check": [_P],
     "chui_treepm_set_launch_shape": [_P, _I32, _I32],
     "chui_treepm_graph_begin": [_P],
     "chui_treepm_graph_end": [_P, C.POINTER(_P)],

@@ -67,6 +67,9 @@ class KdkProblem:
     # inside r_cut and accepted as many nodes.
     record_pair_sets: bool = False
     pair_sets: list = field(default_factory=list)
+    # Set while a graph is recorded: the library then enqueues on torch's
+    # stream (the caller set it with coord.set_stream), so no call waits.
+    graph_mode: bool = False
 
     def __post_init__(self):
         n = self.n_side**3
@@ -100,6 +103,9 @@ class KdkProblem:
     # The library runs on its own stream: each call waits for torch's queued
     # work on its inputs and is waited for before torch reads its outputs.
     def _call(self, fn, *args):
+        if self.graph_mode:
+            fn(*args)
+            return
         torch.cuda.current_stream(self.device).synchronize()
         fn(*args)
         self.coord.synchronize()
@@ -239,17 +245,26 @@ def density_loss(psi: torch.Tensor, target: torch.Tensor, prob: KdkProblem) -> t
     return ((delta - target) ** 2).sum()
 
 
+def gradient_sweeps(psi: torch.Tensor, target: torch.Tensor, prob: KdkProblem):
+    """The forward and adjoint sweeps of density_loss without its final sum:
+    (delta_final - target, d loss / d psi). Element-wise torch work and library
+    calls only, so the sweeps can be recorded into a graph."""
+    with torch.no_grad():
+        x, checkpoints = trajectory_forward(psi.to(torch.float64), prob)
+        diff = prob.cic_density(x) - target
+        lam_final = prob.cic_density_vjp(x, 2.0 * diff)
+        grad = trajectory_backward(prob, checkpoints, x, lam_final)
+    return diff, grad
+
+
 def loss_and_gradient(psi: torch.Tensor, target: torch.Tensor, prob: KdkProblem):
     """density_loss and d loss / d psi by the same forward and adjoint sweeps
     as the autograd Functions, run on the calling thread and its current
     stream without the autograd engine (whose device thread would serialise
     the backward passes of boxes differentiated from several host threads)."""
+    diff, grad = gradient_sweeps(psi, target, prob)
     with torch.no_grad():
-        x, checkpoints = trajectory_forward(psi.to(torch.float64), prob)
-        diff = prob.cic_density(x) - target
         loss = (diff * diff).sum()
-        lam_final = prob.cic_density_vjp(x, 2.0 * diff)
-        grad = trajectory_backward(prob, checkpoints, x, lam_final)
     return loss, grad
 
 

@@ -292,6 +292,87 @@ class EnsembleForward:
         return graph
 
 
+def box_problem(spec: BoxSpec, steps: int, device: str = "cuda:0", checkpoint_every: int = 2):
+    """The differentiable problem of one box, its initial displacements psi
+    (3, N) and its loss target (the CIC contrast of a second seeded
+    displacement of ten times the RMS)."""
+    lattice, psi = box_initial_conditions(spec)
+    del lattice
+    prob = adjoint.KdkProblem(n_side=spec.n_side, box=spec.box, mesh=spec.mesh,
+                              scale_factors=list(np.linspace(spec.a_begin, spec.a_end, steps + 1)),
+                              softening=spec.box / spec.n_side / 30.0, loss_mesh=spec.n_side,
+                              checkpoint_every=checkpoint_every, device=device)
+    tdisp = ics.zeldovich_displacement(spec.n_side, spec.box, spec.spectral_index, 1.0,
+                                       np.random.default_rng(spec.seed + 1))
+    target = prob.cic_density(prob.wrap(prob.lattice + torch.tensor(tdisp, dtype=torch.float64, device=device)))
+    return prob, torch.tensor(psi, dtype=torch.float64, device=device), target
+
+
+class GradientGraph:
+    """d loss / d psi of several boxes recorded as one HIP graph with
+    torch.cuda.graph: each box's forward and adjoint sweeps (adjoint.
+    gradient_sweeps: the FP64 solves and VJPs of its own capture-safe context,
+    the CIC density and its transpose, the element-wise torch work of the KDK
+    steps) form a branch on a stream of its own, forked from and joined into
+    the recording stream. Two eager gradients first prepare every context
+    (the first solve sizes it, the first VJP learns its accepted-list
+    capacity). replay() copies new displacements into the graph's inputs,
+    launches it once, checks every context's fault word and returns the
+    losses (summed outside the graph) and the gradients."""
+
+    def __init__(self, specs, steps: int, device: str = "cuda:0", checkpoint_every: int = 2):
+        self.device = device
+        made = [box_problem(s, steps, device, checkpoint_every) for s in specs]
+        self.problems = [m[0] for m in made]
+        self.psi = [m[1] for m in made]
+        self.targets = [m[2] for m in made]
+        for prob in self.problems:
+            prob.coord.set_capture_safe(True)
+        for _ in range(2):
+            for prob, psi, target in zip(self.problems, self.psi, self.targets):
+                adjoint.loss_and_gradient(psi, target, prob)
+        torch.cuda.synchronize(device)
+        self.main = torch.cuda.Stream(device=device)
+        self.streams = [torch.cuda.Stream(device=device) for _ in specs]
+        for prob, st in zip(self.problems, self.streams):
+            prob.coord.set_stream(st.cuda_stream)
+            prob.graph_mode = True
+        self.graph = torch.cuda.CUDAGraph()
+        self.diff, self.grad = [None] * len(specs), [None] * len(specs)
+        try:
+            with torch.cuda.graph(self.graph, stream=self.main):
+                for b, (prob, st) in enumerate(zip(self.problems, self.streams)):
+                    st.wait_stream(self.main)
+                    with torch.cuda.stream(st):
+                        self.diff[b], self.grad[b] = adjoint.gradient_sweeps(self.psi[b], self.targets[b], prob)
+                for st in self.streams:
+                    self.main.wait_stream(st)
+        finally:
+            for prob in self.problems:
+                prob.graph_mode = False
+
+    def replay(self, psi=None):
+        """psi: optional list of (3, N) displacements copied into the inputs.
+        Returns (losses, gradients); the gradients are the graph's outputs,
+        overwritten by the next replay."""
+        if psi is not None:
+            for dst, src in zip(self.psi, psi):
+                dst.copy_(src)
+        torch.cuda.synchronize(self.device)
+        self.graph.replay()
+        torch.cuda.synchronize(self.device)
+        for prob in self.problems:
+            prob.coord.fault_check()
+        losses = [float((d * d).sum()) for d in self.diff]
+        return losses, self.grad
+
+    def close(self):
+        self.graph = None
+        for prob in self.problems:
+            prob.close()
+        self.problems = []
+
+
 def box_gradient(spec: BoxSpec, steps: int, device: str = "cuda:0", checkpoint_every: int = 2,
                  explicit: bool = False):
     """d loss / d psi of one box: loss = sum of squared differences between the
@@ -347,4 +428,4 @@ def ensemble_gradients(specs, steps: int, workers: int = 8, device: str = "cuda:
 
 
 __all__ = ["BoxSpec", "Ensemble", "EnsembleForward", "autotune", "box_gradient", "box_initial_conditions",
-           "box_options", "ensemble_gradients", "ensemble_specs"]
+           "box_options", "box_problem", "ensemble_gradients", "ensemble_specs", "GradientGraph"]

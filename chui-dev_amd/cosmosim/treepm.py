@@ -132,6 +132,11 @@ class TreePmCoordinator:
         count read nothing back and allocate nothing (chui_treepm_set_capture_safe)."""
         _lib.check(self._lib.chui_treepm_set_capture_safe(self._h, 1 if enable else 0))
 
+    def fault_check(self):
+        """Faults of solves replayed from a graph the caller recorded on this
+        context's stream (chui_treepm_fault_check)."""
+        _lib.check(self._lib.chui_treepm_fault_check(self._h))
+
     def graph_begin(self):
         """Start recording this context's stream into a HIP graph."""
         _lib.check(self._lib.chui_treepm_graph_begin(self._h))

@@ -53,6 +53,7 @@ struct chui_treepm_context {
   // list (and sort scratch), its count, per-source corrections
   DeviceBuffer vjp_nacc, vjp_rec, vjp_rec_alt, vjp_vals, vjp_vals_alt, vjp_sort, vjp_count, vjp_corr;
   std::size_t vjp_accepted_cap = 0;
+  std::size_t vjp_safe_cap = 0;  // accepted-list slots a recorded VJP sorts (learned by eager VJPs)
   DeviceBuffer src64, src32, htab;  // tree-ordered packed sources; H(t) table
   double htab_key[2] = {0.0, 0.0};
   DeviceBuffer h64tab;              // FP64 box walk: degree-5 cells of H(t)
@@ -299,6 +300,16 @@ void record(Ctx* c, Stage st) {
   if (c->events) CHUI_HIP(hipEventRecord(c->ev[st], c->stream));
 }
 
+// Whether the context's stream records a graph: one begun through this library
+// (chui_treepm_graph_begin) or by the caller on the stream it handed over
+// (torch.cuda.graph, hipStreamBeginCapture).
+bool recording(const Ctx* c) {
+  if (c->capturing) return true;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  CHUI_HIP(hipStreamIsCapturing(c->stream, &cs));
+  return cs != hipStreamCaptureStatusNone;
+}
+
 // Walk counters with the sticky traversal-fault word after them; allocated and
 // zeroed once, so the fault word survives the per-solve counter resets.
 unsigned long long* walk_counters(Ctx* c, hipStream_t s) {
@@ -438,6 +449,7 @@ void prepare_h64(Ctx* c, chui::WalkParams& wp, const Split& sp, hipStream_t s) {
   const int cells = static_cast<int>(std::min<double>(chui::kH64MaxCells, std::max(16.0, std::ceil(t_max / 0.04))));
   if (c->h64_key[0] != t_max || c->h64_key[1] != cells || !c->h64tab.ptr) {
     const std::vector<double> tab = chui::gaussian_quintic_table(t_max, cells);
+    chui::require_not_capturing(s, "the FP64 residual table");
     c->h64tab.reserve(tab.size() * sizeof(double));
     CHUI_HIP(hipMemcpyAsync(c->h64tab.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s));
     CHUI_HIP(hipStreamSynchronize(s));
@@ -531,20 +543,21 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
   // recorded into a graph.
   const bool safe_mode = c->capture_safe;
   const bool safe = safe_mode && n > 0 && c->tree.prepared_n == n && c->tree.sized_for_capture;
-  if (c->capturing && !safe)
+  const bool rec = recording(c);
+  if (rec && !safe)
     throw std::runtime_error(
         "graph capture of a solve on a context not prepared for these sources: run one capture-safe solve of "
         "them before capturing");
-  if (safe_mode && (!mixed || adaptive_policy || diag || wcount || near2 || prev))
+  if (safe_mode && (adaptive_policy || diag || wcount || near2 || prev))
     throw chui::UnsupportedError(
-        "capture-safe solves are mixed-precision strict solves without diagnostics, counters, the nearest-pair "
-        "lane or an acceleration history");
+        "capture-safe solves are strict solves without diagnostics, counters, the nearest-pair lane or an "
+        "acceleration history");
   hipStream_t s = c->stream;
-  if (!c->capturing) {
+  if (!rec) {
     harvest(c);
     check_fault(c, false);
   }
-  c->events = !c->capturing && (diag != nullptr || c->profiling);
+  c->events = !rec && (diag != nullptr || c->profiling);
   if (c->events && !c->ev[0])
     for (auto& e : c->ev) CHUI_HIP(hipEventCreate(&e));
   record(c, kEvStart);
@@ -863,7 +876,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
     }
   }
   record(c, kEvWalk);
-  if (!c->capturing) post_fault_word(c, s);  // a graph launch posts it after the replay
+  if (!rec) post_fault_word(c, s);  // a graph launch (or chui_treepm_fault_check) posts it after the replay
   if (wcount || diag) check_fault(c, true);
   if (wcount) {
     unsigned long long wc[chui::kWalkCounters + 1] = {};
@@ -931,7 +944,13 @@ void require_ctx(const Ctx* c) {
   if (!c) throw std::invalid_argument("TreePM context is null");
 }
 
-void set_device(const Ctx* c) { CHUI_HIP(hipSetDevice(c->device)); }
+// Only when another device is current: a stream that records a graph under
+// a global capture mode admits as few runtime calls as possible.
+void set_device(const Ctx* c) {
+  int cur = -1;
+  CHUI_HIP(hipGetDevice(&cur));
+  if (cur != c->device) CHUI_HIP(hipSetDevice(c->device));
+}
 
 }  // namespace
 
@@ -1430,6 +1449,17 @@ chui_status chui_treepm_set_launch_shape(chui_treepm_context* c, int32_t fft_lin
   });
 }
 
+chui_status chui_treepm_fault_check(chui_treepm_context* c) {
+  return guarded([&] {
+    require_ctx(c);
+    set_device(c);
+    if (recording(c)) throw std::logic_error("fault check while the context's stream records a graph");
+    if (!c->counters.ptr) return;
+    post_fault_word(c, c->stream);
+    check_fault(c, true);
+  });
+}
+
 // ---- capture-safe solves and HIP graphs ------------------------------------
 chui_status chui_treepm_set_capture_safe(chui_treepm_context* c, int32_t enable) {
   return guarded([&] {
@@ -1515,8 +1545,21 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
     throw std::invalid_argument("TreePM force VJP lanes are null");
   if (n == 0) return;
   hipStream_t s = c->stream;
-  harvest(c);
-  check_fault(c, false);
+  // capture-safe mode as in solve_device_impl; a recorded VJP also takes its
+  // accepted-node count from the device (the list sorted over the capacity an
+  // eager VJP learned)
+  const bool safe_mode = c->capture_safe;
+  const bool safe = safe_mode && c->tree.prepared_n == n && c->tree.sized_for_capture;
+  const bool rec = recording(c);
+  if (rec && (!safe || (quad && c->vjp_safe_cap == 0)))
+    throw std::runtime_error(
+        "graph capture of a force VJP on a context not prepared for these sources: run one capture-safe VJP of "
+        "them before capturing");
+  if (!rec) {
+    harvest(c);
+    check_fault(c, false);
+  }
+  unsigned long long* const walk_cnt = walk_counters(c, s);
   const Split sp = split_of(*o, c->mesh);
   chui::TreeParams tp;
   tp.n = n;
@@ -1524,6 +1567,9 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
   tp.leaf_size = static_cast<uint32_t>(o->max_leaf_size);
   tp.quadrupole = quad;
   tp.mixed = false;
+  tp.capture_safe = safe;
+  tp.size_for_capture = safe_mode && !safe;
+  tp.fault = walk_cnt + chui::kFaultSlot;
   chui::build_tree(c->tree, x, y, z, m, tp, o->softening_epsilon, s);
   const bool uniform = c->tree.mass_uniform;
   const double um = uniform ? c->tree.uniform_mass : 0.0;
@@ -1567,7 +1613,6 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
                             c->vjp_comp[2].as<double4>()};
   chui::pm_adjoint_fp64(c->vjp_adj, g, po, comp, n, units, s);
 
-  unsigned long long* const walk_cnt = walk_counters(c, s);
   chui::WalkParams wp = base_walk_params(c, o, sp, g.row);
   wp.quad = quad ? 1 : 0;
   wp.n_targets = static_cast<uint32_t>(n);
@@ -1582,12 +1627,14 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
   // accepted-node terms: a list sized 4 per source, or the last VJP's count
   // if larger; a walk that accepts more runs again with the list it needs
   chui::VjpNodeTerms nt;
-  std::size_t cap = std::max<std::size_t>(4 * n + 1024, c->vjp_accepted_cap);
+  std::size_t cap = rec ? c->vjp_safe_cap : std::max<std::size_t>(4 * n + 1024, c->vjp_accepted_cap);
   for (int attempt = 0;; ++attempt) {
     if (quad) {
-      const std::size_t nodes = c->tree.node_count;
+      const std::size_t nodes = c->tree.node_count;  // the capacity after a capture-safe build
+      const std::size_t nodes_sz =
+          c->tree.sized_for_capture ? std::max(nodes, c->tree.bfs_capacity) : nodes;
       cap = std::min<std::size_t>(cap, 0xFFFFFFFFull);
-      c->vjp_nacc.reserve(std::max<std::size_t>(nodes, 1) * 10 * sizeof(double));
+      c->vjp_nacc.reserve(std::max<std::size_t>(nodes_sz, 1) * 10 * sizeof(double));
       c->vjp_rec.reserve(cap * sizeof(uint64_t));
       c->vjp_rec_alt.reserve(cap * sizeof(uint64_t));
       c->vjp_vals.reserve(cap * sizeof(uint32_t));
@@ -1598,6 +1645,7 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
       chui::fill_f64(c->vjp_nacc.as<double>(), 0.0, nodes * 10, s);
       chui::fill_u32(c->vjp_count.as<uint32_t>(), 0u, 1, s);
       chui::fill_f64(c->vjp_corr.as<double>(), 0.0, n * 3, s);
+      if (rec) chui::fill_u64(c->vjp_rec.as<uint64_t>(), ~0ull, cap, s);  // unused slots sort last
       nt.node_acc = c->vjp_nacc.as<double>();
       nt.accepted = c->vjp_rec.as<unsigned long long>();
       nt.accepted_alt = c->vjp_rec_alt.as<unsigned long long>();
@@ -1614,9 +1662,15 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
                         static_cast<const double*>(c->vjp_fwd.force[2]), c->vjp_adj.buf[0].as<double>(), out,
                         walk_cnt, c->h64tab.as<double2>(), quad ? &nt : nullptr, s);
     if (!quad) break;
+    if (rec) {
+      chui::vjp_node_corrections(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, um, c->vjp_lam.as<double4>(),
+                                 nt, static_cast<unsigned int>(cap), out, s, nt.count, walk_cnt + chui::kFaultSlot);
+      break;
+    }
     uint32_t count = 0;
     CHUI_HIP(hipMemcpyAsync(&count, nt.count, sizeof(count), hipMemcpyDeviceToHost, s));
     CHUI_HIP(hipStreamSynchronize(s));
+
     if (count > nt.cap) {
       if (attempt > 0) throw std::logic_error("force VJP: accepted-node count changed between identical walks");
       cap = static_cast<std::size_t>(count) + n;
@@ -1625,9 +1679,23 @@ void force_vjp_impl(Ctx* c, const double* x, const double* y, const double* z, c
     }
     chui::vjp_node_corrections(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, um, c->vjp_lam.as<double4>(), nt,
                                count, out, s);
+    if (safe_mode) {
+      // the list a recorded VJP sorts: twice this count (reserved after the
+      // corrections above have read this one's records)
+      const std::size_t want = std::min<std::size_t>(2 * static_cast<std::size_t>(count) + 1024, 0xFFFFFFFFull);
+      if (want > c->vjp_safe_cap) {
+        CHUI_HIP(hipStreamSynchronize(s));
+        c->vjp_safe_cap = want;
+        c->vjp_rec.reserve(want * sizeof(uint64_t));
+        c->vjp_rec_alt.reserve(want * sizeof(uint64_t));
+        c->vjp_vals.reserve(want * sizeof(uint32_t));
+        c->vjp_vals_alt.reserve(want * sizeof(uint32_t));
+        c->vjp_sort.reserve(chui::radix_sort_scratch_bytes(want));
+      }
+    }
     break;
   }
-  post_fault_word(c, s);
+  if (!rec) post_fault_word(c, s);
 }
 
 chui_status chui_treepm_force_vjp_device(chui_treepm_context* c, const double* x, const double* y, const double* z,

@@ -1017,7 +1017,7 @@ void pm_adjoint_fp64(PmWorkspace& w, const PmGeometry& g, const PmOptions& o, co
     double* mu = w.buf[a].as<double>();
     fill_f64(mu, 0.0, g.complex_cells() * 2, s);
     if (n > 0) {
-      if (units.count == 0) throw std::logic_error("PM adjoint deposit without deposit units");
+      if (units.count == 0 && !units.dev_counts) throw std::logic_error("PM adjoint deposit without deposit units");
       deposit_kernel<double, double4><<<units.blocks, 256, 0, s>>>(deposit_args(g, o), comp[a], 0, 0.0, units, mu);
       CHUI_LAUNCH_CHECK("deposit_kernel");
     }

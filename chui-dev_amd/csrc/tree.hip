@@ -1481,11 +1481,12 @@ UnwrapResult* unwrap_axes(Tree& t, Axes3 p, int naxes, std::size_t n, const doub
 
 void launch_records(Tree& t, bool fp64, bool mixed, hipStream_t s) {
   const std::size_t total = t.node_count;
+  const std::size_t nsz = t.sized_for_capture ? std::max(total, t.bfs_capacity) : total;
   if (fp64) {
-    t.geo64.reserve(total * sizeof(NodeGeo64));
-    t.mom64.reserve(total * sizeof(NodeMom64));
-    t.quad64.reserve(total * sizeof(NodeQuad64));
-    t.child_index.reserve(total * 8 * sizeof(uint32_t));
+    t.geo64.reserve(nsz * sizeof(NodeGeo64));
+    t.mom64.reserve(nsz * sizeof(NodeMom64));
+    t.quad64.reserve(nsz * sizeof(NodeQuad64));
+    t.child_index.reserve(nsz * 8 * sizeof(uint32_t));
   }
   RecordArgs ra{};
   ra.total = static_cast<uint32_t>(total);
@@ -1521,7 +1522,7 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
   if (n > 0xFFFFFFFEull) throw std::overflow_error("Tree gravity source count exceeds the 32-bit tree-index contract");
   const bool safe = P.capture_safe;
   if (safe) {
-    if (!P.mixed || !P.fault) throw std::logic_error("capture-safe tree builds are mixed-precision solves");
+    if (!P.fault) throw std::logic_error("capture-safe tree builds report through a fault word");
     if (n == 0 || n != t.prepared_n || !t.sized_for_capture || t.safe_levels <= 0 || t.bfs_capacity == 0)
       throw std::runtime_error(
           "capture-safe solve on a context not prepared for these sources (run an ordinary solve of the same "

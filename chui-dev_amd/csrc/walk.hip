@@ -2286,7 +2286,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_target_kernel(WalkParams 
     next = take ? q + lk.skip : q + 1;
     return take ? 2 : 3;
   };
-  walk_target_wave(P.node_count, lane, list[w], mps[w], nl, nm, classify, evaluate);
+  walk_target_wave((D.node_count_dev ? *D.node_count_dev : P.node_count), lane, list[w], mps[w], nl, nm, classify, evaluate);
   ax = wave_sum(ax);
   ay = wave_sum(ay);
   az = wave_sum(az);
@@ -2352,6 +2352,7 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
   __shared__ double2 htab[3 * kH64Cells];
   for (int i = threadIdx.x; i < 3 * P.h64_cells; i += kWalkThreads) htab[i] = D.h64tab[i];
   __syncthreads();
+  const uint32_t node_count = D.node_count_dev ? *D.node_count_dev : P.node_count;
   const int w = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   const uint32_t k = xcd_logical_block() * kWalkThreads + threadIdx.x;
@@ -2505,7 +2506,7 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
   };
 
   // lane-by-lane walk of [nxt, end) with walk64_kernel's tests
-  uint32_t nxt = valid ? 0u : P.node_count;
+  uint32_t nxt = valid ? 0u : node_count;
   auto walk = [&]<bool NEAR>(uint32_t end) {
     uint32_t n = wave_min_u32(nxt);
     while (n < end) {
@@ -2680,13 +2681,13 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
       }
     };
     uint32_t p = 0;
-    while (p < P.node_count) {
+    while (p < node_count) {
       const uint32_t q = p + lane;
       uint32_t next = q + 1, lb = 0, lc = 0;
-      const int kind = q < P.node_count ? classify(q, next, lb, lc) : 3;
+      const int kind = q < node_count ? classify(q, next, lb, lc) : 3;
       unsigned long long chain = 0;
       uint32_t cur = p;
-      while (cur < p + kWave && cur < P.node_count) {
+      while (cur < p + kWave && cur < node_count) {
         const uint32_t l = cur - p;
         chain |= 1ull << l;
         cur = readlane_u32(next, l);
@@ -2725,7 +2726,7 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
     while (nl) eval_window(nl);  // JVP: each call takes at least one source or evaluates a full window
     if (wn) evaluate(wn);
   } else {
-    walk.template operator()<false>(P.node_count);
+    walk.template operator()<false>(node_count);
   }
   if (!valid) return;
   static_cast<double*>(out.ax)[slot] = gx + ax;
@@ -2751,11 +2752,20 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(4)
 // within r_cut that sees t directly (p accepts no node holding t): remove
 // M_tp lam_t from s_p, and add to s_t the term p's force contributes,
 // M_pt lam_p = G m_t [F lam_p + 2 F' (r . lam_p) r]. rec is sorted.
-__global__ __launch_bounds__(256) void vjp_pair_fix_kernel(const unsigned long long* __restrict__ rec, uint32_t nrec,
+// A recorded VJP: more accepted (target, node) pairs than list slots.
+__global__ void vjp_count_check_kernel(const unsigned int* __restrict__ count, unsigned int cap,
+                                       unsigned long long* __restrict__ fault) {
+  if (threadIdx.x == 0 && *count > cap) atomicOr(fault, kFaultPrepared);
+}
+
+// nrec_dev (a recorded VJP): the list's real length on the device, nrec its capacity.
+__global__ __launch_bounds__(256) void vjp_pair_fix_kernel(const unsigned long long* __restrict__ rec, uint32_t nrec_cap,
+                                                           const unsigned int* __restrict__ nrec_dev,
                                                            const NodeLink* __restrict__ link,
                                                            const double4* __restrict__ src,
                                                            const double4* __restrict__ lam, int uniform, double um,
                                                            WalkParams P, double* __restrict__ corr) {
+  const uint32_t nrec = nrec_dev ? min(*nrec_dev, nrec_cap) : nrec_cap;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrec) return;
   const uint32_t t = static_cast<uint32_t>(rec[i] >> 32), node = static_cast<uint32_t>(rec[i] & 0xFFFFFFFFull);
@@ -3014,6 +3024,7 @@ void walk_fp64(const Tree& t, const WalkParams& p, const double4* src, int unifo
   d.mom64 = t.mom64.as<NodeMom64>();
   d.quad64 = t.quad64.as<NodeQuad64>();
   d.order = t.order.as<uint32_t>();
+  d.node_count_dev = device_node_count(t);
   d.src64 = src;
   d.uniform_mass = uniform_mass;
   d.tpos = target_pos;
@@ -3158,6 +3169,7 @@ void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int u
   d.mom64 = t.mom64.as<NodeMom64>();
   d.quad64 = t.quad64.as<NodeQuad64>();
   d.order = t.order.as<uint32_t>();
+  d.node_count_dev = device_node_count(t);
   d.src64 = src;
   d.uniform_mass = uniform_mass;
   d.f64x = fx;
@@ -3186,14 +3198,19 @@ void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int u
 
 void vjp_node_corrections(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
                           double uniform_mass_value, const double4* lam, const VjpNodeTerms& nt, unsigned int count,
-                          const WalkOutputs& out, hipStream_t s) {
+                          const WalkOutputs& out, hipStream_t s, const unsigned int* count_dev,
+                          unsigned long long* fault) {
   const std::size_t n = t.n;
   if (n == 0) return;
+  if (count_dev) {
+    vjp_count_check_kernel<<<1, 64, 0, s>>>(count_dev, count, fault);
+    CHUI_LAUNCH_CHECK("vjp_count_check_kernel");
+  }
   if (count > 0) {
     radix_sort_pairs(reinterpret_cast<uint64_t*>(nt.accepted), nt.vals, reinterpret_cast<uint64_t*>(nt.accepted_alt),
                      nt.vals_alt, count, 0, 64, nt.sort_scratch, s);
-    vjp_pair_fix_kernel<<<grid_for(count, 256), 256, 0, s>>>(nt.accepted, count, t.link.as<NodeLink>(), src, lam,
-                                                             uniform_mass, uniform_mass_value, p, nt.corr);
+    vjp_pair_fix_kernel<<<grid_for(count, 256), 256, 0, s>>>(nt.accepted, count, count_dev, t.link.as<NodeLink>(),
+                                                             src, lam, uniform_mass, uniform_mass_value, p, nt.corr);
     CHUI_LAUNCH_CHECK("vjp_pair_fix_kernel");
     vjp_node_push_kernel<<<grid_for(t.node_count, 256), 256, 0, s>>>(
         static_cast<uint32_t>(t.node_count), nt.node_acc, t.link.as<NodeLink>(), t.mom64.as<NodeMom64>(), src,

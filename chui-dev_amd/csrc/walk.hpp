@@ -166,9 +166,14 @@ void walk_fp64_vjp(const Tree& t, const WalkParams& p, const double4* src, int u
 // by the caller): per accepted (t, N), the pairs between t and N's particles
 // that the symmetric pair terms counted on one side only; U and V spread over
 // N's particles; the corrections added to out (caller order).
+// With count_dev (a recorded VJP) `count` is the list's capacity: the list is
+// sorted over it (unused slots hold ~0 and sort last), the corrections read the
+// real count on the device, and a count beyond the capacity sets
+// kFaultPrepared in *fault.
 void vjp_node_corrections(const Tree& t, const WalkParams& p, const double4* src, int uniform_mass,
                           double uniform_mass_value, const double4* lam, const VjpNodeTerms& nt, unsigned int count,
-                          const WalkOutputs& out, hipStream_t s);
+                          const WalkOutputs& out, hipStream_t s, const unsigned int* count_dev = nullptr,
+                          unsigned long long* fault = nullptr);
 // Per-target FP64 walk under the adaptive acceptance. gtab = Gaussian Hermite
 // knots (value, slope) x 4097 (required when p.lookup), prev = previous
 // acceleration magnitude per output slot (nullable).

@@ -226,16 +226,24 @@ chui_status chui_treepm_set_launch_shape(chui_treepm_context* ctx, int32_t fft_l
  * an ordinary one that also sizes the context for it and records the tree's
  * node capacity, depth and mass uniformity; every later solve of that source
  * count reads nothing back to the host and allocates nothing, so it can be
- * recorded. Capture-safe solves are mixed-precision strict solves without
+ * recorded. Capture-safe solves are strict solves (FP64 or mixed) without
  * diagnostics, counters, the nearest-pair lane or a history
- * (CHUI_ERR_UNSUPPORTED otherwise); their results equal the ordinary solve's
- * bit for bit. What an ordinary solve reports at once (non-finite inputs, an
+ * (CHUI_ERR_UNSUPPORTED otherwise); their tree-only lanes equal the ordinary
+ * solve's bit for bit. The FP64 force VJP follows the same mode. The library
+ * detects a stream that records a graph by itself, so a caller may also record
+ * the context's stream with its own capture (torch.cuda.graph); every
+ * recorded solve and VJP must have been prepared by an eager one. What an ordinary solve reports at once (non-finite inputs, an
  * active index out of range, a tree deeper than 21 levels), and sources that
  * outgrow the prepared node capacity, depth (prepared depth + 4) or mass
  * uniformity (CHUI_ERR_RUNTIME: solve once more outside a graph and record
  * again), comes back at the next synchronisation. */
 typedef struct chui_treepm_graph chui_treepm_graph;
 chui_status chui_treepm_set_capture_safe(chui_treepm_context* ctx, int32_t enable);
+/* Reports the fault word of work replayed from a graph the caller recorded on
+ * the context's stream (torch.cuda.graph, hipStreamBeginCapture): copies it
+ * after the stream's work, synchronises and returns the failed solve's status.
+ * Graphs launched through chui_treepm_graph_launch post it by themselves. */
+chui_status chui_treepm_fault_check(chui_treepm_context* ctx);
 /* Records everything issued on the context's stream (solves, kicks, drifts)
  * until graph_end into a graph. The context must be capture-safe, and a solve
  * of each source count to be recorded must have run before. */

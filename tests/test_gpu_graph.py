@@ -13,6 +13,14 @@ counterpart: the reference drives every solve from the host).
   from the host inputs; recording on an unprepared context is refused.
 - The batched ensemble forward recorded as one graph (boxes as branches)
   equals the eager batched forward after every replay.
+- FP64 capture-safe solves equal ordinary ones (tree-only lanes bit for bit),
+  also on a clustered set whose walk accepts quadrupole nodes.
+- A force VJP recorded with torch.cuda.graph on the context's stream (the
+  library detects the recording; the accepted-node list's count stays on the
+  device) equals the eager VJP on every replay.
+- The gradients of several boxes recorded as one graph (each box a branch:
+  its FP64 solves and VJPs, CIC density and transpose and the element-wise
+  KDK work) equal the eager gradients after every replay and for new inputs.
 """
 from __future__ import annotations
 
@@ -53,6 +61,7 @@ def _solve(coord, dev, o, active=None):
     n = int(dev[0].shape[0])
     n_active = n if active is None else int(active.shape[0])
     acc, pm, sr = _lanes(n, n_active)
+    torch.cuda.synchronize()  # the library's stream does not wait for torch's
     coord.solve_active_set_with_pm_cadence(*dev, ForceAccumulator(acc, active, pm, sr), o, True)
     coord.synchronize()
     return [np.stack([a.cpu().numpy() for a in lane]) for lane in (acc, pm, sr)]
@@ -162,10 +171,9 @@ def test_recording_needs_a_prepared_context():
             coord.solve_active_set_with_pm_cadence(*dev, ForceAccumulator(acc, None, pm, sr), o, True)
         assert "prepared" in str(err.value)
         coord.graph_end().close()
-        o.precision = L.PRECISION_FP64
+        o.acceptance_policy = 1  # CHUI_ACCEPT_ADAPTIVE: not a capture-safe solve
         with pytest.raises(L.ChuiError) as err:
-            coord.solve_active_set_with_pm_cadence(*dev, ForceAccumulator(tuple(empty_dev(n, torch.float64)
-                                                                               for _ in range(3))), o, True)
+            coord.solve_active_set_with_pm_cadence(*dev, ForceAccumulator(acc), o, True)
         assert err.value.status == L.CHUI_ERR_UNSUPPORTED
 
 
@@ -200,3 +208,120 @@ def test_ensemble_graph_equals_eager_forward():
                 fwd.run()
         finally:
             graph.close()
+
+
+# ---- FP64 capture-safe solves, recorded VJPs and the gradient graph ----------
+
+def _quadrupole_case():
+    # the clustered set of the quadrupole VJP tests: thousands of (target,
+    # node) pairs accept a node, so the VJP's accepted-node list is not empty
+    from test_gpu_adjoint_quadrupole import BOX as QB, EPS as QE, G as QG, LEAF as QL, MESH as QM, clustered_set
+
+    x, m = clustered_set()
+    o = options_struct(QB, G=QG, eps=QE, leaf=QL, quad=True)
+    return (x[0], x[1], x[2], m), o, QM
+
+
+@pytest.mark.parametrize("name", ["clustered", "lattice_halos", "quadrupole"])
+def test_fp64_capture_safe_solves_equal_ordinary(name):
+    src, o, mesh = _quadrupole_case() if name == "quadrupole" else _case(name)
+    o.precision = L.PRECISION_FP64
+    dev = to_dev(*src)
+    n = int(dev[0].shape[0])
+
+    def solve(coord):
+        acc = tuple(empty_dev(n, torch.float64) for _ in range(3))
+        sr = tuple(torch.zeros(n, dtype=torch.float64, device="cuda:0") for _ in range(3))
+        torch.cuda.synchronize()
+        coord.solve_active_set_with_pm_cadence(*dev, ForceAccumulator(acc, None, None, sr), o, True)
+        coord.synchronize()
+        return np.stack([a.cpu().numpy() for a in acc]), np.stack([a.cpu().numpy() for a in sr])
+
+    with TreePmCoordinator(mesh) as plain:
+        acc0, sr0 = solve(plain)
+    with TreePmCoordinator(mesh) as coord:
+        coord.set_capture_safe(True)
+        for _ in range(3):  # the preparing solve, then capture-safe ones
+            acc, sr = solve(coord)
+            assert np.array_equal(sr, sr0)
+            assert rel_l2(acc, acc0) < 1e-12
+
+
+def test_recorded_force_vjp_equals_eager():
+    # the VJP recorded with torch.cuda.graph on the context's stream (the
+    # library detects the recording and takes the accepted-node count from
+    # the device), replayed three times with eager work between replays
+    src, o, mesh = _quadrupole_case()
+    dev = to_dev(*src)
+    n = int(dev[0].shape[0])
+    rng = np.random.default_rng(3)
+    lam = [torch.from_numpy(rng.normal(size=n)).to("cuda:0") for _ in range(3)]
+
+    def vjp(coord, recorded=False):
+        # eager calls order the library's stream after torch's and wait for
+        # it; a recorded call shares torch's stream
+        out = tuple(torch.zeros(n, dtype=torch.float64, device="cuda:0") for _ in range(3))
+        if not recorded:
+            torch.cuda.synchronize()
+        coord.force_vjp(*dev, lam, out, o)
+        if not recorded:
+            coord.synchronize()
+        return out
+
+    with TreePmCoordinator(mesh) as plain:
+        ref = np.stack([a.cpu().numpy() for a in vjp(plain)])
+    with TreePmCoordinator(mesh) as coord, TreePmCoordinator(mesh) as other:
+        coord.set_capture_safe(True)
+        acc = tuple(empty_dev(n, torch.float64) for _ in range(3))
+        torch.cuda.synchronize()
+        coord.solve_active_set(*dev, ForceAccumulator(acc), o)  # prepares the context
+        coord.synchronize()
+        for _ in range(2):
+            got = vjp(coord)
+            assert rel_l2(np.stack([a.cpu().numpy() for a in got]), ref) < 1e-12
+        stream = torch.cuda.Stream()
+        coord.set_stream(stream.cuda_stream)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            out = vjp(coord, recorded=True)
+        for rep in range(3):
+            for a in out:
+                a.fill_(float("nan"))
+            torch.cuda.synchronize()
+            graph.replay()
+            torch.cuda.synchronize()
+            coord.fault_check()
+            assert rel_l2(np.stack([a.cpu().numpy() for a in out]), ref) < 1e-12, rep
+            vjp(other)  # eager work between replays
+        del graph
+
+
+def test_gradient_graph_equals_eager_gradients():
+    specs = E.ensemble_specs(counts=((16, 2), (24, 1)), seed=91)
+    eager = [E.box_gradient(s, 2, explicit=True) for s in specs]
+    gg = E.GradientGraph(specs, 2)
+    try:
+        for rep in range(3):
+            losses, grads = gg.replay()
+            for (l0, g0, _, _), l1, g1, s in zip(eager, losses, grads, specs):
+                assert abs(l1 - l0) <= 1e-11 * abs(l0), (rep, s)
+                assert rel_l2(g1.cpu().numpy(), g0.cpu().numpy()) < 1e-10, (rep, s)
+            E.box_gradient(specs[0], 2, explicit=True)  # eager work between replays
+            torch.cuda.synchronize()
+        # new inputs go through the recorded graph: the gradient at psi + dpsi
+        rng = np.random.default_rng(5)
+        moved = [p + 0.01 * torch.from_numpy(rng.normal(size=tuple(p.shape))).to(p.device) for p in gg.psi]
+        saved = [p.clone() for p in gg.psi]
+        _, g_new = gg.replay(moved)
+        g_new = [g.clone() for g in g_new]
+        for b, s in enumerate(specs):
+            prob, _, target = E.box_problem(s, 2)
+            try:
+                _, g_ref = E.adjoint.loss_and_gradient(moved[b], target, prob)
+            finally:
+                prob.close()
+            assert rel_l2(g_new[b].cpu().numpy(), g_ref.cpu().numpy()) < 1e-10, s
+        gg.replay(saved)
+    finally:
+        gg.close()
