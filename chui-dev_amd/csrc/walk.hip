@@ -1523,7 +1523,11 @@ __device__ __forceinline__ float4 lds_cell(uint32_t base, uint32_t cell) {
 // loop stays rolled (see lds_cell); the unroller's warning about it is muted.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Wpass-failed"
-template <int SCHEME, bool CUBIC, bool COUNT, bool BOX = false, bool LOG = false, int WAVES = kWalkWaves>
+// DEVN: the traversal bound is the node count on the device (after a
+// capture-safe build); otherwise the kernel argument, which the timed walk
+// keeps (reading the bound from memory cost it 1 ms of 22 at 256^3).
+template <int SCHEME, bool CUBIC, bool COUNT, bool BOX = false, bool LOG = false, int WAVES = kWalkWaves,
+          bool DEVN = false>
 __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(COUNT ? 4 : 7)))
 void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long long* __restrict__ counters,
                     const NodeLink* __restrict__ node_link, const NodeGeo32* __restrict__ node_geo,
@@ -1544,7 +1548,7 @@ void walkmix_kernel(WalkParams P, WalkData D, WalkOutputs out, unsigned long lon
     for (int i = threadIdx.x; i < kHTableCells; i += (WAVES * kWave)) htab[i] = D.htab[i];
   }
   __syncthreads();
-  const uint32_t node_count = D.node_count_dev ? *D.node_count_dev : P.node_count;
+  const uint32_t node_count = DEVN ? *D.node_count_dev : P.node_count;
   const int w = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   const uint32_t k = xcd_logical_block() * (WAVES * kWave) + threadIdx.x;
@@ -2890,11 +2894,18 @@ void launch64(const WalkParams& p, const WalkData& d, const WalkOutputs& out, un
 template <int SCHEME, bool CUBIC>
 void launchmix(const WalkParams& p, const WalkData& d, const WalkOutputs& out, unsigned long long* counters,
                hipStream_t s) {
-  if (p.count)
+  const bool devn = d.node_count_dev != nullptr;
+  if (p.count && !devn)
     walkmix_kernel<SCHEME, CUBIC, true><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
         p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
+  else if (p.count)
+    throw std::logic_error("counting walk after a capture-safe build");
   else if (p.target_per_wave)
     walkmix_target_kernel<SCHEME, CUBIC><<<grid_for(p.n_targets, kWalkWaves), kWalkThreads, 0, s>>>(p, d, out);
+  else if (p.log_cells > 0 && devn)
+    walkmix_kernel<SCHEME, CUBIC, false, true, true, kWalkWaves, true>
+        <<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, p.log_cells * sizeof(float4), s>>>(
+            p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
   else if (p.log_cells > 0 && p.walk_waves == 8)
     walkmix_kernel<SCHEME, CUBIC, false, true, true, 8>
         <<<grid_for(p.n_targets, 8 * kWave), 8 * kWave, p.log_cells * sizeof(float4), s>>>(
@@ -2907,6 +2918,10 @@ void launchmix(const WalkParams& p, const WalkData& d, const WalkOutputs& out, u
     walkmix_kernel<SCHEME, CUBIC, false, true, true>
         <<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, p.log_cells * sizeof(float4), s>>>(
             p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
+  else if (devn)
+    walkmix_kernel<SCHEME, CUBIC, false, true, false, kWalkWaves, true>
+        <<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(p, d, out, counters, d.link, d.geo32, d.mom32,
+                                                                       d.quad32, d.box32);
   else
     walkmix_kernel<SCHEME, CUBIC, false, true><<<grid_for(p.n_targets, kWalkThreads), kWalkThreads, 0, s>>>(
         p, d, out, counters, d.link, d.geo32, d.mom32, d.quad32, d.box32);
