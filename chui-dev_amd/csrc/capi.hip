@@ -86,10 +86,11 @@ struct chui_treepm_context {
   uint64_t* fault_host = nullptr;
   hipEvent_t fault_ev = nullptr;
   bool fault_pending = false;
-  // Capture-safe solves (chui_treepm_set_capture_safe): mixed strict solves
-  // that read nothing back and allocate nothing once a solve of the same
-  // sources has sized the context, so that they can be recorded into a HIP
-  // graph; `capturing` while the context's stream records one.
+  // Capture-safe solves (chui_treepm_set_capture_safe): strict solves (mixed
+  // or FP64) and FP64 force VJPs that read nothing back and allocate nothing
+  // once a solve of the same sources has sized the context, so that they can
+  // be recorded into a HIP graph; `capturing` while the context's stream
+  // records one through chui_treepm_graph_begin.
   bool capture_safe = false;
   bool capturing = false;
   // launch shapes (chui_treepm_set_launch_shape, the autotuner's knobs):

@@ -232,11 +232,12 @@ chui_status chui_treepm_set_launch_shape(chui_treepm_context* ctx, int32_t fft_l
  * solve's bit for bit. The FP64 force VJP follows the same mode. The library
  * detects a stream that records a graph by itself, so a caller may also record
  * the context's stream with its own capture (torch.cuda.graph); every
- * recorded solve and VJP must have been prepared by an eager one. What an ordinary solve reports at once (non-finite inputs, an
- * active index out of range, a tree deeper than 21 levels), and sources that
- * outgrow the prepared node capacity, depth (prepared depth + 4) or mass
- * uniformity (CHUI_ERR_RUNTIME: solve once more outside a graph and record
- * again), comes back at the next synchronisation. */
+ * recorded solve and VJP must have been prepared by an eager one. What an
+ * ordinary solve reports at once (non-finite inputs, an active index out of
+ * range, a tree deeper than 21 levels), and sources that outgrow the prepared
+ * node capacity, depth (prepared depth + 4) or mass uniformity
+ * (CHUI_ERR_RUNTIME: solve once more outside a graph and record again), comes
+ * back at the next synchronisation. */
 typedef struct chui_treepm_graph chui_treepm_graph;
 chui_status chui_treepm_set_capture_safe(chui_treepm_context* ctx, int32_t enable);
 /* Reports the fault word of work replayed from a graph the caller recorded on
