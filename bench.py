@@ -707,6 +707,33 @@ def measure_adaptive(L, coord, dev, w, acc_prev, steps, prev_name):
             out["group_vs_strict"] = {"reference": prev_name, **error_metrics(acc_group, acc_prev)}
             result = acc_group
     out["group_over_per_target"] = out["group_walk"]["value"] / out["per_target_walk"]["value"]
+    # the same group walk in mixed precision: FP64 groups, lists and decisions,
+    # FP32 listed-source pairs, FP32 PM and output lanes
+    o.precision = L.PRECISION_MIXED
+    acc32 = tuple(torch.zeros(n, dtype=torch.float32, device="cuda:0") for _ in range(3))
+    fa32 = ForceAccumulator(acc32, previous_acceleration=prev)
+    ad = L.default_adaptive_options(error_estimator=L.ESTIMATOR_RELATIVE_MONOPOLE, maximum_opening_angle=1.0,
+                                    cutoff_node_policy=L.CUTOFF_COM_DISTANCE, gaussian_pair_lookup=1, group_walk=1)
+    counters = coord.solve_active_set(*dev, fa32, o, want_diagnostics=True, adaptive_options=ad).diagnostics
+    coord.profiling(True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        coord.solve_active_set(*dev, fa32, o, adaptive_options=ad)
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t) / steps
+    prof = coord.profile_read()
+    coord.profiling(False)
+    o.precision = L.PRECISION_FP64
+    acc_mix = np.stack([a.cpu().numpy().astype(np.float64) for a in acc32])
+    out["group_walk_mixed"] = {
+        "value": n / elapsed, "unit": "particles/s", "ms_per_step": 1e3 * elapsed,
+        "walk_ms": prof["walk_ms"] / max(prof["solves"], 1),
+        "pairs_per_target": counters["direct_pair_evaluations"] / n,
+        "accepted_multipoles": counters["accepted_internal_multipoles"],
+        "fallback_targets": counters["group_walk_fallback_targets"],
+        "mixed_vs_fp64_group": error_metrics(acc_mix, result),
+        "over_fp64_group": (n / elapsed) / out["group_walk"]["value"]}
     return out, result
 
 

@@ -70,6 +70,9 @@ struct chui_treepm_context {
   // adaptive policy: Gaussian Hermite knots, leaf of each tree position,
   // group-walk fallback targets and counters, source bounds
   DeviceBuffer gtab, gcoef, leaf_of_pos, fb_pos, fb_slot, gcounters, bounds;
+  // mixed adaptive solves: FP64 short-range sums (3 lanes by output slot) and
+  // nearest-pair distances (by source row) of the FP64 per-target walk
+  DeviceBuffer mix_fb, mix_near;
   hipEvent_t ev[kEvCount] = {};
   bool events = false;
   // profiling mode: events recorded every solve, harvested at the next solve's
@@ -202,8 +205,6 @@ void validate_options(const chui_treepm_options* o, const int mesh[3]) {
     throw std::invalid_argument("Unknown TreePM acceptance policy");
   if (o->target_mapping < CHUI_TARGETS_AUTO || o->target_mapping > CHUI_TARGETS_PER_WAVE)
     throw std::invalid_argument("Unknown target mapping");
-  if (o->acceptance_policy == CHUI_ACCEPT_ADAPTIVE && o->precision != CHUI_PRECISION_FP64)
-    throw chui::UnsupportedError("the adaptive acceptance policy is built for FP64 precision only");
   if (o->acceptance_policy == CHUI_ACCEPT_ADAPTIVE &&
       !(std::isfinite(o->relative_force_tolerance) && o->relative_force_tolerance > 0.0))
     throw std::invalid_argument("numerics.treepm_tree_relative_force_tolerance must be finite and > 0");
@@ -463,6 +464,36 @@ void prepare_h64(Ctx* c, chui::WalkParams& wp, const Split& sp, hipStream_t s) {
   wp.c8_64 = ((0.125 / sp.a) / sp.a) / sp.a;
 }
 
+// The mixed walks' pair arithmetic in H-table units (lengths times
+// sqrt(t_to_cell), so r^2 is the coordinate of the 128-cell table of H) and
+// the table itself, uploaded when its range or split scale changes.
+void prepare_mixed_pairs(Ctx* c, chui::WalkParams& wp, const Split& sp, const chui_treepm_options* o,
+                         bool uniform, double um, hipStream_t s) {
+  const double t_max = wp.rcut2 * (0.25 / (sp.a * sp.a)) * (1.0 + 1.0e-3);
+  const std::vector<float> tab = chui::gaussian_coefficient_table(wp, t_max);  // also sets wp.t_to_cell
+  const double t2c = 0.25 / (sp.a * sp.a) / (t_max / chui::kHTableCells);
+  const double sq = std::sqrt(t2c);
+  for (int a = 0; a < 3; ++a) wp.tab_scale[a] = static_cast<float>(o->box_size[a] / 4294967296.0 * sq);
+  wp.rcut2_tab = static_cast<float>(wp.rcut2 * t2c);
+  // floor 1e-12 (table units, r^2 <= 128 for every pair inside r_cut): a
+  // coincident pair (r^2 = 0) keeps a finite softened term (< 1e18 times
+  // its mass), so its zero separation gives a zero force
+  wp.eps2_tab = static_cast<float>(std::max(wp.eps * wp.eps * t2c, 1.0e-12));
+  wp.c8_tab = static_cast<float>(0.125 / (sp.a * sp.a * sp.a) / (t2c * sq));
+  // with uniform masses the pair kernel leaves the mass to this factor
+  wp.force_scale = static_cast<float>(wp.G * t2c * (uniform ? um : 1.0));
+  wp.tab_to_len2 = static_cast<float>(1.0 / t2c);
+  wp.accept_half_max = static_cast<float>(0.041 * sp.rcut);
+  if (c->htab_key[0] != t_max || c->htab_key[1] != sp.a || !c->htab.ptr) {
+    chui::require_not_capturing(s, "the pair table");
+    c->htab.reserve(tab.size() * sizeof(float));
+    CHUI_HIP(hipMemcpyAsync(c->htab.ptr, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    CHUI_HIP(hipStreamSynchronize(s));
+    c->htab_key[0] = t_max;
+    c->htab_key[1] = sp.a;
+  }
+}
+
 // TreePmOptions defaults of the adaptive/group-walk fields (tree_pm_coupling.hpp:248-270).
 chui_treepm_adaptive_options adaptive_defaults() {
   chui_treepm_adaptive_options a{};
@@ -594,6 +625,15 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
                                                                  o->box_size[1], o->box_size[2],
                                                                  c->src32.as<uint4>());
       CHUI_LAUNCH_CHECK("gather_fixed_kernel");
+      if (adaptive_policy) {
+        // the adaptive decisions (groups, histories, acceptance) are made in
+        // FP64 also in mixed precision: FP64 sources and node records
+        c->src64.reserve(n * sizeof(double4));
+        gather64_kernel<<<chui::grid_for(n, 256), 256, 0, s>>>(c->tree.order.as<uint32_t>(), n, x, y, z, m,
+                                                               c->src64.as<double4>());
+        CHUI_LAUNCH_CHECK("gather64_kernel");
+        chui::ensure_records64(c->tree, s);
+      }
     }
   }
   record(c, kEvMultipoles);
@@ -749,10 +789,41 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
       group_status = CHUI_GROUP_WALK_LOOKUP_SCALING_UNREPRESENTABLE;
     else group_status = CHUI_GROUP_WALK_ACTIVE;
     const double2* gtab = c->gtab.as<double2>();
-    const double* f0 = static_cast<const double*>(pw.force[0]);
-    const double* f1 = static_cast<const double*>(pw.force[1]);
-    const double* f2 = static_cast<const double*>(pw.force[2]);
+    const double* f0 = mixed ? nullptr : static_cast<const double*>(pw.force[0]);
+    const double* f1 = mixed ? nullptr : static_cast<const double*>(pw.force[1]);
+    const double* f2 = mixed ? nullptr : static_cast<const double*>(pw.force[2]);
+    const float* g0 = mixed ? static_cast<const float*>(pw.force[0]) : nullptr;
+    const float* g1 = mixed ? static_cast<const float*>(pw.force[1]) : nullptr;
+    const float* g2 = mixed ? static_cast<const float*>(pw.force[2]) : nullptr;
+    if (mixed) prepare_mixed_pairs(c, wp, sp, o, uniform, um, s);
     auto* wcnt = walk_cnt;
+    // The per-target adaptive walk (targets the group walk hands back, or all
+    // of them). In mixed precision it runs in FP64 without PM into FP64 lanes,
+    // and adaptive_mixed_finish adds the FP32 PM part and writes FP32 outputs.
+    auto per_target = [&](const chui::WalkParams& fw, const uint32_t* pos, const uint32_t* slot) {
+      if (!mixed) {
+        chui::walk_fp64_adaptive(c->tree, fw, c->src64.as<double4>(), uniform ? 1 : 0, gtab, prev, pos, slot, f0, f1,
+                                 f2, out, wcnt, s);
+        return;
+      }
+      const std::size_t slots = n_active;  // output slots are below n_active
+      c->mix_fb.reserve(3 * slots * sizeof(double));
+      double* sx = c->mix_fb.as<double>();
+      double* sy = sx + slots;
+      double* sz = sy + slots;
+      double* nr = nullptr;
+      if (near2) {
+        c->mix_near.reserve(n * sizeof(double));
+        nr = c->mix_near.as<double>();
+      }
+      chui::WalkParams sw = fw;
+      sw.pm_enabled = 0;
+      const chui::WalkOutputs so{sx, sy, sz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nr};
+      chui::walk_fp64_adaptive(c->tree, sw, c->src64.as<double4>(), uniform ? 1 : 0, gtab, prev, pos, slot, nullptr,
+                               nullptr, nullptr, so, wcnt, s);
+      chui::adaptive_mixed_finish(c->tree, fw, c->src32.as<uint4>(), pos, slot, fw.n_targets, fw.identity_targets, sx,
+                                  sy, sz, nr, g0, g1, g2, out, s);
+    };
     if (group_status == CHUI_GROUP_WALK_ACTIVE) {
       c->leaf_of_pos.reserve(n * sizeof(uint32_t));
       chui::build_leaf_of_pos(c->tree, c->leaf_of_pos.as<uint32_t>(), s);
@@ -785,11 +856,17 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
         gp.hi[a] = decode(hi);
       }
       auto* fb_count = reinterpret_cast<uint32_t*>(bkeys + 6);  // zeroed above
-      chui::walk_fp64_group(c->tree, wp, gp, c->src64.as<double4>(), uniform ? 1 : 0, gtab,
-                            c->gcoef.as<double4>(), prev,
-                            c->leaf_of_pos.as<uint32_t>(), target_pos, target_slot, f0, f1, f2, out,
-                            c->fb_pos.as<uint32_t>(), c->fb_slot.as<uint32_t>(), fb_count,
-                            reinterpret_cast<unsigned long long*>(c->gcounters.ptr), s);
+      if (mixed)
+        chui::walk_mixed_group(c->tree, wp, gp, c->src64.as<double4>(), c->src32.as<uint4>(), uniform ? 1 : 0,
+                               c->htab.as<float4>(), prev, c->leaf_of_pos.as<uint32_t>(), target_pos, target_slot,
+                               g0, g1, g2, out, c->fb_pos.as<uint32_t>(), c->fb_slot.as<uint32_t>(), fb_count,
+                               reinterpret_cast<unsigned long long*>(c->gcounters.ptr), s);
+      else
+        chui::walk_fp64_group(c->tree, wp, gp, c->src64.as<double4>(), uniform ? 1 : 0, gtab,
+                              c->gcoef.as<double4>(), prev,
+                              c->leaf_of_pos.as<uint32_t>(), target_pos, target_slot, f0, f1, f2, out,
+                              c->fb_pos.as<uint32_t>(), c->fb_slot.as<uint32_t>(), fb_count,
+                              reinterpret_cast<unsigned long long*>(c->gcounters.ptr), s);
       uint32_t nfb = 0;
       c->tree.readback.read(&nfb, fb_count, sizeof(nfb), s);
       if (nfb > n_active) throw std::logic_error("TreePM group walk fallback list overflow");
@@ -797,12 +874,10 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
         chui::WalkParams fw = wp;
         fw.n_targets = nfb;
         fw.identity_targets = 0;
-        chui::walk_fp64_adaptive(c->tree, fw, c->src64.as<double4>(), uniform ? 1 : 0, gtab, prev,
-                                 c->fb_pos.as<uint32_t>(), c->fb_slot.as<uint32_t>(), f0, f1, f2, out, wcnt, s);
+        per_target(fw, c->fb_pos.as<uint32_t>(), c->fb_slot.as<uint32_t>());
       }
     } else {
-      chui::walk_fp64_adaptive(c->tree, wp, c->src64.as<double4>(), uniform ? 1 : 0, gtab, prev, target_pos,
-                               target_slot, f0, f1, f2, out, wcnt, s);
+      per_target(wp, target_pos, target_slot);
     }
   } else if (n_active > 0 && n > 0) {
     if (!mixed) {
@@ -814,30 +889,7 @@ void solve_device_impl(Ctx* c, const double* x, const double* y, const double* z
                       static_cast<const double*>(pw.force[0]), static_cast<const double*>(pw.force[1]),
                       static_cast<const double*>(pw.force[2]), out, walk_cnt, c->h64tab.as<double2>(), s);
     } else {
-      const double t_max = wp.rcut2 * (0.25 / (sp.a * sp.a)) * (1.0 + 1.0e-3);
-      std::vector<float> tab = chui::gaussian_coefficient_table(wp, t_max);
-      // pair arithmetic in table units: lengths * sqrt(t_to_cell)
-      const double t2c = 0.25 / (sp.a * sp.a) / (t_max / chui::kHTableCells);
-      const double sq = std::sqrt(t2c);
-      for (int a = 0; a < 3; ++a) wp.tab_scale[a] = static_cast<float>(o->box_size[a] / 4294967296.0 * sq);
-      wp.rcut2_tab = static_cast<float>(wp.rcut2 * t2c);
-      // floor 1e-12 (table units, r^2 <= 128 for every pair inside r_cut): a
-      // coincident pair (r^2 = 0) keeps a finite softened term (< 1e18 times
-      // its mass), so its zero separation gives a zero force
-      wp.eps2_tab = static_cast<float>(std::max(wp.eps * wp.eps * t2c, 1.0e-12));
-      wp.c8_tab = static_cast<float>(0.125 / (sp.a * sp.a * sp.a) / (t2c * sq));
-      // with uniform masses the pair kernel leaves the mass to this factor
-      wp.force_scale = static_cast<float>(wp.G * t2c * (uniform ? um : 1.0));
-      wp.tab_to_len2 = static_cast<float>(1.0 / t2c);
-      wp.accept_half_max = static_cast<float>(0.041 * sp.rcut);
-      if (c->htab_key[0] != t_max || c->htab_key[1] != sp.a || !c->htab.ptr) {
-        chui::require_not_capturing(s, "the pair table");
-        c->htab.reserve(tab.size() * sizeof(float));
-        CHUI_HIP(hipMemcpyAsync(c->htab.ptr, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, s));
-        CHUI_HIP(hipStreamSynchronize(s));
-        c->htab_key[0] = t_max;
-        c->htab_key[1] = sp.a;
-      }
+      prepare_mixed_pairs(c, wp, sp, o, uniform, um, s);
       // The timed walk (no counters, 64 targets per wave) takes the residual
       // table indexed by the float bits of s = r^2 + eps^2, in units where
       // r_cut^2 + eps^2 = 1; the counting and per-wave walks keep the

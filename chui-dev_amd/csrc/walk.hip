@@ -1027,6 +1027,17 @@ __global__ __launch_bounds__(kWalkThreads) void walk64_kernel(WalkParams P, Walk
 //      evaluating every listed source keeps all S lanes of a target busy.
 // Groups above 64 targets are evaluated 64 targets at a time, repeating the
 // (identical) tree walk for each.
+//
+// Mixed precision (MIX, group_walkmix_kernel): groups, histories, the tree
+// walk and every acceptance decision stay in FP64, so the lists equal the FP64
+// walk's; the accepted nodes' expansions stay in FP64 as well (a few per
+// target). Only the listed-source pairs change: sources are staged as 32-bit
+// fixed-point coordinates (16 B instead of 32 B), a separation is the wrapped
+// int32 difference scaled to H-table units, and the pair term is the FP32
+// residual of the mixed strict walk (reciprocal square root, the 128-cell
+// cubic table of H in LDS). Each staged chunk's FP32 sums are added to the
+// FP64 totals, so no FP32 sum runs over more than 64 sources per lane. PM
+// interpolation reads the FP32 force meshes at the fixed-point position.
 constexpr int kGroupLeafCap = 192;
 constexpr int kGroupNodeCap = 128;
 constexpr int kGroupStage = 64;  // listed sources staged in LDS per evaluation pass
@@ -1063,19 +1074,24 @@ template <class T> __device__ __forceinline__ T scan_max(T v, int lane) {
   return v;
 }
 
-template <int SCHEME>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void group_walk64_kernel(WalkParams P, GroupParams G, WalkData D,
-                                                              const uint32_t* __restrict__ leaf_of_pos,
-                                                              WalkOutputs out, uint32_t* __restrict__ fb_pos,
-                                                              uint32_t* __restrict__ fb_slot,
-                                                              uint32_t* __restrict__ fb_count,
-                                                              unsigned long long* __restrict__ counters) {
+template <int SCHEME, bool MIX>
+__device__ __forceinline__ void group_walk_body(const WalkParams& P, const GroupParams& G, const WalkData& D,
+                                                const uint32_t* __restrict__ leaf_of_pos, const WalkOutputs& out,
+                                                uint32_t* __restrict__ fb_pos, uint32_t* __restrict__ fb_slot,
+                                                uint32_t* __restrict__ fb_count,
+                                                unsigned long long* __restrict__ counters) {
   __shared__ GroupLeaf leaves[kGroupLeafCap];
   __shared__ uint32_t nodes[kGroupNodeCap];
   __shared__ uint32_t leaf_pre[kGroupLeafCap + 1];
-  __shared__ double4 stage[kGroupStage];
+  __shared__ double4 stage[MIX ? 1 : kGroupStage];
+  __shared__ uint4 stage32[MIX ? kGroupStage : 1];
+  __shared__ float4 htab[MIX ? kHTableCells : 1];
   __shared__ uint32_t stage_j[kGroupStage];
   const int lane = threadIdx.x;
+  if constexpr (MIX) {
+    for (int i = lane; i < kHTableCells; i += kWave) htab[i] = D.htab[i];
+    __syncthreads();
+  }
   const uint32_t bb = blockIdx.x * G.block;
   const uint32_t be = min(bb + G.block, P.n_targets);
   const double L0 = P.L[0], L1 = P.L[1], L2 = P.L[2];
@@ -1175,7 +1191,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
       }
       if (lane == 0) {
         cnt[8] += count;
-        cnt[fall == 1 ? 9 : 10] += count;
+        if (fall == 1) cnt[9] += count;
+        else cnt[10] += count;
       }
       v = ge;
       continue;
@@ -1202,10 +1219,17 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
       const bool valid = static_cast<uint32_t>(t) < T;
       const bool first = t0 == v;  // count the (repeated) tree walk once per group
       uint32_t tp = 0, slot = 0;
+      uint32_t tx = 0, ty = 0, tz = 0;  // MIX: fixed-point target position
       double px = 0.0, py = 0.0, pz = 0.0, ox = 0.0, oy = 0.0, oz = 0.0;
       if (valid) {
         tp = tpos(t0 + t);
         slot = tslot(t0 + t);
+        if constexpr (MIX) {
+          const uint4 f = D.src32[tp];
+          tx = f.x;
+          ty = f.y;
+          tz = f.z;
+        }
         const double4 q = D.src64[tp];
         px = q.x;
         py = q.y;
@@ -1261,11 +1285,47 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
               else hi = mid;
             }
             const uint32_t j = leaves[lo].b + (g - leaf_pre[lo]);
-            stage[lane] = D.src64[j];
+            if constexpr (MIX) stage32[lane] = D.src32[j];
+            else stage[lane] = D.src64[j];
             stage_j[lane] = j;
           }
           __syncthreads();
           if (!valid) continue;
+          if constexpr (MIX) {
+            // FP32 pairs in H-table units (the mixed strict walk's pair term)
+            const float u0 = P.tab_scale[0], u1 = P.tab_scale[1], u2 = P.tab_scale[2];
+            const float rc2 = P.rcut2_tab, e2 = P.eps2_tab, c8 = P.c8_tab;
+            float lx = 0.0f, ly = 0.0f, lz = 0.0f, near_t = INFINITY;
+            uint32_t inside_n = 0;
+            for (uint32_t q = s; q < gn; q += S) {
+              const uint4 sv = stage32[q];
+              const uint32_t j = stage_j[q];
+              const float dx = fx_delta(sv.x, tx, u0), dy = fx_delta(sv.y, ty, u1), dz = fx_delta(sv.z, tz, u2);
+              const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+              const float rinv = __builtin_amdgcn_rsqf(r2 + e2);
+              const float soft = rinv * rinv * rinv;
+              const int ci = static_cast<int>(fminf(r2, static_cast<float>(kHTableCells - 1)));
+              const float fu = __builtin_amdgcn_fractf(r2);
+              const float4 c = htab[ci];
+              const float hcoef = fmaf(fmaf(fmaf(c.w, fu, c.z), fu, c.y), fu, c.x);
+              float fr = fmaf(-hcoef, c8, soft);
+              if (!D.uniform_mass) fr *= __uint_as_float(sv.w);
+              const bool inside = j != tp && r2 <= rc2;
+              const float f = inside ? fr : 0.0f;
+              lx = fmaf(f, dx, lx);
+              ly = fmaf(f, dy, ly);
+              lz = fmaf(f, dz, lz);
+              inside_n += inside ? 1u : 0u;
+              near_t = inside ? fminf(near_t, r2) : near_t;
+            }
+            const double gm = P.force_scale;
+            ax = fma(gm, static_cast<double>(lx), ax);
+            ay = fma(gm, static_cast<double>(ly), ay);
+            az = fma(gm, static_cast<double>(lz), az);
+            cnt[2] += inside_n;
+            nearest = fmin(nearest, static_cast<double>(near_t) * static_cast<double>(P.tab_to_len2));
+            continue;
+          }
           for (uint32_t q = s; q < gn; q += S) {
             const double4 sj = stage[q];
             const uint32_t j = stage_j[q];
@@ -1400,7 +1460,26 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
         az += __shfl_xor(az, off, kWave);
         nearest = fmin(nearest, __shfl_xor(nearest, off, kWave));
       }
-      if (valid && s == 0) {
+      if (MIX && valid && s == 0) {
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        if (P.pm_enabled) interp32<SCHEME>(P, D.f32x, D.f32y, D.f32z, tx, ty, tz, gx, gy, gz);
+        const float sx = static_cast<float>(ax), sy = static_cast<float>(ay), sz = static_cast<float>(az);
+        static_cast<float*>(out.ax)[slot] = gx + sx;
+        static_cast<float*>(out.ay)[slot] = gy + sy;
+        static_cast<float*>(out.az)[slot] = gz + sz;
+        if (out.pm_x) {
+          static_cast<float*>(out.pm_x)[slot] = gx;
+          static_cast<float*>(out.pm_y)[slot] = gy;
+          static_cast<float*>(out.pm_z)[slot] = gz;
+        }
+        const uint32_t row = D.order[tp];
+        if (out.sr_x) {
+          static_cast<float*>(out.sr_x)[row] = sx;
+          static_cast<float*>(out.sr_y)[row] = sy;
+          static_cast<float*>(out.sr_z)[row] = sz;
+        }
+        if (out.near2) static_cast<float*>(out.near2)[row] = static_cast<float>(nearest);
+      } else if (!MIX && valid && s == 0) {
         double gx = 0.0, gy = 0.0, gz = 0.0;
         if (P.pm_enabled) interp64<SCHEME>(P, D.f64x, D.f64y, D.f64z, px, py, pz, gx, gy, gz);
         static_cast<double*>(out.ax)[slot] = gx + ax;
@@ -1422,6 +1501,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     }
     v = ge;
   }
+#pragma unroll
   for (int i = 0; i < kGroupCounters; ++i) {
     if (i == 7 || i == 14) continue;
     const unsigned long long sum = wave_sum(static_cast<unsigned long long>(cnt[i]));
@@ -1431,6 +1511,63 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     if (size_max) atomicMax(&counters[7], size_max);
     if (entries_max) atomicMax(&counters[14], entries_max);
   }
+}
+
+template <int SCHEME>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void group_walk64_kernel(
+    WalkParams P, GroupParams G, WalkData D, const uint32_t* __restrict__ leaf_of_pos, WalkOutputs out,
+    uint32_t* __restrict__ fb_pos, uint32_t* __restrict__ fb_slot, uint32_t* __restrict__ fb_count,
+    unsigned long long* __restrict__ counters) {
+  group_walk_body<SCHEME, false>(P, G, D, leaf_of_pos, out, fb_pos, fb_slot, fb_count, counters);
+}
+
+template <int SCHEME>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void group_walkmix_kernel(
+    WalkParams P, GroupParams G, WalkData D, const uint32_t* __restrict__ leaf_of_pos, WalkOutputs out,
+    uint32_t* __restrict__ fb_pos, uint32_t* __restrict__ fb_slot, uint32_t* __restrict__ fb_count,
+    unsigned long long* __restrict__ counters) {
+  group_walk_body<SCHEME, true>(P, G, D, leaf_of_pos, out, fb_pos, fb_slot, fb_count, counters);
+}
+
+// Mixed adaptive solves: targets the group walk hands back (or every target
+// when the group walk does not run) take the FP64 per-target adaptive walk
+// without PM, written into FP64 lanes by output slot (nearest pair by source
+// row). This kernel adds the FP32 PM interpolation at the fixed-point
+// position and writes the FP32 outputs of those targets. pos/slot: the
+// targets' tree positions and output slots (identity: position k, slot
+// order[k]).
+template <int SCHEME>
+__global__ __launch_bounds__(256) void adaptive_mixed_finish_kernel(WalkParams P, WalkData D,
+                                                                    const uint32_t* __restrict__ pos,
+                                                                    const uint32_t* __restrict__ slot, uint32_t n,
+                                                                    int identity, const double* __restrict__ sx,
+                                                                    const double* __restrict__ sy,
+                                                                    const double* __restrict__ sz,
+                                                                    const double* __restrict__ near64,
+                                                                    WalkOutputs out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t tp = identity ? i : pos[i];
+  const uint32_t sl = identity ? D.order[i] : slot[i];
+  const uint4 f = D.src32[tp];
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  if (P.pm_enabled) interp32<SCHEME>(P, D.f32x, D.f32y, D.f32z, f.x, f.y, f.z, gx, gy, gz);
+  const float ax = static_cast<float>(sx[sl]), ay = static_cast<float>(sy[sl]), az = static_cast<float>(sz[sl]);
+  static_cast<float*>(out.ax)[sl] = gx + ax;
+  static_cast<float*>(out.ay)[sl] = gy + ay;
+  static_cast<float*>(out.az)[sl] = gz + az;
+  if (out.pm_x) {
+    static_cast<float*>(out.pm_x)[sl] = gx;
+    static_cast<float*>(out.pm_y)[sl] = gy;
+    static_cast<float*>(out.pm_z)[sl] = gz;
+  }
+  const uint32_t row = D.order[tp];
+  if (out.sr_x) {
+    static_cast<float*>(out.sr_x)[row] = ax;
+    static_cast<float*>(out.sr_y)[row] = ay;
+    static_cast<float*>(out.sr_z)[row] = az;
+  }
+  if (out.near2) static_cast<float*>(out.near2)[row] = static_cast<float>(near64[row]);
 }
 
 __global__ void leaf_of_pos_kernel(const NodeLink* __restrict__ link, std::size_t nodes,
@@ -3127,6 +3264,63 @@ void walk_fp64_group(const Tree& t, const WalkParams& p, const GroupParams& g, c
     group_walk64_kernel<1><<<blocks, kWave, 0, s>>>(p, g, d, leaf_of_pos, out, fallback_pos, fallback_slot,
                                                     fallback_count, counters);
   CHUI_LAUNCH_CHECK("group_walk64_kernel");
+}
+
+void walk_mixed_group(const Tree& t, const WalkParams& p, const GroupParams& g, const double4* src64,
+                      const uint4* src32, int uniform_mass, const float4* htab, const double* prev,
+                      const uint32_t* leaf_of_pos, const uint32_t* target_pos, const uint32_t* target_slot,
+                      const float* fx, const float* fy, const float* fz, const WalkOutputs& out,
+                      uint32_t* fallback_pos, uint32_t* fallback_slot, uint32_t* fallback_count,
+                      unsigned long long* counters, hipStream_t s) {
+  if (!t.records64 && t.node_count > 0) throw std::logic_error("mixed group walk on a tree without FP64 records");
+  if (p.n_targets == 0) return;
+  if (g.block == 0 || g.block > 256) throw std::invalid_argument("group walk residual block must be in [1, 256]");
+  if (!htab || !src32 || !src64) throw std::logic_error("mixed group walk without its source lanes or pair table");
+  WalkData d{};
+  d.link = t.link.as<NodeLink>();
+  d.geo64 = t.geo64.as<NodeGeo64>();
+  d.mom64 = t.mom64.as<NodeMom64>();
+  d.quad64 = t.quad64.as<NodeQuad64>();
+  d.order = t.order.as<uint32_t>();
+  d.src64 = src64;
+  d.src32 = src32;
+  d.htab = htab;
+  d.uniform_mass = uniform_mass;
+  d.tpos = target_pos;
+  d.tslot = target_slot;
+  d.f32x = fx;
+  d.f32y = fy;
+  d.f32z = fz;
+  d.prev = prev;
+  const unsigned blocks = static_cast<unsigned>((static_cast<std::size_t>(p.n_targets) + g.block - 1) / g.block);
+  if (p.assignment == 0)
+    group_walkmix_kernel<0><<<blocks, kWave, 0, s>>>(p, g, d, leaf_of_pos, out, fallback_pos, fallback_slot,
+                                                     fallback_count, counters);
+  else
+    group_walkmix_kernel<1><<<blocks, kWave, 0, s>>>(p, g, d, leaf_of_pos, out, fallback_pos, fallback_slot,
+                                                     fallback_count, counters);
+  CHUI_LAUNCH_CHECK("group_walkmix_kernel");
+}
+
+void adaptive_mixed_finish(const Tree& t, const WalkParams& p, const uint4* src32, const uint32_t* pos,
+                           const uint32_t* slot, uint32_t n, int identity, const double* sx, const double* sy,
+                           const double* sz, const double* near64, const float* fx, const float* fy,
+                           const float* fz, const WalkOutputs& out, hipStream_t s) {
+  if (n == 0) return;
+  if (out.near2 && !near64) throw std::logic_error("mixed adaptive finish: nearest-pair lane without its FP64 source");
+  if (!identity && (!pos || !slot)) throw std::logic_error("mixed adaptive finish: target lists are null");
+  WalkData d{};
+  d.order = t.order.as<uint32_t>();
+  d.src32 = src32;
+  d.f32x = fx;
+  d.f32y = fy;
+  d.f32z = fz;
+  const unsigned grid = grid_for(n, 256);
+  if (p.assignment == 0)
+    adaptive_mixed_finish_kernel<0><<<grid, 256, 0, s>>>(p, d, pos, slot, n, identity, sx, sy, sz, near64, out);
+  else
+    adaptive_mixed_finish_kernel<1><<<grid, 256, 0, s>>>(p, d, pos, slot, n, identity, sx, sy, sz, near64, out);
+  CHUI_LAUNCH_CHECK("adaptive_mixed_finish_kernel");
 }
 
 void build_leaf_of_pos(const Tree& t, uint32_t* leaf_of_pos, hipStream_t s) {

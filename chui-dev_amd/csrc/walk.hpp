@@ -201,6 +201,26 @@ std::vector<double> gaussian_hermite_knots();
 std::vector<double> gaussian_hermite_monomials(const std::vector<double>& knots);
 
 // Leaf node of every tree position of the last built tree.
+// Mixed-precision group walk: the FP64 group walk's groups, lists and
+// decisions (src64, FP64 records) with FP32 listed-source pairs (src32 in
+// fixed point, the Gaussian coefficient table htab in H-table units) and FP32
+// PM meshes and outputs. Fallback targets as in walk_fp64_group.
+void walk_mixed_group(const Tree& t, const WalkParams& p, const GroupParams& g, const double4* src64,
+                      const uint4* src32, int uniform_mass, const float4* htab, const double* prev,
+                      const uint32_t* leaf_of_pos, const uint32_t* target_pos, const uint32_t* target_slot,
+                      const float* fx, const float* fy, const float* fz, const WalkOutputs& out,
+                      uint32_t* fallback_pos, uint32_t* fallback_slot, uint32_t* fallback_count,
+                      unsigned long long* counters, hipStream_t s);
+
+// Mixed adaptive solves: FP32 outputs of n targets whose short-range sums an
+// FP64 per-target walk wrote into sx/sy/sz (by output slot) and near64 (by
+// source row, nullable unless out.near2), plus the FP32 PM interpolation.
+// identity: targets are tree positions 0..n-1 with slot order[k].
+void adaptive_mixed_finish(const Tree& t, const WalkParams& p, const uint4* src32, const uint32_t* pos,
+                           const uint32_t* slot, uint32_t n, int identity, const double* sx, const double* sy,
+                           const double* sz, const double* near64, const float* fx, const float* fy,
+                           const float* fz, const WalkOutputs& out, hipStream_t s);
+
 void build_leaf_of_pos(const Tree& t, uint32_t* leaf_of_pos, hipStream_t s);
 
 // ltab: the bit-indexed residual table of the timed walk (used when p.log_cells > 0).

@@ -387,9 +387,12 @@ chui_status chui_treepm_adaptive_options_set(chui_treepm_adaptive_options* adapt
  * squared separation of the nearest distinct direct-summed source inside
  * r_cut, +infinity when none (short_range_nearest_pair_r2_comoving,
  * tree_pm_coupling.hpp:154-159). Device arrays and element types as in
- * chui_treepm_solve_device. The adaptive policy is FP64 only; with
- * acceptance_policy = CHUI_ACCEPT_STRICT this is the strict solve in either
- * precision plus the nearest-pair lane (mixed: FP32, from the counting walk).
+ * chui_treepm_solve_device. In mixed precision (FP32 lanes) the group walk
+ * makes every decision in FP64 and evaluates the listed-source pairs in FP32;
+ * the per-target adaptive walk runs in FP64 and its sums are rounded to the
+ * FP32 lanes. With acceptance_policy = CHUI_ACCEPT_STRICT this is the strict
+ * solve in either precision plus the nearest-pair lane (mixed: FP32, from the
+ * counting walk).
  * counters may be NULL (no synchronisation then). */
 chui_status chui_treepm_solve_adaptive_device(chui_treepm_context* ctx,
                                               const double* pos_x, const double* pos_y, const double* pos_z,

@@ -216,11 +216,12 @@ def test_python_mirror_raises_reference_exception_types():
     assert np.dtype(np.float64).itemsize == 8
 
 
-def test_adaptive_policy_is_fp64_only(lib):
+def test_adaptive_policy_validates_in_both_precisions(lib):
     o = valid_options()
     o.acceptance_policy = 1
-    o.precision = L.PRECISION_MIXED
-    assert lib.chui_treepm_options_validate(C.byref(o), MESH) == L.CHUI_ERR_UNSUPPORTED
+    for precision in (L.PRECISION_FP64, L.PRECISION_MIXED):
+        o.precision = precision
+        assert lib.chui_treepm_options_validate(C.byref(o), MESH) == L.CHUI_OK
 
 
 def test_adaptive_options_mirror_reference_defaults(lib):
