@@ -1367,7 +1367,10 @@ __device__ __forceinline__ void group_walk_body(const WalkParams& P, const Group
             ++cnt[1];
             continue;
           }
-          node_accel64_call(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
+          // inline: an out-of-line call here saves and restores the live
+          // registers through scratch on every accepted node
+          if (P.radial) node_accel_radial64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
+          else node_accel64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
         }
         __syncthreads();
         nl = 0;
@@ -1513,8 +1516,11 @@ __device__ __forceinline__ void group_walk_body(const WalkParams& P, const Group
   }
 }
 
+// Two waves per SIMD (256 VGPRs): with the node expansions inline the body
+// needs about that many; at 168 VGPRs (3 waves) it spilled 384-512 bytes per
+// lane and ran 2% (FP64) to 29% (mixed) slower at 256^3.
 template <int SCHEME>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void group_walk64_kernel(
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void group_walk64_kernel(
     WalkParams P, GroupParams G, WalkData D, const uint32_t* __restrict__ leaf_of_pos, WalkOutputs out,
     uint32_t* __restrict__ fb_pos, uint32_t* __restrict__ fb_slot, uint32_t* __restrict__ fb_count,
     unsigned long long* __restrict__ counters) {
@@ -1522,7 +1528,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
 }
 
 template <int SCHEME>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void group_walkmix_kernel(
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void group_walkmix_kernel(
     WalkParams P, GroupParams G, WalkData D, const uint32_t* __restrict__ leaf_of_pos, WalkOutputs out,
     uint32_t* __restrict__ fb_pos, uint32_t* __restrict__ fb_slot, uint32_t* __restrict__ fb_count,
     unsigned long long* __restrict__ counters) {
