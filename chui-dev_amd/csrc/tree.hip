@@ -694,7 +694,7 @@ __device__ __forceinline__ uint32_t lower_bound_digit(const uint64_t* __restrict
 // One level of the octree per three launches, with the level's node range read
 // from the device (lv[d], lv[d + 1]), so the host issues every level without
 // reading anything back and reads the level starts and flags once at the end.
-// A level's nodes are split into kLevelBlocks contiguous chunks, one per
+// A level's nodes are split into up to kLevelBlocks contiguous chunks, one per
 // block. Count: eight threads per node, thread o of a splitting node
 // binary-searches the first particle of octant o (the searches are
 // independent, so a level costs one search depth rather than seven); the
@@ -712,9 +712,19 @@ constexpr int kLvLeaves = kLvTooDeep + 3;               // nodes without childre
 constexpr int kLvTotal = kLvTooDeep + 4;                // node count (capture-safe builds)
 constexpr int kLvWords = kLvTooDeep + 5;
 
+// nblocks: the grid of the count and emit passes of level d (at most
+// kLevelBlocks); the offsets pass scans that many block sums.
 struct LevelDevArgs {
-  uint32_t d, leaf_size, capacity, sorted_levels;
+  uint32_t d, leaf_size, capacity, sorted_levels, nblocks;
 };
+
+// Grid of the count and emit passes of level d: level d holds at most
+// min(8^d, n) nodes and a block decides 32 nodes per step, so the shallow
+// levels launch a few blocks instead of kLevelBlocks.
+inline uint32_t level_blocks(int d, std::size_t n) {
+  const uint64_t bound = d >= 8 ? static_cast<uint64_t>(n) : std::min<uint64_t>(1ull << (3 * d), n);
+  return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(kLevelBlocks, (bound + 31) / 32)));
+}
 
 // exclusive prefix sum over the block (blockDim.x a multiple of the wave); total to *total
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
@@ -806,10 +816,10 @@ __global__ __launch_bounds__(kLevelBlocks / 2) void level_offsets_dev_kernel(Lev
   __shared__ uint32_t wsum[kLevelBlocks / 2 / kWave];
   __shared__ uint32_t tot;
   const uint32_t i = 2 * threadIdx.x;
-  const uint32_t v0 = bsum[i], v1 = bsum[i + 1];
+  const uint32_t v0 = i < a.nblocks ? bsum[i] : 0u, v1 = i + 1 < a.nblocks ? bsum[i + 1] : 0u;
   const uint32_t off = block_exclusive_scan(v0 + v1, wsum, &tot);
-  bsum[i] = off;
-  bsum[i + 1] = off + v0;
+  if (i < a.nblocks) bsum[i] = off;
+  if (i + 1 < a.nblocks) bsum[i + 1] = off + v0;
   if (threadIdx.x == 0) {
     const uint32_t le = lv[a.d + 1];
     const uint64_t next = static_cast<uint64_t>(le) + tot;
@@ -1657,14 +1667,14 @@ void build_tree(Tree& t, const double* x, const double* y, const double* z, cons
     auto issue_levels = [&](int d0, int d1) {
       for (int d = d0; d < d1; ++d) {
         const LevelDevArgs la{static_cast<uint32_t>(d), P.leaf_size, static_cast<uint32_t>(cap),
-                              static_cast<uint32_t>(partial ? kPartialSortLevels : kMaxLevels)};
-        level_count_dev_kernel<<<kLevelBlocks, 256, 0, s>>>(
+                              static_cast<uint32_t>(partial ? kPartialSortLevels : kMaxLevels), level_blocks(d, n)};
+        level_count_dev_kernel<<<la.nblocks, 256, 0, s>>>(
             la, lv, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
             t.bfs_geo.as<double4>(), keys, t.bfs_nchild.as<uint32_t>(), t.level_bounds.as<uint32_t>(), bsum);
         CHUI_LAUNCH_CHECK("level_count_dev_kernel");
         level_offsets_dev_kernel<<<1, kLevelBlocks / 2, 0, s>>>(la, lv, bsum);
         CHUI_LAUNCH_CHECK("level_offsets_dev_kernel");
-        level_emit_dev_kernel<<<kLevelBlocks, 256, 0, s>>>(
+        level_emit_dev_kernel<<<la.nblocks, 256, 0, s>>>(
             la, lv, t.bfs_begin.as<uint32_t>(), t.bfs_count.as<uint32_t>(), t.bfs_meta.as<uint32_t>(),
             t.bfs_geo.as<double4>(), t.level_bounds.as<uint32_t>(), t.bfs_nchild.as<uint32_t>(), bsum,
             t.bfs_first_child.as<uint32_t>());
