@@ -1074,7 +1074,7 @@ template <class T> __device__ __forceinline__ T scan_max(T v, int lane) {
   return v;
 }
 
-template <int SCHEME, bool MIX>
+template <int SCHEME, bool MIX, bool NODE32>
 __device__ __forceinline__ void group_walk_body(const WalkParams& P, const GroupParams& G, const WalkData& D,
                                                 const uint32_t* __restrict__ leaf_of_pos, const WalkOutputs& out,
                                                 uint32_t* __restrict__ fb_pos, uint32_t* __restrict__ fb_slot,
@@ -1367,10 +1367,24 @@ __device__ __forceinline__ void group_walk_body(const WalkParams& P, const Group
             ++cnt[1];
             continue;
           }
-          // inline: an out-of-line call here saves and restores the live
-          // registers through scratch on every accepted node
-          if (P.radial) node_accel_radial64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
-          else node_accel64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
+          if constexpr (NODE32) {
+            // the mixed strict walk's FP32 expansion from the fixed-point
+            // centre of mass (the cutoff decision above stays FP64)
+            const NodeMom32 M32 = D.mom32[nd];
+            const float s0 = P.cubic ? 1.0f : P.fx_to_len[0], s1 = P.cubic ? 1.0f : P.fx_to_len[1],
+                        s2 = P.cubic ? 1.0f : P.fx_to_len[2], tl = P.cubic ? P.fx_to_len[0] : 1.0f;
+            float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+            node_accel32(M32, D.quad32[nd], fx_delta(M32.comx, tx, s0) * tl, fx_delta(M32.comy, ty, s1) * tl,
+                         fx_delta(M32.comz, tz, s2) * tl, P, cx, cy, cz);
+            ax += cx;
+            ay += cy;
+            az += cz;
+          } else {
+            // inline: an out-of-line call here saves and restores the live
+            // registers through scratch on every accepted node
+            if (P.radial) node_accel_radial64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
+            else node_accel64(M, D.quad64[nd], dx, dy, dz, P, ax, ay, az);
+          }
         }
         __syncthreads();
         nl = 0;
@@ -1516,23 +1530,27 @@ __device__ __forceinline__ void group_walk_body(const WalkParams& P, const Group
   }
 }
 
-// Two waves per SIMD (256 VGPRs): with the node expansions inline the body
-// needs about that many; at 168 VGPRs (3 waves) it spilled 384-512 bytes per
-// lane and ran 2% (FP64) to 29% (mixed) slower at 256^3.
+// Two waves per SIMD (256 VGPRs) where the FP64 node expansions are inline:
+// the body needs about that many; at 168 VGPRs (3 waves) it spilled 384-512
+// bytes per lane and ran 2% (FP64) to 29% (mixed) slower at 256^3. The mixed
+// walk with FP32 node expansions (RADIAL false) holds 142 VGPRs and runs 3
+// waves per SIMD (4 waves: 128 VGPRs with spills, 16% slower).
 template <int SCHEME>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void group_walk64_kernel(
     WalkParams P, GroupParams G, WalkData D, const uint32_t* __restrict__ leaf_of_pos, WalkOutputs out,
     uint32_t* __restrict__ fb_pos, uint32_t* __restrict__ fb_slot, uint32_t* __restrict__ fb_count,
     unsigned long long* __restrict__ counters) {
-  group_walk_body<SCHEME, false>(P, G, D, leaf_of_pos, out, fb_pos, fb_slot, fb_count, counters);
+  group_walk_body<SCHEME, false, false>(P, G, D, leaf_of_pos, out, fb_pos, fb_slot, fb_count, counters);
 }
 
-template <int SCHEME>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void group_walkmix_kernel(
+// RADIAL: the radial-coefficient node expansion (FP64; the FP32 expansion is
+// the plain monopole + quadrupole one).
+template <int SCHEME, bool RADIAL>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(RADIAL ? 2 : 3))) void group_walkmix_kernel(
     WalkParams P, GroupParams G, WalkData D, const uint32_t* __restrict__ leaf_of_pos, WalkOutputs out,
     uint32_t* __restrict__ fb_pos, uint32_t* __restrict__ fb_slot, uint32_t* __restrict__ fb_count,
     unsigned long long* __restrict__ counters) {
-  group_walk_body<SCHEME, true>(P, G, D, leaf_of_pos, out, fb_pos, fb_slot, fb_count, counters);
+  group_walk_body<SCHEME, true, !RADIAL>(P, G, D, leaf_of_pos, out, fb_pos, fb_slot, fb_count, counters);
 }
 
 // Mixed adaptive solves: targets the group walk hands back (or every target
@@ -3291,6 +3309,8 @@ void walk_mixed_group(const Tree& t, const WalkParams& p, const GroupParams& g, 
   d.src64 = src64;
   d.src32 = src32;
   d.htab = htab;
+  d.mom32 = t.mom32.as<NodeMom32>();
+  d.quad32 = t.quad32.as<NodeQuad32>();
   d.uniform_mass = uniform_mass;
   d.tpos = target_pos;
   d.tslot = target_slot;
@@ -3299,12 +3319,17 @@ void walk_mixed_group(const Tree& t, const WalkParams& p, const GroupParams& g, 
   d.f32z = fz;
   d.prev = prev;
   const unsigned blocks = static_cast<unsigned>((static_cast<std::size_t>(p.n_targets) + g.block - 1) / g.block);
-  if (p.assignment == 0)
-    group_walkmix_kernel<0><<<blocks, kWave, 0, s>>>(p, g, d, leaf_of_pos, out, fallback_pos, fallback_slot,
-                                                     fallback_count, counters);
-  else
-    group_walkmix_kernel<1><<<blocks, kWave, 0, s>>>(p, g, d, leaf_of_pos, out, fallback_pos, fallback_slot,
-                                                     fallback_count, counters);
+  if (!t.mom32.ptr || !t.quad32.ptr) throw std::logic_error("mixed group walk on a tree without FP32 node records");
+  auto go = [&](auto kernel) {
+    kernel<<<blocks, kWave, 0, s>>>(p, g, d, leaf_of_pos, out, fallback_pos, fallback_slot, fallback_count, counters);
+  };
+  if (p.assignment == 0) {
+    if (p.radial) go(group_walkmix_kernel<0, true>);
+    else go(group_walkmix_kernel<0, false>);
+  } else {
+    if (p.radial) go(group_walkmix_kernel<1, true>);
+    else go(group_walkmix_kernel<1, false>);
+  }
   CHUI_LAUNCH_CHECK("group_walkmix_kernel");
 }
 
